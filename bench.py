@@ -1,0 +1,199 @@
+"""Benchmark: batched multi-view raster + interpolate, forward + backward, on one GPU.
+
+Workload (one "step"): 16 views of uneven size (512x512 ... 1920x1080, twice;
+15,163,392 pixels in total) of a seeded 400x250-quad heightfield (200,000
+triangles, 100,651 vertices, heights N(0, 0.05)) with 8 float attributes per
+vertex N(0,1):
+    batched rasterize forward (tile binning + raster, derivatives on)
+ -> batched interpolate forward (A=8)
+ -> L2 loss gradient against N(0,1) targets (2 * (img - target))
+ -> batched interpolate backward (attribute + raster gradients)
+ -> batched rasterize backward (per-view clip-position gradients).
+Inputs are resident in HBM before the timed region.  Metric: Mpix/s = pixels / step time.
+
+Also reported: the roofline of the dominant kernel (algorithmic bytes per launch
+over its HIP-event-timed average duration, against 8 TB/s) and the CPU baseline
+(the oracle, the reference's algorithm restated in C++, on the host cores, over
+a bounded sample of the same workload).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(V, T, A, pixels):
+    """Every API-boundary array read once / written once (reference layouts)."""
+    n = len(pixels)
+    P = sum(pixels)
+    mesh = 16 * V + 12 * T
+    return {
+        "raster_fwd": n * mesh + 32 * P,
+        "raster_bwd": n * (mesh + 16 * V) + 32 * P,
+        "interp_fwd": 16 * P + 4 * A * P + n * (4 * A * V + 12 * T),
+        "interp_bwd": 16 * P + 4 * A * P + 16 * P + n * (4 * A * V + 12 * T) + 4 * A * V,
+    }
+
+
+def cpu_baseline(scene, views, budget_s=20.0):
+    """Oracle chain (bins + per-pixel raster, interp fwd, interp bwd with per-vertex
+    pixel lists, per-triangle raster bwd + CSR reduce) on a prefix of the views."""
+    import oracle
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(os.cpu_count() or 1, 16)
+    oracle.set_num_threads(threads)
+    done_px, t_total, used = 0, 0.0, 0
+    for k in range(len(scene.clips)):
+        if k >= views:
+            break
+        clip, res, tgt = scene.clips[k], scene.resolutions[k], scene.targets[k]
+        t0 = time.perf_counter()
+        raster, deriv, _ = oracle.rasterize_forward(clip, scene.indices, res)
+        out = oracle.interpolate_forward(scene.attributes, scene.indices, raster)
+        g = (2.0 * (out - tgt)).astype(np.float32)
+        _, gr = oracle.interpolate_backward(scene.attributes, scene.indices, raster, g)
+        oracle.rasterize_backward(clip, scene.indices, raster, gr)
+        t_total += time.perf_counter() - t0
+        done_px += res[0] * res[1]
+        used += 1
+        if t_total > budget_s:
+            break
+    return {
+        "value": done_px / t_total / 1e6,
+        "unit": "Mpix/s",
+        "cores": oracle.num_threads(),
+        "kind": "port",
+        "sample": f"views 0-{used - 1} of the 16-view workload ({done_px} px), full fwd+bwd chain; the attribute "
+                  "gradient uses per-vertex pixel lists (the reference's sums in its order, not its O(V*A*P) loop)",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-views", type=int, default=16)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU layout "
+              "(views sharded across GPUs, one gradient all-reduce per step) is design-only.", file=sys.stderr)
+        sys.exit(2)
+
+    import ctypes
+
+    import torch
+
+    from photara_drender_amd import _native, scenes
+
+    scene = scenes.multiview_scene()
+    dev = torch.device("cuda:0")
+    V, T, A = scene.clips[0].shape[0], scene.indices.shape[0], scene.attributes.shape[1]
+    pixels = [h * w for h, w in scene.resolutions]
+    P = sum(pixels)
+    offs = np.concatenate([[0], np.cumsum(pixels)])
+    idx = torch.tensor(scene.indices.view(np.int32), device=dev)
+    attr = torch.tensor(scene.attributes, device=dev)
+    clips = torch.tensor(np.stack(scene.clips), device=dev)
+    target = torch.tensor(np.concatenate([t.reshape(-1, A) for t in scene.targets]), device=dev)
+    raster = torch.empty((P, 4), device=dev)
+    deriv = torch.empty((P, 4), device=dev)
+    interp = torch.empty((P, A), device=dev)
+    grad = torch.empty((P, A), device=dev)
+    grad_raster = torch.empty((P, 4), device=dev)
+    grad_attr = torch.empty((V, A), device=dev)
+    grad_pos = torch.empty((len(pixels), V, 4), device=dev)
+
+    n = len(pixels)
+    rviews = (_native.RasterView * n)()
+    iviews = (_native.InterpView * n)()
+    for k, (h, w) in enumerate(scene.resolutions):
+        o = int(offs[k])
+        rv = rviews[k]
+        rv.positions = clips[k].data_ptr()
+        rv.width, rv.height = w, h
+        rv.viewport = (ctypes.c_float * 4)(0, 0, 0, 0)
+        rv.raster = raster[o].data_ptr()
+        rv.derivatives = deriv[o].data_ptr()
+        rv.grad_raster = grad_raster[o].data_ptr()
+        rv.grad_positions = grad_pos[k].data_ptr()
+        iv = iviews[k]
+        iv.width, iv.height = w, h
+        iv.raster = raster[o].data_ptr()
+        iv.out = interp[o].data_ptr()
+        iv.grad = grad[o].data_ptr()
+        iv.grad_raster = grad_raster[o].data_ptr()
+
+    ctx = _native.Context(0, flags=0)  # device-input validation off: inputs are known-good
+    lib = _native.lib()
+    stream = torch.cuda.current_stream().cuda_stream
+
+    def step():
+        _native.check(lib.pdr_rasterize_forward_batched(ctx.handle, rviews, n, V, idx.data_ptr(), T, 0, 1, stream))
+        _native.check(lib.pdr_interpolate_forward_batched(ctx.handle, iviews, n, attr.data_ptr(), V, A,
+                                                          idx.data_ptr(), T, stream))
+        torch.sub(interp, target, out=grad).mul_(2.0)
+        _native.check(lib.pdr_interpolate_backward_batched(ctx.handle, iviews, n, attr.data_ptr(), V, A,
+                                                           idx.data_ptr(), T, grad_attr.data_ptr(), stream))
+        _native.check(lib.pdr_rasterize_backward_batched(ctx.handle, rviews, n, V, idx.data_ptr(), T, stream))
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    ctx.set_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = elapsed / args.steps * 1e3
+    kernels = {}
+    for name in ("bin_count", "bin_fill", "raster_fwd", "interp_fwd", "interp_bwd", "raster_bwd"):
+        ms, launches = ctx.kernel_time(name)
+        if launches:
+            kernels[name] = ms / launches
+    ctx.set_timing(False)
+    alg = algorithmic_bytes(V, T, A, pixels)
+    dominant = max((k for k in kernels if k in alg), key=lambda k: kernels[k])
+    achieved = alg[dominant] / (kernels[dominant] * 1e-3) / 1e9
+    result = {
+        "metric": "Mpix/s raster+interp fwd+bwd (16 uneven views, 200k tris, A=8)",
+        "value": P / (ms_per_step * 1e-3) / 1e6,
+        "unit": "Mpix/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded heightfield, N(0,1) attributes and targets)",
+        "config": {"workload": "batched multi-view raster: 16 views 512^2..1920x1080 (15,163,392 px), "
+                               "heightfield 200,000 tris, 8 attributes, fwd+bwd",
+                   "views": n, "pixels": P, "triangles": T, "vertices": V, "attributes": A},
+        "kernel_ms": {k: round(v, 4) for k, v in kernels.items()},
+        "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "alg_bytes_per_launch": alg[dominant]},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        result["cpu_baseline"] = cpu_baseline(scene, args.cpu_views)
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()

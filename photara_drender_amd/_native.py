@@ -1,0 +1,146 @@
+"""ctypes binding of ``libpdr.so`` (the C ABI of ``include/pdr.h``).
+
+Used by the zero-copy PyTorch path (device pointers, torch's current stream)
+and by the benchmark.  Loading fails loudly when the library is missing: there
+is no fallback implementation.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpdr.so")
+
+_F = ctypes.c_float
+_U32 = ctypes.c_uint32
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+
+PDR_FLAG_VALIDATE_DEVICE_INPUTS = 1
+PDR_FLAG_DETERMINISTIC = 2
+
+CULL_MODES = {"none": 0, "back": 1, "front": 2}
+ADDRESS_MODES = {"clamp": 0, "wrap": 1, "mirror": 2}
+
+_ERRORS = {1: ValueError, 2: IndexError, 3: OverflowError, 4: RuntimeError}
+
+
+class RasterView(ctypes.Structure):
+    _fields_ = [("positions", _P), ("width", _U32), ("height", _U32), ("viewport", _F * 4), ("raster", _P),
+                ("derivatives", _P), ("grad_raster", _P), ("grad_positions", _P)]
+
+
+class InterpView(ctypes.Structure):
+    _fields_ = [("width", _U32), ("height", _U32), ("raster", _P), ("out", _P), ("grad", _P), ("grad_raster", _P)]
+
+
+class DeviceInfo(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 256), ("vendor_id", _U32), ("device_id", _U32), ("api_version", _U32),
+                ("supports_ray_query", _U32)]
+
+
+class RefineView(ctypes.Structure):
+    _fields_ = [("width", _U32), ("height", _U32), ("channels", _U32), ("image", _P), ("visibility_mask", _P),
+                ("world_to_clip", _F * 16), ("has_viewport", _I), ("viewport", _F * 4)]
+
+
+class RefineOptions(ctypes.Structure):
+    _fields_ = [("width", _U32), ("height", _U32), ("steps", _U32), ("batch_size", _U32), ("learning_rate", _F),
+                ("minimum_learning_rate", _F), ("photometric_epsilon", _F), ("adam_beta1", _F), ("adam_beta2", _F),
+                ("adam_epsilon", _F), ("clamp_min", _F), ("clamp_max", _F), ("seam_polish_steps", _U32),
+                ("seam_learning_rate", _F), ("seam_epsilon", _F), ("tv_weight", _F), ("tv_epsilon", _F),
+                ("chart_ids", _P), ("joint_seam_weight", _F), ("use_graph", _U32), ("checkpoint_every", _U32),
+                ("checkpoints", _P)]
+
+
+_SIGNATURES = {
+    "pdr_last_error": (ctypes.c_char_p, []),
+    "pdr_context_create": (_I, [_U32, _I, ctypes.POINTER(_P)]),
+    "pdr_context_destroy": (None, [_P]),
+    "pdr_context_set_flags": (_I, [_P, _U32]),
+    "pdr_context_flags": (_U32, [_P]),
+    "pdr_context_device_info": (_I, [_P, ctypes.POINTER(DeviceInfo)]),
+    "pdr_enumerate_devices": (_I, [ctypes.POINTER(DeviceInfo), _U32, ctypes.POINTER(_U32)]),
+    "pdr_synchronize": (_I, [_P, _P]),
+    "pdr_context_set_timing": (_I, [_P, _I]),
+    "pdr_context_kernel_time": (_I, [_P, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
+                                     ctypes.POINTER(ctypes.c_uint64)]),
+    "pdr_rasterize_forward": (_I, [_P, _P, _U32, _P, _U32, _U32, _U32, _U32, _I, _P, _P, _P, _P, _P]),
+    "pdr_rasterize_forward_batched": (_I, [_P, ctypes.POINTER(RasterView), _U32, _U32, _P, _U32, _U32, _I, _P]),
+    "pdr_rasterize_backward": (_I, [_P, _P, _U32, _P, _U32, _P, _U32, _U32, _P, _P, _P, _P]),
+    "pdr_rasterize_backward_batched": (_I, [_P, ctypes.POINTER(RasterView), _U32, _U32, _P, _U32, _P]),
+    "pdr_interpolate_forward": (_I, [_P, _P, _U32, _U32, _P, _U32, _P, _U32, _U32, _P, _P]),
+    "pdr_interpolate_backward": (_I, [_P, _P, _U32, _U32, _P, _U32, _P, _U32, _U32, _P, _P, _P, _P]),
+    "pdr_interpolate_forward_batched": (_I, [_P, ctypes.POINTER(InterpView), _U32, _P, _U32, _U32, _P, _U32, _P]),
+    "pdr_interpolate_backward_batched": (_I, [_P, ctypes.POINTER(InterpView), _U32, _P, _U32, _U32, _P, _U32, _P,
+                                              _P]),
+    "pdr_texture_forward": (_I, [_P, _P, _U32, _U32, _U32, _U32, _P, _P, _U32, _U32, _P, _P]),
+    "pdr_texture_backward": (_I, [_P, _P, _U32, _U32, _U32, _U32, _P, _P, _U32, _U32, _P, _I, _P, _P, _P]),
+    "pdr_refine_options_default": (None, [ctypes.POINTER(RefineOptions)]),
+    "pdr_refine": (_I, [_P, _P, _P, _U32, _P, _P, _U32, ctypes.POINTER(RefineView), _U32, _P, _U32,
+                        ctypes.POINTER(RefineOptions), _P, _P, _P, ctypes.POINTER(ctypes.c_double)]),
+}
+
+EXPORTED_SYMBOLS = tuple(_SIGNATURES)
+
+_lib = None
+
+
+def lib():
+    """The loaded libpdr (raises OSError if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise OSError(f"libpdr.so not found at {LIB_PATH}; run __graft_entry__.build() first")
+        handle = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+    return _lib
+
+
+def check(status: int) -> None:
+    if status:
+        raise _ERRORS.get(status, RuntimeError)(lib().pdr_last_error().decode())
+
+
+class Context:
+    """Owning handle of a pdr_context (device index, stream, scratch)."""
+
+    def __init__(self, device_index: int = 0, flags: int | None = None):
+        h = _P()
+        check(lib().pdr_context_create(device_index, 0, ctypes.byref(h)))
+        self.handle = h.value
+        if flags is not None:
+            check(lib().pdr_context_set_flags(self.handle, flags))
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().pdr_context_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_flags(self, flags: int) -> None:
+        check(lib().pdr_context_set_flags(self.handle, flags))
+
+    def set_timing(self, enable: bool) -> None:
+        check(lib().pdr_context_set_timing(self.handle, int(bool(enable))))
+
+    def kernel_time(self, name: str):
+        """(total milliseconds, launches) of one kernel since timing was enabled."""
+        ms = ctypes.c_double()
+        n = ctypes.c_uint64()
+        check(lib().pdr_context_kernel_time(self.handle, name.encode(), ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
+    @property
+    def flags(self) -> int:
+        return int(lib().pdr_context_flags(self.handle))

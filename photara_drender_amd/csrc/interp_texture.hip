@@ -1,0 +1,367 @@
+// Attribute interpolation and bilinear texture sampling, forward and backward.
+//
+// Forward kernels run one lane per pixel over all views of a batch
+// (shaders/interpolate_forward.hlsl:15-54, texture_forward.hlsl:37-79) and
+// produce the reference's values bit for bit.
+//
+// Backward kernels turn the reference's gathers into per-pixel scatters:
+//  * interpolation: grad_raster is per pixel (interpolate_grad_raster.hlsl, exact);
+//    the attribute gradient (interpolate_grad_attributes.hlsl loops over every
+//    pixel for every (vertex, attribute): O(V*A*P)) becomes 3*A terms per pixel,
+//    summed per triangle in an LDS hash table over a 16x16 tile and flushed with
+//    float atomics;
+//  * texture: grad_uv is per pixel (texture_grad_uv.hlsl, exact); the texel
+//    gradient (texture_grad_texture.hlsl over the CPU-built CSR of
+//    src/rasterizer.cpp:187-258) becomes 4*C atomic terms per pixel, with the
+//    reference's double-precision bilinear fractions.
+// Each scattered term is the reference's term exactly; only the order in which
+// terms are summed differs.
+#include "pdr_internal.hpp"
+
+namespace pdr {
+namespace {
+
+constexpr uint32_t kBlock = 256;
+
+__device__ __forceinline__ uint32_t find_pixel_view(const PixelTable& pt, uint32_t gp) {
+    uint32_t k = 0;
+    while (k + 1 < pt.n && gp >= pt.v[k + 1].pixel_base) ++k;
+    return k;
+}
+
+__device__ __forceinline__ uint32_t find_tile_view(const PixelTable& pt, uint32_t g) {
+    uint32_t k = 0;
+    while (k + 1 < pt.n && g >= pt.v[k + 1].tile_base) ++k;
+    return k;
+}
+
+template <int kA>
+__global__ __launch_bounds__(kBlock) void k_interp_fwd(PixelTable pt, const float* __restrict__ attr, uint32_t V,
+                                                       uint32_t A_rt, const uint32_t* __restrict__ idx, uint32_t T) {
+    const uint32_t A = kA > 0 ? static_cast<uint32_t>(kA) : A_rt;
+    const uint32_t gp = blockIdx.x * kBlock + threadIdx.x;
+    if (gp >= pt.total_pixels) return;
+    const PixelView& pv = pt.v[find_pixel_view(pt, gp)];
+    const uint32_t p = gp - pv.pixel_base;
+    const float4 r = reinterpret_cast<const float4*>(pv.raster)[p];
+    float* out = pv.out0 + static_cast<size_t>(p) * A;
+    const uint32_t id = decode_id(r.w);
+    uint32_t i0 = 0, i1 = 0, i2 = 0;
+    bool ok = id != 0 && id <= T;
+    if (ok) {
+        i0 = idx[(id - 1) * 3 + 0];
+        i1 = idx[(id - 1) * 3 + 1];
+        i2 = idx[(id - 1) * 3 + 2];
+        ok = i0 < V && i1 < V && i2 < V;
+    }
+    if (!ok) {
+        for (uint32_t a = 0; a < A; ++a) out[a] = 0.0f;
+        return;
+    }
+    const float u = r.x, v = r.y;
+    const float w = 1.0f - u - v;
+    const float* a0 = attr + static_cast<size_t>(i0) * A;
+    const float* a1 = attr + static_cast<size_t>(i1) * A;
+    const float* a2 = attr + static_cast<size_t>(i2) * A;
+    if constexpr (kA == 8) {
+        float o[8];
+#pragma unroll
+        for (int a = 0; a < 8; ++a) o[a] = u * a0[a] + v * a1[a] + w * a2[a];
+        reinterpret_cast<float4*>(out)[0] = make_float4(o[0], o[1], o[2], o[3]);
+        reinterpret_cast<float4*>(out)[1] = make_float4(o[4], o[5], o[6], o[7]);
+    } else if constexpr (kA == 2) {
+        reinterpret_cast<float2*>(out)[0] = make_float2(u * a0[0] + v * a1[0] + w * a2[0], u * a0[1] + v * a1[1] + w * a2[1]);
+    } else {
+        for (uint32_t a = 0; a < A; ++a) out[a] = u * a0[a] + v * a1[a] + w * a2[a];
+    }
+}
+
+constexpr uint32_t kSlots = 256;        // <= pixels per tile, so probing always ends
+constexpr uint32_t kMaxHashedAttrs = 8;  // 3*A accumulators per slot
+
+__device__ __forceinline__ uint32_t hash_slot(uint32_t* keys, uint32_t key) {
+    uint32_t h = (key * 2654435761u) >> 24;
+    while (true) {
+        const uint32_t old = atomicCAS(&keys[h], 0u, key);
+        if (old == 0u || old == key) return h;
+        h = (h + 1) & (kSlots - 1);
+    }
+}
+
+// One 16x16 tile per workgroup.  grad_raster is written per pixel; the
+// attribute gradient terms b_c * g[p,a] of the pixel's triangle accumulate in
+// LDS per triangle, then one atomic per (triangle corner, attribute).
+__global__ __launch_bounds__(kBlock) void k_interp_bwd(PixelTable pt, const float* __restrict__ attr, uint32_t V,
+                                                       uint32_t A, const uint32_t* __restrict__ idx, uint32_t T,
+                                                       float* __restrict__ grad_attr) {
+    __shared__ uint32_t keys[kSlots];
+    __shared__ float acc[kSlots * 3 * kMaxHashedAttrs];
+    const bool hashed = A <= kMaxHashedAttrs;
+    const uint32_t stride = 3 * A;
+    if (hashed) {
+        for (uint32_t s = threadIdx.x; s < kSlots; s += kBlock) keys[s] = 0u;
+        for (uint32_t s = threadIdx.x; s < kSlots * stride; s += kBlock) acc[s] = 0.0f;
+        __syncthreads();
+    }
+    const uint32_t g = xcd_remap(blockIdx.x, gridDim.x);
+    const PixelView& pv = pt.v[find_tile_view(pt, g)];
+    const uint32_t local = g - pv.tile_base;
+    const uint32_t px = (local % pv.tiles_x) * kTile + (threadIdx.x & 15u);
+    const uint32_t py = (local / pv.tiles_x) * kTile + (threadIdx.x >> 4);
+    if (px < pv.width && py < pv.height) {
+        const size_t p = static_cast<size_t>(py) * pv.width + px;
+        const float4 r = reinterpret_cast<const float4*>(pv.raster)[p];
+        const uint32_t id = decode_id(r.w);
+        const float* gin = pv.in0 + p * A;
+        float gx = 0.0f, gy = 0.0f;
+        if (id != 0 && id <= T) {
+            const uint32_t i0 = idx[(id - 1) * 3 + 0], i1 = idx[(id - 1) * 3 + 1], i2 = idx[(id - 1) * 3 + 2];
+            const bool in_range = i0 < V && i1 < V && i2 < V;
+            const float u = r.x, v = r.y;
+            const float w = 1.0f - u - v;
+            uint32_t slot = 0;
+            if (hashed) slot = hash_slot(keys, id);
+            for (uint32_t a = 0; a < A; ++a) {
+                const float og = gin[a];
+                if (in_range) {
+                    // interpolate_grad_raster.hlsl:153-161
+                    const float a0 = attr[static_cast<size_t>(i0) * A + a];
+                    const float a1 = attr[static_cast<size_t>(i1) * A + a];
+                    const float a2 = attr[static_cast<size_t>(i2) * A + a];
+                    gx += og * (a0 - a2);
+                    gy += og * (a1 - a2);
+                }
+                // interpolate_grad_attributes.hlsl:97-110 (vertex indices >= V never match)
+                const float t0 = u * og, t1 = v * og, t2 = w * og;
+                if (hashed) {
+                    atomicAdd(&acc[slot * stride + a], t0);
+                    atomicAdd(&acc[slot * stride + A + a], t1);
+                    atomicAdd(&acc[slot * stride + 2 * A + a], t2);
+                } else {
+                    if (i0 < V) unsafeAtomicAdd(&grad_attr[static_cast<size_t>(i0) * A + a], t0);
+                    if (i1 < V) unsafeAtomicAdd(&grad_attr[static_cast<size_t>(i1) * A + a], t1);
+                    if (i2 < V) unsafeAtomicAdd(&grad_attr[static_cast<size_t>(i2) * A + a], t2);
+                }
+            }
+        }
+        reinterpret_cast<float4*>(pv.out1)[p] = make_float4(gx, gy, 0.0f, 0.0f);
+    }
+    if (!hashed) return;
+    __syncthreads();
+    for (uint32_t s = threadIdx.x; s < kSlots; s += kBlock) {
+        const uint32_t key = keys[s];
+        if (!key) continue;
+        const uint32_t t = key - 1;
+        for (int c = 0; c < 3; ++c) {
+            const uint32_t vi = idx[t * 3 + c];
+            if (vi >= V) continue;
+            for (uint32_t a = 0; a < A; ++a)
+                unsafeAtomicAdd(&grad_attr[static_cast<size_t>(vi) * A + a], acc[s * stride + c * A + a]);
+        }
+    }
+}
+
+struct Bilinear {
+    uint32_t o00, o10, o01, o11;  // texel indices
+    float w0, w1, w2, w3;
+    float fx, fy;
+};
+
+// texture_forward.hlsl:224-242 (float fractions)
+__device__ __forceinline__ Bilinear bilinear_f32(float u, float v, uint32_t tw, uint32_t th, uint32_t mode) {
+    const float tx = u * static_cast<float>(tw) - 0.5f;
+    const float ty = v * static_cast<float>(th) - 0.5f;
+    const float flx = floorf(tx), fly = floorf(ty);
+    const int32_t bx = f2i_sat(flx), by = f2i_sat(fly);
+    Bilinear b;
+    b.fx = tx - flx;
+    b.fy = ty - fly;
+    const int32_t x0 = address_coordinate(bx, tw, mode), x1 = address_coordinate(add1(bx), tw, mode);
+    const int32_t y0 = address_coordinate(by, th, mode), y1 = address_coordinate(add1(by), th, mode);
+    b.o00 = static_cast<uint32_t>(y0) * tw + static_cast<uint32_t>(x0);
+    b.o10 = static_cast<uint32_t>(y0) * tw + static_cast<uint32_t>(x1);
+    b.o01 = static_cast<uint32_t>(y1) * tw + static_cast<uint32_t>(x0);
+    b.o11 = static_cast<uint32_t>(y1) * tw + static_cast<uint32_t>(x1);
+    b.w0 = (1.0f - b.fx) * (1.0f - b.fy);
+    b.w1 = b.fx * (1.0f - b.fy);
+    b.w2 = (1.0f - b.fx) * b.fy;
+    b.w3 = b.fx * b.fy;
+    return b;
+}
+
+// build_texture_sample_adjacency (src/rasterizer.cpp:207-235): double fractions.
+__device__ __forceinline__ Bilinear bilinear_f64(float u, float v, uint32_t tw, uint32_t th, uint32_t mode) {
+    const double tx = static_cast<double>(u) * tw - 0.5;
+    const double ty = static_cast<double>(v) * th - 0.5;
+    const double flx = floor(tx), fly = floor(ty);
+    const int32_t bx = d2i_sat(flx), by = d2i_sat(fly);
+    Bilinear b;
+    b.fx = static_cast<float>(tx - flx);
+    b.fy = static_cast<float>(ty - fly);
+    const int32_t x0 = address_coordinate(bx, tw, mode), x1 = address_coordinate(add1(bx), tw, mode);
+    const int32_t y0 = address_coordinate(by, th, mode), y1 = address_coordinate(add1(by), th, mode);
+    b.o00 = static_cast<uint32_t>(y0) * tw + static_cast<uint32_t>(x0);
+    b.o10 = static_cast<uint32_t>(y0) * tw + static_cast<uint32_t>(x1);
+    b.o01 = static_cast<uint32_t>(y1) * tw + static_cast<uint32_t>(x0);
+    b.o11 = static_cast<uint32_t>(y1) * tw + static_cast<uint32_t>(x1);
+    b.w0 = (1.0f - b.fx) * (1.0f - b.fy);
+    b.w1 = b.fx * (1.0f - b.fy);
+    b.w2 = (1.0f - b.fx) * b.fy;
+    b.w3 = b.fx * b.fy;
+    return b;
+}
+
+template <int kC>
+__global__ __launch_bounds__(kBlock) void k_texture_fwd(PixelTable pt, const float* __restrict__ tex, uint32_t tw,
+                                                        uint32_t th, uint32_t C_rt, uint32_t mode) {
+    const uint32_t C = kC > 0 ? static_cast<uint32_t>(kC) : C_rt;
+    const uint32_t gp = blockIdx.x * kBlock + threadIdx.x;
+    if (gp >= pt.total_pixels) return;
+    const PixelView& pv = pt.v[find_pixel_view(pt, gp)];
+    const uint32_t p = gp - pv.pixel_base;
+    float* out = pv.out0 + static_cast<size_t>(p) * C;
+    if (decode_id(pv.raster[static_cast<size_t>(p) * 4 + 3]) == 0) {
+        if constexpr (kC == 4) {
+            reinterpret_cast<float4*>(out)[0] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        } else {
+            for (uint32_t c = 0; c < C; ++c) out[c] = 0.0f;
+        }
+        return;
+    }
+    const float2 uv = reinterpret_cast<const float2*>(pv.in0)[p];
+    const Bilinear b = bilinear_f32(uv.x, uv.y, tw, th, mode);
+    if constexpr (kC == 4) {
+        const float4* T4 = reinterpret_cast<const float4*>(tex);
+        const float4 c00 = T4[b.o00], c10 = T4[b.o10], c01 = T4[b.o01], c11 = T4[b.o11];
+        reinterpret_cast<float4*>(out)[0] =
+            make_float4(c00.x * b.w0 + c10.x * b.w1 + c01.x * b.w2 + c11.x * b.w3,
+                        c00.y * b.w0 + c10.y * b.w1 + c01.y * b.w2 + c11.y * b.w3,
+                        c00.z * b.w0 + c10.z * b.w1 + c01.z * b.w2 + c11.z * b.w3,
+                        c00.w * b.w0 + c10.w * b.w1 + c01.w * b.w2 + c11.w * b.w3);
+    } else {
+        for (uint32_t c = 0; c < C; ++c)
+            out[c] = tex[static_cast<size_t>(b.o00) * C + c] * b.w0 + tex[static_cast<size_t>(b.o10) * C + c] * b.w1 +
+                     tex[static_cast<size_t>(b.o01) * C + c] * b.w2 + tex[static_cast<size_t>(b.o11) * C + c] * b.w3;
+    }
+}
+
+// texture_grad_texture (scatter form) + texture_grad_uv for one view.
+template <int kC>
+__global__ __launch_bounds__(kBlock) void k_texture_bwd(PixelTable pt, const float* __restrict__ tex, uint32_t tw,
+                                                        uint32_t th, uint32_t C_rt, uint32_t mode, int compute_uv,
+                                                        float* __restrict__ grad_tex) {
+    const uint32_t C = kC > 0 ? static_cast<uint32_t>(kC) : C_rt;
+    const uint32_t gp = blockIdx.x * kBlock + threadIdx.x;
+    if (gp >= pt.total_pixels) return;
+    const PixelView& pv = pt.v[find_pixel_view(pt, gp)];
+    const uint32_t p = gp - pv.pixel_base;
+    const float idf = pv.raster[static_cast<size_t>(p) * 4 + 3];
+    const float2 uv = reinterpret_cast<const float2*>(pv.in0)[p];
+    const float* g = pv.in1 + static_cast<size_t>(p) * C;
+    const bool finite = __builtin_isfinite(uv.x) && __builtin_isfinite(uv.y);
+    if (idf != 0.0f && finite) {
+        const Bilinear b = bilinear_f64(uv.x, uv.y, tw, th, mode);
+        for (uint32_t c = 0; c < C; ++c) {
+            const float gv = g[c];
+            unsafeAtomicAdd(&grad_tex[static_cast<size_t>(b.o00) * C + c], b.w0 * gv);
+            unsafeAtomicAdd(&grad_tex[static_cast<size_t>(b.o10) * C + c], b.w1 * gv);
+            unsafeAtomicAdd(&grad_tex[static_cast<size_t>(b.o01) * C + c], b.w2 * gv);
+            unsafeAtomicAdd(&grad_tex[static_cast<size_t>(b.o11) * C + c], b.w3 * gv);
+        }
+    }
+    if (!compute_uv) return;
+    float2 gu = make_float2(0.0f, 0.0f);
+    if (decode_id(idf) != 0) {
+        const Bilinear b = bilinear_f32(uv.x, uv.y, tw, th, mode);
+        for (uint32_t c = 0; c < C; ++c) {
+            const float c00 = tex[static_cast<size_t>(b.o00) * C + c];
+            const float c10 = tex[static_cast<size_t>(b.o10) * C + c];
+            const float c01 = tex[static_cast<size_t>(b.o01) * C + c];
+            const float c11 = tex[static_cast<size_t>(b.o11) * C + c];
+            const float gv = g[c];
+            const float dx = (1.0f - b.fy) * (c10 - c00) + b.fy * (c11 - c01);
+            const float dy = (1.0f - b.fx) * (c01 - c00) + b.fx * (c11 - c10);
+            gu.x += gv * (dx * static_cast<float>(tw));
+            gu.y += gv * (dy * static_cast<float>(th));
+        }
+    }
+    reinterpret_cast<float2*>(pv.out1)[p] = gu;
+}
+
+// src/rasterizer.cpp:198-211 checks, reported as the first offending pixel of
+// each kind (flags[0]: non-finite uv, flags[1]: |texel coordinate| > 1e9).
+__global__ __launch_bounds__(kBlock) void k_texture_check_uv(PixelTable pt, uint32_t tw, uint32_t th,
+                                                             uint32_t* flags) {
+    const uint32_t gp = blockIdx.x * kBlock + threadIdx.x;
+    if (gp >= pt.total_pixels) return;
+    const PixelView& pv = pt.v[find_pixel_view(pt, gp)];
+    const uint32_t p = gp - pv.pixel_base;
+    if (pv.raster[static_cast<size_t>(p) * 4 + 3] == 0.0f) return;
+    const float2 uv = reinterpret_cast<const float2*>(pv.in0)[p];
+    if (!__builtin_isfinite(uv.x) || !__builtin_isfinite(uv.y)) {
+        atomicMin(&flags[0], gp);
+        return;
+    }
+    const double tx = static_cast<double>(uv.x) * tw - 0.5, ty = static_cast<double>(uv.y) * th - 0.5;
+    if (fabs(tx) > 1.0e9 || fabs(ty) > 1.0e9) atomicMin(&flags[1], gp);
+}
+
+uint32_t pixel_grid(uint32_t n) { return (n + kBlock - 1) / kBlock; }
+
+}  // namespace
+
+hipError_t interp_forward(const PixelTable& pt, const float* attr, uint32_t V, uint32_t A, const uint32_t* idx,
+                          uint32_t T, hipStream_t st) {
+    if (!pt.total_pixels) return hipSuccess;
+    const uint32_t grid = pixel_grid(pt.total_pixels);
+    bool aligned = (reinterpret_cast<uintptr_t>(attr) % 16) == 0;
+    for (uint32_t k = 0; k < pt.n; ++k) aligned &= (reinterpret_cast<uintptr_t>(pt.v[k].out0) % 16) == 0;
+    if (A == 8 && aligned)
+        k_interp_fwd<8><<<grid, kBlock, 0, st>>>(pt, attr, V, A, idx, T);
+    else if (A == 2 && aligned)
+        k_interp_fwd<2><<<grid, kBlock, 0, st>>>(pt, attr, V, A, idx, T);
+    else
+        k_interp_fwd<0><<<grid, kBlock, 0, st>>>(pt, attr, V, A, idx, T);
+    return hipGetLastError();
+}
+
+hipError_t interp_backward(const PixelTable& pt, const float* attr, uint32_t V, uint32_t A, const uint32_t* idx,
+                           uint32_t T, float* grad_attr, hipStream_t st) {
+    if (!pt.total_tiles) return hipSuccess;
+    k_interp_bwd<<<pt.total_tiles, kBlock, 0, st>>>(pt, attr, V, A, idx, T, grad_attr);
+    return hipGetLastError();
+}
+
+hipError_t texture_forward(const PixelTable& pt, const float* tex, uint32_t tw, uint32_t th, uint32_t C, uint32_t mode,
+                           hipStream_t st) {
+    if (!pt.total_pixels) return hipSuccess;
+    const uint32_t grid = pixel_grid(pt.total_pixels);
+    bool aligned = (reinterpret_cast<uintptr_t>(tex) % 16) == 0;
+    for (uint32_t k = 0; k < pt.n; ++k) aligned &= (reinterpret_cast<uintptr_t>(pt.v[k].out0) % 16) == 0;
+    if (C == 4 && aligned)
+        k_texture_fwd<4><<<grid, kBlock, 0, st>>>(pt, tex, tw, th, C, mode);
+    else
+        k_texture_fwd<0><<<grid, kBlock, 0, st>>>(pt, tex, tw, th, C, mode);
+    return hipGetLastError();
+}
+
+hipError_t texture_backward(const PixelTable& pt, const float* tex, uint32_t tw, uint32_t th, uint32_t C,
+                            uint32_t mode, int compute_uv, float* grad_tex, hipStream_t st) {
+    if (!pt.total_pixels) return hipSuccess;
+    const uint32_t grid = pixel_grid(pt.total_pixels);
+    if (C == 4)
+        k_texture_bwd<4><<<grid, kBlock, 0, st>>>(pt, tex, tw, th, C, mode, compute_uv, grad_tex);
+    else if (C == 3)
+        k_texture_bwd<3><<<grid, kBlock, 0, st>>>(pt, tex, tw, th, C, mode, compute_uv, grad_tex);
+    else
+        k_texture_bwd<0><<<grid, kBlock, 0, st>>>(pt, tex, tw, th, C, mode, compute_uv, grad_tex);
+    return hipGetLastError();
+}
+
+hipError_t texture_check_uv(const PixelTable& pt, uint32_t tw, uint32_t th, uint32_t* flags, hipStream_t st) {
+    if (!pt.total_pixels) return hipSuccess;
+    k_texture_check_uv<<<pixel_grid(pt.total_pixels), kBlock, 0, st>>>(pt, tw, th, flags);
+    return hipGetLastError();
+}
+
+}  // namespace pdr

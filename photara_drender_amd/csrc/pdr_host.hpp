@@ -1,0 +1,340 @@
+// Host-side runtime shared by the libpdr C ABI translation units: the context
+// (stream, device scratch, pinned readback area), error plumbing, host staging
+// and argument validation with the reference's messages.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "pdr.h"
+#include "pdr_internal.hpp"
+
+namespace pdrh {
+
+extern thread_local std::string g_last_error;
+
+struct PdrError {
+    pdr_status status;
+    std::string message;
+};
+
+[[noreturn]] inline void raise(pdr_status s, const std::string& m) { throw PdrError{s, m}; }
+
+inline void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) raise(PDR_RUNTIME, std::string(what) + " failed: " + hipGetErrorString(e));
+}
+
+template <class F>
+pdr_status guarded(F&& f) {
+    try {
+        f();
+        return PDR_OK;
+    } catch (const PdrError& e) {
+        g_last_error = e.message;
+        return e.status;
+    } catch (const std::bad_alloc&) {
+        g_last_error = "out of host memory";
+        return PDR_RUNTIME;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return PDR_RUNTIME;
+    }
+}
+
+inline bool is_device_pointer(const void* p) {
+    if (!p) return false;
+    hipPointerAttribute_t attr;
+    const hipError_t e = hipPointerGetAttributes(&attr, p);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged;
+}
+
+// All non-null arrays of one call must live on the same side.
+inline bool classify(std::initializer_list<const void*> ptrs) {
+    int device = -1;
+    for (const void* p : ptrs) {
+        if (!p) continue;
+        const int d = is_device_pointer(p) ? 1 : 0;
+        if (device < 0) device = d;
+        else if (device != d)
+            raise(PDR_INVALID_ARGUMENT, "all arrays of one call must be host memory or all device memory");
+    }
+    return device == 1;
+}
+
+}  // namespace pdrh
+
+struct pdr_context_s {
+    int device = 0;
+    uint32_t flags = PDR_FLAG_VALIDATE_DEVICE_INPUTS;
+    hipStream_t stream = nullptr;
+    hipStream_t last_stream = nullptr;
+    hipEvent_t handoff = nullptr;
+    std::mutex mutex;
+    uint32_t* pinned = nullptr;  // small host-visible readback area
+    struct Slot {
+        void* ptr = nullptr;
+        size_t bytes = 0;
+    };
+    std::unordered_map<std::string, Slot> scratch;
+
+    // Optional per-kernel HIP-event timing (pdr_context_set_timing).
+    bool timing = false;
+    struct Region {
+        std::string name;
+        hipEvent_t start, stop;
+    };
+    std::vector<Region> pending;
+    std::vector<hipEvent_t> event_pool;
+    std::unordered_map<std::string, std::pair<double, uint64_t>> totals;
+
+    hipEvent_t take_event() {
+        if (!event_pool.empty()) {
+            hipEvent_t e = event_pool.back();
+            event_pool.pop_back();
+            return e;
+        }
+        hipEvent_t e = nullptr;
+        pdrh::hip_check(hipEventCreate(&e), "hipEventCreate");
+        return e;
+    }
+
+    // Runs f (which enqueues kernels on st) between two events when timing is on.
+    template <class F>
+    void timed(const char* name, hipStream_t st, F&& f) {
+        if (!timing) {
+            f();
+            return;
+        }
+        Region r{name, take_event(), take_event()};
+        pdrh::hip_check(hipEventRecord(r.start, st), "hipEventRecord");
+        f();
+        pdrh::hip_check(hipEventRecord(r.stop, st), "hipEventRecord");
+        pending.push_back(r);
+    }
+
+    void resolve_timing() {
+        for (Region& r : pending) {
+            pdrh::hip_check(hipEventSynchronize(r.stop), "hipEventSynchronize");
+            float ms = 0.0f;
+            pdrh::hip_check(hipEventElapsedTime(&ms, r.start, r.stop), "hipEventElapsedTime");
+            auto& t = totals[r.name];
+            t.first += ms;
+            t.second += 1;
+            event_pool.push_back(r.start);
+            event_pool.push_back(r.stop);
+        }
+        pending.clear();
+    }
+
+    // Makes `st` the stream that owns the scratch; orders it after the previous owner.
+    hipStream_t use_stream(void* user) {
+        hipStream_t st = user ? static_cast<hipStream_t>(user) : stream;
+        if (last_stream && last_stream != st) {
+            pdrh::hip_check(hipEventRecord(handoff, last_stream), "hipEventRecord");
+            pdrh::hip_check(hipStreamWaitEvent(st, handoff, 0), "hipStreamWaitEvent");
+        }
+        last_stream = st;
+        return st;
+    }
+
+    void* get(const std::string& key, size_t bytes) {
+        Slot& s = scratch[key];
+        if (bytes == 0) bytes = 16;
+        if (s.bytes < bytes) {
+            if (s.ptr) {
+                pdrh::hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+                pdrh::hip_check(hipFree(s.ptr), "hipFree");
+                s.ptr = nullptr;
+                s.bytes = 0;
+            }
+            const size_t grown = std::max(bytes, s.bytes + s.bytes / 2);
+            pdrh::hip_check(hipMalloc(&s.ptr, grown), "hipMalloc");
+            s.bytes = grown;
+        }
+        return s.ptr;
+    }
+    template <class T>
+    T* get_t(const std::string& key, size_t count) {
+        return static_cast<T*>(get(key, count * sizeof(T)));
+    }
+
+    ~pdr_context_s() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (auto& kv : scratch)
+            if (kv.second.ptr) (void)hipFree(kv.second.ptr);
+        for (auto& r : pending) {
+            (void)hipEventDestroy(r.start);
+            (void)hipEventDestroy(r.stop);
+        }
+        for (hipEvent_t e : event_pool) (void)hipEventDestroy(e);
+        if (pinned) (void)hipHostFree(pinned);
+        if (handoff) (void)hipEventDestroy(handoff);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace pdrh {
+
+struct Lock {
+    std::unique_lock<std::mutex> lock;
+    explicit Lock(pdr_context ctx) : lock(ctx->mutex) { hip_check(hipSetDevice(ctx->device), "hipSetDevice"); }
+};
+
+inline void require_ctx(pdr_context ctx) {
+    if (!ctx) raise(PDR_INVALID_ARGUMENT, "context is null");
+}
+
+inline void require_ptr(const void* p, const char* name) {
+    if (!p) raise(PDR_INVALID_ARGUMENT, std::string(name) + " is null");
+}
+
+// src/rasterizer.cpp:48-59
+inline void resolve_viewport(uint32_t W, uint32_t H, const float* vp, float out[4]) {
+    if (vp) {
+        std::memcpy(out, vp, 4 * sizeof(float));
+    } else {
+        out[0] = 0.0f;
+        out[1] = 0.0f;
+        out[2] = static_cast<float>(W);
+        out[3] = static_cast<float>(H);
+    }
+    if (!std::isfinite(out[0]) || !std::isfinite(out[1]) || !std::isfinite(out[2]) || !std::isfinite(out[3]) ||
+        out[2] <= 0.0f || out[3] <= 0.0f)
+        raise(PDR_INVALID_ARGUMENT, "viewport must contain finite values and positive dimensions");
+}
+
+// Copies `bytes` from host `src` into scratch `key` and returns the device copy.
+template <class T>
+const T* stage_in(pdr_context ctx, const std::string& key, const T* src, size_t count, hipStream_t st) {
+    if (!src || count == 0) return nullptr;
+    T* d = ctx->get_t<T>(key, count);
+    hip_check(hipMemcpyAsync(d, src, count * sizeof(T), hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+    return d;
+}
+
+inline void copy_out(void* host, const void* dev, size_t bytes, hipStream_t st) {
+    if (!host || !bytes) return;
+    hip_check(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
+}
+
+inline void sync(hipStream_t st) { hip_check(hipStreamSynchronize(st), "hipStreamSynchronize"); }
+
+inline void check_indices_host(const uint32_t* idx, size_t n, uint32_t V, pdr_status s, const char* msg) {
+    for (size_t i = 0; i < n; ++i)
+        if (idx[i] >= V) raise(s, msg);
+}
+
+inline void check_indices_device(pdr_context ctx, const uint32_t* idx, size_t n, uint32_t V, hipStream_t st, pdr_status s,
+                          const char* msg) {
+    if (!(ctx->flags & PDR_FLAG_VALIDATE_DEVICE_INPUTS)) return;
+    uint32_t* flag = ctx->get_t<uint32_t>("idx_flag", 1);
+    hip_check(pdr::fill_u32(flag, 0u, 1, st), "fill");
+    hip_check(pdr::check_indices(idx, n, V, flag, st), "check_indices");
+    hip_check(hipMemcpyAsync(ctx->pinned, flag, 4, hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
+    sync(st);
+    if (ctx->pinned[0]) raise(s, msg);
+}
+
+inline pdr::ViewDesc make_view(const float* pos, uint32_t W, uint32_t H, const float vp[4]) {
+    pdr::ViewDesc v{};
+    v.pos = pos;
+    v.width = W;
+    v.height = H;
+    v.tiles_x = (W + pdr::kTile - 1) / pdr::kTile;
+    v.tiles_y = (H + pdr::kTile - 1) / pdr::kTile;
+    v.vx = vp[0];
+    v.vy = vp[1];
+    v.vw = vp[2];
+    v.vh = vp[3];
+    return v;
+}
+
+inline void finalize_table(pdr::ViewTable& t) {
+    uint32_t tiles = 0, pixels = 0;
+    for (uint32_t k = 0; k < t.n; ++k) {
+        t.v[k].tile_base = tiles;
+        t.v[k].pixel_base = pixels;
+        const uint64_t nt = static_cast<uint64_t>(tiles) + static_cast<uint64_t>(t.v[k].tiles_x) * t.v[k].tiles_y;
+        const uint64_t np = static_cast<uint64_t>(pixels) + static_cast<uint64_t>(t.v[k].width) * t.v[k].height;
+        if (nt > 0x7FFFFFFFu || np > 0xFFFFFFFFu) raise(PDR_OVERFLOW, "Rasterization output size overflow");
+        tiles = static_cast<uint32_t>(nt);
+        pixels = static_cast<uint32_t>(np);
+    }
+    t.total_tiles = tiles;
+    t.total_pixels = pixels;
+}
+
+inline pdr::PixelTable pixel_table_one(const float* raster, const float* in0, const float* in1, float* out0, float* out1,
+                                uint32_t W, uint32_t H) {
+    pdr::PixelTable t{};
+    t.n = 1;
+    pdr::PixelView& v = t.v[0];
+    v.raster = raster;
+    v.in0 = in0;
+    v.in1 = in1;
+    v.out0 = out0;
+    v.out1 = out1;
+    v.width = W;
+    v.height = H;
+    v.tiles_x = (W + pdr::kTile - 1) / pdr::kTile;
+    v.tile_base = 0;
+    v.pixel_base = 0;
+    t.total_tiles = v.tiles_x * ((H + pdr::kTile - 1) / pdr::kTile);
+    t.total_pixels = W * H;
+    return t;
+}
+
+// Bins + raster forward of one table of views (device pointers), on stream st.
+inline void run_raster_forward(pdr_context ctx, const pdr::ViewTable& vt, const uint32_t* idx, uint32_t T, uint32_t V,
+                        uint32_t cull, int want_deriv, hipStream_t st) {
+    if (!vt.total_tiles) return;
+    pdr::BinScratch bs;
+    bs.counts = ctx->get_t<uint32_t>("bin_counts", vt.total_tiles);
+    bs.offsets = ctx->get_t<uint32_t>("bin_offsets", vt.total_tiles + 1);
+    bs.scan_temp_bytes = pdr::bin_scan_temp_bytes(vt.total_tiles);
+    bs.scan_temp = ctx->get("bin_scan_temp", bs.scan_temp_bytes);
+    ctx->timed("bin_count", st, [&] { hip_check(pdr::bin_count(vt, idx, T, V, bs, st), "bin_count"); });
+    hip_check(hipMemcpyAsync(ctx->pinned, bs.offsets + vt.total_tiles, 4, hipMemcpyDeviceToHost, st),
+              "hipMemcpyAsync");
+    sync(st);
+    const uint32_t total = ctx->pinned[0];
+    bs.list = ctx->get_t<uint32_t>("bin_list", std::max<uint32_t>(total, 1));
+    ctx->timed("bin_fill", st, [&] { hip_check(pdr::bin_fill(vt, idx, T, V, bs, st), "bin_fill"); });
+    ctx->timed("raster_fwd", st,
+               [&] { hip_check(pdr::raster_forward(vt, idx, T, V, cull, want_deriv, bs, st), "raster_forward"); });
+}
+
+inline void validate_mesh_counts(const float* pos, uint32_t V, const uint32_t* idx, uint32_t T) {
+    if (!pos || V == 0) raise(PDR_INVALID_ARGUMENT, "clip_positions must have shape [vertex_count, 4]");
+    if (!idx || T == 0) raise(PDR_INVALID_ARGUMENT, "triangle_indices must have shape [triangle_count, 3]");
+}
+
+inline uint32_t validate_texture(uint32_t tw, uint32_t th, uint32_t C, uint32_t W, uint32_t H) {
+    // src/rasterizer.cpp:260-289
+    if (tw == 0 || th == 0 || C == 0)
+        raise(PDR_INVALID_ARGUMENT, "Texture width, height, and channel_count must be positive");
+    if (tw > static_cast<uint32_t>(std::numeric_limits<int32_t>::max() / 2) ||
+        th > static_cast<uint32_t>(std::numeric_limits<int32_t>::max() / 2))
+        raise(PDR_OUT_OF_RANGE, "Texture dimensions exceed portable addressing limits");
+    if (W == 0 || H == 0) raise(PDR_INVALID_ARGUMENT, "raster_output has inconsistent dimensions");
+    const uint64_t P = static_cast<uint64_t>(W) * H;
+    if (P > 0xFFFFFFFFull || static_cast<uint64_t>(tw) * th > 0xFFFFFFFFull)
+        raise(PDR_OVERFLOW, "Texture operation exceeds backend element limits");
+    return static_cast<uint32_t>(P);
+}
+
+}  // namespace pdrh

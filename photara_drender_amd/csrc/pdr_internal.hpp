@@ -1,0 +1,125 @@
+// Host-side interface between the C ABI (pdr_api.cpp) and the kernel launchers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "pdr_device.hpp"
+
+namespace pdr {
+
+// Per-view table passed by value as a kernel argument (<= kMaxViews views).
+struct ViewDesc {
+    const float* pos;         // [V,4] clip positions
+    float* raster;            // [H,W,4]
+    float* deriv;             // [H,W,4] or null
+    const float* grad_raster; // [H,W,4]
+    float* grad_pos;          // [V,4]
+    uint32_t width, height;
+    uint32_t tiles_x, tiles_y;
+    float vx, vy, vw, vh;     // resolved viewport
+    uint32_t tile_base;       // first global tile of this view
+    uint32_t pixel_base;      // first global pixel of this view
+};
+
+struct ViewTable {
+    ViewDesc v[kMaxViews];
+    uint32_t n;
+    uint32_t total_tiles;
+    uint32_t total_pixels;
+};
+
+// Generic per-view pixel table for interpolation / texture kernels.
+struct PixelView {
+    const float* raster;   // [H,W,4]
+    const float* in0;      // kernel-specific input  [H,W,*]
+    const float* in1;      // kernel-specific input  [H,W,*]
+    float* out0;           // kernel-specific output [H,W,*]
+    float* out1;           // kernel-specific output [H,W,*]
+    uint32_t width, height;
+    uint32_t tiles_x;
+    uint32_t tile_base;
+    uint32_t pixel_base;
+};
+
+struct PixelTable {
+    PixelView v[kMaxViews];
+    uint32_t n;
+    uint32_t total_tiles;
+    uint32_t total_pixels;
+};
+
+// Scratch needed by the binning pass (owned by the context).
+struct BinScratch {
+    uint32_t* counts = nullptr;   // [total_tiles]
+    uint32_t* offsets = nullptr;  // [total_tiles + 1]
+    uint32_t* list = nullptr;     // [capacity]
+    void* scan_temp = nullptr;
+    size_t scan_temp_bytes = 0;
+};
+
+// ---- raster.hip
+size_t bin_scan_temp_bytes(uint32_t total_tiles);
+hipError_t bin_count(const ViewTable& views, const uint32_t* idx, uint32_t T, uint32_t V, const BinScratch& s,
+                     hipStream_t st);
+hipError_t bin_fill(const ViewTable& views, const uint32_t* idx, uint32_t T, uint32_t V, const BinScratch& s,
+                    hipStream_t st);
+hipError_t raster_forward(const ViewTable& views, const uint32_t* idx, uint32_t T, uint32_t V, uint32_t cull,
+                          int want_deriv, const BinScratch& s, hipStream_t st);
+// Output arrays (grad_pos / grad_attr / grad_tex) must be zeroed by the caller.
+hipError_t raster_backward(const ViewTable& views, const uint32_t* idx, uint32_t T, uint32_t V, hipStream_t st);
+
+// ---- interp_texture.hip
+hipError_t interp_forward(const PixelTable& views, const float* attr, uint32_t V, uint32_t A, const uint32_t* idx,
+                          uint32_t T, hipStream_t st);
+hipError_t interp_backward(const PixelTable& views, const float* attr, uint32_t V, uint32_t A, const uint32_t* idx,
+                           uint32_t T, float* grad_attr, hipStream_t st);
+hipError_t texture_forward(const PixelTable& views, const float* tex, uint32_t tw, uint32_t th, uint32_t C,
+                           uint32_t mode, hipStream_t st);
+hipError_t texture_backward(const PixelTable& views, const float* tex, uint32_t tw, uint32_t th, uint32_t C,
+                            uint32_t mode, int compute_uv, float* grad_tex, hipStream_t st);
+// uv validity check of texture_backward's covered pixels; flags[0] |= 1 non-finite, 2 too large
+hipError_t texture_check_uv(const PixelTable& views, uint32_t tw, uint32_t th, uint32_t* flags, hipStream_t st);
+// max index check; flags[0] = 1 when any index >= V
+hipError_t check_indices(const uint32_t* idx, size_t n, uint32_t V, uint32_t* flags, hipStream_t st);
+
+// ---- common kernels (raster.hip)
+hipError_t fill_u32(uint32_t* p, uint32_t value, size_t n, hipStream_t st);
+hipError_t fill_f32(float* p, float value, size_t n, hipStream_t st);
+
+// ---- refine.hip
+struct RefineStepTables {
+    const float* lr;        // [steps]
+    const float* bc1;       // [steps]
+    const float* bc2;       // [steps]
+    const float* norm;      // [steps]
+    const uint32_t* views;  // [steps * batch]
+};
+
+struct RefineView {
+    const float* uv;      // [P,2]
+    const float* target;  // [P,3]
+    const float* mask;    // [P]
+    uint32_t pixels;
+};
+
+hipError_t transform_positions(const float* pos3, uint32_t V, const float* m16_host, float* out4, hipStream_t st);
+hipError_t refine_view_inputs(const float* image, uint32_t channels, const float* vis, const float* raster, uint32_t P,
+                              float* target3, float* mask, double* partials, uint32_t npartials, hipStream_t st);
+uint32_t refine_partial_count(uint32_t P);
+hipError_t sum_partials(const double* partials, uint32_t n, double* out, hipStream_t st);
+hipError_t photometric_step(const RefineView* views_dev, uint32_t batch, uint32_t max_pixels, const float* tex,
+                            uint32_t tw, uint32_t th, float eps, const RefineStepTables& tables,
+                            const uint32_t* step_counter, float* grad, float* loss, hipStream_t st);
+hipError_t seam_step(const float* tex, uint32_t tw, uint32_t th, const float* pairs, uint32_t npairs, float eps,
+                     float norm, float weight, const uint32_t* step_counter, uint32_t loss_base, float* grad,
+                     float* loss, hipStream_t st);
+hipError_t tv_step(const float* tex, uint32_t tw, uint32_t th, const int32_t* chart, float eps, float weight_over_pairs,
+                   float* grad, hipStream_t st);
+hipError_t adam_step(float* tex, float* grad, float* m, float* v, uint32_t n, const float* lr_table,
+                     const float* bc1_table, const float* bc2_table, float b1, float b2, float eps, float cmin,
+                     float cmax, uint32_t* step_counter, uint32_t* done_counter, hipStream_t st);
+hipError_t tv_pair_count(const int32_t* chart, uint32_t tw, uint32_t th, unsigned long long* out, hipStream_t st);
+
+}  // namespace pdr

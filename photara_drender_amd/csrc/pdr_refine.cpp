@@ -1,0 +1,321 @@
+// pdr_refine: TextureRefiner::refine (src/texture_refiner.cpp:174-454) on the GPU.
+//
+// Precompute per view (texture_refiner.cpp:206-254): transform -> bin + raster
+// (no derivatives) -> interpolate uv -> target / mask / mask weight, all on the
+// device.  The optimisation loop keeps the reference's schedule exactly
+// (round-robin views, cosine learning rate, bias corrections computed on the
+// host with the same float library calls, seam polish with a reset Adam) but
+// reads every per-step scalar from device tables indexed by a device step
+// counter, so a block of steps is captured once as a hipGraph and replayed.
+#include <chrono>
+
+#include "pdr_host.hpp"
+
+using namespace pdrh;
+
+namespace {
+
+struct DeviceBuffers {
+    std::vector<void*> ptrs;
+    ~DeviceBuffers() {
+        for (void* p : ptrs) (void)hipFree(p);
+    }
+    template <class T>
+    T* alloc(size_t count) {
+        void* p = nullptr;
+        hip_check(hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)), "hipMalloc");
+        ptrs.push_back(p);
+        return static_cast<T*>(p);
+    }
+};
+
+struct GraphExec {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~GraphExec() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+};
+
+template <class T>
+const T* device_view(DeviceBuffers& bufs, const T* p, size_t count, hipStream_t st) {
+    if (!p) return nullptr;
+    if (is_device_pointer(p)) return p;
+    T* d = bufs.alloc<T>(count);
+    hip_check(hipMemcpyAsync(d, p, count * sizeof(T), hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+    return d;
+}
+
+void copy_any(void* dst, const void* src, size_t bytes, hipStream_t st) {
+    if (!dst || !bytes) return;
+    hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, st), "hipMemcpyAsync");
+}
+
+}  // namespace
+
+extern "C" {
+
+void pdr_refine_options_default(pdr_refine_options* o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    // include/photara_drender/types.hpp:267-283
+    o->width = 1024;
+    o->height = 1024;
+    o->steps = 1000;
+    o->batch_size = 4;
+    o->learning_rate = 5e-3f;
+    o->minimum_learning_rate = 2.5e-4f;
+    o->photometric_epsilon = 1e-3f;
+    o->adam_beta1 = 0.9f;
+    o->adam_beta2 = 0.999f;
+    o->adam_epsilon = 1e-8f;
+    o->clamp_min = 0.0f;
+    o->clamp_max = 1.0f;
+    o->seam_polish_steps = 30;
+    o->seam_learning_rate = 1e-3f;
+    o->seam_epsilon = 1e-4f;
+    o->tv_epsilon = 1e-4f;
+    o->use_graph = 1;
+}
+
+pdr_status pdr_refine(pdr_context ctx, const float* initial_texture, const float* positions, uint32_t V,
+                      const float* uv, const uint32_t* idx, uint32_t T, const pdr_refine_view* views, uint32_t nviews,
+                      const float* seam_pairs, uint32_t npairs, const pdr_refine_options* options, float* color_out,
+                      float* loss_out, float* seam_loss_out, double* times_out2) {
+    return guarded([&] {
+        require_ctx(ctx);
+        pdr_refine_options o;
+        if (options)
+            o = *options;
+        else
+            pdr_refine_options_default(&o);
+        // src/texture_refiner.cpp:182-204
+        const bool idx_device = is_device_pointer(idx);
+        if (!positions || V == 0 || !uv || !idx || T == 0)
+            raise(PDR_INVALID_ARGUMENT, "texture refinement mesh arrays have inconsistent shapes");
+        if (!idx_device)
+            check_indices_host(idx, static_cast<size_t>(T) * 3, V, PDR_INVALID_ARGUMENT,
+                               "texture refinement mesh arrays have inconsistent shapes");
+        const uint64_t texels = static_cast<uint64_t>(o.width) * o.height;
+        const uint64_t nvalues = texels * 3;
+        if (o.width == 0 || o.height == 0 || !initial_texture || !views || nviews == 0 || o.steps == 0 ||
+            o.batch_size == 0 || !std::isfinite(o.learning_rate) || o.learning_rate <= 0.0f ||
+            !std::isfinite(o.minimum_learning_rate) || o.minimum_learning_rate < 0.0f ||
+            o.minimum_learning_rate > o.learning_rate || !std::isfinite(o.photometric_epsilon) ||
+            o.photometric_epsilon <= 0.0f || !(o.adam_beta1 > 0.0f && o.adam_beta1 < 1.0f) ||
+            !(o.adam_beta2 > 0.0f && o.adam_beta2 < 1.0f) || !(o.clamp_min < o.clamp_max) ||
+            (npairs && !seam_pairs) || !std::isfinite(o.seam_learning_rate) || o.seam_learning_rate <= 0.0f ||
+            !std::isfinite(o.seam_epsilon) || o.seam_epsilon <= 0.0f)
+            raise(PDR_INVALID_ARGUMENT, "texture refinement options are invalid");
+        if (nvalues > 0xFFFFFFFFull) raise(PDR_OVERFLOW, "texture refinement atlas exceeds the shader element limit");
+        for (uint32_t k = 0; k < nviews; ++k) {
+            const pdr_refine_view& v = views[k];
+            if (v.width == 0 || v.height == 0 || (v.channels != 3 && v.channels != 4))
+                raise(PDR_INVALID_ARGUMENT, "texture refinement images must contain RGB or RGBA data");
+            if (!v.image || static_cast<uint64_t>(v.width) * v.height > 0xFFFFFFFFull)
+                raise(PDR_INVALID_ARGUMENT, "texture refinement image or mask has an invalid shape");
+        }
+        const float tv_eps = o.tv_epsilon > 0.0f ? o.tv_epsilon : 1e-4f;
+
+        Lock lock(ctx);
+        const hipStream_t st = ctx->use_stream(nullptr);
+        DeviceBuffers bufs;
+        if (idx_device)
+            check_indices_device(ctx, idx, static_cast<size_t>(T) * 3, V, st, PDR_INVALID_ARGUMENT,
+                                 "texture refinement mesh arrays have inconsistent shapes");
+
+        // ---------------------------------------------------------- precompute
+        sync(st);
+        const auto t0 = std::chrono::steady_clock::now();
+        const float* dpos = device_view(bufs, positions, static_cast<size_t>(V) * 3, st);
+        const float* duv = device_view(bufs, uv, static_cast<size_t>(V) * 2, st);
+        const uint32_t* didx = device_view(bufs, idx, static_cast<size_t>(T) * 3, st);
+        float* clip = bufs.alloc<float>(static_cast<size_t>(V) * 4);
+        uint32_t max_pixels = 0;
+        for (uint32_t k = 0; k < nviews; ++k) max_pixels = std::max(max_pixels, views[k].width * views[k].height);
+        float* raster = bufs.alloc<float>(static_cast<size_t>(max_pixels) * 4);
+        double* partials = bufs.alloc<double>(pdr::refine_partial_count(max_pixels) + 1);
+        std::vector<pdr::RefineView> rviews(nviews);
+        std::vector<double> valid_weight(nviews);
+        for (uint32_t k = 0; k < nviews; ++k) {
+            const pdr_refine_view& v = views[k];
+            const uint32_t P = v.width * v.height;
+            const float* img = device_view(bufs, v.image, static_cast<size_t>(P) * v.channels, st);
+            const float* vis = device_view(bufs, v.visibility_mask, P, st);
+            hip_check(pdr::transform_positions(dpos, V, v.world_to_clip, clip, st), "transform_positions");
+            float vp[4];
+            resolve_viewport(v.width, v.height, v.has_viewport ? v.viewport : nullptr, vp);
+            pdr::ViewTable vt{};
+            vt.n = 1;
+            vt.v[0] = make_view(clip, v.width, v.height, vp);
+            vt.v[0].raster = raster;
+            finalize_table(vt);
+            run_raster_forward(ctx, vt, didx, T, V, 0, 0, st);
+            float* vuv = bufs.alloc<float>(static_cast<size_t>(P) * 2);
+            float* target = bufs.alloc<float>(static_cast<size_t>(P) * 3);
+            float* mask = bufs.alloc<float>(P);
+            const pdr::PixelTable pt = pixel_table_one(raster, nullptr, nullptr, vuv, nullptr, v.width, v.height);
+            hip_check(pdr::interp_forward(pt, duv, V, 2, didx, T, st), "interp_forward");
+            const uint32_t np = pdr::refine_partial_count(P);
+            hip_check(pdr::refine_view_inputs(img, v.channels, vis, raster, P, target, mask, partials, np, st),
+                      "refine_view_inputs");
+            hip_check(pdr::sum_partials(partials, np, partials + np, st), "sum_partials");
+            double w = 0.0;
+            hip_check(hipMemcpyAsync(&w, partials + np, sizeof(double), hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
+            sync(st);
+            valid_weight[k] = w;
+            rviews[k] = pdr::RefineView{vuv, target, mask, P};
+        }
+        pdr::RefineView* dviews = bufs.alloc<pdr::RefineView>(nviews);
+        hip_check(hipMemcpyAsync(dviews, rviews.data(), nviews * sizeof(pdr::RefineView), hipMemcpyHostToDevice, st),
+                  "hipMemcpyAsync");
+        sync(st);
+        const auto t1 = std::chrono::steady_clock::now();
+
+        // ---------------------------------------------------------- state + schedule
+        const uint32_t n = static_cast<uint32_t>(nvalues);
+        const uint32_t seam_steps = npairs ? o.seam_polish_steps : 0;
+        float* tex = bufs.alloc<float>(n);
+        float* grad = bufs.alloc<float>(n);
+        float* m1 = bufs.alloc<float>(n);
+        float* m2 = bufs.alloc<float>(n);
+        float* loss = bufs.alloc<float>(o.steps + seam_steps);
+        uint32_t* counters = bufs.alloc<uint32_t>(4);  // step, done
+        copy_any(tex, initial_texture, static_cast<size_t>(n) * 4, st);
+        hip_check(pdr::fill_f32(grad, 0.0f, n, st), "fill");
+        hip_check(pdr::fill_f32(m1, 0.0f, n, st), "fill");
+        hip_check(pdr::fill_f32(m2, 0.0f, n, st), "fill");
+        hip_check(pdr::fill_f32(loss, 0.0f, o.steps + seam_steps, st), "fill");
+        hip_check(pdr::fill_u32(counters, 0u, 4, st), "fill");
+        const float* dpairs = device_view(bufs, seam_pairs, static_cast<size_t>(npairs) * 4, st);
+
+        // src/texture_refiner.cpp:315-356
+        std::vector<float> lr(o.steps), bc1(o.steps), bc2(o.steps), norm(o.steps);
+        std::vector<uint32_t> vidx(static_cast<size_t>(o.steps) * o.batch_size);
+        const float pi = std::acos(-1.0F);
+        for (uint32_t s = 0; s < o.steps; ++s) {
+            double bw = 0.0;
+            for (uint32_t b = 0; b < o.batch_size; ++b) {
+                const size_t vi = (static_cast<size_t>(s) * o.batch_size + b) % nviews;
+                bw += valid_weight[vi];
+                vidx[static_cast<size_t>(s) * o.batch_size + b] = static_cast<uint32_t>(vi);
+            }
+            norm[s] = bw > 0.0 ? static_cast<float>(1.0 / (bw * 3.0)) : 0.0F;
+            const float phase = static_cast<float>(s) / static_cast<float>(o.steps);
+            lr[s] = o.minimum_learning_rate +
+                    0.5F * (o.learning_rate - o.minimum_learning_rate) * (1.0F + std::cos(pi * phase));
+            bc1[s] = 1.0F - std::pow(o.adam_beta1, static_cast<float>(s + 1));
+            bc2[s] = 1.0F - std::pow(o.adam_beta2, static_cast<float>(s + 1));
+        }
+        float* d_lr = bufs.alloc<float>(o.steps);
+        float* d_bc1 = bufs.alloc<float>(o.steps);
+        float* d_bc2 = bufs.alloc<float>(o.steps);
+        float* d_norm = bufs.alloc<float>(o.steps);
+        uint32_t* d_vidx = bufs.alloc<uint32_t>(vidx.size());
+        copy_any(d_lr, lr.data(), lr.size() * 4, st);
+        copy_any(d_bc1, bc1.data(), bc1.size() * 4, st);
+        copy_any(d_bc2, bc2.data(), bc2.size() * 4, st);
+        copy_any(d_norm, norm.data(), norm.size() * 4, st);
+        copy_any(d_vidx, vidx.data(), vidx.size() * 4, st);
+        pdr::RefineStepTables tables{d_lr, d_bc1, d_bc2, d_norm, d_vidx};
+
+        // extended objective (python/photara_drender/atlas.py:84-144)
+        const int32_t* dchart = nullptr;
+        float tv_scale = 0.0f;
+        if (o.tv_weight != 0.0f) {
+            dchart = device_view(bufs, o.chart_ids, static_cast<size_t>(texels), st);
+            unsigned long long* cnt = bufs.alloc<unsigned long long>(1);
+            hip_check(hipMemsetAsync(cnt, 0, sizeof(unsigned long long), st), "hipMemsetAsync");
+            hip_check(pdr::tv_pair_count(dchart, o.width, o.height, cnt, st), "tv_pair_count");
+            unsigned long long pairs = 0;
+            hip_check(hipMemcpyAsync(&pairs, cnt, sizeof(pairs), hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
+            sync(st);
+            tv_scale = o.tv_weight / (static_cast<float>(std::max<unsigned long long>(pairs, 1)) * 3.0f);
+        }
+        const float seam_norm = npairs ? 1.0F / static_cast<float>(npairs * 3) : 0.0f;
+        sync(st);
+
+        // ---------------------------------------------------------- optimisation
+        const auto t2 = std::chrono::steady_clock::now();
+        auto one_step = [&](hipStream_t s) {
+            hip_check(pdr::photometric_step(dviews, o.batch_size, max_pixels, tex, o.width, o.height,
+                                            o.photometric_epsilon, tables, counters, grad, loss, s),
+                      "photometric_step");
+            if (o.joint_seam_weight != 0.0f && npairs)
+                hip_check(pdr::seam_step(tex, o.width, o.height, dpairs, npairs, o.seam_epsilon, seam_norm,
+                                         o.joint_seam_weight, counters, 0, grad, nullptr, s),
+                          "seam_step");
+            if (o.tv_weight != 0.0f)
+                hip_check(pdr::tv_step(tex, o.width, o.height, dchart, tv_eps, tv_scale, grad, s), "tv_step");
+            hip_check(pdr::adam_step(tex, grad, m1, m2, n, d_lr, d_bc1, d_bc2, o.adam_beta1, o.adam_beta2,
+                                     o.adam_epsilon, o.clamp_min, o.clamp_max, counters, counters + 1, s),
+                      "adam_step");
+        };
+        const uint32_t block = o.checkpoint_every ? o.checkpoint_every : std::min<uint32_t>(o.steps, 50);
+        auto capture = [&](uint32_t count, GraphExec& g) {
+            hip_check(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+            for (uint32_t k = 0; k < count; ++k) one_step(st);
+            hip_check(hipStreamEndCapture(st, &g.graph), "hipStreamEndCapture");
+            hip_check(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+        };
+        GraphExec full, tail;
+        const uint32_t nblocks = o.steps / block, rest = o.steps % block;
+        if (o.use_graph) {
+            if (nblocks) capture(block, full);
+            if (rest) capture(rest, tail);
+        }
+        uint32_t done = 0, ckpt = 0;
+        const size_t atlas_bytes = static_cast<size_t>(n) * 4;
+        for (uint32_t b = 0; b <= nblocks; ++b) {
+            const uint32_t count = b < nblocks ? block : rest;
+            if (!count) continue;
+            if (o.use_graph)
+                hip_check(hipGraphLaunch(b < nblocks ? full.exec : tail.exec, st), "hipGraphLaunch");
+            else
+                for (uint32_t k = 0; k < count; ++k) one_step(st);
+            done += count;
+            if (o.checkpoint_every && o.checkpoints && done % o.checkpoint_every == 0)
+                copy_any(reinterpret_cast<char*>(o.checkpoints) + static_cast<size_t>(ckpt++) * atlas_bytes, tex,
+                         atlas_bytes, st);
+        }
+        if (seam_steps) {
+            // texture_refiner.cpp:371-415: reset Adam, constant seam learning rate
+            std::vector<float> slr(seam_steps, o.seam_learning_rate), sb1(seam_steps), sb2(seam_steps);
+            for (uint32_t k = 0; k < seam_steps; ++k) {
+                sb1[k] = 1.0F - std::pow(o.adam_beta1, static_cast<float>(k + 1));
+                sb2[k] = 1.0F - std::pow(o.adam_beta2, static_cast<float>(k + 1));
+            }
+            float* d_slr = bufs.alloc<float>(seam_steps);
+            float* d_sb1 = bufs.alloc<float>(seam_steps);
+            float* d_sb2 = bufs.alloc<float>(seam_steps);
+            copy_any(d_slr, slr.data(), seam_steps * 4, st);
+            copy_any(d_sb1, sb1.data(), seam_steps * 4, st);
+            copy_any(d_sb2, sb2.data(), seam_steps * 4, st);
+            hip_check(pdr::fill_f32(grad, 0.0f, n, st), "fill");
+            hip_check(pdr::fill_f32(m1, 0.0f, n, st), "fill");
+            hip_check(pdr::fill_f32(m2, 0.0f, n, st), "fill");
+            hip_check(pdr::fill_u32(counters, 0u, 2, st), "fill");
+            for (uint32_t k = 0; k < seam_steps; ++k) {
+                hip_check(pdr::seam_step(tex, o.width, o.height, dpairs, npairs, o.seam_epsilon, seam_norm, 1.0f,
+                                         counters, o.steps, grad, loss, st),
+                          "seam_step");
+                hip_check(pdr::adam_step(tex, grad, m1, m2, n, d_slr, d_sb1, d_sb2, o.adam_beta1, o.adam_beta2,
+                                         o.adam_epsilon, o.clamp_min, o.clamp_max, counters, counters + 1, st),
+                          "adam_step");
+            }
+        }
+        sync(st);
+        const auto t3 = std::chrono::steady_clock::now();
+        copy_any(color_out, tex, atlas_bytes, st);
+        copy_any(loss_out, loss, static_cast<size_t>(o.steps) * 4, st);
+        if (seam_steps) copy_any(seam_loss_out, loss + o.steps, static_cast<size_t>(seam_steps) * 4, st);
+        sync(st);
+        if (times_out2) {
+            times_out2[0] = std::chrono::duration<double>(t1 - t0).count();
+            times_out2[1] = std::chrono::duration<double>(t3 - t2).count();
+        }
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,330 @@
+// Atlas refinement kernels (src/texture_refiner.cpp:174-454).
+//
+//  k_transform        clip = M [p,1] per vertex (texture_refiner.cpp:36-50)
+//  k_view_inputs      target RGB, mask and per-block mask sums of one view
+//                     (texture_refiner.cpp:228-242)
+//  k_photometric      atlas_photometric_gradient.hlsl:45-108 for all views of a
+//                     step in one launch; per-step scalars come from device
+//                     tables indexed by a device step counter so the step loop
+//                     can be captured once and replayed
+//  k_seam             atlas_seam_gradient.hlsl:50-100
+//  k_tv               gradient of the chart-aware total variation of
+//                     python/photara_drender/atlas.py:84-115 (extension)
+//  k_adam             atlas_adam_update.hlsl:20-43, float4-vectorised; the
+//                     last workgroup to finish advances the step counter
+//
+// Gradient scatters use hardware float atomics (the reference emulates them
+// with compare-exchange loops); per-64-pixel loss partials use the shaders'
+// tree order exactly.
+#include "pdr_internal.hpp"
+
+namespace pdr {
+namespace {
+
+constexpr uint32_t kBlock = 256;
+
+struct M16 {
+    float m[16];
+};
+
+__global__ void k_transform(const float* __restrict__ pos3, uint32_t V, M16 M, float4* __restrict__ out) {
+    const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
+    if (v >= V) return;
+    const float x = pos3[v * 3 + 0], y = pos3[v * 3 + 1], z = pos3[v * 3 + 2];
+    float r[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) r[k] = M.m[k * 4] * x + M.m[k * 4 + 1] * y + M.m[k * 4 + 2] * z + M.m[k * 4 + 3];
+    out[v] = make_float4(r[0], r[1], r[2], r[3]);
+}
+
+// Block partials of the mask sum in double (fixed order: lanes, then waves).
+__global__ __launch_bounds__(kBlock) void k_view_inputs(const float* __restrict__ image, uint32_t channels,
+                                                        const float* __restrict__ vis, const float* __restrict__ raster,
+                                                        uint32_t P, float* __restrict__ target,
+                                                        float* __restrict__ mask, double* __restrict__ partials) {
+    __shared__ double wsum[kBlock];
+    const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+    double w = 0.0;
+    if (p < P) {
+        target[static_cast<size_t>(p) * 3 + 0] = image[static_cast<size_t>(p) * channels + 0];
+        target[static_cast<size_t>(p) * 3 + 1] = image[static_cast<size_t>(p) * channels + 1];
+        target[static_cast<size_t>(p) * 3 + 2] = image[static_cast<size_t>(p) * channels + 2];
+        float m = 0.0f;
+        if (raster[static_cast<size_t>(p) * 4 + 3] != 0.0f) m = vis ? clampf(vis[p], 0.0f, 1.0f) : 1.0f;
+        mask[p] = m;
+        w = m;
+    }
+    wsum[threadIdx.x] = w;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (uint32_t i = 0; i < kBlock; ++i) s += wsum[i];
+        partials[blockIdx.x] = s;
+    }
+}
+
+__global__ void k_sum_partials(const double* __restrict__ partials, uint32_t n, double* __restrict__ out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double s = 0.0;
+    for (uint32_t i = 0; i < n; ++i) s += partials[i];
+    *out = s;
+}
+
+struct AtlasTap {
+    uint32_t t[4];
+    float w[4];
+};
+
+// atlas_photometric_gradient.hlsl:98-117 / atlas_seam_gradient.hlsl:182-199 (clamp addressing)
+__device__ __forceinline__ AtlasTap atlas_tap(float u, float v, uint32_t tw, uint32_t th) {
+    const float tx = u * static_cast<float>(tw) - 0.5f, ty = v * static_cast<float>(th) - 0.5f;
+    const float flx = floorf(tx), fly = floorf(ty);
+    const int32_t bx = f2i_sat(flx), by = f2i_sat(fly);
+    const float fx = tx - flx, fy = ty - fly;
+    const int32_t wm = static_cast<int32_t>(tw) - 1, hm = static_cast<int32_t>(th) - 1;
+    auto cl = [](int32_t c, int32_t hi) { return static_cast<uint32_t>(c < 0 ? 0 : (c > hi ? hi : c)); };
+    const uint32_t x0 = cl(bx, wm), x1 = cl(add1(bx), wm), y0 = cl(by, hm), y1 = cl(add1(by), hm);
+    AtlasTap a;
+    a.t[0] = y0 * tw + x0;
+    a.t[1] = y0 * tw + x1;
+    a.t[2] = y1 * tw + x0;
+    a.t[3] = y1 * tw + x1;
+    a.w[0] = (1.0f - fx) * (1.0f - fy);
+    a.w[1] = fx * (1.0f - fy);
+    a.w[2] = (1.0f - fx) * fy;
+    a.w[3] = fx * fy;
+    return a;
+}
+
+__global__ __launch_bounds__(kBlock) void k_photometric(const RefineView* __restrict__ views, uint32_t batch,
+                                                        const float* __restrict__ tex, uint32_t tw, uint32_t th,
+                                                        float eps, RefineStepTables tables,
+                                                        const uint32_t* __restrict__ step_counter,
+                                                        float* __restrict__ grad, float* __restrict__ loss) {
+    const uint32_t step = *step_counter;
+    const uint32_t b = blockIdx.y;
+    const RefineView& rv = views[tables.views[static_cast<size_t>(step) * batch + b]];
+    const float norm = tables.norm[step];
+    const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+    if (blockIdx.x * kBlock >= rv.pixels) return;  // whole block past this view (uniform)
+    float local = 0.0f;
+    if (p < rv.pixels) {
+        const float m = rv.mask[p];
+        if (m > 0.0f) {
+            const float2 uv = reinterpret_cast<const float2*>(rv.uv)[p];
+            const AtlasTap a = atlas_tap(uv.x, uv.y, tw, th);
+#pragma unroll
+            for (int ch = 0; ch < 3; ++ch) {
+                float rendered = 0.0f;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) rendered += tex[static_cast<size_t>(a.t[k]) * 3 + ch] * a.w[k];
+                const float d = rendered - rv.target[static_cast<size_t>(p) * 3 + ch];
+                const float r = sqrtf(d * d + eps * eps);
+                const float og = d / r * m * norm;
+                local += r * m * norm;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) unsafeAtomicAdd(&grad[static_cast<size_t>(a.t[k]) * 3 + ch], og * a.w[k]);
+            }
+        }
+    }
+    const float s = wave_tree_sum(local);
+    if ((threadIdx.x & 63u) == 0 && s != 0.0f) unsafeAtomicAdd(&loss[step], s);
+}
+
+__global__ __launch_bounds__(kBlock) void k_seam(const float* __restrict__ tex, uint32_t tw, uint32_t th,
+                                                 const float* __restrict__ pairs, uint32_t npairs, float eps,
+                                                 float norm, float weight, const uint32_t* __restrict__ step_counter,
+                                                 uint32_t loss_base, float* __restrict__ grad,
+                                                 float* __restrict__ loss) {
+    const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
+    float local = 0.0f;
+    if (q < npairs) {
+        const float4 pr = reinterpret_cast<const float4*>(pairs)[q];
+        const AtlasTap a0 = atlas_tap(pr.x, pr.y, tw, th);
+        const AtlasTap a1 = atlas_tap(pr.z, pr.w, tw, th);
+#pragma unroll
+        for (int ch = 0; ch < 3; ++ch) {
+            float c0 = 0.0f, c1 = 0.0f;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                c0 += tex[static_cast<size_t>(a0.t[k]) * 3 + ch] * a0.w[k];
+                c1 += tex[static_cast<size_t>(a1.t[k]) * 3 + ch] * a1.w[k];
+            }
+            const float d = c0 - c1;
+            const float r = sqrtf(d * d + eps * eps);
+            float g = d / r * norm;
+            local += r * norm;
+            if (weight != 1.0f) g = g * weight;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                unsafeAtomicAdd(&grad[static_cast<size_t>(a0.t[k]) * 3 + ch], g * a0.w[k]);
+                unsafeAtomicAdd(&grad[static_cast<size_t>(a1.t[k]) * 3 + ch], -g * a1.w[k]);
+            }
+        }
+    }
+    const float s = wave_tree_sum(local);
+    if (loss && (threadIdx.x & 63u) == 0 && s != 0.0f) unsafeAtomicAdd(&loss[loss_base + *step_counter], s);
+}
+
+// d/dt of weight * sum_pairs sqrt((t_b - t_a)^2 + eps^2) / (pairs * 3) over
+// horizontal and vertical neighbour pairs inside one chart.
+__global__ __launch_bounds__(kBlock) void k_tv(const float* __restrict__ tex, uint32_t tw, uint32_t th,
+                                               const int32_t* __restrict__ chart, float eps, float scale,
+                                               float* __restrict__ grad) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= tw * th) return;
+    const uint32_t x = i % tw, y = i / tw;
+    const int32_t c = chart ? chart[i] : 0;
+    if (c < 0) return;
+    const float* t = tex + static_cast<size_t>(i) * 3;
+    float g[3] = {0.0f, 0.0f, 0.0f};
+    auto pair = [&](uint32_t j, float sign) {
+        const int32_t cj = chart ? chart[j] : 0;
+        if (cj != c) return;
+        const float* u = tex + static_cast<size_t>(j) * 3;
+#pragma unroll
+        for (int ch = 0; ch < 3; ++ch) {
+            const float d = sign > 0.0f ? t[ch] - u[ch] : u[ch] - t[ch];  // forward difference of the pair
+            g[ch] += sign * (d / sqrtf(d * d + eps * eps));
+        }
+    };
+    if (x > 0) pair(i - 1, 1.0f);        // pair (x-1, x): d = t - left, dL/dt = +d/r
+    if (x + 1 < tw) pair(i + 1, -1.0f);  // pair (x, x+1): d = right - t, dL/dt = -d/r
+    if (y > 0) pair(i - tw, 1.0f);
+    if (y + 1 < th) pair(i + tw, -1.0f);
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) grad[static_cast<size_t>(i) * 3 + ch] += g[ch] * scale;
+}
+
+__global__ void k_tv_pairs(const int32_t* __restrict__ chart, uint32_t tw, uint32_t th,
+                           unsigned long long* __restrict__ out) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    unsigned long long n = 0;
+    if (i < tw * th) {
+        const uint32_t x = i % tw, y = i / tw;
+        const int32_t c = chart ? chart[i] : 0;
+        if (c >= 0) {
+            if (x + 1 < tw && (chart ? chart[i + 1] : 0) == c) ++n;
+            if (y + 1 < th && (chart ? chart[i + tw] : 0) == c) ++n;
+        }
+    }
+    if (n) atomicAdd(out, n);
+}
+
+__device__ __forceinline__ float adam_one(float p, float& g, float& m, float& v, float lr, float b1, float b2,
+                                          float eps, float bc1, float bc2, float cmin, float cmax) {
+    const float m1 = b1 * m + (1.0f - b1) * g;
+    const float m2 = b2 * v + (1.0f - b2) * g * g;
+    m = m1;
+    v = m2;
+    g = 0.0f;
+    const float c1 = m1 / bc1, c2 = m2 / bc2;
+    return clampf(p - lr * c1 / (sqrtf(c2) + eps), cmin, cmax);
+}
+
+__global__ __launch_bounds__(kBlock) void k_adam(float* __restrict__ tex, float* __restrict__ grad,
+                                                 float* __restrict__ mom1, float* __restrict__ mom2, uint32_t n,
+                                                 const float* __restrict__ lr_t, const float* __restrict__ bc1_t,
+                                                 const float* __restrict__ bc2_t, float b1, float b2, float eps,
+                                                 float cmin, float cmax, uint32_t* step_counter,
+                                                 uint32_t* done_counter) {
+    const uint32_t step = *step_counter;
+    const float lr = lr_t[step], bc1 = bc1_t[step], bc2 = bc2_t[step];
+    const uint32_t n4 = n / 4;
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n4; i += gridDim.x * kBlock) {
+        float4 p = reinterpret_cast<float4*>(tex)[i];
+        float4 g = reinterpret_cast<float4*>(grad)[i];
+        float4 m = reinterpret_cast<float4*>(mom1)[i];
+        float4 v = reinterpret_cast<float4*>(mom2)[i];
+        p.x = adam_one(p.x, g.x, m.x, v.x, lr, b1, b2, eps, bc1, bc2, cmin, cmax);
+        p.y = adam_one(p.y, g.y, m.y, v.y, lr, b1, b2, eps, bc1, bc2, cmin, cmax);
+        p.z = adam_one(p.z, g.z, m.z, v.z, lr, b1, b2, eps, bc1, bc2, cmin, cmax);
+        p.w = adam_one(p.w, g.w, m.w, v.w, lr, b1, b2, eps, bc1, bc2, cmin, cmax);
+        reinterpret_cast<float4*>(tex)[i] = p;
+        reinterpret_cast<float4*>(grad)[i] = g;
+        reinterpret_cast<float4*>(mom1)[i] = m;
+        reinterpret_cast<float4*>(mom2)[i] = v;
+    }
+    if (blockIdx.x == 0) {
+        for (uint32_t i = n4 * 4 + threadIdx.x; i < n; i += kBlock)
+            tex[i] = adam_one(tex[i], grad[i], mom1[i], mom2[i], lr, b1, b2, eps, bc1, bc2, cmin, cmax);
+    }
+    // The last workgroup to finish advances the step (every workgroup has read
+    // `step` before it arrives here).
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const uint32_t done = atomicAdd(done_counter, 1u);
+        if (done == gridDim.x - 1) {
+            *done_counter = 0u;
+            *step_counter = step + 1;
+            __threadfence();
+        }
+    }
+}
+
+}  // namespace
+
+hipError_t transform_positions(const float* pos3, uint32_t V, const float* m16, float* out4, hipStream_t st) {
+    M16 M;
+    for (int k = 0; k < 16; ++k) M.m[k] = m16[k];
+    k_transform<<<(V + kBlock - 1) / kBlock, kBlock, 0, st>>>(pos3, V, M, reinterpret_cast<float4*>(out4));
+    return hipGetLastError();
+}
+
+uint32_t refine_partial_count(uint32_t P) { return (P + kBlock - 1) / kBlock; }
+
+hipError_t refine_view_inputs(const float* image, uint32_t channels, const float* vis, const float* raster, uint32_t P,
+                              float* target3, float* mask, double* partials, uint32_t npartials, hipStream_t st) {
+    (void)npartials;
+    k_view_inputs<<<refine_partial_count(P), kBlock, 0, st>>>(image, channels, vis, raster, P, target3, mask,
+                                                             partials);
+    return hipGetLastError();
+}
+
+hipError_t sum_partials(const double* partials, uint32_t n, double* out, hipStream_t st) {
+    k_sum_partials<<<1, 64, 0, st>>>(partials, n, out);
+    return hipGetLastError();
+}
+
+hipError_t photometric_step(const RefineView* views_dev, uint32_t batch, uint32_t max_pixels, const float* tex,
+                            uint32_t tw, uint32_t th, float eps, const RefineStepTables& tables,
+                            const uint32_t* step_counter, float* grad, float* loss, hipStream_t st) {
+    const dim3 grid((max_pixels + kBlock - 1) / kBlock, batch);
+    k_photometric<<<grid, kBlock, 0, st>>>(views_dev, batch, tex, tw, th, eps, tables, step_counter, grad, loss);
+    return hipGetLastError();
+}
+
+hipError_t seam_step(const float* tex, uint32_t tw, uint32_t th, const float* pairs, uint32_t npairs, float eps,
+                     float norm, float weight, const uint32_t* step_counter, uint32_t loss_base, float* grad,
+                     float* loss, hipStream_t st) {
+    if (!npairs) return hipSuccess;
+    k_seam<<<(npairs + kBlock - 1) / kBlock, kBlock, 0, st>>>(tex, tw, th, pairs, npairs, eps, norm, weight,
+                                                             step_counter, loss_base, grad, loss);
+    return hipGetLastError();
+}
+
+hipError_t tv_step(const float* tex, uint32_t tw, uint32_t th, const int32_t* chart, float eps,
+                   float weight_over_pairs, float* grad, hipStream_t st) {
+    k_tv<<<(tw * th + kBlock - 1) / kBlock, kBlock, 0, st>>>(tex, tw, th, chart, eps, weight_over_pairs, grad);
+    return hipGetLastError();
+}
+
+hipError_t tv_pair_count(const int32_t* chart, uint32_t tw, uint32_t th, unsigned long long* out, hipStream_t st) {
+    k_tv_pairs<<<(tw * th + kBlock - 1) / kBlock, kBlock, 0, st>>>(chart, tw, th, out);
+    return hipGetLastError();
+}
+
+hipError_t adam_step(float* tex, float* grad, float* m, float* v, uint32_t n, const float* lr_table,
+                     const float* bc1_table, const float* bc2_table, float b1, float b2, float eps, float cmin,
+                     float cmax, uint32_t* step_counter, uint32_t* done_counter, hipStream_t st) {
+    const uint32_t n4 = n / 4;
+    uint32_t grid = (n4 + kBlock - 1) / kBlock;
+    if (grid > 2048) grid = 2048;
+    if (grid == 0) grid = 1;
+    k_adam<<<grid, kBlock, 0, st>>>(tex, grad, m, v, n, lr_table, bc1_table, bc2_table, b1, b2, eps, cmin, cmax,
+                                    step_counter, done_counter);
+    return hipGetLastError();
+}
+
+}  // namespace pdr

@@ -1,0 +1,197 @@
+"""Seeded synthetic scenes for the benchmark workloads and tests.
+
+Every generator is deterministic in its seed (``numpy.random.default_rng``).
+Cameras follow the reference convention: a row-major ``world_to_clip`` with
+``clip = M @ [x, y, z, 1]`` and GL-style depth in [-1, 1]
+(``python/photara_drender/colmap.py:131-153``).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+
+# ---------------------------------------------------------------- meshes
+
+
+def icosphere(level: int, radius: float = 1.0, jitter: float = 0.0, seed: int = 0):
+    """Icosahedron subdivided ``level`` times: 20*4**level triangles.
+
+    Each vertex is scaled by ``radius * (1 + U(-jitter, jitter))``.
+    Returns (positions[V,3] f32, indices[T,3] u32).
+    """
+    t = (1.0 + math.sqrt(5.0)) / 2.0
+    verts = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+             (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    verts = [np.array(v, np.float64) / np.linalg.norm(v) for v in verts]
+    faces = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2),
+             (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11),
+             (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    for _ in range(level):
+        cache: dict = {}
+        nf = []
+
+        def mid(a, b):
+            key = (min(a, b), max(a, b))
+            if key not in cache:
+                m = verts[a] + verts[b]
+                verts.append(m / np.linalg.norm(m))
+                cache[key] = len(verts) - 1
+            return cache[key]
+
+        for a, b, c in faces:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        faces = nf
+    pos = np.array(verts, np.float64)
+    rng = np.random.default_rng(seed)
+    scale = radius * (1.0 + rng.uniform(-jitter, jitter, size=(pos.shape[0], 1)))
+    return (pos * scale).astype(np.float32), np.array(faces, np.uint32)
+
+
+def spherical_uv(positions: np.ndarray) -> np.ndarray:
+    """u = 0.5 + atan2(z, x)/(2 pi), v = 0.5 + asin(y/|p|)/pi."""
+    p = positions.astype(np.float64)
+    r = np.linalg.norm(p, axis=1)
+    u = 0.5 + np.arctan2(p[:, 2], p[:, 0]) / (2 * math.pi)
+    v = 0.5 + np.arcsin(np.clip(p[:, 1] / r, -1, 1)) / math.pi
+    return np.stack([u, v], 1).astype(np.float32)
+
+
+def uv_sphere(segments: int = 32, rings: int = 33, radius: float = 1.0):
+    """UV sphere with 2*segments*(rings-1) triangles (poles duplicated per segment)."""
+    pos, uv = [], []
+    for r in range(rings):
+        theta = math.pi * r / (rings - 1)
+        for s in range(segments + 1):
+            phi = 2 * math.pi * s / segments
+            pos.append((radius * math.sin(theta) * math.cos(phi), radius * math.cos(theta),
+                        radius * math.sin(theta) * math.sin(phi)))
+            uv.append((s / segments, 1.0 - r / (rings - 1)))
+    idx = []
+    row = segments + 1
+    for r in range(rings - 1):
+        for s in range(segments):
+            a, b = r * row + s, r * row + s + 1
+            c, d = (r + 1) * row + s, (r + 1) * row + s + 1
+            idx += [(a, c, b), (b, c, d)]
+    return np.array(pos, np.float32), np.array(uv, np.float32), np.array(idx, np.uint32)
+
+
+def heightfield(nx: int = 400, ny: int = 250, extent=(1.0, 0.625), sigma: float = 0.05, seed: int = 2):
+    """Grid of nx*ny quads over [-ex,ex]x[-ey,ey] (2*nx*ny triangles), z ~ N(0, sigma)."""
+    rng = np.random.default_rng(seed)
+    xs = np.linspace(-extent[0], extent[0], nx + 1)
+    ys = np.linspace(-extent[1], extent[1], ny + 1)
+    gx, gy = np.meshgrid(xs, ys)
+    gz = rng.normal(0.0, sigma, size=gx.shape)
+    pos = np.stack([gx, gz, gy], -1).reshape(-1, 3).astype(np.float32)  # y up
+    i = np.arange(ny)[:, None] * (nx + 1) + np.arange(nx)[None, :]
+    a, b, c, d = i, i + 1, i + nx + 1, i + nx + 2
+    idx = np.stack([np.stack([a, c, b], -1), np.stack([b, c, d], -1)], 2).reshape(-1, 3).astype(np.uint32)
+    return pos, idx
+
+
+# ---------------------------------------------------------------- cameras
+
+
+def look_at(eye, target=(0.0, 0.0, 0.0), up=(0.0, 1.0, 0.0)) -> np.ndarray:
+    eye, target, up = (np.asarray(v, np.float64) for v in (eye, target, up))
+    f = target - eye
+    f /= np.linalg.norm(f)
+    s = np.cross(f, up)
+    if np.linalg.norm(s) < 1e-9:
+        s = np.cross(f, (1.0, 0.0, 0.0))
+    s /= np.linalg.norm(s)
+    u = np.cross(s, f)
+    m = np.eye(4)
+    m[0, :3], m[1, :3], m[2, :3] = s, u, -f
+    m[:3, 3] = -m[:3, :3] @ eye
+    return m
+
+
+def perspective(fov_y_deg: float, aspect: float, near: float = 0.1, far: float = 10.0) -> np.ndarray:
+    f = 1.0 / math.tan(math.radians(fov_y_deg) / 2)
+    m = np.zeros((4, 4))
+    m[0, 0], m[1, 1] = f / aspect, f
+    m[2, 2], m[2, 3] = (far + near) / (near - far), 2 * far * near / (near - far)
+    m[3, 2] = -1.0
+    return m
+
+
+def world_to_clip(eye, aspect, fov=45.0, target=(0.0, 0.0, 0.0), near=0.1, far=10.0) -> np.ndarray:
+    """Row-major float32 [4,4]."""
+    return (perspective(fov, aspect, near, far) @ look_at(eye, target)).astype(np.float32)
+
+
+def random_direction(rng) -> np.ndarray:
+    v = rng.normal(size=3)
+    return v / np.linalg.norm(v)
+
+
+def to_clip(positions: np.ndarray, m: np.ndarray) -> np.ndarray:
+    """float32 clip positions [V,4] = M @ [p,1] (host-side helper for tests and scenes)."""
+    p = np.concatenate([positions.astype(np.float64), np.ones((positions.shape[0], 1))], 1)
+    return (p @ m.astype(np.float64).T).astype(np.float32)
+
+
+# ---------------------------------------------------------------- workloads
+
+
+@dataclass
+class ChainScene:
+    """One view of a textured mesh: rasterize -> interpolate(uv) -> texture."""
+
+    clip: np.ndarray  # [V,4] f32
+    indices: np.ndarray  # [T,3] u32
+    uv: np.ndarray  # [V,2] f32
+    texture: np.ndarray  # [th,tw,C] f32
+    target: np.ndarray  # [H,W,C] f32
+    resolution: tuple  # (H, W)
+
+
+def chain_scene(level=5, res=(512, 512), tex_size=1024, channels=4, seed=1) -> ChainScene:
+    """Plain fwd+bwd chain: jittered icosphere, spherical UV, U[0,1] texture and target,
+    look-at camera from a random direction at distance 3, fov 45 deg."""
+    rng = np.random.default_rng(seed)
+    pos, idx = icosphere(level, 1.0, 0.01, seed)
+    uv = spherical_uv(pos)
+    eye = 3.0 * random_direction(rng)
+    m = world_to_clip(eye, res[1] / res[0], 45.0)
+    tex = rng.uniform(0, 1, size=(tex_size, tex_size, channels)).astype(np.float32)
+    target = rng.uniform(0, 1, size=(res[0], res[1], channels)).astype(np.float32)
+    return ChainScene(to_clip(pos, m), idx, uv, tex, target, res)
+
+
+MULTIVIEW_SIZES = [(512, 512), (480, 640), (600, 800), (768, 1024), (720, 1280), (1024, 1280), (900, 1600),
+                   (1080, 1920)]  # (H, W), used twice
+
+
+@dataclass
+class MultiViewScene:
+    clips: list  # per view [V,4] f32
+    indices: np.ndarray
+    attributes: np.ndarray  # [V,A]
+    targets: list  # per view [H,W,A]
+    resolutions: list  # (H,W)
+
+
+def multiview_scene(nx=400, ny=250, views=16, attributes=8, seed=2, sizes=None) -> MultiViewScene:
+    """Batched multi-view raster: heightfield (2*nx*ny tris), N(0,1) attributes and
+    targets, ``views`` cameras looking at the grid from 30-80 deg elevation."""
+    rng = np.random.default_rng(seed)
+    pos, idx = heightfield(nx, ny, seed=seed)
+    attrs = rng.normal(size=(pos.shape[0], attributes)).astype(np.float32)
+    sizes = sizes or (MULTIVIEW_SIZES * ((views + len(MULTIVIEW_SIZES) - 1) // len(MULTIVIEW_SIZES)))[:views]
+    clips, targets = [], []
+    for k, (h, w) in enumerate(sizes):
+        elev = math.radians(rng.uniform(30, 80))
+        azim = rng.uniform(0, 2 * math.pi)
+        dist = rng.uniform(2.2, 2.8)
+        eye = dist * np.array([math.cos(elev) * math.cos(azim), math.sin(elev), math.cos(elev) * math.sin(azim)])
+        m = world_to_clip(eye, w / h, 45.0)
+        clips.append(to_clip(pos, m))
+        targets.append(rng.normal(size=(h, w, attributes)).astype(np.float32))
+    return MultiViewScene(clips, idx, attrs, targets, list(sizes))
