@@ -5,6 +5,11 @@ reference's API (python/photara_drender/__init__.py:10-60 for the exported names
 Importing the package loads the native module ``_photara_drender`` (pybind11 over
 libpdr.so); it fails loudly if the native build is missing.
 """
+try:  # torch first: libpdr then binds to torch's HIP runtime (one runtime per process)
+    import torch as _torch  # noqa: F401
+except ImportError:
+    _torch = None
+
 from ._photara_drender import DeviceInfo, Rasterizer, TextureBaker, enumerate_devices
 
 try:

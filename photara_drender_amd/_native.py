@@ -91,6 +91,10 @@ def lib():
     """The loaded libpdr (raises OSError if it has not been built)."""
     global _lib
     if _lib is None:
+        try:  # bind libpdr to torch's HIP runtime when torch is present (one runtime per process)
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         if not os.path.exists(LIB_PATH):
             raise OSError(f"libpdr.so not found at {LIB_PATH}; run __graft_entry__.build() first")
         handle = ctypes.CDLL(LIB_PATH)

@@ -1,0 +1,398 @@
+"""Pins of the CPU oracle (no GPU): closed forms, invariants, adjoints, central
+finite differences in FP64, independent implementations (torch autograd,
+torch.optim.Adam) and the reference's own test assertions.
+
+Each check is chosen so that a plausible mistake in the oracle (dropped term,
+wrong sign or index, transposed operand) fails it.
+"""
+import math
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+import oracle
+from photara_drender_amd import scenes
+
+TRIANGLE = np.array([[-0.75, -0.75, 0.0, 1.0], [0.75, -0.75, 0.0, 1.0], [0.0, 0.75, 0.0, 1.0]], np.float32)
+ONE = np.array([[0, 1, 2]], np.uint32)
+
+
+def exact_edges(p, px, py, W, H):
+    """Exact rational edge functions cross(s_j - x, s_k - x) of pixel (px, py)."""
+    sx = Fraction(2 * px + 1, W) - 1
+    sy = Fraction(2 * py + 1, H) - 1
+    s = [(Fraction(float(v[0])) / Fraction(float(v[3])), Fraction(float(v[1])) / Fraction(float(v[3]))) for v in p]
+    e = []
+    for j, k in ((1, 2), (2, 0), (0, 1)):
+        ax, ay = s[j][0] - sx, s[j][1] - sy
+        bx, by = s[k][0] - sx, s[k][1] - sy
+        e.append(ax * by - ay * bx)
+    area = (s[1][0] - s[0][0]) * (s[2][1] - s[0][1]) - (s[1][1] - s[0][1]) * (s[2][0] - s[0][0])
+    return e, area
+
+
+@pytest.mark.parametrize("res", [(32, 32), (17, 23), (64, 48)])
+def test_coverage_matches_exact_edge_test(res):
+    """Away from the +-1e-7 band the float coverage equals the exact rational test."""
+    H, W = res
+    raster, _, _ = oracle.rasterize_forward(TRIANGLE, ONE, res)
+    ambiguous = 0
+    for py in range(H):
+        for px in range(W):
+            e, area = exact_edges(TRIANGLE, px, py, W, H)
+            sign = 1 if area > 0 else -1
+            vals = [float(x) * sign for x in e]
+            covered = raster[py, px, 3] == 1.0
+            if min(vals) > 1e-6:
+                assert covered, (px, py)
+            elif min(vals) < -1e-6:
+                assert not covered, (px, py)
+            else:
+                ambiguous += 1
+    assert ambiguous < H + W
+
+
+def test_native_reference_assertions():
+    """tests/native_tests.cpp:135-157 and :235-246 against the oracle."""
+    raster, deriv, vp = oracle.rasterize_forward(TRIANGLE, ONE, (32, 32))
+    assert raster.shape == (32, 32, 4) and deriv.shape == raster.shape
+    assert raster[..., 3].sum() > 100.0
+    grad = np.zeros_like(raster)
+    grad[..., 0] = (raster[..., 3] > 0).astype(np.float32)
+    g = oracle.rasterize_backward(TRIANGLE, ONE, raster, grad)
+    assert np.isfinite(g).all()
+    eps = 1e-3
+    pp, pn = TRIANGLE.copy(), TRIANGLE.copy()
+    pp[0, 0] += eps
+    pn[0, 0] -= eps
+    fd = (oracle.rasterize_forward(pp, ONE, (32, 32))[0][..., 0].sum() -
+          oracle.rasterize_forward(pn, ONE, (32, 32))[0][..., 0].sum()) / (2 * eps)
+    assert abs(g[0, 0] - fd) < 0.2
+    vr, _, _ = oracle.rasterize_forward(TRIANGLE, ONE, (32, 32), viewport=(8.0, 4.0, 16.0, 20.0))
+    mask = np.ones((32, 32), bool)
+    mask[4:24, 8:24] = False
+    assert (vr[mask, 3] == 0).all()
+
+
+def test_affine_barycentrics_and_derivatives_closed_form():
+    """w = 1: barycentrics are affine in the pixel centre, derivatives constant."""
+    H, W = 40, 50
+    raster, deriv, _ = oracle.rasterize_forward(TRIANGLE, ONE, (H, W))
+    p = TRIANGLE.astype(np.float64)
+    m = np.array([[p[0, 0] - p[2, 0], p[1, 0] - p[2, 0]], [p[0, 1] - p[2, 1], p[1, 1] - p[2, 1]]])
+    minv = np.linalg.inv(m)
+    ys, xs = np.nonzero(raster[..., 3])
+    for py, px in zip(ys, xs):
+        s = np.array([(2 * px + 1) / W - 1, (2 * py + 1) / H - 1])
+        b = np.clip(minv @ (s - p[2, :2]), 0, 1)
+        np.testing.assert_allclose(raster[py, px, :2], b, atol=2e-6)
+    # d(b0,b1)/d(pixel) = minv * (2/W, 2/H)
+    expect = np.array([minv[0, 0] * 2 / W, minv[0, 1] * 2 / H, minv[1, 0] * 2 / W, minv[1, 1] * 2 / H])
+    inside = raster[..., 3] > 0
+    np.testing.assert_allclose(deriv[inside], np.broadcast_to(expect, deriv[inside].shape), rtol=1e-5, atol=1e-6)
+    assert (raster[~inside] == 0).all() and (deriv[~inside] == 0).all()
+
+
+def test_depth_rule_and_tie_break():
+    """Nearest depth wins; equal depth -> lower triangle index (rasterize_forward.hlsl:136-141)."""
+    near = np.array([[-0.9, -0.9, 0.2, 1.0], [0.9, -0.9, 0.2, 1.0], [0.0, 0.9, 0.2, 1.0]], np.float32)
+    far = near.copy()
+    far[:, 2] = 0.6
+    pos = np.concatenate([far, near])
+    idx = np.array([[0, 1, 2], [3, 4, 5]], np.uint32)
+    r, _, _ = oracle.rasterize_forward(pos, idx, (24, 24))
+    covered = r[..., 3] > 0
+    assert (r[covered, 3] == 2.0).all() and np.allclose(r[covered, 2], 0.2)
+    # swap order: the near triangle keeps winning (order independence)
+    r2, _, _ = oracle.rasterize_forward(pos, idx[::-1].copy(), (24, 24))
+    assert (r2[covered, 3] == 1.0).all()
+    # equal depth: lower index wins whatever the order
+    same = np.concatenate([near, near[[1, 0, 2]]])
+    r3, _, _ = oracle.rasterize_forward(same, idx, (24, 24))
+    assert (r3[r3[..., 3] > 0, 3] == 1.0).all()
+
+
+def test_cull_modes():
+    ccw = TRIANGLE
+    cw = TRIANGLE[[1, 0, 2]].copy()
+    for pos, back_kept in ((ccw, True), (cw, False)):
+        rb, _, _ = oracle.rasterize_forward(pos, ONE, (16, 16), cull_mode=1)
+        rf, _, _ = oracle.rasterize_forward(pos, ONE, (16, 16), cull_mode=2)
+        assert (rb[..., 3].sum() > 0) == back_kept
+        assert (rf[..., 3].sum() > 0) == (not back_kept)
+
+
+def test_behind_camera_triangles_are_dropped():
+    pos = TRIANGLE.copy()
+    pos[1, 3] = 0.0
+    r, _, _ = oracle.rasterize_forward(pos, ONE, (16, 16))
+    assert r[..., 3].sum() == 0
+
+
+def test_bins_hold_every_winner():
+    s = scenes.chain_scene(level=4, res=(96, 112), tex_size=8, seed=2)
+    r, _, _ = oracle.rasterize_forward(s.clip, s.indices, s.resolution)
+    offsets, lst = oracle.build_tile_bins(s.clip, s.indices, 112, 96)
+    tiles_x = (112 + 15) // 16
+    ys, xs = np.nonzero(r[..., 3])
+    for y, x in zip(ys, xs):
+        t = (y // 16) * tiles_x + x // 16
+        assert int(r[y, x, 3]) - 1 in lst[offsets[t]:offsets[t + 1]]
+
+
+def test_interpolate_identity_attributes():
+    raster, _, _ = oracle.rasterize_forward(TRIANGLE, ONE, (20, 16))
+    out = oracle.interpolate_forward(np.eye(3, dtype=np.float32), ONE, raster)
+    cov = raster[..., 3] > 0
+    np.testing.assert_array_equal(out[cov, 0], raster[cov, 0])
+    np.testing.assert_array_equal(out[cov, 1], raster[cov, 1])
+    np.testing.assert_allclose(out[cov, 2], 1 - raster[cov, 0] - raster[cov, 1], atol=1e-7)
+    assert (out[~cov] == 0).all()
+
+
+def _chain(seed=3, level=3, res=(48, 56), tex=16, c=3):
+    s = scenes.chain_scene(level=level, res=res, tex_size=tex, channels=c, seed=seed)
+    r, _, _ = oracle.rasterize_forward(s.clip, s.indices, s.resolution)
+    return s, r
+
+
+def test_interpolate_adjoint_and_reference_mode():
+    s, r = _chain()
+    rng = np.random.default_rng(0)
+    A = 5
+    x = rng.standard_normal((s.clip.shape[0], A)).astype(np.float32)
+    y = rng.standard_normal(r.shape[:2] + (A,)).astype(np.float32)
+    jx = oracle.interpolate_forward(x, s.indices, r)
+    jty, _ = oracle.interpolate_backward(x, s.indices, r, y)
+    lhs = float((jx.astype(np.float64) * y).sum())
+    rhs = float((x.astype(np.float64) * jty).sum())
+    assert abs(lhs - rhs) <= 1e-5 * (abs(lhs) + 1)
+    ref_a, ref_r = oracle.interpolate_backward(x, s.indices, r, y, reference_mode=True)
+    fast_a, fast_r = oracle.interpolate_backward(x, s.indices, r, y, reference_mode=False)
+    np.testing.assert_array_equal(ref_a, fast_a)  # same sums, same order
+    np.testing.assert_array_equal(ref_r, fast_r)
+
+
+def test_interpolate_grad_raster_finite_difference():
+    s, r = _chain()
+    rng = np.random.default_rng(1)
+    attr = rng.standard_normal((s.clip.shape[0], 4))
+    g = rng.standard_normal(r.shape[:2] + (4,))
+    _, gr = oracle.interpolate_backward(attr, s.indices, r, g)
+    rd = r.astype(np.float64)
+    h = 1e-6
+    for ch in (0, 1):
+        rp, rn = rd.copy(), rd.copy()
+        rp[..., ch] += h
+        rn[..., ch] -= h
+        fd = ((oracle.interpolate_forward_f64(attr, s.indices, rp) -
+               oracle.interpolate_forward_f64(attr, s.indices, rn)) * g).sum(-1) / (2 * h)
+        np.testing.assert_allclose(gr[..., ch], fd, rtol=1e-4, atol=1e-4)
+
+
+def test_texture_constant_and_linear():
+    s, r = _chain()
+    cov = r[..., 3] > 0
+    uv = oracle.interpolate_forward(s.uv, s.indices, r)
+    const = np.full((8, 8, 3), 0.375, np.float32)
+    out = oracle.texture_forward(const, uv, r)
+    np.testing.assert_allclose(out[cov], 0.375, rtol=1e-6)
+    g = np.ones_like(out)
+    _, gu = oracle.texture_backward(const, uv, r, g)
+    assert np.abs(gu).max() < 1e-5
+    W = 64
+    lin = np.tile(np.arange(W, dtype=np.float32)[None, :, None], (4, 1, 1))
+    u = np.linspace(0.1, 0.9, 37, dtype=np.float32)
+    uvl = np.stack([u, np.full_like(u, 0.5)], -1)[None]
+    rl = np.zeros((1, 37, 4), np.float32)
+    rl[..., 3] = 1
+    out = oracle.texture_forward(lin, uvl, rl)
+    np.testing.assert_allclose(out[0, :, 0], u * W - 0.5, rtol=1e-5)
+    _, gu = oracle.texture_backward(lin, uvl, rl, np.ones((1, 37, 1), np.float32))
+    np.testing.assert_allclose(gu[0, :, 0], W, rtol=1e-5)
+
+
+@pytest.mark.parametrize("mode", ["clamp", "wrap", "mirror"])
+def test_texture_address_modes_closed_form(mode):
+    W, H = 5, 3
+    tex = np.arange(W * H, dtype=np.float32).reshape(H, W, 1)
+    xs = np.arange(-12, 13)
+    uv = np.stack([(xs + 0.5) / W, np.full(xs.shape, 1.5 / H)], -1).astype(np.float32)[None]
+    r = np.zeros((1, xs.size, 4), np.float32)
+    r[..., 3] = 1
+    out = oracle.texture_forward(tex, uv, r, mode)[0, :, 0]
+    if mode == "clamp":
+        col = np.clip(xs, 0, W - 1)
+    elif mode == "wrap":
+        col = np.mod(xs, W)
+    else:
+        m = np.mod(xs, 2 * W)
+        col = np.where(m < W, m, 2 * W - 1 - m)
+    np.testing.assert_allclose(out, tex[1, col, 0], atol=1e-5)
+
+
+def test_texture_adjoint():
+    s, r = _chain(c=4)
+    uv = oracle.interpolate_forward(s.uv, s.indices, r)
+    rng = np.random.default_rng(2)
+    x = rng.standard_normal(s.texture.shape).astype(np.float32)
+    y = rng.standard_normal(r.shape[:2] + (4,)).astype(np.float32)
+    jx = oracle.texture_forward(x, uv, r)
+    jty, _ = oracle.texture_backward(x, uv, r, y)
+    lhs = float((jx.astype(np.float64) * y).sum())
+    rhs = float((x.astype(np.float64) * jty).sum())
+    assert abs(lhs - rhs) <= 1e-5 * (abs(lhs) + 1)
+
+
+def test_texture_grad_uv_finite_difference():
+    s, r = _chain(c=2)
+    uv = oracle.interpolate_forward(s.uv, s.indices, r).astype(np.float64)
+    g = np.random.default_rng(3).standard_normal(r.shape[:2] + (2,))
+    _, gu = oracle.texture_backward(s.texture, uv.astype(np.float32), r, g.astype(np.float32))
+    h = 1e-7
+    rd = r.astype(np.float64)
+    # texel-boundary crossings make bilinear FD one-sided; compare where both sides stay in one cell
+    for ch, size in ((0, s.texture.shape[1]), (1, s.texture.shape[0])):
+        up, un = uv.copy(), uv.copy()
+        up[..., ch] += h
+        un[..., ch] -= h
+        fd = ((oracle.texture_forward_f64(s.texture, up, rd) - oracle.texture_forward_f64(s.texture, un, rd)) *
+              g).sum(-1) / (2 * h)
+        t = uv[..., ch] * size - 0.5
+        frac = t - np.floor(t)
+        ok = (r[..., 3] > 0) & (frac > 1e-3) & (frac < 1 - 1e-3)
+        np.testing.assert_allclose(gu[ok, ch], fd[ok], rtol=2e-3, atol=2e-3)
+
+
+def test_rasterize_backward_finite_difference_fp64():
+    """Analytic position gradient vs central FD (h=1e-4) of the FP64 raster with frozen
+    visibility.  The reference backward uses 1/(area +- 1e-6): relative bias ~1e-6/|area|."""
+    pos, uv, idx = scenes.uv_sphere(32, 33)
+    m = scenes.world_to_clip(np.array([0.3, 0.8, 2.7]), 1.0)
+    clip = scenes.to_clip(pos, m)
+    res = (128, 128)
+    r, _, _ = oracle.rasterize_forward(clip, idx, res)
+    rng = np.random.default_rng(4)
+    g = rng.standard_normal(r.shape).astype(np.float32)
+    g[..., 2:] = 0
+    analytic = oracle.rasterize_backward(clip, idx, r, g)
+    h = 1e-4
+    base = clip.astype(np.float64)
+    used = np.unique(idx[(np.unique(r[..., 3][r[..., 3] > 0]).astype(np.int64) - 1)])
+    checked = 0
+    for v in rng.choice(used, 12, replace=False):
+        for c in (0, 1, 3):
+            p1, p2 = base.copy(), base.copy()
+            p1[v, c] += h
+            p2[v, c] -= h
+            f1 = oracle.raster_fixed_f64(p1, idx, r)
+            f2 = oracle.raster_fixed_f64(p2, idx, r)
+            fd = (((f1 - f2)[..., :2]) * g[..., :2]).sum() / (2 * h)
+            assert abs(analytic[v, c] - fd) <= 2e-3 * max(abs(fd), 1.0), (v, c, analytic[v, c], fd)
+            checked += 1
+    assert checked == 36
+    assert (analytic[:, 2] == 0).all()
+
+
+def test_photometric_gradient_matches_torch_autograd():
+    torch = pytest.importorskip("torch")
+    rng = np.random.default_rng(5)
+    th, tw, P = 12, 10, 257
+    tex = rng.uniform(0, 1, (th, tw, 3)).astype(np.float32)
+    uv = rng.uniform(-0.05, 1.05, (P, 2)).astype(np.float32)
+    target = rng.uniform(0, 1, (P, 3)).astype(np.float32)
+    mask = (rng.uniform(0, 1, P) * (rng.uniform(0, 1, P) > 0.3)).astype(np.float32)
+    eps, norm = 1e-3, 1.0 / (3.0 * mask.sum())
+    grad, loss = oracle.photometric_gradient(tex, uv, target, mask, eps, norm)
+    # independent torch restatement of the loss (clamp bilinear, Charbonnier), autograd for the gradient
+    t = torch.tensor(tex, dtype=torch.float64, requires_grad=True)
+    tx = torch.tensor(uv[:, 0], dtype=torch.float64) * tw - 0.5
+    ty = torch.tensor(uv[:, 1], dtype=torch.float64) * th - 0.5
+    x0, y0 = torch.floor(tx), torch.floor(ty)
+    fx, fy = tx - x0, ty - y0
+    x0, y0 = x0.long(), y0.long()
+    cx = lambda v: v.clamp(0, tw - 1)
+    cy = lambda v: v.clamp(0, th - 1)
+    c = (t[cy(y0), cx(x0)] * ((1 - fx) * (1 - fy))[:, None] + t[cy(y0), cx(x0 + 1)] * (fx * (1 - fy))[:, None] +
+         t[cy(y0 + 1), cx(x0)] * ((1 - fx) * fy)[:, None] + t[cy(y0 + 1), cx(x0 + 1)] * (fx * fy)[:, None])
+    m = torch.tensor(mask, dtype=torch.float64)[:, None]
+    L = (torch.sqrt((c - torch.tensor(target, dtype=torch.float64)) ** 2 + eps ** 2) * m).sum() * norm
+    L.backward()
+    np.testing.assert_allclose(grad, t.grad.numpy(), rtol=1e-4, atol=1e-7)
+    assert abs(loss - L.item()) <= 1e-5 * L.item()
+
+
+def test_seam_gradient_symmetry_and_zero_pairs():
+    rng = np.random.default_rng(6)
+    tex = rng.uniform(0, 1, (9, 11, 3)).astype(np.float32)
+    pairs = rng.uniform(0, 1, (70, 2, 2)).astype(np.float32)
+    eps, norm = 1e-4, float(np.float32(1.0) / np.float32(70 * 3))
+    g, loss = oracle.seam_gradient(tex, pairs, eps, norm)
+    g2, loss2 = oracle.seam_gradient(tex, pairs[:, ::-1].copy(), eps, norm)
+    np.testing.assert_allclose(g, g2, rtol=1e-5, atol=1e-8)
+    assert abs(loss - loss2) < 1e-6
+    assert abs(g.sum()) < 1e-5  # every +g*w has a matching -g*w (weights sum to 1)
+    same = np.repeat(pairs[:, :1], 2, axis=1)
+    g3, loss3 = oracle.seam_gradient(tex, same, eps, norm)
+    assert np.abs(g3).max() == 0.0
+    assert abs(loss3 - 70 * 3 * eps * norm) < 1e-8
+
+
+def test_adam_matches_torch_optim():
+    torch = pytest.importorskip("torch")
+    rng = np.random.default_rng(7)
+    n = 1000
+    tex = rng.uniform(0.2, 0.8, n).astype(np.float32)
+    m = np.zeros(n, np.float32)
+    v = np.zeros(n, np.float32)
+    p = torch.tensor(tex.astype(np.float64), requires_grad=True)
+    opt = torch.optim.Adam([p], lr=1e-2, betas=(0.9, 0.999), eps=1e-8)
+    for step in range(20):
+        g = rng.standard_normal(n).astype(np.float32) * 0.1
+        grad = g.copy()
+        bc1 = 1.0 - 0.9 ** (step + 1)
+        bc2 = 1.0 - 0.999 ** (step + 1)
+        oracle.adam_update(tex, grad, m, v, 1e-2, 0.9, 0.999, 1e-8, bc1, bc2, -10.0, 10.0)
+        assert (grad == 0).all()  # atlas_adam_update.hlsl:42 zeroes the gradient
+        p.grad = torch.tensor(g.astype(np.float64))
+        opt.step()
+        np.testing.assert_allclose(tex, p.detach().numpy(), rtol=1e-5, atol=1e-6)
+
+
+def test_refine_schedule():
+    s = oracle.refine_schedule(100, 4, [10.0, 0.0, 30.0], 5e-3, 2.5e-4, 0.9, 0.999, seam_steps=5)
+    assert s["lr"][0] == np.float32(5e-3)
+    for k in (0, 37, 99):
+        expect = 2.5e-4 + 0.5 * (5e-3 - 2.5e-4) * (1 + math.cos(math.pi * k / 100))
+        assert abs(s["lr"][k] - expect) < 1e-8
+        b1, b2 = float(np.float32(0.9)), float(np.float32(0.999))  # the betas are float options
+        assert abs(s["bc1"][k] - (1 - b1 ** (k + 1))) < 1e-6
+        assert abs(s["bc2"][k] - (1 - b2 ** (k + 1))) < 1e-7
+        views = [(k * 4 + b) % 3 for b in range(4)]
+        assert list(s["view_idx"][k * 4:k * 4 + 4]) == views
+        w = sum([10.0, 0.0, 30.0][i] for i in views)
+        assert abs(s["norm"][k] - 1.0 / (3.0 * w)) < 1e-9
+    assert len(s["seam_bc1"]) == 5
+
+
+def quad_scene(n=16):
+    """A [-1,1]^2 quad facing an identity camera with uv = [0,1]^2."""
+    pos = np.array([[-1, -1, 0], [1, -1, 0], [1, 1, 0], [-1, 1, 0]], np.float32)
+    uv = np.array([[0, 0], [1, 0], [1, 1], [0, 1]], np.float32)
+    idx = np.array([[0, 1, 2], [0, 2, 3]], np.uint32)
+    img = np.zeros((n, n, 4), np.float32)
+    img[..., 0], img[..., 1], img[..., 3] = 0.75, 0.25, 1.0
+    return pos, uv, idx, img
+
+
+def test_refine_loss_decreases_like_python_smoke():
+    """tests/python_smoke.py:83-100: 3 steps, batch 1, lr 0.05 -> 0.01, loss decreases."""
+    pos, uv, idx, img = quad_scene()
+    init = np.full((16, 16, 3), 0.25, np.float32)
+    color, hist, seam = oracle.refine(init, pos, uv, idx, [img], np.eye(4, dtype=np.float32)[None], steps=3,
+                                      batch_size=1, learning_rate=0.05, minimum_learning_rate=0.01,
+                                      seam_polish_steps=0)
+    assert color.shape == (16, 16, 3) and hist.shape == (3,) and np.isfinite(hist).all()
+    assert hist[-1] < hist[0]
+    assert seam.shape == (0,)
