@@ -91,11 +91,14 @@ __device__ __forceinline__ uint32_t hash_slot(uint32_t* keys, uint32_t key) {
 // One 16x16 tile per workgroup.  grad_raster is written per pixel; the
 // attribute gradient terms b_c * g[p,a] of the pixel's triangle accumulate in
 // LDS per triangle, then one atomic per (triangle corner, attribute).
+// kA > 0: compile-time attribute count with vector loads (kA % 4 == 0).
+template <int kA>
 __global__ __launch_bounds__(kBlock) void k_interp_bwd(PixelTable pt, const float* __restrict__ attr, uint32_t V,
-                                                       uint32_t A, const uint32_t* __restrict__ idx, uint32_t T,
+                                                       uint32_t A_rt, const uint32_t* __restrict__ idx, uint32_t T,
                                                        float* __restrict__ grad_attr) {
     __shared__ uint32_t keys[kSlots];
     __shared__ float acc[kSlots * 3 * kMaxHashedAttrs];
+    const uint32_t A = kA > 0 ? static_cast<uint32_t>(kA) : A_rt;
     const bool hashed = A <= kMaxHashedAttrs;
     const uint32_t stride = 3 * A;
     if (hashed) {
@@ -119,28 +122,56 @@ __global__ __launch_bounds__(kBlock) void k_interp_bwd(PixelTable pt, const floa
             const bool in_range = i0 < V && i1 < V && i2 < V;
             const float u = r.x, v = r.y;
             const float w = 1.0f - u - v;
-            uint32_t slot = 0;
-            if (hashed) slot = hash_slot(keys, id);
-            for (uint32_t a = 0; a < A; ++a) {
-                const float og = gin[a];
-                if (in_range) {
-                    // interpolate_grad_raster.hlsl:153-161
-                    const float a0 = attr[static_cast<size_t>(i0) * A + a];
-                    const float a1 = attr[static_cast<size_t>(i1) * A + a];
-                    const float a2 = attr[static_cast<size_t>(i2) * A + a];
-                    gx += og * (a0 - a2);
-                    gy += og * (a1 - a2);
+            if constexpr (kA > 0) {
+                // vector path; out-of-range vertices contribute nothing (flush skips them)
+                float og[kA], a0[kA], a1[kA], a2[kA];
+                const float4 zero4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+                for (int k = 0; k < kA / 4; ++k) {
+                    const float4 gv = reinterpret_cast<const float4*>(gin)[k];
+                    const float4 v0 = in_range ? reinterpret_cast<const float4*>(attr + static_cast<size_t>(i0) * kA)[k] : zero4;
+                    const float4 v1 = in_range ? reinterpret_cast<const float4*>(attr + static_cast<size_t>(i1) * kA)[k] : zero4;
+                    const float4 v2 = in_range ? reinterpret_cast<const float4*>(attr + static_cast<size_t>(i2) * kA)[k] : zero4;
+                    og[4 * k] = gv.x; og[4 * k + 1] = gv.y; og[4 * k + 2] = gv.z; og[4 * k + 3] = gv.w;
+                    a0[4 * k] = v0.x; a0[4 * k + 1] = v0.y; a0[4 * k + 2] = v0.z; a0[4 * k + 3] = v0.w;
+                    a1[4 * k] = v1.x; a1[4 * k + 1] = v1.y; a1[4 * k + 2] = v1.z; a1[4 * k + 3] = v1.w;
+                    a2[4 * k] = v2.x; a2[4 * k + 1] = v2.y; a2[4 * k + 2] = v2.z; a2[4 * k + 3] = v2.w;
                 }
-                // interpolate_grad_attributes.hlsl:97-110 (vertex indices >= V never match)
-                const float t0 = u * og, t1 = v * og, t2 = w * og;
-                if (hashed) {
-                    atomicAdd(&acc[slot * stride + a], t0);
-                    atomicAdd(&acc[slot * stride + A + a], t1);
-                    atomicAdd(&acc[slot * stride + 2 * A + a], t2);
-                } else {
-                    if (i0 < V) unsafeAtomicAdd(&grad_attr[static_cast<size_t>(i0) * A + a], t0);
-                    if (i1 < V) unsafeAtomicAdd(&grad_attr[static_cast<size_t>(i1) * A + a], t1);
-                    if (i2 < V) unsafeAtomicAdd(&grad_attr[static_cast<size_t>(i2) * A + a], t2);
+                const uint32_t slot = hash_slot(keys, id);
+#pragma unroll
+                for (int a = 0; a < kA; ++a) {
+                    if (in_range) {
+                        gx += og[a] * (a0[a] - a2[a]);
+                        gy += og[a] * (a1[a] - a2[a]);
+                    }
+                    atomicAdd(&acc[slot * stride + a], u * og[a]);
+                    atomicAdd(&acc[slot * stride + kA + a], v * og[a]);
+                    atomicAdd(&acc[slot * stride + 2 * kA + a], w * og[a]);
+                }
+            } else {
+                uint32_t slot = 0;
+                if (hashed) slot = hash_slot(keys, id);
+                for (uint32_t a = 0; a < A; ++a) {
+                    const float og = gin[a];
+                    if (in_range) {
+                        // interpolate_grad_raster.hlsl:153-161
+                        const float a0 = attr[static_cast<size_t>(i0) * A + a];
+                        const float a1 = attr[static_cast<size_t>(i1) * A + a];
+                        const float a2 = attr[static_cast<size_t>(i2) * A + a];
+                        gx += og * (a0 - a2);
+                        gy += og * (a1 - a2);
+                    }
+                    // interpolate_grad_attributes.hlsl:97-110 (vertex indices >= V never match)
+                    const float t0 = u * og, t1 = v * og, t2 = w * og;
+                    if (hashed) {
+                        atomicAdd(&acc[slot * stride + a], t0);
+                        atomicAdd(&acc[slot * stride + A + a], t1);
+                        atomicAdd(&acc[slot * stride + 2 * A + a], t2);
+                    } else {
+                        if (i0 < V) unsafeAtomicAdd(&grad_attr[static_cast<size_t>(i0) * A + a], t0);
+                        if (i1 < V) unsafeAtomicAdd(&grad_attr[static_cast<size_t>(i1) * A + a], t1);
+                        if (i2 < V) unsafeAtomicAdd(&grad_attr[static_cast<size_t>(i2) * A + a], t2);
+                    }
                 }
             }
         }
@@ -328,7 +359,12 @@ hipError_t interp_forward(const PixelTable& pt, const float* attr, uint32_t V, u
 hipError_t interp_backward(const PixelTable& pt, const float* attr, uint32_t V, uint32_t A, const uint32_t* idx,
                            uint32_t T, float* grad_attr, hipStream_t st) {
     if (!pt.total_tiles) return hipSuccess;
-    k_interp_bwd<<<pt.total_tiles, kBlock, 0, st>>>(pt, attr, V, A, idx, T, grad_attr);
+    bool aligned = (reinterpret_cast<uintptr_t>(attr) % 16) == 0;
+    for (uint32_t k = 0; k < pt.n; ++k) aligned &= (reinterpret_cast<uintptr_t>(pt.v[k].in0) % 16) == 0;
+    if (A == 8 && aligned)
+        k_interp_bwd<8><<<pt.total_tiles, kBlock, 0, st>>>(pt, attr, V, A, idx, T, grad_attr);
+    else
+        k_interp_bwd<0><<<pt.total_tiles, kBlock, 0, st>>>(pt, attr, V, A, idx, T, grad_attr);
     return hipGetLastError();
 }
 

@@ -238,8 +238,7 @@ pdr_status pdr_rasterize_backward(pdr_context ctx, const float* pos, uint32_t V,
         vt.v[0].grad_raster = dgrad;
         vt.v[0].grad_pos = dout;
         finalize_table(vt);
-        hip_check(pdr::fill_f32(dout, 0.0f, static_cast<size_t>(V) * 4, st), "fill");
-        ctx->timed("raster_bwd", st, [&] { hip_check(pdr::raster_backward(vt, didx, T, V, st), "raster_backward"); });
+        run_raster_backward(ctx, vt, didx, T, V, st);
         if (!device) {
             copy_out(grad_pos_out, dout, static_cast<size_t>(V) * 16, st);
             sync(st);
@@ -280,10 +279,7 @@ pdr_status pdr_rasterize_backward_batched(pdr_context ctx, const pdr_raster_view
                 vt.v[k].grad_pos = in.grad_positions;
             }
             finalize_table(vt);
-            for (uint32_t k = 0; k < vt.n; ++k)
-                hip_check(pdr::fill_f32(vt.v[k].grad_pos, 0.0f, static_cast<size_t>(V) * 4, st), "fill");
-            ctx->timed("raster_bwd", st,
-                       [&] { hip_check(pdr::raster_backward(vt, idx, T, V, st), "raster_backward"); });
+            run_raster_backward(ctx, vt, idx, T, V, st);
         }
     });
 }

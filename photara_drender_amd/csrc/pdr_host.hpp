@@ -90,6 +90,7 @@ struct pdr_context_s {
         size_t bytes = 0;
     };
     std::unordered_map<std::string, Slot> scratch;
+    bool keys_clean = false;  // raster key buffer is all-ones
 
     // Optional per-kernel HIP-event timing (pdr_context_set_timing).
     bool timing = false;
@@ -264,10 +265,14 @@ inline pdr::ViewDesc make_view(const float* pos, uint32_t W, uint32_t H, const f
 }
 
 inline void finalize_table(pdr::ViewTable& t) {
-    uint32_t tiles = 0, pixels = 0;
+    uint32_t tiles = 0, pixels = 0, cols = 0, rows = 0;
     for (uint32_t k = 0; k < t.n; ++k) {
         t.v[k].tile_base = tiles;
         t.v[k].pixel_base = pixels;
+        t.v[k].col_base = cols;
+        t.v[k].row_base = rows;
+        cols += t.v[k].width;
+        rows += t.v[k].height;
         const uint64_t nt = static_cast<uint64_t>(tiles) + static_cast<uint64_t>(t.v[k].tiles_x) * t.v[k].tiles_y;
         const uint64_t np = static_cast<uint64_t>(pixels) + static_cast<uint64_t>(t.v[k].width) * t.v[k].height;
         if (nt > 0x7FFFFFFFu || np > 0xFFFFFFFFu) raise(PDR_OVERFLOW, "Rasterization output size overflow");
@@ -298,24 +303,63 @@ inline pdr::PixelTable pixel_table_one(const float* raster, const float* in0, co
     return t;
 }
 
-// Bins + raster forward of one table of views (device pointers), on stream st.
+inline uint32_t table_cols(const pdr::ViewTable& vt) {
+    return vt.v[vt.n - 1].col_base + vt.v[vt.n - 1].width;
+}
+inline uint32_t table_rows(const pdr::ViewTable& vt) {
+    return vt.v[vt.n - 1].row_base + vt.v[vt.n - 1].height;
+}
+
+// Scratch of the triangle-parallel raster for one table of views: sample
+// tables filled, key buffer guaranteed all-ones (the resolve pass restores it).
+inline pdr::RasterScratch raster_scratch(pdr_context ctx, const pdr::ViewTable& vt, uint32_t T, hipStream_t st,
+                                         bool with_keys) {
+    pdr::RasterScratch s;
+    s.sx = ctx->get_t<float>("sample_x", table_cols(vt));
+    s.sy = ctx->get_t<float>("sample_y", table_rows(vt));
+    s.queue = ctx->get_t<uint2>("large_queue", static_cast<size_t>(vt.n) * T);
+    s.counters = ctx->get_t<uint32_t>("large_counters", 4);
+    if (with_keys) {
+        const size_t before = ctx->scratch["raster_keys"].bytes;
+        s.keys = ctx->get_t<unsigned long long>("raster_keys", vt.total_pixels);
+        const size_t after = ctx->scratch["raster_keys"].bytes;
+        if (after != before || !ctx->keys_clean) hip_check(pdr::fill_keys(s.keys, after / 8, st), "fill_keys");
+    }
+    hip_check(pdr::sample_tables(vt, s.sx, s.sy, st), "sample_tables");
+    return s;
+}
+
+// Triangle-parallel raster forward of one table of views (device pointers), on stream st.
 inline void run_raster_forward(pdr_context ctx, const pdr::ViewTable& vt, const uint32_t* idx, uint32_t T, uint32_t V,
-                        uint32_t cull, int want_deriv, hipStream_t st) {
-    if (!vt.total_tiles) return;
-    pdr::BinScratch bs;
-    bs.counts = ctx->get_t<uint32_t>("bin_counts", vt.total_tiles);
-    bs.offsets = ctx->get_t<uint32_t>("bin_offsets", vt.total_tiles + 1);
-    bs.scan_temp_bytes = pdr::bin_scan_temp_bytes(vt.total_tiles);
-    bs.scan_temp = ctx->get("bin_scan_temp", bs.scan_temp_bytes);
-    ctx->timed("bin_count", st, [&] { hip_check(pdr::bin_count(vt, idx, T, V, bs, st), "bin_count"); });
-    hip_check(hipMemcpyAsync(ctx->pinned, bs.offsets + vt.total_tiles, 4, hipMemcpyDeviceToHost, st),
-              "hipMemcpyAsync");
-    sync(st);
-    const uint32_t total = ctx->pinned[0];
-    bs.list = ctx->get_t<uint32_t>("bin_list", std::max<uint32_t>(total, 1));
-    ctx->timed("bin_fill", st, [&] { hip_check(pdr::bin_fill(vt, idx, T, V, bs, st), "bin_fill"); });
-    ctx->timed("raster_fwd", st,
-               [&] { hip_check(pdr::raster_forward(vt, idx, T, V, cull, want_deriv, bs, st), "raster_forward"); });
+                               uint32_t cull, int want_deriv, hipStream_t st) {
+    if (!vt.total_pixels) return;
+    const pdr::RasterScratch s = raster_scratch(ctx, vt, T, st, true);
+    ctx->keys_clean = false;
+    ctx->timed("raster_candidates", st,
+               [&] { hip_check(pdr::raster_candidates(vt, idx, T, V, cull, s, st), "raster_candidates"); });
+    ctx->timed("raster_resolve", st,
+               [&] { hip_check(pdr::raster_resolve(vt, idx, want_deriv, s, st), "raster_resolve"); });
+    ctx->keys_clean = true;
+}
+
+// Per-triangle corner sums + ordered vertex reduce for one table of views.
+inline void run_raster_backward(pdr_context ctx, const pdr::ViewTable& vt, const uint32_t* idx, uint32_t T, uint32_t V,
+                                hipStream_t st) {
+    const pdr::RasterScratch s = raster_scratch(ctx, vt, T, st, false);
+    pdr::VertexCsr csr;
+    csr.counts = ctx->get_t<uint32_t>("csr_counts", static_cast<size_t>(V) + 1);
+    csr.offsets = ctx->get_t<uint32_t>("csr_offsets", static_cast<size_t>(V) + 1);
+    csr.iota = ctx->get_t<uint32_t>("csr_iota", static_cast<size_t>(T) * 3);
+    csr.keys_out = ctx->get_t<uint32_t>("csr_keys", static_cast<size_t>(T) * 3);
+    csr.corners = ctx->get_t<uint32_t>("csr_corners", static_cast<size_t>(T) * 3);
+    csr.temp_bytes = pdr::csr_temp_bytes(T, V);
+    csr.temp = ctx->get("csr_temp", csr.temp_bytes);
+    float4* corner = ctx->get_t<float4>("corner_grads", static_cast<size_t>(vt.n) * T * 3);
+    ctx->timed("vertex_csr", st, [&] { hip_check(pdr::build_vertex_csr(idx, T, V, csr, st), "build_vertex_csr"); });
+    ctx->timed("corner_grads", st,
+               [&] { hip_check(pdr::corner_gradients(vt, idx, T, V, s, corner, st), "corner_gradients"); });
+    ctx->timed("vertex_reduce", st,
+               [&] { hip_check(pdr::vertex_reduce(vt, T, V, csr, corner, st), "vertex_reduce"); });
 }
 
 inline void validate_mesh_counts(const float* pos, uint32_t V, const uint32_t* idx, uint32_t T) {

@@ -21,6 +21,8 @@ struct ViewDesc {
     float vx, vy, vw, vh;     // resolved viewport
     uint32_t tile_base;       // first global tile of this view
     uint32_t pixel_base;      // first global pixel of this view
+    uint32_t col_base;        // first entry of this view in the column sample table
+    uint32_t row_base;        // first entry of this view in the row sample table
 };
 
 struct ViewTable {
@@ -50,25 +52,42 @@ struct PixelTable {
     uint32_t total_pixels;
 };
 
-// Scratch needed by the binning pass (owned by the context).
-struct BinScratch {
-    uint32_t* counts = nullptr;   // [total_tiles]
-    uint32_t* offsets = nullptr;  // [total_tiles + 1]
-    uint32_t* list = nullptr;     // [capacity]
-    void* scan_temp = nullptr;
-    size_t scan_temp_bytes = 0;
+// Scratch of the triangle-parallel raster (owned by the context).
+struct RasterScratch {
+    unsigned long long* keys = nullptr;  // [total_pixels] (depth, id) keys; all ~0 between calls
+    float* sx = nullptr;                 // [sum of widths] pixel-centre NDC x per column
+    float* sy = nullptr;                 // [sum of heights] pixel-centre NDC y per row
+    uint2* queue = nullptr;              // [views * T] (view, triangle) with large rectangles
+    uint32_t* counters = nullptr;        // [4] queue length, next queue entry
+};
+
+// Vertex -> corner adjacency (corners ascending per vertex).
+struct VertexCsr {
+    uint32_t* counts = nullptr;   // [V+1]
+    uint32_t* offsets = nullptr;  // [V+1]
+    uint32_t* iota = nullptr;     // [3T]
+    uint32_t* keys_out = nullptr; // [3T]
+    uint32_t* corners = nullptr;  // [3T]
+    void* temp = nullptr;
+    size_t temp_bytes = 0;
 };
 
 // ---- raster.hip
-size_t bin_scan_temp_bytes(uint32_t total_tiles);
-hipError_t bin_count(const ViewTable& views, const uint32_t* idx, uint32_t T, uint32_t V, const BinScratch& s,
-                     hipStream_t st);
-hipError_t bin_fill(const ViewTable& views, const uint32_t* idx, uint32_t T, uint32_t V, const BinScratch& s,
-                    hipStream_t st);
-hipError_t raster_forward(const ViewTable& views, const uint32_t* idx, uint32_t T, uint32_t V, uint32_t cull,
-                          int want_deriv, const BinScratch& s, hipStream_t st);
-// Output arrays (grad_pos / grad_attr / grad_tex) must be zeroed by the caller.
-hipError_t raster_backward(const ViewTable& views, const uint32_t* idx, uint32_t T, uint32_t V, hipStream_t st);
+hipError_t fill_keys(unsigned long long* keys, size_t n, hipStream_t st);
+hipError_t sample_tables(const ViewTable& views, float* sx, float* sy, hipStream_t st);
+// k_raster_small + k_raster_large: (depth, id) keys of every covered candidate pixel
+hipError_t raster_candidates(const ViewTable& views, const uint32_t* idx, uint32_t T, uint32_t V, uint32_t cull,
+                             const RasterScratch& s, hipStream_t st);
+// k_raster_resolve: raster / derivatives from the keys; resets the keys
+hipError_t raster_resolve(const ViewTable& views, const uint32_t* idx, int want_deriv, const RasterScratch& s,
+                          hipStream_t st);
+size_t csr_temp_bytes(uint32_t T, uint32_t V);
+hipError_t build_vertex_csr(const uint32_t* idx, uint32_t T, uint32_t V, const VertexCsr& csr, hipStream_t st);
+// corner: [views * 3T] float4 per-triangle corner gradients
+hipError_t corner_gradients(const ViewTable& views, const uint32_t* idx, uint32_t T, uint32_t V,
+                            const RasterScratch& s, float4* corner, hipStream_t st);
+hipError_t vertex_reduce(const ViewTable& views, uint32_t T, uint32_t V, const VertexCsr& csr, const float4* corner,
+                         hipStream_t st);
 
 // ---- interp_texture.hip
 hipError_t interp_forward(const PixelTable& views, const float* attr, uint32_t V, uint32_t A, const uint32_t* idx,

@@ -68,9 +68,26 @@ FORWARD_CASES = {
 }
 
 
+def tiny_soup(seed, n=400):
+    """Sub-pixel, epsilon-scale and sliver triangles whose epsilon-inclusive coverage
+    reaches past their bounding boxes (rasterize_forward.hlsl:103-113)."""
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(-1.05, 1.05, size=(n, 1, 2))
+    size = np.where(rng.random((n, 1, 1)) < 0.5, 1e-5, 2e-3)
+    v = np.zeros((n, 3, 4), np.float32)
+    v[..., :2] = c + rng.uniform(-1, 1, size=(n, 3, 2)) * size
+    v[::3, 2, :2] = 0.5 * (v[::3, 0, :2] + v[::3, 1, :2]) + 1e-7  # slivers
+    v[..., 2] = rng.uniform(-1, 1, size=(n, 3))
+    v[..., 3] = 1.0
+    return v.reshape(-1, 4), np.arange(3 * n, dtype=np.uint32).reshape(n, 3)
+
+
 def _case(name):
     if name in FORWARD_CASES:
         return FORWARD_CASES[name]
+    if name.startswith("tiny_"):
+        v, idx = tiny_soup(int(name.split("_")[1]))
+        return v, idx, (61, 53), None, 0
     if name == "icosphere_512":
         s = scenes.chain_scene(level=5, res=(512, 512), tex_size=64, seed=1)
         return s.clip, s.indices, s.resolution, None, 0
@@ -84,7 +101,8 @@ def _case(name):
     return v, idx, (97, 131), None, cull
 
 
-ALL_CASES = list(FORWARD_CASES) + ["icosphere_512", "uvsphere_128", "soup_1", "soup_2", "soupback_3", "soupfront_4"]
+ALL_CASES = list(FORWARD_CASES) + ["icosphere_512", "uvsphere_128", "soup_1", "soup_2", "soupback_3", "soupfront_4",
+                                   "tiny_5", "tiny_6"]
 
 
 @pytest.mark.parametrize("case", ALL_CASES)
@@ -124,7 +142,7 @@ def test_native_reference_checks(rasterizer):
 
 
 @pytest.mark.parametrize("case", ["native_triangle_32", "native_viewport_32", "icosphere_512", "uvsphere_128",
-                                  "soup_1", "soupback_3"])
+                                  "soup_1", "soupback_3", "tiny_5"])
 def test_rasterize_backward(rasterizer, case):
     pos, idx, res, vp, cull = _case(case)
     raster, _, _ = oracle.rasterize_forward(pos, idx, res, cull, False, vp)
@@ -134,7 +152,8 @@ def test_rasterize_backward(rasterizer, case):
     g = rasterizer.backward(pos, idx, raster, grad, vp)
     ref = oracle.rasterize_backward(pos, idx, raster, grad, vp)
     assert (g[:, 2] == 0).all()
-    close_sum(g, ref)
+    # per-triangle sums in the reference's pixel order, vertex sums in its corner order
+    bits_equal(g, ref)
 
 
 @pytest.mark.parametrize("attrs", [1, 2, 3, 8, 11])
@@ -233,3 +252,56 @@ def test_torch_device_path_matches_numpy_path(rasterizer):
     close_sum(uvv.grad.cpu().numpy(), ref_ga, rel=1e-3)
     ref_gp = oracle.rasterize_backward(s.clip, s.indices, ref_r, ref_gr)
     close_sum(pos.grad.cpu().numpy(), ref_gp, rel=1e-3)
+
+
+def test_batched_views_equal_single_views(rasterizer):
+    """pdr_*_batched over uneven views == the single-view entry points, per view."""
+    torch = pytest.importorskip("torch")
+    import ctypes
+
+    from photara_drender_amd import _native
+
+    sc = scenes.multiview_scene(nx=60, ny=40, views=5, attributes=3, seed=9,
+                                sizes=[(64, 64), (48, 80), (100, 70), (33, 17), (128, 96)])
+    dev = torch.device("cuda:0")
+    V, T, A = sc.clips[0].shape[0], sc.indices.shape[0], 3
+    ctx = _native.Context(0)
+    lib = _native.lib()
+    idx = torch.tensor(sc.indices.view(np.int32), device=dev)
+    attr = torch.tensor(sc.attributes, device=dev)
+    n = len(sc.clips)
+    clips = [torch.tensor(c, device=dev) for c in sc.clips]
+    rasters = [torch.empty((h, w, 4), device=dev) for h, w in sc.resolutions]
+    derivs = [torch.empty((h, w, 4), device=dev) for h, w in sc.resolutions]
+    outs = [torch.empty((h, w, A), device=dev) for h, w in sc.resolutions]
+    grads = [torch.tensor(t, device=dev) for t in sc.targets]
+    grs = [torch.empty((h, w, 4), device=dev) for h, w in sc.resolutions]
+    gpos = [torch.empty((V, 4), device=dev) for _ in range(n)]
+    ga = torch.empty((V, A), device=dev)
+    rv = (_native.RasterView * n)()
+    iv = (_native.InterpView * n)()
+    for k, (h, w) in enumerate(sc.resolutions):
+        rv[k].positions, rv[k].width, rv[k].height = clips[k].data_ptr(), w, h
+        rv[k].viewport = (ctypes.c_float * 4)(0, 0, 0, 0)
+        rv[k].raster, rv[k].derivatives = rasters[k].data_ptr(), derivs[k].data_ptr()
+        rv[k].grad_raster, rv[k].grad_positions = grs[k].data_ptr(), gpos[k].data_ptr()
+        iv[k].width, iv[k].height, iv[k].raster = w, h, rasters[k].data_ptr()
+        iv[k].out, iv[k].grad, iv[k].grad_raster = outs[k].data_ptr(), grads[k].data_ptr(), grs[k].data_ptr()
+    s = torch.cuda.current_stream().cuda_stream
+    _native.check(lib.pdr_rasterize_forward_batched(ctx.handle, rv, n, V, idx.data_ptr(), T, 0, 1, s))
+    _native.check(lib.pdr_interpolate_forward_batched(ctx.handle, iv, n, attr.data_ptr(), V, A, idx.data_ptr(), T, s))
+    _native.check(lib.pdr_interpolate_backward_batched(ctx.handle, iv, n, attr.data_ptr(), V, A, idx.data_ptr(), T,
+                                                       ga.data_ptr(), s))
+    _native.check(lib.pdr_rasterize_backward_batched(ctx.handle, rv, n, V, idx.data_ptr(), T, s))
+    torch.cuda.synchronize()
+    ga_ref = np.zeros((V, A), np.float64)
+    for k in range(n):
+        ref_r, ref_d, _ = oracle.rasterize_forward(sc.clips[k], sc.indices, sc.resolutions[k])
+        bits_equal(rasters[k].cpu().numpy(), ref_r)
+        bits_equal(derivs[k].cpu().numpy(), ref_d)
+        bits_equal(outs[k].cpu().numpy(), oracle.interpolate_forward(sc.attributes, sc.indices, ref_r))
+        g_a, g_r = oracle.interpolate_backward(sc.attributes, sc.indices, ref_r, sc.targets[k])
+        bits_equal(grs[k].cpu().numpy(), g_r)
+        ga_ref += g_a
+        bits_equal(gpos[k].cpu().numpy(), oracle.rasterize_backward(sc.clips[k], sc.indices, ref_r, g_r))
+    close_sum(ga.cpu().numpy(), ga_ref)
