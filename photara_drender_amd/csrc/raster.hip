@@ -42,8 +42,10 @@ namespace pdr {
 namespace {
 
 constexpr uint32_t kBlock = 256;
-constexpr uint32_t kInlinePixels = 48;   // candidate rect area handled by one lane
-constexpr uint32_t kSeqPixels = 4096;    // backward box area summed by one lane
+constexpr uint32_t kInlinePixels = 4;    // candidate rectangle tested by its own lane
+constexpr uint32_t kWavePixels = 16384;  // candidate rectangle tested by one wave
+constexpr uint32_t kInlineBox = 8;       // backward box summed by its own lane
+constexpr uint32_t kWaveBox = 65536;     // backward box summed by one wave (in pixel order)
 constexpr uint32_t kLargeBlocks = 1024;  // persistent workgroups for queued work
 
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n) {
@@ -237,29 +239,94 @@ __device__ __forceinline__ unsigned long long test_pixel(const Tri& r, uint32_t 
     return (static_cast<unsigned long long>(order_key(depth)) << 32) | static_cast<unsigned long long>(t + 1u);
 }
 
+// Intra-wave hand-off of LDS records written by other lanes of the same wave.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Keys only decrease, so a stale read can only make the filter let an atomic through.
+__device__ __forceinline__ void key_min(unsigned long long* p, unsigned long long key) {
+    if (key < __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) atomicMin(p, key);
+}
+
+struct TriRec {
+    float4 p0, p1, p2;
+    float s0x, s0y, s1x, s1y, s2x, s2y, sgn;
+    int32_t x0, y0;
+    uint32_t w, h, t;
+};
+
+// One 64-triangle batch per wave: each lane sets up one triangle; rectangles of
+// at most kInlinePixels are tested by their own lane, larger ones (up to
+// kWavePixels) by the whole wave, one pixel per lane, and larger still are
+// queued for k_raster_large.
 __global__ __launch_bounds__(kBlock) void k_raster_small(ViewTable vt, const uint32_t* __restrict__ idx, uint32_t T,
                                                          uint32_t V, uint32_t cull, const float* __restrict__ sxt,
                                                          const float* __restrict__ syt,
                                                          unsigned long long* __restrict__ keys,
                                                          uint2* __restrict__ queue, uint32_t* __restrict__ counters) {
+    __shared__ TriRec recs[kBlock];
     const uint32_t view = blockIdx.y;
     const ViewDesc& vd = vt.v[view];
-    for (uint32_t t = blockIdx.x * kBlock + threadIdx.x; t < T; t += gridDim.x * kBlock) {
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    TriRec* wrecs = recs + wave * 64;
+    unsigned long long* kbase = keys + vd.pixel_base;
+    const uint32_t stride = gridDim.x * kBlock;
+    for (uint32_t batch = (blockIdx.x * 4 + wave) * 64; batch < T; batch += stride) {
+        const uint32_t t = batch + lane;
         Tri r;
-        if (!setup_forward(vd, idx, t, V, cull, r)) continue;
-        const uint32_t w = static_cast<uint32_t>(r.x1 - r.x0 + 1), h = static_cast<uint32_t>(r.y1 - r.y0 + 1);
-        if (w * h > kInlinePixels) {
+        const bool ok = t < T && setup_forward(vd, idx, t, V, cull, r);
+        const uint32_t w = ok ? static_cast<uint32_t>(r.x1 - r.x0 + 1) : 0u;
+        const uint32_t h = ok ? static_cast<uint32_t>(r.y1 - r.y0 + 1) : 0u;
+        const uint32_t area = w * h;
+        bool deferred = false;
+        if (ok && area <= kInlinePixels) {
+            for (int32_t y = r.y0; y <= r.y1; ++y) {
+                const float sy = syt[vd.row_base + y];
+                for (int32_t x = r.x0; x <= r.x1; ++x) {
+                    const unsigned long long key = test_pixel(r, t, sxt[vd.col_base + x], sy);
+                    if (key != ~0ull) key_min(&kbase[static_cast<size_t>(y) * vd.width + x], key);
+                }
+            }
+        } else if (ok && area > kWavePixels) {
             queue[atomicAdd(&counters[0], 1u)] = make_uint2(view, t);
-            continue;
+        } else if (ok) {
+            deferred = true;
+            TriRec& q = wrecs[lane];
+            q.p0 = r.p0;
+            q.p1 = r.p1;
+            q.p2 = r.p2;
+            q.s0x = r.s0x; q.s0y = r.s0y; q.s1x = r.s1x; q.s1y = r.s1y; q.s2x = r.s2x; q.s2y = r.s2y;
+            q.sgn = r.sgn;
+            q.x0 = r.x0;
+            q.y0 = r.y0;
+            q.w = w;
+            q.h = h;
+            q.t = t;
         }
-        unsigned long long* base = keys + vd.pixel_base;
-        for (int32_t y = r.y0; y <= r.y1; ++y) {
-            const float sy = syt[vd.row_base + y];
-            for (int32_t x = r.x0; x <= r.x1; ++x) {
-                const unsigned long long key = test_pixel(r, t, sxt[vd.col_base + x], sy);
-                if (key != ~0ull) atomicMin(&base[static_cast<size_t>(y) * vd.width + x], key);
+        wave_lds_sync();
+        unsigned long long pending = __ballot(deferred);
+        while (pending) {
+            const uint32_t j = __ffsll(pending) - 1;
+            pending &= pending - 1;
+            const TriRec& q = wrecs[j];
+            Tri rr;
+            rr.p0 = q.p0; rr.p1 = q.p1; rr.p2 = q.p2;
+            rr.s0x = q.s0x; rr.s0y = q.s0y; rr.s1x = q.s1x; rr.s1y = q.s1y; rr.s2x = q.s2x; rr.s2y = q.s2y;
+            rr.sgn = q.sgn;
+            const uint32_t qw = q.w, qa = q.w * q.h, qt = q.t;
+            for (uint32_t c = 0; c < qa; c += 64) {
+                const uint32_t i = c + lane;
+                if (i < qa) {
+                    const int32_t x = q.x0 + static_cast<int32_t>(i % qw), y = q.y0 + static_cast<int32_t>(i / qw);
+                    const unsigned long long key = test_pixel(rr, qt, sxt[vd.col_base + x], syt[vd.row_base + y]);
+                    if (key != ~0ull) key_min(&kbase[static_cast<size_t>(y) * vd.width + x], key);
+                }
             }
         }
+        wave_lds_sync();
     }
 }
 
@@ -286,7 +353,7 @@ __global__ __launch_bounds__(kBlock) void k_raster_large(ViewTable vt, const uin
         for (uint32_t i = threadIdx.x; i < w * h; i += kBlock) {
             const int32_t x = r.x0 + static_cast<int32_t>(i % w), y = r.y0 + static_cast<int32_t>(i / w);
             const unsigned long long key = test_pixel(r, q.y, sxt[vd.col_base + x], syt[vd.row_base + y]);
-            if (key != ~0ull) atomicMin(&base[static_cast<size_t>(y) * vd.width + x], key);
+            if (key != ~0ull) key_min(&base[static_cast<size_t>(y) * vd.width + x], key);
         }
     }
 }
@@ -408,27 +475,123 @@ __device__ __forceinline__ void corner_terms(const BwdTri& b, const ViewDesc& vd
     g[2][2] += -sx * g2x - sy * g2y;
 }
 
+// Terms of one pixel (rasterize_backward.hlsl:94-151): returns false when the
+// pixel does not contribute; t9 = (x, y, w) for corners 0..2.
+__device__ __forceinline__ bool pixel_terms(const float4* p, const ViewDesc& vd, uint32_t t, int32_t x, int32_t y,
+                                            const float* __restrict__ sxt, const float* __restrict__ syt,
+                                            float t9[9]) {
+    const size_t pix = static_cast<size_t>(y) * vd.width + static_cast<size_t>(x);
+    if (decode_id(vd.raster[pix * 4 + 3]) != t + 1u) return false;
+    const float2 lg = reinterpret_cast<const float2*>(vd.grad_raster)[pix * 2];
+    if (lg.x == 0.0f && lg.y == 0.0f) return false;
+    const float sx = sxt[vd.col_base + x], sy = syt[vd.row_base + y];
+    const float q0x = p[0].x - sx * p[0].w, q0y = p[0].y - sy * p[0].w;
+    const float q1x = p[1].x - sx * p[1].w, q1y = p[1].y - sy * p[1].w;
+    const float q2x = p[2].x - sx * p[2].w, q2y = p[2].y - sy * p[2].w;
+    const float area = cross2(q1x, q1y, q2x, q2y) + cross2(q2x, q2y, q0x, q0y) + cross2(q0x, q0y, q1x, q1y);
+    if (fabsf(area) <= 1e-12f) return false;
+    const float eps = area >= 0.0f ? 1e-6f : -1e-6f;
+    const float inv = 1.0f / (area + eps);
+    const float a0 = cross2(q1x, q1y, q2x, q2y);
+    const float a1 = cross2(q2x, q2y, q0x, q0y);
+    const float b0 = a0 * inv, b1 = a1 * inv;
+    const float gb0 = lg.x * inv, gb1 = lg.y * inv;
+    const float gc = gb0 * b0 + gb1 * b1;
+    t9[0] = gc * (q2y - q1y) - gb1 * q2y;
+    t9[1] = gc * (q1x - q2x) + gb1 * q2x;
+    t9[3] = gc * (q0y - q2y) + gb0 * q2y;
+    t9[4] = gc * (q2x - q0x) - gb0 * q2x;
+    t9[6] = gc * (q1y - q0y) - gb0 * q1y + gb1 * q0y;
+    t9[7] = gc * (q0x - q1x) + gb0 * q1x - gb1 * q0x;
+    t9[2] = -sx * t9[0] - sy * t9[1];
+    t9[5] = -sx * t9[3] - sy * t9[4];
+    t9[8] = -sx * t9[6] - sy * t9[7];
+    return true;
+}
+
+struct BoxRec {
+    float4 p[3];
+    int32_t x0, y0;
+    uint32_t w, h, t;
+};
+
+// Per-triangle corner gradients summed over the reference's bounding box in
+// its row-major order.  Small boxes: the triangle's own lane loops; medium:
+// the wave tests 64 pixels at a time and adds the contributing pixels' terms
+// in ascending pixel order (ballot + readlane), which is the reference's order.
 __global__ __launch_bounds__(kBlock) void k_corner_small(ViewTable vt, const uint32_t* __restrict__ idx, uint32_t T,
                                                          uint32_t V, const float* __restrict__ sxt,
                                                          const float* __restrict__ syt, float4* __restrict__ corner,
                                                          uint2* __restrict__ queue, uint32_t* __restrict__ counters) {
+    __shared__ BoxRec recs[kBlock];
     const uint32_t view = blockIdx.y;
     const ViewDesc& vd = vt.v[view];
-    for (uint32_t t = blockIdx.x * kBlock + threadIdx.x; t < T; t += gridDim.x * kBlock) {
-        float4* out = corner + (static_cast<size_t>(view) * T + t) * 3;
-        BwdTri b;
-        float g[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-        if (setup_backward(vd, idx, t, V, b) && b.x1 > b.x0 && b.y1 > b.y0) {
-            const uint32_t area = static_cast<uint32_t>(b.x1 - b.x0) * static_cast<uint32_t>(b.y1 - b.y0);
-            if (area > kSeqPixels) {
-                queue[atomicAdd(&counters[0], 1u)] = make_uint2(view, t);
-                continue;
-            }
-            for (int32_t y = b.y0; y < b.y1; ++y)
-                for (int32_t x = b.x0; x < b.x1; ++x) corner_terms(b, vd, t, x, y, sxt, syt, g);
-        }
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    BoxRec* wrecs = recs + wave * 64;
+    float4* cview = corner + static_cast<size_t>(view) * T * 3;
+    const uint32_t stride = gridDim.x * kBlock;
+    for (uint32_t batch = (blockIdx.x * 4 + wave) * 64; batch < T; batch += stride) {
+        const uint32_t t = batch + lane;
+        BwdTri b{};
+        const bool ok = t < T && setup_backward(vd, idx, t, V, b) && b.x1 > b.x0 && b.y1 > b.y0;
+        const uint32_t w = ok ? static_cast<uint32_t>(b.x1 - b.x0) : 0u, h = ok ? static_cast<uint32_t>(b.y1 - b.y0) : 0u;
+        const uint32_t area = w * h;
+        bool deferred = false;
+        if (t < T && (!ok || area <= kInlineBox)) {
+            float g[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+            for (int32_t y = b.y0; ok && y < b.y1; ++y)
+                for (int32_t x = b.x0; x < b.x1; ++x) {
+                    float t9[9];
+                    if (pixel_terms(b.p, vd, t, x, y, sxt, syt, t9))
 #pragma unroll
-        for (int c = 0; c < 3; ++c) out[c] = make_float4(g[c][0], g[c][1], 0.0f, g[c][2]);
+                        for (int k = 0; k < 9; ++k) g[k] += t9[k];
+                }
+#pragma unroll
+            for (int c = 0; c < 3; ++c) cview[t * 3 + c] = make_float4(g[c * 3], g[c * 3 + 1], 0.0f, g[c * 3 + 2]);
+        } else if (ok && area > kWaveBox) {
+            queue[atomicAdd(&counters[0], 1u)] = make_uint2(view, t);
+        } else if (ok) {
+            deferred = true;
+            BoxRec& q = wrecs[lane];
+            q.p[0] = b.p[0];
+            q.p[1] = b.p[1];
+            q.p[2] = b.p[2];
+            q.x0 = b.x0;
+            q.y0 = b.y0;
+            q.w = w;
+            q.h = h;
+            q.t = t;
+        }
+        wave_lds_sync();
+        unsigned long long pending = __ballot(deferred);
+        while (pending) {
+            const uint32_t j = __ffsll(pending) - 1;
+            pending &= pending - 1;
+            const BoxRec& q = wrecs[j];
+            const float4 pj[3] = {q.p[0], q.p[1], q.p[2]};
+            const uint32_t qw = q.w, qa = q.w * q.h, qt = q.t;
+            float g[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+            for (uint32_t c = 0; c < qa; c += 64) {
+                const uint32_t i = c + lane;
+                float t9[9];
+                bool hit = false;
+                if (i < qa)
+                    hit = pixel_terms(pj, vd, qt, q.x0 + static_cast<int32_t>(i % qw), q.y0 + static_cast<int32_t>(i / qw),
+                                      sxt, syt, t9);
+                unsigned long long m = __ballot(hit);
+                while (m) {
+                    const int src = __ffsll(m) - 1;
+                    m &= m - 1;
+#pragma unroll
+                    for (int k = 0; k < 9; ++k)
+                        g[k] += __int_as_float(__builtin_amdgcn_readlane(__float_as_int(t9[k]), src));
+                }
+            }
+            if (lane == 0)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) cview[qt * 3 + c] = make_float4(g[c * 3], g[c * 3 + 1], 0.0f, g[c * 3 + 2]);
+        }
+        wave_lds_sync();
     }
 }
 
