@@ -195,3 +195,140 @@ def multiview_scene(nx=400, ny=250, views=16, attributes=8, seed=2, sizes=None) 
         clips.append(to_clip(pos, m))
         targets.append(rng.normal(size=(h, w, attributes)).astype(np.float32))
     return MultiViewScene(clips, idx, attrs, targets, list(sizes))
+
+
+# ---------------------------------------------------------------- atlas refinement scene
+
+
+@dataclass
+class AtlasScene:
+    positions: np.ndarray  # [V_uv, 3] f32 (geometry duplicated at chart borders)
+    uv: np.ndarray  # [V_uv, 2] f32 atlas coordinates
+    indices: np.ndarray  # [T, 3] u32
+    vertex_remap: np.ndarray  # [V_uv] u32 index of the shared geometric vertex
+    face_chart_ids: np.ndarray  # [T] u32
+    surface: np.ndarray  # [V_uv, 2] f32 continuous surface coordinates in [0,1]^2
+    chart_texels: np.ndarray  # [H, W] i32 chart id per texel (-1 in gutters)
+    atlas_size: int
+
+
+def atlas_scene(nx=1000, ny=500, charts_x=16, charts_y=16, atlas_size=2048, gutter=2, sigma=0.02, seed=3,
+                extent=(1.0, 0.5)) -> AtlasScene:
+    """Perturbed grid of nx*ny quads (2*nx*ny triangles), cut into charts_x*charts_y
+    rectangular charts; chart (i,j) maps to the interior of atlas cell (i,j) with a
+    ``gutter``-texel margin.  UV vertices are duplicated on chart borders."""
+    rng = np.random.default_rng(seed)
+    gx = np.linspace(-extent[0], extent[0], nx + 1)
+    gy = np.linspace(-extent[1], extent[1], ny + 1)
+    heights = rng.normal(0.0, sigma, size=(ny + 1, nx + 1))
+    qx = np.array_split(np.arange(nx), charts_x)
+    qy = np.array_split(np.arange(ny), charts_y)
+    cell_w, cell_h = atlas_size / charts_x, atlas_size / charts_y
+    pos, uv, surf, remap, idx, fchart = [], [], [], [], [], []
+    chart_tex = np.full((atlas_size, atlas_size), -1, np.int32)
+    base = 0
+    for cj, ys in enumerate(qy):
+        for ci, xs in enumerate(qx):
+            x0, x1, y0, y1 = xs[0], xs[-1] + 1, ys[0], ys[-1] + 1
+            vx = np.arange(x0, x1 + 1)
+            vy = np.arange(y0, y1 + 1)
+            mx, my = np.meshgrid(vx, vy)
+            n_vx, n_vy = vx.size, vy.size
+            pos.append(np.stack([gx[mx], heights[my, mx], gy[my]], -1).reshape(-1, 3))
+            surf.append(np.stack([mx / nx, my / ny], -1).reshape(-1, 2))
+            remap.append((my * (nx + 1) + mx).reshape(-1))
+            # chart interior in texels: [cx0+gutter, cx1-gutter] mapped linearly over the chart's quads
+            tx0 = ci * cell_w + gutter
+            tx1 = (ci + 1) * cell_w - gutter
+            ty0 = cj * cell_h + gutter
+            ty1 = (cj + 1) * cell_h - gutter
+            u = (tx0 + (mx - x0) / max(x1 - x0, 1) * (tx1 - tx0)) / atlas_size
+            v = (ty0 + (my - y0) / max(y1 - y0, 1) * (ty1 - ty0)) / atlas_size
+            uv.append(np.stack([u, v], -1).reshape(-1, 2))
+            a = base + (np.arange(n_vy - 1)[:, None] * n_vx + np.arange(n_vx - 1)[None, :])
+            b, c, d = a + 1, a + n_vx, a + n_vx + 1
+            tri = np.stack([np.stack([a, c, b], -1), np.stack([b, c, d], -1)], 2).reshape(-1, 3)
+            idx.append(tri)
+            fchart.append(np.full(tri.shape[0], cj * charts_x + ci))
+            chart_tex[int(np.floor(ty0)):int(np.ceil(ty1)), int(np.floor(tx0)):int(np.ceil(tx1))] = cj * charts_x + ci
+            base += n_vx * n_vy
+    return AtlasScene(np.concatenate(pos).astype(np.float32), np.concatenate(uv).astype(np.float32),
+                      np.concatenate(idx).astype(np.uint32), np.concatenate(remap).astype(np.uint32),
+                      np.concatenate(fchart).astype(np.uint32), np.concatenate(surf).astype(np.float32), chart_tex,
+                      atlas_size)
+
+
+def seam_uv_pairs(indices, uv, vertex_remap, face_chart_ids, samples_per_edge=4) -> np.ndarray:
+    """Seam sample pairs (examples/refine_colmap_texture.py:36-73): edges keyed by
+    their sorted original-vertex pair; keys used by exactly two faces of different
+    charts; ``samples_per_edge`` points at (k+0.5)/n along both sides.
+    Returns [N, 2, 2] f32 (pair order is free: losses and gradients do not depend on it)."""
+    keys, lo_uv, hi_uv, charts = [], [], [], []
+    for first, second in ((0, 1), (1, 2), (2, 0)):
+        va, vb = indices[:, first], indices[:, second]
+        oa, ob = vertex_remap[va].astype(np.uint64), vertex_remap[vb].astype(np.uint64)
+        swap = oa > ob
+        keys.append((np.minimum(oa, ob) << np.uint64(32)) | np.maximum(oa, ob))
+        lo_uv.append(np.where(swap[:, None], uv[vb], uv[va]))
+        hi_uv.append(np.where(swap[:, None], uv[va], uv[vb]))
+        charts.append(face_chart_ids)
+    key = np.concatenate(keys)
+    lo, hi, ch = np.concatenate(lo_uv), np.concatenate(hi_uv), np.concatenate(charts)
+    order = np.argsort(key, kind="stable")
+    key, lo, hi, ch = key[order], lo[order], hi[order], ch[order]
+    starts = np.r_[0, np.flatnonzero(key[1:] != key[:-1]) + 1]
+    ends = np.r_[starts[1:], key.size]
+    starts = starts[(ends - starts) == 2]
+    starts = starts[ch[starts] != ch[starts + 1]]
+    if starts.size == 0:
+        return np.empty((0, 2, 2), np.float32)
+    t = ((np.arange(samples_per_edge, dtype=np.float32) + 0.5) / samples_per_edge)[None, :, None]
+    first = lo[starts, None] * (1.0 - t) + hi[starts, None] * t
+    second = lo[starts + 1, None] * (1.0 - t) + hi[starts + 1, None] * t
+    return np.ascontiguousarray(np.stack((first, second), axis=2).reshape(-1, 2, 2), dtype=np.float32)
+
+
+def ground_truth(surface: np.ndarray, seed: int = 3) -> np.ndarray:
+    """Procedural RGB in surface coordinates: 6 seeded sinusoids + a checker, clamped to [0,1]."""
+    rng = np.random.default_rng(seed + 100)
+    s = surface.astype(np.float64)
+    out = np.full(s.shape[:-1] + (3,), 0.5)
+    for _ in range(6):
+        f = rng.uniform(2, 24, size=2)
+        ph = rng.uniform(0, 2 * math.pi)
+        amp = rng.uniform(0.03, 0.1, size=3)
+        out += amp * np.sin(2 * math.pi * (f[0] * s[..., 0:1] + f[1] * s[..., 1:2]) + ph)
+    checker = ((np.floor(s[..., 0] * 16) + np.floor(s[..., 1] * 8)) % 2)[..., None]
+    out += 0.15 * (checker - 0.5)
+    return np.clip(out, 0.0, 1.0).astype(np.float32)
+
+
+def atlas_views(scene: AtlasScene, views=64, size=(768, 1024), seed=3, fov=45.0):
+    """world_to_clip matrices of cameras above the grid (elevation 35-75 deg)."""
+    rng = np.random.default_rng(seed + 200)
+    mats = []
+    h, w = size
+    for _ in range(views):
+        elev = math.radians(rng.uniform(35, 75))
+        azim = rng.uniform(0, 2 * math.pi)
+        dist = rng.uniform(1.0, 1.8)
+        target = np.array([rng.uniform(-0.6, 0.6), 0.0, rng.uniform(-0.3, 0.3)])
+        eye = target + dist * np.array([math.cos(elev) * math.cos(azim), math.sin(elev), math.cos(elev) * math.sin(azim)])
+        mats.append(world_to_clip(eye, w / h, fov, target=target, near=0.05, far=10.0))
+    return np.stack(mats).astype(np.float32)
+
+
+def ellipse_masks(views, size, seed=3, count=3):
+    """Seeded binary visibility masks: 1 except inside a few random ellipses."""
+    rng = np.random.default_rng(seed + 300)
+    h, w = size
+    yy, xx = np.mgrid[0:h, 0:w]
+    masks = []
+    for _ in range(views):
+        m = np.ones((h, w), np.float32)
+        for _ in range(count):
+            cx, cy = rng.uniform(0, w), rng.uniform(0, h)
+            ax, ay = rng.uniform(0.04, 0.15) * w, rng.uniform(0.04, 0.15) * h
+            m[((xx - cx) / ax) ** 2 + ((yy - cy) / ay) ** 2 < 1] = 0.0
+        masks.append(m)
+    return masks
