@@ -82,6 +82,89 @@ def cpu_baseline(scene, views, budget_s=20.0):
     }
 
 
+def refine_benchmark(steps=1000, checkpoint_every=50, cpu=True):
+    """Configuration 3: 2048^2 RGB atlas, 1M-triangle 256-chart grid, 64 views 1024x768
+    with noise and masks, 1000 graph-captured Adam steps (lr 1e-2 cosine to 2.5e-4,
+    chart TV 1e-3, joint seam 1e-2, then the reference's 30-step seam polish)."""
+    import torch
+
+    from photara_drender_amd import _native, workloads
+    from photara_drender_amd.refine import refine_atlas
+
+    ctx = _native.Context(0, flags=0)
+    t0 = time.perf_counter()
+    prob = workloads.refine_problem(ctx)
+    build_s = time.perf_counter() - t0
+    s = prob.atlas_size
+    init = torch.full((s, s, 3), 0.5, device=prob.positions.device)
+    kw = dict(steps=steps, batch_size=4, learning_rate=1e-2, minimum_learning_rate=2.5e-4, seam_polish_steps=30,
+              tv_weight=1e-3, chart_ids=prob.chart_texels, joint_seam_weight=1e-2, use_graph=True)
+    args = (ctx, init, prob.positions, prob.uv, prob.indices, prob.images, prob.world_to_clip, prob.masks, None,
+            prob.seam_pairs)
+    refine_atlas(*args, **dict(kw, steps=20, seam_polish_steps=2))  # warm-up (module load, scratch)
+    timed = refine_atlas(*args, **kw)
+    ck = refine_atlas(*args, **dict(kw, checkpoint_every=checkpoint_every))
+    psnr = [workloads.atlas_psnr(ctx, prob, ck.checkpoints[i]) for i in range(ck.checkpoints.shape[0])]
+    final = workloads.atlas_psnr(ctx, prob, timed.color)
+    final_noisy = workloads.atlas_psnr(ctx, prob, timed.color, against="noisy")
+    initial = workloads.atlas_psnr(ctx, prob, init)
+    target = psnr[-1] - 0.1
+    steps_needed = next((checkpoint_every * (i + 1) for i, p in enumerate(psnr) if p >= target), steps)
+    per_step = timed.optimization_seconds / (steps + 30)
+    out = {
+        "workload": "2048^2 RGB atlas, 1,000,000-tri grid in 256 charts (2-texel gutter), 64 views 1024x768, "
+                    "noise 0.02 + ellipse masks, 90,000 seam pairs; init 0.5 grey",
+        "steps": steps, "seconds_to_refine": timed.precompute_seconds + timed.optimization_seconds,
+        "precompute_s": timed.precompute_seconds, "optimize_s": timed.optimization_seconds,
+        "ms_per_step": per_step * 1e3, "psnr_initial_db": initial, "psnr_final_db": final,
+        "psnr_final_vs_noisy_db": final_noisy, "psnr_checkpoints_db": [round(p, 3) for p in psnr],
+        "steps_to_final_minus_0.1db": steps_needed,
+        "seconds_to_final_minus_0.1db": timed.precompute_seconds + steps_needed * per_step,
+        "problem_build_s": build_s,
+    }
+    if cpu:
+        out["cpu_baseline"] = refine_cpu_estimate(prob, steps)
+    return out
+
+
+def refine_cpu_estimate(prob, steps, views=2, sample_steps=4):
+    """Oracle refine on the host: precompute of ``views`` views and ``sample_steps``
+    steps (batch 4) timed, extrapolated to 64 views and steps + 30 seam steps."""
+    import oracle
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(os.cpu_count() or 1, 16)
+    oracle.set_num_threads(threads)
+    pos = prob.positions.cpu().numpy()
+    uv = prob.uv.cpu().numpy()
+    idx = prob.indices.cpu().numpy().view(np.uint32)
+    cached = []
+    t0 = time.perf_counter()
+    for k in range(views):
+        clip = oracle.transform_positions(pos, prob.world_to_clip[k])
+        img = prob.images[k].cpu().numpy()
+        raster, _, _ = oracle.rasterize_forward(clip, idx, img.shape[:2], want_deriv=False)
+        iuv = oracle.interpolate_forward(uv, idx, raster)
+        target, mask, wsum = oracle.refine_view_inputs(img, raster, prob.masks[k].cpu().numpy())
+        cached.append((iuv.reshape(-1, 2), target, mask, wsum))
+    pre = (time.perf_counter() - t0) / views
+    s = prob.atlas_size
+    tex = np.full((s, s, 3), 0.5, np.float32)
+    grad, m, v = np.zeros_like(tex), np.zeros_like(tex), np.zeros_like(tex)
+    t0 = time.perf_counter()
+    for step in range(sample_steps):
+        loss = 0.0
+        for b in range(4):
+            iuv, target, mask, wsum = cached[(step * 4 + b) % views]
+            _, loss = oracle.photometric_gradient(tex, iuv, target, mask, 1e-3, 1.0 / (3 * 4 * wsum), grad, loss)
+        oracle.adam_update(tex, grad, m, v, 1e-2, 0.9, 0.999, 1e-8, 1 - 0.9 ** (step + 1), 1 - 0.999 ** (step + 1),
+                           0.0, 1.0)
+    per_step = (time.perf_counter() - t0) / sample_steps
+    return {"value": pre * len(prob.images) + per_step * (steps + 30), "unit": "s", "cores": oracle.num_threads(),
+            "kind": "port", "sample": f"precompute of {views} views and {sample_steps} optimisation steps "
+                                      "(photometric gradient + Adam) timed, extrapolated to 64 views and "
+                                      f"{steps}+30 steps; extension terms (TV, joint seam) not included"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -89,6 +172,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-views", type=int, default=16)
+    ap.add_argument("--no-refine", action="store_true", help="skip the atlas-refinement workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU layout "
@@ -197,6 +281,8 @@ def main():
     }
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(scene, args.cpu_views)
+    if not args.no_refine:
+        result["refine"] = refine_benchmark(cpu=not args.no_cpu_baseline)
     print(json.dumps(result))
 
 

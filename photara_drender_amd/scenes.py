@@ -288,15 +288,28 @@ def seam_uv_pairs(indices, uv, vertex_remap, face_chart_ids, samples_per_edge=4)
     return np.ascontiguousarray(np.stack((first, second), axis=2).reshape(-1, 2, 2), dtype=np.float32)
 
 
-def ground_truth(surface: np.ndarray, seed: int = 3) -> np.ndarray:
-    """Procedural RGB in surface coordinates: 6 seeded sinusoids + a checker, clamped to [0,1]."""
+def ground_truth_params(seed: int = 3):
     rng = np.random.default_rng(seed + 100)
+    return [(rng.uniform(2, 24, size=2), rng.uniform(0, 2 * math.pi), rng.uniform(0.03, 0.1, size=3))
+            for _ in range(6)]
+
+
+def ground_truth(surface, seed: int = 3):
+    """Procedural RGB in surface coordinates: 6 seeded sinusoids + a checker, clamped to [0,1].
+    ``surface`` [..., 2] is a numpy array (float32 result) or a torch tensor (same device)."""
+    if hasattr(surface, "data_ptr"):
+        import torch
+
+        s = surface.to(torch.float64)
+        out = torch.full(s.shape[:-1] + (3,), 0.5, dtype=torch.float64, device=s.device)
+        for f, ph, amp in ground_truth_params(seed):
+            wave = torch.sin(2 * math.pi * (float(f[0]) * s[..., 0:1] + float(f[1]) * s[..., 1:2]) + ph)
+            out += torch.tensor(amp, dtype=torch.float64, device=s.device) * wave
+        checker = torch.remainder(torch.floor(s[..., 0] * 16) + torch.floor(s[..., 1] * 8), 2)[..., None]
+        return torch.clamp(out + 0.15 * (checker - 0.5), 0.0, 1.0).to(torch.float32)
     s = surface.astype(np.float64)
     out = np.full(s.shape[:-1] + (3,), 0.5)
-    for _ in range(6):
-        f = rng.uniform(2, 24, size=2)
-        ph = rng.uniform(0, 2 * math.pi)
-        amp = rng.uniform(0.03, 0.1, size=3)
+    for f, ph, amp in ground_truth_params(seed):
         out += amp * np.sin(2 * math.pi * (f[0] * s[..., 0:1] + f[1] * s[..., 1:2]) + ph)
     checker = ((np.floor(s[..., 0] * 16) + np.floor(s[..., 1] * 8)) % 2)[..., None]
     out += 0.15 * (checker - 0.5)
