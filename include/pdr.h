@@ -8,8 +8,9 @@
  * buffer, like the reference's synchronous Vulkan submits) or device memory
  * (the call enqueues its kernels on `stream` and returns; results are valid
  * once the stream reaches them).  Mixing host and device arrays in one call is
- * PDR_INVALID_ARGUMENT.  `stream` is a hipStream_t; NULL selects the context's
- * own stream.  Layouts are the reference's: row-major, float32 / uint32,
+ * PDR_INVALID_ARGUMENT.  `stream` is a hipStream_t (NULL = the HIP default
+ * stream, as everywhere in HIP); host-memory calls run on the context's own
+ * stream.  Layouts are the reference's: row-major, float32 / uint32,
  * images [height, width, channels], lower-left origin.
  *
  * Errors: a non-zero pdr_status; pdr_last_error() returns the message of the
@@ -85,7 +86,7 @@ pdr_status pdr_enumerate_devices(pdr_device_info* infos, uint32_t capacity, uint
 pdr_status pdr_context_set_timing(pdr_context ctx, int enable);
 pdr_status pdr_context_kernel_time(pdr_context ctx, const char* name, double* total_ms, uint64_t* launches);
 
-/* Blocks until all work the context enqueued on `stream` has finished. */
+/* Blocks until `stream` (NULL = default stream) and the context's own stream are idle. */
 pdr_status pdr_synchronize(pdr_context ctx, void* stream);
 
 /* Rasterizer::forward — include/photara_drender/rasterizer.hpp:22-25,

@@ -110,7 +110,8 @@ pdr_status pdr_context_kernel_time(pdr_context ctx, const char* name, double* to
 pdr_status pdr_synchronize(pdr_context ctx, void* stream) {
     return guarded([&] {
         require_ctx(ctx);
-        sync(stream ? static_cast<hipStream_t>(stream) : ctx->stream);
+        sync(static_cast<hipStream_t>(stream));
+        sync(ctx->stream);
     });
 }
 
@@ -132,7 +133,7 @@ pdr_status pdr_rasterize_forward(pdr_context ctx, const float* pos, uint32_t V, 
         resolve_viewport(W, H, viewport4, vp);
         if (viewport_out4) std::memcpy(viewport_out4, vp, sizeof(vp));
         Lock lock(ctx);
-        const hipStream_t st = ctx->use_stream(device ? stream : nullptr);
+        const hipStream_t st = ctx->use_stream(device, stream);
         const size_t P = static_cast<size_t>(W) * H;
         const float* dpos = pos;
         const uint32_t* didx = idx;
@@ -180,7 +181,7 @@ pdr_status pdr_rasterize_forward_batched(pdr_context ctx, const pdr_raster_view*
                 raise(PDR_INVALID_ARGUMENT, "batched rasterization requires device memory");
         }
         Lock lock(ctx);
-        const hipStream_t st = ctx->use_stream(stream);
+        const hipStream_t st = ctx->use_stream(true, stream);
         check_indices_device(ctx, idx, static_cast<size_t>(T) * 3, V, st, PDR_OUT_OF_RANGE,
                              "triangle_indices contains an invalid vertex index");
         for (uint32_t base = 0; base < nviews; base += pdr::kMaxViews) {
@@ -216,7 +217,7 @@ pdr_status pdr_rasterize_backward(pdr_context ctx, const float* pos, uint32_t V,
         float vp[4];
         resolve_viewport(W, H, viewport4, vp);
         Lock lock(ctx);
-        const hipStream_t st = ctx->use_stream(device ? stream : nullptr);
+        const hipStream_t st = ctx->use_stream(device, stream);
         const size_t P = static_cast<size_t>(W) * H;
         const float *dpos = pos, *draster = raster, *dgrad = grad_raster;
         const uint32_t* didx = idx;
@@ -263,7 +264,7 @@ pdr_status pdr_rasterize_backward_batched(pdr_context ctx, const pdr_raster_view
                 raise(PDR_INVALID_ARGUMENT, "batched rasterization requires device memory");
         }
         Lock lock(ctx);
-        const hipStream_t st = ctx->use_stream(stream);
+        const hipStream_t st = ctx->use_stream(true, stream);
         check_indices_device(ctx, idx, static_cast<size_t>(T) * 3, V, st, PDR_OUT_OF_RANGE,
                              "triangle_indices contains an invalid vertex index");
         for (uint32_t base = 0; base < nviews; base += pdr::kMaxViews) {
@@ -302,7 +303,7 @@ pdr_status pdr_interpolate_forward(pdr_context ctx, const float* attr, uint32_t 
         if (P64 > 0xFFFFFFFFull) raise(PDR_OVERFLOW, "Interpolation pixel count exceeds backend limits");
         require_ptr(out, "out");
         Lock lock(ctx);
-        const hipStream_t st = ctx->use_stream(device ? stream : nullptr);
+        const hipStream_t st = ctx->use_stream(device, stream);
         const size_t P = static_cast<size_t>(P64);
         const float *dattr = attr, *draster = raster;
         const uint32_t* didx = idx;
@@ -346,7 +347,7 @@ pdr_status pdr_interpolate_backward(pdr_context ctx, const float* attr, uint32_t
         require_ptr(grad_attr, "grad_attr");
         require_ptr(grad_raster, "grad_raster");
         Lock lock(ctx);
-        const hipStream_t st = ctx->use_stream(device ? stream : nullptr);
+        const hipStream_t st = ctx->use_stream(device, stream);
         const size_t P = static_cast<size_t>(P64);
         const float *dattr = attr, *draster = raster, *dgrad = grad;
         const uint32_t* didx = idx;
@@ -417,7 +418,7 @@ pdr_status pdr_interpolate_forward_batched(pdr_context ctx, const pdr_interp_vie
                 raise(PDR_INVALID_ARGUMENT, "batched interpolation requires device memory");
         }
         Lock lock(ctx);
-        const hipStream_t st = ctx->use_stream(stream);
+        const hipStream_t st = ctx->use_stream(true, stream);
         check_indices_device(ctx, idx, static_cast<size_t>(T) * 3, V, st, PDR_OUT_OF_RANGE,
                              "triangle_indices contains an invalid attribute vertex index");
         for (uint32_t base = 0; base < nviews; base += pdr::kMaxViews) {
@@ -447,7 +448,7 @@ pdr_status pdr_interpolate_backward_batched(pdr_context ctx, const pdr_interp_vi
                 raise(PDR_INVALID_ARGUMENT, "batched interpolation requires device memory");
         }
         Lock lock(ctx);
-        const hipStream_t st = ctx->use_stream(stream);
+        const hipStream_t st = ctx->use_stream(true, stream);
         check_indices_device(ctx, idx, static_cast<size_t>(T) * 3, V, st, PDR_INVALID_ARGUMENT,
                              "triangle_indices is incompatible with vertex_attributes");
         if (nviews > pdr::kMaxViews)
@@ -475,7 +476,7 @@ pdr_status pdr_texture_forward(pdr_context ctx, const float* tex, uint32_t tw, u
         require_ptr(out, "out");
         const bool device = classify({tex, uv, raster, out});
         Lock lock(ctx);
-        const hipStream_t st = ctx->use_stream(device ? stream : nullptr);
+        const hipStream_t st = ctx->use_stream(device, stream);
         const float *dtex = tex, *duv = uv, *draster = raster;
         float* dout = out;
         if (!device) {
@@ -515,7 +516,7 @@ pdr_status pdr_texture_backward(pdr_context ctx, const float* tex, uint32_t tw, 
         if (compute_uv) require_ptr(grad_uv, "grad_uv");
         const bool device = classify({tex, uv, raster, grad, grad_tex, grad_uv});
         Lock lock(ctx);
-        const hipStream_t st = ctx->use_stream(device ? stream : nullptr);
+        const hipStream_t st = ctx->use_stream(device, stream);
         const float *dtex = tex, *duv = uv, *draster = raster, *dgrad = grad;
         float *dgt = grad_tex, *dgu = grad_uv;
         if (!device) {

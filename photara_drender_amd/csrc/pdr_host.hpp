@@ -82,6 +82,7 @@ struct pdr_context_s {
     uint32_t flags = PDR_FLAG_VALIDATE_DEVICE_INPUTS;
     hipStream_t stream = nullptr;
     hipStream_t last_stream = nullptr;
+    bool have_last = false;
     hipEvent_t handoff = nullptr;
     std::mutex mutex;
     uint32_t* pinned = nullptr;  // small host-visible readback area
@@ -141,14 +142,18 @@ struct pdr_context_s {
         pending.clear();
     }
 
-    // Makes `st` the stream that owns the scratch; orders it after the previous owner.
-    hipStream_t use_stream(void* user) {
-        hipStream_t st = user ? static_cast<hipStream_t>(user) : stream;
-        if (last_stream && last_stream != st) {
+    // Stream of a call: device-memory calls run on the caller's stream (NULL is
+    // the HIP default stream); host-memory calls stage through the context's
+    // own stream and synchronise it.  The chosen stream is ordered after the
+    // previous owner of the context's scratch.
+    hipStream_t use_stream(bool device, void* user) {
+        hipStream_t st = device ? static_cast<hipStream_t>(user) : stream;
+        if (have_last && last_stream != st) {
             pdrh::hip_check(hipEventRecord(handoff, last_stream), "hipEventRecord");
             pdrh::hip_check(hipStreamWaitEvent(st, handoff, 0), "hipStreamWaitEvent");
         }
         last_stream = st;
+        have_last = true;
         return st;
     }
 

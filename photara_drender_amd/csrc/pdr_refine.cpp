@@ -119,14 +119,15 @@ pdr_status pdr_refine(pdr_context ctx, const float* initial_texture, const float
         const float tv_eps = o.tv_epsilon > 0.0f ? o.tv_epsilon : 1e-4f;
 
         Lock lock(ctx);
-        const hipStream_t st = ctx->use_stream(nullptr);
+        const hipStream_t st = ctx->use_stream(false, nullptr);
         DeviceBuffers bufs;
         if (idx_device)
             check_indices_device(ctx, idx, static_cast<size_t>(T) * 3, V, st, PDR_INVALID_ARGUMENT,
                                  "texture refinement mesh arrays have inconsistent shapes");
 
         // ---------------------------------------------------------- precompute
-        sync(st);
+        // device inputs may still be in flight on other streams of the caller
+        hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
         const auto t0 = std::chrono::steady_clock::now();
         const float* dpos = device_view(bufs, positions, static_cast<size_t>(V) * 3, st);
         const float* duv = device_view(bufs, uv, static_cast<size_t>(V) * 2, st);
