@@ -84,6 +84,11 @@ _SIGNATURES = {
 
 EXPORTED_SYMBOLS = tuple(_SIGNATURES)
 
+# Host-only self-test hooks exported by libpdr outside the public header.
+_TESTING_SIGNATURES = {
+    "pdr_testing_candidate_region": (_I, [_P, _U32, _U32, _P, _U32, _I, _P, _P]),
+}
+
 _lib = None
 
 
@@ -98,7 +103,7 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise OSError(f"libpdr.so not found at {LIB_PATH}; run __graft_entry__.build() first")
         handle = ctypes.CDLL(LIB_PATH)
-        for name, (res, args) in _SIGNATURES.items():
+        for name, (res, args) in {**_SIGNATURES, **_TESTING_SIGNATURES}.items():
             fn = getattr(handle, name)
             fn.restype = res
             fn.argtypes = args

@@ -16,14 +16,14 @@ constexpr uint32_t kTile = 16;         // src/rasterizer.cpp:19 RASTER_TILE_SIZE
 constexpr uint32_t kTilePixels = kTile * kTile;
 constexpr uint32_t kMaxViews = 32;     // views per batched launch (kernel-argument table)
 
-__device__ __forceinline__ int32_t f2i_sat(float f) {
+__host__ __device__ __forceinline__ int32_t f2i_sat(float f) {
     if (f != f) return 0;
     if (f >= 2147483648.0f) return 2147483647;
     if (f <= -2147483648.0f) return (-2147483647 - 1);
     return static_cast<int32_t>(f);
 }
 
-__device__ __forceinline__ int32_t d2i_sat(double f) {
+__host__ __device__ __forceinline__ int32_t d2i_sat(double f) {
     if (f != f) return 0;
     if (f >= 2147483648.0) return 2147483647;
     if (f <= -2147483648.0) return (-2147483647 - 1);
@@ -31,23 +31,23 @@ __device__ __forceinline__ int32_t d2i_sat(double f) {
 }
 
 // (uint)round(id) with HLSL round = half-to-even; negative/NaN ids decode as 0.
-__device__ __forceinline__ uint32_t decode_id(float f) {
+__host__ __device__ __forceinline__ uint32_t decode_id(float f) {
     const float r = __builtin_rintf(f);
     if (!(r > 0.0f)) return 0u;
     if (r >= 4294967296.0f) return 0xFFFFFFFFu;
     return static_cast<uint32_t>(r);
 }
 
-__device__ __forceinline__ float cross2(float ax, float ay, float bx, float by) { return ax * by - ay * bx; }
+__host__ __device__ __forceinline__ float cross2(float ax, float ay, float bx, float by) { return ax * by - ay * bx; }
 
-__device__ __forceinline__ float maxf_std(float a, float b) { return (a < b) ? b : a; }
-__device__ __forceinline__ float minf_std(float a, float b) { return (b < a) ? b : a; }
-__device__ __forceinline__ float clampf(float x, float lo, float hi) { return minf_std(maxf_std(x, lo), hi); }
+__host__ __device__ __forceinline__ float maxf_std(float a, float b) { return (a < b) ? b : a; }
+__host__ __device__ __forceinline__ float minf_std(float a, float b) { return (b < a) ? b : a; }
+__host__ __device__ __forceinline__ float clampf(float x, float lo, float hi) { return minf_std(maxf_std(x, lo), hi); }
 
-__device__ __forceinline__ int32_t add1(int32_t c) { return static_cast<int32_t>(static_cast<uint32_t>(c) + 1u); }
+__host__ __device__ __forceinline__ int32_t add1(int32_t c) { return static_cast<int32_t>(static_cast<uint32_t>(c) + 1u); }
 
 // texture_forward.hlsl:16-35 address_coordinate
-__device__ __forceinline__ int32_t address_coordinate(int32_t c, uint32_t size, uint32_t mode) {
+__host__ __device__ __forceinline__ int32_t address_coordinate(int32_t c, uint32_t size, uint32_t mode) {
     const int32_t n = static_cast<int32_t>(size);
     if (mode == 0) return c < 0 ? 0 : (c > n - 1 ? n - 1 : c);
     if (mode == 1) {

@@ -322,7 +322,7 @@ inline pdr::RasterScratch raster_scratch(pdr_context ctx, const pdr::ViewTable& 
     pdr::RasterScratch s;
     s.sx = ctx->get_t<float>("sample_x", table_cols(vt));
     s.sy = ctx->get_t<float>("sample_y", table_rows(vt));
-    s.queue = ctx->get_t<uint2>("large_queue", static_cast<size_t>(vt.n) * T);
+    s.queue = ctx->get_t<uint4>("large_queue", static_cast<size_t>(vt.n) * T);
     s.counters = ctx->get_t<uint32_t>("large_counters", 4);
     if (with_keys) {
         const size_t before = ctx->scratch["raster_keys"].bytes;
@@ -360,9 +360,14 @@ inline void run_raster_backward(pdr_context ctx, const pdr::ViewTable& vt, const
     csr.temp_bytes = pdr::csr_temp_bytes(T, V);
     csr.temp = ctx->get("csr_temp", csr.temp_bytes);
     float4* corner = ctx->get_t<float4>("corner_grads", static_cast<size_t>(vt.n) * T * 3);
+    pdr::RasterScratch sb = s;
+    sb.cid = ctx->get_t<uint32_t>("backward_ids", vt.total_pixels);
+    sb.view_flags = ctx->get_t<uint32_t>("backward_view_flags", pdr::kMaxViews);
     ctx->timed("vertex_csr", st, [&] { hip_check(pdr::build_vertex_csr(idx, T, V, csr, st), "build_vertex_csr"); });
+    ctx->timed("bwd_prepass", st,
+               [&] { hip_check(pdr::backward_prepass(vt, idx, T, V, sb, st), "backward_prepass"); });
     ctx->timed("corner_grads", st,
-               [&] { hip_check(pdr::corner_gradients(vt, idx, T, V, s, corner, st), "corner_gradients"); });
+               [&] { hip_check(pdr::corner_gradients(vt, idx, T, V, sb, corner, st), "corner_gradients"); });
     ctx->timed("vertex_reduce", st,
                [&] { hip_check(pdr::vertex_reduce(vt, T, V, csr, corner, st), "vertex_reduce"); });
 }

@@ -57,8 +57,10 @@ struct RasterScratch {
     unsigned long long* keys = nullptr;  // [total_pixels] (depth, id) keys; all ~0 between calls
     float* sx = nullptr;                 // [sum of widths] pixel-centre NDC x per column
     float* sy = nullptr;                 // [sum of heights] pixel-centre NDC y per row
-    uint2* queue = nullptr;              // [views * T] (view, triangle) with large rectangles
+    uint4* queue = nullptr;              // [views * T] (view, triangle, slabs, 0) of large triangles
     uint32_t* counters = nullptr;        // [4] queue length, next queue entry
+    uint32_t* cid = nullptr;             // [total_pixels] backward: compared id per pixel
+    uint32_t* view_flags = nullptr;      // [kMaxViews] backward: view walked over whole boxes
 };
 
 // Vertex -> corner adjacency (corners ascending per vertex).
@@ -83,7 +85,10 @@ hipError_t raster_resolve(const ViewTable& views, const uint32_t* idx, int want_
                           hipStream_t st);
 size_t csr_temp_bytes(uint32_t T, uint32_t V);
 hipError_t build_vertex_csr(const uint32_t* idx, uint32_t T, uint32_t V, const VertexCsr& csr, hipStream_t st);
-// corner: [views * 3T] float4 per-triangle corner gradients
+// k_bwd_prepass: per-pixel compared ids and per-view whole-box flags (raster, grad_raster of each view)
+hipError_t backward_prepass(const ViewTable& views, const uint32_t* idx, uint32_t T, uint32_t V,
+                            const RasterScratch& s, hipStream_t st);
+// corner: [views * 3T] float4 per-triangle corner gradients (after backward_prepass)
 hipError_t corner_gradients(const ViewTable& views, const uint32_t* idx, uint32_t T, uint32_t V,
                             const RasterScratch& s, float4* corner, hipStream_t st);
 hipError_t vertex_reduce(const ViewTable& views, uint32_t T, uint32_t V, const VertexCsr& csr, const float4* corner,

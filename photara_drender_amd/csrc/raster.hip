@@ -4,17 +4,19 @@
 // of src/rasterizer.cpp:61-138):
 //   k_sample_tables   pixel-centre NDC of every column / row of every view
 //                     (rasterize_forward.hlsl:55-57, computed once per call);
-//   k_raster_small    one lane per (view, triangle): the triangle's candidate
-//                     pixels are those of its 16x16 bins (the reference's pixel
-//                     bounds, tile-expanded) intersected with a conservative
-//                     bound of every pixel that can pass the reference's
-//                     epsilon-inclusive edge test.  Each candidate runs the
-//                     reference's coverage / depth expressions and, if
-//                     covered, does a 64-bit atomicMin of (depth, id) into the
-//                     pixel's key.  Triangles with large candidate rectangles
-//                     are queued;
-//   k_raster_large    persistent workgroups walk the queued rectangles, one
-//                     pixel per lane;
+//   k_raster_small    one lane per (view, triangle) sets the triangle up
+//                     (raster_geom.hpp): its candidate pixels are those of its
+//                     16x16 bins (the reference's pixel bounds, tile-expanded)
+//                     that lie in conservative per-row spans of every pixel
+//                     able to pass the reference's epsilon-inclusive edge
+//                     test.  Each candidate runs the reference's coverage /
+//                     depth expressions and, if covered, does a 64-bit
+//                     atomicMin of (depth, id) into the pixel's key.  Small
+//                     triangles are walked by their own lane, medium ones by
+//                     the wave (rows' spans concatenated, 64 pixels per step),
+//                     large ones are queued;
+//   k_raster_large    the queued triangles' 16-row slabs, dealt round-robin to
+//                     every wave of a persistent grid;
 //   k_raster_resolve  one lane per pixel: decodes the winning triangle,
 //                     recomputes its barycentrics / depth / derivatives with
 //                     the reference's expressions, writes raster and
@@ -25,10 +27,18 @@
 // are canonicalised to +0.0 in the key (they compare equal in the reference).
 //
 // Backward (shaders/rasterize_backward.hlsl:36-158 + reduce_vertex_gradients.hlsl):
-//   k_corner_small    one lane per (view, triangle) loops over the reference's
-//                     screen bounding box in the reference's row-major order
-//                     and sums the corner gradients in registers (bit-exact
-//                     per-triangle sums); larger boxes are queued;
+//   k_bwd_prepass     per pixel: the id the backward compares (0 when the
+//                     pixel's grad_raster.xy is zero) and a per-view flag when
+//                     a pixel names a triangle whose candidate region does not
+//                     hold it (only possible for a raster not produced by the
+//                     forward; such views are walked over whole boxes);
+//   k_corner_small    per (view, triangle): the corner gradients summed over
+//                     the reference's screen bounding box (restricted to the
+//                     candidate region, outside which no pixel names the
+//                     triangle) in the reference's row-major order, by the
+//                     triangle's lane or, for larger regions, by the wave with
+//                     an ordered ballot/readlane fold (bit-exact per-triangle
+//                     sums); boxes over kWaveBox pixels are queued;
 //   k_corner_large    queued boxes, one workgroup each, fixed-order reduction;
 //   k_vertex_reduce   per vertex, corner gradients summed in ascending corner
 //                     order over a vertex -> corner adjacency built on the GPU
@@ -37,15 +47,16 @@
 #include <hipcub/hipcub.hpp>
 
 #include "pdr_internal.hpp"
+#include "raster_geom.hpp"
 
 namespace pdr {
 namespace {
 
 constexpr uint32_t kBlock = 256;
-constexpr uint32_t kInlinePixels = 4;    // candidate rectangle tested by its own lane
-constexpr uint32_t kWavePixels = 16384;  // candidate rectangle tested by one wave
-constexpr uint32_t kInlineBox = 8;       // backward box summed by its own lane
-constexpr uint32_t kWaveBox = 65536;     // backward box summed by one wave (in pixel order)
+constexpr float kWaveWork = 16384.0f;    // forward: candidate estimate walked by one wave (else queued)
+constexpr uint32_t kSlabRows = 16;       // forward: rows per queued work item
+constexpr float kFlatWork = 512.0f;      // backward: region estimate walked in the wave's concatenated row list
+constexpr uint32_t kWaveBox = 65536;     // backward: box pixels summed by one wave in pixel order (else queued)
 constexpr uint32_t kLargeBlocks = 1024;  // persistent workgroups for queued work
 
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n) {
@@ -90,141 +101,15 @@ __global__ void k_sample_tables(ViewTable vt, float* __restrict__ sx, float* __r
     }
 }
 
-struct Tri {
-    float4 p0, p1, p2;
-    float s0x, s0y, s1x, s1y, s2x, s2y, sgn;
-    int32_t x0, x1, y0, y1;  // inclusive candidate rectangle
-};
-
-// Pixel range whose centres can lie in { x : sgn * E_i(x) >= -eps for all i },
-// E_i the exact edge functions of the screen triangle, eps the reference's
-// 1e-7 plus a bound on the float evaluation error of the edge test.  Returns
-// false when the region is not usefully bounded (near-degenerate triangles);
-// the caller then keeps the whole bin rectangle.
-__device__ bool halo_rect(const Tri& r, const ViewDesc& vd, int32_t cx0, int32_t cx1, int32_t cy0, int32_t cy1,
-                          int32_t* hx0, int32_t* hx1, int32_t* hy0, int32_t* hy1) {
-    const double S[3][2] = {{r.s0x, r.s0y}, {r.s1x, r.s1y}, {r.s2x, r.s2y}};
-    const double vx = vd.vx, vy = vd.vy, vw = vd.vw, vh = vd.vh;
-    auto ndc_x = [&](double px) { return ((px - vx) + 0.5) * 2.0 / vw - 1.0; };
-    auto ndc_y = [&](double py) { return ((py - vy) + 0.5) * 2.0 / vh - 1.0; };
-    const double nx0 = ndc_x(cx0), nx1 = ndc_x(cx1), ny0 = ndc_y(cy0), ny1 = ndc_y(cy1);
-    double M = 0.0, cmax = 0.0;
-    for (int j = 0; j < 3; ++j) {
-        M = fmax(M, fmax(fmax(fabs(S[j][0] - nx0), fabs(S[j][0] - nx1)), fmax(fabs(S[j][1] - ny0), fabs(S[j][1] - ny1))));
-        cmax = fmax(cmax, fmax(fabs(S[j][0]), fabs(S[j][1])));
-    }
-    // float edge test error <= ~4u (|ax||by| + |ay||bx|) <= 8u M^2; take 32u M^2
-    const double eps = 1.0000001e-7 + 32.0 * 5.9604644775390625e-8 * M * M + 1e-15 * cmax * cmax;
-    double a[3], b[3], c[3];
-    const int jk[3][2] = {{1, 2}, {2, 0}, {0, 1}};
-    for (int i = 0; i < 3; ++i) {
-        const double* sj = S[jk[i][0]];
-        const double* sk = S[jk[i][1]];
-        // E_i(x, y) = (sj.x sk.y - sj.y sk.x) + x (sj.y - sk.y) + y (sk.x - sj.x)
-        a[i] = r.sgn * (sj[1] - sk[1]);
-        b[i] = r.sgn * (sk[0] - sj[0]);
-        c[i] = -r.sgn * (sj[0] * sk[1] - sj[1] * sk[0]) - eps;  // region: a x + b y >= c
-    }
-    double xmin = 1e300, xmax = -1e300, ymin = 1e300, ymax = -1e300;
-    for (int i = 0; i < 3; ++i) {
-        const int j = (i + 1) % 3;
-        const double det = a[i] * b[j] - a[j] * b[i];
-        const double scale = (fabs(a[i]) + fabs(b[i])) * (fabs(a[j]) + fabs(b[j]));
-        if (!(fabs(det) > 1e-9 * scale)) return false;
-        const double x = (c[i] * b[j] - c[j] * b[i]) / det;
-        const double y = (a[i] * c[j] - a[j] * c[i]) / det;
-        if (!__builtin_isfinite(x) || !__builtin_isfinite(y)) return false;
-        xmin = fmin(xmin, x);
-        xmax = fmax(xmax, x);
-        ymin = fmin(ymin, y);
-        ymax = fmax(ymax, y);
-    }
-    // pixel-centre rounding of the float sample position + double rounding above
-    const double pad = 1e-5 * (1.0 + fmax(fabs(xmin), fmax(fabs(xmax), fmax(fabs(ymin), fabs(ymax)))));
-    auto to_px = [](double ndc, double v0, double vsize) { return (ndc + 1.0) * vsize * 0.5 + v0 - 0.5; };
-    const double fx0 = ceil(to_px(xmin - pad, vx, vw) - 1e-6), fx1 = floor(to_px(xmax + pad, vx, vw) + 1e-6);
-    const double fy0 = ceil(to_px(ymin - pad, vy, vh) - 1e-6), fy1 = floor(to_px(ymax + pad, vy, vh) + 1e-6);
-    auto cl = [](double v) { return static_cast<int32_t>(v < -2.0e9 ? -2.0e9 : (v > 2.0e9 ? 2.0e9 : v)); };
-    *hx0 = cl(fx0);
-    *hx1 = cl(fx1);
-    *hy0 = cl(fy0);
-    *hy1 = cl(fy1);
-    return true;
-}
-
-// Binning validity (src/rasterizer.cpp:73-100), shader validity
-// (rasterize_forward.hlsl:75-101) and the candidate rectangle.
-__device__ bool setup_forward(const ViewDesc& vd, const uint32_t* __restrict__ idx, uint32_t t, uint32_t V,
-                              uint32_t cull, Tri& r) {
-    const uint32_t i0 = idx[t * 3 + 0], i1 = idx[t * 3 + 1], i2 = idx[t * 3 + 2];
-    if (i0 >= V || i1 >= V || i2 >= V) return false;
-    const float4* P = reinterpret_cast<const float4*>(vd.pos);
-    r.p0 = P[i0];
-    r.p1 = P[i1];
-    r.p2 = P[i2];
-    const float4* ps[3] = {&r.p0, &r.p1, &r.p2};
-    float s[3][2];
-    float min_x = __builtin_inff(), min_y = __builtin_inff();
-    float max_x = -__builtin_inff(), max_y = -__builtin_inff();
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        const float w = ps[c]->w;
-        if (!(w > 1e-8f) || !__builtin_isfinite(w)) return false;
-        s[c][0] = ps[c]->x / w;
-        s[c][1] = ps[c]->y / w;
-        if (!__builtin_isfinite(s[c][0]) || !__builtin_isfinite(s[c][1])) return false;
-        min_x = minf_std(min_x, s[c][0]);
-        min_y = minf_std(min_y, s[c][1]);
-        max_x = maxf_std(max_x, s[c][0]);
-        max_y = maxf_std(max_y, s[c][1]);
-    }
-    if (max_x < -1.0f || min_x > 1.0f || max_y < -1.0f || min_y > 1.0f) return false;
-    r.s0x = s[0][0];
-    r.s0y = s[0][1];
-    r.s1x = s[1][0];
-    r.s1y = s[1][1];
-    r.s2x = s[2][0];
-    r.s2y = s[2][1];
-    const float sa = cross2(r.s1x - r.s0x, r.s1y - r.s0y, r.s2x - r.s0x, r.s2y - r.s0y);
-    if (fabsf(sa) <= 1e-12f) return false;
-    if ((cull == 1 && sa <= 0.0f) || (cull == 2 && sa >= 0.0f)) return false;
-    r.sgn = sa > 0.0f ? 1.0f : -1.0f;
-    // src/rasterizer.cpp:101-116 bounds, expanded to whole 16x16 tiles
-    const int64_t wm = static_cast<int64_t>(vd.width) - 1, hm = static_cast<int64_t>(vd.height) - 1;
-    auto cl = [](int64_t v, int64_t hi) { return static_cast<int32_t>(v < 0 ? 0 : (v > hi ? hi : v)); };
-    const int32_t bx0 = cl(f2i_sat(floorf((min_x * 0.5f + 0.5f) * vd.vw + vd.vx)), wm);
-    const int32_t bx1 = cl(static_cast<int64_t>(f2i_sat(ceilf((max_x * 0.5f + 0.5f) * vd.vw + vd.vx))) - 1, wm);
-    const int32_t by0 = cl(f2i_sat(floorf((min_y * 0.5f + 0.5f) * vd.vh + vd.vy)), hm);
-    const int32_t by1 = cl(static_cast<int64_t>(f2i_sat(ceilf((max_y * 0.5f + 0.5f) * vd.vh + vd.vy))) - 1, hm);
-    int32_t cx0 = (bx0 / static_cast<int32_t>(kTile)) * kTile;
-    int32_t cx1 = static_cast<int32_t>(min<int64_t>((bx1 / static_cast<int32_t>(kTile)) * kTile + kTile - 1, wm));
-    int32_t cy0 = (by0 / static_cast<int32_t>(kTile)) * kTile;
-    int32_t cy1 = static_cast<int32_t>(min<int64_t>((by1 / static_cast<int32_t>(kTile)) * kTile + kTile - 1, hm));
-    int32_t hx0, hx1, hy0, hy1;
-    if (halo_rect(r, vd, cx0, cx1, cy0, cy1, &hx0, &hx1, &hy0, &hy1)) {
-        cx0 = max(cx0, hx0);
-        cx1 = min(cx1, hx1);
-        cy0 = max(cy0, hy0);
-        cy1 = min(cy1, hy1);
-    }
-    r.x0 = cx0;
-    r.x1 = cx1;
-    r.y0 = cy0;
-    r.y1 = cy1;
-    return cx0 <= cx1 && cy0 <= cy1;
-}
-
 __device__ __forceinline__ uint32_t order_key(float depth) {
     const uint32_t u = __float_as_uint(depth + 0.0f);  // -0 -> +0
     return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
 }
 
 // rasterize_forward.hlsl:103-135 for one candidate pixel; returns the key or ~0.
+template <class Tri>
 __device__ __forceinline__ unsigned long long test_pixel(const Tri& r, uint32_t t, float sx, float sy) {
-    const float e0 = cross2(r.s1x - sx, r.s1y - sy, r.s2x - sx, r.s2y - sy);
-    const float e1 = cross2(r.s2x - sx, r.s2y - sy, r.s0x - sx, r.s0y - sy);
-    const float e2 = cross2(r.s0x - sx, r.s0y - sy, r.s1x - sx, r.s1y - sy);
-    if (e0 * r.sgn < -1e-7f || e1 * r.sgn < -1e-7f || e2 * r.sgn < -1e-7f) return ~0ull;
+    if (!edge_pass(r, sx, sy)) return ~0ull;
     const float q0x = r.p0.x - sx * r.p0.w, q0y = r.p0.y - sy * r.p0.w;
     const float q1x = r.p1.x - sx * r.p1.w, q1y = r.p1.y - sy * r.p1.w;
     const float q2x = r.p2.x - sx * r.p2.w, q2y = r.p2.y - sy * r.p2.w;
@@ -239,89 +124,198 @@ __device__ __forceinline__ unsigned long long test_pixel(const Tri& r, uint32_t 
     return (static_cast<unsigned long long>(order_key(depth)) << 32) | static_cast<unsigned long long>(t + 1u);
 }
 
-// Intra-wave hand-off of LDS records written by other lanes of the same wave.
+// Keys only decrease, so a stale read can only make the filter let an atomic through.
+__device__ __forceinline__ void key_min(unsigned long long* p, unsigned long long key) {
+    if (key < __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) atomicMin(p, key);
+}
+
+// Intra-wave hand-off of LDS data written by other lanes of the same wave.
 __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Keys only decrease, so a stale read can only make the filter let an atomic through.
-__device__ __forceinline__ void key_min(unsigned long long* p, unsigned long long key) {
-    if (key < __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) atomicMin(p, key);
+// ---- wave-uniform broadcast of lane j's values (j uniform)
+__device__ __forceinline__ float bcf(float v, int j) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
+}
+__device__ __forceinline__ int32_t bci(int32_t v, int j) { return __builtin_amdgcn_readlane(v, j); }
+__device__ __forceinline__ float4 bcf4(const float4& v, int j) {
+    return make_float4(bcf(v.x, j), bcf(v.y, j), bcf(v.z, j), bcf(v.w, j));
+}
+__device__ __forceinline__ void bc_spans(const LineSpans& s, int j, LineSpans& o) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        o.u[i] = bcf(s.u[i], j);
+        o.v[i] = bcf(s.v[i], j);
+    }
+    o.lower = static_cast<uint32_t>(bci(static_cast<int32_t>(s.lower), j));
+    o.upper = static_cast<uint32_t>(bci(static_cast<int32_t>(s.upper), j));
+    o.cols = static_cast<uint32_t>(bci(static_cast<int32_t>(s.cols), j));
 }
 
-struct TriRec {
+// Inclusive prefix sum over the 64 lanes (all lanes active).
+__device__ __forceinline__ int32_t wave_incl_scan(int32_t v, uint32_t lane) {
+#pragma unroll
+    for (int s = 1; s < 64; s <<= 1) {
+        const int32_t o = __shfl_up(v, s, 64);
+        if (lane >= static_cast<uint32_t>(s)) v += o;
+    }
+    return v;
+}
+
+// Smallest lane l with incl[l] > m (incl non-decreasing; all lanes active).
+__device__ __forceinline__ int32_t wave_find(int32_t incl, int32_t m) {
+    int32_t pos = 0;
+#pragma unroll
+    for (int s = 32; s >= 1; s >>= 1) {
+        const int32_t v = __shfl(incl, pos + s - 1, 64);
+        if (v <= m) pos += s;
+    }
+    return pos;
+}
+
+// ------------------------------------------------------------------ forward
+
+__device__ __forceinline__ bool load_corners(const ViewDesc& vd, const uint32_t* __restrict__ idx, uint32_t t,
+                                             uint32_t V, float4& p0, float4& p1, float4& p2) {
+    const uint32_t i0 = idx[t * 3 + 0], i1 = idx[t * 3 + 1], i2 = idx[t * 3 + 2];
+    if (i0 >= V || i1 >= V || i2 >= V) return false;
+    const float4* P = reinterpret_cast<const float4*>(vd.pos);
+    p0 = P[i0];
+    p1 = P[i1];
+    p2 = P[i2];
+    return true;
+}
+
+__device__ __forceinline__ bool setup_forward_tri(const ViewDesc& vd, const uint32_t* __restrict__ idx, uint32_t t,
+                                                  uint32_t V, uint32_t cull, bool allow_cols, ScreenTri& r,
+                                                  LineSpans& sp) {
     float4 p0, p1, p2;
-    float s0x, s0y, s1x, s1y, s2x, s2y, sgn;
-    int32_t x0, y0;
-    uint32_t w, h, t;
+    if (!load_corners(vd, idx, t, V, p0, p1, p2)) return false;
+    return setup_screen(vd, p0, p1, p2, cull, allow_cols, r, sp);
+}
+
+// A triangle of a wave's batch, in LDS: corners, screen coordinates, spans and
+// the range of its lines in the batch's concatenated line list.
+struct alignas(16) TriRec {
+    float4 p0, p1, p2;
+    float s0x, s0y, s1x, s1y;
+    float s2x, s2y, sgn;
+    uint32_t t;
+    float u[3];
+    uint32_t lower;
+    float v[3];
+    uint32_t upper;
+    int32_t a0, a1;  // span range along the lines
+    int32_t lbase;   // line coordinate of list entry f is lbase + f
+    uint32_t cols;   // lines are columns
 };
 
-// One 64-triangle batch per wave: each lane sets up one triangle; rectangles of
-// at most kInlinePixels are tested by their own lane, larger ones (up to
-// kWavePixels) by the whole wave, one pixel per lane, and larger still are
-// queued for k_raster_large.
+__device__ __forceinline__ LineSpans rec_spans(const TriRec& q) {
+    LineSpans s;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        s.u[i] = q.u[i];
+        s.v[i] = q.v[i];
+    }
+    s.lower = q.lower;
+    s.upper = q.upper;
+    s.cols = q.cols;
+    return s;
+}
+
+// One 64-triangle batch per wave.  Each lane sets one triangle up; the lines
+// (rows, or columns for triangles taller than wide) of the batch's triangles
+// are concatenated and dealt 64 at a time to the lanes, each lane computing
+// its line's span; the spans of those 64 lines are concatenated in turn and
+// dealt 64 pixels at a time.  Every lane tests one candidate pixel per step
+// whatever the triangles' sizes.  Triangles above kWaveWork candidates are
+// queued in 16-row slabs for k_raster_large.
 __global__ __launch_bounds__(kBlock) void k_raster_small(ViewTable vt, const uint32_t* __restrict__ idx, uint32_t T,
                                                          uint32_t V, uint32_t cull, const float* __restrict__ sxt,
                                                          const float* __restrict__ syt,
                                                          unsigned long long* __restrict__ keys,
-                                                         uint2* __restrict__ queue, uint32_t* __restrict__ counters) {
-    __shared__ TriRec recs[kBlock];
+                                                         uint4* __restrict__ queue, uint32_t* __restrict__ counters) {
+    __shared__ TriRec recs_all[kBlock];
     const uint32_t view = blockIdx.y;
     const ViewDesc& vd = vt.v[view];
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
-    TriRec* wrecs = recs + wave * 64;
+    TriRec* recs = recs_all + wave * 64;
     unsigned long long* kbase = keys + vd.pixel_base;
+    const float* srow = syt + vd.row_base;
+    const float* scol = sxt + vd.col_base;
+    const float ax[2] = {span_alpha(vd, 0), span_alpha(vd, 1)};
+    const float bx[2] = {span_beta(vd, 0), span_beta(vd, 1)};
     const uint32_t stride = gridDim.x * kBlock;
-    for (uint32_t batch = (blockIdx.x * 4 + wave) * 64; batch < T; batch += stride) {
+    for (uint32_t batch = (blockIdx.x * (kBlock / 64) + wave) * 64; batch < T; batch += stride) {
         const uint32_t t = batch + lane;
-        Tri r;
-        const bool ok = t < T && setup_forward(vd, idx, t, V, cull, r);
-        const uint32_t w = ok ? static_cast<uint32_t>(r.x1 - r.x0 + 1) : 0u;
-        const uint32_t h = ok ? static_cast<uint32_t>(r.y1 - r.y0 + 1) : 0u;
-        const uint32_t area = w * h;
-        bool deferred = false;
-        if (ok && area <= kInlinePixels) {
-            for (int32_t y = r.y0; y <= r.y1; ++y) {
-                const float sy = syt[vd.row_base + y];
-                for (int32_t x = r.x0; x <= r.x1; ++x) {
-                    const unsigned long long key = test_pixel(r, t, sxt[vd.col_base + x], sy);
-                    if (key != ~0ull) key_min(&kbase[static_cast<size_t>(y) * vd.width + x], key);
-                }
-            }
-        } else if (ok && area > kWavePixels) {
-            queue[atomicAdd(&counters[0], 1u)] = make_uint2(view, t);
+        ScreenTri r;
+        LineSpans sp;
+        const bool ok = t < T && setup_forward_tri(vd, idx, t, V, cull, true, r, sp);
+        int32_t lines = 0;
+        if (ok && r.work > kWaveWork) {
+            const uint32_t slabs = static_cast<uint32_t>(r.y1 - r.y0) / kSlabRows + 1u;
+            queue[atomicAdd(&counters[0], 1u)] = make_uint4(view, t, slabs, 0u);
         } else if (ok) {
-            deferred = true;
-            TriRec& q = wrecs[lane];
+            lines = sp.cols ? r.x1 - r.x0 + 1 : r.y1 - r.y0 + 1;
+        }
+        const int32_t incl_l = wave_incl_scan(lines, lane);
+        const int32_t nlines = __builtin_amdgcn_readlane(incl_l, 63);
+        if (lines > 0) {
+            TriRec& q = recs[lane];
             q.p0 = r.p0;
             q.p1 = r.p1;
             q.p2 = r.p2;
-            q.s0x = r.s0x; q.s0y = r.s0y; q.s1x = r.s1x; q.s1y = r.s1y; q.s2x = r.s2x; q.s2y = r.s2y;
+            q.s0x = r.s0x;
+            q.s0y = r.s0y;
+            q.s1x = r.s1x;
+            q.s1y = r.s1y;
+            q.s2x = r.s2x;
+            q.s2y = r.s2y;
             q.sgn = r.sgn;
-            q.x0 = r.x0;
-            q.y0 = r.y0;
-            q.w = w;
-            q.h = h;
             q.t = t;
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                q.u[i] = sp.u[i];
+                q.v[i] = sp.v[i];
+            }
+            q.lower = sp.lower;
+            q.upper = sp.upper;
+            q.cols = sp.cols;
+            q.a0 = sp.cols ? r.y0 : r.x0;
+            q.a1 = sp.cols ? r.y1 : r.x1;
+            q.lbase = (sp.cols ? r.x0 : r.y0) - (incl_l - lines);
         }
         wave_lds_sync();
-        unsigned long long pending = __ballot(deferred);
-        while (pending) {
-            const uint32_t j = __ffsll(pending) - 1;
-            pending &= pending - 1;
-            const TriRec& q = wrecs[j];
-            Tri rr;
-            rr.p0 = q.p0; rr.p1 = q.p1; rr.p2 = q.p2;
-            rr.s0x = q.s0x; rr.s0y = q.s0y; rr.s1x = q.s1x; rr.s1y = q.s1y; rr.s2x = q.s2x; rr.s2y = q.s2y;
-            rr.sgn = q.sgn;
-            const uint32_t qw = q.w, qa = q.w * q.h, qt = q.t;
-            for (uint32_t c = 0; c < qa; c += 64) {
-                const uint32_t i = c + lane;
-                if (i < qa) {
-                    const int32_t x = q.x0 + static_cast<int32_t>(i % qw), y = q.y0 + static_cast<int32_t>(i / qw);
-                    const unsigned long long key = test_pixel(rr, qt, sxt[vd.col_base + x], syt[vd.row_base + y]);
+        for (int32_t lc = 0; lc < nlines; lc += 64) {
+            const int32_t f = lc + static_cast<int32_t>(lane);
+            const int32_t o = wave_find(incl_l, f);
+            int32_t lo = 0, hi = -1, line = 0;
+            float sl = 0.0f;
+            if (f < nlines) {
+                const TriRec& q = recs[o];
+                line = q.lbase + f;
+                sl = q.cols ? scol[line] : srow[line];
+                line_span(rec_spans(q), sl, ax[q.cols], bx[q.cols], q.a0, q.a1, &lo, &hi);
+            }
+            const int32_t len = hi >= lo ? hi - lo + 1 : 0;
+            const int32_t incl_p = wave_incl_scan(len, lane);
+            const int32_t npix = __builtin_amdgcn_readlane(incl_p, 63);
+            const int32_t aoff = lo - (incl_p - len);
+            for (int32_t pc = 0; pc < npix; pc += 64) {
+                const int32_t m = pc + static_cast<int32_t>(lane);
+                const int32_t k = wave_find(incl_p, m);
+                const int32_t a = __shfl(aoff, k, 64) + m;
+                const int32_t ln = __shfl(line, k, 64);
+                const int32_t oo = __shfl(o, k, 64);
+                const float sln = __shfl(sl, k, 64);
+                if (m < npix) {
+                    const TriRec& q = recs[oo];
+                    const int32_t x = q.cols ? ln : a, y = q.cols ? a : ln;
+                    const float sx = q.cols ? sln : scol[x], sy = q.cols ? srow[y] : sln;
+                    const unsigned long long key = test_pixel(q, q.t, sx, sy);
                     if (key != ~0ull) key_min(&kbase[static_cast<size_t>(y) * vd.width + x], key);
                 }
             }
@@ -330,31 +324,86 @@ __global__ __launch_bounds__(kBlock) void k_raster_small(ViewTable vt, const uin
     }
 }
 
+// Rows [ya, yb] of a wave-uniform triangle with row spans, 64 rows at a time.
+__device__ __forceinline__ void wave_forward(const ScreenTri& r, const LineSpans& sp, uint32_t t, int32_t ya,
+                                             int32_t yb, const ViewDesc& vd, uint32_t lane,
+                                             const float* __restrict__ sxt, const float* __restrict__ syt,
+                                             unsigned long long* __restrict__ kbase) {
+    const float* srow = syt + vd.row_base;
+    const float* scol = sxt + vd.col_base;
+    const float alpha = span_alpha(vd, 0), beta = span_beta(vd, 0);
+    for (int32_t y0 = ya; y0 <= yb; y0 += 64) {
+        const int32_t y = y0 + static_cast<int32_t>(lane);
+        int32_t lo = 0, hi = -1;
+        float sy = 0.0f;
+        if (y <= yb) {
+            sy = srow[y];
+            line_span(sp, sy, alpha, beta, r.x0, r.x1, &lo, &hi);
+        }
+        const int32_t len = hi >= lo ? hi - lo + 1 : 0;
+        const int32_t incl = wave_incl_scan(len, lane);
+        const int32_t total = __builtin_amdgcn_readlane(incl, 63);
+        const int32_t xoff = lo - (incl - len);
+        for (int32_t base = 0; base < total; base += 64) {
+            const int32_t m = base + static_cast<int32_t>(lane);
+            const int32_t rl = wave_find(incl, m);
+            const int32_t x = __shfl(xoff, rl, 64) + m;
+            const float syr = __shfl(sy, rl, 64);
+            if (m < total) {
+                const unsigned long long key = test_pixel(r, t, scol[x], syr);
+                if (key != ~0ull) key_min(&kbase[static_cast<size_t>(y0 + rl) * vd.width + x], key);
+            }
+        }
+    }
+}
+
+// Queued triangles, split into 16-row slabs dealt round-robin to every wave of
+// the grid (slab k of the concatenated slab list goes to wave k mod waves).
 __global__ __launch_bounds__(kBlock) void k_raster_large(ViewTable vt, const uint32_t* __restrict__ idx, uint32_t T,
                                                          uint32_t V, uint32_t cull, const float* __restrict__ sxt,
                                                          const float* __restrict__ syt,
                                                          unsigned long long* __restrict__ keys,
-                                                         const uint2* __restrict__ queue,
-                                                         uint32_t* __restrict__ counters) {
-    __shared__ uint32_t entry;
+                                                         const uint4* __restrict__ queue,
+                                                         const uint32_t* __restrict__ counters) {
+    __shared__ uint32_t nslab[kBlock];
+    __shared__ unsigned long long before[kBlock];
+    __shared__ unsigned long long batch_total;
     const uint32_t count = counters[0];
-    while (true) {
-        if (threadIdx.x == 0) entry = atomicAdd(&counters[1], 1u);
+    const uint32_t waves = gridDim.x * (kBlock / 64);
+    const uint32_t wid = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6), lane = threadIdx.x & 63u;
+    unsigned long long cum = 0;
+    for (uint32_t b0 = 0; b0 < count; b0 += kBlock) {
+        nslab[threadIdx.x] = b0 + threadIdx.x < count ? queue[b0 + threadIdx.x].z : 0u;
         __syncthreads();
-        const uint32_t e = entry;
-        __syncthreads();
-        if (e >= count) return;
-        const uint2 q = queue[e];
-        const ViewDesc& vd = vt.v[q.x];
-        Tri r;
-        if (!setup_forward(vd, idx, q.y, V, cull, r)) continue;
-        const uint32_t w = static_cast<uint32_t>(r.x1 - r.x0 + 1), h = static_cast<uint32_t>(r.y1 - r.y0 + 1);
-        unsigned long long* base = keys + vd.pixel_base;
-        for (uint32_t i = threadIdx.x; i < w * h; i += kBlock) {
-            const int32_t x = r.x0 + static_cast<int32_t>(i % w), y = r.y0 + static_cast<int32_t>(i / w);
-            const unsigned long long key = test_pixel(r, q.y, sxt[vd.col_base + x], syt[vd.row_base + y]);
-            if (key != ~0ull) key_min(&base[static_cast<size_t>(y) * vd.width + x], key);
+        if (threadIdx.x == 0) {
+            unsigned long long s = 0;
+            for (uint32_t i = 0; i < kBlock; ++i) {
+                before[i] = s;
+                s += nslab[i];
+            }
+            batch_total = s;
         }
+        __syncthreads();
+        const uint32_t n_in = count - b0 < kBlock ? count - b0 : kBlock;
+        for (uint32_t j = 0; j < n_in; ++j) {
+            const uint32_t n = nslab[j];
+            const uint32_t first = static_cast<uint32_t>((cum + before[j]) % waves);
+            uint32_t s = wid >= first ? wid - first : wid + waves - first;
+            if (s >= n) continue;
+            const uint4 q = queue[b0 + j];
+            const ViewDesc& vd = vt.v[q.x];
+            ScreenTri r;
+            LineSpans sp;
+            if (!setup_forward_tri(vd, idx, q.y, V, cull, false, r, sp)) continue;
+            for (; s < n; s += waves) {
+                const int32_t ya = r.y0 + static_cast<int32_t>(s * kSlabRows);
+                const int32_t yb = ya + static_cast<int32_t>(kSlabRows) - 1 < r.y1 ? ya + static_cast<int32_t>(kSlabRows) - 1
+                                                                                  : r.y1;
+                wave_forward(r, sp, q.y, ya, yb, vd, lane, sxt, syt, keys + vd.pixel_base);
+            }
+        }
+        cum += batch_total;
+        __syncthreads();
     }
 }
 
@@ -409,82 +458,55 @@ __global__ __launch_bounds__(kBlock) void k_raster_resolve(ViewTable vt, const u
 
 // ----------------------------------------------------------------- backward
 
-struct BwdTri {
-    float4 p[3];
-    int32_t x0, x1, y0, y1;  // [x0, x1) x [y0, y1), rasterize_backward.hlsl:75-87
-};
-
-__device__ bool setup_backward(const ViewDesc& vd, const uint32_t* __restrict__ idx, uint32_t t, uint32_t V,
-                               BwdTri& b) {
-    const uint32_t i0 = idx[t * 3 + 0], i1 = idx[t * 3 + 1], i2 = idx[t * 3 + 2];
-    if (i0 >= V || i1 >= V || i2 >= V) return false;
-    const float4* P = reinterpret_cast<const float4*>(vd.pos);
-    b.p[0] = P[i0];
-    b.p[1] = P[i1];
-    b.p[2] = P[i2];
-    if (b.p[0].w <= 1e-8f || b.p[1].w <= 1e-8f || b.p[2].w <= 1e-8f) return false;
-    const float s0x = b.p[0].x / b.p[0].w, s0y = b.p[0].y / b.p[0].w;
-    const float s1x = b.p[1].x / b.p[1].w, s1y = b.p[1].y / b.p[1].w;
-    const float s2x = b.p[2].x / b.p[2].w, s2y = b.p[2].y / b.p[2].w;
-    const float mnx = minf_std(s0x, minf_std(s1x, s2x)), mny = minf_std(s0y, minf_std(s1y, s2y));
-    const float mxx = maxf_std(s0x, maxf_std(s1x, s2x)), mxy = maxf_std(s0y, maxf_std(s1y, s2y));
-    auto cl = [](int32_t v, int32_t hi) { return v < 0 ? 0 : (v > hi ? hi : v); };
-    const int32_t W = static_cast<int32_t>(vd.width), H = static_cast<int32_t>(vd.height);
-    b.x0 = cl(f2i_sat(floorf((mnx * 0.5f + 0.5f) * vd.vw + vd.vx)), W);
-    b.x1 = cl(f2i_sat(ceilf((mxx * 0.5f + 0.5f) * vd.vw + vd.vx)), W);
-    b.y0 = cl(f2i_sat(floorf((mny * 0.5f + 0.5f) * vd.vh + vd.vy)), H);
-    b.y1 = cl(f2i_sat(ceilf((mxy * 0.5f + 0.5f) * vd.vh + vd.vy)), H);
-    return true;
+// Per pixel: the id a triangle's backward loop compares against
+// (rasterize_backward.hlsl:96-100: decoded id, 0 when grad_raster.xy is zero).
+// The backward visits only the forward candidate region of each triangle
+// (rows and spans of raster_geom.hpp) inside its bounding box; a pixel that
+// names triangle t, lies in t's box and is outside that region (which the
+// forward never produces, but a caller-supplied raster can) marks its view,
+// and the view's triangles are then walked over their whole boxes.  A pixel
+// that passes t's float edge test is inside t's region, which skips the
+// region computation for every pixel of a forward-produced raster.
+__global__ __launch_bounds__(kBlock) void k_bwd_prepass(ViewTable vt, const uint32_t* __restrict__ idx, uint32_t T,
+                                                        uint32_t V, const float* __restrict__ sxt,
+                                                        const float* __restrict__ syt, uint32_t* __restrict__ cid,
+                                                        uint32_t* __restrict__ view_flags) {
+    const uint32_t gp = blockIdx.x * kBlock + threadIdx.x;
+    if (gp >= vt.total_pixels) return;
+    const uint32_t view = find_view(vt, gp);
+    const ViewDesc& vd = vt.v[view];
+    const uint32_t p = gp - vd.pixel_base;
+    const uint32_t id = decode_id(vd.raster[static_cast<size_t>(p) * 4 + 3]);
+    const float2 lg = reinterpret_cast<const float2*>(vd.grad_raster)[static_cast<size_t>(p) * 2];
+    uint32_t c = 0;
+    if (id != 0u && id <= T && !(lg.x == 0.0f && lg.y == 0.0f)) {
+        c = id;
+        float4 p0, p1, p2;
+        int32_t bx0, bx1, by0, by1;
+        const int32_t x = static_cast<int32_t>(p % vd.width), y = static_cast<int32_t>(p / vd.width);
+        if (load_corners(vd, idx, id - 1u, V, p0, p1, p2) && backward_box(vd, p0, p1, p2, &bx0, &bx1, &by0, &by1) &&
+            x >= bx0 && x < bx1 && y >= by0 && y < by1) {
+            const float sx = sxt[vd.col_base + x], sy = syt[vd.row_base + y];
+            ScreenTri r;
+            float bb[4];
+            bool inside = screen_coords(p0, p1, p2, 0u, r, bb) && edge_pass(r, sx, sy);
+            if (!inside) {
+                LineSpans sp;
+                if (setup_screen(vd, p0, p1, p2, 0u, false, r, sp) && y >= r.y0 && y <= r.y1) {
+                    int32_t lo, hi;
+                    line_span(sp, sy, span_alpha(vd, 0), span_beta(vd, 0), r.x0, r.x1, &lo, &hi);
+                    inside = x >= lo && x <= hi;
+                }
+            }
+            if (!inside) atomicOr(&view_flags[view], 1u);
+        }
+    }
+    cid[gp] = c;
 }
 
-// rasterize_backward.hlsl:96-151 for one pixel of triangle t; adds into g (x, y, w per corner).
-__device__ __forceinline__ void corner_terms(const BwdTri& b, const ViewDesc& vd, uint32_t t, int32_t x, int32_t y,
-                                             const float* __restrict__ sxt, const float* __restrict__ syt,
-                                             float g[3][3]) {
-    const size_t pix = static_cast<size_t>(y) * vd.width + static_cast<size_t>(x);
-    if (decode_id(vd.raster[pix * 4 + 3]) != t + 1u) return;
-    const float2 lg = reinterpret_cast<const float2*>(vd.grad_raster)[pix * 2];
-    if (lg.x == 0.0f && lg.y == 0.0f) return;
-    const float sx = sxt[vd.col_base + x], sy = syt[vd.row_base + y];
-    const float q0x = b.p[0].x - sx * b.p[0].w, q0y = b.p[0].y - sy * b.p[0].w;
-    const float q1x = b.p[1].x - sx * b.p[1].w, q1y = b.p[1].y - sy * b.p[1].w;
-    const float q2x = b.p[2].x - sx * b.p[2].w, q2y = b.p[2].y - sy * b.p[2].w;
-    const float area = cross2(q1x, q1y, q2x, q2y) + cross2(q2x, q2y, q0x, q0y) + cross2(q0x, q0y, q1x, q1y);
-    if (fabsf(area) <= 1e-12f) return;
-    const float eps = area >= 0.0f ? 1e-6f : -1e-6f;
-    const float inv = 1.0f / (area + eps);
-    const float a0 = cross2(q1x, q1y, q2x, q2y);
-    const float a1 = cross2(q2x, q2y, q0x, q0y);
-    const float b0 = a0 * inv, b1 = a1 * inv;
-    const float gb0 = lg.x * inv, gb1 = lg.y * inv;
-    const float gc = gb0 * b0 + gb1 * b1;
-    const float g0x = gc * (q2y - q1y) - gb1 * q2y;
-    const float g1x = gc * (q0y - q2y) + gb0 * q2y;
-    const float g2x = gc * (q1y - q0y) - gb0 * q1y + gb1 * q0y;
-    const float g0y = gc * (q1x - q2x) + gb1 * q2x;
-    const float g1y = gc * (q2x - q0x) - gb0 * q2x;
-    const float g2y = gc * (q0x - q1x) + gb0 * q1x - gb1 * q0x;
-    g[0][0] += g0x;
-    g[0][1] += g0y;
-    g[0][2] += -sx * g0x - sy * g0y;
-    g[1][0] += g1x;
-    g[1][1] += g1y;
-    g[1][2] += -sx * g1x - sy * g1y;
-    g[2][0] += g2x;
-    g[2][1] += g2y;
-    g[2][2] += -sx * g2x - sy * g2y;
-}
-
-// Terms of one pixel (rasterize_backward.hlsl:94-151): returns false when the
-// pixel does not contribute; t9 = (x, y, w) for corners 0..2.
-__device__ __forceinline__ bool pixel_terms(const float4* p, const ViewDesc& vd, uint32_t t, int32_t x, int32_t y,
-                                            const float* __restrict__ sxt, const float* __restrict__ syt,
-                                            float t9[9]) {
-    const size_t pix = static_cast<size_t>(y) * vd.width + static_cast<size_t>(x);
-    if (decode_id(vd.raster[pix * 4 + 3]) != t + 1u) return false;
-    const float2 lg = reinterpret_cast<const float2*>(vd.grad_raster)[pix * 2];
-    if (lg.x == 0.0f && lg.y == 0.0f) return false;
-    const float sx = sxt[vd.col_base + x], sy = syt[vd.row_base + y];
+// Terms of one pixel (rasterize_backward.hlsl:101-151): false when the pixel's
+// area is degenerate; t9 = (x, y, w) for corners 0..2.
+__device__ __forceinline__ bool pixel_terms(const float4* p, float sx, float sy, float2 lg, float t9[9]) {
     const float q0x = p[0].x - sx * p[0].w, q0y = p[0].y - sy * p[0].w;
     const float q1x = p[1].x - sx * p[1].w, q1y = p[1].y - sy * p[1].w;
     const float q2x = p[2].x - sx * p[2].w, q2y = p[2].y - sy * p[2].w;
@@ -509,98 +531,265 @@ __device__ __forceinline__ bool pixel_terms(const float4* p, const ViewDesc& vd,
     return true;
 }
 
-struct BoxRec {
+// Rows [ya, yb] and columns [xa, xb] of triangle t visited by the backward
+// (inside the row spans unless the view is walked over whole boxes).
+struct BwdRegion {
     float4 p[3];
-    int32_t x0, y0;
-    uint32_t w, h, t;
+    LineSpans sp;
+    int32_t ya, yb, xa, xb;
+    uint32_t box;  // bounding-box pixel count
+    float work;    // estimated visited pixels + rows
 };
 
+__device__ __forceinline__ bool setup_backward(const ViewDesc& vd, const uint32_t* __restrict__ idx, uint32_t t,
+                                               uint32_t V, bool full, BwdRegion& g) {
+    int32_t bx0, bx1, by0, by1;
+    if (!load_corners(vd, idx, t, V, g.p[0], g.p[1], g.p[2])) return false;
+    if (!backward_box(vd, g.p[0], g.p[1], g.p[2], &bx0, &bx1, &by0, &by1) || bx1 <= bx0 || by1 <= by0) return false;
+    g.box = static_cast<uint32_t>(bx1 - bx0) * static_cast<uint32_t>(by1 - by0);
+    if (full) {
+        g.ya = by0;
+        g.yb = by1 - 1;
+        g.xa = bx0;
+        g.xb = bx1 - 1;
+        g.sp.lower = g.sp.upper = g.sp.cols = 0u;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) g.sp.u[i] = g.sp.v[i] = 0.0f;
+        g.work = static_cast<float>(g.box);
+        return true;
+    }
+    ScreenTri r;
+    if (!setup_screen(vd, g.p[0], g.p[1], g.p[2], 0u, false, r, g.sp)) return false;
+    g.ya = r.y0 > by0 ? r.y0 : by0;
+    g.yb = r.y1 < by1 - 1 ? r.y1 : by1 - 1;
+    g.xa = r.x0 > bx0 ? r.x0 : bx0;
+    g.xb = r.x1 < bx1 - 1 ? r.x1 : bx1 - 1;
+    g.work = r.work;
+    return g.ya <= g.yb && g.xa <= g.xb;
+}
+
+// A triangle's rows in its wave batch's concatenated row list.
+struct alignas(16) BwdRec {
+    float4 p[3];
+    float u[3];
+    uint32_t lower;
+    float v[3];
+    uint32_t upper;
+    int32_t xa, xb, lbase, pad;
+};
+
+constexpr uint32_t kList = 64;  // matched pixels buffered per wave before the per-triangle fold
+
+// Lane `lane` adds, in list order, the buffered terms it owns (entries with
+// owner == lane are contiguous: the list is in batch order).
+__device__ __forceinline__ void fold_list(const float* __restrict__ lterm, const uint32_t* __restrict__ lown,
+                                          uint32_t n, uint32_t lane, bool mine, float s[9]) {
+    wave_lds_sync();
+    if (mine) {
+        uint32_t a = 0, b = n;  // first entry with owner >= lane
+        while (a < b) {
+            const uint32_t h = (a + b) >> 1;
+            if (lown[h] < lane) a = h + 1; else b = h;
+        }
+        for (uint32_t e = a; e < n && lown[e] == lane; ++e)
+#pragma unroll
+            for (int k = 0; k < 9; ++k) s[k] += lterm[e * 9 + k];
+    }
+    wave_lds_sync();
+}
+
 // Per-triangle corner gradients summed over the reference's bounding box in
-// its row-major order.  Small boxes: the triangle's own lane loops; medium:
-// the wave tests 64 pixels at a time and adds the contributing pixels' terms
-// in ascending pixel order (ballot + readlane), which is the reference's order.
+// its row-major order.  Each wave takes a batch of 64 triangles; the rows of
+// triangles with at most kFlatWork estimated pixels are concatenated and
+// walked 64 pixels at a time (as the forward); pixels naming their triangle
+// compute their terms and append them, in walk order, to a per-wave list,
+// which each triangle's lane then adds in order (the walk order restricted to
+// one triangle is the reference's row-major order).  Larger triangles (boxes
+// up to kWaveBox pixels) are walked by the whole wave, one at a time, adding
+// the contributing pixels' terms in ascending pixel order through
+// ballot + readlane; larger boxes are queued for k_corner_large.
 __global__ __launch_bounds__(kBlock) void k_corner_small(ViewTable vt, const uint32_t* __restrict__ idx, uint32_t T,
                                                          uint32_t V, const float* __restrict__ sxt,
-                                                         const float* __restrict__ syt, float4* __restrict__ corner,
-                                                         uint2* __restrict__ queue, uint32_t* __restrict__ counters) {
-    __shared__ BoxRec recs[kBlock];
+                                                         const float* __restrict__ syt,
+                                                         const uint32_t* __restrict__ cid,
+                                                         const uint32_t* __restrict__ view_flags,
+                                                         float4* __restrict__ corner, uint4* __restrict__ queue,
+                                                         uint32_t* __restrict__ counters) {
+    __shared__ BwdRec recs_all[kBlock];
+    __shared__ float lterm_all[kBlock / 64][kList * 9];
+    __shared__ uint32_t lown_all[kBlock / 64][kList];
     const uint32_t view = blockIdx.y;
     const ViewDesc& vd = vt.v[view];
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
-    BoxRec* wrecs = recs + wave * 64;
+    BwdRec* recs = recs_all + wave * 64;
+    float* lterm = lterm_all[wave];
+    uint32_t* lown = lown_all[wave];
+    const bool full = view_flags[view] != 0u;
+    const float alpha = span_alpha(vd, 0), beta = span_beta(vd, 0);
+    const float* srow = syt + vd.row_base;
+    const float* scol = sxt + vd.col_base;
+    const uint32_t* cv = cid + vd.pixel_base;
+    const float2* gr = reinterpret_cast<const float2*>(vd.grad_raster);
     float4* cview = corner + static_cast<size_t>(view) * T * 3;
     const uint32_t stride = gridDim.x * kBlock;
-    for (uint32_t batch = (blockIdx.x * 4 + wave) * 64; batch < T; batch += stride) {
+    for (uint32_t batch = (blockIdx.x * (kBlock / 64) + wave) * 64; batch < T; batch += stride) {
         const uint32_t t = batch + lane;
-        BwdTri b{};
-        const bool ok = t < T && setup_backward(vd, idx, t, V, b) && b.x1 > b.x0 && b.y1 > b.y0;
-        const uint32_t w = ok ? static_cast<uint32_t>(b.x1 - b.x0) : 0u, h = ok ? static_cast<uint32_t>(b.y1 - b.y0) : 0u;
-        const uint32_t area = w * h;
+        BwdRegion g;
+        const bool ok = t < T && setup_backward(vd, idx, t, V, full, g);
+        int32_t rows = 0;
         bool deferred = false;
-        if (t < T && (!ok || area <= kInlineBox)) {
-            float g[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-            for (int32_t y = b.y0; ok && y < b.y1; ++y)
-                for (int32_t x = b.x0; x < b.x1; ++x) {
-                    float t9[9];
-                    if (pixel_terms(b.p, vd, t, x, y, sxt, syt, t9))
+        if (t < T && !ok) {
 #pragma unroll
-                        for (int k = 0; k < 9; ++k) g[k] += t9[k];
-                }
-#pragma unroll
-            for (int c = 0; c < 3; ++c) cview[t * 3 + c] = make_float4(g[c * 3], g[c * 3 + 1], 0.0f, g[c * 3 + 2]);
-        } else if (ok && area > kWaveBox) {
-            queue[atomicAdd(&counters[0], 1u)] = make_uint2(view, t);
-        } else if (ok) {
+            for (int c = 0; c < 3; ++c) cview[t * 3 + c] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        } else if (ok && g.box > kWaveBox) {
+            queue[atomicAdd(&counters[0], 1u)] = make_uint4(view, t, 0u, 0u);
+        } else if (ok && g.work > kFlatWork) {
             deferred = true;
-            BoxRec& q = wrecs[lane];
-            q.p[0] = b.p[0];
-            q.p[1] = b.p[1];
-            q.p[2] = b.p[2];
-            q.x0 = b.x0;
-            q.y0 = b.y0;
-            q.w = w;
-            q.h = h;
-            q.t = t;
+        } else if (ok) {
+            rows = g.yb - g.ya + 1;
         }
+        const int32_t incl_l = wave_incl_scan(rows, lane);
+        const int32_t nrows = __builtin_amdgcn_readlane(incl_l, 63);
+        if (rows > 0) {
+            BwdRec& q = recs[lane];
+            q.p[0] = g.p[0];
+            q.p[1] = g.p[1];
+            q.p[2] = g.p[2];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                q.u[i] = g.sp.u[i];
+                q.v[i] = g.sp.v[i];
+            }
+            q.lower = g.sp.lower;
+            q.upper = g.sp.upper;
+            q.xa = g.xa;
+            q.xb = g.xb;
+            q.lbase = g.ya - (incl_l - rows);
+        }
+        float s[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        uint32_t nlist = 0;
         wave_lds_sync();
-        unsigned long long pending = __ballot(deferred);
-        while (pending) {
-            const uint32_t j = __ffsll(pending) - 1;
-            pending &= pending - 1;
-            const BoxRec& q = wrecs[j];
-            const float4 pj[3] = {q.p[0], q.p[1], q.p[2]};
-            const uint32_t qw = q.w, qa = q.w * q.h, qt = q.t;
-            float g[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-            for (uint32_t c = 0; c < qa; c += 64) {
-                const uint32_t i = c + lane;
+        for (int32_t lc = 0; lc < nrows; lc += 64) {
+            const int32_t f = lc + static_cast<int32_t>(lane);
+            const int32_t o = wave_find(incl_l, f);
+            int32_t lo = 0, hi = -1, y = 0;
+            float sy = 0.0f;
+            if (f < nrows) {
+                const BwdRec& q = recs[o];
+                LineSpans sp;
+#pragma unroll
+                for (int i = 0; i < 3; ++i) {
+                    sp.u[i] = q.u[i];
+                    sp.v[i] = q.v[i];
+                }
+                sp.lower = q.lower;
+                sp.upper = q.upper;
+                sp.cols = 0u;
+                y = q.lbase + f;
+                sy = srow[y];
+                line_span(sp, sy, alpha, beta, q.xa, q.xb, &lo, &hi);
+            }
+            const int32_t len = hi >= lo ? hi - lo + 1 : 0;
+            const int32_t incl_p = wave_incl_scan(len, lane);
+            const int32_t npix = __builtin_amdgcn_readlane(incl_p, 63);
+            const int32_t xoff = lo - (incl_p - len);
+            for (int32_t pc = 0; pc < npix; pc += 64) {
+                const int32_t m = pc + static_cast<int32_t>(lane);
+                const int32_t k = wave_find(incl_p, m);
+                const int32_t x = __shfl(xoff, k, 64) + m;
+                const int32_t yy = __shfl(y, k, 64);
+                const int32_t oo = __shfl(o, k, 64);
+                const float syy = __shfl(sy, k, 64);
                 float t9[9];
                 bool hit = false;
-                if (i < qa)
-                    hit = pixel_terms(pj, vd, qt, q.x0 + static_cast<int32_t>(i % qw), q.y0 + static_cast<int32_t>(i / qw),
-                                      sxt, syt, t9);
-                unsigned long long m = __ballot(hit);
-                while (m) {
-                    const int src = __ffsll(m) - 1;
-                    m &= m - 1;
+                if (m < npix) {
+                    const size_t pix = static_cast<size_t>(yy) * vd.width + static_cast<size_t>(x);
+                    if (cv[pix] == batch + static_cast<uint32_t>(oo) + 1u)
+                        hit = pixel_terms(recs[oo].p, scol[x], syy, gr[pix * 2], t9);
+                }
+                const unsigned long long bal = __ballot(hit);
+                const uint32_t cnt = static_cast<uint32_t>(__popcll(bal));
+                if (nlist + cnt > kList) {
+                    fold_list(lterm, lown, nlist, lane, rows > 0, s);
+                    nlist = 0;
+                }
+                if (hit) {
+                    const uint32_t pos = nlist + __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(bal >> 32),
+                                                                           __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(bal), 0u));
 #pragma unroll
-                    for (int k = 0; k < 9; ++k)
-                        g[k] += __int_as_float(__builtin_amdgcn_readlane(__float_as_int(t9[k]), src));
+                    for (int kk = 0; kk < 9; ++kk) lterm[pos * 9 + kk] = t9[kk];
+                    lown[pos] = static_cast<uint32_t>(oo);
+                }
+                nlist += cnt;
+            }
+        }
+        fold_list(lterm, lown, nlist, lane, rows > 0, s);
+        if (rows > 0)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) cview[t * 3 + c] = make_float4(s[c * 3], s[c * 3 + 1], 0.0f, s[c * 3 + 2]);
+        unsigned long long pending = __ballot(deferred);
+        while (pending) {
+            const int j = __ffsll(pending) - 1;
+            pending &= pending - 1;
+            BwdRegion q;
+            q.p[0] = bcf4(g.p[0], j);
+            q.p[1] = bcf4(g.p[1], j);
+            q.p[2] = bcf4(g.p[2], j);
+            bc_spans(g.sp, j, q.sp);
+            q.ya = bci(g.ya, j);
+            q.yb = bci(g.yb, j);
+            q.xa = bci(g.xa, j);
+            q.xb = bci(g.xb, j);
+            const uint32_t qt = batch + static_cast<uint32_t>(j);
+            float w[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+            for (int32_t y0 = q.ya; y0 <= q.yb; y0 += 64) {
+                const int32_t y = y0 + static_cast<int32_t>(lane);
+                int32_t lo = 0, hi = -1;
+                float sy = 0.0f;
+                if (y <= q.yb) {
+                    sy = srow[y];
+                    line_span(q.sp, sy, alpha, beta, q.xa, q.xb, &lo, &hi);
+                }
+                const int32_t len = hi >= lo ? hi - lo + 1 : 0;
+                const int32_t incl = wave_incl_scan(len, lane);
+                const int32_t total = __builtin_amdgcn_readlane(incl, 63);
+                const int32_t xoff = lo - (incl - len);
+                for (int32_t base = 0; base < total; base += 64) {
+                    const int32_t m = base + static_cast<int32_t>(lane);
+                    const int32_t rl = wave_find(incl, m);
+                    const int32_t x = __shfl(xoff, rl, 64) + m;
+                    const float syr = __shfl(sy, rl, 64);
+                    float t9[9];
+                    bool hit = false;
+                    if (m < total) {
+                        const size_t pix = static_cast<size_t>(y0 + rl) * vd.width + static_cast<size_t>(x);
+                        if (cv[pix] == qt + 1u) hit = pixel_terms(q.p, scol[x], syr, gr[pix * 2], t9);
+                    }
+                    unsigned long long mk = __ballot(hit);
+                    while (mk) {
+                        const int src = __ffsll(mk) - 1;
+                        mk &= mk - 1;
+#pragma unroll
+                        for (int kk = 0; kk < 9; ++kk) w[kk] += bcf(t9[kk], src);
+                    }
                 }
             }
             if (lane == 0)
 #pragma unroll
-                for (int c = 0; c < 3; ++c) cview[qt * 3 + c] = make_float4(g[c * 3], g[c * 3 + 1], 0.0f, g[c * 3 + 2]);
+                for (int c = 0; c < 3; ++c) cview[qt * 3 + c] = make_float4(w[c * 3], w[c * 3 + 1], 0.0f, w[c * 3 + 2]);
         }
-        wave_lds_sync();
     }
 }
 
-// Large boxes: each lane sums a strided subset of the box, then a fixed-order
-// tree over the workgroup (deterministic; not the reference's summation order).
+// Boxes over kWaveBox pixels: each lane sums a strided subset of the box, then
+// a fixed-order tree over the workgroup (deterministic; not the reference's
+// summation order).
 __global__ __launch_bounds__(kBlock) void k_corner_large(ViewTable vt, const uint32_t* __restrict__ idx, uint32_t T,
                                                          uint32_t V, const float* __restrict__ sxt,
-                                                         const float* __restrict__ syt, float4* __restrict__ corner,
-                                                         const uint2* __restrict__ queue,
+                                                         const float* __restrict__ syt,
+                                                         const uint32_t* __restrict__ cid,
+                                                         const uint32_t* __restrict__ view_flags,
+                                                         float4* __restrict__ corner, const uint4* __restrict__ queue,
                                                          uint32_t* __restrict__ counters) {
     __shared__ uint32_t entry;
     __shared__ float red[9][kBlock];
@@ -611,22 +800,33 @@ __global__ __launch_bounds__(kBlock) void k_corner_large(ViewTable vt, const uin
         const uint32_t e = entry;
         __syncthreads();
         if (e >= count) return;
-        const uint2 q = queue[e];
+        const uint4 q = queue[e];
         const ViewDesc& vd = vt.v[q.x];
-        BwdTri b;
-        setup_backward(vd, idx, q.y, V, b);
-        const uint32_t w = static_cast<uint32_t>(b.x1 - b.x0), h = static_cast<uint32_t>(b.y1 - b.y0);
-        float g[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-        for (uint32_t i = threadIdx.x; i < w * h; i += kBlock)
-            corner_terms(b, vd, q.y, b.x0 + static_cast<int32_t>(i % w), b.y0 + static_cast<int32_t>(i / w), sxt,
-                         syt, g);
+        BwdRegion g;
+        const bool ok = setup_backward(vd, idx, q.y, V, view_flags[q.x] != 0u, g);
+        const float alpha = span_alpha(vd, 0), beta = span_beta(vd, 0);
+        const uint32_t* cv = cid + vd.pixel_base;
+        const float2* gr = reinterpret_cast<const float2*>(vd.grad_raster);
+        float s[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (int32_t y = g.ya + static_cast<int32_t>(threadIdx.x >> 6); ok && y <= g.yb; y += kBlock / 64) {
+            const float sy = syt[vd.row_base + y];
+            int32_t lo, hi;
+            line_span(g.sp, sy, alpha, beta, g.xa, g.xb, &lo, &hi);
+            for (int32_t x = lo + static_cast<int32_t>(threadIdx.x & 63u); x <= hi; x += 64) {
+                const size_t pix = static_cast<size_t>(y) * vd.width + static_cast<size_t>(x);
+                float t9[9];
+                if (cv[pix] == q.y + 1u && pixel_terms(g.p, sxt[vd.col_base + x], sy, gr[pix * 2], t9))
 #pragma unroll
-        for (int k = 0; k < 9; ++k) red[k][threadIdx.x] = g[k / 3][k % 3];
+                    for (int k = 0; k < 9; ++k) s[k] += t9[k];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 9; ++k) red[k][threadIdx.x] = s[k];
         __syncthreads();
-        for (uint32_t s = kBlock / 2; s > 0; s >>= 1) {
-            if (threadIdx.x < s)
+        for (uint32_t h = kBlock / 2; h > 0; h >>= 1) {
+            if (threadIdx.x < h)
 #pragma unroll
-                for (int k = 0; k < 9; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + s];
+                for (int k = 0; k < 9; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + h];
             __syncthreads();
         }
         if (threadIdx.x == 0) {
@@ -723,6 +923,15 @@ hipError_t raster_resolve(const ViewTable& vt, const uint32_t* idx, int want_der
     return hipGetLastError();
 }
 
+hipError_t backward_prepass(const ViewTable& vt, const uint32_t* idx, uint32_t T, uint32_t V, const RasterScratch& s,
+                            hipStream_t st) {
+    hipError_t e = fill_u32(s.view_flags, 0u, kMaxViews, st);
+    if (e != hipSuccess) return e;
+    k_bwd_prepass<<<(vt.total_pixels + kBlock - 1) / kBlock, kBlock, 0, st>>>(vt, idx, T, V, s.sx, s.sy, s.cid,
+                                                                             s.view_flags);
+    return hipGetLastError();
+}
+
 size_t csr_temp_bytes(uint32_t T, uint32_t V) {
     size_t a = 0, b = 0;
     (void)hipcub::DeviceScan::ExclusiveSum(nullptr, a, static_cast<const uint32_t*>(nullptr),
@@ -759,9 +968,11 @@ hipError_t corner_gradients(const ViewTable& vt, const uint32_t* idx, uint32_t T
                             float4* corner, hipStream_t st) {
     hipError_t e = fill_u32(s.counters, 0u, 4, st);
     if (e != hipSuccess) return e;
-    k_corner_small<<<dim3(tri_grid(T), vt.n), kBlock, 0, st>>>(vt, idx, T, V, s.sx, s.sy, corner, s.queue, s.counters);
+    k_corner_small<<<dim3(tri_grid(T), vt.n), kBlock, 0, st>>>(vt, idx, T, V, s.sx, s.sy, s.cid, s.view_flags, corner,
+                                                               s.queue, s.counters);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    k_corner_large<<<kLargeBlocks, kBlock, 0, st>>>(vt, idx, T, V, s.sx, s.sy, corner, s.queue, s.counters);
+    k_corner_large<<<kLargeBlocks, kBlock, 0, st>>>(vt, idx, T, V, s.sx, s.sy, s.cid, s.view_flags, corner, s.queue,
+                                                    s.counters);
     return hipGetLastError();
 }
 
@@ -773,3 +984,46 @@ hipError_t vertex_reduce(const ViewTable& vt, uint32_t T, uint32_t V, const Vert
 }
 
 }  // namespace pdr
+
+// Self-test hook (host only, no device work): the candidate region of one
+// triangle given by its clip positions (12 floats) in a W x H view with the
+// resolved viewport (vx, vy, vw, vh), as the forward (allow_cols = 1) or the
+// backward (allow_cols = 0) walks it.  Writes the candidate rectangle
+// (x0, x1, y0, y1) and mask[y * W + x] = 1 for every candidate pixel.  Returns
+// 1 when the triangle has candidates, 0 otherwise.
+extern "C" int pdr_testing_candidate_region(const float* clip12, uint32_t width, uint32_t height,
+                                            const float* viewport4, uint32_t cull, int allow_cols, int32_t* rect4,
+                                            uint8_t* mask) {
+    pdr::ViewDesc vd{};
+    vd.width = width;
+    vd.height = height;
+    vd.vx = viewport4[0];
+    vd.vy = viewport4[1];
+    vd.vw = viewport4[2];
+    vd.vh = viewport4[3];
+    const float4 p0 = make_float4(clip12[0], clip12[1], clip12[2], clip12[3]);
+    const float4 p1 = make_float4(clip12[4], clip12[5], clip12[6], clip12[7]);
+    const float4 p2 = make_float4(clip12[8], clip12[9], clip12[10], clip12[11]);
+    pdr::ScreenTri r;
+    pdr::LineSpans sp;
+    for (size_t i = 0; i < static_cast<size_t>(width) * height; ++i) mask[i] = 0;
+    if (!pdr::setup_screen(vd, p0, p1, p2, cull, allow_cols != 0, r, sp)) return 0;
+    rect4[0] = r.x0;
+    rect4[1] = r.x1;
+    rect4[2] = r.y0;
+    rect4[3] = r.y1;
+    const float alpha = pdr::span_alpha(vd, sp.cols), beta = pdr::span_beta(vd, sp.cols);
+    const int32_t l0 = sp.cols ? r.x0 : r.y0, l1 = sp.cols ? r.x1 : r.y1;
+    for (int32_t l = l0; l <= l1; ++l) {
+        // the line's sample of k_sample_tables
+        const float s = sp.cols ? (static_cast<float>(l) - vd.vx + 0.5f) * 2.0f / vd.vw - 1.0f
+                                : (static_cast<float>(l) - vd.vy + 0.5f) * 2.0f / vd.vh - 1.0f;
+        int32_t lo, hi;
+        pdr::line_span(sp, s, alpha, beta, sp.cols ? r.y0 : r.x0, sp.cols ? r.y1 : r.x1, &lo, &hi);
+        for (int32_t a = lo; a <= hi; ++a) {
+            const int32_t x = sp.cols ? l : a, y = sp.cols ? a : l;
+            mask[static_cast<size_t>(y) * width + x] = 1;
+        }
+    }
+    return 1;
+}

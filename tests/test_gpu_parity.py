@@ -156,6 +156,29 @@ def test_rasterize_backward(rasterizer, case):
     bits_equal(g, ref)
 
 
+@pytest.mark.parametrize("variant", ["moved", "random_ids", "other_viewport"])
+def test_rasterize_backward_foreign_raster(rasterizer, variant):
+    """A raster that the forward did not produce for these positions: pixels may name
+    triangles that do not cover them.  The reference's backward loop only compares ids
+    inside each triangle's box (rasterize_backward.hlsl:88-100), so those pixels count."""
+    pos, idx, res, vp, cull = _case("soup_2")
+    rng = np.random.default_rng(21)
+    if variant == "moved":
+        moved = pos + rng.normal(0, 0.02, pos.shape).astype(np.float32) * pos[:, 3:4]
+        raster, _, _ = oracle.rasterize_forward(moved, idx, res, 0, False, vp)
+    elif variant == "random_ids":
+        raster, _, _ = oracle.rasterize_forward(pos, idx, res, 0, False, vp)
+        ids = rng.integers(-2, idx.shape[0] + 3, size=res).astype(np.float32)
+        ids[rng.random(res) < 0.2] += 0.5  # half-way ids round to even
+        raster[..., 3] = ids
+    else:
+        raster, _, _ = oracle.rasterize_forward(pos, idx, res, 0, False, (5.0, -3.0, 120.0, 90.0))
+    grad = rng.standard_normal(raster.shape).astype(np.float32)
+    g = rasterizer.backward(pos, idx, raster, grad, vp)
+    ref = oracle.rasterize_backward(pos, idx, raster, grad, vp)
+    bits_equal(g, ref)
+
+
 @pytest.mark.parametrize("attrs", [1, 2, 3, 8, 11])
 def test_interpolate(rasterizer, attrs):
     s = scenes.chain_scene(level=4, res=(160, 192), tex_size=32, seed=attrs)
