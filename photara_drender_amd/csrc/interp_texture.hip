@@ -8,8 +8,8 @@
 //  * interpolation: grad_raster is per pixel (interpolate_grad_raster.hlsl, exact);
 //    the attribute gradient (interpolate_grad_attributes.hlsl loops over every
 //    pixel for every (vertex, attribute): O(V*A*P)) becomes 3*A terms per pixel,
-//    summed per triangle in an LDS hash table over a 16x16 tile and flushed with
-//    float atomics;
+//    summed per vertex in an LDS hash table over a 64x64 tile and flushed with
+//    one float atomic per (vertex, attribute) of the tile;
 //  * texture: grad_uv is per pixel (texture_grad_uv.hlsl, exact); the texel
 //    gradient (texture_grad_texture.hlsl over the CPU-built CSR of
 //    src/rasterizer.cpp:187-258) becomes 4*C atomic terms per pixel, with the
@@ -76,120 +76,210 @@ __global__ __launch_bounds__(kBlock) void k_interp_fwd(PixelTable pt, const floa
     }
 }
 
-constexpr uint32_t kSlots = 256;        // <= pixels per tile, so probing always ends
-constexpr uint32_t kMaxHashedAttrs = 8;  // 3*A accumulators per slot
+constexpr uint32_t kVertexSlots = 1024;  // vertex accumulators per tile
+constexpr uint32_t kMaxHashedAttrs = 8;  // attributes per accumulator
 
-__device__ __forceinline__ uint32_t hash_slot(uint32_t* keys, uint32_t key) {
-    uint32_t h = (key * 2654435761u) >> 24;
-    while (true) {
-        const uint32_t old = atomicCAS(&keys[h], 0u, key);
-        if (old == 0u || old == key) return h;
-        h = (h + 1) & (kSlots - 1);
+// Slot of vertex key (index + 1) in the tile's table (linear probing, at most
+// 32 probes), or kVertexSlots when none is free along the probe sequence.
+__device__ __forceinline__ uint32_t vertex_slot(uint32_t* keys, uint32_t key) {
+    uint32_t h = (key * 2654435761u) >> 22;
+#pragma unroll 1
+    for (int probe = 0; probe < 32; ++probe) {
+        const uint32_t cur = __hip_atomic_load(&keys[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (cur == key) return h;
+        if (cur == 0u) {
+            const uint32_t old = atomicCAS(&keys[h], 0u, key);
+            if (old == 0u || old == key) return h;
+        }
+        h = (h + 1) & (kVertexSlots - 1);
+    }
+    return kVertexSlots;
+}
+
+// Adds a vertex's accumulated terms into the tile table (or global memory when
+// the table has no free slot on its probe sequence).
+__device__ __forceinline__ void flush_vertex(uint32_t* keys, float* acc, float* __restrict__ grad_attr, uint32_t A,
+                                             uint32_t v, const float* val) {
+    const uint32_t slot = vertex_slot(keys, v + 1u);
+    if (slot < kVertexSlots) {
+        float* dst = acc + slot * A;
+#pragma unroll
+        for (uint32_t a = 0; a < kMaxHashedAttrs; ++a)
+            if (a < A) atomicAdd(&dst[a], val[a]);
+    } else {
+        float* dst = grad_attr + static_cast<size_t>(v) * A;
+#pragma unroll
+        for (uint32_t a = 0; a < kMaxHashedAttrs; ++a)
+            if (a < A) unsafeAtomicAdd(&dst[a], val[a]);
     }
 }
 
-// One 16x16 tile per workgroup.  grad_raster is written per pixel; the
-// attribute gradient terms b_c * g[p,a] of the pixel's triangle accumulate in
-// LDS per triangle, then one atomic per (triangle corner, attribute).
-// kA > 0: compile-time attribute count with vector loads (kA % 4 == 0).
+// One kGradTile x kGradTile tile per workgroup (A <= kMaxHashedAttrs); each
+// wave walks a 16-row band, one column per lane, top to bottom.
+// grad_raster is written per pixel (interpolate_grad_raster.hlsl, exact).  The
+// attribute-gradient terms b_c * g[p, a] (interpolate_grad_attributes.hlsl:
+// 97-110) accumulate in registers per vertex of the lane's current triangle;
+// consecutive pixels of a column mostly belong to the same or an adjacent
+// triangle, so a vertex's sum leaves the registers only when the walk moves
+// to a triangle without it.  It then goes into a per-tile LDS table keyed by
+// vertex, and the table into global memory with one float atomic per
+// (vertex, attribute) of the tile.  kA = 8: vector loads (16-byte aligned rows).
 template <int kA>
 __global__ __launch_bounds__(kBlock) void k_interp_bwd(PixelTable pt, const float* __restrict__ attr, uint32_t V,
                                                        uint32_t A_rt, const uint32_t* __restrict__ idx, uint32_t T,
                                                        float* __restrict__ grad_attr) {
-    __shared__ uint32_t keys[kSlots];
-    __shared__ float acc[kSlots * 3 * kMaxHashedAttrs];
+    constexpr uint32_t kNone = 0xFFFFFFFFu;
+    constexpr uint32_t kM = kMaxHashedAttrs;
+    __shared__ uint32_t keys[kVertexSlots];
+    __shared__ float acc[kVertexSlots * kMaxHashedAttrs];
     const uint32_t A = kA > 0 ? static_cast<uint32_t>(kA) : A_rt;
-    const bool hashed = A <= kMaxHashedAttrs;
-    const uint32_t stride = 3 * A;
-    if (hashed) {
-        for (uint32_t s = threadIdx.x; s < kSlots; s += kBlock) keys[s] = 0u;
-        for (uint32_t s = threadIdx.x; s < kSlots * stride; s += kBlock) acc[s] = 0.0f;
-        __syncthreads();
-    }
+    for (uint32_t s = threadIdx.x; s < kVertexSlots; s += kBlock) keys[s] = 0u;
+    for (uint32_t s = threadIdx.x; s < kVertexSlots * A; s += kBlock) acc[s] = 0.0f;
+    __syncthreads();
     const uint32_t g = xcd_remap(blockIdx.x, gridDim.x);
     const PixelView& pv = pt.v[find_tile_view(pt, g)];
     const uint32_t local = g - pv.tile_base;
-    const uint32_t px = (local % pv.tiles_x) * kTile + (threadIdx.x & 15u);
-    const uint32_t py = (local / pv.tiles_x) * kTile + (threadIdx.x >> 4);
-    if (px < pv.width && py < pv.height) {
+    const uint32_t px = (local % pv.tiles_x) * kGradTile + (threadIdx.x & 63u);
+    const uint32_t band = (local / pv.tiles_x) * kGradTile + (threadIdx.x >> 6) * (kGradTile / (kBlock / 64));
+    uint32_t cur = 0;                   // current triangle id (raster encoding), 0: none
+    uint32_t ev[3] = {kNone, kNone, kNone};  // vertex of each accumulator = corner of cur
+    float ea[3][kM];                    // per-corner accumulated terms
+    float av[3][kM];                    // attribute rows of cur's corners (for grad_raster)
+    bool in_range = false;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (uint32_t a = 0; a < kM; ++a) ea[c][a] = av[c][a] = 0.0f;
+    for (uint32_t row = 0; row < kGradTile / (kBlock / 64); ++row) {
+        const uint32_t py = band + row;
+        if (px >= pv.width || py >= pv.height) break;
         const size_t p = static_cast<size_t>(py) * pv.width + px;
         const float4 r = reinterpret_cast<const float4*>(pv.raster)[p];
         const uint32_t id = decode_id(r.w);
-        const float* gin = pv.in0 + p * A;
         float gx = 0.0f, gy = 0.0f;
         if (id != 0 && id <= T) {
-            const uint32_t i0 = idx[(id - 1) * 3 + 0], i1 = idx[(id - 1) * 3 + 1], i2 = idx[(id - 1) * 3 + 2];
-            const bool in_range = i0 < V && i1 < V && i2 < V;
+            if (id != cur) {
+                uint32_t iv[3] = {idx[(id - 1) * 3 + 0], idx[(id - 1) * 3 + 1], idx[(id - 1) * 3 + 2]};
+                in_range = iv[0] < V && iv[1] < V && iv[2] < V;
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+                    if (iv[c] >= V) iv[c] = kNone;  // never matches (interpolate_grad_attributes.hlsl)
+                // keep each accumulator whose vertex the new triangle has (each used once)
+                int src[3] = {-1, -1, -1};
+                uint32_t used = 0;
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+#pragma unroll
+                    for (int e = 0; e < 3; ++e)
+                        if (src[c] < 0 && !((used >> e) & 1u) && ev[e] != kNone && ev[e] == iv[c]) {
+                            src[c] = e;
+                            used |= 1u << e;
+                        }
+#pragma unroll
+                for (int e = 0; e < 3; ++e)
+                    if (!((used >> e) & 1u) && ev[e] != kNone) flush_vertex(keys, acc, grad_attr, A, ev[e], ea[e]);
+                float na[3][kM];
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+#pragma unroll
+                    for (uint32_t a = 0; a < kM; ++a)
+                        na[c][a] = src[c] == 0 ? ea[0][a] : (src[c] == 1 ? ea[1][a] : (src[c] == 2 ? ea[2][a] : 0.0f));
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    ev[c] = iv[c];
+#pragma unroll
+                    for (uint32_t a = 0; a < kM; ++a) ea[c][a] = na[c][a];
+                }
+                if (in_range) {
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        const float* row_c = attr + static_cast<size_t>(iv[c]) * A;
+                        if constexpr (kA == 8) {
+                            const float4 x0 = reinterpret_cast<const float4*>(row_c)[0];
+                            const float4 x1 = reinterpret_cast<const float4*>(row_c)[1];
+                            av[c][0] = x0.x; av[c][1] = x0.y; av[c][2] = x0.z; av[c][3] = x0.w;
+                            av[c][4] = x1.x; av[c][5] = x1.y; av[c][6] = x1.z; av[c][7] = x1.w;
+                        } else {
+#pragma unroll
+                            for (uint32_t a = 0; a < kM; ++a)
+                                if (a < A) av[c][a] = row_c[a];
+                        }
+                    }
+                }
+                cur = id;
+            }
+            float og[kM];
+            const float* gin = pv.in0 + p * A;
+            if constexpr (kA == 8) {
+                const float4 g0 = reinterpret_cast<const float4*>(gin)[0];
+                const float4 g1 = reinterpret_cast<const float4*>(gin)[1];
+                og[0] = g0.x; og[1] = g0.y; og[2] = g0.z; og[3] = g0.w;
+                og[4] = g1.x; og[5] = g1.y; og[6] = g1.z; og[7] = g1.w;
+            } else {
+#pragma unroll
+                for (uint32_t a = 0; a < kM; ++a) og[a] = a < A ? gin[a] : 0.0f;
+            }
             const float u = r.x, v = r.y;
             const float w = 1.0f - u - v;
-            if constexpr (kA > 0) {
-                // vector path; out-of-range vertices contribute nothing (flush skips them)
-                float og[kA], a0[kA], a1[kA], a2[kA];
-                const float4 zero4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 #pragma unroll
-                for (int k = 0; k < kA / 4; ++k) {
-                    const float4 gv = reinterpret_cast<const float4*>(gin)[k];
-                    const float4 v0 = in_range ? reinterpret_cast<const float4*>(attr + static_cast<size_t>(i0) * kA)[k] : zero4;
-                    const float4 v1 = in_range ? reinterpret_cast<const float4*>(attr + static_cast<size_t>(i1) * kA)[k] : zero4;
-                    const float4 v2 = in_range ? reinterpret_cast<const float4*>(attr + static_cast<size_t>(i2) * kA)[k] : zero4;
-                    og[4 * k] = gv.x; og[4 * k + 1] = gv.y; og[4 * k + 2] = gv.z; og[4 * k + 3] = gv.w;
-                    a0[4 * k] = v0.x; a0[4 * k + 1] = v0.y; a0[4 * k + 2] = v0.z; a0[4 * k + 3] = v0.w;
-                    a1[4 * k] = v1.x; a1[4 * k + 1] = v1.y; a1[4 * k + 2] = v1.z; a1[4 * k + 3] = v1.w;
-                    a2[4 * k] = v2.x; a2[4 * k + 1] = v2.y; a2[4 * k + 2] = v2.z; a2[4 * k + 3] = v2.w;
+            for (uint32_t a = 0; a < kM; ++a) {
+                if (a >= A) break;
+                if (in_range) {
+                    // interpolate_grad_raster.hlsl:153-161
+                    gx += og[a] * (av[0][a] - av[2][a]);
+                    gy += og[a] * (av[1][a] - av[2][a]);
                 }
-                const uint32_t slot = hash_slot(keys, id);
-#pragma unroll
-                for (int a = 0; a < kA; ++a) {
-                    if (in_range) {
-                        gx += og[a] * (a0[a] - a2[a]);
-                        gy += og[a] * (a1[a] - a2[a]);
-                    }
-                    atomicAdd(&acc[slot * stride + a], u * og[a]);
-                    atomicAdd(&acc[slot * stride + kA + a], v * og[a]);
-                    atomicAdd(&acc[slot * stride + 2 * kA + a], w * og[a]);
-                }
-            } else {
-                uint32_t slot = 0;
-                if (hashed) slot = hash_slot(keys, id);
-                for (uint32_t a = 0; a < A; ++a) {
-                    const float og = gin[a];
-                    if (in_range) {
-                        // interpolate_grad_raster.hlsl:153-161
-                        const float a0 = attr[static_cast<size_t>(i0) * A + a];
-                        const float a1 = attr[static_cast<size_t>(i1) * A + a];
-                        const float a2 = attr[static_cast<size_t>(i2) * A + a];
-                        gx += og * (a0 - a2);
-                        gy += og * (a1 - a2);
-                    }
-                    // interpolate_grad_attributes.hlsl:97-110 (vertex indices >= V never match)
-                    const float t0 = u * og, t1 = v * og, t2 = w * og;
-                    if (hashed) {
-                        atomicAdd(&acc[slot * stride + a], t0);
-                        atomicAdd(&acc[slot * stride + A + a], t1);
-                        atomicAdd(&acc[slot * stride + 2 * A + a], t2);
-                    } else {
-                        if (i0 < V) unsafeAtomicAdd(&grad_attr[static_cast<size_t>(i0) * A + a], t0);
-                        if (i1 < V) unsafeAtomicAdd(&grad_attr[static_cast<size_t>(i1) * A + a], t1);
-                        if (i2 < V) unsafeAtomicAdd(&grad_attr[static_cast<size_t>(i2) * A + a], t2);
-                    }
-                }
+                ea[0][a] += u * og[a];
+                ea[1][a] += v * og[a];
+                ea[2][a] += w * og[a];
             }
         }
         reinterpret_cast<float4*>(pv.out1)[p] = make_float4(gx, gy, 0.0f, 0.0f);
     }
-    if (!hashed) return;
+#pragma unroll
+    for (int e = 0; e < 3; ++e)
+        if (ev[e] != kNone) flush_vertex(keys, acc, grad_attr, A, ev[e], ea[e]);
     __syncthreads();
-    for (uint32_t s = threadIdx.x; s < kSlots; s += kBlock) {
+    for (uint32_t s = threadIdx.x; s < kVertexSlots; s += kBlock) {
         const uint32_t key = keys[s];
         if (!key) continue;
-        const uint32_t t = key - 1;
-        for (int c = 0; c < 3; ++c) {
-            const uint32_t vi = idx[t * 3 + c];
-            if (vi >= V) continue;
-            for (uint32_t a = 0; a < A; ++a)
-                unsafeAtomicAdd(&grad_attr[static_cast<size_t>(vi) * A + a], acc[s * stride + c * A + a]);
+        float* dst = grad_attr + static_cast<size_t>(key - 1u) * A;
+        for (uint32_t a = 0; a < A; ++a) unsafeAtomicAdd(&dst[a], acc[s * A + a]);
+    }
+}
+
+// A > kMaxHashedAttrs: per-pixel float atomics.
+__global__ __launch_bounds__(kBlock) void k_interp_bwd_direct(PixelTable pt, const float* __restrict__ attr,
+                                                              uint32_t V, uint32_t A, const uint32_t* __restrict__ idx,
+                                                              uint32_t T, float* __restrict__ grad_attr) {
+    const uint32_t gp = blockIdx.x * kBlock + threadIdx.x;
+    if (gp >= pt.total_pixels) return;
+    const PixelView& pv = pt.v[find_pixel_view(pt, gp)];
+    const uint32_t p = gp - pv.pixel_base;
+    const float4 r = reinterpret_cast<const float4*>(pv.raster)[p];
+    const uint32_t id = decode_id(r.w);
+    const float* gin = pv.in0 + static_cast<size_t>(p) * A;
+    float gx = 0.0f, gy = 0.0f;
+    if (id != 0 && id <= T) {
+        const uint32_t i0 = idx[(id - 1) * 3 + 0], i1 = idx[(id - 1) * 3 + 1], i2 = idx[(id - 1) * 3 + 2];
+        const bool in_range = i0 < V && i1 < V && i2 < V;
+        const float u = r.x, v = r.y;
+        const float w = 1.0f - u - v;
+        for (uint32_t a = 0; a < A; ++a) {
+            const float og = gin[a];
+            if (in_range) {
+                const float a0 = attr[static_cast<size_t>(i0) * A + a];
+                const float a1 = attr[static_cast<size_t>(i1) * A + a];
+                const float a2 = attr[static_cast<size_t>(i2) * A + a];
+                gx += og * (a0 - a2);
+                gy += og * (a1 - a2);
+            }
+            if (i0 < V) unsafeAtomicAdd(&grad_attr[static_cast<size_t>(i0) * A + a], u * og);
+            if (i1 < V) unsafeAtomicAdd(&grad_attr[static_cast<size_t>(i1) * A + a], v * og);
+            if (i2 < V) unsafeAtomicAdd(&grad_attr[static_cast<size_t>(i2) * A + a], w * og);
         }
     }
+    reinterpret_cast<float4*>(pv.out1)[p] = make_float4(gx, gy, 0.0f, 0.0f);
 }
 
 struct Bilinear {
@@ -363,8 +453,10 @@ hipError_t interp_backward(const PixelTable& pt, const float* attr, uint32_t V, 
     for (uint32_t k = 0; k < pt.n; ++k) aligned &= (reinterpret_cast<uintptr_t>(pt.v[k].in0) % 16) == 0;
     if (A == 8 && aligned)
         k_interp_bwd<8><<<pt.total_tiles, kBlock, 0, st>>>(pt, attr, V, A, idx, T, grad_attr);
-    else
+    else if (A <= kMaxHashedAttrs)
         k_interp_bwd<0><<<pt.total_tiles, kBlock, 0, st>>>(pt, attr, V, A, idx, T, grad_attr);
+    else
+        k_interp_bwd_direct<<<pixel_grid(pt.total_pixels), kBlock, 0, st>>>(pt, attr, V, A, idx, T, grad_attr);
     return hipGetLastError();
 }
 

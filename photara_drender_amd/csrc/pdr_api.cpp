@@ -388,10 +388,10 @@ static pdr::PixelTable pixel_table_many(const pdr_interp_view* views, uint32_t n
         v.out1 = backward ? in.grad_raster : nullptr;
         v.width = in.width;
         v.height = in.height;
-        v.tiles_x = (in.width + pdr::kTile - 1) / pdr::kTile;
+        v.tiles_x = (in.width + pdr::kGradTile - 1) / pdr::kGradTile;
         v.tile_base = tiles;
         v.pixel_base = pixels;
-        const uint64_t nt = tiles + static_cast<uint64_t>(v.tiles_x) * ((in.height + pdr::kTile - 1) / pdr::kTile);
+        const uint64_t nt = tiles + static_cast<uint64_t>(v.tiles_x) * ((in.height + pdr::kGradTile - 1) / pdr::kGradTile);
         const uint64_t np = pixels + static_cast<uint64_t>(in.width) * in.height;
         if (nt > 0x7FFFFFFFu || np > 0xFFFFFFFFu) raise(PDR_OVERFLOW, "Interpolation pixel count exceeds backend limits");
         tiles = static_cast<uint32_t>(nt);

@@ -300,10 +300,10 @@ inline pdr::PixelTable pixel_table_one(const float* raster, const float* in0, co
     v.out1 = out1;
     v.width = W;
     v.height = H;
-    v.tiles_x = (W + pdr::kTile - 1) / pdr::kTile;
+    v.tiles_x = (W + pdr::kGradTile - 1) / pdr::kGradTile;
     v.tile_base = 0;
     v.pixel_base = 0;
-    t.total_tiles = v.tiles_x * ((H + pdr::kTile - 1) / pdr::kTile);
+    t.total_tiles = v.tiles_x * ((H + pdr::kGradTile - 1) / pdr::kGradTile);
     t.total_pixels = W * H;
     return t;
 }

@@ -40,8 +40,8 @@ struct PixelView {
     float* out0;           // kernel-specific output [H,W,*]
     float* out1;           // kernel-specific output [H,W,*]
     uint32_t width, height;
-    uint32_t tiles_x;
-    uint32_t tile_base;
+    uint32_t tiles_x;     // kGradTile x kGradTile tiles per row (interpolation backward)
+    uint32_t tile_base;   // first such tile of this view
     uint32_t pixel_base;
 };
 

@@ -92,6 +92,15 @@ __device__ __forceinline__ uint32_t find_view(const ViewTable& vt, uint32_t g) {
     return k;
 }
 
+// Pixel-centre NDC of column x / row y (rasterize_forward.hlsl:55-57); the
+// kernels evaluate it inline (a correctly rounded division) rather than load it.
+__device__ __forceinline__ float sample_x(const ViewDesc& vd, int32_t x) {
+    return (static_cast<float>(x) - vd.vx + 0.5f) * 2.0f / vd.vw - 1.0f;
+}
+__device__ __forceinline__ float sample_y(const ViewDesc& vd, int32_t y) {
+    return (static_cast<float>(y) - vd.vy + 0.5f) * 2.0f / vd.vh - 1.0f;
+}
+
 // rasterize_forward.hlsl:55-57 for every column and row of every view.
 __global__ void k_sample_tables(ViewTable vt, float* __restrict__ sx, float* __restrict__ sy) {
     const ViewDesc& vd = vt.v[blockIdx.y];
@@ -155,25 +164,51 @@ __device__ __forceinline__ void bc_spans(const LineSpans& s, int j, LineSpans& o
     o.cols = static_cast<uint32_t>(bci(static_cast<int32_t>(s.cols), j));
 }
 
-// Inclusive prefix sum over the 64 lanes (all lanes active).
-__device__ __forceinline__ int32_t wave_incl_scan(int32_t v, uint32_t lane) {
-#pragma unroll
-    for (int s = 1; s < 64; s <<= 1) {
-        const int32_t o = __shfl_up(v, s, 64);
-        if (lane >= static_cast<uint32_t>(s)) v += o;
-    }
+template <int Ctrl, int RowMask>
+__device__ __forceinline__ int32_t dpp_move(int32_t old, int32_t v) {
+    return __builtin_amdgcn_update_dpp(old, v, Ctrl, RowMask, 0xf, false);
+}
+
+// Inclusive prefix sum over the 64 lanes (all lanes active): Hillis-Steele
+// within each 16-lane row (DPP row_shr 1, 2, 4, 8), then the row totals
+// (row_bcast 15 into rows 1 and 3, row_bcast 31 into rows 2 and 3).
+__device__ __forceinline__ int32_t wave_incl_scan(int32_t v) {
+    v += dpp_move<0x111, 0xf>(0, v);
+    v += dpp_move<0x112, 0xf>(0, v);
+    v += dpp_move<0x114, 0xf>(0, v);
+    v += dpp_move<0x118, 0xf>(0, v);
+    v += dpp_move<0x142, 0xa>(0, v);
+    v += dpp_move<0x143, 0xc>(0, v);
     return v;
 }
 
-// Smallest lane l with incl[l] > m (incl non-decreasing; all lanes active).
-__device__ __forceinline__ int32_t wave_find(int32_t incl, int32_t m) {
-    int32_t pos = 0;
-#pragma unroll
-    for (int s = 32; s >= 1; s >>= 1) {
-        const int32_t v = __shfl(incl, pos + s - 1, 64);
-        if (v <= m) pos += s;
-    }
-    return pos;
+// Inclusive prefix maximum over the 64 lanes (values >= -1; all lanes active).
+__device__ __forceinline__ int32_t wave_incl_max(int32_t v) {
+    v = max(v, dpp_move<0x111, 0xf>(-1, v));
+    v = max(v, dpp_move<0x112, 0xf>(-1, v));
+    v = max(v, dpp_move<0x114, 0xf>(-1, v));
+    v = max(v, dpp_move<0x118, 0xf>(-1, v));
+    v = max(v, dpp_move<0x142, 0xa>(-1, v));
+    v = max(v, dpp_move<0x143, 0xc>(-1, v));
+    return v;
+}
+
+// The lanes' segments [incl - len, incl) concatenated in lane order: the lane
+// owning position pc + lane (meaningful when it is below the total).  The
+// owner of pc is the number of lanes ending at or before pc; every lane whose
+// non-empty segment starts inside [pc, pc + 64) writes its id at its start in
+// the wave's 64 LDS slots, and a prefix maximum carries it forward.
+__device__ __forceinline__ int32_t chunk_owner(int32_t incl, int32_t len, int32_t pc, uint32_t lane,
+                                               int32_t* __restrict__ slot) {
+    const int32_t first = static_cast<int32_t>(__popcll(__ballot(incl <= pc)));
+    slot[lane] = -1;
+    wave_lds_sync();
+    const int32_t s = incl - len - pc;
+    if (len > 0 && s >= 0 && s < 64) slot[s] = static_cast<int32_t>(lane);
+    wave_lds_sync();
+    const int32_t v = wave_incl_max(slot[lane]);
+    wave_lds_sync();
+    return v > first ? v : first;
 }
 
 // ------------------------------------------------------------------ forward
@@ -197,34 +232,22 @@ __device__ __forceinline__ bool setup_forward_tri(const ViewDesc& vd, const uint
     return setup_screen(vd, p0, p1, p2, cull, allow_cols, r, sp);
 }
 
-// A triangle of a wave's batch, in LDS: corners, screen coordinates, spans and
-// the range of its lines in the batch's concatenated line list.
-struct alignas(16) TriRec {
-    float4 p0, p1, p2;
-    float s0x, s0y, s1x, s1y;
-    float s2x, s2y, sgn;
-    uint32_t t;
-    float u[3];
-    uint32_t lower;
-    float v[3];
-    uint32_t upper;
-    int32_t a0, a1;  // span range along the lines
-    int32_t lbase;   // line coordinate of list entry f is lbase + f
-    uint32_t cols;   // lines are columns
+// The triangles of a block's four 64-triangle batches in LDS, one array per
+// field group (lanes reading different triangles hit different banks).
+struct FwdRecs {
+    float4 p0[kBlock], p1[kBlock], p2[kBlock];
+    float4 sa[kBlock];  // s0x, s0y, s1x, s1y
+    float4 sb[kBlock];  // s2x, s2y, sgn, triangle index bits
+    float4 su[kBlock];  // span coefficients u0..u2, lower-bound mask bits
+    float4 sv[kBlock];  // span coefficients v0..v2, upper-bound mask bits
+    int4 ln[kBlock];    // span range along the lines a0, a1; line base; lines are columns
+    int32_t slot[kBlock];
 };
 
-__device__ __forceinline__ LineSpans rec_spans(const TriRec& q) {
-    LineSpans s;
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-        s.u[i] = q.u[i];
-        s.v[i] = q.v[i];
-    }
-    s.lower = q.lower;
-    s.upper = q.upper;
-    s.cols = q.cols;
-    return s;
-}
+struct RecTri {  // what test_pixel reads
+    float4 p0, p1, p2;
+    float s0x, s0y, s1x, s1y, s2x, s2y, sgn;
+};
 
 // One 64-triangle batch per wave.  Each lane sets one triangle up; the lines
 // (rows, or columns for triangles taller than wide) of the batch's triangles
@@ -238,84 +261,97 @@ __global__ __launch_bounds__(kBlock) void k_raster_small(ViewTable vt, const uin
                                                          const float* __restrict__ syt,
                                                          unsigned long long* __restrict__ keys,
                                                          uint4* __restrict__ queue, uint32_t* __restrict__ counters) {
-    __shared__ TriRec recs_all[kBlock];
+    __shared__ FwdRecs R;
     const uint32_t view = blockIdx.y;
     const ViewDesc& vd = vt.v[view];
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
-    TriRec* recs = recs_all + wave * 64;
+    const uint32_t w0 = wave * 64;
+    int32_t* slot = R.slot + w0;
     unsigned long long* kbase = keys + vd.pixel_base;
-    const float* srow = syt + vd.row_base;
-    const float* scol = sxt + vd.col_base;
     const float ax[2] = {span_alpha(vd, 0), span_alpha(vd, 1)};
     const float bx[2] = {span_beta(vd, 0), span_beta(vd, 1)};
     const uint32_t stride = gridDim.x * kBlock;
     for (uint32_t batch = (blockIdx.x * (kBlock / 64) + wave) * 64; batch < T; batch += stride) {
         const uint32_t t = batch + lane;
-        ScreenTri r;
-        LineSpans sp;
-        const bool ok = t < T && setup_forward_tri(vd, idx, t, V, cull, true, r, sp);
         int32_t lines = 0;
-        if (ok && r.work > kWaveWork) {
-            const uint32_t slabs = static_cast<uint32_t>(r.y1 - r.y0) / kSlabRows + 1u;
-            queue[atomicAdd(&counters[0], 1u)] = make_uint4(view, t, slabs, 0u);
-        } else if (ok) {
-            lines = sp.cols ? r.x1 - r.x0 + 1 : r.y1 - r.y0 + 1;
-        }
-        const int32_t incl_l = wave_incl_scan(lines, lane);
-        const int32_t nlines = __builtin_amdgcn_readlane(incl_l, 63);
-        if (lines > 0) {
-            TriRec& q = recs[lane];
-            q.p0 = r.p0;
-            q.p1 = r.p1;
-            q.p2 = r.p2;
-            q.s0x = r.s0x;
-            q.s0y = r.s0y;
-            q.s1x = r.s1x;
-            q.s1y = r.s1y;
-            q.s2x = r.s2x;
-            q.s2y = r.s2y;
-            q.sgn = r.sgn;
-            q.t = t;
-#pragma unroll
-            for (int i = 0; i < 3; ++i) {
-                q.u[i] = sp.u[i];
-                q.v[i] = sp.v[i];
+        {
+            ScreenTri r;
+            LineSpans sp;
+            const bool ok = t < T && setup_forward_tri(vd, idx, t, V, cull, true, r, sp);
+            if (ok && r.work > kWaveWork) {
+                const uint32_t slabs = static_cast<uint32_t>(r.y1 - r.y0) / kSlabRows + 1u;
+                queue[atomicAdd(&counters[0], 1u)] = make_uint4(view, t, slabs, 0u);
+            } else if (ok) {
+                lines = sp.cols ? r.x1 - r.x0 + 1 : r.y1 - r.y0 + 1;
+                const uint32_t i = w0 + lane;
+                R.p0[i] = r.p0;
+                R.p1[i] = r.p1;
+                R.p2[i] = r.p2;
+                R.sa[i] = make_float4(r.s0x, r.s0y, r.s1x, r.s1y);
+                R.sb[i] = make_float4(r.s2x, r.s2y, r.sgn, __uint_as_float(t));
+                R.su[i] = make_float4(sp.u[0], sp.u[1], sp.u[2], __uint_as_float(sp.lower));
+                R.sv[i] = make_float4(sp.v[0], sp.v[1], sp.v[2], __uint_as_float(sp.upper));
+                R.ln[i] = make_int4(sp.cols ? r.y0 : r.x0, sp.cols ? r.y1 : r.x1, sp.cols ? r.x0 : r.y0,
+                                    static_cast<int32_t>(sp.cols));
             }
-            q.lower = sp.lower;
-            q.upper = sp.upper;
-            q.cols = sp.cols;
-            q.a0 = sp.cols ? r.y0 : r.x0;
-            q.a1 = sp.cols ? r.y1 : r.x1;
-            q.lbase = (sp.cols ? r.x0 : r.y0) - (incl_l - lines);
         }
+        const int32_t incl_l = wave_incl_scan(lines);
+        const int32_t nlines = __builtin_amdgcn_readlane(incl_l, 63);
+        if (lines > 0) R.ln[w0 + lane].z -= incl_l - lines;  // line coordinate = base + list position
         wave_lds_sync();
         for (int32_t lc = 0; lc < nlines; lc += 64) {
             const int32_t f = lc + static_cast<int32_t>(lane);
-            const int32_t o = wave_find(incl_l, f);
-            int32_t lo = 0, hi = -1, line = 0;
+            const int32_t o = chunk_owner(incl_l, lines, lc, lane, slot);
+            int32_t lo = 0, hi = -1, line = 0, cols = 0;
             float sl = 0.0f;
             if (f < nlines) {
-                const TriRec& q = recs[o];
-                line = q.lbase + f;
-                sl = q.cols ? scol[line] : srow[line];
-                line_span(rec_spans(q), sl, ax[q.cols], bx[q.cols], q.a0, q.a1, &lo, &hi);
+                const uint32_t i = w0 + static_cast<uint32_t>(o);
+                const int4 ln = R.ln[i];
+                const float4 su = R.su[i], sv = R.sv[i];
+                LineSpans sp;
+                sp.u[0] = su.x;
+                sp.u[1] = su.y;
+                sp.u[2] = su.z;
+                sp.lower = __float_as_uint(su.w);
+                sp.v[0] = sv.x;
+                sp.v[1] = sv.y;
+                sp.v[2] = sv.z;
+                sp.upper = __float_as_uint(sv.w);
+                sp.cols = static_cast<uint32_t>(ln.w);
+                cols = ln.w;
+                line = ln.z + f;
+                sl = cols ? sample_x(vd, line) : sample_y(vd, line);
+                line_span(sp, sl, ax[cols], bx[cols], ln.x, ln.y, &lo, &hi);
             }
             const int32_t len = hi >= lo ? hi - lo + 1 : 0;
-            const int32_t incl_p = wave_incl_scan(len, lane);
+            const int32_t incl_p = wave_incl_scan(len);
             const int32_t npix = __builtin_amdgcn_readlane(incl_p, 63);
             const int32_t aoff = lo - (incl_p - len);
             for (int32_t pc = 0; pc < npix; pc += 64) {
                 const int32_t m = pc + static_cast<int32_t>(lane);
-                const int32_t k = wave_find(incl_p, m);
+                const int32_t k = chunk_owner(incl_p, len, pc, lane, slot);
                 const int32_t a = __shfl(aoff, k, 64) + m;
                 const int32_t ln = __shfl(line, k, 64);
                 const int32_t oo = __shfl(o, k, 64);
+                const int32_t cc = __shfl(cols, k, 64);
                 const float sln = __shfl(sl, k, 64);
                 if (m < npix) {
-                    const TriRec& q = recs[oo];
-                    const int32_t x = q.cols ? ln : a, y = q.cols ? a : ln;
-                    const float sx = q.cols ? sln : scol[x], sy = q.cols ? srow[y] : sln;
-                    const unsigned long long key = test_pixel(q, q.t, sx, sy);
+                    const uint32_t i = w0 + static_cast<uint32_t>(oo);
+                    RecTri q;
+                    q.p0 = R.p0[i];
+                    q.p1 = R.p1[i];
+                    q.p2 = R.p2[i];
+                    const float4 sa = R.sa[i], sb = R.sb[i];
+                    q.s0x = sa.x;
+                    q.s0y = sa.y;
+                    q.s1x = sa.z;
+                    q.s1y = sa.w;
+                    q.s2x = sb.x;
+                    q.s2y = sb.y;
+                    q.sgn = sb.z;
+                    const int32_t x = cc ? ln : a, y = cc ? a : ln;
+                    const float sx = cc ? sln : sample_x(vd, x), sy = cc ? sample_y(vd, y) : sln;
+                    const unsigned long long key = test_pixel(q, __float_as_uint(sb.w), sx, sy);
                     if (key != ~0ull) key_min(&kbase[static_cast<size_t>(y) * vd.width + x], key);
                 }
             }
@@ -326,31 +362,29 @@ __global__ __launch_bounds__(kBlock) void k_raster_small(ViewTable vt, const uin
 
 // Rows [ya, yb] of a wave-uniform triangle with row spans, 64 rows at a time.
 __device__ __forceinline__ void wave_forward(const ScreenTri& r, const LineSpans& sp, uint32_t t, int32_t ya,
-                                             int32_t yb, const ViewDesc& vd, uint32_t lane,
+                                             int32_t yb, const ViewDesc& vd, uint32_t lane, int32_t* slot,
                                              const float* __restrict__ sxt, const float* __restrict__ syt,
                                              unsigned long long* __restrict__ kbase) {
-    const float* srow = syt + vd.row_base;
-    const float* scol = sxt + vd.col_base;
     const float alpha = span_alpha(vd, 0), beta = span_beta(vd, 0);
     for (int32_t y0 = ya; y0 <= yb; y0 += 64) {
         const int32_t y = y0 + static_cast<int32_t>(lane);
         int32_t lo = 0, hi = -1;
         float sy = 0.0f;
         if (y <= yb) {
-            sy = srow[y];
+            sy = sample_y(vd, y);
             line_span(sp, sy, alpha, beta, r.x0, r.x1, &lo, &hi);
         }
         const int32_t len = hi >= lo ? hi - lo + 1 : 0;
-        const int32_t incl = wave_incl_scan(len, lane);
+        const int32_t incl = wave_incl_scan(len);
         const int32_t total = __builtin_amdgcn_readlane(incl, 63);
         const int32_t xoff = lo - (incl - len);
         for (int32_t base = 0; base < total; base += 64) {
             const int32_t m = base + static_cast<int32_t>(lane);
-            const int32_t rl = wave_find(incl, m);
+            const int32_t rl = chunk_owner(incl, len, base, lane, slot);
             const int32_t x = __shfl(xoff, rl, 64) + m;
             const float syr = __shfl(sy, rl, 64);
             if (m < total) {
-                const unsigned long long key = test_pixel(r, t, scol[x], syr);
+                const unsigned long long key = test_pixel(r, t, sample_x(vd, x), syr);
                 if (key != ~0ull) key_min(&kbase[static_cast<size_t>(y0 + rl) * vd.width + x], key);
             }
         }
@@ -368,6 +402,7 @@ __global__ __launch_bounds__(kBlock) void k_raster_large(ViewTable vt, const uin
     __shared__ uint32_t nslab[kBlock];
     __shared__ unsigned long long before[kBlock];
     __shared__ unsigned long long batch_total;
+    __shared__ int32_t slots[kBlock];
     const uint32_t count = counters[0];
     const uint32_t waves = gridDim.x * (kBlock / 64);
     const uint32_t wid = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6), lane = threadIdx.x & 63u;
@@ -399,7 +434,7 @@ __global__ __launch_bounds__(kBlock) void k_raster_large(ViewTable vt, const uin
                 const int32_t ya = r.y0 + static_cast<int32_t>(s * kSlabRows);
                 const int32_t yb = ya + static_cast<int32_t>(kSlabRows) - 1 < r.y1 ? ya + static_cast<int32_t>(kSlabRows) - 1
                                                                                   : r.y1;
-                wave_forward(r, sp, q.y, ya, yb, vd, lane, sxt, syt, keys + vd.pixel_base);
+                wave_forward(r, sp, q.y, ya, yb, vd, lane, slots + (threadIdx.x & ~63u), sxt, syt, keys + vd.pixel_base);
             }
         }
         cum += batch_total;
@@ -568,17 +603,20 @@ __device__ __forceinline__ bool setup_backward(const ViewDesc& vd, const uint32_
     return g.ya <= g.yb && g.xa <= g.xb;
 }
 
-// A triangle's rows in its wave batch's concatenated row list.
-struct alignas(16) BwdRec {
-    float4 p[3];
-    float u[3];
-    uint32_t lower;
-    float v[3];
-    uint32_t upper;
-    int32_t xa, xb, lbase, pad;
-};
-
+// The flat-class triangles of a block's four batches in LDS (one array per
+// field group), the per-wave owner slots and the per-wave lists of matched
+// pixels' terms.
 constexpr uint32_t kList = 64;  // matched pixels buffered per wave before the per-triangle fold
+
+struct BwdRecs {
+    float4 p0[kBlock], p1[kBlock], p2[kBlock];
+    float4 su[kBlock];  // span coefficients u0..u2, lower-bound mask bits
+    float4 sv[kBlock];  // span coefficients v0..v2, upper-bound mask bits
+    int4 rg[kBlock];    // xa, xb, row base, unused
+    int32_t slot[kBlock];
+    float term[kBlock / 64][kList * 9];
+    uint32_t owner[kBlock / 64][kList];
+};
 
 // Lane `lane` adds, in list order, the buffered terms it owns (entries with
 // owner == lane are contiguous: the list is in batch order).
@@ -599,103 +637,149 @@ __device__ __forceinline__ void fold_list(const float* __restrict__ lterm, const
 }
 
 // Per-triangle corner gradients summed over the reference's bounding box in
-// its row-major order.  Each wave takes a batch of 64 triangles; the rows of
-// triangles with at most kFlatWork estimated pixels are concatenated and
-// walked 64 pixels at a time (as the forward); pixels naming their triangle
-// compute their terms and append them, in walk order, to a per-wave list,
-// which each triangle's lane then adds in order (the walk order restricted to
-// one triangle is the reference's row-major order).  Larger triangles (boxes
-// up to kWaveBox pixels) are walked by the whole wave, one at a time, adding
-// the contributing pixels' terms in ascending pixel order through
-// ballot + readlane; larger boxes are queued for k_corner_large.
-__global__ __launch_bounds__(kBlock) void k_corner_small(ViewTable vt, const uint32_t* __restrict__ idx, uint32_t T,
+// its row-major order.  Each wave takes a batch of 64 triangles.  Triangles
+// with more than kFlatWork estimated pixels (boxes up to kWaveBox pixels) are
+// walked by the whole wave, one at a time, adding the contributing pixels'
+// terms in ascending pixel order through ballot + readlane; larger boxes are
+// queued for k_corner_large.  The rows of the other triangles are
+// concatenated and walked 64 pixels at a time (as the forward); pixels naming
+// their triangle compute their terms and append them, in walk order, to a
+// per-wave list, which each triangle's lane then adds in order (the walk
+// order restricted to one triangle is the reference's row-major order).
+__global__ __launch_bounds__(kBlock, 4) void k_corner_small(ViewTable vt, const uint32_t* __restrict__ idx, uint32_t T,
                                                          uint32_t V, const float* __restrict__ sxt,
                                                          const float* __restrict__ syt,
                                                          const uint32_t* __restrict__ cid,
                                                          const uint32_t* __restrict__ view_flags,
                                                          float4* __restrict__ corner, uint4* __restrict__ queue,
                                                          uint32_t* __restrict__ counters) {
-    __shared__ BwdRec recs_all[kBlock];
-    __shared__ float lterm_all[kBlock / 64][kList * 9];
-    __shared__ uint32_t lown_all[kBlock / 64][kList];
+    __shared__ BwdRecs R;
     const uint32_t view = blockIdx.y;
     const ViewDesc& vd = vt.v[view];
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
-    BwdRec* recs = recs_all + wave * 64;
-    float* lterm = lterm_all[wave];
-    uint32_t* lown = lown_all[wave];
+    const uint32_t w0 = wave * 64;
+    int32_t* slot = R.slot + w0;
+    float* lterm = R.term[wave];
+    uint32_t* lown = R.owner[wave];
     const bool full = view_flags[view] != 0u;
     const float alpha = span_alpha(vd, 0), beta = span_beta(vd, 0);
-    const float* srow = syt + vd.row_base;
-    const float* scol = sxt + vd.col_base;
     const uint32_t* cv = cid + vd.pixel_base;
     const float2* gr = reinterpret_cast<const float2*>(vd.grad_raster);
     float4* cview = corner + static_cast<size_t>(view) * T * 3;
     const uint32_t stride = gridDim.x * kBlock;
     for (uint32_t batch = (blockIdx.x * (kBlock / 64) + wave) * 64; batch < T; batch += stride) {
         const uint32_t t = batch + lane;
-        BwdRegion g;
-        const bool ok = t < T && setup_backward(vd, idx, t, V, full, g);
         int32_t rows = 0;
-        bool deferred = false;
-        if (t < T && !ok) {
+        {
+            BwdRegion g;
+            const bool ok = t < T && setup_backward(vd, idx, t, V, full, g);
+            bool deferred = false;
+            if (t < T && !ok) {
 #pragma unroll
-            for (int c = 0; c < 3; ++c) cview[t * 3 + c] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        } else if (ok && g.box > kWaveBox) {
-            queue[atomicAdd(&counters[0], 1u)] = make_uint4(view, t, 0u, 0u);
-        } else if (ok && g.work > kFlatWork) {
-            deferred = true;
-        } else if (ok) {
-            rows = g.yb - g.ya + 1;
-        }
-        const int32_t incl_l = wave_incl_scan(rows, lane);
-        const int32_t nrows = __builtin_amdgcn_readlane(incl_l, 63);
-        if (rows > 0) {
-            BwdRec& q = recs[lane];
-            q.p[0] = g.p[0];
-            q.p[1] = g.p[1];
-            q.p[2] = g.p[2];
-#pragma unroll
-            for (int i = 0; i < 3; ++i) {
-                q.u[i] = g.sp.u[i];
-                q.v[i] = g.sp.v[i];
+                for (int c = 0; c < 3; ++c) cview[t * 3 + c] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            } else if (ok && g.box > kWaveBox) {
+                queue[atomicAdd(&counters[0], 1u)] = make_uint4(view, t, 0u, 0u);
+            } else if (ok && g.work > kFlatWork) {
+                deferred = true;
+            } else if (ok) {
+                rows = g.yb - g.ya + 1;
+                const uint32_t i = w0 + lane;
+                R.p0[i] = g.p[0];
+                R.p1[i] = g.p[1];
+                R.p2[i] = g.p[2];
+                R.su[i] = make_float4(g.sp.u[0], g.sp.u[1], g.sp.u[2], __uint_as_float(g.sp.lower));
+                R.sv[i] = make_float4(g.sp.v[0], g.sp.v[1], g.sp.v[2], __uint_as_float(g.sp.upper));
+                R.rg[i] = make_int4(g.xa, g.xb, g.ya, 0);
             }
-            q.lower = g.sp.lower;
-            q.upper = g.sp.upper;
-            q.xa = g.xa;
-            q.xb = g.xb;
-            q.lbase = g.ya - (incl_l - rows);
+            unsigned long long pending = __ballot(deferred);
+            while (pending) {
+                const int j = __ffsll(pending) - 1;
+                pending &= pending - 1;
+                BwdRegion q;
+                q.p[0] = bcf4(g.p[0], j);
+                q.p[1] = bcf4(g.p[1], j);
+                q.p[2] = bcf4(g.p[2], j);
+                bc_spans(g.sp, j, q.sp);
+                q.ya = bci(g.ya, j);
+                q.yb = bci(g.yb, j);
+                q.xa = bci(g.xa, j);
+                q.xb = bci(g.xb, j);
+                const uint32_t qt = batch + static_cast<uint32_t>(j);
+                float w[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+                for (int32_t y0 = q.ya; y0 <= q.yb; y0 += 64) {
+                    const int32_t y = y0 + static_cast<int32_t>(lane);
+                    int32_t lo = 0, hi = -1;
+                    float sy = 0.0f;
+                    if (y <= q.yb) {
+                        sy = sample_y(vd, y);
+                        line_span(q.sp, sy, alpha, beta, q.xa, q.xb, &lo, &hi);
+                    }
+                    const int32_t len = hi >= lo ? hi - lo + 1 : 0;
+                    const int32_t incl = wave_incl_scan(len);
+                    const int32_t total = __builtin_amdgcn_readlane(incl, 63);
+                    const int32_t xoff = lo - (incl - len);
+                    for (int32_t base = 0; base < total; base += 64) {
+                        const int32_t m = base + static_cast<int32_t>(lane);
+                        const int32_t rl = chunk_owner(incl, len, base, lane, slot);
+                        const int32_t x = __shfl(xoff, rl, 64) + m;
+                        const float syr = __shfl(sy, rl, 64);
+                        float t9[9];
+                        bool hit = false;
+                        if (m < total) {
+                            const size_t pix = static_cast<size_t>(y0 + rl) * vd.width + static_cast<size_t>(x);
+                            if (cv[pix] == qt + 1u) hit = pixel_terms(q.p, sample_x(vd, x), syr, gr[pix * 2], t9);
+                        }
+                        unsigned long long mk = __ballot(hit);
+                        while (mk) {
+                            const int src = __ffsll(mk) - 1;
+                            mk &= mk - 1;
+#pragma unroll
+                            for (int kk = 0; kk < 9; ++kk) w[kk] += bcf(t9[kk], src);
+                        }
+                    }
+                }
+                if (lane == 0)
+#pragma unroll
+                    for (int c = 0; c < 3; ++c)
+                        cview[qt * 3 + c] = make_float4(w[c * 3], w[c * 3 + 1], 0.0f, w[c * 3 + 2]);
+            }
         }
+        const int32_t incl_l = wave_incl_scan(rows);
+        const int32_t nrows = __builtin_amdgcn_readlane(incl_l, 63);
+        if (rows > 0) R.rg[w0 + lane].z -= incl_l - rows;  // row = base + list position
         float s[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
         uint32_t nlist = 0;
         wave_lds_sync();
         for (int32_t lc = 0; lc < nrows; lc += 64) {
             const int32_t f = lc + static_cast<int32_t>(lane);
-            const int32_t o = wave_find(incl_l, f);
+            const int32_t o = chunk_owner(incl_l, rows, lc, lane, slot);
             int32_t lo = 0, hi = -1, y = 0;
             float sy = 0.0f;
             if (f < nrows) {
-                const BwdRec& q = recs[o];
+                const uint32_t i = w0 + static_cast<uint32_t>(o);
+                const int4 rg = R.rg[i];
+                const float4 su = R.su[i], sv = R.sv[i];
                 LineSpans sp;
-#pragma unroll
-                for (int i = 0; i < 3; ++i) {
-                    sp.u[i] = q.u[i];
-                    sp.v[i] = q.v[i];
-                }
-                sp.lower = q.lower;
-                sp.upper = q.upper;
+                sp.u[0] = su.x;
+                sp.u[1] = su.y;
+                sp.u[2] = su.z;
+                sp.lower = __float_as_uint(su.w);
+                sp.v[0] = sv.x;
+                sp.v[1] = sv.y;
+                sp.v[2] = sv.z;
+                sp.upper = __float_as_uint(sv.w);
                 sp.cols = 0u;
-                y = q.lbase + f;
-                sy = srow[y];
-                line_span(sp, sy, alpha, beta, q.xa, q.xb, &lo, &hi);
+                y = rg.z + f;
+                sy = sample_y(vd, y);
+                line_span(sp, sy, alpha, beta, rg.x, rg.y, &lo, &hi);
             }
             const int32_t len = hi >= lo ? hi - lo + 1 : 0;
-            const int32_t incl_p = wave_incl_scan(len, lane);
+            const int32_t incl_p = wave_incl_scan(len);
             const int32_t npix = __builtin_amdgcn_readlane(incl_p, 63);
             const int32_t xoff = lo - (incl_p - len);
             for (int32_t pc = 0; pc < npix; pc += 64) {
                 const int32_t m = pc + static_cast<int32_t>(lane);
-                const int32_t k = wave_find(incl_p, m);
+                const int32_t k = chunk_owner(incl_p, len, pc, lane, slot);
                 const int32_t x = __shfl(xoff, k, 64) + m;
                 const int32_t yy = __shfl(y, k, 64);
                 const int32_t oo = __shfl(o, k, 64);
@@ -704,8 +788,11 @@ __global__ __launch_bounds__(kBlock) void k_corner_small(ViewTable vt, const uin
                 bool hit = false;
                 if (m < npix) {
                     const size_t pix = static_cast<size_t>(yy) * vd.width + static_cast<size_t>(x);
-                    if (cv[pix] == batch + static_cast<uint32_t>(oo) + 1u)
-                        hit = pixel_terms(recs[oo].p, scol[x], syy, gr[pix * 2], t9);
+                    if (cv[pix] == batch + static_cast<uint32_t>(oo) + 1u) {
+                        const uint32_t i = w0 + static_cast<uint32_t>(oo);
+                        const float4 pp[3] = {R.p0[i], R.p1[i], R.p2[i]};
+                        hit = pixel_terms(pp, sample_x(vd, x), syy, gr[pix * 2], t9);
+                    }
                 }
                 const unsigned long long bal = __ballot(hit);
                 const uint32_t cnt = static_cast<uint32_t>(__popcll(bal));
@@ -727,57 +814,6 @@ __global__ __launch_bounds__(kBlock) void k_corner_small(ViewTable vt, const uin
         if (rows > 0)
 #pragma unroll
             for (int c = 0; c < 3; ++c) cview[t * 3 + c] = make_float4(s[c * 3], s[c * 3 + 1], 0.0f, s[c * 3 + 2]);
-        unsigned long long pending = __ballot(deferred);
-        while (pending) {
-            const int j = __ffsll(pending) - 1;
-            pending &= pending - 1;
-            BwdRegion q;
-            q.p[0] = bcf4(g.p[0], j);
-            q.p[1] = bcf4(g.p[1], j);
-            q.p[2] = bcf4(g.p[2], j);
-            bc_spans(g.sp, j, q.sp);
-            q.ya = bci(g.ya, j);
-            q.yb = bci(g.yb, j);
-            q.xa = bci(g.xa, j);
-            q.xb = bci(g.xb, j);
-            const uint32_t qt = batch + static_cast<uint32_t>(j);
-            float w[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-            for (int32_t y0 = q.ya; y0 <= q.yb; y0 += 64) {
-                const int32_t y = y0 + static_cast<int32_t>(lane);
-                int32_t lo = 0, hi = -1;
-                float sy = 0.0f;
-                if (y <= q.yb) {
-                    sy = srow[y];
-                    line_span(q.sp, sy, alpha, beta, q.xa, q.xb, &lo, &hi);
-                }
-                const int32_t len = hi >= lo ? hi - lo + 1 : 0;
-                const int32_t incl = wave_incl_scan(len, lane);
-                const int32_t total = __builtin_amdgcn_readlane(incl, 63);
-                const int32_t xoff = lo - (incl - len);
-                for (int32_t base = 0; base < total; base += 64) {
-                    const int32_t m = base + static_cast<int32_t>(lane);
-                    const int32_t rl = wave_find(incl, m);
-                    const int32_t x = __shfl(xoff, rl, 64) + m;
-                    const float syr = __shfl(sy, rl, 64);
-                    float t9[9];
-                    bool hit = false;
-                    if (m < total) {
-                        const size_t pix = static_cast<size_t>(y0 + rl) * vd.width + static_cast<size_t>(x);
-                        if (cv[pix] == qt + 1u) hit = pixel_terms(q.p, scol[x], syr, gr[pix * 2], t9);
-                    }
-                    unsigned long long mk = __ballot(hit);
-                    while (mk) {
-                        const int src = __ffsll(mk) - 1;
-                        mk &= mk - 1;
-#pragma unroll
-                        for (int kk = 0; kk < 9; ++kk) w[kk] += bcf(t9[kk], src);
-                    }
-                }
-            }
-            if (lane == 0)
-#pragma unroll
-                for (int c = 0; c < 3; ++c) cview[qt * 3 + c] = make_float4(w[c * 3], w[c * 3 + 1], 0.0f, w[c * 3 + 2]);
-        }
     }
 }
 
