@@ -4,14 +4,16 @@ Workload (one "step"): 16 views of uneven size (512x512 ... 1920x1080, twice;
 15,163,392 pixels in total) of a seeded 400x250-quad heightfield (200,000
 triangles, 100,651 vertices, heights N(0, 0.05)) with 8 float attributes per
 vertex N(0,1):
-    batched rasterize forward (tile binning + raster, derivatives on)
+    batched rasterize forward (triangle-parallel candidates + resolve, derivatives on)
  -> batched interpolate forward (A=8)
  -> L2 loss gradient against N(0,1) targets (img - target, loss 0.5 * sum of squares)
  -> batched interpolate backward (attribute + raster gradients)
  -> batched rasterize backward (per-view clip-position gradients).
 Inputs are resident in HBM before the timed region.  Metric: Mpix/s = pixels / step time.
 
-Also reported: the roofline of the dominant kernel (algorithmic bytes per launch
+Also reported: the single-view textured chain (512x512 icosphere, rasterize ->
+interpolate -> texture, fwd+bwd), the atlas refinement (2048^2 atlas, 64 views,
+1000 graph-captured steps), the roofline of the dominant kernel (algorithmic bytes per launch
 over its HIP-event-timed average duration, against 8 TB/s) and the CPU baseline
 (the oracle, the reference's algorithm restated in C++, on the host cores, over
 a bounded sample of the same workload).
@@ -42,6 +44,7 @@ def algorithmic_bytes(V, T, A, pixels):
         "raster_candidates": n * mesh,
         "raster_resolve": n * mesh + 32 * P,
         # backward raster: raster id + grad_raster.xy per pixel, mesh; per-vertex gradients out
+        "bwd_prepass": 32 * P,
         "corner_grads": n * mesh + 32 * P,
         "vertex_reduce": n * 16 * V,
         "interp_fwd": 16 * P + 4 * A * P + n * (4 * A * V + 12 * T),
@@ -80,6 +83,101 @@ def cpu_baseline(scene, views, budget_s=20.0):
         "sample": f"views 0-{used - 1} of the 16-view workload ({done_px} px), full fwd+bwd chain; the attribute "
                   "gradient uses per-vertex pixel lists (the reference's sums in its order, not its O(V*A*P) loop)",
     }
+
+
+def chain_benchmark(steps=50, warmup=5, cpu=True):
+    """Configuration 1: one 512x512 view of a level-5 icosphere (20,480 tris), spherical
+    UVs, 1024^2 RGBA texture; rasterize -> interpolate (uv, A=2) -> texture (clamp),
+    loss sum (img - target)^2, backward to positions, uv attributes and texels.
+    Mpix/s = 262,144 / (summed library kernel time per chain; the loss op excluded)."""
+    import ctypes
+
+    import torch
+
+    from photara_drender_amd import _native, scenes
+
+    sc = scenes.chain_scene(level=5, res=(512, 512), tex_size=1024, seed=1)
+    dev = torch.device("cuda:0")
+    H, W = sc.resolution
+    C = sc.texture.shape[2]
+    th, tw = sc.texture.shape[:2]
+    V, T = sc.clip.shape[0], sc.indices.shape[0]
+    clip = torch.tensor(sc.clip, device=dev)
+    idx = torch.tensor(sc.indices.view(np.int32), device=dev)
+    uv = torch.tensor(sc.uv, device=dev)
+    tex = torch.tensor(sc.texture, device=dev)
+    target = torch.tensor(sc.target, device=dev)
+    raster = torch.empty((H, W, 4), device=dev)
+    deriv = torch.empty((H, W, 4), device=dev)
+    uvs = torch.empty((H, W, 2), device=dev)
+    color = torch.empty((H, W, C), device=dev)
+    g_color = torch.empty((H, W, C), device=dev)
+    g_tex = torch.empty((th, tw, C), device=dev)
+    g_uvs = torch.empty((H, W, 2), device=dev)
+    g_uv = torch.empty((V, 2), device=dev)
+    g_raster = torch.empty((H, W, 4), device=dev)
+    g_pos = torch.empty((V, 4), device=dev)
+    vp_out = (ctypes.c_float * 4)()
+    ctx = _native.Context(0, flags=0)
+    lib = _native.lib()
+    st = torch.cuda.current_stream().cuda_stream
+    ck = _native.check
+
+    def step():
+        ck(lib.pdr_rasterize_forward(ctx.handle, clip.data_ptr(), V, idx.data_ptr(), T, W, H, 0, 1, None,
+                                     raster.data_ptr(), deriv.data_ptr(), vp_out, st))
+        ck(lib.pdr_interpolate_forward(ctx.handle, uv.data_ptr(), V, 2, idx.data_ptr(), T, raster.data_ptr(), W, H,
+                                       uvs.data_ptr(), st))
+        ck(lib.pdr_texture_forward(ctx.handle, tex.data_ptr(), tw, th, C, 0, uvs.data_ptr(), raster.data_ptr(), W, H,
+                                   color.data_ptr(), st))
+        torch.sub(color, target, alpha=1.0, out=g_color).mul_(2.0)  # d/dimg of sum (img - target)^2
+        ck(lib.pdr_texture_backward(ctx.handle, tex.data_ptr(), tw, th, C, 0, uvs.data_ptr(), raster.data_ptr(), W, H,
+                                    g_color.data_ptr(), 1, g_tex.data_ptr(), g_uvs.data_ptr(), st))
+        ck(lib.pdr_interpolate_backward(ctx.handle, uv.data_ptr(), V, 2, idx.data_ptr(), T, raster.data_ptr(), W, H,
+                                        g_uvs.data_ptr(), g_uv.data_ptr(), g_raster.data_ptr(), st))
+        ck(lib.pdr_rasterize_backward(ctx.handle, clip.data_ptr(), V, idx.data_ptr(), T, raster.data_ptr(), W, H,
+                                      g_raster.data_ptr(), None, g_pos.data_ptr(), st))
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    ctx.set_timing(True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    wall_ms = (time.perf_counter() - t0) / steps * 1e3
+    kernels = {}
+    for name in ("raster_candidates", "raster_resolve", "interp_fwd", "texture_fwd", "texture_bwd", "interp_bwd",
+                 "vertex_csr", "bwd_prepass", "corner_grads", "vertex_reduce"):
+        ms, launches = ctx.kernel_time(name)
+        if launches:
+            kernels[name] = ms / steps
+    ctx.set_timing(False)
+    chain_ms = sum(kernels.values())
+    out = {"workload": "512x512 view, icosphere level 5 (20,480 tris), 1024^2x4 texture, rasterize -> interpolate "
+                       "(uv) -> texture, fwd+bwd",
+           "value": H * W / (chain_ms * 1e-3) / 1e6, "unit": "Mpix/s", "chain_kernel_ms": chain_ms,
+           "wall_ms_per_chain": wall_ms, "kernel_ms": {k: round(v, 4) for k, v in kernels.items()}}
+    if cpu:
+        import oracle
+
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(os.cpu_count() or 1, 16)
+        oracle.set_num_threads(threads)
+        times = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            r, _, _ = oracle.rasterize_forward(sc.clip, sc.indices, (H, W))
+            u = oracle.interpolate_forward(sc.uv, sc.indices, r)
+            c = oracle.texture_forward(sc.texture, u, r)
+            gt, gu = oracle.texture_backward(sc.texture, u, r, (2.0 * (c - sc.target)).astype(np.float32))
+            _, gr = oracle.interpolate_backward(sc.uv, sc.indices, r, gu)
+            oracle.rasterize_backward(sc.clip, sc.indices, r, gr)
+            times.append(time.perf_counter() - t0)
+        out["cpu_baseline"] = {"value": H * W / float(np.median(times)) / 1e6, "unit": "Mpix/s",
+                               "cores": oracle.num_threads(), "kind": "port",
+                               "sample": "the whole chain, median of 3 runs (loss included)"}
+    return out
 
 
 def refine_benchmark(steps=1000, checkpoint_every=50, cpu=True):
@@ -173,6 +271,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-views", type=int, default=16)
     ap.add_argument("--no-refine", action="store_true", help="skip the atlas-refinement workload")
+    ap.add_argument("--no-chain", action="store_true", help="skip the single-view textured chain workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU layout "
@@ -248,8 +347,8 @@ def main():
     elapsed = time.perf_counter() - t0
     ms_per_step = elapsed / args.steps * 1e3
     kernels = {}
-    for name in ("raster_candidates", "raster_resolve", "interp_fwd", "interp_bwd", "vertex_csr", "corner_grads",
-                 "vertex_reduce"):
+    for name in ("raster_candidates", "raster_resolve", "interp_fwd", "interp_bwd", "vertex_csr", "bwd_prepass",
+                 "corner_grads", "vertex_reduce"):
         ms, launches = ctx.kernel_time(name)
         if launches:
             kernels[name] = ms / launches
@@ -281,6 +380,8 @@ def main():
     }
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(scene, args.cpu_views)
+    if not args.no_chain:
+        result["chain_512"] = chain_benchmark(cpu=not args.no_cpu_baseline)
     if not args.no_refine:
         result["refine"] = refine_benchmark(cpu=not args.no_cpu_baseline)
     print(json.dumps(result))

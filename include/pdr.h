@@ -214,6 +214,57 @@ pdr_status pdr_refine(pdr_context ctx, const float* initial_texture, const float
                       const float* seam_pairs, uint32_t npairs, const pdr_refine_options* options, float* color_out,
                       float* loss_out, float* seam_loss_out, double* times_out2);
 
+/* ------------------------------------------------------------------ bake */
+
+/* ProjectionView — include/photara_drender/types.hpp:135-152. */
+typedef struct pdr_bake_view {
+    uint32_t width, height, channels; /* channels 3 or 4 */
+    const float* image;               /* [H,W,channels] in [0,1] */
+    const float* visibility_mask;     /* [H,W] or NULL */
+    float world_to_clip[16];          /* row-major, clip = M [p,1] */
+    float camera_position[3];
+    int has_viewport;
+    float viewport[4];
+    float weight;          /* default 1 */
+    float depth_bias;      /* default 1e-3 */
+    float min_view_cosine; /* default 0.05 */
+} pdr_bake_view;
+
+/* TextureBakeOptions — types.hpp:154-179.  blend_mode: 0 best_view,
+ * 1 weighted_average, 2 softmax.  visibility_mode: 0 shadow_map,
+ * 1 hybrid_ray_query, 2 ray_query.  This backend has no ray queries: like the
+ * reference on such a device, ray-query modes fall back to the shadow map when
+ * allow_visibility_fallback is set and fail otherwise. */
+typedef struct pdr_bake_options {
+    uint32_t width, height;
+    uint32_t blend_mode;
+    uint32_t visibility_mode;
+    uint32_t pcf_radius;
+    float ray_origin_bias;
+    int allow_visibility_fallback;
+    float mask_floor;
+    float softmax_scale; /* <= 0: scene-relative scale (pdr_suggest_softmax_scale) */
+} pdr_bake_options;
+
+void pdr_bake_options_default(pdr_bake_options* options);
+void pdr_bake_view_default(pdr_bake_view* view);
+
+/* suggest_softmax_scale — src/texture_padding.cpp:187-222.  Host arrays. */
+float pdr_suggest_softmax_scale(const float* positions3, uint32_t vertex_count, const float* camera_positions3,
+                                uint32_t camera_count);
+
+/* TextureBaker::bake — src/texture_baker.cpp:156-361 + shaders/project_texture.hlsl.
+ * Mesh arrays host or device; images and masks host or device per view.
+ * Outputs (host or device, all [height,width]): color_out [.,.,4] (RGB, valid),
+ * confidence_out, source_view_out (u32, 0xFFFFFFFF none), valid_out,
+ * coverage_out, face_ids_out (u32, 0xFFFFFFFF outside the mesh; NULL to skip);
+ * used_ray_query_out (always 0 here; NULL to skip). */
+pdr_status pdr_bake(pdr_context ctx, const float* positions, const float* normals, const float* uv,
+                    uint32_t vertex_count, const uint32_t* indices, uint32_t triangle_count,
+                    const pdr_bake_view* views, uint32_t view_count, const pdr_bake_options* options,
+                    float* color_out, float* confidence_out, uint32_t* source_view_out, float* valid_out,
+                    float* coverage_out, uint32_t* face_ids_out, int* used_ray_query_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -104,6 +104,46 @@ struct ProjectionView {
     float min_view_cosine = 0.05F;
 };
 
+// include/photara_drender/types.hpp:89-103 of the reference
+enum class ProjectionBlendMode : std::uint32_t {
+    best_view = 0,
+    weighted_average = 1,
+    softmax = 2,
+};
+
+enum class VisibilityMode : std::uint32_t {
+    shadow_map = 0,
+    hybrid_ray_query = 1,
+    ray_query = 2,
+};
+
+// types.hpp:154-179
+struct TextureBakeOptions {
+    std::uint32_t width = 1024;
+    std::uint32_t height = 1024;
+    ProjectionBlendMode blend_mode = ProjectionBlendMode::best_view;
+    VisibilityMode visibility_mode = VisibilityMode::ray_query;
+    std::uint32_t pcf_radius = 1;
+    float ray_origin_bias = 1e-4F;
+    bool allow_visibility_fallback = true;
+    float mask_floor = 0.0F;
+    float softmax_scale = 0.0F;
+};
+
+// types.hpp:181-199 (filled_mask stays empty: pad_texture_atlas is not part of this backend)
+struct TextureBakeOutput {
+    std::uint32_t width = 0;
+    std::uint32_t height = 0;
+    std::vector<float> color;
+    std::vector<float> confidence;
+    std::vector<std::uint32_t> source_view;
+    std::vector<float> valid_mask;
+    std::vector<float> coverage_mask;
+    std::vector<std::uint32_t> face_ids;
+    std::vector<float> filled_mask;
+    bool used_ray_query = false;
+};
+
 struct TextureRefineOptions {
     std::uint32_t width = 1024;
     std::uint32_t height = 1024;

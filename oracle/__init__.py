@@ -295,3 +295,71 @@ def texture_forward_f64(tex, uv, raster, address_mode="clamp"):
     lib().orc_texture_forward_f64(_p(tex, _F64), tw, th, c, ADDRESS_MODES[address_mode], _p(uv, _F64),
                                   _p(raster, _F64), h * w, _p(out, _F64))
     return out
+
+
+class _BakeView(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("channels", ctypes.c_uint32),
+                ("image", ctypes.c_void_p), ("mask", ctypes.c_void_p), ("world_to_clip", ctypes.c_float * 16),
+                ("camera", ctypes.c_float * 3), ("viewport", ctypes.c_float * 4), ("weight", ctypes.c_float),
+                ("depth_bias", ctypes.c_float), ("min_view_cosine", ctypes.c_float)]
+
+
+def suggest_softmax_scale(positions, camera_positions):
+    """src/texture_padding.cpp:187-222"""
+    pos, cams = _f32(positions), _f32(camera_positions)
+    fn = lib().orc_suggest_softmax_scale
+    fn.restype = ctypes.c_float
+    return float(fn(_p(pos, _F32), pos.shape[0], _p(cams, _F32), cams.shape[0]))
+
+
+BLEND_MODES = {"best_view": 0, "weighted_average": 1, "softmax": 2}
+
+
+def bake(positions, normals, uv, indices, images, world_to_clip, camera_positions, visibility_masks=None,
+         viewports=None, resolution=(1024, 1024), blend_mode="weighted_average", pcf_radius=1, softmax_scale=0.0,
+         mask_floor=0.0, weights=None, depth_bias=1e-3, min_view_cosine=0.05):
+    """TextureBaker::bake (src/texture_baker.cpp:156-361) with shadow-map visibility.
+    Returns (color [H,W,4], confidence, source_view (u32), valid, coverage, face_ids (u32))."""
+    pos, nrm, uvs, idx = _f32(positions), _f32(normals), _f32(uv), _u32(indices)
+    n = len(images)
+    mats = _f32(np.asarray(world_to_clip).reshape(n, 16))
+    cams = _f32(np.asarray(camera_positions).reshape(n, 3))
+    mode = BLEND_MODES[blend_mode]
+    scale = float(softmax_scale)
+    if mode == 2 and not (scale > 0 and np.isfinite(scale)):
+        scale = suggest_softmax_scale(pos, cams)
+        if not (scale > 0 and np.isfinite(scale)):
+            scale = 1.0
+    keep, views = [], (_BakeView * n)()
+    for k in range(n):
+        img = _f32(images[k])
+        h, w, c = img.shape
+        v = views[k]
+        v.width, v.height, v.channels = w, h, c
+        v.image = img.ctypes.data
+        keep.append(img)
+        m = None if visibility_masks is None else visibility_masks[k]
+        if m is not None:
+            m = _f32(m)
+            if mask_floor > 0:
+                m = np.ascontiguousarray(np.maximum(m, np.float32(mask_floor)), np.float32)
+            keep.append(m)
+            v.mask = m.ctypes.data
+        v.world_to_clip = (ctypes.c_float * 16)(*mats[k].tolist())
+        v.camera = (ctypes.c_float * 3)(*cams[k].tolist())
+        vp = None if viewports is None else viewports[k]
+        v.viewport = (ctypes.c_float * 4)(*([0.0, 0.0, float(w), float(h)] if vp is None else [float(x) for x in vp]))
+        wk = 1.0 if weights is None else float(weights[k])
+        v.weight = wk * scale if mode == 2 else wk
+        v.depth_bias, v.min_view_cosine = depth_bias, min_view_cosine
+    H, W = resolution
+    color = np.empty((H, W, 4), np.float32)
+    conf = np.empty((H, W), np.float32)
+    src = np.empty((H, W), np.uint32)
+    valid = np.empty((H, W), np.float32)
+    cov = np.empty((H, W), np.float32)
+    face = np.empty((H, W), np.uint32)
+    _check(lib().orc_bake(_p(pos, _F32), _p(nrm, _F32), _p(uvs, _F32), pos.shape[0], _p(idx, _U32), idx.shape[0],
+                          views, n, W, H, mode, pcf_radius, _p(color, _F32), _p(conf, _F32), _p(src, _U32),
+                          _p(valid, _F32), _p(cov, _F32), _p(face, _U32)))
+    return color, conf, src, valid, cov, face

@@ -823,4 +823,196 @@ void orc_texture_forward_f64(const double* tex, uint32_t tw, uint32_t th, uint32
     }
 }
 
+namespace {
+
+// shaders/project_texture.hlsl:36-54: bilinear sample with coordinates clamped
+// to the image; lerp(a, b, t) = a + t * (b - a).
+inline float bake_lerp(float a, float b, float t) { return a + t * (b - a); }
+
+float bake_photo(const orc_bake_view& v, float px, float py, uint32_t ch) {
+    const float tx = px - 0.5f, ty = py - 0.5f;
+    const float flx = std::floor(tx), fly = std::floor(ty);
+    const int32_t bx = f2i_sat(flx), by = f2i_sat(fly);
+    const float fx = tx - flx, fy = ty - fly;
+    const int32_t wm = static_cast<int32_t>(v.width) - 1, hm = static_cast<int32_t>(v.height) - 1;
+    auto at = [&](int32_t x, int32_t y) {
+        x = std::min(std::max(x, 0), wm);
+        y = std::min(std::max(y, 0), hm);
+        return v.image[(static_cast<size_t>(y) * v.width + x) * v.channels + ch];
+    };
+    return bake_lerp(bake_lerp(at(bx, by), at(add1(bx), by), fx), bake_lerp(at(bx, add1(by)), at(add1(bx), add1(by)), fx),
+                     fy);
+}
+
+// shaders/project_texture.hlsl:56-73
+float bake_mask(const orc_bake_view& v, float px, float py) {
+    const float tx = px - 0.5f, ty = py - 0.5f;
+    const float flx = std::floor(tx), fly = std::floor(ty);
+    const int32_t bx = f2i_sat(flx), by = f2i_sat(fly);
+    const float fx = tx - flx, fy = ty - fly;
+    const int32_t wm = static_cast<int32_t>(v.width) - 1, hm = static_cast<int32_t>(v.height) - 1;
+    auto at = [&](int32_t x, int32_t y) {
+        x = std::min(std::max(x, 0), wm);
+        y = std::min(std::max(y, 0), hm);
+        return v.mask[static_cast<size_t>(y) * v.width + x];
+    };
+    return bake_lerp(bake_lerp(at(bx, by), at(add1(bx), by), fx), bake_lerp(at(bx, add1(by)), at(add1(bx), add1(by)), fx),
+                     fy);
+}
+
+// shaders/project_texture.hlsl:75-99: share of the PCF neighbourhood whose
+// shadow-raster depth (plus the depth bias) is not in front of the sample.
+float bake_shadow(const orc_bake_view& v, const float* shadow, float px, float py, float depth, uint32_t r) {
+    const int32_t cx = f2i_sat(std::floor(px)), cy = f2i_sat(std::floor(py));
+    float visible = 0.0f, count = 0.0f;
+    for (int32_t y = -static_cast<int32_t>(r); y <= static_cast<int32_t>(r); ++y)
+        for (int32_t x = -static_cast<int32_t>(r); x <= static_cast<int32_t>(r); ++x) {
+            const int32_t qx = cx + x, qy = cy + y;
+            if (qx < 0 || qy < 0 || qx >= static_cast<int32_t>(v.width) || qy >= static_cast<int32_t>(v.height)) continue;
+            const size_t o = (static_cast<size_t>(qy) * v.width + qx) * 4;
+            if (decode_id(shadow[o + 3]) != 0u) {
+                visible += depth <= shadow[o + 2] + v.depth_bias ? 1.0f : 0.0f;
+                count += 1.0f;
+            }
+        }
+    return count > 0.0f ? visible / count : 0.0f;
+}
+
+}  // namespace
+
+float orc_suggest_softmax_scale(const float* pos3, uint32_t V, const float* cams, uint32_t nc) {
+    const size_t stride = std::max<size_t>(1, V / 10000);
+    double best = std::numeric_limits<double>::infinity();
+    for (size_t v = 0; v < V; v += stride) {
+        const double x = pos3[v * 3], y = pos3[v * 3 + 1], z = pos3[v * 3 + 2];
+        if (!std::isfinite(x) || !std::isfinite(y) || !std::isfinite(z)) continue;
+        for (size_t c = 0; c < nc; ++c) {
+            const double dx = cams[c * 3] - x, dy = cams[c * 3 + 1] - y, dz = cams[c * 3 + 2] - z;
+            best = std::min(best, dx * dx + dy * dy + dz * dz);
+        }
+    }
+    if (!std::isfinite(best) || best <= 0.0) return 0.0f;
+    return static_cast<float>(20.0 * best);
+}
+
+int orc_bake(const float* pos3, const float* normal3, const float* uv2, uint32_t V, const uint32_t* idx, uint32_t T,
+             const orc_bake_view* views, uint32_t nviews, uint32_t W, uint32_t H, uint32_t blend, uint32_t pcf,
+             float* color4, float* confidence, uint32_t* source_view, float* valid, float* coverage,
+             uint32_t* face_ids) {
+    const size_t P = static_cast<size_t>(W) * H;
+    // src/texture_baker.cpp:85-94 atlas clip positions, then the atlas raster
+    std::vector<float> aclip(static_cast<size_t>(V) * 4);
+    for (size_t v = 0; v < V; ++v) {
+        aclip[v * 4] = uv2[v * 2] * 2.0f - 1.0f;
+        aclip[v * 4 + 1] = uv2[v * 2 + 1] * 2.0f - 1.0f;
+        aclip[v * 4 + 2] = 0.0f;
+        aclip[v * 4 + 3] = 1.0f;
+    }
+    std::vector<float> araster(P * 4), apos(P * 3), anrm(P * 3);
+    float vp_out[4];
+    int rc = orc_rasterize_forward(aclip.data(), V, idx, T, W, H, 0, 0, nullptr, araster.data(), nullptr, vp_out);
+    if (rc) return rc;
+    orc_interpolate_forward(pos3, V, 3, idx, T, araster.data(), static_cast<uint32_t>(P), apos.data());
+    orc_interpolate_forward(normal3, V, 3, idx, T, araster.data(), static_cast<uint32_t>(P), anrm.data());
+    // src/texture_baker.cpp:193-204 coverage and face ids
+    for (size_t p = 0; p < P; ++p) {
+        const float id = std::round(araster[p * 4 + 3]);
+        coverage[p] = id != 0.0f ? 1.0f : 0.0f;
+        if (face_ids) face_ids[p] = coverage[p] > 0.5f ? static_cast<uint32_t>(id) - 1u : 0xFFFFFFFFu;
+    }
+    std::vector<float> acc(P * 3, 0.0f), accw(P, 0.0f), best(P, 0.0f);
+    for (size_t p = 0; p < P; ++p) source_view[p] = 0xFFFFFFFFu;
+    for (uint32_t vi = 0; vi < nviews; ++vi) {
+        const orc_bake_view& v = views[vi];
+        std::vector<float> clip(static_cast<size_t>(V) * 4), shadow(static_cast<size_t>(v.width) * v.height * 4);
+        orc_transform_positions(pos3, V, v.world_to_clip, clip.data());
+        rc = orc_rasterize_forward(clip.data(), V, idx, T, v.width, v.height, 0, 0, v.viewport, shadow.data(), nullptr,
+                                   vp_out);
+        if (rc) return rc;
+        const float* m = v.world_to_clip;
+#pragma omp parallel for schedule(static)
+        for (int64_t pi = 0; pi < static_cast<int64_t>(P); ++pi) {
+            const size_t p = static_cast<size_t>(pi);
+            if (decode_id(araster[p * 4 + 3]) == 0u) continue;  // project_texture.hlsl:121-125
+            const float x = apos[p * 3], y = apos[p * 3 + 1], z = apos[p * 3 + 2];
+            const float n0 = anrm[p * 3], n1 = anrm[p * 3 + 1], n2 = anrm[p * 3 + 2];
+            const float nl = std::sqrt(n0 * n0 + n1 * n1 + n2 * n2);
+            const float nx = n0 / nl, ny = n1 / nl, nz = n2 / nl;
+            float c[4];
+            for (int r = 0; r < 4; ++r) c[r] = m[r * 4] * x + m[r * 4 + 1] * y + m[r * 4 + 2] * z + m[r * 4 + 3] * 1.0f;
+            if (c[3] <= 1e-8f) continue;
+            const float ndx = c[0] / c[3], ndy = c[1] / c[3], ndz = c[2] / c[3];
+            if (ndx < -1.0f || ndy < -1.0f || ndx > 1.0f || ndy > 1.0f || ndz < -1.0f || ndz > 1.0f) continue;
+            const float px = v.viewport[0] + (ndx * 0.5f + 0.5f) * v.viewport[2];
+            const float py = v.viewport[1] + (ndy * 0.5f + 0.5f) * v.viewport[3];
+            if (px < v.viewport[0] || py < v.viewport[1] || px >= v.viewport[0] + v.viewport[2] ||
+                py >= v.viewport[1] + v.viewport[3])
+                continue;
+            float vis = bake_shadow(v, shadow.data(), px, py, ndz, pcf);
+            if (v.mask) vis *= clampf(bake_mask(v, px, py), 0.0f, 1.0f);
+            const float dx = v.camera[0] - x, dy = v.camera[1] - y, dz = v.camera[2] - z;
+            const float dl = std::sqrt(dx * dx + dy * dy + dz * dz);
+            const float vx = dx / dl, vy = dy / dl, vz = dz / dl;
+            const float cosv = nx * vx + ny * vy + nz * vz;
+            if (cosv < v.min_view_cosine || vis <= 0.0f) continue;
+            const float col[3] = {bake_photo(v, px, py, 0), bake_photo(v, px, py, 1), bake_photo(v, px, py, 2)};
+            const float alpha = v.channels == 4 ? bake_photo(v, px, py, 3) : 1.0f;
+            if (blend == 2) {  // project_texture.hlsl:176-206
+                const float dist = std::max(dl, 1e-6f);
+                const float footprint = cosv / (dist * dist);
+                const float e = clampf(vis * alpha * footprint * v.weight, 0.0f, 64.0f);
+                const float prev = best[p];
+                const float cur = std::max(prev, e);
+                if (cur > prev) {
+                    const float rs = std::exp(prev - cur);
+                    acc[p * 3] *= rs;
+                    acc[p * 3 + 1] *= rs;
+                    acc[p * 3 + 2] *= rs;
+                    accw[p] *= rs;
+                    best[p] = cur;
+                }
+                if (e >= prev) source_view[p] = vi;
+                const float w = std::exp(e - cur);
+                if (w <= 0.0f) continue;
+                acc[p * 3] += col[0] * w;
+                acc[p * 3 + 1] += col[1] * w;
+                acc[p * 3 + 2] += col[2] * w;
+                accw[p] += w;
+                continue;
+            }
+            const float conf = vis * alpha * cosv * v.weight;  // project_texture.hlsl:208-246
+            if (conf <= 0.0f) continue;
+            if (conf > best[p]) {
+                best[p] = conf;
+                source_view[p] = vi;
+            }
+            if (blend == 0) {
+                if (conf >= accw[p]) {
+                    acc[p * 3] = col[0];
+                    acc[p * 3 + 1] = col[1];
+                    acc[p * 3 + 2] = col[2];
+                    accw[p] = conf;
+                }
+            } else {
+                acc[p * 3] += col[0] * conf;
+                acc[p * 3 + 1] += col[1] * conf;
+                acc[p * 3 + 2] += col[2] * conf;
+                accw[p] += conf;
+            }
+        }
+    }
+    // src/texture_baker.cpp:339-357
+    for (size_t p = 0; p < P; ++p) {
+        const float w = accw[p];
+        confidence[p] = w;
+        valid[p] = w > 0.0f ? 1.0f : 0.0f;
+        const float d = blend != 0 && w > 0.0f ? w : 1.0f;
+        color4[p * 4] = acc[p * 3] / d;
+        color4[p * 4 + 1] = acc[p * 3 + 1] / d;
+        color4[p * 4 + 2] = acc[p * 3 + 2] / d;
+        color4[p * 4 + 3] = valid[p];
+    }
+    return 0;
+}
+
 }  // extern "C"

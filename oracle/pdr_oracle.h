@@ -120,6 +120,30 @@ void orc_texture_forward_f64(const double* tex, uint32_t tw, uint32_t th, uint32
                              uint32_t address_mode, const double* uv, const double* raster,
                              uint32_t P, double* out);
 
+/* TextureBaker::bake (src/texture_baker.cpp:156-361) with shadow-map
+ * visibility (the reference's path on devices without ray queries) and the
+ * projection of shaders/project_texture.hlsl:117-247; views are projected in
+ * order, each texel's state carried from view to view.  viewport is the
+ * resolved (x, y, w, h); weight is the effective per-view weight (already
+ * multiplied by the softmax scale in softmax mode). */
+typedef struct orc_bake_view {
+    uint32_t width, height, channels;
+    const float* image;  /* [h, w, channels] */
+    const float* mask;   /* [h, w] (floored by the caller when a mask floor applies) or NULL */
+    float world_to_clip[16];
+    float camera[3];
+    float viewport[4];
+    float weight, depth_bias, min_view_cosine;
+} orc_bake_view;
+
+int orc_bake(const float* pos3, const float* normal3, const float* uv2, uint32_t V, const uint32_t* idx,
+             uint32_t T, const orc_bake_view* views, uint32_t nviews, uint32_t width, uint32_t height,
+             uint32_t blend_mode, uint32_t pcf_radius, float* color4, float* confidence, uint32_t* source_view,
+             float* valid, float* coverage, uint32_t* face_ids);
+
+/* src/texture_padding.cpp:187-222 */
+float orc_suggest_softmax_scale(const float* pos3, uint32_t V, const float* cameras3, uint32_t ncameras);
+
 #ifdef __cplusplus
 }
 #endif

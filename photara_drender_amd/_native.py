@@ -54,6 +54,18 @@ class RefineOptions(ctypes.Structure):
                 ("checkpoints", _P)]
 
 
+class BakeView(ctypes.Structure):
+    _fields_ = [("width", _U32), ("height", _U32), ("channels", _U32), ("image", _P), ("visibility_mask", _P),
+                ("world_to_clip", _F * 16), ("camera_position", _F * 3), ("has_viewport", _I), ("viewport", _F * 4),
+                ("weight", _F), ("depth_bias", _F), ("min_view_cosine", _F)]
+
+
+class BakeOptions(ctypes.Structure):
+    _fields_ = [("width", _U32), ("height", _U32), ("blend_mode", _U32), ("visibility_mode", _U32),
+                ("pcf_radius", _U32), ("ray_origin_bias", _F), ("allow_visibility_fallback", _I), ("mask_floor", _F),
+                ("softmax_scale", _F)]
+
+
 _SIGNATURES = {
     "pdr_last_error": (ctypes.c_char_p, []),
     "pdr_context_create": (_I, [_U32, _I, ctypes.POINTER(_P)]),
@@ -78,6 +90,11 @@ _SIGNATURES = {
     "pdr_texture_forward": (_I, [_P, _P, _U32, _U32, _U32, _U32, _P, _P, _U32, _U32, _P, _P]),
     "pdr_texture_backward": (_I, [_P, _P, _U32, _U32, _U32, _U32, _P, _P, _U32, _U32, _P, _I, _P, _P, _P]),
     "pdr_refine_options_default": (None, [ctypes.POINTER(RefineOptions)]),
+    "pdr_bake_options_default": (None, [ctypes.POINTER(BakeOptions)]),
+    "pdr_bake_view_default": (None, [ctypes.POINTER(BakeView)]),
+    "pdr_suggest_softmax_scale": (_F, [_P, _U32, _P, _U32]),
+    "pdr_bake": (_I, [_P, _P, _P, _P, _U32, _P, _U32, ctypes.POINTER(BakeView), _U32, ctypes.POINTER(BakeOptions), _P,
+                      _P, _P, _P, _P, _P, ctypes.POINTER(_I)]),
     "pdr_refine": (_I, [_P, _P, _P, _U32, _P, _P, _U32, ctypes.POINTER(RefineView), _U32, _P, _U32,
                         ctypes.POINTER(RefineOptions), _P, _P, _P, ctypes.POINTER(ctypes.c_double)]),
 }

@@ -219,7 +219,100 @@ private:
 class PythonTextureBaker {
 public:
     PythonTextureBaker(std::uint32_t device_index, bool enable_validation)
-        : context_(ContextOptions{device_index, enable_validation}), refiner_(context_) {}
+        : context_(ContextOptions{device_index, enable_validation}), baker_(context_), refiner_(context_) {}
+
+    // python/bindings.cpp:161-287
+    py::tuple bake(const FloatInput& positions, const FloatInput& normals, const FloatInput& uv,
+                   const U32Input& indices, const py::list& images, const FloatInput& world_to_clip,
+                   const FloatInput& camera_positions, const py::object& visibility_masks, const py::object& viewports,
+                   const std::pair<std::uint32_t, std::uint32_t>& resolution, const std::string& blend_mode,
+                   const std::string& visibility_mode, std::uint32_t pcf_radius, bool allow_visibility_fallback,
+                   float softmax_scale, float mask_floor) {
+        if (positions.ndim() != 2 || positions.shape(1) != 3 || positions.shape(0) <= 0)
+            throw std::invalid_argument("positions must have shape [vertex_count, 3]");
+        if (normals.ndim() != 2 || normals.shape(0) != positions.shape(0) || normals.shape(1) != 3)
+            throw std::invalid_argument("normals must have shape [vertex_count, 3]");
+        if (uv.ndim() != 2 || uv.shape(0) != positions.shape(0) || uv.shape(1) != 2)
+            throw std::invalid_argument("uv must have shape [vertex_count, 2]");
+        validate_indices(indices);
+        const auto view_count = static_cast<py::ssize_t>(images.size());
+        if (view_count <= 0 || world_to_clip.ndim() != 3 || world_to_clip.shape(0) != view_count ||
+            world_to_clip.shape(1) != 4 || world_to_clip.shape(2) != 4)
+            throw std::invalid_argument("world_to_clip must have shape [view_count, 4, 4]");
+        if (camera_positions.ndim() != 2 || camera_positions.shape(0) != view_count || camera_positions.shape(1) != 3)
+            throw std::invalid_argument("camera_positions must have shape [view_count, 3]");
+        py::list mask_list, viewport_list;
+        if (!visibility_masks.is_none()) {
+            mask_list = visibility_masks.cast<py::list>();
+            if (static_cast<py::ssize_t>(mask_list.size()) != view_count)
+                throw std::invalid_argument("visibility_masks must contain one entry per view");
+        }
+        if (!viewports.is_none()) {
+            viewport_list = viewports.cast<py::list>();
+            if (static_cast<py::ssize_t>(viewport_list.size()) != view_count)
+                throw std::invalid_argument("viewports must contain one entry per view");
+        }
+        std::vector<ProjectionView> views(static_cast<std::size_t>(view_count));
+        for (py::ssize_t i = 0; i < view_count; ++i) {
+            const auto image = images[static_cast<std::size_t>(i)].cast<FloatInput>();
+            if (image.ndim() != 3 || image.shape(0) <= 0 || image.shape(1) <= 0 ||
+                (image.shape(2) != 3 && image.shape(2) != 4))
+                throw std::invalid_argument("each image must have shape [height, width, 3 or 4]");
+            auto& view = views[static_cast<std::size_t>(i)];
+            view.height = static_cast<std::uint32_t>(image.shape(0));
+            view.width = static_cast<std::uint32_t>(image.shape(1));
+            view.channel_count = static_cast<std::uint32_t>(image.shape(2));
+            view.image.assign(image.data(), image.data() + image.size());
+            std::copy_n(world_to_clip.data() + i * 16, 16, view.world_to_clip.begin());
+            std::copy_n(camera_positions.data() + i * 3, 3, view.camera_position.begin());
+            if (!visibility_masks.is_none() && !mask_list[static_cast<std::size_t>(i)].is_none()) {
+                const auto mask = mask_list[static_cast<std::size_t>(i)].cast<FloatInput>();
+                if (mask.ndim() != 2 || mask.shape(0) != image.shape(0) || mask.shape(1) != image.shape(1))
+                    throw std::invalid_argument("each visibility mask must have shape [image_height, image_width]");
+                view.visibility_mask.assign(mask.data(), mask.data() + mask.size());
+            }
+            if (!viewports.is_none() && !viewport_list[static_cast<std::size_t>(i)].is_none()) {
+                const auto v = viewport_list[static_cast<std::size_t>(i)].cast<std::array<float, 4>>();
+                view.viewport = Viewport{v[0], v[1], v[2], v[3]};
+            }
+        }
+        TextureBakeOptions options;
+        options.height = resolution.first;
+        options.width = resolution.second;
+        options.pcf_radius = pcf_radius;
+        options.allow_visibility_fallback = allow_visibility_fallback;
+        if (blend_mode == "best_view")
+            options.blend_mode = ProjectionBlendMode::best_view;
+        else if (blend_mode == "weighted_average")
+            options.blend_mode = ProjectionBlendMode::weighted_average;
+        else if (blend_mode == "softmax")
+            options.blend_mode = ProjectionBlendMode::softmax;
+        else
+            throw std::invalid_argument("blend_mode must be 'best_view', 'weighted_average', or 'softmax'");
+        options.softmax_scale = softmax_scale;
+        options.mask_floor = mask_floor;
+        if (visibility_mode == "shadow_map")
+            options.visibility_mode = VisibilityMode::shadow_map;
+        else if (visibility_mode == "hybrid_ray_query")
+            options.visibility_mode = VisibilityMode::hybrid_ray_query;
+        else if (visibility_mode == "ray_query")
+            options.visibility_mode = VisibilityMode::ray_query;
+        else
+            throw std::invalid_argument("visibility_mode must be 'shadow_map', 'hybrid_ray_query', or 'ray_query'");
+        TextureBakeOutput out;
+        {
+            py::gil_scoped_release release;
+            out = baker_.bake(as_span(positions), as_span(normals), as_span(uv), as_span(indices), views, options);
+        }
+        const auto H = static_cast<py::ssize_t>(out.height), W = static_cast<py::ssize_t>(out.width);
+        auto color = to_array(std::move(out.color), {H, W, 4});
+        auto confidence = to_array(std::move(out.confidence), {H, W});
+        py::array_t<std::uint32_t> source({H, W});
+        std::copy(out.source_view.begin(), out.source_view.end(), source.mutable_data());
+        auto valid = to_array(std::move(out.valid_mask), {H, W});
+        auto coverage = to_array(std::move(out.coverage_mask), {H, W});
+        return py::make_tuple(color, confidence, source, valid, coverage, out.used_ray_query);
+    }
 
     // python/bindings.cpp:289-404
     py::tuple refine_texture(const FloatInput& initial_texture, const FloatInput& positions, const FloatInput& uv,
@@ -312,6 +405,7 @@ public:
 
 private:
     Context context_;
+    TextureBaker baker_;
     TextureRefiner refiner_;
 };
 
@@ -354,6 +448,13 @@ PYBIND11_MODULE(_photara_drender, m) {
         .def_property_readonly("device_info", &PythonTextureBaker::device_info,
                                py::return_value_policy::reference_internal)
         .def_property_readonly("context_handle", &PythonTextureBaker::context_handle)
+        .def("bake", &PythonTextureBaker::bake, py::arg("positions"), py::arg("normals"), py::arg("uv"),
+             py::arg("indices"), py::arg("images"), py::arg("world_to_clip"), py::arg("camera_positions"),
+             py::arg("visibility_masks") = py::none(), py::arg("viewports") = py::none(),
+             py::arg("resolution") = std::pair<std::uint32_t, std::uint32_t>{1024U, 1024U},
+             py::arg("blend_mode") = "weighted_average", py::arg("visibility_mode") = "ray_query",
+             py::arg("pcf_radius") = 1, py::arg("allow_visibility_fallback") = true, py::arg("softmax_scale") = 0.0f,
+             py::arg("mask_floor") = 0.0f)
         .def("refine_texture", &PythonTextureBaker::refine_texture, py::arg("initial_texture"), py::arg("positions"),
              py::arg("uv"), py::arg("indices"), py::arg("images"), py::arg("world_to_clip"),
              py::arg("visibility_masks") = py::none(), py::arg("viewports") = py::none(),
@@ -363,4 +464,10 @@ PYBIND11_MODULE(_photara_drender, m) {
              py::arg("seam_learning_rate") = 1e-3F);
 
     m.def("enumerate_devices", &Context::enumerate_devices);
+    m.def(
+        "suggest_softmax_scale",
+        [](const FloatInput& positions, const FloatInput& camera_positions) {
+            return suggest_softmax_scale(as_span(positions), as_span(camera_positions));
+        },
+        py::arg("positions"), py::arg("camera_positions"));
 }

@@ -391,4 +391,34 @@ inline uint32_t validate_texture(uint32_t tw, uint32_t th, uint32_t C, uint32_t 
     return static_cast<uint32_t>(P);
 }
 
+// Call-local device allocations, freed when the call returns.
+struct DeviceBuffers {
+    std::vector<void*> ptrs;
+    ~DeviceBuffers() {
+        for (void* p : ptrs) (void)hipFree(p);
+    }
+    template <class T>
+    T* alloc(size_t count) {
+        void* p = nullptr;
+        hip_check(hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)), "hipMalloc");
+        ptrs.push_back(p);
+        return static_cast<T*>(p);
+    }
+};
+
+// Device copy of a host array (device arrays are used in place).
+template <class T>
+const T* device_view(DeviceBuffers& bufs, const T* p, size_t count, hipStream_t st) {
+    if (!p) return nullptr;
+    if (is_device_pointer(p)) return p;
+    T* d = bufs.alloc<T>(count);
+    hip_check(hipMemcpyAsync(d, p, count * sizeof(T), hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+    return d;
+}
+
+inline void copy_any(void* dst, const void* src, size_t bytes, hipStream_t st) {
+    if (!dst || !bytes || dst == src) return;
+    hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, st), "hipMemcpyAsync");
+}
+
 }  // namespace pdrh

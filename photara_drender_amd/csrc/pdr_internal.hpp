@@ -146,4 +146,27 @@ hipError_t adam_step(float* tex, float* grad, float* m, float* v, uint32_t n, co
                      float cmax, uint32_t* step_counter, uint32_t* done_counter, hipStream_t st);
 hipError_t tv_pair_count(const int32_t* chart, uint32_t tw, uint32_t th, unsigned long long* out, hipStream_t st);
 
+// ---- bake.hip
+// One projection view of the bake, in device memory (read by k_bake_project).
+struct BakeView {
+    const float* shadow;  // [h,w,4] raster of the view (shadow map)
+    const float* photo;   // [h,w,channels]
+    const float* mask;    // [h,w] or null
+    float m[16];          // world_to_clip, row-major
+    float cam[3];
+    float weight;         // effective weight (times the softmax scale in softmax mode)
+    float depth_bias, min_cos, mask_floor;
+    float vp[4];          // resolved viewport
+    uint32_t width, height, channels, index;
+};
+
+hipError_t atlas_clip_positions(const float* uv, uint32_t V, float* out4, hipStream_t st);
+hipError_t bake_init(uint32_t P, float* acc3, float* accw, float* best, uint32_t* src, hipStream_t st);
+hipError_t bake_project(const float* araster, const float* apos, const float* anrm, uint32_t P, const BakeView* views,
+                        uint32_t nviews, uint32_t blend, uint32_t pcf, float* acc3, float* accw, float* best,
+                        uint32_t* src, hipStream_t st);
+hipError_t bake_finish(uint32_t P, uint32_t blend, const float* araster, const float* acc3, const float* accw,
+                       float* color4, float* confidence, float* valid, float* coverage, uint32_t* face_ids,
+                       hipStream_t st);
+
 }  // namespace pdr

@@ -15,20 +15,6 @@ using namespace pdrh;
 
 namespace {
 
-struct DeviceBuffers {
-    std::vector<void*> ptrs;
-    ~DeviceBuffers() {
-        for (void* p : ptrs) (void)hipFree(p);
-    }
-    template <class T>
-    T* alloc(size_t count) {
-        void* p = nullptr;
-        hip_check(hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)), "hipMalloc");
-        ptrs.push_back(p);
-        return static_cast<T*>(p);
-    }
-};
-
 struct GraphExec {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -37,20 +23,6 @@ struct GraphExec {
         if (graph) (void)hipGraphDestroy(graph);
     }
 };
-
-template <class T>
-const T* device_view(DeviceBuffers& bufs, const T* p, size_t count, hipStream_t st) {
-    if (!p) return nullptr;
-    if (is_device_pointer(p)) return p;
-    T* d = bufs.alloc<T>(count);
-    hip_check(hipMemcpyAsync(d, p, count * sizeof(T), hipMemcpyHostToDevice, st), "hipMemcpyAsync");
-    return d;
-}
-
-void copy_any(void* dst, const void* src, size_t bytes, hipStream_t st) {
-    if (!dst || !bytes) return;
-    hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, st), "hipMemcpyAsync");
-}
 
 }  // namespace
 

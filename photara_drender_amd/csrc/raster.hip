@@ -249,7 +249,7 @@ struct RecTri {  // what test_pixel reads
     float s0x, s0y, s1x, s1y, s2x, s2y, sgn;
 };
 
-// One 64-triangle batch per wave.  Each lane sets one triangle up; the lines
+// One batch of up to 64 triangles per wave.  Each lane sets one triangle up; the lines
 // (rows, or columns for triangles taller than wide) of the batch's triangles
 // are concatenated and dealt 64 at a time to the lanes, each lane computing
 // its line's span; the spans of those 64 lines are concatenated in turn and
@@ -260,7 +260,8 @@ __global__ __launch_bounds__(kBlock) void k_raster_small(ViewTable vt, const uin
                                                          uint32_t V, uint32_t cull, const float* __restrict__ sxt,
                                                          const float* __restrict__ syt,
                                                          unsigned long long* __restrict__ keys,
-                                                         uint4* __restrict__ queue, uint32_t* __restrict__ counters) {
+                                                         uint4* __restrict__ queue, uint32_t* __restrict__ counters,
+                                                         uint32_t bsz) {
     __shared__ FwdRecs R;
     const uint32_t view = blockIdx.y;
     const ViewDesc& vd = vt.v[view];
@@ -270,14 +271,14 @@ __global__ __launch_bounds__(kBlock) void k_raster_small(ViewTable vt, const uin
     unsigned long long* kbase = keys + vd.pixel_base;
     const float ax[2] = {span_alpha(vd, 0), span_alpha(vd, 1)};
     const float bx[2] = {span_beta(vd, 0), span_beta(vd, 1)};
-    const uint32_t stride = gridDim.x * kBlock;
-    for (uint32_t batch = (blockIdx.x * (kBlock / 64) + wave) * 64; batch < T; batch += stride) {
+    const uint32_t stride = gridDim.x * (kBlock / 64) * bsz;
+    for (uint32_t batch = (blockIdx.x * (kBlock / 64) + wave) * bsz; batch < T; batch += stride) {
         const uint32_t t = batch + lane;
         int32_t lines = 0;
         {
             ScreenTri r;
             LineSpans sp;
-            const bool ok = t < T && setup_forward_tri(vd, idx, t, V, cull, true, r, sp);
+            const bool ok = lane < bsz && t < T && setup_forward_tri(vd, idx, t, V, cull, true, r, sp);
             if (ok && r.work > kWaveWork) {
                 const uint32_t slabs = static_cast<uint32_t>(r.y1 - r.y0) / kSlabRows + 1u;
                 queue[atomicAdd(&counters[0], 1u)] = make_uint4(view, t, slabs, 0u);
@@ -614,30 +615,43 @@ struct BwdRecs {
     float4 sv[kBlock];  // span coefficients v0..v2, upper-bound mask bits
     int4 rg[kBlock];    // xa, xb, row base, unused
     int32_t slot[kBlock];
+    int32_t seg_first[kBlock];  // per lane: first list entry of its triangle (-1: none)
+    int32_t seg_end[kBlock];    // per lane: one past its triangle's last list entry
     float term[kBlock / 64][kList * 9];
-    uint32_t owner[kBlock / 64][kList];
 };
 
-// Lane `lane` adds, in list order, the buffered terms it owns (entries with
-// owner == lane are contiguous: the list is in batch order).
-__device__ __forceinline__ void fold_list(const float* __restrict__ lterm, const uint32_t* __restrict__ lown,
-                                          uint32_t n, uint32_t lane, bool mine, float s[9]) {
+// Lane `lane` adds, in list order, the buffered terms of its triangle
+// (entries [seg_first[lane], seg_end[lane]) of the list; they are contiguous
+// because the list is in batch order), then clears its segment.
+__device__ __forceinline__ void fold_list(const float* __restrict__ lterm, int32_t* seg_first, int32_t* seg_end,
+                                          uint32_t lane, float s[9]) {
     wave_lds_sync();
-    if (mine) {
-        uint32_t a = 0, b = n;  // first entry with owner >= lane
-        while (a < b) {
-            const uint32_t h = (a + b) >> 1;
-            if (lown[h] < lane) a = h + 1; else b = h;
+    const int32_t first = seg_first[lane];
+    if (first >= 0) {
+        const int32_t end = seg_end[lane];
+        int32_t e = first;
+        // entries are loaded four at a time (independent reads) and added in order
+        for (; e + 4 <= end; e += 4) {
+            float v[4][9];
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+#pragma unroll
+                for (int k = 0; k < 9; ++k) v[j][k] = lterm[(e + j) * 9 + k];
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+#pragma unroll
+                for (int k = 0; k < 9; ++k) s[k] += v[j][k];
         }
-        for (uint32_t e = a; e < n && lown[e] == lane; ++e)
+        for (; e < end; ++e)
 #pragma unroll
             for (int k = 0; k < 9; ++k) s[k] += lterm[e * 9 + k];
     }
+    seg_first[lane] = -1;
     wave_lds_sync();
 }
 
 // Per-triangle corner gradients summed over the reference's bounding box in
-// its row-major order.  Each wave takes a batch of 64 triangles.  Triangles
+// its row-major order.  Each wave takes a batch of up to 64 triangles.  Triangles
 // with more than kFlatWork estimated pixels (boxes up to kWaveBox pixels) are
 // walked by the whole wave, one at a time, adding the contributing pixels'
 // terms in ascending pixel order through ballot + readlane; larger boxes are
@@ -652,7 +666,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_corner_small(ViewTable vt, const 
                                                          const uint32_t* __restrict__ cid,
                                                          const uint32_t* __restrict__ view_flags,
                                                          float4* __restrict__ corner, uint4* __restrict__ queue,
-                                                         uint32_t* __restrict__ counters) {
+                                                         uint32_t* __restrict__ counters, uint32_t bsz) {
     __shared__ BwdRecs R;
     const uint32_t view = blockIdx.y;
     const ViewDesc& vd = vt.v[view];
@@ -660,21 +674,23 @@ __global__ __launch_bounds__(kBlock, 4) void k_corner_small(ViewTable vt, const 
     const uint32_t w0 = wave * 64;
     int32_t* slot = R.slot + w0;
     float* lterm = R.term[wave];
-    uint32_t* lown = R.owner[wave];
     const bool full = view_flags[view] != 0u;
     const float alpha = span_alpha(vd, 0), beta = span_beta(vd, 0);
     const uint32_t* cv = cid + vd.pixel_base;
     const float2* gr = reinterpret_cast<const float2*>(vd.grad_raster);
     float4* cview = corner + static_cast<size_t>(view) * T * 3;
-    const uint32_t stride = gridDim.x * kBlock;
-    for (uint32_t batch = (blockIdx.x * (kBlock / 64) + wave) * 64; batch < T; batch += stride) {
+    int32_t* seg_first = R.seg_first + w0;
+    int32_t* seg_end = R.seg_end + w0;
+    const uint32_t stride = gridDim.x * (kBlock / 64) * bsz;
+    for (uint32_t batch = (blockIdx.x * (kBlock / 64) + wave) * bsz; batch < T; batch += stride) {
         const uint32_t t = batch + lane;
+        const bool mine = lane < bsz && t < T;
         int32_t rows = 0;
         {
             BwdRegion g;
-            const bool ok = t < T && setup_backward(vd, idx, t, V, full, g);
+            const bool ok = mine && setup_backward(vd, idx, t, V, full, g);
             bool deferred = false;
-            if (t < T && !ok) {
+            if (mine && !ok) {
 #pragma unroll
                 for (int c = 0; c < 3; ++c) cview[t * 3 + c] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             } else if (ok && g.box > kWaveBox) {
@@ -749,6 +765,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_corner_small(ViewTable vt, const 
         if (rows > 0) R.rg[w0 + lane].z -= incl_l - rows;  // row = base + list position
         float s[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
         uint32_t nlist = 0;
+        seg_first[lane] = -1;
         wave_lds_sync();
         for (int32_t lc = 0; lc < nrows; lc += 64) {
             const int32_t f = lc + static_cast<int32_t>(lane);
@@ -788,7 +805,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_corner_small(ViewTable vt, const 
                 bool hit = false;
                 if (m < npix) {
                     const size_t pix = static_cast<size_t>(yy) * vd.width + static_cast<size_t>(x);
-                    if (cv[pix] == batch + static_cast<uint32_t>(oo) + 1u) {
+                    if (cv[pix] == batch + static_cast<uint32_t>(oo) + 1u) {  // oo < bsz: a flat-class lane
                         const uint32_t i = w0 + static_cast<uint32_t>(oo);
                         const float4 pp[3] = {R.p0[i], R.p1[i], R.p2[i]};
                         hit = pixel_terms(pp, sample_x(vd, x), syy, gr[pix * 2], t9);
@@ -797,20 +814,28 @@ __global__ __launch_bounds__(kBlock, 4) void k_corner_small(ViewTable vt, const 
                 const unsigned long long bal = __ballot(hit);
                 const uint32_t cnt = static_cast<uint32_t>(__popcll(bal));
                 if (nlist + cnt > kList) {
-                    fold_list(lterm, lown, nlist, lane, rows > 0, s);
+                    fold_list(lterm, seg_first, seg_end, lane, s);
                     nlist = 0;
                 }
+                // neighbouring hits in lane order: a hit whose predecessor belongs to
+                // another triangle starts its triangle's run, one whose successor does ends it
+                const unsigned long long below = bal & ((1ull << lane) - 1ull);
+                const unsigned long long above = bal & ~((2ull << lane) - 1ull);
+                const int32_t prev = below ? 63 - __builtin_clzll(below) : static_cast<int32_t>(lane);
+                const int32_t next = above ? __builtin_ctzll(above) : static_cast<int32_t>(lane);
+                const int32_t prev_o = __shfl(oo, prev, 64), next_o = __shfl(oo, next, 64);
                 if (hit) {
                     const uint32_t pos = nlist + __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(bal >> 32),
                                                                            __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(bal), 0u));
 #pragma unroll
                     for (int kk = 0; kk < 9; ++kk) lterm[pos * 9 + kk] = t9[kk];
-                    lown[pos] = static_cast<uint32_t>(oo);
+                    if ((!below || prev_o != oo) && seg_first[oo] < 0) seg_first[oo] = static_cast<int32_t>(pos);
+                    if (!above || next_o != oo) seg_end[oo] = static_cast<int32_t>(pos) + 1;
                 }
                 nlist += cnt;
             }
         }
-        fold_list(lterm, lown, nlist, lane, rows > 0, s);
+        fold_list(lterm, seg_first, seg_end, lane, s);
         if (rows > 0)
 #pragma unroll
             for (int c = 0; c < 3; ++c) cview[t * 3 + c] = make_float4(s[c * 3], s[c * 3 + 1], 0.0f, s[c * 3 + 2]);
@@ -907,7 +932,17 @@ __global__ void k_check_indices(const uint32_t* idx, size_t n, uint32_t V, uint3
     if (__any(bad) && (threadIdx.x & 63u) == 0) atomicOr(flags, 1u);
 }
 
-uint32_t tri_grid(uint32_t T) { return (T + kBlock - 1) / kBlock < 8192 ? (T + kBlock - 1) / kBlock : 8192; }
+// Triangles per wave batch: 64, or fewer for small meshes so that the launch
+// still has about 4096 waves (16 per CU) to overlap their latencies.
+uint32_t tri_batch(uint32_t T, uint32_t views) {
+    const uint64_t per = (static_cast<uint64_t>(T) * views + 4095) / 4096;
+    return per >= 64 ? 64u : (per <= 8 ? 8u : static_cast<uint32_t>(per));
+}
+
+uint32_t tri_grid(uint32_t T, uint32_t bsz) {
+    const uint32_t g = (T + (kBlock / 64) * bsz - 1) / ((kBlock / 64) * bsz);
+    return g < 8192 ? g : 8192;
+}
 
 }  // namespace
 
@@ -945,8 +980,9 @@ hipError_t raster_candidates(const ViewTable& vt, const uint32_t* idx, uint32_t 
                              const RasterScratch& s, hipStream_t st) {
     hipError_t e = fill_u32(s.counters, 0u, 4, st);
     if (e != hipSuccess) return e;
-    k_raster_small<<<dim3(tri_grid(T), vt.n), kBlock, 0, st>>>(vt, idx, T, V, cull, s.sx, s.sy, s.keys, s.queue,
-                                                               s.counters);
+    const uint32_t bsz = tri_batch(T, vt.n);
+    k_raster_small<<<dim3(tri_grid(T, bsz), vt.n), kBlock, 0, st>>>(vt, idx, T, V, cull, s.sx, s.sy, s.keys, s.queue,
+                                                                    s.counters, bsz);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     k_raster_large<<<kLargeBlocks, kBlock, 0, st>>>(vt, idx, T, V, cull, s.sx, s.sy, s.keys, s.queue, s.counters);
     return hipGetLastError();
@@ -1004,8 +1040,9 @@ hipError_t corner_gradients(const ViewTable& vt, const uint32_t* idx, uint32_t T
                             float4* corner, hipStream_t st) {
     hipError_t e = fill_u32(s.counters, 0u, 4, st);
     if (e != hipSuccess) return e;
-    k_corner_small<<<dim3(tri_grid(T), vt.n), kBlock, 0, st>>>(vt, idx, T, V, s.sx, s.sy, s.cid, s.view_flags, corner,
-                                                               s.queue, s.counters);
+    const uint32_t bsz = tri_batch(T, vt.n);
+    k_corner_small<<<dim3(tri_grid(T, bsz), vt.n), kBlock, 0, st>>>(vt, idx, T, V, s.sx, s.sy, s.cid, s.view_flags,
+                                                                    corner, s.queue, s.counters, bsz);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     k_corner_large<<<kLargeBlocks, kBlock, 0, st>>>(vt, idx, T, V, s.sx, s.sy, s.cid, s.view_flags, corner, s.queue,
                                                     s.counters);

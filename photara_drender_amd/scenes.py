@@ -331,6 +331,25 @@ def atlas_views(scene: AtlasScene, views=64, size=(768, 1024), seed=3, fov=45.0)
     return np.stack(mats).astype(np.float32)
 
 
+def camera_center(world_to_clip_matrix) -> np.ndarray:
+    """World position of a perspective camera: the point whose clip x, y and w vanish."""
+    m = np.asarray(world_to_clip_matrix, np.float64).reshape(4, 4)
+    rows = m[[0, 1, 3]]
+    return np.linalg.solve(rows[:, :3], -rows[:, 3]).astype(np.float32)
+
+
+def vertex_normals(positions, indices) -> np.ndarray:
+    """Area-weighted unit vertex normals of a triangle mesh."""
+    p = np.asarray(positions, np.float64)
+    t = np.asarray(indices, np.int64)
+    fn = np.cross(p[t[:, 1]] - p[t[:, 0]], p[t[:, 2]] - p[t[:, 0]])
+    n = np.zeros_like(p)
+    for c in range(3):
+        np.add.at(n, t[:, c], fn)
+    n /= np.maximum(np.linalg.norm(n, axis=1, keepdims=True), 1e-30)
+    return n.astype(np.float32)
+
+
 def ellipse_masks(views, size, seed=3, count=3):
     """Seeded binary visibility masks: 1 except inside a few random ellipses."""
     rng = np.random.default_rng(seed + 300)
