@@ -265,6 +265,48 @@ pdr_status pdr_bake(pdr_context ctx, const float* positions, const float* normal
                     float* color_out, float* confidence_out, uint32_t* source_view_out, float* valid_out,
                     float* coverage_out, uint32_t* face_ids_out, int* used_ray_query_out);
 
+/* ------------------------------------------------------- warm-start MLP */
+/* The learned warm start has no counterpart in the reference: a per-texel MLP
+ * 40 -> 64 -> 64 -> 3 (GELU, tanh form) that predicts the refined atlas colour
+ * from the texel's projected samples, added to their confidence-weighted mean.
+ * Parameters: PDR_MLP_PARAMS float32 values,
+ *   W1 [40,64] | b1 [64] | W2 [64,64] | b2 [64] | W3 [64,3] | b3 [3].
+ * Matrix products take bfloat16 operands (round to nearest even) and
+ * accumulate in float32.  Arrays are host or device memory as above; features
+ * must be 16-byte aligned. */
+#define PDR_MLP_FEATURES 40
+#define PDR_MLP_PARAMS 6979
+
+/* Features of the warm start, computed like pdr_bake (same arguments,
+ * validation and shadow-map visibility; blend_mode is ignored): for every
+ * covered texel, the 8 samples of highest weighted-average confidence
+ * (visibility * alpha * cos * weight, project_texture.hlsl:208-246), in
+ * descending order (ties keep view order), each [R, G, B, confidence, cos];
+ * unused slots and uncovered texels are zero.  features_out [height, width, 40]. */
+pdr_status pdr_bake_features(pdr_context ctx, const float* positions, const float* normals, const float* uv,
+                             uint32_t vertex_count, const uint32_t* indices, uint32_t triangle_count,
+                             const pdr_bake_view* views, uint32_t view_count, const pdr_bake_options* options,
+                             float* features_out);
+
+/* out_rgb [count, 3]: network output plus the samples' confidence-weighted mean
+ * colour; 0 for texels without samples (the bake's colour there). */
+pdr_status pdr_mlp_infer(pdr_context ctx, const float* features, uint64_t count, const float* params, float* out_rgb,
+                         void* stream);
+
+/* Gradient of loss = sum_t valid_t |y_t - target_t|^2 / (3 max(sum_t valid_t, 1))
+ * with respect to params: grad_out [PDR_MLP_PARAMS]; loss_count_out [2] =
+ * (loss, sum of valid) or NULL; valid [count] (0/1) or NULL for all texels. */
+pdr_status pdr_mlp_gradient(pdr_context ctx, const float* features, const float* target, const float* valid,
+                            uint64_t count, const float* params, float* grad_out, float* loss_count_out,
+                            void* stream);
+
+/* The gradient above followed by one Adam step (step counts from 1) that
+ * updates params, adam_m and adam_v in place. */
+pdr_status pdr_mlp_adam_step(pdr_context ctx, const float* features, const float* target, const float* valid,
+                             uint64_t count, float* params, float* adam_m, float* adam_v, uint32_t step,
+                             float learning_rate, float beta1, float beta2, float epsilon, float* loss_count_out,
+                             void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -315,16 +315,13 @@ def suggest_softmax_scale(positions, camera_positions):
 BLEND_MODES = {"best_view": 0, "weighted_average": 1, "softmax": 2}
 
 
-def bake(positions, normals, uv, indices, images, world_to_clip, camera_positions, visibility_masks=None,
-         viewports=None, resolution=(1024, 1024), blend_mode="weighted_average", pcf_radius=1, softmax_scale=0.0,
-         mask_floor=0.0, weights=None, depth_bias=1e-3, min_view_cosine=0.05):
-    """TextureBaker::bake (src/texture_baker.cpp:156-361) with shadow-map visibility.
-    Returns (color [H,W,4], confidence, source_view (u32), valid, coverage, face_ids (u32))."""
-    pos, nrm, uvs, idx = _f32(positions), _f32(normals), _f32(uv), _u32(indices)
+def _bake_views(positions, images, world_to_clip, camera_positions, visibility_masks, viewports, mode,
+                softmax_scale, mask_floor, weights, depth_bias, min_view_cosine):
+    """orc_bake_view records of the views (and the arrays they point into)."""
+    pos = _f32(positions)
     n = len(images)
     mats = _f32(np.asarray(world_to_clip).reshape(n, 16))
     cams = _f32(np.asarray(camera_positions).reshape(n, 3))
-    mode = BLEND_MODES[blend_mode]
     scale = float(softmax_scale)
     if mode == 2 and not (scale > 0 and np.isfinite(scale)):
         scale = suggest_softmax_scale(pos, cams)
@@ -352,6 +349,19 @@ def bake(positions, normals, uv, indices, images, world_to_clip, camera_position
         wk = 1.0 if weights is None else float(weights[k])
         v.weight = wk * scale if mode == 2 else wk
         v.depth_bias, v.min_view_cosine = depth_bias, min_view_cosine
+    return views, keep
+
+
+def bake(positions, normals, uv, indices, images, world_to_clip, camera_positions, visibility_masks=None,
+         viewports=None, resolution=(1024, 1024), blend_mode="weighted_average", pcf_radius=1, softmax_scale=0.0,
+         mask_floor=0.0, weights=None, depth_bias=1e-3, min_view_cosine=0.05):
+    """TextureBaker::bake (src/texture_baker.cpp:156-361) with shadow-map visibility.
+    Returns (color [H,W,4], confidence, source_view (u32), valid, coverage, face_ids (u32))."""
+    pos, nrm, uvs, idx = _f32(positions), _f32(normals), _f32(uv), _u32(indices)
+    n = len(images)
+    mode = BLEND_MODES[blend_mode]
+    views, keep = _bake_views(pos, images, world_to_clip, camera_positions, visibility_masks, viewports, mode,
+                              softmax_scale, mask_floor, weights, depth_bias, min_view_cosine)
     H, W = resolution
     color = np.empty((H, W, 4), np.float32)
     conf = np.empty((H, W), np.float32)
@@ -363,3 +373,20 @@ def bake(positions, normals, uv, indices, images, world_to_clip, camera_position
                           views, n, W, H, mode, pcf_radius, _p(color, _F32), _p(conf, _F32), _p(src, _U32),
                           _p(valid, _F32), _p(cov, _F32), _p(face, _U32)))
     return color, conf, src, valid, cov, face
+
+
+def bake_features(positions, normals, uv, indices, images, world_to_clip, camera_positions, visibility_masks=None,
+                  viewports=None, resolution=(1024, 1024), pcf_radius=1, mask_floor=0.0, weights=None,
+                  depth_bias=1e-3, min_view_cosine=0.05):
+    """Warm-start features [H, W, 40]: per covered texel the 8 samples of highest
+    weighted-average confidence (project_texture.hlsl:208), descending, ties in
+    view order, each [R, G, B, confidence, cos]; zero elsewhere."""
+    pos, nrm, uvs, idx = _f32(positions), _f32(normals), _f32(uv), _u32(indices)
+    n = len(images)
+    views, keep = _bake_views(pos, images, world_to_clip, camera_positions, visibility_masks, viewports, 1, 0.0,
+                              mask_floor, weights, depth_bias, min_view_cosine)
+    H, W = resolution
+    feats = np.empty((H, W, 40), np.float32)
+    _check(lib().orc_bake_features(_p(pos, _F32), _p(nrm, _F32), _p(uvs, _F32), pos.shape[0], _p(idx, _U32),
+                                   idx.shape[0], views, n, W, H, pcf_radius, _p(feats, _F32)))
+    return feats

@@ -878,6 +878,79 @@ float bake_shadow(const orc_bake_view& v, const float* shadow, float px, float p
     return count > 0.0f ? visible / count : 0.0f;
 }
 
+// One view's sample at atlas texel p (shaders/project_texture.hlsl:126-175):
+// false when the view contributes nothing.
+struct BakeSample {
+    float vis, cosv, dl, col[3], alpha;
+};
+
+bool project_sample(const orc_bake_view& v, const float* shadow, const float* apos, const float* anrm, size_t p,
+                    uint32_t pcf, BakeSample& s) {
+    const float x = apos[p * 3], y = apos[p * 3 + 1], z = apos[p * 3 + 2];
+    const float n0 = anrm[p * 3], n1 = anrm[p * 3 + 1], n2 = anrm[p * 3 + 2];
+    const float nl = std::sqrt(n0 * n0 + n1 * n1 + n2 * n2);
+    const float nx = n0 / nl, ny = n1 / nl, nz = n2 / nl;
+    const float* m = v.world_to_clip;
+    float c[4];
+    for (int r = 0; r < 4; ++r) c[r] = m[r * 4] * x + m[r * 4 + 1] * y + m[r * 4 + 2] * z + m[r * 4 + 3] * 1.0f;
+    if (c[3] <= 1e-8f) return false;
+    const float ndx = c[0] / c[3], ndy = c[1] / c[3], ndz = c[2] / c[3];
+    if (ndx < -1.0f || ndy < -1.0f || ndx > 1.0f || ndy > 1.0f || ndz < -1.0f || ndz > 1.0f) return false;
+    const float px = v.viewport[0] + (ndx * 0.5f + 0.5f) * v.viewport[2];
+    const float py = v.viewport[1] + (ndy * 0.5f + 0.5f) * v.viewport[3];
+    if (px < v.viewport[0] || py < v.viewport[1] || px >= v.viewport[0] + v.viewport[2] ||
+        py >= v.viewport[1] + v.viewport[3])
+        return false;
+    float vis = bake_shadow(v, shadow, px, py, ndz, pcf);
+    if (v.mask) vis *= clampf(bake_mask(v, px, py), 0.0f, 1.0f);
+    const float dx = v.camera[0] - x, dy = v.camera[1] - y, dz = v.camera[2] - z;
+    const float dl = std::sqrt(dx * dx + dy * dy + dz * dz);
+    const float vx = dx / dl, vy = dy / dl, vz = dz / dl;
+    const float cosv = nx * vx + ny * vy + nz * vz;
+    if (cosv < v.min_view_cosine || vis <= 0.0f) return false;
+    s.vis = vis;
+    s.cosv = cosv;
+    s.dl = dl;
+    for (uint32_t ch = 0; ch < 3; ++ch) s.col[ch] = bake_photo(v, px, py, ch);
+    s.alpha = v.channels == 4 ? bake_photo(v, px, py, 3) : 1.0f;
+    return true;
+}
+
+// Atlas raster of uv * 2 - 1 and the texels' interpolated positions and
+// normals (src/texture_baker.cpp:85-94, 183-192).
+int atlas_frames(const float* pos3, const float* normal3, const float* uv2, uint32_t V, const uint32_t* idx, uint32_t T,
+                 uint32_t W, uint32_t H, std::vector<float>& araster, std::vector<float>& apos,
+                 std::vector<float>& anrm) {
+    const size_t P = static_cast<size_t>(W) * H;
+    std::vector<float> aclip(static_cast<size_t>(V) * 4);
+    for (size_t v = 0; v < V; ++v) {
+        aclip[v * 4] = uv2[v * 2] * 2.0f - 1.0f;
+        aclip[v * 4 + 1] = uv2[v * 2 + 1] * 2.0f - 1.0f;
+        aclip[v * 4 + 2] = 0.0f;
+        aclip[v * 4 + 3] = 1.0f;
+    }
+    araster.assign(P * 4, 0.0f);
+    apos.assign(P * 3, 0.0f);
+    anrm.assign(P * 3, 0.0f);
+    float vp_out[4];
+    const int rc = orc_rasterize_forward(aclip.data(), V, idx, T, W, H, 0, 0, nullptr, araster.data(), nullptr, vp_out);
+    if (rc) return rc;
+    orc_interpolate_forward(pos3, V, 3, idx, T, araster.data(), static_cast<uint32_t>(P), apos.data());
+    orc_interpolate_forward(normal3, V, 3, idx, T, araster.data(), static_cast<uint32_t>(P), anrm.data());
+    return 0;
+}
+
+// Shadow raster of one view (src/texture_baker.cpp:266-276).
+int shadow_map(const float* pos3, uint32_t V, const uint32_t* idx, uint32_t T, const orc_bake_view& v,
+               std::vector<float>& shadow) {
+    std::vector<float> clip(static_cast<size_t>(V) * 4);
+    shadow.assign(static_cast<size_t>(v.width) * v.height * 4, 0.0f);
+    orc_transform_positions(pos3, V, v.world_to_clip, clip.data());
+    float vp_out[4];
+    return orc_rasterize_forward(clip.data(), V, idx, T, v.width, v.height, 0, 0, v.viewport, shadow.data(), nullptr,
+                                 vp_out);
+}
+
 }  // namespace
 
 float orc_suggest_softmax_scale(const float* pos3, uint32_t V, const float* cams, uint32_t nc) {
@@ -900,63 +973,29 @@ int orc_bake(const float* pos3, const float* normal3, const float* uv2, uint32_t
              float* color4, float* confidence, uint32_t* source_view, float* valid, float* coverage,
              uint32_t* face_ids) {
     const size_t P = static_cast<size_t>(W) * H;
-    // src/texture_baker.cpp:85-94 atlas clip positions, then the atlas raster
-    std::vector<float> aclip(static_cast<size_t>(V) * 4);
-    for (size_t v = 0; v < V; ++v) {
-        aclip[v * 4] = uv2[v * 2] * 2.0f - 1.0f;
-        aclip[v * 4 + 1] = uv2[v * 2 + 1] * 2.0f - 1.0f;
-        aclip[v * 4 + 2] = 0.0f;
-        aclip[v * 4 + 3] = 1.0f;
-    }
-    std::vector<float> araster(P * 4), apos(P * 3), anrm(P * 3);
-    float vp_out[4];
-    int rc = orc_rasterize_forward(aclip.data(), V, idx, T, W, H, 0, 0, nullptr, araster.data(), nullptr, vp_out);
+    std::vector<float> araster, apos, anrm;
+    int rc = atlas_frames(pos3, normal3, uv2, V, idx, T, W, H, araster, apos, anrm);
     if (rc) return rc;
-    orc_interpolate_forward(pos3, V, 3, idx, T, araster.data(), static_cast<uint32_t>(P), apos.data());
-    orc_interpolate_forward(normal3, V, 3, idx, T, araster.data(), static_cast<uint32_t>(P), anrm.data());
     // src/texture_baker.cpp:193-204 coverage and face ids
     for (size_t p = 0; p < P; ++p) {
         const float id = std::round(araster[p * 4 + 3]);
         coverage[p] = id != 0.0f ? 1.0f : 0.0f;
         if (face_ids) face_ids[p] = coverage[p] > 0.5f ? static_cast<uint32_t>(id) - 1u : 0xFFFFFFFFu;
     }
-    std::vector<float> acc(P * 3, 0.0f), accw(P, 0.0f), best(P, 0.0f);
+    std::vector<float> acc(P * 3, 0.0f), accw(P, 0.0f), best(P, 0.0f), shadow;
     for (size_t p = 0; p < P; ++p) source_view[p] = 0xFFFFFFFFu;
     for (uint32_t vi = 0; vi < nviews; ++vi) {
         const orc_bake_view& v = views[vi];
-        std::vector<float> clip(static_cast<size_t>(V) * 4), shadow(static_cast<size_t>(v.width) * v.height * 4);
-        orc_transform_positions(pos3, V, v.world_to_clip, clip.data());
-        rc = orc_rasterize_forward(clip.data(), V, idx, T, v.width, v.height, 0, 0, v.viewport, shadow.data(), nullptr,
-                                   vp_out);
+        rc = shadow_map(pos3, V, idx, T, v, shadow);
         if (rc) return rc;
-        const float* m = v.world_to_clip;
 #pragma omp parallel for schedule(static)
         for (int64_t pi = 0; pi < static_cast<int64_t>(P); ++pi) {
             const size_t p = static_cast<size_t>(pi);
             if (decode_id(araster[p * 4 + 3]) == 0u) continue;  // project_texture.hlsl:121-125
-            const float x = apos[p * 3], y = apos[p * 3 + 1], z = apos[p * 3 + 2];
-            const float n0 = anrm[p * 3], n1 = anrm[p * 3 + 1], n2 = anrm[p * 3 + 2];
-            const float nl = std::sqrt(n0 * n0 + n1 * n1 + n2 * n2);
-            const float nx = n0 / nl, ny = n1 / nl, nz = n2 / nl;
-            float c[4];
-            for (int r = 0; r < 4; ++r) c[r] = m[r * 4] * x + m[r * 4 + 1] * y + m[r * 4 + 2] * z + m[r * 4 + 3] * 1.0f;
-            if (c[3] <= 1e-8f) continue;
-            const float ndx = c[0] / c[3], ndy = c[1] / c[3], ndz = c[2] / c[3];
-            if (ndx < -1.0f || ndy < -1.0f || ndx > 1.0f || ndy > 1.0f || ndz < -1.0f || ndz > 1.0f) continue;
-            const float px = v.viewport[0] + (ndx * 0.5f + 0.5f) * v.viewport[2];
-            const float py = v.viewport[1] + (ndy * 0.5f + 0.5f) * v.viewport[3];
-            if (px < v.viewport[0] || py < v.viewport[1] || px >= v.viewport[0] + v.viewport[2] ||
-                py >= v.viewport[1] + v.viewport[3])
-                continue;
-            float vis = bake_shadow(v, shadow.data(), px, py, ndz, pcf);
-            if (v.mask) vis *= clampf(bake_mask(v, px, py), 0.0f, 1.0f);
-            const float dx = v.camera[0] - x, dy = v.camera[1] - y, dz = v.camera[2] - z;
-            const float dl = std::sqrt(dx * dx + dy * dy + dz * dz);
-            const float vx = dx / dl, vy = dy / dl, vz = dz / dl;
-            const float cosv = nx * vx + ny * vy + nz * vz;
-            if (cosv < v.min_view_cosine || vis <= 0.0f) continue;
-            const float col[3] = {bake_photo(v, px, py, 0), bake_photo(v, px, py, 1), bake_photo(v, px, py, 2)};
-            const float alpha = v.channels == 4 ? bake_photo(v, px, py, 3) : 1.0f;
+            BakeSample s;
+            if (!project_sample(v, shadow.data(), apos.data(), anrm.data(), p, pcf, s)) continue;
+            const float vis = s.vis, cosv = s.cosv, dl = s.dl, alpha = s.alpha;
+            const float* col = s.col;
             if (blend == 2) {  // project_texture.hlsl:176-206
                 const float dist = std::max(dl, 1e-6f);
                 const float footprint = cosv / (dist * dist);
@@ -1011,6 +1050,42 @@ int orc_bake(const float* pos3, const float* normal3, const float* uv2, uint32_t
         color4[p * 4 + 1] = acc[p * 3 + 1] / d;
         color4[p * 4 + 2] = acc[p * 3 + 2] / d;
         color4[p * 4 + 3] = valid[p];
+    }
+    return 0;
+}
+
+int orc_bake_features(const float* pos3, const float* normal3, const float* uv2, uint32_t V, const uint32_t* idx,
+                      uint32_t T, const orc_bake_view* views, uint32_t nviews, uint32_t W, uint32_t H, uint32_t pcf,
+                      float* features40) {
+    const size_t P = static_cast<size_t>(W) * H;
+    const int K = 8;
+    std::vector<float> araster, apos, anrm, shadow;
+    int rc = atlas_frames(pos3, normal3, uv2, V, idx, T, W, H, araster, apos, anrm);
+    if (rc) return rc;
+    std::fill(features40, features40 + P * 5 * K, 0.0f);
+    for (uint32_t vi = 0; vi < nviews; ++vi) {
+        const orc_bake_view& v = views[vi];
+        rc = shadow_map(pos3, V, idx, T, v, shadow);
+        if (rc) return rc;
+#pragma omp parallel for schedule(static)
+        for (int64_t pi = 0; pi < static_cast<int64_t>(P); ++pi) {
+            const size_t p = static_cast<size_t>(pi);
+            if (decode_id(araster[p * 4 + 3]) == 0u) continue;
+            BakeSample s;
+            if (!project_sample(v, shadow.data(), apos.data(), anrm.data(), p, pcf, s)) continue;
+            const float conf = s.vis * s.alpha * s.cosv * v.weight;  // project_texture.hlsl:208
+            if (conf <= 0.0f) continue;
+            // the list is sorted by confidence, descending; a new sample goes after every slot of equal or
+            // higher confidence, and the last slot falls off
+            float* f = features40 + p * 5 * K;
+            int at = 0;
+            while (at < K && f[at * 5 + 3] >= conf) ++at;
+            if (at == K) continue;
+            for (int i = K - 1; i > at; --i)
+                for (int e = 0; e < 5; ++e) f[i * 5 + e] = f[(i - 1) * 5 + e];
+            const float entry[5] = {s.col[0], s.col[1], s.col[2], conf, s.cosv};
+            for (int e = 0; e < 5; ++e) f[at * 5 + e] = entry[e];
+        }
     }
     return 0;
 }

@@ -141,6 +141,14 @@ int orc_bake(const float* pos3, const float* normal3, const float* uv2, uint32_t
              uint32_t blend_mode, uint32_t pcf_radius, float* color4, float* confidence, uint32_t* source_view,
              float* valid, float* coverage, uint32_t* face_ids);
 
+/* Warm-start features (project-specified): per covered atlas texel the 8
+ * samples of highest weighted-average confidence vis * alpha * cos * weight
+ * (project_texture.hlsl:208), descending, ties in view order, each
+ * [R, G, B, confidence, cos]; zero elsewhere.  features40 [height, width, 40]. */
+int orc_bake_features(const float* pos3, const float* normal3, const float* uv2, uint32_t V, const uint32_t* idx,
+                      uint32_t T, const orc_bake_view* views, uint32_t nviews, uint32_t width, uint32_t height,
+                      uint32_t pcf_radius, float* features40);
+
 /* src/texture_padding.cpp:187-222 */
 float orc_suggest_softmax_scale(const float* pos3, uint32_t V, const float* cameras3, uint32_t ncameras);
 

@@ -97,7 +97,15 @@ _SIGNATURES = {
                       _P, _P, _P, _P, _P, ctypes.POINTER(_I)]),
     "pdr_refine": (_I, [_P, _P, _P, _U32, _P, _P, _U32, ctypes.POINTER(RefineView), _U32, _P, _U32,
                         ctypes.POINTER(RefineOptions), _P, _P, _P, ctypes.POINTER(ctypes.c_double)]),
+    "pdr_bake_features": (_I, [_P, _P, _P, _P, _U32, _P, _U32, ctypes.POINTER(BakeView), _U32,
+                               ctypes.POINTER(BakeOptions), _P]),
+    "pdr_mlp_infer": (_I, [_P, _P, ctypes.c_uint64, _P, _P, _P]),
+    "pdr_mlp_gradient": (_I, [_P, _P, _P, _P, ctypes.c_uint64, _P, _P, _P, _P]),
+    "pdr_mlp_adam_step": (_I, [_P, _P, _P, _P, ctypes.c_uint64, _P, _P, _P, _U32, _F, _F, _F, _F, _P, _P]),
 }
+
+MLP_FEATURES = 40
+MLP_PARAMS = 6979
 
 EXPORTED_SYMBOLS = tuple(_SIGNATURES)
 

@@ -74,6 +74,55 @@ __device__ __forceinline__ float bake_shadow(const BakeView& v, float px, float 
     return count > 0.0f ? visible / count : 0.0f;
 }
 
+// One view's sample at a texel (project_texture.hlsl:126-175): projection into
+// the view, visibility (shadow map, mask), view cosine and the photo's colour;
+// false when the view contributes nothing.
+struct BakeSample {
+    float vis, cosv, dl, cr, cg, cb, alpha;
+};
+
+__device__ __forceinline__ bool bake_sample(const BakeView& v, float x, float y, float z, float nx, float ny, float nz,
+                                            int32_t pcf, BakeSample& s) {
+    const float* m = v.m;
+    float c[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) c[r] = m[r * 4] * x + m[r * 4 + 1] * y + m[r * 4 + 2] * z + m[r * 4 + 3] * 1.0f;
+    if (c[3] <= 1e-8f) return false;
+    const float ndx = c[0] / c[3], ndy = c[1] / c[3], ndz = c[2] / c[3];
+    if (ndx < -1.0f || ndy < -1.0f || ndx > 1.0f || ndy > 1.0f || ndz < -1.0f || ndz > 1.0f) return false;
+    const float px = v.vp[0] + (ndx * 0.5f + 0.5f) * v.vp[2];
+    const float py = v.vp[1] + (ndy * 0.5f + 0.5f) * v.vp[3];
+    if (px < v.vp[0] || py < v.vp[1] || px >= v.vp[0] + v.vp[2] || py >= v.vp[1] + v.vp[3]) return false;
+    float vis = bake_shadow(v, px, py, ndz, pcf);
+    if (v.mask) vis *= clampf(bake_mask(v, px, py), 0.0f, 1.0f);
+    const float dx = v.cam[0] - x, dy = v.cam[1] - y, dz = v.cam[2] - z;
+    const float dl = sqrtf(dx * dx + dy * dy + dz * dz);
+    const float vx = dx / dl, vy = dy / dl, vz = dz / dl;
+    const float cosv = nx * vx + ny * vy + nz * vz;
+    if (cosv < v.min_cos || vis <= 0.0f) return false;
+    s.vis = vis;
+    s.cosv = cosv;
+    s.dl = dl;
+    s.cr = bake_photo(v, px, py, 0);
+    s.cg = bake_photo(v, px, py, 1);
+    s.cb = bake_photo(v, px, py, 2);
+    s.alpha = v.channels == 4 ? bake_photo(v, px, py, 3) : 1.0f;
+    return true;
+}
+
+// Texel position and unit normal (project_texture.hlsl:126-131).
+__device__ __forceinline__ void texel_frame(const float* __restrict__ apos, const float* __restrict__ anrm, uint32_t p,
+                                            float pos[3], float nrm[3]) {
+    pos[0] = apos[p * 3];
+    pos[1] = apos[p * 3 + 1];
+    pos[2] = apos[p * 3 + 2];
+    const float n0 = anrm[p * 3], n1 = anrm[p * 3 + 1], n2 = anrm[p * 3 + 2];
+    const float nl = sqrtf(n0 * n0 + n1 * n1 + n2 * n2);
+    nrm[0] = n0 / nl;
+    nrm[1] = n1 / nl;
+    nrm[2] = n2 / nl;
+}
+
 __global__ __launch_bounds__(kBlock) void k_bake_project(const float* __restrict__ araster,
                                                          const float* __restrict__ apos,
                                                          const float* __restrict__ anrm, uint32_t P,
@@ -83,34 +132,16 @@ __global__ __launch_bounds__(kBlock) void k_bake_project(const float* __restrict
                                                          uint32_t* __restrict__ src) {
     const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
     if (p >= P || decode_id(araster[static_cast<size_t>(p) * 4 + 3]) == 0u) return;  // project_texture.hlsl:121-125
-    const float x = apos[p * 3], y = apos[p * 3 + 1], z = apos[p * 3 + 2];
-    const float n0 = anrm[p * 3], n1 = anrm[p * 3 + 1], n2 = anrm[p * 3 + 2];
-    const float nl = sqrtf(n0 * n0 + n1 * n1 + n2 * n2);
-    const float nx = n0 / nl, ny = n1 / nl, nz = n2 / nl;
+    float pos[3], nrm[3];
+    texel_frame(apos, anrm, p, pos, nrm);
     float a0 = acc3[p * 3], a1 = acc3[p * 3 + 1], a2 = acc3[p * 3 + 2];
     float aw = accw[p], bst = best[p];
     uint32_t sv = src[p];
     for (uint32_t k = 0; k < nviews; ++k) {
         const BakeView& v = views[k];
-        const float* m = v.m;
-        float c[4];
-#pragma unroll
-        for (int r = 0; r < 4; ++r) c[r] = m[r * 4] * x + m[r * 4 + 1] * y + m[r * 4 + 2] * z + m[r * 4 + 3] * 1.0f;
-        if (c[3] <= 1e-8f) continue;
-        const float ndx = c[0] / c[3], ndy = c[1] / c[3], ndz = c[2] / c[3];
-        if (ndx < -1.0f || ndy < -1.0f || ndx > 1.0f || ndy > 1.0f || ndz < -1.0f || ndz > 1.0f) continue;
-        const float px = v.vp[0] + (ndx * 0.5f + 0.5f) * v.vp[2];
-        const float py = v.vp[1] + (ndy * 0.5f + 0.5f) * v.vp[3];
-        if (px < v.vp[0] || py < v.vp[1] || px >= v.vp[0] + v.vp[2] || py >= v.vp[1] + v.vp[3]) continue;
-        float vis = bake_shadow(v, px, py, ndz, pcf);
-        if (v.mask) vis *= clampf(bake_mask(v, px, py), 0.0f, 1.0f);
-        const float dx = v.cam[0] - x, dy = v.cam[1] - y, dz = v.cam[2] - z;
-        const float dl = sqrtf(dx * dx + dy * dy + dz * dz);
-        const float vx = dx / dl, vy = dy / dl, vz = dz / dl;
-        const float cosv = nx * vx + ny * vy + nz * vz;
-        if (cosv < v.min_cos || vis <= 0.0f) continue;
-        const float cr = bake_photo(v, px, py, 0), cg = bake_photo(v, px, py, 1), cb = bake_photo(v, px, py, 2);
-        const float alpha = v.channels == 4 ? bake_photo(v, px, py, 3) : 1.0f;
+        BakeSample s;
+        if (!bake_sample(v, pos[0], pos[1], pos[2], nrm[0], nrm[1], nrm[2], pcf, s)) continue;
+        const float vis = s.vis, cosv = s.cosv, dl = s.dl, cr = s.cr, cg = s.cg, cb = s.cb, alpha = s.alpha;
         if (blend == 2) {  // project_texture.hlsl:176-206, online softmax
             const float dist = maxf_std(dl, 1e-6f);
             const float footprint = cosv / (dist * dist);
@@ -162,6 +193,56 @@ __global__ __launch_bounds__(kBlock) void k_bake_project(const float* __restrict
     src[p] = sv;
 }
 
+// Warm-start features: for each covered texel the kFeatureViews samples of
+// highest weighted-average confidence vis * alpha * cos * weight, in
+// descending order (ties keep view order), each slot [R, G, B, confidence,
+// cos]; unused slots are zero.  feats [P, 40] carries the list between launches.
+constexpr int kFeatureViews = 8;
+
+__global__ __launch_bounds__(kBlock) void k_bake_features(const float* __restrict__ araster,
+                                                          const float* __restrict__ apos,
+                                                          const float* __restrict__ anrm, uint32_t P,
+                                                          const BakeView* __restrict__ views, uint32_t nviews,
+                                                          int32_t pcf, float* __restrict__ feats) {
+    const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= P || decode_id(araster[static_cast<size_t>(p) * 4 + 3]) == 0u) return;
+    float pos[3], nrm[3];
+    texel_frame(apos, anrm, p, pos, nrm);
+    float f[kFeatureViews * 5];
+    float4* row = reinterpret_cast<float4*>(feats + static_cast<size_t>(p) * kFeatureViews * 5);
+#pragma unroll
+    for (int i = 0; i < kFeatureViews * 5 / 4; ++i) {
+        const float4 q = row[i];
+        f[4 * i] = q.x;
+        f[4 * i + 1] = q.y;
+        f[4 * i + 2] = q.z;
+        f[4 * i + 3] = q.w;
+    }
+    for (uint32_t k = 0; k < nviews; ++k) {
+        const BakeView& v = views[k];
+        BakeSample s;
+        if (!bake_sample(v, pos[0], pos[1], pos[2], nrm[0], nrm[1], nrm[2], pcf, s)) continue;
+        const float conf = s.vis * s.alpha * s.cosv * v.weight;  // project_texture.hlsl:208
+        if (conf <= 0.0f) continue;
+        // insertion into the descending list: the carried entry swaps with every smaller slot
+        float c[5] = {s.cr, s.cg, s.cb, conf, s.cosv};
+#pragma unroll
+        for (int i = 0; i < kFeatureViews; ++i) {
+            if (c[3] > f[5 * i + 3]) {
+#pragma unroll
+                for (int e = 0; e < 5; ++e) {
+                    const float t = f[5 * i + e];
+                    f[5 * i + e] = c[e];
+                    c[e] = t;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < kFeatureViews * 5 / 4; ++i)
+        row[i] = make_float4(f[4 * i], f[4 * i + 1], f[4 * i + 2], f[4 * i + 3]);
+}
+
 // src/texture_baker.cpp:193-204 and :339-357: outputs from the blend state.
 __global__ void k_bake_finish(uint32_t P, uint32_t blend, const float* __restrict__ araster,
                               const float* __restrict__ acc3, const float* __restrict__ accw, float4* __restrict__ color,
@@ -211,6 +292,13 @@ hipError_t bake_project(const float* araster, const float* apos, const float* an
                         uint32_t* src, hipStream_t st) {
     k_bake_project<<<grid_of(P), kBlock, 0, st>>>(araster, apos, anrm, P, views, nviews, blend,
                                                   static_cast<int32_t>(pcf), acc3, accw, best, src);
+    return hipGetLastError();
+}
+
+hipError_t bake_features(const float* araster, const float* apos, const float* anrm, uint32_t P,
+                         const BakeView* views, uint32_t nviews, uint32_t pcf, float* feats, hipStream_t st) {
+    k_bake_features<<<grid_of(P), kBlock, 0, st>>>(araster, apos, anrm, P, views, nviews, static_cast<int32_t>(pcf),
+                                                   feats);
     return hipGetLastError();
 }
 

@@ -169,4 +169,20 @@ hipError_t bake_finish(uint32_t P, uint32_t blend, const float* araster, const f
                        float* color4, float* confidence, float* valid, float* coverage, uint32_t* face_ids,
                        hipStream_t st);
 
+// Warm-start features [P, 40]: zero before the first launch, then one launch per view chunk.
+hipError_t bake_features(const float* araster, const float* apos, const float* anrm, uint32_t P,
+                         const BakeView* views, uint32_t nviews, uint32_t pcf, float* feats, hipStream_t st);
+
+// ---- mlp.hip: the warm-start MLP 40 -> 64 -> 64 -> 3
+constexpr uint32_t kMlpFeatures = 40;
+constexpr uint32_t kMlpParams = 6979;
+constexpr uint32_t kMlpPartStride = kMlpParams + 2;  // per-wave partials: gradient, loss sum, valid count
+constexpr uint32_t kMlpMaxWaves = 2048;
+uint32_t mlp_grad_waves(uint64_t n);
+hipError_t mlp_infer(const float* x, uint32_t n, const float* w, float* out, hipStream_t st);
+hipError_t mlp_grad(const float* x, const float* target, const float* valid, uint32_t n, const float* w, float* part,
+                    uint32_t waves, hipStream_t st);
+hipError_t mlp_reduce(const float* part, uint32_t waves, float* grad, float* loss_count, float* params, float* m1,
+                      float* m2, float lr, float beta1, float beta2, float eps, float b1c, float b2c, hipStream_t st);
+
 }  // namespace pdr

@@ -1,0 +1,276 @@
+"""Learned warm start of the atlas refinement (project-specified; the reference
+has no counterpart).
+
+Per texel, the 8 highest-confidence projected samples ``[R, G, B, confidence,
+cos]`` (pdr_bake_features) go through a 40 -> 64 -> 64 -> 3 GELU network on
+matrix cores (pdr_mlp_infer); its output, added to the samples'
+confidence-weighted mean colour, starts the Adam refinement in place of the
+weighted-average bake.  The network is trained on the GPU by the library's own
+gradient kernels (pdr_mlp_adam_step) on (features, refined atlas) pairs that
+the plain method - bake followed by the full refinement - produces on seeded
+scenes.  The plain method stays the fallback and the judge: a warm-started
+refinement is scored by the same photometric PSNR.
+
+Run ``python -m photara_drender_amd.warmstart --train`` to regenerate the
+shipped weights (``warmstart_mlp.npy``).
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import math
+import os
+import sys
+import time
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _native
+
+WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "warmstart_mlp.npy")
+IN, HID, OUT = 40, 64, 3
+PARAMS = _native.MLP_PARAMS
+
+# Training scenes: the 2048^2 / 64-view refinement workload scaled to a 512^2
+# atlas with 16 views (same texels per chart, mesh vertices per texel and pixels
+# per texel over all views).
+TRAIN_SCENE = dict(nx=250, ny=125, charts=4, atlas_size=512, views=16, size=(384, 512), noise=0.02)
+TRAIN_SEED0 = 1000
+
+
+def init_params(seed=0) -> np.ndarray:
+    """He-initialised hidden layers and a zero output layer (the network starts as the base colour)."""
+    rng = np.random.default_rng(seed)
+    w1 = rng.normal(0.0, math.sqrt(2.0 / IN), (IN, HID))
+    w2 = rng.normal(0.0, math.sqrt(2.0 / HID), (HID, HID))
+    parts = [w1, np.zeros(HID), w2, np.zeros(HID), np.zeros((HID, OUT)), np.zeros(OUT)]
+    return np.concatenate([p.ravel() for p in parts]).astype(np.float32)
+
+
+def _stream():
+    import torch
+
+    return torch.cuda.current_stream().cuda_stream
+
+
+class WarmStartMLP:
+    """Float32 master parameters (and Adam moments) of the warm-start network on one GPU."""
+
+    def __init__(self, context, params=None, device="cuda:0"):
+        import torch
+
+        self.context = context
+        self.device = torch.device(device)
+        p = init_params(0) if params is None else np.asarray(params, np.float32).reshape(-1)
+        if p.size != PARAMS:
+            raise ValueError(f"warm-start parameters must have {PARAMS} values")
+        self.params = torch.tensor(p, device=self.device)
+        self.m = torch.zeros_like(self.params)
+        self.v = torch.zeros_like(self.params)
+        self.step = 0
+        self._loss = torch.zeros(2, dtype=torch.float32, device=self.device)
+
+    @classmethod
+    def load(cls, context, path: str = WEIGHTS, device="cuda:0") -> "WarmStartMLP":
+        return cls(context, np.load(path, allow_pickle=False), device)
+
+    def save(self, path: str) -> None:
+        np.save(path, self.params.detach().cpu().numpy().astype(np.float32), allow_pickle=False)
+
+    def _features(self, features):
+        import torch
+
+        x = features.reshape(-1, IN)
+        if x.dtype != torch.float32 or not x.is_contiguous() or x.data_ptr() % 16:
+            x = x.to(torch.float32).contiguous().clone()
+        return x
+
+    def predict(self, features, clamp=(0.0, 1.0)):
+        """Warm-start colour [..., 3] of features [..., 40] (0 where a texel has no sample)."""
+        import torch
+
+        x = self._features(features)
+        out = torch.empty((x.shape[0], OUT), dtype=torch.float32, device=x.device)
+        _native.check(_native.lib().pdr_mlp_infer(self.context.handle, x.data_ptr(), x.shape[0],
+                                                  self.params.data_ptr(), out.data_ptr(), _stream()))
+        if clamp is not None:
+            out.clamp_(clamp[0], clamp[1])
+        return out.reshape(features.shape[:-1] + (OUT,))
+
+    def gradient(self, features, target, valid=None):
+        """(gradient [PARAMS], loss, valid count) of the mean squared error over valid texels."""
+        import torch
+
+        x = self._features(features)
+        t = target.reshape(-1, OUT).to(torch.float32).contiguous()
+        v = None if valid is None else valid.reshape(-1).to(torch.float32).contiguous()
+        g = torch.empty_like(self.params)
+        lc = torch.empty(2, dtype=torch.float32, device=x.device)
+        _native.check(_native.lib().pdr_mlp_gradient(self.context.handle, x.data_ptr(), t.data_ptr(),
+                                                     None if v is None else v.data_ptr(), x.shape[0],
+                                                     self.params.data_ptr(), g.data_ptr(), lc.data_ptr(), _stream()))
+        return g, float(lc[0]), float(lc[1])
+
+    def train_step(self, features, target, valid=None, lr=1e-3, betas=(0.9, 0.999), eps=1e-8):
+        """One Adam step on a batch; returns the device tensor (loss, valid count) of the batch."""
+        import torch
+
+        x = self._features(features)
+        t = target.reshape(-1, OUT).to(torch.float32).contiguous()
+        v = None if valid is None else valid.reshape(-1).to(torch.float32).contiguous()
+        self.step += 1
+        _native.check(_native.lib().pdr_mlp_adam_step(
+            self.context.handle, x.data_ptr(), t.data_ptr(), None if v is None else v.data_ptr(), x.shape[0],
+            self.params.data_ptr(), self.m.data_ptr(), self.v.data_ptr(), self.step, float(lr), float(betas[0]),
+            float(betas[1]), float(eps), self._loss.data_ptr(), _stream()))
+        return self._loss
+
+
+def problem_features(context, problem, **kw):
+    """Warm-start features [S, S, 40] of a refinement problem's views (same inputs as its bake)."""
+    from .bake import bake_features
+
+    return bake_features(context, problem.positions, problem.normals, problem.uv, problem.indices, problem.images,
+                         problem.world_to_clip, problem.camera_positions, problem.masks,
+                         resolution=(problem.atlas_size, problem.atlas_size), **kw)
+
+
+def warm_start_atlas(context, problem, mlp: WarmStartMLP):
+    """Returns (atlas [S, S, 3], features, seconds): features and MLP inference, timed on the GPU stream."""
+    import torch
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    feats = problem_features(context, problem)
+    atlas = mlp.predict(feats).contiguous()
+    torch.cuda.synchronize()
+    return atlas, feats, time.perf_counter() - t0
+
+
+def plain_refine_kwargs(problem, steps=1000):
+    """The plain method's refinement: graph-captured Adam, lr 1e-2 cosine to 2.5e-4, batch 4,
+    chart TV 1e-3, joint seam 1e-2, then the 30-step seam polish."""
+    return dict(steps=steps, batch_size=4, learning_rate=1e-2, minimum_learning_rate=2.5e-4, seam_polish_steps=30,
+                tv_weight=1e-3, chart_ids=problem.chart_texels, joint_seam_weight=1e-2, use_graph=True)
+
+
+def refine_problem_atlas(context, problem, init, checkpoint_every=0, **overrides):
+    from .refine import refine_atlas
+
+    kw = plain_refine_kwargs(problem)
+    kw.update(overrides)
+    return refine_atlas(context, init, problem.positions, problem.uv, problem.indices, problem.images,
+                        problem.world_to_clip, problem.masks, None, problem.seam_pairs,
+                        checkpoint_every=checkpoint_every, **kw)
+
+
+@dataclass
+class TrainingSet:
+    features: object  # [N, 40] device
+    target: object    # [N, 3] device
+    scenes: int
+    seconds: float
+
+
+def training_pairs(context, seed, log=None):
+    """(features [n, 40], refined colour [n, 3]) over the texels with samples of one
+    seeded training scene; the target is the plain method's refined atlas."""
+    import torch
+
+    from . import workloads
+
+    prob = workloads.refine_problem(context, seed=seed, **TRAIN_SCENE)
+    init, _, _ = workloads.bake_initial_atlas(context, prob)
+    refined = refine_problem_atlas(context, prob, init).color
+    feats = problem_features(context, prob)
+    has = feats[..., 3::5].sum(-1) > 0
+    if log:
+        log(f"scene {seed}: bake {workloads.atlas_psnr(context, prob, init):.2f} dB -> refined "
+            f"{workloads.atlas_psnr(context, prob, refined):.2f} dB, {int(has.sum())} texels")
+    out = feats[has].contiguous(), refined[has].contiguous()
+    del prob, init, refined, feats
+    torch.cuda.empty_cache()
+    return out
+
+
+def build_training_set(context, scenes=200, seed0=TRAIN_SEED0, log=print) -> TrainingSet:
+    import torch
+
+    t0 = time.perf_counter()
+    xs, ys = [], []
+    for k in range(scenes):
+        x, y = training_pairs(context, seed0 + k, log)
+        xs.append(x)
+        ys.append(y)
+    x = torch.cat(xs)
+    y = torch.cat(ys)
+    return TrainingSet(x, y, scenes, time.perf_counter() - t0)
+
+
+def train(context, data: TrainingSet, steps=4000, batch=65536, lr=2e-3, min_lr=2e-5, seed=0, log=print,
+          mlp: WarmStartMLP | None = None) -> WarmStartMLP:
+    """Adam with a cosine learning rate over shuffled mini-batches, all on the GPU."""
+    import torch
+
+    mlp = mlp or WarmStartMLP(context, init_params(seed))
+    n = data.features.shape[0]
+    gen = torch.Generator(device=data.features.device)
+    gen.manual_seed(seed)
+    order, pos = None, n
+    t0 = time.perf_counter()
+    running = torch.zeros(2, dtype=torch.float64, device=data.features.device)
+    for step in range(steps):
+        if pos + batch > n:
+            order = torch.randperm(n, generator=gen, device=data.features.device)
+            pos = 0
+        sel = order[pos:pos + batch]
+        pos += batch
+        rate = min_lr + 0.5 * (lr - min_lr) * (1.0 + math.cos(math.pi * step / max(steps - 1, 1)))
+        lc = mlp.train_step(data.features[sel], data.target[sel], None, rate)
+        running += lc.to(torch.float64)
+        if log and ((step + 1) % 500 == 0 or step + 1 == steps):
+            r = running.cpu().numpy()
+            log(f"step {step + 1}: loss {r[0] / 500:.3e} (rmse {math.sqrt(max(r[0], 0) / 500):.4f}), lr {rate:.2e}, "
+                f"{time.perf_counter() - t0:.1f} s")
+            running.zero_()
+    torch.cuda.synchronize()
+    return mlp
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--train", action="store_true", help="generate the training scenes and train")
+    ap.add_argument("--scenes", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=4000)
+    ap.add_argument("--batch", type=int, default=65536)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--out", default=WEIGHTS)
+    args = ap.parse_args(argv)
+    if not args.train:
+        ap.print_help()
+        return 0
+
+    def log(msg):
+        print(msg, flush=True)
+
+    ctx = _native.Context(0, flags=0)
+    data = build_training_set(ctx, args.scenes, log=log)
+    log(f"training set: {data.features.shape[0]} texels from {data.scenes} scenes in {data.seconds:.1f} s")
+    t0 = time.perf_counter()
+    mlp = train(ctx, data, steps=args.steps, batch=args.batch, lr=args.lr, log=log)
+    train_s = time.perf_counter() - t0
+    mlp.save(args.out)
+    meta = {"scenes": data.scenes, "seed0": TRAIN_SEED0, "scene": {k: list(v) if isinstance(v, tuple) else v
+                                                                 for k, v in TRAIN_SCENE.items()},
+            "texels": int(data.features.shape[0]), "steps": args.steps, "batch": args.batch, "lr": args.lr,
+            "data_seconds": data.seconds, "train_seconds": train_s}
+    with open(os.path.splitext(args.out)[0] + ".json", "w") as f:
+        json.dump(meta, f, indent=1)
+    log(json.dumps(meta))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

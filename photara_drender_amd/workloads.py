@@ -32,6 +32,8 @@ class RefineProblem:
     view_raster: list  # per view [H,W,4]
     atlas_size: int
     ground_truth_atlas: object  # [S,S,3] ground truth sampled at texel centres (reference point)
+    normals: object = None  # [V,3] device f32 area-weighted vertex normals
+    camera_positions: np.ndarray = None  # [N,3] f32 camera centres
 
 
 def _clip(pos, m):
@@ -107,9 +109,31 @@ def refine_problem(ctx, nx=1000, ny=500, charts=16, atlas_size=2048, views=64, s
             sv = (ys[0] + (tv - ty0) / (ty1 - ty0) * (ys[-1] + 1 - ys[0])) / ny
             gt_atlas[m] = scenes.ground_truth(np.stack([su, sv], -1), seed)
     torch.cuda.synchronize()
+    normals = torch.tensor(scenes.vertex_normals(sc.positions, sc.indices), device=dev)
+    cams = np.stack([scenes.camera_center(m) for m in mats])
     return RefineProblem(pos, uv, idx, images, clean, [torch.tensor(m, device=dev) for m in masks_np], mats,
                          torch.tensor(pairs, device=dev), sc.chart_texels, view_uv, view_raster, atlas_size,
-                         torch.tensor(gt_atlas, device=dev))
+                         torch.tensor(gt_atlas, device=dev), normals, cams)
+
+
+def bake_initial_atlas(ctx, problem: RefineProblem, blend_mode="weighted_average"):
+    """The plain method's starting atlas: TextureBaker::bake of the noisy views
+    (shadow-map visibility, the views' masks), RGB of the baked colour.  Returns
+    (atlas [S,S,3], bake result, seconds)."""
+    import time
+
+    import torch
+
+    from .bake import bake_atlas
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = bake_atlas(ctx, problem.positions, problem.normals, problem.uv, problem.indices, problem.images,
+                     problem.world_to_clip, problem.camera_positions, problem.masks,
+                     resolution=(problem.atlas_size, problem.atlas_size), blend_mode=blend_mode)
+    atlas = res.color[..., :3].contiguous()
+    torch.cuda.synchronize()
+    return atlas, res, time.perf_counter() - t0
 
 
 def atlas_psnr(ctx, problem: RefineProblem, atlas, against="clean") -> float:
