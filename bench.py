@@ -215,7 +215,8 @@ def refine_benchmark(steps=1000, checkpoint_every=50, cpu=True):
     per_step = timed.optimization_seconds / (steps + 30)
     out = {
         "workload": "2048^2 RGB atlas, 1,000,000-tri grid in 256 charts (2-texel gutter), 64 views 1024x768, "
-                    "noise 0.02 + ellipse masks, 90,000 seam pairs; initial atlas = weighted-average bake",
+                    "noise 0.02 + ellipse masks, 90,000 seam pairs; initial atlas = weighted-average bake + 8-texel "
+                    "guard-band padding",
         "steps": steps, "seconds_to_refine": bake_s + timed.precompute_seconds + timed.optimization_seconds,
         "bake_s": bake_s, "precompute_s": timed.precompute_seconds, "optimize_s": timed.optimization_seconds,
         "ms_per_step": per_step * 1e3, "psnr_initial_db": initial, "psnr_final_db": final,

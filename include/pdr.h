@@ -265,6 +265,34 @@ pdr_status pdr_bake(pdr_context ctx, const float* positions, const float* normal
                     float* color_out, float* confidence_out, uint32_t* source_view_out, float* valid_out,
                     float* coverage_out, uint32_t* face_ids_out, int* used_ray_query_out);
 
+/* ------------------------------------------------------------- padding */
+
+/* TexturePaddingOptions — include/photara_drender/texture_padding.hpp:33-44. */
+typedef struct pdr_padding_options {
+    uint32_t margin;                /* guard band in texels (default 4) */
+    int fill_unobserved;            /* fill covered texels without samples (default 1) */
+    uint32_t maximum_fill_distance; /* cap of that fill in texels, 0 = unlimited */
+    float valid_threshold;          /* a texel is a colour source when valid >= threshold (default 0.5) */
+} pdr_padding_options;
+
+/* TexturePaddingStats — texture_padding.hpp:46-57. */
+typedef struct pdr_padding_stats {
+    uint64_t valid_texels, gutter_texels, unobserved_texels, remaining_unobserved, filled_texels;
+    double seconds;
+} pdr_padding_stats;
+
+void pdr_padding_options_default(pdr_padding_options* options);
+
+/* pad_texture_atlas — texture_padding.hpp:59-64, src/texture_padding.cpp:226-351
+ * (8SSEDT nearest-source field, then the guard band and the unobserved fill).
+ * color [height,width,channels] (3 or 4) is updated in place; valid_mask
+ * [height,width]; coverage_mask [height,width] or NULL (no coverage: every
+ * texel without a sample is a gutter texel); filled_mask [height,width] in/out
+ * or NULL (no earlier fills, none recorded); stats_out may be NULL. */
+pdr_status pdr_pad_texture_atlas(pdr_context ctx, float* color, uint32_t width, uint32_t height, uint32_t channels,
+                                 const float* valid_mask, const float* coverage_mask, float* filled_mask,
+                                 const pdr_padding_options* options, pdr_padding_stats* stats_out, void* stream);
+
 /* ------------------------------------------------------- warm-start MLP */
 /* The learned warm start has no counterpart in the reference: a per-texel MLP
  * 40 -> 64 -> 64 -> 3 (GELU, tanh form) that predicts the refined atlas colour

@@ -390,3 +390,23 @@ def bake_features(positions, normals, uv, indices, images, world_to_clip, camera
     _check(lib().orc_bake_features(_p(pos, _F32), _p(nrm, _F32), _p(uvs, _F32), pos.shape[0], _p(idx, _U32),
                                    idx.shape[0], views, n, W, H, pcf_radius, _p(feats, _F32)))
     return feats
+
+
+def pad_texture_atlas(color, valid_mask, margin=4, coverage_mask=None, filled_mask=None, fill_unobserved=True,
+                      maximum_fill_distance=0, valid_threshold=0.5):
+    """pad_texture_atlas (src/texture_padding.cpp:226-351, TexturePaddingOptions
+    defaults of include/photara_drender/texture_padding.hpp:33-44).  Returns
+    (color, filled_mask, stats) with stats keys valid_texels, gutter_texels,
+    unobserved_texels, remaining_unobserved, filled_texels."""
+    color = np.array(color, np.float32, copy=True, order="C")
+    H, W, C = color.shape
+    valid = _f32(valid_mask)
+    cov = None if coverage_mask is None else _f32(coverage_mask)
+    filled = np.zeros((H, W), np.float32) if filled_mask is None else np.array(filled_mask, np.float32, copy=True)
+    st = np.zeros(5, np.uint64)
+    _check(lib().orc_pad_texture_atlas(_p(color, _F32), W, H, C, _p(valid, _F32),
+                                       None if cov is None else _p(cov, _F32), _p(filled, _F32), int(margin),
+                                       int(bool(fill_unobserved)), int(maximum_fill_distance),
+                                       ctypes.c_float(valid_threshold), st.ctypes.data_as(ctypes.c_void_p)))
+    keys = ("valid_texels", "gutter_texels", "unobserved_texels", "remaining_unobserved", "filled_texels")
+    return color, filled, {k: int(v) for k, v in zip(keys, st)}

@@ -1090,4 +1090,89 @@ int orc_bake_features(const float* pos3, const float* normal3, const float* uv2,
     return 0;
 }
 
+
+int orc_pad_texture_atlas(float* color, uint32_t W, uint32_t H, uint32_t C, const float* valid, const float* coverage,
+                          float* filled, uint32_t margin, int fill_unobserved, uint32_t max_fill, float threshold,
+                          uint64_t* st) {
+    const int32_t kSide = 16384, kFar = 12000;  // src/texture_padding.cpp:18-21
+    if (W == 0 || H == 0) return fail(1, "cannot pad an empty atlas");
+    if (W > static_cast<uint32_t>(kSide) || H > static_cast<uint32_t>(kSide))
+        return fail(1, "atlas dimensions exceed the supported maximum");
+    if (C != 3 && C != 4) return fail(1, "atlas color must have 3 or 4 channels");
+    if (!(threshold > 0.0f && threshold <= 1.0f)) return fail(1, "valid_threshold must be within (0, 1]");
+    const size_t P = static_cast<size_t>(W) * H;
+    std::vector<float> no_fill;
+    if (!filled) {
+        no_fill.assign(P, 0.0f);
+        filled = no_fill.data();
+    }
+    for (int i = 0; i < 5; ++i) st[i] = 0;
+    // sources: valid texels and earlier covered fills (:259-276)
+    std::vector<uint8_t> src(P, 0);
+    for (size_t p = 0; p < P; ++p) {
+        if (valid[p] >= threshold) {
+            src[p] = 1;
+            ++st[0];
+        } else if (filled[p] > 0.5f && coverage && coverage[p] > 0.5f) {
+            src[p] = 1;
+            ++st[2];
+            ++st[4];
+        } else {
+            filled[p] = 0.0f;
+        }
+    }
+    if (st[0] == 0 && st[4] == 0) return 0;
+    // nearest-source offsets (:37-97)
+    std::vector<int32_t> ox(P), oy(P);
+    for (size_t p = 0; p < P; ++p) ox[p] = oy[p] = src[p] ? 0 : kFar;
+    auto try_from = [&](int32_t x, int32_t y, int32_t nx, int32_t ny) {
+        if (nx < 0 || ny < 0 || nx >= static_cast<int32_t>(W) || ny >= static_cast<int32_t>(H)) return;
+        const size_t c = static_cast<size_t>(y) * W + x, n = static_cast<size_t>(ny) * W + nx;
+        const int64_t cx = ox[n] + (nx - x), cy = oy[n] + (ny - y);
+        if (cx * cx + cy * cy >= static_cast<int64_t>(ox[c]) * ox[c] + static_cast<int64_t>(oy[c]) * oy[c]) return;
+        ox[c] = static_cast<int32_t>(std::min<int64_t>(std::max<int64_t>(cx, -kSide), kSide));
+        oy[c] = static_cast<int32_t>(std::min<int64_t>(std::max<int64_t>(cy, -kSide), kSide));
+    };
+    for (int32_t y = 0; y < static_cast<int32_t>(H); ++y)
+        for (int32_t x = 0; x < static_cast<int32_t>(W); ++x) {
+            try_from(x, y, x - 1, y);
+            try_from(x, y, x, y - 1);
+            try_from(x, y, x - 1, y - 1);
+            try_from(x, y, x + 1, y - 1);
+        }
+    for (int32_t y = static_cast<int32_t>(H) - 1; y >= 0; --y)
+        for (int32_t x = static_cast<int32_t>(W) - 1; x >= 0; --x) {
+            try_from(x, y, x + 1, y);
+            try_from(x, y, x, y + 1);
+            try_from(x, y, x + 1, y + 1);
+            try_from(x, y, x - 1, y + 1);
+        }
+    // fills (:289-348); sources are never written, so reading in place is order-independent
+    const int64_t margin2 = static_cast<int64_t>(margin) * margin;
+    const int64_t limit2 = max_fill == 0 ? INT64_MAX : static_cast<int64_t>(max_fill) * max_fill;
+    const int64_t none2 = 2LL * kFar * kFar;
+    for (uint32_t y = 0; y < H; ++y)
+        for (uint32_t x = 0; x < W; ++x) {
+            const size_t p = static_cast<size_t>(y) * W + x;
+            if (src[p]) continue;
+            const int64_t d2 = static_cast<int64_t>(ox[p]) * ox[p] + static_cast<int64_t>(oy[p]) * oy[p];
+            if (d2 >= none2) continue;
+            const int64_t sx = static_cast<int64_t>(x) + ox[p], sy = static_cast<int64_t>(y) + oy[p];
+            if (sx < 0 || sy < 0 || sx >= W || sy >= H) continue;
+            const size_t s = static_cast<size_t>(sy) * W + static_cast<size_t>(sx);
+            const bool covered = coverage && coverage[p] > 0.5f;
+            const bool fill = covered ? (fill_unobserved && d2 <= limit2) : (margin > 0 && d2 <= margin2);
+            if (!fill) {
+                if (covered) ++st[3];
+                continue;
+            }
+            for (uint32_t c = 0; c < C; ++c) color[p * C + c] = color[s * C + c];
+            if (C == 4) color[p * C + 3] = 1.0f;
+            filled[p] = 1.0f;
+            ++st[4];
+            ++st[covered ? 2 : 1];
+        }
+    return 0;
+}
+
 }  // extern "C"

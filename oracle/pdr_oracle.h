@@ -149,6 +149,19 @@ int orc_bake_features(const float* pos3, const float* normal3, const float* uv2,
                       uint32_t T, const orc_bake_view* views, uint32_t nviews, uint32_t width, uint32_t height,
                       uint32_t pcf_radius, float* features40);
 
+/* pad_texture_atlas (src/texture_padding.cpp:226-351): Danielsson 8SSEDT
+ * nearest-source field (:37-97; forward sweep relaxing left, up, up-left,
+ * up-right; backward sweep right, down, down-right, down-left; a candidate
+ * replaces the current offset only when strictly closer), then gutter texels
+ * within `margin` and, with fill_unobserved, covered texels within
+ * maximum_fill_distance (0 = unlimited) copy their source's colour.
+ * color [h,w,c] and filled_mask [h,w] are updated in place (filled_mask may be
+ * NULL: no earlier fills); coverage NULL = no coverage information.
+ * stats5 = (valid, gutter, unobserved, remaining_unobserved, filled). */
+int orc_pad_texture_atlas(float* color, uint32_t width, uint32_t height, uint32_t channels, const float* valid_mask,
+                          const float* coverage, float* filled_mask, uint32_t margin, int fill_unobserved,
+                          uint32_t maximum_fill_distance, float valid_threshold, uint64_t* stats5);
+
 /* src/texture_padding.cpp:187-222 */
 float orc_suggest_softmax_scale(const float* pos3, uint32_t V, const float* cameras3, uint32_t ncameras);
 

@@ -11,6 +11,7 @@ except ImportError:
     _torch = None
 
 from ._photara_drender import DeviceInfo, Rasterizer, TextureBaker, enumerate_devices
+from .bake import pad_texture_atlas
 
 try:
     from .torch import interpolate, rasterize, texture
@@ -46,6 +47,7 @@ __all__ = [
     "interpolate",
     "masked_charbonnier_loss",
     "optimize_texture_atlas",
+    "pad_texture_atlas",
     "rasterize",
     "render_textured_mesh",
     "seam_consistency_loss",

@@ -66,6 +66,16 @@ class BakeOptions(ctypes.Structure):
                 ("softmax_scale", _F)]
 
 
+class PaddingOptions(ctypes.Structure):
+    _fields_ = [("margin", _U32), ("fill_unobserved", _I), ("maximum_fill_distance", _U32), ("valid_threshold", _F)]
+
+
+class PaddingStats(ctypes.Structure):
+    _fields_ = [("valid_texels", ctypes.c_uint64), ("gutter_texels", ctypes.c_uint64),
+                ("unobserved_texels", ctypes.c_uint64), ("remaining_unobserved", ctypes.c_uint64),
+                ("filled_texels", ctypes.c_uint64), ("seconds", ctypes.c_double)]
+
+
 _SIGNATURES = {
     "pdr_last_error": (ctypes.c_char_p, []),
     "pdr_context_create": (_I, [_U32, _I, ctypes.POINTER(_P)]),
@@ -97,6 +107,9 @@ _SIGNATURES = {
                       _P, _P, _P, _P, _P, ctypes.POINTER(_I)]),
     "pdr_refine": (_I, [_P, _P, _P, _U32, _P, _P, _U32, ctypes.POINTER(RefineView), _U32, _P, _U32,
                         ctypes.POINTER(RefineOptions), _P, _P, _P, ctypes.POINTER(ctypes.c_double)]),
+    "pdr_padding_options_default": (None, [ctypes.POINTER(PaddingOptions)]),
+    "pdr_pad_texture_atlas": (_I, [_P, _P, _U32, _U32, _U32, _P, _P, _P, ctypes.POINTER(PaddingOptions),
+                                   ctypes.POINTER(PaddingStats), _P]),
     "pdr_bake_features": (_I, [_P, _P, _P, _P, _U32, _P, _U32, ctypes.POINTER(BakeView), _U32,
                                ctypes.POINTER(BakeOptions), _P]),
     "pdr_mlp_infer": (_I, [_P, _P, ctypes.c_uint64, _P, _P, _P]),

@@ -121,3 +121,55 @@ def bake_features(context, positions, normals, uv, indices, images: Sequence, wo
                                                   int(pos.shape[0]), idx.data_ptr(), int(idx.shape[0]), views,
                                                   len(images), ctypes.byref(o), feats.data_ptr()))
     return feats
+
+
+def pad_texture_atlas(color, valid_mask, padding: int, *, coverage_mask=None, fill_unobserved: bool = False,
+                      maximum_fill_distance: int = 0) -> np.ndarray:
+    """Extend valid atlas texels into empty neighbours (python/photara_drender/baking.py:33-80).
+
+    Returns a padded copy of ``color``; ``padding`` is the guard width in texels.
+    With ``fill_unobserved`` the covered texels (``coverage_mask``) that never
+    received a sample take the colour of the nearest valid texel."""
+    from ._photara_drender import pad_texture_atlas as _pad
+
+    color = np.asarray(color, dtype=np.float32)
+    valid = np.asarray(valid_mask, dtype=bool)
+    if color.ndim != 3 or color.shape[2] not in (3, 4):
+        raise ValueError("color must have shape [height, width, 3 or 4]")
+    if valid.shape != color.shape[:2]:
+        raise ValueError("valid_mask must match the color dimensions")
+    if padding < 0:
+        raise ValueError("padding must be non-negative")
+    if (padding == 0 and not fill_unobserved) or not valid.any():
+        return np.ascontiguousarray(color.copy())
+    coverage = None
+    if coverage_mask is not None:
+        coverage = np.ascontiguousarray(np.asarray(coverage_mask, dtype=np.float32))
+        if coverage.shape != valid.shape:
+            raise ValueError("coverage_mask must match the color dimensions")
+    if maximum_fill_distance < 0:
+        raise ValueError("maximum_fill_distance must be non-negative")
+    padded, _filled, _gutter, _unobserved, _remaining = _pad(np.ascontiguousarray(color), valid.astype(np.float32),
+                                                             int(padding), coverage, bool(fill_unobserved),
+                                                             int(maximum_fill_distance))
+    return padded
+
+
+def pad_atlas(context, color, valid_mask, margin: int = 8, *, coverage_mask=None, filled_mask=None,
+              fill_unobserved: bool = False, maximum_fill_distance: int = 0, valid_threshold: float = 0.5) -> dict:
+    """pdr_pad_texture_atlas on device tensors: pads ``color`` [H, W, 3|4] in place
+    (and records fills in ``filled_mask`` when given); returns the padding statistics."""
+    handle = context.handle if hasattr(context, "handle") else context.context_handle
+    o = _native.PaddingOptions(int(margin), int(bool(fill_unobserved)), int(maximum_fill_distance),
+                               float(valid_threshold))
+    s = _native.PaddingStats()
+    if not (color.is_contiguous() and valid_mask.is_contiguous()):
+        raise ValueError("color and valid_mask must be contiguous")
+    import torch
+
+    _native.check(_native.lib().pdr_pad_texture_atlas(
+        handle, color.data_ptr(), int(color.shape[1]), int(color.shape[0]), int(color.shape[2]),
+        valid_mask.data_ptr(), None if coverage_mask is None else coverage_mask.contiguous().data_ptr(),
+        None if filled_mask is None else filled_mask.data_ptr(), ctypes.byref(o), ctypes.byref(s),
+        torch.cuda.current_stream().cuda_stream))
+    return {k: getattr(s, k) for k, _ in _native.PaddingStats._fields_}

@@ -470,4 +470,41 @@ PYBIND11_MODULE(_photara_drender, m) {
             return suggest_softmax_scale(as_span(positions), as_span(camera_positions));
         },
         py::arg("positions"), py::arg("camera_positions"));
+    // python/bindings.cpp:786-834 of the reference: (color, filled_mask, gutter, unobserved, remaining)
+    m.def(
+        "pad_texture_atlas",
+        [](const FloatInput& color, const FloatInput& valid_mask, std::uint32_t margin, const py::object& coverage_mask,
+           bool fill_unobserved, std::uint32_t maximum_fill_distance) {
+            const auto ci = color.request();
+            const auto vi = valid_mask.request();
+            if (ci.ndim != 3 || ci.shape[0] <= 0 || ci.shape[1] <= 0 || (ci.shape[2] != 3 && ci.shape[2] != 4))
+                throw std::invalid_argument("color must have shape [height, width, 3 or 4]");
+            if (vi.ndim != 2 || vi.shape[0] != ci.shape[0] || vi.shape[1] != ci.shape[1])
+                throw std::invalid_argument("valid_mask must match the color dimensions");
+            TextureBakeOutput output;
+            output.width = static_cast<std::uint32_t>(ci.shape[1]);
+            output.height = static_cast<std::uint32_t>(ci.shape[0]);
+            output.color.assign(color.data(), color.data() + color.size());
+            output.valid_mask.assign(valid_mask.data(), valid_mask.data() + valid_mask.size());
+            if (!coverage_mask.is_none()) {
+                const auto coverage = py::cast<FloatInput>(coverage_mask);
+                const auto cov = coverage.request();
+                if (cov.ndim != 2 || cov.shape[0] != ci.shape[0] || cov.shape[1] != ci.shape[1])
+                    throw std::invalid_argument("coverage_mask must match the color dimensions");
+                output.coverage_mask.assign(coverage.data(), coverage.data() + coverage.size());
+            }
+            TexturePaddingOptions options;
+            options.margin = margin;
+            options.fill_unobserved = fill_unobserved;
+            options.maximum_fill_distance = maximum_fill_distance;
+            const auto stats = pad_texture_atlas(output, options);
+            py::array_t<float> out_color({ci.shape[0], ci.shape[1], ci.shape[2]});
+            std::memcpy(out_color.mutable_data(), output.color.data(), output.color.size() * sizeof(float));
+            py::array_t<float> out_filled({ci.shape[0], ci.shape[1]});
+            std::memcpy(out_filled.mutable_data(), output.filled_mask.data(), output.filled_mask.size() * sizeof(float));
+            return py::make_tuple(out_color, out_filled, stats.gutter_texels, stats.unobserved_texels,
+                                  stats.remaining_unobserved);
+        },
+        py::arg("color"), py::arg("valid_mask"), py::arg("margin") = 4, py::arg("coverage_mask") = py::none(),
+        py::arg("fill_unobserved") = true, py::arg("maximum_fill_distance") = 0);
 }

@@ -1,0 +1,176 @@
+// pad_texture_atlas (src/texture_padding.cpp:226-351) on gfx950.
+//
+// The nearest-source field is Danielsson's 8SSEDT (src/texture_padding.cpp:
+// 37-97): a forward raster sweep relaxing each texel from its left, upper,
+// upper-left and upper-right neighbours, then the mirrored backward sweep.
+// A texel's forward value depends on its left neighbour (same sweep) and on
+// three texels of the row above, so texel (x, y) can be settled at step
+// t = x + 2 y: the sweep runs as a wavefront of W + 2 (H - 1) steps in one
+// workgroup, one row per lane and step (rows y, y + R, y + 2 R, ... share a
+// lane, R = ceil(W / 2), and are never active together).  Each row keeps its
+// last four settled texels in an LDS ring, which is everything the next row
+// reads.  The backward sweep is the forward sweep on the mirrored grid with
+// negated offsets (same relaxation order, same squared lengths), so results
+// equal the sequential sweeps exactly.  Preparation and the fill are one
+// texel per lane over the whole grid.
+#include "pdr_internal.hpp"
+
+namespace pdr {
+namespace {
+
+constexpr int32_t kFar = 12000;   // offset of "no source yet"
+constexpr int32_t kSide = 16384;  // offsets are clamped to +-kSide (int16 storage)
+constexpr uint32_t kSweepThreads = 1024;
+constexpr uint32_t kBlock = 256;
+
+__device__ __forceinline__ uint32_t pack(int32_t x, int32_t y) {
+    return (static_cast<uint32_t>(x) & 0xFFFFu) | (static_cast<uint32_t>(y) << 16);
+}
+__device__ __forceinline__ int32_t off_x(uint32_t v) { return static_cast<int16_t>(v & 0xFFFFu); }
+__device__ __forceinline__ int32_t off_y(uint32_t v) { return static_cast<int16_t>(v >> 16); }
+
+__device__ __forceinline__ int32_t clamp_side(int32_t v) { return v < -kSide ? -kSide : (v > kSide ? kSide : v); }
+
+// cur <- neighbour's offset + (dx, dy) when strictly closer (squares fit in uint32: |offset| <= kSide + 1).
+__device__ __forceinline__ void relax(int32_t& cx, int32_t& cy, uint32_t nb, int32_t dx, int32_t dy) {
+    const int32_t x = off_x(nb) + dx, y = off_y(nb) + dy;
+    const uint32_t cand = static_cast<uint32_t>(x * x) + static_cast<uint32_t>(y * y);
+    const uint32_t cur = static_cast<uint32_t>(cx * cx) + static_cast<uint32_t>(cy * cy);
+    if (cand < cur) {
+        cx = clamp_side(x);
+        cy = clamp_side(y);
+    }
+}
+
+__device__ __forceinline__ void count(unsigned long long* counter, bool pred) {
+    const unsigned long long mask = __ballot(pred);
+    if (mask && (threadIdx.x & 63u) == static_cast<uint32_t>(__ffsll(static_cast<long long>(mask)) - 1))
+        atomicAdd(counter, static_cast<unsigned long long>(__popcll(mask)));
+}
+
+// Sources (:259-276): valid texels and earlier covered fills; other texels lose their fill flag.
+// counters: valid, gutter, unobserved, remaining, filled.
+__global__ void k_pad_prepare(uint32_t P, const float* __restrict__ valid, const float* __restrict__ coverage,
+                              float* __restrict__ filled, float threshold, uint8_t* __restrict__ source,
+                              unsigned long long* __restrict__ counters) {
+    const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+    bool is_valid = false, refill = false;
+    if (p < P) {
+        is_valid = valid[p] >= threshold;
+        refill = !is_valid && filled && filled[p] > 0.5f && coverage && coverage[p] > 0.5f;
+        if (!is_valid && !refill && filled) filled[p] = 0.0f;
+        source[p] = is_valid || refill ? 1 : 0;
+    }
+    count(counters + 0, is_valid);
+    count(counters + 2, refill);
+    count(counters + 4, refill);
+}
+
+// One 8SSEDT sweep as a wavefront; mirror = the backward sweep.  field holds
+// packed int16 offsets: read as the initial state of the backward sweep,
+// written by both.  ring: R x 4 settled offsets (dynamic LDS).
+__global__ __launch_bounds__(kSweepThreads) void k_pad_sweep(uint32_t W, uint32_t H, const uint8_t* __restrict__ source,
+                                                              uint32_t* __restrict__ field, int mirror) {
+    extern __shared__ uint32_t ring[];
+    const uint32_t R = (W + 1u) / 2u;
+    const uint32_t steps = W + 2u * (H - 1u);
+    for (uint32_t t = 0; t < steps; ++t) {
+        for (uint32_t c = threadIdx.x; c < R; c += kSweepThreads) {
+            if (t < 2u * c) continue;
+            const uint32_t k = (t - 2u * c) / (2u * R);
+            const uint32_t x = t - 2u * c - 2u * R * k, y = c + k * R;
+            if (x >= W || y >= H) continue;
+            const uint32_t rx = mirror ? W - 1u - x : x, ry = mirror ? H - 1u - y : y;
+            const size_t idx = static_cast<size_t>(ry) * W + rx;
+            int32_t cx, cy;
+            if (mirror) {
+                const uint32_t v = field[idx];
+                cx = -off_x(v);
+                cy = -off_y(v);
+            } else {
+                cx = cy = source[idx] ? 0 : kFar;
+            }
+            if (x > 0) relax(cx, cy, ring[c * 4u + ((t - 1u) & 3u)], -1, 0);
+            if (y > 0) {
+                const uint32_t pc = c == 0 ? R - 1u : c - 1u;
+                relax(cx, cy, ring[pc * 4u + ((t - 2u) & 3u)], 0, -1);
+                if (x > 0) relax(cx, cy, ring[pc * 4u + ((t - 3u) & 3u)], -1, -1);
+                if (x + 1u < W) relax(cx, cy, ring[pc * 4u + ((t - 1u) & 3u)], 1, -1);
+            }
+            ring[c * 4u + (t & 3u)] = pack(cx, cy);
+            field[idx] = mirror ? pack(-cx, -cy) : pack(cx, cy);
+        }
+        __syncthreads();
+    }
+}
+
+// Fills (:289-348).  Sources are never written; colours are read from `src_color`
+// (the atlas itself, or a copy of it when a far sentinel offset could point at
+// a non-source texel inside the atlas).
+__global__ void k_pad_fill(uint32_t W, uint32_t H, uint32_t C, const uint32_t* __restrict__ field,
+                           const uint8_t* __restrict__ source, const float* __restrict__ coverage,
+                           unsigned long long margin2, unsigned long long limit2, int fill_unobserved,
+                           const float* src_color, float* color, float* __restrict__ filled,
+                           unsigned long long* __restrict__ counters) {
+    const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+    bool gutter = false, unobserved = false, remaining = false;
+    if (p < W * H && !source[p]) {
+        const uint32_t v = field[p];
+        const int64_t ox = off_x(v), oy = off_y(v);
+        const unsigned long long d2 = static_cast<unsigned long long>(ox * ox + oy * oy);
+        const int64_t sx = static_cast<int64_t>(p % W) + ox, sy = static_cast<int64_t>(p / W) + oy;
+        if (d2 < 2ull * kFar * kFar && sx >= 0 && sy >= 0 && sx < W && sy < H) {
+            const bool covered = coverage && coverage[p] > 0.5f;
+            const bool fill = covered ? (fill_unobserved && d2 <= limit2) : (margin2 > 0 && d2 <= margin2);
+            if (fill) {
+                const size_t s = static_cast<size_t>(sy) * W + static_cast<size_t>(sx);
+                for (uint32_t c = 0; c < C; ++c) color[static_cast<size_t>(p) * C + c] = src_color[s * C + c];
+                if (C == 4) color[static_cast<size_t>(p) * C + 3] = 1.0f;
+                if (filled) filled[p] = 1.0f;
+                gutter = !covered;
+                unobserved = covered;
+            } else {
+                remaining = covered;
+            }
+        }
+    }
+    count(counters + 1, gutter);
+    count(counters + 2, unobserved);
+    count(counters + 3, remaining);
+    count(counters + 4, gutter || unobserved);
+}
+
+uint32_t blocks_of(uint64_t n) { return static_cast<uint32_t>((n + kBlock - 1) / kBlock); }
+
+}  // namespace
+
+hipError_t pad_prepare(uint32_t P, const float* valid, const float* coverage, float* filled, float threshold,
+                       uint8_t* source, unsigned long long* counters, hipStream_t st) {
+    k_pad_prepare<<<blocks_of(P), kBlock, 0, st>>>(P, valid, coverage, filled, threshold, source, counters);
+    return hipGetLastError();
+}
+
+hipError_t pad_field(uint32_t W, uint32_t H, const uint8_t* source, uint32_t* field, hipStream_t st) {
+    const size_t lds = static_cast<size_t>((W + 1u) / 2u) * 4u * sizeof(uint32_t);
+    if (lds > 64u * 1024u) {
+        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_pad_sweep),
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+        if (e != hipSuccess) return e;
+    }
+    k_pad_sweep<<<1, kSweepThreads, lds, st>>>(W, H, source, field, 0);
+    k_pad_sweep<<<1, kSweepThreads, lds, st>>>(W, H, source, field, 1);
+    return hipGetLastError();
+}
+
+hipError_t pad_fill(uint32_t W, uint32_t H, uint32_t C, const uint32_t* field, const uint8_t* source,
+                    const float* coverage, uint32_t margin, int fill_unobserved, uint32_t max_fill,
+                    const float* src_color, float* color, float* filled, unsigned long long* counters,
+                    hipStream_t st) {
+    const unsigned long long margin2 = static_cast<unsigned long long>(margin) * margin;
+    const unsigned long long limit2 = max_fill == 0 ? ~0ull : static_cast<unsigned long long>(max_fill) * max_fill;
+    k_pad_fill<<<blocks_of(static_cast<uint64_t>(W) * H), kBlock, 0, st>>>(
+        W, H, C, field, source, coverage, margin2, limit2, fill_unobserved, src_color, color, filled, counters);
+    return hipGetLastError();
+}
+
+}  // namespace pdr

@@ -110,6 +110,7 @@ hipError_t check_indices(const uint32_t* idx, size_t n, uint32_t V, uint32_t* fl
 
 // ---- common kernels (raster.hip)
 hipError_t fill_u32(uint32_t* p, uint32_t value, size_t n, hipStream_t st);
+hipError_t fill_u64(unsigned long long* p, unsigned long long value, size_t n, hipStream_t st);
 hipError_t fill_f32(float* p, float value, size_t n, hipStream_t st);
 
 // ---- refine.hip
@@ -184,5 +185,14 @@ hipError_t mlp_grad(const float* x, const float* target, const float* valid, uin
                     uint32_t waves, hipStream_t st);
 hipError_t mlp_reduce(const float* part, uint32_t waves, float* grad, float* loss_count, float* params, float* m1,
                       float* m2, float lr, float beta1, float beta2, float eps, float b1c, float b2c, hipStream_t st);
+
+// ---- padding.hip: pad_texture_atlas
+// counters: valid, gutter, unobserved, remaining, filled (zeroed by the caller).
+hipError_t pad_prepare(uint32_t P, const float* valid, const float* coverage, float* filled, float threshold,
+                       uint8_t* source, unsigned long long* counters, hipStream_t st);
+hipError_t pad_field(uint32_t W, uint32_t H, const uint8_t* source, uint32_t* field, hipStream_t st);
+hipError_t pad_fill(uint32_t W, uint32_t H, uint32_t C, const uint32_t* field, const uint8_t* source,
+                    const float* coverage, uint32_t margin, int fill_unobserved, uint32_t max_fill,
+                    const float* src_color, float* color, float* filled, unsigned long long* counters, hipStream_t st);
 
 }  // namespace pdr

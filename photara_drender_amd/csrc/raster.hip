@@ -958,6 +958,12 @@ hipError_t fill_f32(float* p, float value, size_t n, hipStream_t st) {
     return hipGetLastError();
 }
 
+hipError_t fill_u64(unsigned long long* p, unsigned long long value, size_t n, hipStream_t st) {
+    if (!n) return hipSuccess;
+    k_fill_u64<<<grid_for(n), kBlock, 0, st>>>(p, value, n);
+    return hipGetLastError();
+}
+
 hipError_t fill_keys(unsigned long long* p, size_t n, hipStream_t st) {
     if (!n) return hipSuccess;
     k_fill_u64<<<grid_for(n), kBlock, 0, st>>>(p, ~0ull, n);

@@ -17,7 +17,6 @@ shipped weights (``warmstart_mlp.npy``).
 from __future__ import annotations
 
 import argparse
-import ctypes
 import json
 import math
 import os
@@ -137,14 +136,19 @@ def problem_features(context, problem, **kw):
                          resolution=(problem.atlas_size, problem.atlas_size), **kw)
 
 
-def warm_start_atlas(context, problem, mlp: WarmStartMLP):
-    """Returns (atlas [S, S, 3], features, seconds): features and MLP inference, timed on the GPU stream."""
+def warm_start_atlas(context, problem, mlp: WarmStartMLP, padding=8):
+    """Returns (atlas [S, S, 3], features, seconds): features, MLP inference and
+    the same guard-band padding as the plain method's bake, timed to completion."""
     import torch
+
+    from .bake import pad_atlas
 
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     feats = problem_features(context, problem)
     atlas = mlp.predict(feats).contiguous()
+    if padding:
+        pad_atlas(context, atlas, (feats[..., 3::5].sum(-1) > 0).to(torch.float32), padding)
     torch.cuda.synchronize()
     return atlas, feats, time.perf_counter() - t0
 
@@ -221,6 +225,10 @@ def train(context, data: TrainingSet, steps=4000, batch=65536, lr=2e-3, min_lr=2
     order, pos = None, n
     t0 = time.perf_counter()
     running = torch.zeros(2, dtype=torch.float64, device=data.features.device)
+    if log:
+        _, base_loss, _ = mlp.gradient(data.features[:1 << 20], data.target[:1 << 20])
+        log(f"initial loss {base_loss:.3e} (rmse {math.sqrt(base_loss):.4f})")
+    window = 0
     for step in range(steps):
         if pos + batch > n:
             order = torch.randperm(n, generator=gen, device=data.features.device)
@@ -230,13 +238,67 @@ def train(context, data: TrainingSet, steps=4000, batch=65536, lr=2e-3, min_lr=2
         rate = min_lr + 0.5 * (lr - min_lr) * (1.0 + math.cos(math.pi * step / max(steps - 1, 1)))
         lc = mlp.train_step(data.features[sel], data.target[sel], None, rate)
         running += lc.to(torch.float64)
+        window += 1
         if log and ((step + 1) % 500 == 0 or step + 1 == steps):
-            r = running.cpu().numpy()
-            log(f"step {step + 1}: loss {r[0] / 500:.3e} (rmse {math.sqrt(max(r[0], 0) / 500):.4f}), lr {rate:.2e}, "
+            r = float(running[0]) / window
+            log(f"step {step + 1}: loss {r:.3e} (rmse {math.sqrt(max(r, 0)):.4f}), lr {rate:.2e}, "
                 f"{time.perf_counter() - t0:.1f} s")
             running.zero_()
+            window = 0
     torch.cuda.synchronize()
     return mlp
+
+
+def _steps_to(psnr, target, every):
+    return next((every * (i + 1) for i, p in enumerate(psnr) if p >= target), None)
+
+
+def evaluate(context, problem, mlp: WarmStartMLP, warm=None, checkpoint_every=25, log=None):
+    """Plain bake + full refinement against the MLP warm start on one problem.
+
+    Runs are timed without checkpoints (pdr_refine is synchronous), then
+    repeated with atlas checkpoints every ``checkpoint_every`` steps for the PSNR
+    curve (masked, covered pixels of every view against the noise-free renders).
+    The target is the plain method's final PSNR - 0.1 dB; time to target =
+    initial atlas + precompute + steps to target x time per step.  ``warm``
+    overrides the warm start's refinement options (default: the plain schedule
+    started at a learning rate of 2e-3); the ablation runs the plain method's
+    initial atlas with those same options, which isolates the network's part."""
+    from . import workloads
+
+    warm = dict(learning_rate=2e-3) if warm is None else dict(warm)
+
+    def run(init, init_s, target=None, **kw):
+        timed = refine_problem_atlas(context, problem, init, **kw)
+        ck = refine_problem_atlas(context, problem, init, checkpoint_every=checkpoint_every, **kw)
+        steps = kw.get("steps", 1000)
+        psnr = [workloads.atlas_psnr(context, problem, ck.checkpoints[i]) for i in range(ck.checkpoints.shape[0])]
+        per_step = timed.optimization_seconds / (steps + 30)
+        out = dict(init_s=init_s, init_psnr_db=workloads.atlas_psnr(context, problem, init),
+                   final_psnr_db=workloads.atlas_psnr(context, problem, timed.color),
+                   precompute_s=timed.precompute_seconds, optimize_s=timed.optimization_seconds,
+                   ms_per_step=per_step * 1e3, psnr_checkpoints_db=[round(p, 3) for p in psnr])
+        if target is None:
+            target = psnr[-1] - 0.1
+        n = _steps_to(psnr, target, checkpoint_every)
+        out["steps_to_target"] = n
+        out["seconds_to_target"] = None if n is None else init_s + timed.precompute_seconds + n * per_step
+        return out, target
+
+    workloads.bake_initial_atlas(context, problem)  # warm-up
+    init, _, bake_s = workloads.bake_initial_atlas(context, problem)
+    plain, target = run(init, bake_s)
+    warm_start_atlas(context, problem, mlp)  # warm-up
+    winit, _, ws_s = warm_start_atlas(context, problem, mlp)
+    ws, _ = run(winit, ws_s, target, **warm)
+    ablation, _ = run(init, bake_s, target, **warm)
+    ws["options"] = ablation["options"] = warm
+    speed = lambda r: None if r["seconds_to_target"] is None else plain["seconds_to_target"] / r["seconds_to_target"]
+    out = {"target_psnr_db": target, "plain": plain, "warm_start": ws, "ablation_plain_init_warm_options": ablation,
+           "speedup_time_to_target": speed(ws), "ablation_speedup": speed(ablation)}
+    if log:
+        log(json.dumps(out))
+    return out
 
 
 def main(argv=None) -> int:
@@ -247,8 +309,12 @@ def main(argv=None) -> int:
     ap.add_argument("--batch", type=int, default=65536)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--out", default=WEIGHTS)
+    ap.add_argument("--evaluate", action="store_true",
+                    help="compare plain bake + refinement with the warm start on the 2048^2 / 64-view scene")
+    ap.add_argument("--warm-lr", type=float, nargs="*", default=[2e-3],
+                    help="starting learning rates of the warm-started refinement to evaluate")
     args = ap.parse_args(argv)
-    if not args.train:
+    if not (args.train or args.evaluate):
         ap.print_help()
         return 0
 
@@ -256,6 +322,14 @@ def main(argv=None) -> int:
         print(msg, flush=True)
 
     ctx = _native.Context(0, flags=0)
+    if not args.train:
+        from . import workloads
+
+        mlp = WarmStartMLP.load(ctx, args.out)
+        prob = workloads.refine_problem(ctx)
+        for lr in args.warm_lr:
+            evaluate(ctx, prob, mlp, warm=dict(learning_rate=lr), log=log)
+        return 0
     data = build_training_set(ctx, args.scenes, log=log)
     log(f"training set: {data.features.shape[0]} texels from {data.scenes} scenes in {data.seconds:.1f} s")
     t0 = time.perf_counter()

@@ -116,15 +116,17 @@ def refine_problem(ctx, nx=1000, ny=500, charts=16, atlas_size=2048, views=64, s
                          torch.tensor(gt_atlas, device=dev), normals, cams)
 
 
-def bake_initial_atlas(ctx, problem: RefineProblem, blend_mode="weighted_average"):
+def bake_initial_atlas(ctx, problem: RefineProblem, blend_mode="weighted_average", padding=8):
     """The plain method's starting atlas: TextureBaker::bake of the noisy views
-    (shadow-map visibility, the views' masks), RGB of the baked colour.  Returns
-    (atlas [S,S,3], bake result, seconds)."""
+    (shadow-map visibility, the views' masks), RGB of the baked colour, then
+    pad_texture_atlas with a ``padding``-texel guard band as the reference's
+    refinement example does before refining (examples/refine_colmap_texture.py:
+    240-242).  Returns (atlas [S,S,3], bake result, seconds)."""
     import time
 
     import torch
 
-    from .bake import bake_atlas
+    from .bake import bake_atlas, pad_atlas
 
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -132,6 +134,8 @@ def bake_initial_atlas(ctx, problem: RefineProblem, blend_mode="weighted_average
                      problem.world_to_clip, problem.camera_positions, problem.masks,
                      resolution=(problem.atlas_size, problem.atlas_size), blend_mode=blend_mode)
     atlas = res.color[..., :3].contiguous()
+    if padding:
+        pad_atlas(ctx, atlas, res.valid, padding)
     torch.cuda.synchronize()
     return atlas, res, time.perf_counter() - t0
 
