@@ -31,6 +31,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+BF16_PEAK_TFLOPS = 2500.0  # dense bf16 matrix-core peak of MI355X
 HBM_PEAK_GBS = 8000.0
 
 
@@ -180,14 +181,16 @@ def chain_benchmark(steps=50, warmup=5, cpu=True):
     return out
 
 
-def refine_benchmark(steps=1000, checkpoint_every=50, cpu=True):
+def refine_benchmark(steps=1000, checkpoint_every=25, cpu=True):
     """Configuration 3: 2048^2 RGB atlas, 1M-triangle 256-chart grid, 64 views 1024x768
     with noise and masks, 1000 graph-captured Adam steps (lr 1e-2 cosine to 2.5e-4,
-    chart TV 1e-3, joint seam 1e-2, then the reference's 30-step seam polish)."""
+    chart TV 1e-3, joint seam 1e-2, then the reference's 30-step seam polish) from
+    the padded weighted-average bake; next to it the learned warm start (features,
+    MLP, padding, refinement started at lr 3e-3) and the same schedule from the
+    plain initial atlas (ablation)."""
     import torch
 
-    from photara_drender_amd import _native, workloads
-    from photara_drender_amd.refine import refine_atlas
+    from photara_drender_amd import _native, warmstart, workloads
 
     ctx = _native.Context(0, flags=0)
     t0 = time.perf_counter()
@@ -195,39 +198,76 @@ def refine_benchmark(steps=1000, checkpoint_every=50, cpu=True):
     build_s = time.perf_counter() - t0
     s = prob.atlas_size
     grey = torch.full((s, s, 3), 0.5, device=prob.positions.device)
-    init = grey
-    kw = dict(steps=steps, batch_size=4, learning_rate=1e-2, minimum_learning_rate=2.5e-4, seam_polish_steps=30,
-              tv_weight=1e-3, chart_ids=prob.chart_texels, joint_seam_weight=1e-2, use_graph=True)
-    args = (ctx, init, prob.positions, prob.uv, prob.indices, prob.images, prob.world_to_clip, prob.masks, None,
-            prob.seam_pairs)
-    refine_atlas(*args, **dict(kw, steps=20, seam_polish_steps=2))  # warm-up (module load, scratch)
-    workloads.bake_initial_atlas(ctx, prob)  # warm-up
-    init, _, bake_s = workloads.bake_initial_atlas(ctx, prob)
-    args = (ctx, init) + args[2:]
-    timed = refine_atlas(*args, **kw)
-    ck = refine_atlas(*args, **dict(kw, checkpoint_every=checkpoint_every))
-    psnr = [workloads.atlas_psnr(ctx, prob, ck.checkpoints[i]) for i in range(ck.checkpoints.shape[0])]
-    final = workloads.atlas_psnr(ctx, prob, timed.color)
+    warmstart.refine_problem_atlas(ctx, prob, grey, steps=20, seam_polish_steps=2)  # warm-up (module load, scratch)
+    mlp = warmstart.WarmStartMLP.load(ctx)
+    ev = warmstart.evaluate(ctx, prob, mlp, warm=dict(learning_rate=3e-3), checkpoint_every=checkpoint_every)
+    plain, warm, abl = ev["plain"], ev["warm_start"], ev["ablation_plain_init_warm_options"]
+    timed = warmstart.refine_problem_atlas(ctx, prob, workloads.bake_initial_atlas(ctx, prob)[0])
     final_noisy = workloads.atlas_psnr(ctx, prob, timed.color, against="noisy")
-    initial = workloads.atlas_psnr(ctx, prob, init)
-    target = psnr[-1] - 0.1
-    steps_needed = next((checkpoint_every * (i + 1) for i, p in enumerate(psnr) if p >= target), steps)
-    per_step = timed.optimization_seconds / (steps + 30)
     out = {
         "workload": "2048^2 RGB atlas, 1,000,000-tri grid in 256 charts (2-texel gutter), 64 views 1024x768, "
                     "noise 0.02 + ellipse masks, 90,000 seam pairs; initial atlas = weighted-average bake + 8-texel "
                     "guard-band padding",
-        "steps": steps, "seconds_to_refine": bake_s + timed.precompute_seconds + timed.optimization_seconds,
-        "bake_s": bake_s, "precompute_s": timed.precompute_seconds, "optimize_s": timed.optimization_seconds,
-        "ms_per_step": per_step * 1e3, "psnr_initial_db": initial, "psnr_final_db": final,
-        "psnr_final_vs_noisy_db": final_noisy, "psnr_checkpoints_db": [round(p, 3) for p in psnr],
-        "steps_to_final_minus_0.1db": steps_needed,
-        "seconds_to_final_minus_0.1db": bake_s + timed.precompute_seconds + steps_needed * per_step,
+        "steps": steps, "seconds_to_refine": plain["init_s"] + plain["precompute_s"] + plain["optimize_s"],
+        "bake_s": plain["init_s"], "precompute_s": plain["precompute_s"], "optimize_s": plain["optimize_s"],
+        "ms_per_step": plain["ms_per_step"], "psnr_initial_db": plain["init_psnr_db"],
+        "psnr_final_db": plain["final_psnr_db"], "psnr_final_vs_noisy_db": final_noisy,
+        "psnr_checkpoints_db": plain["psnr_checkpoints_db"], "checkpoint_every": checkpoint_every,
+        "steps_to_final_minus_0.1db": plain["steps_to_target"],
+        "seconds_to_final_minus_0.1db": plain["seconds_to_target"],
         "problem_build_s": build_s,
+        "warm_start": {
+            "method": "features (top-8 samples) -> MLP 40-64-64-3 (bf16 MFMA) -> 8-texel padding -> refinement "
+                      "started at lr 3e-3 (cosine to 2.5e-4)",
+            "init_s": warm["init_s"], "psnr_initial_db": warm["init_psnr_db"], "psnr_final_db": warm["final_psnr_db"],
+            "steps_to_target": warm["steps_to_target"], "seconds_to_target": warm["seconds_to_target"],
+            "speedup_time_to_target": ev["speedup_time_to_target"],
+            "psnr_checkpoints_db": warm["psnr_checkpoints_db"],
+            "ablation_plain_init_same_schedule": {
+                "steps_to_target": abl["steps_to_target"], "seconds_to_target": abl["seconds_to_target"],
+                "speedup_time_to_target": ev["ablation_speedup"], "psnr_final_db": abl["final_psnr_db"]},
+        },
+        "mlp": mlp_benchmark(ctx, prob, mlp),
     }
     if cpu:
         out["cpu_baseline"] = refine_cpu_estimate(prob, steps)
     return out
+
+
+def mlp_benchmark(ctx, prob, mlp, reps=20):
+    """Matrix-core throughput of the warm-start network on the problem's 2048^2 features:
+    inference (13,696 FLOP / texel) and training steps (gradient + Adam, 3 x the
+    forward FLOP plus the weight-gradient products) on 65,536-texel batches."""
+    import torch
+
+    from photara_drender_amd import warmstart
+
+    feats = warmstart.problem_features(ctx, prob).reshape(-1, 40).contiguous()
+    n = feats.shape[0]
+    ctx.set_timing(True)
+    for _ in range(reps):
+        mlp.predict(feats, clamp=None)
+    batch = feats[:65536].contiguous()
+    target = torch.rand((65536, 3), device=feats.device)
+    trainer = warmstart.WarmStartMLP(ctx, mlp.params.cpu().numpy())
+    for _ in range(reps):
+        trainer.train_step(batch, target, None, 1e-4)
+    torch.cuda.synchronize()
+    infer_ms, k1 = ctx.kernel_time("mlp_infer")
+    grad_ms, k2 = ctx.kernel_time("mlp_grad")
+    red_ms, k3 = ctx.kernel_time("mlp_reduce")
+    ctx.set_timing(False)
+    fwd = 2 * (40 * 64 + 64 * 64 + 64 * 3)
+    infer_ms /= max(k1, 1)
+    grad_ms /= max(k2, 1)
+    red_ms /= max(k3, 1)
+    infer_tf = n * fwd / (infer_ms * 1e-3) / 1e12
+    train_tf = 65536 * 3 * fwd / (grad_ms * 1e-3) / 1e12
+    return {"texels": n, "infer_ms": infer_ms, "infer_tflops": infer_tf, "infer_mfma_frac": infer_tf / BF16_PEAK_TFLOPS,
+            "infer_gbs": n * 172 / (infer_ms * 1e-3) / 1e9, "grad_ms_65536": grad_ms, "reduce_adam_ms": red_ms,
+            "train_tflops": train_tf, "train_mfma_frac": train_tf / BF16_PEAK_TFLOPS,
+            "note": "FLOP counted as 2 x multiply-accumulates of the 40-64-64-3 network (13,696 per texel forward, "
+                    "3x for a training step); bytes = 160 B features + 12 B colour per texel"}
 
 
 def refine_cpu_estimate(prob, steps, views=2, sample_steps=4):

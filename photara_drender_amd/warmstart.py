@@ -33,9 +33,9 @@ IN, HID, OUT = 40, 64, 3
 PARAMS = _native.MLP_PARAMS
 
 # Training scenes: the 2048^2 / 64-view refinement workload scaled to a 512^2
-# atlas with 16 views (same texels per chart, mesh vertices per texel and pixels
-# per texel over all views).
-TRAIN_SCENE = dict(nx=250, ny=125, charts=4, atlas_size=512, views=16, size=(384, 512), noise=0.02)
+# atlas (same texels per chart, mesh vertices per texel, views per texel and
+# pixels per texel in each view: 64 views of 192 x 256).
+TRAIN_SCENE = dict(nx=250, ny=125, charts=4, atlas_size=512, views=64, size=(192, 256), noise=0.02)
 TRAIN_SEED0 = 1000
 
 
