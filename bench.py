@@ -35,6 +35,86 @@ BF16_PEAK_TFLOPS = 2500.0  # dense bf16 matrix-core peak of MI355X
 HBM_PEAK_GBS = 8000.0
 
 
+TRAFFIC_PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                               "raster_interp_16view_hbm_traffic_pmc.txt")
+COPY_BYTES = 1 << 30  # calibration copy of the traffic probe
+# library timing name -> HIP kernel whose counters it covers
+KERNEL_OF = {"interp_bwd": "k_interp_bwd", "interp_fwd": "k_interp_fwd", "raster_resolve": "k_raster_resolve",
+             "corner_grads": "k_corner_small", "bwd_prepass": "k_bwd_prepass", "vertex_reduce": "k_vertex_reduce",
+             "raster_candidates": "k_raster_small"}
+
+
+def _pmc_rows(directory):
+    """Per-dispatch counter values of a rocprofv3 --pmc run: {(dispatch, kernel): {counter: value}}."""
+    import csv
+    import glob
+
+    out = {}
+    for path in glob.glob(os.path.join(directory, "**", "*counter_collection.csv"), recursive=True):
+        with open(path) as f:
+            for r in csv.DictReader(f):
+                key = (r["Dispatch_Id"], r["Kernel_Name"])
+                out.setdefault(key, {}).setdefault(r["Counter_Name"], 0.0)
+                out[key][r["Counter_Name"]] += float(r["Counter_Value"])
+    return out
+
+
+def _short(name):
+    for k in sorted(set(KERNEL_OF.values()) | {"k_raster_large", "k_corner_large", "k_sample_tables"}, key=len,
+                    reverse=True):
+        if k in name:
+            return k
+    if "copyBuffer" in name or "copy" in name.lower():
+        return "copy"
+    return None
+
+
+def traffic_summary(fetch_dir, write_dir, out_path, commands):
+    """HBM bytes per launch from FETCH_SIZE and WRITE_SIZE (KiB per dispatch), each scaled by the
+    factor that makes the probe's 1 GiB device copy read and write exactly 1 GiB."""
+    per = {}
+    for counter, d in (("FETCH_SIZE", fetch_dir), ("WRITE_SIZE", write_dir)):
+        for (disp, name), vals in _pmc_rows(d).items():
+            k = _short(name)
+            if k and counter in vals:
+                per.setdefault(k, {}).setdefault(counter, []).append(vals[counter])
+    mean = lambda k, c: sum(per[k][c]) / len(per[k][c])
+    copy_fetch = max(per["copy"]["FETCH_SIZE"]) * 1024.0
+    copy_write = max(per["copy"]["WRITE_SIZE"]) * 1024.0
+    ff, fw = COPY_BYTES / copy_fetch, COPY_BYTES / copy_write
+    lines = ["HBM traffic per launch of the 16-view raster/interp fwd+bwd workload (200,000 triangles, 15,163,392 px,",
+             "A = 8), from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (KiB per dispatch, mean over dispatches),",
+             "each scaled by the factor that makes a 1 GiB device-to-device copy read and write 1 GiB.", "",
+             "commands:"] + ["  " + c for c in commands] + [
+             "", f"calibration: copy FETCH_SIZE {copy_fetch / 1024:.0f} KiB -> factor {ff:.4f}; "
+                 f"WRITE_SIZE {copy_write / 1024:.0f} KiB -> factor {fw:.4f}", "",
+             f"{'kernel':20s} {'dispatches':>10s} {'FETCH KiB':>12s} {'WRITE KiB':>12s} {'bytes/launch':>14s}"]
+    rows = []
+    for k in sorted(per):
+        if k == "copy" or "FETCH_SIZE" not in per[k] or "WRITE_SIZE" not in per[k]:
+            continue
+        b = mean(k, "FETCH_SIZE") * 1024 * ff + mean(k, "WRITE_SIZE") * 1024 * fw
+        lines.append(f"{k:20s} {len(per[k]['FETCH_SIZE']):10d} {mean(k, 'FETCH_SIZE'):12.0f} "
+                     f"{mean(k, 'WRITE_SIZE'):12.0f} {b:14.0f}")
+        rows.append(f"traffic {k} {b:.0f}")
+    with open(out_path, "w") as f:
+        f.write("\n".join(lines + [""] + rows) + "\n")
+    return out_path
+
+
+def load_traffic(kernel):
+    """Calibrated HBM bytes per launch of ``kernel`` from the committed --pmc summary (None if absent)."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 3 and parts[0] == "traffic" and parts[1] == kernel:
+                    return float(parts[2])
+    except OSError:
+        pass
+    return None
+
+
 def algorithmic_bytes(V, T, A, pixels):
     """Every API-boundary array read once / written once (reference layouts)."""
     n = len(pixels)
@@ -317,7 +397,19 @@ def main():
     ap.add_argument("--cpu-views", type=int, default=16)
     ap.add_argument("--no-refine", action="store_true", help="skip the atlas-refinement workload")
     ap.add_argument("--no-chain", action="store_true", help="skip the single-view textured chain workload")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="run 3 steps of the headline workload and a 1 GiB device copy (for rocprofv3 --pmc runs)")
+    ap.add_argument("--traffic-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+                    help="write the HBM traffic summary from two rocprofv3 --pmc output folders")
     args = ap.parse_args()
+    if args.traffic_summary:
+        cmds = [f"rocprofv3 --pmc FETCH_SIZE --output-format csv -d {args.traffic_summary[0]} -o pmc -- "
+                "python3 bench.py --traffic-probe",
+                f"rocprofv3 --pmc WRITE_SIZE --output-format csv -d {args.traffic_summary[1]} -o pmc -- "
+                "python3 bench.py --traffic-probe",
+                f"python3 bench.py --traffic-summary {args.traffic_summary[0]} {args.traffic_summary[1]}"]
+        print(traffic_summary(args.traffic_summary[0], args.traffic_summary[1], TRAFFIC_PROFILE, cmds))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU layout "
               "(views sharded across GPUs, one gradient all-reduce per step) is design-only.", file=sys.stderr)
@@ -380,6 +472,18 @@ def main():
                                                            idx.data_ptr(), T, grad_attr.data_ptr(), stream))
         _native.check(lib.pdr_rasterize_backward_batched(ctx.handle, rviews, n, V, idx.data_ptr(), T, stream))
 
+    if args.traffic_probe:
+        step()
+        torch.cuda.synchronize()
+        for _ in range(2):
+            step()
+        src = torch.ones(COPY_BYTES // 4, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print("traffic probe: 3 steps and a 1 GiB copy", flush=True)
+        return
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
@@ -419,7 +523,7 @@ def main():
                    "views": n, "pixels": P, "triangles": T, "vertices": V, "attributes": A},
         "kernel_ms": {k: round(v, 4) for k, v in kernels.items()},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic(KERNEL_OF.get(dominant, dominant)),
                      "alg_bytes_per_launch": alg[dominant]},
         "cpu_baseline": None,
     }

@@ -127,6 +127,12 @@ struct RefineView {
     const float* target;  // [P,3]
     const float* mask;    // [P]
     uint32_t pixels;
+    // bilinear taps of the pixels with mask > 0, grouped by texel (ascending):
+    const uint32_t* texels;       // [ntexels] texels the view samples
+    const uint32_t* tap_offsets;  // [ntexels + 1] entry range of each texel
+    const uint32_t* taps;         // [entries] pixel * 4 + tap, ascending within a texel
+    const float* tap_weights;     // [entries] bilinear weight of the tap
+    uint32_t ntexels;
 };
 
 hipError_t transform_positions(const float* pos3, uint32_t V, const float* m16_host, float* out4, hipStream_t st);
@@ -134,9 +140,18 @@ hipError_t refine_view_inputs(const float* image, uint32_t channels, const float
                               float* target3, float* mask, double* partials, uint32_t npartials, hipStream_t st);
 uint32_t refine_partial_count(uint32_t P);
 hipError_t sum_partials(const double* partials, uint32_t n, double* out, hipStream_t st);
-hipError_t photometric_step(const RefineView* views_dev, uint32_t batch, uint32_t max_pixels, const float* tex,
-                            uint32_t tw, uint32_t th, float eps, const RefineStepTables& tables,
-                            const uint32_t* step_counter, float* grad, float* loss, hipStream_t st);
+hipError_t photometric_step(const RefineView* views_dev, uint32_t batch, uint32_t max_pixels, uint32_t max_texels,
+                            const float* tex, uint32_t tw, uint32_t th, float eps, const RefineStepTables& tables,
+                            const uint32_t* step_counter, float4* resid, float* grad, float* loss, hipStream_t st);
+// Precompute of the gather form: tap keys / values of one view, weights of the sorted entries, last offset.
+hipError_t photo_taps(const float* uv, const float* mask, uint32_t P, uint32_t tw, uint32_t th, uint32_t* keys,
+                      uint32_t* vals, uint32_t* count, hipStream_t st);
+hipError_t photo_tap_weights(const uint32_t* vals, uint32_t E, const float* uv, uint32_t tw, uint32_t th, float* w,
+                             hipStream_t st);
+size_t photo_group_temp_bytes(uint32_t n);
+hipError_t photo_group_taps(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out, uint32_t* vals_out,
+                            uint32_t n, int end_bit, uint32_t E, uint32_t* texels, uint32_t* runlen,
+                            uint32_t* offsets, uint32_t* runs, void* temp, size_t temp_bytes, hipStream_t st);
 hipError_t seam_step(const float* tex, uint32_t tw, uint32_t th, const float* pairs, uint32_t npairs, float eps,
                      float norm, float weight, const uint32_t* step_counter, uint32_t loss_base, float* grad,
                      float* loss, hipStream_t st);

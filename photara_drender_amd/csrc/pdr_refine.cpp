@@ -111,6 +111,21 @@ pdr_status pdr_refine(pdr_context ctx, const float* initial_texture, const float
         double* partials = bufs.alloc<double>(pdr::refine_partial_count(max_pixels) + 1);
         std::vector<pdr::RefineView> rviews(nviews);
         std::vector<double> valid_weight(nviews);
+        // gather form of the photometric gradient: per view, the bilinear taps grouped by texel
+        const uint32_t ntaps = max_pixels * 4u;
+        uint32_t* tap_keys = bufs.alloc<uint32_t>(ntaps);
+        uint32_t* tap_vals = bufs.alloc<uint32_t>(ntaps);
+        uint32_t* tap_keys_sorted = bufs.alloc<uint32_t>(ntaps);
+        uint32_t* tap_vals_sorted = bufs.alloc<uint32_t>(ntaps);
+        uint32_t* group_texels = bufs.alloc<uint32_t>(ntaps);
+        uint32_t* group_runlen = bufs.alloc<uint32_t>(ntaps);
+        uint32_t* group_offsets = bufs.alloc<uint32_t>(ntaps + 1);
+        uint32_t* group_counts = bufs.alloc<uint32_t>(2);  // masked pixels, runs
+        const size_t group_temp_bytes = pdr::photo_group_temp_bytes(ntaps);
+        void* group_temp = bufs.alloc<unsigned char>(group_temp_bytes);
+        int key_bits = 1;
+        while (key_bits < 32 && (1ull << key_bits) <= static_cast<unsigned long long>(o.width) * o.height) ++key_bits;
+        uint32_t max_texels = 0;
         for (uint32_t k = 0; k < nviews; ++k) {
             const pdr_refine_view& v = views[k];
             const uint32_t P = v.width * v.height;
@@ -138,7 +153,37 @@ pdr_status pdr_refine(pdr_context ctx, const float* initial_texture, const float
             hip_check(hipMemcpyAsync(&w, partials + np, sizeof(double), hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
             sync(st);
             valid_weight[k] = w;
-            rviews[k] = pdr::RefineView{vuv, target, mask, P};
+            hip_check(pdr::fill_u32(group_counts, 0u, 2, st), "fill");
+            hip_check(pdr::photo_taps(vuv, mask, P, o.width, o.height, tap_keys, tap_vals, group_counts, st),
+                      "photo_taps");
+            uint32_t masked = 0;
+            hip_check(hipMemcpyAsync(&masked, group_counts, sizeof(masked), hipMemcpyDeviceToHost, st),
+                      "hipMemcpyAsync");
+            sync(st);
+            const uint32_t E = masked * 4u;
+            hip_check(pdr::photo_group_taps(tap_keys, tap_vals, tap_keys_sorted, tap_vals_sorted, P * 4u, key_bits, E,
+                                            group_texels, group_runlen, group_offsets, group_counts + 1, group_temp,
+                                            group_temp_bytes, st),
+                      "photo_group_taps");
+            uint32_t runs = 0;
+            if (E) {
+                hip_check(hipMemcpyAsync(&runs, group_counts + 1, sizeof(runs), hipMemcpyDeviceToHost, st),
+                          "hipMemcpyAsync");
+                sync(st);
+            }
+            uint32_t* view_texels = bufs.alloc<uint32_t>(runs);
+            uint32_t* view_offsets = bufs.alloc<uint32_t>(static_cast<size_t>(runs) + 1);
+            uint32_t* taps = bufs.alloc<uint32_t>(E);
+            float* tap_w = bufs.alloc<float>(E);
+            if (E) {
+                copy_any(view_texels, group_texels, static_cast<size_t>(runs) * 4, st);
+                copy_any(view_offsets, group_offsets, (static_cast<size_t>(runs) + 1) * 4, st);
+                copy_any(taps, tap_vals_sorted, static_cast<size_t>(E) * 4, st);
+                hip_check(pdr::photo_tap_weights(taps, E, vuv, o.width, o.height, tap_w, st), "photo_tap_weights");
+            }
+            max_texels = std::max(max_texels, runs);
+            rviews[k] = pdr::RefineView{vuv, target, mask, P, view_texels, view_offsets, taps, tap_w, runs};
+            sync(st);  // the grouping scratch is reused by the next view
         }
         pdr::RefineView* dviews = bufs.alloc<pdr::RefineView>(nviews);
         hip_check(hipMemcpyAsync(dviews, rviews.data(), nviews * sizeof(pdr::RefineView), hipMemcpyHostToDevice, st),
@@ -154,6 +199,7 @@ pdr_status pdr_refine(pdr_context ctx, const float* initial_texture, const float
         float* m1 = bufs.alloc<float>(n);
         float* m2 = bufs.alloc<float>(n);
         float* loss = bufs.alloc<float>(o.steps + seam_steps);
+        float4* resid = bufs.alloc<float4>(static_cast<size_t>(o.batch_size) * max_pixels);
         uint32_t* counters = bufs.alloc<uint32_t>(4);  // step, done
         copy_any(tex, initial_texture, static_cast<size_t>(n) * 4, st);
         hip_check(pdr::fill_f32(grad, 0.0f, n, st), "fill");
@@ -212,8 +258,8 @@ pdr_status pdr_refine(pdr_context ctx, const float* initial_texture, const float
         // ---------------------------------------------------------- optimisation
         const auto t2 = std::chrono::steady_clock::now();
         auto one_step = [&](hipStream_t s) {
-            hip_check(pdr::photometric_step(dviews, o.batch_size, max_pixels, tex, o.width, o.height,
-                                            o.photometric_epsilon, tables, counters, grad, loss, s),
+            hip_check(pdr::photometric_step(dviews, o.batch_size, max_pixels, max_texels, tex, o.width, o.height,
+                                            o.photometric_epsilon, tables, counters, resid, grad, loss, s),
                       "photometric_step");
             if (o.joint_seam_weight != 0.0f && npairs)
                 hip_check(pdr::seam_step(tex, o.width, o.height, dpairs, npairs, o.seam_epsilon, seam_norm,

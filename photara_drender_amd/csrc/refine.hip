@@ -3,19 +3,29 @@
 //  k_transform        clip = M [p,1] per vertex (texture_refiner.cpp:36-50)
 //  k_view_inputs      target RGB, mask and per-block mask sums of one view
 //                     (texture_refiner.cpp:228-242)
-//  k_photometric      atlas_photometric_gradient.hlsl:45-108 for all views of a
-//                     step in one launch; per-step scalars come from device
-//                     tables indexed by a device step counter so the step loop
-//                     can be captured once and replayed
+//  k_photo_residual   atlas_photometric_gradient.hlsl:45-108 for all views of a
+//                     step in one launch, up to the per-pixel output gradient
+//                     d / r * m * norm (and the loss); per-step scalars come
+//                     from device tables indexed by a device step counter so
+//                     the step loop can be captured once and replayed
+//  k_photo_gather     the texel gradient as a gather: each view's bilinear taps
+//                     are inverted once at precompute (k_photo_taps, a radix
+//                     sort by texel, run-length offsets, k_photo_tap_weights),
+//                     so a step sums sum_taps og * w per texel of a view with a
+//                     plain read-modify-write, one batch slot after another -
+//                     the same terms as the reference's per-pixel atomic
+//                     scatter, in a fixed order and without atomics
 //  k_seam             atlas_seam_gradient.hlsl:50-100
 //  k_tv               gradient of the chart-aware total variation of
 //                     python/photara_drender/atlas.py:84-115 (extension)
 //  k_adam             atlas_adam_update.hlsl:20-43, float4-vectorised; the
 //                     last workgroup to finish advances the step counter
 //
-// Gradient scatters use hardware float atomics (the reference emulates them
-// with compare-exchange loops); per-64-pixel loss partials use the shaders'
-// tree order exactly.
+// The seam and TV scatters use hardware float atomics (the reference emulates
+// them with compare-exchange loops); per-64-pixel loss partials use the
+// shaders' tree order exactly.
+#include <hipcub/hipcub.hpp>
+
 #include "pdr_internal.hpp"
 
 namespace pdr {
@@ -63,11 +73,21 @@ __global__ __launch_bounds__(kBlock) void k_view_inputs(const float* __restrict_
     }
 }
 
-__global__ void k_sum_partials(const double* __restrict__ partials, uint32_t n, double* __restrict__ out) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// Fixed-order sum: 256 lanes add contiguous chunks, lane 0 adds the chunk sums in order.
+__global__ __launch_bounds__(kBlock) void k_sum_partials(const double* __restrict__ partials, uint32_t n,
+                                                         double* __restrict__ out) {
+    __shared__ double chunk[kBlock];
+    const uint32_t per = (n + kBlock - 1) / kBlock;
+    const uint32_t b = threadIdx.x * per, e = b + per < n ? b + per : n;
     double s = 0.0;
-    for (uint32_t i = 0; i < n; ++i) s += partials[i];
-    *out = s;
+    for (uint32_t i = b; i < e; ++i) s += partials[i];
+    chunk[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (uint32_t i = 0; i < kBlock; ++i) t += chunk[i];
+        *out = t;
+    }
 }
 
 struct AtlasTap {
@@ -96,11 +116,12 @@ __device__ __forceinline__ AtlasTap atlas_tap(float u, float v, uint32_t tw, uin
     return a;
 }
 
-__global__ __launch_bounds__(kBlock) void k_photometric(const RefineView* __restrict__ views, uint32_t batch,
-                                                        const float* __restrict__ tex, uint32_t tw, uint32_t th,
-                                                        float eps, RefineStepTables tables,
-                                                        const uint32_t* __restrict__ step_counter,
-                                                        float* __restrict__ grad, float* __restrict__ loss) {
+__global__ __launch_bounds__(kBlock) void k_photo_residual(const RefineView* __restrict__ views, uint32_t batch,
+                                                           const float* __restrict__ tex, uint32_t tw, uint32_t th,
+                                                           float eps, RefineStepTables tables,
+                                                           const uint32_t* __restrict__ step_counter,
+                                                           float4* __restrict__ resid, uint32_t max_pixels,
+                                                           float* __restrict__ loss) {
     const uint32_t step = *step_counter;
     const uint32_t b = blockIdx.y;
     const RefineView& rv = views[tables.views[static_cast<size_t>(step) * batch + b]];
@@ -113,6 +134,7 @@ __global__ __launch_bounds__(kBlock) void k_photometric(const RefineView* __rest
         if (m > 0.0f) {
             const float2 uv = reinterpret_cast<const float2*>(rv.uv)[p];
             const AtlasTap a = atlas_tap(uv.x, uv.y, tw, th);
+            float og[3];
 #pragma unroll
             for (int ch = 0; ch < 3; ++ch) {
                 float rendered = 0.0f;
@@ -120,15 +142,80 @@ __global__ __launch_bounds__(kBlock) void k_photometric(const RefineView* __rest
                 for (int k = 0; k < 4; ++k) rendered += tex[static_cast<size_t>(a.t[k]) * 3 + ch] * a.w[k];
                 const float d = rendered - rv.target[static_cast<size_t>(p) * 3 + ch];
                 const float r = sqrtf(d * d + eps * eps);
-                const float og = d / r * m * norm;
+                og[ch] = d / r * m * norm;
                 local += r * m * norm;
-#pragma unroll
-                for (int k = 0; k < 4; ++k) unsafeAtomicAdd(&grad[static_cast<size_t>(a.t[k]) * 3 + ch], og * a.w[k]);
             }
+            resid[static_cast<size_t>(b) * max_pixels + p] = make_float4(og[0], og[1], og[2], 0.0f);
         }
     }
     const float s = wave_tree_sum(local);
     if ((threadIdx.x & 63u) == 0 && s != 0.0f) unsafeAtomicAdd(&loss[step], s);
+}
+
+// Texel gradient of batch slot b: one lane per texel the slot's view samples.
+__global__ __launch_bounds__(kBlock) void k_photo_gather(const RefineView* __restrict__ views, uint32_t batch,
+                                                         uint32_t b, RefineStepTables tables,
+                                                         const uint32_t* __restrict__ step_counter,
+                                                         const float4* __restrict__ resid, float* __restrict__ grad) {
+    const uint32_t step = *step_counter;
+    const RefineView& rv = views[tables.views[static_cast<size_t>(step) * batch + b]];
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= rv.ntexels) return;
+    const uint32_t e1 = rv.tap_offsets[i + 1];
+    float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
+    for (uint32_t e = rv.tap_offsets[i]; e < e1; ++e) {
+        const float w = rv.tap_weights[e];
+        const float4 r = resid[rv.taps[e] >> 2];
+        a0 += r.x * w;
+        a1 += r.y * w;
+        a2 += r.z * w;
+    }
+    const size_t t = static_cast<size_t>(rv.texels[i]) * 3;
+    grad[t] += a0;
+    grad[t + 1] += a1;
+    grad[t + 2] += a2;
+}
+
+// Tap entries of one view: for every pixel with mask > 0 its 4 bilinear taps
+// (key = texel, value = pixel * 4 + tap); other pixels get key tw * th, which
+// sorts past every texel.  count += pixels with mask > 0.
+__global__ void k_photo_taps(const float* __restrict__ uv, const float* __restrict__ mask, uint32_t P, uint32_t tw,
+                             uint32_t th, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
+                             uint32_t* __restrict__ count) {
+    const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+    bool on = false;
+    if (p < P) {
+        on = mask[p] > 0.0f;
+        AtlasTap a{};
+        if (on) {
+            const float2 c = reinterpret_cast<const float2*>(uv)[p];
+            a = atlas_tap(c.x, c.y, tw, th);
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            keys[static_cast<size_t>(p) * 4 + k] = on ? a.t[k] : tw * th;
+            vals[static_cast<size_t>(p) * 4 + k] = p * 4u + static_cast<uint32_t>(k);
+        }
+    }
+    const unsigned long long mk = __ballot(on);
+    if (mk && (threadIdx.x & 63u) == static_cast<uint32_t>(__ffsll(static_cast<long long>(mk)) - 1))
+        atomicAdd(count, static_cast<uint32_t>(__popcll(mk)));
+}
+
+// Bilinear weight of every sorted tap entry (the same expression as the scatter's).
+__global__ void k_photo_tap_weights(const uint32_t* __restrict__ vals, uint32_t E, const float* __restrict__ uv,
+                                    uint32_t tw, uint32_t th, float* __restrict__ w) {
+    const uint32_t e = blockIdx.x * kBlock + threadIdx.x;
+    if (e >= E) return;
+    const uint32_t v = vals[e];
+    const float2 c = reinterpret_cast<const float2*>(uv)[v >> 2];
+    const AtlasTap a = atlas_tap(c.x, c.y, tw, th);
+    w[e] = a.w[v & 3u];
+}
+
+// offsets[U] = E (the run starts come from an exclusive scan of the run lengths).
+__global__ void k_set_last(uint32_t* __restrict__ offsets, const uint32_t* __restrict__ runs, uint32_t E) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) offsets[*runs] = E;
 }
 
 __global__ __launch_bounds__(kBlock) void k_seam(const float* __restrict__ tex, uint32_t tw, uint32_t th,
@@ -283,15 +370,66 @@ hipError_t refine_view_inputs(const float* image, uint32_t channels, const float
 }
 
 hipError_t sum_partials(const double* partials, uint32_t n, double* out, hipStream_t st) {
-    k_sum_partials<<<1, 64, 0, st>>>(partials, n, out);
+    k_sum_partials<<<1, kBlock, 0, st>>>(partials, n, out);
     return hipGetLastError();
 }
 
-hipError_t photometric_step(const RefineView* views_dev, uint32_t batch, uint32_t max_pixels, const float* tex,
-                            uint32_t tw, uint32_t th, float eps, const RefineStepTables& tables,
-                            const uint32_t* step_counter, float* grad, float* loss, hipStream_t st) {
+hipError_t photometric_step(const RefineView* views_dev, uint32_t batch, uint32_t max_pixels, uint32_t max_texels,
+                            const float* tex, uint32_t tw, uint32_t th, float eps, const RefineStepTables& tables,
+                            const uint32_t* step_counter, float4* resid, float* grad, float* loss, hipStream_t st) {
     const dim3 grid((max_pixels + kBlock - 1) / kBlock, batch);
-    k_photometric<<<grid, kBlock, 0, st>>>(views_dev, batch, tex, tw, th, eps, tables, step_counter, grad, loss);
+    k_photo_residual<<<grid, kBlock, 0, st>>>(views_dev, batch, tex, tw, th, eps, tables, step_counter, resid,
+                                              max_pixels, loss);
+    const uint32_t gblocks = (max_texels + kBlock - 1) / kBlock;
+    for (uint32_t b = 0; b < batch; ++b)
+        k_photo_gather<<<gblocks ? gblocks : 1, kBlock, 0, st>>>(views_dev, batch, b, tables, step_counter,
+                                                                 resid + static_cast<size_t>(b) * max_pixels, grad);
+    return hipGetLastError();
+}
+
+hipError_t photo_taps(const float* uv, const float* mask, uint32_t P, uint32_t tw, uint32_t th, uint32_t* keys,
+                      uint32_t* vals, uint32_t* count, hipStream_t st) {
+    k_photo_taps<<<(P + kBlock - 1) / kBlock, kBlock, 0, st>>>(uv, mask, P, tw, th, keys, vals, count);
+    return hipGetLastError();
+}
+
+hipError_t photo_tap_weights(const uint32_t* vals, uint32_t E, const float* uv, uint32_t tw, uint32_t th, float* w,
+                             hipStream_t st) {
+    if (!E) return hipSuccess;
+    k_photo_tap_weights<<<(E + kBlock - 1) / kBlock, kBlock, 0, st>>>(vals, E, uv, tw, th, w);
+    return hipGetLastError();
+}
+
+size_t photo_group_temp_bytes(uint32_t n) {
+    size_t a = 0, b = 0, c = 0;
+    (void)hipcub::DeviceRadixSort::SortPairs(nullptr, a, static_cast<const uint32_t*>(nullptr),
+                                             static_cast<uint32_t*>(nullptr), static_cast<const uint32_t*>(nullptr),
+                                             static_cast<uint32_t*>(nullptr), static_cast<int>(n));
+    (void)hipcub::DeviceRunLengthEncode::Encode(nullptr, b, static_cast<const uint32_t*>(nullptr),
+                                                static_cast<uint32_t*>(nullptr), static_cast<uint32_t*>(nullptr),
+                                                static_cast<uint32_t*>(nullptr), static_cast<int>(n));
+    (void)hipcub::DeviceScan::ExclusiveSum(nullptr, c, static_cast<const uint32_t*>(nullptr),
+                                           static_cast<uint32_t*>(nullptr), static_cast<int>(n));
+    return a > b ? (a > c ? a : c) : (b > c ? b : c);
+}
+
+// Sorts n tap entries by texel (stable), then groups the first E (the taps of
+// pixels with mask > 0): texels [runs], run lengths, offsets [runs] (the last
+// offset is set by set_last_offset once runs is known on the device).
+hipError_t photo_group_taps(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out, uint32_t* vals_out,
+                            uint32_t n, int end_bit, uint32_t E, uint32_t* texels, uint32_t* runlen,
+                            uint32_t* offsets, uint32_t* runs, void* temp, size_t temp_bytes, hipStream_t st) {
+    size_t bytes = temp_bytes;
+    hipError_t e = hipcub::DeviceRadixSort::SortPairs(temp, bytes, keys_in, keys_out, vals_in, vals_out,
+                                                      static_cast<int>(n), 0, end_bit, st);
+    if (e != hipSuccess || E == 0) return e;
+    bytes = temp_bytes;
+    e = hipcub::DeviceRunLengthEncode::Encode(temp, bytes, keys_out, texels, runlen, runs, static_cast<int>(E), st);
+    if (e != hipSuccess) return e;
+    bytes = temp_bytes;
+    e = hipcub::DeviceScan::ExclusiveSum(temp, bytes, runlen, offsets, static_cast<int>(E), st);
+    if (e != hipSuccess) return e;
+    k_set_last<<<1, 64, 0, st>>>(offsets, runs, E);
     return hipGetLastError();
 }
 
