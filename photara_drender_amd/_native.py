@@ -125,6 +125,7 @@ EXPORTED_SYMBOLS = tuple(_SIGNATURES)
 # Host-only self-test hooks exported by libpdr outside the public header.
 _TESTING_SIGNATURES = {
     "pdr_testing_candidate_region": (_I, [_P, _U32, _U32, _P, _U32, _I, _P, _P]),
+    "pdr_testing_tv_gradient": (_I, [_P, _U32, _U32, _P, _F, _F, _P]),
 }
 
 _lib = None

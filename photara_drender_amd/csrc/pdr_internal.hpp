@@ -152,14 +152,31 @@ size_t photo_group_temp_bytes(uint32_t n);
 hipError_t photo_group_taps(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out, uint32_t* vals_out,
                             uint32_t n, int end_bit, uint32_t E, uint32_t* texels, uint32_t* runlen,
                             uint32_t* offsets, uint32_t* runs, void* temp, size_t temp_bytes, hipStream_t st);
+// Taps of fixed sample points grouped by texel (see RefineView).
+struct TapGroups {
+    const uint32_t* texels;
+    const uint32_t* offsets;
+    const uint32_t* taps;
+    const float* weights;
+    uint32_t ntexels;
+};
 hipError_t seam_step(const float* tex, uint32_t tw, uint32_t th, const float* pairs, uint32_t npairs, float eps,
-                     float norm, float weight, const uint32_t* step_counter, uint32_t loss_base, float* grad,
-                     float* loss, hipStream_t st);
-hipError_t tv_step(const float* tex, uint32_t tw, uint32_t th, const int32_t* chart, float eps, float weight_over_pairs,
-                   float* grad, hipStream_t st);
+                     float norm, float weight, const uint32_t* step_counter, uint32_t loss_base, const TapGroups& taps,
+                     float4* resid, float* grad, float* loss, hipStream_t st);
+hipError_t seam_taps(const float* pairs, uint32_t npairs, uint32_t tw, uint32_t th, uint32_t* keys, uint32_t* vals,
+                     hipStream_t st);
+hipError_t seam_tap_weights(const uint32_t* vals, uint32_t E, const float* pairs, uint32_t tw, uint32_t th, float* w,
+                            hipStream_t st);
+hipError_t adam_tv_step(const float* tex, float* tex_out, float* grad, float* m, float* v, uint32_t tw, uint32_t th,
+                        const int32_t* chart, float tv_eps, float tv_scale, const float* lr_table,
+                        const float* bc1_table, const float* bc2_table, float b1, float b2, float eps, float cmin,
+                        float cmax, const uint32_t* step_counter, hipStream_t st);
+hipError_t advance_step(uint32_t* step_counter, hipStream_t st);
+hipError_t tv_gradient(const float* tex, uint32_t tw, uint32_t th, const int32_t* chart, float eps, float scale,
+                       float* grad, hipStream_t st);
 hipError_t adam_step(float* tex, float* grad, float* m, float* v, uint32_t n, const float* lr_table,
                      const float* bc1_table, const float* bc2_table, float b1, float b2, float eps, float cmin,
-                     float cmax, uint32_t* step_counter, uint32_t* done_counter, hipStream_t st);
+                     float cmax, const uint32_t* step_counter, hipStream_t st);
 hipError_t tv_pair_count(const int32_t* chart, uint32_t tw, uint32_t th, unsigned long long* out, hipStream_t st);
 
 // ---- bake.hip

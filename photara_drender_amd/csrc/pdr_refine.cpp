@@ -200,6 +200,7 @@ pdr_status pdr_refine(pdr_context ctx, const float* initial_texture, const float
         float* m2 = bufs.alloc<float>(n);
         float* loss = bufs.alloc<float>(o.steps + seam_steps);
         float4* resid = bufs.alloc<float4>(static_cast<size_t>(o.batch_size) * max_pixels);
+        float4* seam_resid = bufs.alloc<float4>(npairs);
         uint32_t* counters = bufs.alloc<uint32_t>(4);  // step, done
         copy_any(tex, initial_texture, static_cast<size_t>(n) * 4, st);
         hip_check(pdr::fill_f32(grad, 0.0f, n, st), "fill");
@@ -208,6 +209,32 @@ pdr_status pdr_refine(pdr_context ctx, const float* initial_texture, const float
         hip_check(pdr::fill_f32(loss, 0.0f, o.steps + seam_steps, st), "fill");
         hip_check(pdr::fill_u32(counters, 0u, 4, st), "fill");
         const float* dpairs = device_view(bufs, seam_pairs, static_cast<size_t>(npairs) * 4, st);
+        // the seam pairs' taps grouped by texel (gather form of the seam gradient)
+        pdr::TapGroups seam_groups{nullptr, nullptr, nullptr, nullptr, 0};
+        if (npairs) {
+            const uint32_t E = npairs * 8u;
+            uint32_t* keys = bufs.alloc<uint32_t>(E);
+            uint32_t* vals = bufs.alloc<uint32_t>(E);
+            uint32_t* keys_sorted = bufs.alloc<uint32_t>(E);
+            uint32_t* vals_sorted = bufs.alloc<uint32_t>(E);
+            uint32_t* texels_tmp = bufs.alloc<uint32_t>(E);
+            uint32_t* runlen = bufs.alloc<uint32_t>(E);
+            uint32_t* offsets = bufs.alloc<uint32_t>(static_cast<size_t>(E) + 1);
+            uint32_t* runs_dev = bufs.alloc<uint32_t>(1);
+            const size_t temp_bytes = pdr::photo_group_temp_bytes(E);
+            void* temp = bufs.alloc<unsigned char>(temp_bytes);
+            hip_check(pdr::seam_taps(dpairs, npairs, o.width, o.height, keys, vals, st), "seam_taps");
+            hip_check(pdr::photo_group_taps(keys, vals, keys_sorted, vals_sorted, E, key_bits, E, texels_tmp, runlen,
+                                            offsets, runs_dev, temp, temp_bytes, st),
+                      "photo_group_taps");
+            uint32_t runs = 0;
+            hip_check(hipMemcpyAsync(&runs, runs_dev, sizeof(runs), hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
+            sync(st);
+            float* weights = bufs.alloc<float>(E);
+            hip_check(pdr::seam_tap_weights(vals_sorted, E, dpairs, o.width, o.height, weights, st),
+                      "seam_tap_weights");
+            seam_groups = pdr::TapGroups{texels_tmp, offsets, vals_sorted, weights, runs};
+        }
 
         // src/texture_refiner.cpp:315-356
         std::vector<float> lr(o.steps), bc1(o.steps), bc2(o.steps), norm(o.steps);
@@ -256,48 +283,65 @@ pdr_status pdr_refine(pdr_context ctx, const float* initial_texture, const float
         sync(st);
 
         // ---------------------------------------------------------- optimisation
+        // With the TV term, Adam reads each texel's neighbours from the pre-step
+        // atlas and writes the new one to the other buffer: `cur` is the buffer
+        // holding the current atlas (host-tracked; a captured block of steps
+        // starts from a fixed parity).
+        const bool fused_tv = o.tv_weight != 0.0f;
+        float* texbuf[2] = {tex, fused_tv ? bufs.alloc<float>(n) : tex};
         const auto t2 = std::chrono::steady_clock::now();
-        auto one_step = [&](hipStream_t s) {
-            hip_check(pdr::photometric_step(dviews, o.batch_size, max_pixels, max_texels, tex, o.width, o.height,
-                                            o.photometric_epsilon, tables, counters, resid, grad, loss, s),
+        auto one_step = [&](hipStream_t s, uint32_t c) {
+            hip_check(pdr::photometric_step(dviews, o.batch_size, max_pixels, max_texels, texbuf[c], o.width,
+                                            o.height, o.photometric_epsilon, tables, counters, resid, grad, loss, s),
                       "photometric_step");
             if (o.joint_seam_weight != 0.0f && npairs)
-                hip_check(pdr::seam_step(tex, o.width, o.height, dpairs, npairs, o.seam_epsilon, seam_norm,
-                                         o.joint_seam_weight, counters, 0, grad, nullptr, s),
+                hip_check(pdr::seam_step(texbuf[c], o.width, o.height, dpairs, npairs, o.seam_epsilon, seam_norm,
+                                         o.joint_seam_weight, counters, 0, seam_groups, seam_resid, grad, nullptr, s),
                           "seam_step");
-            if (o.tv_weight != 0.0f)
-                hip_check(pdr::tv_step(tex, o.width, o.height, dchart, tv_eps, tv_scale, grad, s), "tv_step");
-            hip_check(pdr::adam_step(tex, grad, m1, m2, n, d_lr, d_bc1, d_bc2, o.adam_beta1, o.adam_beta2,
-                                     o.adam_epsilon, o.clamp_min, o.clamp_max, counters, counters + 1, s),
-                      "adam_step");
+            if (fused_tv)
+                hip_check(pdr::adam_tv_step(texbuf[c], texbuf[c ^ 1u], grad, m1, m2, o.width, o.height, dchart, tv_eps,
+                                            tv_scale, d_lr, d_bc1, d_bc2, o.adam_beta1, o.adam_beta2, o.adam_epsilon,
+                                            o.clamp_min, o.clamp_max, counters, s),
+                          "adam_tv_step");
+            else
+                hip_check(pdr::adam_step(texbuf[c], grad, m1, m2, n, d_lr, d_bc1, d_bc2, o.adam_beta1, o.adam_beta2,
+                                         o.adam_epsilon, o.clamp_min, o.clamp_max, counters, s),
+                          "adam_step");
+            hip_check(pdr::advance_step(counters, s), "advance_step");
+            return fused_tv ? c ^ 1u : c;
         };
         const uint32_t block = o.checkpoint_every ? o.checkpoint_every : std::min<uint32_t>(o.steps, 50);
-        auto capture = [&](uint32_t count, GraphExec& g) {
+        auto capture = [&](uint32_t count, uint32_t c, GraphExec& g) {
             hip_check(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-            for (uint32_t k = 0; k < count; ++k) one_step(st);
+            for (uint32_t k = 0; k < count; ++k) c = one_step(st, c);
             hip_check(hipStreamEndCapture(st, &g.graph), "hipStreamEndCapture");
             hip_check(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0), "hipGraphInstantiate");
         };
-        GraphExec full, tail;
         const uint32_t nblocks = o.steps / block, rest = o.steps % block;
+        const uint32_t flips = fused_tv ? 1u : 0u;
+        GraphExec full[2], tail;
         if (o.use_graph) {
-            if (nblocks) capture(block, full);
-            if (rest) capture(rest, tail);
+            if (nblocks) capture(block, 0u, full[0]);
+            if (nblocks > 1 && (block & flips)) capture(block, 1u, full[1]);
+            if (rest) capture(rest, (nblocks * block) & flips, tail);
         }
-        uint32_t done = 0, ckpt = 0;
+        uint32_t done = 0, ckpt = 0, cur = 0;
         const size_t atlas_bytes = static_cast<size_t>(n) * 4;
         for (uint32_t b = 0; b <= nblocks; ++b) {
             const uint32_t count = b < nblocks ? block : rest;
             if (!count) continue;
-            if (o.use_graph)
-                hip_check(hipGraphLaunch(b < nblocks ? full.exec : tail.exec, st), "hipGraphLaunch");
-            else
-                for (uint32_t k = 0; k < count; ++k) one_step(st);
+            if (o.use_graph) {
+                hip_check(hipGraphLaunch(b < nblocks ? full[cur].exec : tail.exec, st), "hipGraphLaunch");
+                cur ^= count & flips;
+            } else {
+                for (uint32_t k = 0; k < count; ++k) cur = one_step(st, cur);
+            }
             done += count;
             if (o.checkpoint_every && o.checkpoints && done % o.checkpoint_every == 0)
-                copy_any(reinterpret_cast<char*>(o.checkpoints) + static_cast<size_t>(ckpt++) * atlas_bytes, tex,
-                         atlas_bytes, st);
+                copy_any(reinterpret_cast<char*>(o.checkpoints) + static_cast<size_t>(ckpt++) * atlas_bytes,
+                         texbuf[cur], atlas_bytes, st);
         }
+        tex = texbuf[cur];
         if (seam_steps) {
             // texture_refiner.cpp:371-415: reset Adam, constant seam learning rate
             std::vector<float> slr(seam_steps, o.seam_learning_rate), sb1(seam_steps), sb2(seam_steps);
@@ -317,11 +361,12 @@ pdr_status pdr_refine(pdr_context ctx, const float* initial_texture, const float
             hip_check(pdr::fill_u32(counters, 0u, 2, st), "fill");
             for (uint32_t k = 0; k < seam_steps; ++k) {
                 hip_check(pdr::seam_step(tex, o.width, o.height, dpairs, npairs, o.seam_epsilon, seam_norm, 1.0f,
-                                         counters, o.steps, grad, loss, st),
+                                         counters, o.steps, seam_groups, seam_resid, grad, loss, st),
                           "seam_step");
                 hip_check(pdr::adam_step(tex, grad, m1, m2, n, d_slr, d_sb1, d_sb2, o.adam_beta1, o.adam_beta2,
-                                         o.adam_epsilon, o.clamp_min, o.clamp_max, counters, counters + 1, st),
+                                         o.adam_epsilon, o.clamp_min, o.clamp_max, counters, st),
                           "adam_step");
+                hip_check(pdr::advance_step(counters, st), "advance_step");
             }
         }
         sync(st);
@@ -335,6 +380,16 @@ pdr_status pdr_refine(pdr_context ctx, const float* initial_texture, const float
             times_out2[1] = std::chrono::duration<double>(t3 - t2).count();
         }
     });
+}
+
+// Self-test hook outside the public header: the chart-TV gradient that the
+// refiner's fused Adam step adds (scale * d/dt per texel), on device arrays,
+// synchronously on the default stream.  Returns a hipError_t.
+int pdr_testing_tv_gradient(const float* tex, uint32_t tw, uint32_t th, const int32_t* chart, float eps, float scale,
+                            float* grad) {
+    hipError_t e = pdr::tv_gradient(tex, tw, th, chart, eps, scale, grad, nullptr);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    return static_cast<int>(e);
 }
 
 }  // extern "C"

@@ -15,15 +15,17 @@
 //                     plain read-modify-write, one batch slot after another -
 //                     the same terms as the reference's per-pixel atomic
 //                     scatter, in a fixed order and without atomics
-//  k_seam             atlas_seam_gradient.hlsl:50-100
-//  k_tv               gradient of the chart-aware total variation of
-//                     python/photara_drender/atlas.py:84-115 (extension)
-//  k_adam             atlas_adam_update.hlsl:20-43, float4-vectorised; the
-//                     last workgroup to finish advances the step counter
+//  k_seam_residual /  atlas_seam_gradient.hlsl:50-100, in the same residual +
+//  k_seam_gather      gather form over the (fixed) seam pairs' taps
+//  k_adam             atlas_adam_update.hlsl:20-43, float4-vectorised
+//  k_adam_tv          the same update with the gradient of the chart-aware
+//                     total variation of python/photara_drender/atlas.py:
+//                     84-115 (extension) added per texel first
+//  k_advance_step     moves the device step counter on after a step
 //
-// The seam and TV scatters use hardware float atomics (the reference emulates
-// them with compare-exchange loops); per-64-pixel loss partials use the
-// shaders' tree order exactly.
+// No gradient uses float atomics (the reference emulates them with
+// compare-exchange loops); per-64-pixel loss partials use the shaders' tree
+// order exactly.
 #include <hipcub/hipcub.hpp>
 
 #include "pdr_internal.hpp"
@@ -218,17 +220,22 @@ __global__ void k_set_last(uint32_t* __restrict__ offsets, const uint32_t* __res
     if (threadIdx.x == 0 && blockIdx.x == 0) offsets[*runs] = E;
 }
 
-__global__ __launch_bounds__(kBlock) void k_seam(const float* __restrict__ tex, uint32_t tw, uint32_t th,
-                                                 const float* __restrict__ pairs, uint32_t npairs, float eps,
-                                                 float norm, float weight, const uint32_t* __restrict__ step_counter,
-                                                 uint32_t loss_base, float* __restrict__ grad,
-                                                 float* __restrict__ loss) {
+// Seam pairs (atlas_seam_gradient.hlsl:50-100) up to the per-pair output
+// gradient g = d / r * norm (* weight); the texel gradient is gathered by
+// k_seam_gather over the pairs' taps grouped by texel (k_seam_taps at setup).
+__global__ __launch_bounds__(kBlock) void k_seam_residual(const float* __restrict__ tex, uint32_t tw, uint32_t th,
+                                                          const float* __restrict__ pairs, uint32_t npairs, float eps,
+                                                          float norm, float weight,
+                                                          const uint32_t* __restrict__ step_counter,
+                                                          uint32_t loss_base, float4* __restrict__ resid,
+                                                          float* __restrict__ loss) {
     const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
     float local = 0.0f;
     if (q < npairs) {
         const float4 pr = reinterpret_cast<const float4*>(pairs)[q];
         const AtlasTap a0 = atlas_tap(pr.x, pr.y, tw, th);
         const AtlasTap a1 = atlas_tap(pr.z, pr.w, tw, th);
+        float g3[3];
 #pragma unroll
         for (int ch = 0; ch < 3; ++ch) {
             float c0 = 0.0f, c1 = 0.0f;
@@ -242,29 +249,76 @@ __global__ __launch_bounds__(kBlock) void k_seam(const float* __restrict__ tex, 
             float g = d / r * norm;
             local += r * norm;
             if (weight != 1.0f) g = g * weight;
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                unsafeAtomicAdd(&grad[static_cast<size_t>(a0.t[k]) * 3 + ch], g * a0.w[k]);
-                unsafeAtomicAdd(&grad[static_cast<size_t>(a1.t[k]) * 3 + ch], -g * a1.w[k]);
-            }
+            g3[ch] = g;
         }
+        resid[q] = make_float4(g3[0], g3[1], g3[2], 0.0f);
     }
     const float s = wave_tree_sum(local);
     if (loss && (threadIdx.x & 63u) == 0 && s != 0.0f) unsafeAtomicAdd(&loss[loss_base + *step_counter], s);
 }
 
-// d/dt of weight * sum_pairs sqrt((t_b - t_a)^2 + eps^2) / (pairs * 3) over
-// horizontal and vertical neighbour pairs inside one chart.
-__global__ __launch_bounds__(kBlock) void k_tv(const float* __restrict__ tex, uint32_t tw, uint32_t th,
-                                               const int32_t* __restrict__ chart, float eps, float scale,
-                                               float* __restrict__ grad) {
+// Texel gradient of the seam term: entries (pair * 8 + side * 4 + tap) with the
+// tap weight, negated on the second side, grouped by texel.
+__global__ __launch_bounds__(kBlock) void k_seam_gather(const uint32_t* __restrict__ texels,
+                                                        const uint32_t* __restrict__ offsets,
+                                                        const uint32_t* __restrict__ taps,
+                                                        const float* __restrict__ weights, uint32_t ntexels,
+                                                        const float4* __restrict__ resid, float* __restrict__ grad) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= tw * th) return;
+    if (i >= ntexels) return;
+    const uint32_t e1 = offsets[i + 1];
+    float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
+    for (uint32_t e = offsets[i]; e < e1; ++e) {
+        const float w = weights[e];
+        const float4 r = resid[taps[e] >> 3];
+        a0 += r.x * w;
+        a1 += r.y * w;
+        a2 += r.z * w;
+    }
+    const size_t t = static_cast<size_t>(texels[i]) * 3;
+    grad[t] += a0;
+    grad[t + 1] += a1;
+    grad[t + 2] += a2;
+}
+
+__global__ void k_seam_taps(const float* __restrict__ pairs, uint32_t npairs, uint32_t tw, uint32_t th,
+                            uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
+    if (q >= npairs) return;
+    const float4 pr = reinterpret_cast<const float4*>(pairs)[q];
+    const AtlasTap a0 = atlas_tap(pr.x, pr.y, tw, th);
+    const AtlasTap a1 = atlas_tap(pr.z, pr.w, tw, th);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        keys[static_cast<size_t>(q) * 8 + k] = a0.t[k];
+        vals[static_cast<size_t>(q) * 8 + k] = q * 8u + static_cast<uint32_t>(k);
+        keys[static_cast<size_t>(q) * 8 + 4 + k] = a1.t[k];
+        vals[static_cast<size_t>(q) * 8 + 4 + k] = q * 8u + 4u + static_cast<uint32_t>(k);
+    }
+}
+
+__global__ void k_seam_tap_weights(const uint32_t* __restrict__ vals, uint32_t E, const float* __restrict__ pairs,
+                                   uint32_t tw, uint32_t th, float* __restrict__ w) {
+    const uint32_t e = blockIdx.x * kBlock + threadIdx.x;
+    if (e >= E) return;
+    const uint32_t v = vals[e];
+    const float4 pr = reinterpret_cast<const float4*>(pairs)[v >> 3];
+    const bool second = (v & 4u) != 0u;
+    const AtlasTap a = second ? atlas_tap(pr.z, pr.w, tw, th) : atlas_tap(pr.x, pr.y, tw, th);
+    const float wt = a.w[v & 3u];
+    w[e] = second ? -wt : wt;
+}
+
+// d/dt of weight * sum_pairs sqrt((t_b - t_a)^2 + eps^2) / (pairs * 3) over
+// horizontal and vertical neighbour pairs inside one chart, at texel i
+// (g = 0 outside every chart).
+__device__ __forceinline__ void tv_texel(const float* __restrict__ tex, uint32_t tw, uint32_t th,
+                                         const int32_t* __restrict__ chart, float eps, uint32_t i, float g[3]) {
+    g[0] = g[1] = g[2] = 0.0f;
     const uint32_t x = i % tw, y = i / tw;
     const int32_t c = chart ? chart[i] : 0;
     if (c < 0) return;
     const float* t = tex + static_cast<size_t>(i) * 3;
-    float g[3] = {0.0f, 0.0f, 0.0f};
     auto pair = [&](uint32_t j, float sign) {
         const int32_t cj = chart ? chart[j] : 0;
         if (cj != c) return;
@@ -279,8 +333,6 @@ __global__ __launch_bounds__(kBlock) void k_tv(const float* __restrict__ tex, ui
     if (x + 1 < tw) pair(i + 1, -1.0f);  // pair (x, x+1): d = right - t, dL/dt = -d/r
     if (y > 0) pair(i - tw, 1.0f);
     if (y + 1 < th) pair(i + tw, -1.0f);
-#pragma unroll
-    for (int ch = 0; ch < 3; ++ch) grad[static_cast<size_t>(i) * 3 + ch] += g[ch] * scale;
 }
 
 __global__ void k_tv_pairs(const int32_t* __restrict__ chart, uint32_t tw, uint32_t th,
@@ -309,12 +361,55 @@ __device__ __forceinline__ float adam_one(float p, float& g, float& m, float& v,
     return clampf(p - lr * c1 / (sqrtf(c2) + eps), cmin, cmax);
 }
 
+// Adam with the chart-TV gradient (scale = weight / (pairs * 3)) added to each
+// texel's gradient first, as the separate TV pass would: one lane per texel.
+// Every texel reads its neighbours' pre-step values: the updated atlas goes to
+// tex_out (a second buffer), and the caller swaps the two.
+__global__ __launch_bounds__(kBlock) void k_adam_tv(const float* __restrict__ tex, float* __restrict__ tex_out,
+                                                    float* __restrict__ grad, float* __restrict__ mom1,
+                                                    float* __restrict__ mom2, uint32_t tw, uint32_t th,
+                                                    const int32_t* __restrict__ chart, float tv_eps, float tv_scale,
+                                                    const float* __restrict__ lr_t, const float* __restrict__ bc1_t,
+                                                    const float* __restrict__ bc2_t, float b1, float b2, float eps,
+                                                    float cmin, float cmax, const uint32_t* __restrict__ step_counter) {
+    const uint32_t step = *step_counter;
+    const float lr = lr_t[step], bc1 = bc1_t[step], bc2 = bc2_t[step];
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < tw * th; i += gridDim.x * kBlock) {
+        float g[3];
+        tv_texel(tex, tw, th, chart, tv_eps, i, g);
+#pragma unroll
+        for (int ch = 0; ch < 3; ++ch) {
+            const size_t j = static_cast<size_t>(i) * 3 + ch;
+            float gj = grad[j] + g[ch] * tv_scale;
+            float m = mom1[j], v = mom2[j];
+            tex_out[j] = adam_one(tex[j], gj, m, v, lr, b1, b2, eps, bc1, bc2, cmin, cmax);
+            grad[j] = gj;
+            mom1[j] = m;
+            mom2[j] = v;
+        }
+    }
+}
+
+// The TV gradient alone (scale * d/dt per texel), for tests of tv_texel.
+__global__ void k_tv_gradient(const float* __restrict__ tex, uint32_t tw, uint32_t th,
+                              const int32_t* __restrict__ chart, float eps, float scale, float* __restrict__ grad) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= tw * th) return;
+    float g[3];
+    tv_texel(tex, tw, th, chart, eps, i, g);
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) grad[static_cast<size_t>(i) * 3 + ch] = g[ch] * scale;
+}
+
+__global__ void k_advance_step(uint32_t* step_counter) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *step_counter += 1u;
+}
+
 __global__ __launch_bounds__(kBlock) void k_adam(float* __restrict__ tex, float* __restrict__ grad,
                                                  float* __restrict__ mom1, float* __restrict__ mom2, uint32_t n,
                                                  const float* __restrict__ lr_t, const float* __restrict__ bc1_t,
                                                  const float* __restrict__ bc2_t, float b1, float b2, float eps,
-                                                 float cmin, float cmax, uint32_t* step_counter,
-                                                 uint32_t* done_counter) {
+                                                 float cmin, float cmax, const uint32_t* __restrict__ step_counter) {
     const uint32_t step = *step_counter;
     const float lr = lr_t[step], bc1 = bc1_t[step], bc2 = bc2_t[step];
     const uint32_t n4 = n / 4;
@@ -335,18 +430,6 @@ __global__ __launch_bounds__(kBlock) void k_adam(float* __restrict__ tex, float*
     if (blockIdx.x == 0) {
         for (uint32_t i = n4 * 4 + threadIdx.x; i < n; i += kBlock)
             tex[i] = adam_one(tex[i], grad[i], mom1[i], mom2[i], lr, b1, b2, eps, bc1, bc2, cmin, cmax);
-    }
-    // The last workgroup to finish advances the step (every workgroup has read
-    // `step` before it arrives here).
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        const uint32_t done = atomicAdd(done_counter, 1u);
-        if (done == gridDim.x - 1) {
-            *done_counter = 0u;
-            *step_counter = step + 1;
-            __threadfence();
-        }
     }
 }
 
@@ -434,17 +517,50 @@ hipError_t photo_group_taps(const uint32_t* keys_in, const uint32_t* vals_in, ui
 }
 
 hipError_t seam_step(const float* tex, uint32_t tw, uint32_t th, const float* pairs, uint32_t npairs, float eps,
-                     float norm, float weight, const uint32_t* step_counter, uint32_t loss_base, float* grad,
-                     float* loss, hipStream_t st) {
+                     float norm, float weight, const uint32_t* step_counter, uint32_t loss_base, const TapGroups& taps,
+                     float4* resid, float* grad, float* loss, hipStream_t st) {
     if (!npairs) return hipSuccess;
-    k_seam<<<(npairs + kBlock - 1) / kBlock, kBlock, 0, st>>>(tex, tw, th, pairs, npairs, eps, norm, weight,
-                                                             step_counter, loss_base, grad, loss);
+    k_seam_residual<<<(npairs + kBlock - 1) / kBlock, kBlock, 0, st>>>(tex, tw, th, pairs, npairs, eps, norm, weight,
+                                                                      step_counter, loss_base, resid, loss);
+    if (taps.ntexels)
+        k_seam_gather<<<(taps.ntexels + kBlock - 1) / kBlock, kBlock, 0, st>>>(
+            taps.texels, taps.offsets, taps.taps, taps.weights, taps.ntexels, resid, grad);
     return hipGetLastError();
 }
 
-hipError_t tv_step(const float* tex, uint32_t tw, uint32_t th, const int32_t* chart, float eps,
-                   float weight_over_pairs, float* grad, hipStream_t st) {
-    k_tv<<<(tw * th + kBlock - 1) / kBlock, kBlock, 0, st>>>(tex, tw, th, chart, eps, weight_over_pairs, grad);
+hipError_t seam_taps(const float* pairs, uint32_t npairs, uint32_t tw, uint32_t th, uint32_t* keys, uint32_t* vals,
+                     hipStream_t st) {
+    if (!npairs) return hipSuccess;
+    k_seam_taps<<<(npairs + kBlock - 1) / kBlock, kBlock, 0, st>>>(pairs, npairs, tw, th, keys, vals);
+    return hipGetLastError();
+}
+
+hipError_t seam_tap_weights(const uint32_t* vals, uint32_t E, const float* pairs, uint32_t tw, uint32_t th, float* w,
+                            hipStream_t st) {
+    if (!E) return hipSuccess;
+    k_seam_tap_weights<<<(E + kBlock - 1) / kBlock, kBlock, 0, st>>>(vals, E, pairs, tw, th, w);
+    return hipGetLastError();
+}
+
+hipError_t adam_tv_step(const float* tex, float* tex_out, float* grad, float* m, float* v, uint32_t tw, uint32_t th,
+                        const int32_t* chart, float tv_eps, float tv_scale, const float* lr_table,
+                        const float* bc1_table, const float* bc2_table, float b1, float b2, float eps, float cmin,
+                        float cmax, const uint32_t* step_counter, hipStream_t st) {
+    uint32_t grid = (tw * th + kBlock - 1) / kBlock;
+    if (grid > 8192) grid = 8192;
+    k_adam_tv<<<grid, kBlock, 0, st>>>(tex, tex_out, grad, m, v, tw, th, chart, tv_eps, tv_scale, lr_table, bc1_table,
+                                       bc2_table, b1, b2, eps, cmin, cmax, step_counter);
+    return hipGetLastError();
+}
+
+hipError_t tv_gradient(const float* tex, uint32_t tw, uint32_t th, const int32_t* chart, float eps, float scale,
+                       float* grad, hipStream_t st) {
+    k_tv_gradient<<<(tw * th + kBlock - 1) / kBlock, kBlock, 0, st>>>(tex, tw, th, chart, eps, scale, grad);
+    return hipGetLastError();
+}
+
+hipError_t advance_step(uint32_t* step_counter, hipStream_t st) {
+    k_advance_step<<<1, 64, 0, st>>>(step_counter);
     return hipGetLastError();
 }
 
@@ -455,13 +571,13 @@ hipError_t tv_pair_count(const int32_t* chart, uint32_t tw, uint32_t th, unsigne
 
 hipError_t adam_step(float* tex, float* grad, float* m, float* v, uint32_t n, const float* lr_table,
                      const float* bc1_table, const float* bc2_table, float b1, float b2, float eps, float cmin,
-                     float cmax, uint32_t* step_counter, uint32_t* done_counter, hipStream_t st) {
+                     float cmax, const uint32_t* step_counter, hipStream_t st) {
     const uint32_t n4 = n / 4;
     uint32_t grid = (n4 + kBlock - 1) / kBlock;
-    if (grid > 2048) grid = 2048;
+    if (grid > 8192) grid = 8192;
     if (grid == 0) grid = 1;
     k_adam<<<grid, kBlock, 0, st>>>(tex, grad, m, v, n, lr_table, bc1_table, bc2_table, b1, b2, eps, cmin, cmax,
-                                    step_counter, done_counter);
+                                    step_counter);
     return hipGetLastError();
 }
 

@@ -138,3 +138,59 @@ def test_refine_validation_messages():
         baker.refine_texture(init, pos, uv, np.array([[0, 1, 9]], np.uint32), [img], eye)
     with pytest.raises(ValueError, match="each image must have shape"):
         baker.refine_texture(init, pos, uv, idx, [img[..., :2].copy()], eye)
+
+
+def test_refine_tv_gradient_matches_autograd():
+    """The chart-TV gradient the fused Adam step adds (pdr_testing_tv_gradient) against
+    torch autograd (float64) of atlas_total_variation (python/photara_drender/atlas.py:
+    84-115), texels with chart id < 0 outside every pair."""
+    torch = pytest.importorskip("torch")
+    from photara_drender_amd import _native
+    from photara_drender_amd.atlas import atlas_total_variation
+
+    rng = np.random.default_rng(21)
+    h, w = 29, 37
+    tex = rng.uniform(0, 1, (h, w, 3)).astype(np.float32)
+    tex[5:9, 5:9] = 0.5  # flat patch: differences below epsilon
+    chart = (np.arange(w)[None, :] // 9 + 4 * (np.arange(h)[:, None] // 10)).astype(np.int32)
+    chart[rng.uniform(size=(h, w)) < 0.1] = -1
+    eps, weight = 1e-4, 3e-2
+    t64 = torch.tensor(tex.astype(np.float64), requires_grad=True)
+    c = torch.tensor(chart)
+    loss = atlas_total_variation(t64, valid_mask=c >= 0, chart_ids=c, epsilon=eps) * weight
+    loss.backward()
+    pairs = int(((chart[:, 1:] == chart[:, :-1]) & (chart[:, 1:] >= 0)).sum()
+                + ((chart[1:] == chart[:-1]) & (chart[1:] >= 0)).sum())
+    dev = torch.device("cuda:0")
+    _native.Context(0)
+    dt, dc = torch.tensor(tex, device=dev), torch.tensor(chart, device=dev)
+    g = torch.empty_like(dt)
+    assert _native.lib().pdr_testing_tv_gradient(dt.data_ptr(), w, h, dc.data_ptr(), eps, weight / (pairs * 3),
+                                                 g.data_ptr()) == 0
+    np.testing.assert_allclose(g.cpu().numpy(), t64.grad.numpy(), rtol=1e-5, atol=1e-9)
+
+
+def test_refine_graph_matches_eager_with_tv_and_odd_blocks():
+    """Graph replay equals the eager loop with the fused TV + Adam step, whose atlas
+    alternates between two buffers, for blocks of an odd number of steps (7) and a
+    2-step tail: checkpoints and the final atlas."""
+    torch = pytest.importorskip("torch")
+    from photara_drender_amd import _native
+    from photara_drender_amd.refine import refine_atlas
+
+    sc, mats, masks, images, pairs = small_atlas(views=4)
+    dev = torch.device("cuda:0")
+    ctx = _native.Context(0)
+    t = lambda a: torch.tensor(np.ascontiguousarray(a), device=dev)
+    args = (t(np.full((64, 64, 3), 0.5, np.float32)), t(sc.positions), t(sc.uv), t(sc.indices.view(np.int32)),
+            [t(im) for im in images], mats)
+    kw = dict(visibility_masks=[t(m) for m in masks], seam_uv_pairs=t(pairs), steps=30, batch_size=2,
+              learning_rate=1e-2, seam_polish_steps=4, tv_weight=1e-2, chart_ids=sc.chart_texels,
+              joint_seam_weight=1e-2)
+    a = refine_atlas(ctx, *args, use_graph=True, checkpoint_every=7, **kw)
+    b = refine_atlas(ctx, *args, use_graph=False, checkpoint_every=7, **kw)
+    assert a.checkpoints.shape == (4, 64, 64, 3)
+    np.testing.assert_allclose(a.checkpoints.cpu().numpy(), b.checkpoints.cpu().numpy(), atol=2e-3)
+    np.testing.assert_allclose(a.color.cpu().numpy(), b.color.cpu().numpy(), atol=2e-3)
+    np.testing.assert_allclose(a.loss_history, b.loss_history, rtol=1e-4)
+    assert not torch.equal(a.checkpoints[0], a.checkpoints[1])
