@@ -53,7 +53,14 @@ enum {
     /* Gradient reductions in the reference's summation order (per-triangle /
      * per-texel gathers over sorted adjacency lists) instead of atomics.
      * Bit-reproducible; slower.  Default: off. */
-    PDR_FLAG_DETERMINISTIC = 1u << 1
+    PDR_FLAG_DETERMINISTIC = 1u << 1,
+    /* Autotune the raster's launch parameters (wave / queue thresholds,
+     * triangles per wave): the first call of each workload class (view count,
+     * log2 of the triangle and pixel counts) times every candidate with HIP
+     * events and caches the fastest in the context; later calls use it.
+     * Results do not depend on the choice.  Not applied while the stream is
+     * being captured.  Default: off.  No counterpart in the reference. */
+    PDR_FLAG_AUTOTUNE = 1u << 2
 };
 
 typedef struct pdr_device_info {
@@ -85,6 +92,15 @@ pdr_status pdr_enumerate_devices(pdr_device_info* infos, uint32_t capacity, uint
  * refiner's wall-clock phases, src/texture_refiner.cpp:420-435). */
 pdr_status pdr_context_set_timing(pdr_context ctx, int enable);
 pdr_status pdr_context_kernel_time(pdr_context ctx, const char* name, double* total_ms, uint64_t* launches);
+
+/* The autotuner's cache as text, one class per line:
+ * "<kind> <views> <log2 triangles> <log2 pixels> <wave_work> <fwd_waves> <flat_work> <bwd_waves>".
+ * export: writes up to capacity bytes (NUL-terminated) and the needed size
+ * (with the NUL) to *length; import: adds or replaces the listed classes;
+ * clear: forgets every class. */
+pdr_status pdr_autotune_export(pdr_context ctx, char* buffer, size_t capacity, size_t* length);
+pdr_status pdr_autotune_import(pdr_context ctx, const char* text);
+pdr_status pdr_autotune_clear(pdr_context ctx);
 
 /* Blocks until `stream` (NULL = default stream) and the context's own stream are idle. */
 pdr_status pdr_synchronize(pdr_context ctx, void* stream);

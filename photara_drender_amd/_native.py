@@ -19,6 +19,7 @@ _I = ctypes.c_int
 
 PDR_FLAG_VALIDATE_DEVICE_INPUTS = 1
 PDR_FLAG_DETERMINISTIC = 2
+PDR_FLAG_AUTOTUNE = 4
 
 CULL_MODES = {"none": 0, "back": 1, "front": 2}
 ADDRESS_MODES = {"clamp": 0, "wrap": 1, "mirror": 2}
@@ -85,6 +86,9 @@ _SIGNATURES = {
     "pdr_context_device_info": (_I, [_P, ctypes.POINTER(DeviceInfo)]),
     "pdr_enumerate_devices": (_I, [ctypes.POINTER(DeviceInfo), _U32, ctypes.POINTER(_U32)]),
     "pdr_synchronize": (_I, [_P, _P]),
+    "pdr_autotune_export": (_I, [_P, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),
+    "pdr_autotune_import": (_I, [_P, ctypes.c_char_p]),
+    "pdr_autotune_clear": (_I, [_P]),
     "pdr_context_set_timing": (_I, [_P, _I]),
     "pdr_context_kernel_time": (_I, [_P, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
                                      ctypes.POINTER(ctypes.c_uint64)]),
@@ -192,3 +196,17 @@ class Context:
     @property
     def flags(self) -> int:
         return int(lib().pdr_context_flags(self.handle))
+
+    def autotune_export(self) -> str:
+        """The autotuner's cache (one workload class per line)."""
+        n = ctypes.c_size_t()
+        check(lib().pdr_autotune_export(self.handle, None, 0, ctypes.byref(n)))
+        buf = ctypes.create_string_buffer(n.value)
+        check(lib().pdr_autotune_export(self.handle, buf, n.value, ctypes.byref(n)))
+        return buf.value.decode()
+
+    def autotune_import(self, text: str) -> None:
+        check(lib().pdr_autotune_import(self.handle, text.encode()))
+
+    def autotune_clear(self) -> None:
+        check(lib().pdr_autotune_clear(self.handle))

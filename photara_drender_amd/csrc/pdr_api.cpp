@@ -107,6 +107,69 @@ pdr_status pdr_context_kernel_time(pdr_context ctx, const char* name, double* to
     });
 }
 
+pdr_status pdr_autotune_export(pdr_context ctx, char* buffer, size_t capacity, size_t* length) {
+    return guarded([&] {
+        require_ctx(ctx);
+        std::string text;
+        {
+            Lock lock(ctx);
+            for (const auto& kv : ctx->tuned) {
+                char line[256];
+                std::snprintf(line, sizeof(line), "%s %.9g %u %.9g %u\n", kv.first.c_str(),
+                              static_cast<double>(kv.second.wave_work), kv.second.fwd_waves,
+                              static_cast<double>(kv.second.flat_work), kv.second.bwd_waves);
+                text += line;
+            }
+        }
+        if (length) *length = text.size() + 1;
+        if (buffer && capacity) {
+            const size_t n = std::min(capacity - 1, text.size());
+            std::memcpy(buffer, text.data(), n);
+            buffer[n] = '\0';
+        }
+    });
+}
+
+pdr_status pdr_autotune_import(pdr_context ctx, const char* text) {
+    return guarded([&] {
+        require_ctx(ctx);
+        require_ptr(text, "text");
+        std::map<std::string, pdr::TuneParams> parsed;
+        const char* p = text;
+        while (*p) {
+            const char* e = std::strchr(p, '\n');
+            const std::string line(p, e ? static_cast<size_t>(e - p) : std::strlen(p));
+            p = e ? e + 1 : p + line.size();
+            if (line.find_first_not_of(" \t\r") == std::string::npos) continue;
+            char kind[32];
+            unsigned views = 0, lt = 0, lp = 0, fw = 0, bw = 0;
+            double ww = 0.0, flat = 0.0;
+            if (std::sscanf(line.c_str(), "%31s %u %u %u %lf %u %lf %u", kind, &views, &lt, &lp, &ww, &fw, &flat,
+                            &bw) != 8 ||
+                (std::strcmp(kind, "forward") != 0 && std::strcmp(kind, "backward") != 0) || !(ww > 0.0) ||
+                !(flat > 0.0) || fw == 0 || bw == 0)
+                raise(PDR_INVALID_ARGUMENT, "autotune cache line is malformed: " + line);
+            pdr::TuneParams t;
+            t.wave_work = static_cast<float>(ww);
+            t.fwd_waves = fw;
+            t.flat_work = static_cast<float>(flat);
+            t.bwd_waves = bw;
+            parsed[std::string(kind) + " " + std::to_string(views) + " " + std::to_string(lt) + " " +
+                   std::to_string(lp)] = t;
+        }
+        Lock lock(ctx);
+        for (const auto& kv : parsed) ctx->tuned[kv.first] = kv.second;
+    });
+}
+
+pdr_status pdr_autotune_clear(pdr_context ctx) {
+    return guarded([&] {
+        require_ctx(ctx);
+        Lock lock(ctx);
+        ctx->tuned.clear();
+    });
+}
+
 pdr_status pdr_synchronize(pdr_context ctx, void* stream) {
     return guarded([&] {
         require_ctx(ctx);

@@ -11,6 +11,7 @@
 #include <cstring>
 #include <limits>
 #include <memory>
+#include <map>
 #include <mutex>
 #include <string>
 #include <unordered_map>
@@ -92,6 +93,7 @@ struct pdr_context_s {
     };
     std::unordered_map<std::string, Slot> scratch;
     bool keys_clean = false;  // raster key buffer is all-ones
+    std::map<std::string, pdr::TuneParams> tuned;  // autotuner: chosen launch parameters per workload class
 
     // Optional per-kernel HIP-event timing (pdr_context_set_timing).
     bool timing = false;
@@ -334,14 +336,108 @@ inline pdr::RasterScratch raster_scratch(pdr_context ctx, const pdr::ViewTable& 
     return s;
 }
 
+// ---------------------------------------------------------------- autotuner
+// Workload class of a raster call: its kind, view count and the log2 buckets
+// of its triangle and pixel counts.
+inline std::string tune_key(const char* kind, const pdr::ViewTable& vt, uint32_t T) {
+    auto lg = [](uint64_t x) {
+        int b = 0;
+        while (b < 63 && (1ull << b) < x) ++b;
+        return b;
+    };
+    return std::string(kind) + " " + std::to_string(vt.n) + " " + std::to_string(lg(T)) + " " +
+           std::to_string(lg(vt.total_pixels));
+}
+
+inline bool stream_capturing(hipStream_t st) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(st, &cs) != hipSuccess) {
+        (void)hipGetLastError();
+        return true;
+    }
+    return cs != hipStreamCaptureStatusNone;
+}
+
+// Launch parameters of a call: the cached choice of its class; else, with
+// PDR_FLAG_AUTOTUNE set and the stream not being captured, every candidate is
+// run (after one warm-up run) and timed three times with HIP events, and the
+// fastest (best of its three) is cached for the class unless the defaults
+// (the first candidate) are within 3 % of it; else the defaults.  `run`
+// enqueues the tuned work on st (each candidate produces the same results).
+template <class Run>
+pdr::TuneParams tuned_params(pdr_context ctx, const std::string& key, const std::vector<pdr::TuneParams>& cands,
+                             hipStream_t st, Run&& run) {
+    const auto it = ctx->tuned.find(key);
+    if (it != ctx->tuned.end()) return it->second;
+    if (!(ctx->flags & PDR_FLAG_AUTOTUNE) || stream_capturing(st)) return pdr::TuneParams{};
+    hipEvent_t a = ctx->take_event(), b = ctx->take_event();
+    run(cands[0]);
+    float best = 0.0f, first = 0.0f;
+    pdr::TuneParams choice = cands[0];
+    for (size_t c = 0; c < cands.size(); ++c) {
+        float fastest = 0.0f;
+        for (int rep = 0; rep < 3; ++rep) {
+            hip_check(hipEventRecord(a, st), "hipEventRecord");
+            run(cands[c]);
+            hip_check(hipEventRecord(b, st), "hipEventRecord");
+            hip_check(hipEventSynchronize(b), "hipEventSynchronize");
+            float ms = 0.0f;
+            hip_check(hipEventElapsedTime(&ms, a, b), "hipEventElapsedTime");
+            fastest = rep == 0 ? ms : std::min(fastest, ms);
+        }
+        if (c == 0) first = fastest;
+        if (c == 0 || fastest < best) {
+            best = fastest;
+            choice = cands[c];
+        }
+    }
+    if (first <= best * 1.03f) choice = cands[0];
+    ctx->event_pool.push_back(a);
+    ctx->event_pool.push_back(b);
+    ctx->tuned[key] = choice;
+    return choice;
+}
+
+// Candidates; the first is the defaults.
+inline std::vector<pdr::TuneParams> forward_candidates() {
+    std::vector<pdr::TuneParams> out{pdr::TuneParams{}};
+    for (float work : {4096.0f, 16384.0f, 65536.0f})
+        for (uint32_t waves : {2048u, 4096u, 8192u}) {
+            if (work == out[0].wave_work && waves == out[0].fwd_waves) continue;
+            pdr::TuneParams p;
+            p.wave_work = work;
+            p.fwd_waves = waves;
+            out.push_back(p);
+        }
+    return out;
+}
+
+inline std::vector<pdr::TuneParams> backward_candidates() {
+    std::vector<pdr::TuneParams> out{pdr::TuneParams{}};
+    for (float work : {256.0f, 512.0f, 1024.0f, 2048.0f})
+        for (uint32_t waves : {2048u, 4096u, 8192u}) {
+            if (work == out[0].flat_work && waves == out[0].bwd_waves) continue;
+            pdr::TuneParams p;
+            p.flat_work = work;
+            p.bwd_waves = waves;
+            out.push_back(p);
+        }
+    return out;
+}
+
 // Triangle-parallel raster forward of one table of views (device pointers), on stream st.
 inline void run_raster_forward(pdr_context ctx, const pdr::ViewTable& vt, const uint32_t* idx, uint32_t T, uint32_t V,
                                uint32_t cull, int want_deriv, hipStream_t st) {
     if (!vt.total_pixels) return;
     const pdr::RasterScratch s = raster_scratch(ctx, vt, T, st, true);
     ctx->keys_clean = false;
+    const pdr::TuneParams tp =
+        tuned_params(ctx, tune_key("forward", vt, T), forward_candidates(), st, [&](const pdr::TuneParams& c) {
+            hip_check(pdr::raster_candidates(vt, idx, T, V, cull, s, c, st), "raster_candidates");
+            hip_check(pdr::raster_resolve(vt, idx, want_deriv, s, st), "raster_resolve");  // resets the keys
+        });
     ctx->timed("raster_candidates", st,
-               [&] { hip_check(pdr::raster_candidates(vt, idx, T, V, cull, s, st), "raster_candidates"); });
+               [&] { hip_check(pdr::raster_candidates(vt, idx, T, V, cull, s, tp, st), "raster_candidates"); });
     ctx->timed("raster_resolve", st,
                [&] { hip_check(pdr::raster_resolve(vt, idx, want_deriv, s, st), "raster_resolve"); });
     ctx->keys_clean = true;
@@ -366,8 +462,12 @@ inline void run_raster_backward(pdr_context ctx, const pdr::ViewTable& vt, const
     ctx->timed("vertex_csr", st, [&] { hip_check(pdr::build_vertex_csr(idx, T, V, csr, st), "build_vertex_csr"); });
     ctx->timed("bwd_prepass", st,
                [&] { hip_check(pdr::backward_prepass(vt, idx, T, V, sb, st), "backward_prepass"); });
+    const pdr::TuneParams tp =
+        tuned_params(ctx, tune_key("backward", vt, T), backward_candidates(), st, [&](const pdr::TuneParams& c) {
+            hip_check(pdr::corner_gradients(vt, idx, T, V, sb, corner, c, st), "corner_gradients");
+        });
     ctx->timed("corner_grads", st,
-               [&] { hip_check(pdr::corner_gradients(vt, idx, T, V, sb, corner, st), "corner_gradients"); });
+               [&] { hip_check(pdr::corner_gradients(vt, idx, T, V, sb, corner, tp, st), "corner_gradients"); });
     ctx->timed("vertex_reduce", st,
                [&] { hip_check(pdr::vertex_reduce(vt, T, V, csr, corner, st), "vertex_reduce"); });
 }

@@ -78,8 +78,18 @@ struct VertexCsr {
 hipError_t fill_keys(unsigned long long* keys, size_t n, hipStream_t st);
 hipError_t sample_tables(const ViewTable& views, float* sx, float* sy, hipStream_t st);
 // k_raster_small + k_raster_large: (depth, id) keys of every covered candidate pixel
+// Launch parameters of the raster kernels, chosen per workload class by the
+// autotuner (every choice gives the same results: the forward's winner is
+// order-free and both backward paths keep the reference's per-triangle order).
+struct TuneParams {
+    float wave_work = 16384.0f;  // forward: candidate estimate walked by one wave (else queued as slabs)
+    uint32_t fwd_waves = 4096;   // forward: target wave count (sets triangles per wave, 8..64)
+    float flat_work = 512.0f;    // backward: region estimate walked in the concatenated row list
+    uint32_t bwd_waves = 4096;   // backward: target wave count
+};
+
 hipError_t raster_candidates(const ViewTable& views, const uint32_t* idx, uint32_t T, uint32_t V, uint32_t cull,
-                             const RasterScratch& s, hipStream_t st);
+                             const RasterScratch& s, const TuneParams& tp, hipStream_t st);
 // k_raster_resolve: raster / derivatives from the keys; resets the keys
 hipError_t raster_resolve(const ViewTable& views, const uint32_t* idx, int want_deriv, const RasterScratch& s,
                           hipStream_t st);
@@ -90,7 +100,7 @@ hipError_t backward_prepass(const ViewTable& views, const uint32_t* idx, uint32_
                             const RasterScratch& s, hipStream_t st);
 // corner: [views * 3T] float4 per-triangle corner gradients (after backward_prepass)
 hipError_t corner_gradients(const ViewTable& views, const uint32_t* idx, uint32_t T, uint32_t V,
-                            const RasterScratch& s, float4* corner, hipStream_t st);
+                            const RasterScratch& s, float4* corner, const TuneParams& tp, hipStream_t st);
 hipError_t vertex_reduce(const ViewTable& views, uint32_t T, uint32_t V, const VertexCsr& csr, const float4* corner,
                          hipStream_t st);
 

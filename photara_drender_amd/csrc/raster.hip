@@ -53,10 +53,11 @@ namespace pdr {
 namespace {
 
 constexpr uint32_t kBlock = 256;
-constexpr float kWaveWork = 16384.0f;    // forward: candidate estimate walked by one wave (else queued)
 constexpr uint32_t kSlabRows = 16;       // forward: rows per queued work item
-constexpr float kFlatWork = 512.0f;      // backward: region estimate walked in the wave's concatenated row list
 constexpr uint32_t kWaveBox = 65536;     // backward: box pixels summed by one wave in pixel order (else queued)
+// The forward's wave-walk limit (TuneParams::wave_work, else queued) and the
+// backward's concatenated-walk limit (TuneParams::flat_work, else the ordered
+// wave fold) are launch parameters chosen per workload class.
 constexpr uint32_t kLargeBlocks = 1024;  // persistent workgroups for queued work
 
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n) {
@@ -254,14 +255,14 @@ struct RecTri {  // what test_pixel reads
 // are concatenated and dealt 64 at a time to the lanes, each lane computing
 // its line's span; the spans of those 64 lines are concatenated in turn and
 // dealt 64 pixels at a time.  Every lane tests one candidate pixel per step
-// whatever the triangles' sizes.  Triangles above kWaveWork candidates are
+// whatever the triangles' sizes.  Triangles above wave_work candidates are
 // queued in 16-row slabs for k_raster_large.
 __global__ __launch_bounds__(kBlock) void k_raster_small(ViewTable vt, const uint32_t* __restrict__ idx, uint32_t T,
                                                          uint32_t V, uint32_t cull, const float* __restrict__ sxt,
                                                          const float* __restrict__ syt,
                                                          unsigned long long* __restrict__ keys,
                                                          uint4* __restrict__ queue, uint32_t* __restrict__ counters,
-                                                         uint32_t bsz) {
+                                                         uint32_t bsz, float wave_work) {
     __shared__ FwdRecs R;
     const uint32_t view = blockIdx.y;
     const ViewDesc& vd = vt.v[view];
@@ -279,7 +280,7 @@ __global__ __launch_bounds__(kBlock) void k_raster_small(ViewTable vt, const uin
             ScreenTri r;
             LineSpans sp;
             const bool ok = lane < bsz && t < T && setup_forward_tri(vd, idx, t, V, cull, true, r, sp);
-            if (ok && r.work > kWaveWork) {
+            if (ok && r.work > wave_work) {
                 const uint32_t slabs = static_cast<uint32_t>(r.y1 - r.y0) / kSlabRows + 1u;
                 queue[atomicAdd(&counters[0], 1u)] = make_uint4(view, t, slabs, 0u);
             } else if (ok) {
@@ -652,7 +653,7 @@ __device__ __forceinline__ void fold_list(const float* __restrict__ lterm, int32
 
 // Per-triangle corner gradients summed over the reference's bounding box in
 // its row-major order.  Each wave takes a batch of up to 64 triangles.  Triangles
-// with more than kFlatWork estimated pixels (boxes up to kWaveBox pixels) are
+// with more than flat_work estimated pixels (boxes up to kWaveBox pixels) are
 // walked by the whole wave, one at a time, adding the contributing pixels'
 // terms in ascending pixel order through ballot + readlane; larger boxes are
 // queued for k_corner_large.  The rows of the other triangles are
@@ -666,7 +667,8 @@ __global__ __launch_bounds__(kBlock, 4) void k_corner_small(ViewTable vt, const 
                                                          const uint32_t* __restrict__ cid,
                                                          const uint32_t* __restrict__ view_flags,
                                                          float4* __restrict__ corner, uint4* __restrict__ queue,
-                                                         uint32_t* __restrict__ counters, uint32_t bsz) {
+                                                         uint32_t* __restrict__ counters, uint32_t bsz,
+                                                         float flat_work) {
     __shared__ BwdRecs R;
     const uint32_t view = blockIdx.y;
     const ViewDesc& vd = vt.v[view];
@@ -695,7 +697,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_corner_small(ViewTable vt, const 
                 for (int c = 0; c < 3; ++c) cview[t * 3 + c] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             } else if (ok && g.box > kWaveBox) {
                 queue[atomicAdd(&counters[0], 1u)] = make_uint4(view, t, 0u, 0u);
-            } else if (ok && g.work > kFlatWork) {
+            } else if (ok && g.work > flat_work) {
                 deferred = true;
             } else if (ok) {
                 rows = g.yb - g.ya + 1;
@@ -933,9 +935,9 @@ __global__ void k_check_indices(const uint32_t* idx, size_t n, uint32_t V, uint3
 }
 
 // Triangles per wave batch: 64, or fewer for small meshes so that the launch
-// still has about 4096 waves (16 per CU) to overlap their latencies.
-uint32_t tri_batch(uint32_t T, uint32_t views) {
-    const uint64_t per = (static_cast<uint64_t>(T) * views + 4095) / 4096;
+// still has about `waves` waves to overlap their latencies.
+uint32_t tri_batch(uint32_t T, uint32_t views, uint32_t waves) {
+    const uint64_t per = (static_cast<uint64_t>(T) * views + waves - 1) / waves;
     return per >= 64 ? 64u : (per <= 8 ? 8u : static_cast<uint32_t>(per));
 }
 
@@ -983,12 +985,12 @@ hipError_t sample_tables(const ViewTable& vt, float* sx, float* sy, hipStream_t 
 }
 
 hipError_t raster_candidates(const ViewTable& vt, const uint32_t* idx, uint32_t T, uint32_t V, uint32_t cull,
-                             const RasterScratch& s, hipStream_t st) {
+                             const RasterScratch& s, const TuneParams& tp, hipStream_t st) {
     hipError_t e = fill_u32(s.counters, 0u, 4, st);
     if (e != hipSuccess) return e;
-    const uint32_t bsz = tri_batch(T, vt.n);
+    const uint32_t bsz = tri_batch(T, vt.n, tp.fwd_waves);
     k_raster_small<<<dim3(tri_grid(T, bsz), vt.n), kBlock, 0, st>>>(vt, idx, T, V, cull, s.sx, s.sy, s.keys, s.queue,
-                                                                    s.counters, bsz);
+                                                                    s.counters, bsz, tp.wave_work);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     k_raster_large<<<kLargeBlocks, kBlock, 0, st>>>(vt, idx, T, V, cull, s.sx, s.sy, s.keys, s.queue, s.counters);
     return hipGetLastError();
@@ -1043,12 +1045,12 @@ hipError_t build_vertex_csr(const uint32_t* idx, uint32_t T, uint32_t V, const V
 }
 
 hipError_t corner_gradients(const ViewTable& vt, const uint32_t* idx, uint32_t T, uint32_t V, const RasterScratch& s,
-                            float4* corner, hipStream_t st) {
+                            float4* corner, const TuneParams& tp, hipStream_t st) {
     hipError_t e = fill_u32(s.counters, 0u, 4, st);
     if (e != hipSuccess) return e;
-    const uint32_t bsz = tri_batch(T, vt.n);
+    const uint32_t bsz = tri_batch(T, vt.n, tp.bwd_waves);
     k_corner_small<<<dim3(tri_grid(T, bsz), vt.n), kBlock, 0, st>>>(vt, idx, T, V, s.sx, s.sy, s.cid, s.view_flags,
-                                                                    corner, s.queue, s.counters, bsz);
+                                                                    corner, s.queue, s.counters, bsz, tp.flat_work);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     k_corner_large<<<kLargeBlocks, kBlock, 0, st>>>(vt, idx, T, V, s.sx, s.sy, s.cid, s.view_flags, corner, s.queue,
                                                     s.counters);
