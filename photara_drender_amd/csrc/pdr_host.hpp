@@ -450,15 +450,15 @@ inline void run_raster_backward(pdr_context ctx, const pdr::ViewTable& vt, const
     pdr::VertexCsr csr;
     csr.counts = ctx->get_t<uint32_t>("csr_counts", static_cast<size_t>(V) + 1);
     csr.offsets = ctx->get_t<uint32_t>("csr_offsets", static_cast<size_t>(V) + 1);
-    csr.iota = ctx->get_t<uint32_t>("csr_iota", static_cast<size_t>(T) * 3);
-    csr.keys_out = ctx->get_t<uint32_t>("csr_keys", static_cast<size_t>(T) * 3);
     csr.corners = ctx->get_t<uint32_t>("csr_corners", static_cast<size_t>(T) * 3);
+    csr.slot_of = ctx->get_t<uint32_t>("csr_slot_of", static_cast<size_t>(T) * 3);
     csr.temp_bytes = pdr::csr_temp_bytes(T, V);
     csr.temp = ctx->get("csr_temp", csr.temp_bytes);
     float4* corner = ctx->get_t<float4>("corner_grads", static_cast<size_t>(vt.n) * T * 3);
     pdr::RasterScratch sb = s;
     sb.cid = ctx->get_t<uint32_t>("backward_ids", vt.total_pixels);
     sb.view_flags = ctx->get_t<uint32_t>("backward_view_flags", pdr::kMaxViews);
+    sb.slot_of = csr.slot_of;
     ctx->timed("vertex_csr", st, [&] { hip_check(pdr::build_vertex_csr(idx, T, V, csr, st), "build_vertex_csr"); });
     ctx->timed("bwd_prepass", st,
                [&] { hip_check(pdr::backward_prepass(vt, idx, T, V, sb, st), "backward_prepass"); });

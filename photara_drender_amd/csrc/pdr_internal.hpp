@@ -61,15 +61,15 @@ struct RasterScratch {
     uint32_t* counters = nullptr;        // [4] queue length, next queue entry
     uint32_t* cid = nullptr;             // [total_pixels] backward: compared id per pixel
     uint32_t* view_flags = nullptr;      // [kMaxViews] backward: view walked over whole boxes
+    const uint32_t* slot_of = nullptr;   // [3T] backward: where each corner's gradient is written (VertexCsr)
 };
 
 // Vertex -> corner adjacency (corners ascending per vertex).
 struct VertexCsr {
     uint32_t* counts = nullptr;   // [V+1]
     uint32_t* offsets = nullptr;  // [V+1]
-    uint32_t* iota = nullptr;     // [3T]
-    uint32_t* keys_out = nullptr; // [3T]
-    uint32_t* corners = nullptr;  // [3T]
+    uint32_t* corners = nullptr;  // [3T] corner ids, ascending per vertex
+    uint32_t* slot_of = nullptr;  // [3T] position of each corner in `corners`
     void* temp = nullptr;
     size_t temp_bytes = 0;
 };

@@ -42,7 +42,10 @@
 //   k_corner_large    queued boxes, one workgroup each, fixed-order reduction;
 //   k_vertex_reduce   per vertex, corner gradients summed in ascending corner
 //                     order over a vertex -> corner adjacency built on the GPU
-//                     (counting + stable radix sort), as the reference's CSR.
+//                     as the reference's CSR (counts, exclusive scan, atomic
+//                     placement, then each vertex's short list sorted by one
+//                     lane).  The corner kernels write each corner's gradient
+//                     at its position in that list, so the reduce streams.
 // No float atomics: the backward is deterministic.
 #include <hipcub/hipcub.hpp>
 
@@ -76,10 +79,6 @@ __global__ void k_fill_u64(unsigned long long* p, unsigned long long v, size_t n
     for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n;
          i += static_cast<size_t>(gridDim.x) * blockDim.x)
         p[i] = v;
-}
-
-__global__ void k_iota(uint32_t* p, uint32_t n) {
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = i;
 }
 
 uint32_t grid_for(size_t n, uint32_t block = kBlock) {
@@ -666,6 +665,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_corner_small(ViewTable vt, const 
                                                          const float* __restrict__ syt,
                                                          const uint32_t* __restrict__ cid,
                                                          const uint32_t* __restrict__ view_flags,
+                                                         const uint32_t* __restrict__ slot_of,
                                                          float4* __restrict__ corner, uint4* __restrict__ queue,
                                                          uint32_t* __restrict__ counters, uint32_t bsz,
                                                          float flat_work) {
@@ -694,7 +694,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_corner_small(ViewTable vt, const 
             bool deferred = false;
             if (mine && !ok) {
 #pragma unroll
-                for (int c = 0; c < 3; ++c) cview[t * 3 + c] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                for (int c = 0; c < 3; ++c) cview[slot_of[t * 3 + c]] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             } else if (ok && g.box > kWaveBox) {
                 queue[atomicAdd(&counters[0], 1u)] = make_uint4(view, t, 0u, 0u);
             } else if (ok && g.work > flat_work) {
@@ -759,7 +759,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_corner_small(ViewTable vt, const 
                 if (lane == 0)
 #pragma unroll
                     for (int c = 0; c < 3; ++c)
-                        cview[qt * 3 + c] = make_float4(w[c * 3], w[c * 3 + 1], 0.0f, w[c * 3 + 2]);
+                        cview[slot_of[qt * 3 + c]] = make_float4(w[c * 3], w[c * 3 + 1], 0.0f, w[c * 3 + 2]);
             }
         }
         const int32_t incl_l = wave_incl_scan(rows);
@@ -840,7 +840,8 @@ __global__ __launch_bounds__(kBlock, 4) void k_corner_small(ViewTable vt, const 
         fold_list(lterm, seg_first, seg_end, lane, s);
         if (rows > 0)
 #pragma unroll
-            for (int c = 0; c < 3; ++c) cview[t * 3 + c] = make_float4(s[c * 3], s[c * 3 + 1], 0.0f, s[c * 3 + 2]);
+            for (int c = 0; c < 3; ++c)
+                cview[slot_of[t * 3 + c]] = make_float4(s[c * 3], s[c * 3 + 1], 0.0f, s[c * 3 + 2]);
     }
 }
 
@@ -852,6 +853,7 @@ __global__ __launch_bounds__(kBlock) void k_corner_large(ViewTable vt, const uin
                                                          const float* __restrict__ syt,
                                                          const uint32_t* __restrict__ cid,
                                                          const uint32_t* __restrict__ view_flags,
+                                                         const uint32_t* __restrict__ slot_of,
                                                          float4* __restrict__ corner, const uint4* __restrict__ queue,
                                                          uint32_t* __restrict__ counters) {
     __shared__ uint32_t entry;
@@ -893,8 +895,9 @@ __global__ __launch_bounds__(kBlock) void k_corner_large(ViewTable vt, const uin
             __syncthreads();
         }
         if (threadIdx.x == 0) {
-            float4* out = corner + (static_cast<size_t>(q.x) * T + q.y) * 3;
-            for (int c = 0; c < 3; ++c) out[c] = make_float4(red[c * 3][0], red[c * 3 + 1][0], 0.0f, red[c * 3 + 2][0]);
+            float4* out = corner + static_cast<size_t>(q.x) * T * 3;
+            for (int c = 0; c < 3; ++c)
+                out[slot_of[q.y * 3 + c]] = make_float4(red[c * 3][0], red[c * 3 + 1][0], 0.0f, red[c * 3 + 2][0]);
         }
         __syncthreads();
     }
@@ -905,10 +908,50 @@ __global__ void k_count_corners(const uint32_t* __restrict__ idx, uint32_t n, ui
         if (idx[i] < V) atomicAdd(&counts[idx[i]], 1u);
 }
 
-// reduce_vertex_gradients.hlsl:12-29 for every view
+// Places every corner in its vertex's list (order within a list fixed by
+// k_sort_corner_lists); corners with an invalid vertex go past offsets[V].
+// counts (after the scan) is consumed as a per-vertex cursor; counts[V] is 0.
+__global__ void k_place_corners(const uint32_t* __restrict__ idx, uint32_t n, uint32_t V,
+                                const uint32_t* __restrict__ offsets, uint32_t* __restrict__ counts,
+                                uint32_t* __restrict__ corners, uint32_t* __restrict__ slot_of) {
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        const uint32_t v = idx[i];
+        uint32_t j;
+        if (v < V) {
+            j = offsets[v] + atomicSub(&counts[v], 1u) - 1u;
+        } else {
+            j = offsets[V] + atomicAdd(&counts[V], 1u);
+            slot_of[i] = j;
+        }
+        corners[j] = i;
+    }
+}
+
+// One lane per vertex: its list sorted ascending (Shell sort, Ciura's gaps),
+// then the inverse map corner -> list position.
+__global__ void k_sort_corner_lists(uint32_t V, const uint32_t* __restrict__ offsets, uint32_t* __restrict__ corners,
+                                    uint32_t* __restrict__ slot_of) {
+    const uint32_t gaps[8] = {701u, 301u, 132u, 57u, 23u, 10u, 4u, 1u};
+    for (uint32_t v = blockIdx.x * kBlock + threadIdx.x; v < V; v += gridDim.x * kBlock) {
+        const uint32_t b = offsets[v], n = offsets[v + 1] - b;
+        uint32_t* a = corners + b;
+        for (int g = 0; g < 8; ++g) {
+            const uint32_t gap = gaps[g];
+            for (uint32_t i = gap; i < n; ++i) {
+                const uint32_t x = a[i];
+                uint32_t j = i;
+                for (; j >= gap && a[j - gap] > x; j -= gap) a[j] = a[j - gap];
+                a[j] = x;
+            }
+        }
+        for (uint32_t i = 0; i < n; ++i) slot_of[a[i]] = b + i;
+    }
+}
+
+// reduce_vertex_gradients.hlsl:12-29 for every view: the corner gradients sit
+// in the adjacency's order, so each vertex sums a contiguous run.
 __global__ __launch_bounds__(kBlock) void k_vertex_reduce(ViewTable vt, uint32_t T, uint32_t V,
                                                           const uint32_t* __restrict__ offsets,
-                                                          const uint32_t* __restrict__ corners,
                                                           const float4* __restrict__ corner) {
     const uint32_t view = blockIdx.y;
     const ViewDesc& vd = vt.v[view];
@@ -916,7 +959,7 @@ __global__ __launch_bounds__(kBlock) void k_vertex_reduce(ViewTable vt, uint32_t
     for (uint32_t v = blockIdx.x * kBlock + threadIdx.x; v < V; v += gridDim.x * kBlock) {
         float4 s = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         for (uint32_t a = offsets[v]; a < offsets[v + 1]; ++a) {
-            const float4 c = cg[corners[a]];
+            const float4 c = cg[a];
             s.x += c.x;
             s.y += c.y;
             s.z += c.z;
@@ -1013,18 +1056,15 @@ hipError_t backward_prepass(const ViewTable& vt, const uint32_t* idx, uint32_t T
 }
 
 size_t csr_temp_bytes(uint32_t T, uint32_t V) {
-    size_t a = 0, b = 0;
+    (void)T;
+    size_t a = 0;
     (void)hipcub::DeviceScan::ExclusiveSum(nullptr, a, static_cast<const uint32_t*>(nullptr),
                                            static_cast<uint32_t*>(nullptr), static_cast<int>(V + 1));
-    (void)hipcub::DeviceRadixSort::SortPairs(nullptr, b, static_cast<const uint32_t*>(nullptr),
-                                             static_cast<uint32_t*>(nullptr), static_cast<const uint32_t*>(nullptr),
-                                             static_cast<uint32_t*>(nullptr), static_cast<int>(T * 3));
-    return a > b ? a : b;
+    return a;
 }
 
-// Vertex -> corner adjacency with corners ascending per vertex
-// (src/rasterizer.cpp:145-162): counts + exclusive scan for the offsets, a
-// stable radix sort of (vertex, corner) pairs for the lists.
+// Vertex -> corner adjacency with corners ascending per vertex, and each
+// corner's position in it.
 hipError_t build_vertex_csr(const uint32_t* idx, uint32_t T, uint32_t V, const VertexCsr& c, hipStream_t st) {
     const uint32_t n = T * 3;
     hipError_t e = fill_u32(c.counts, 0u, static_cast<size_t>(V) + 1, st);
@@ -1035,13 +1075,10 @@ hipError_t build_vertex_csr(const uint32_t* idx, uint32_t T, uint32_t V, const V
     if ((e = hipcub::DeviceScan::ExclusiveSum(c.temp, bytes, c.counts, c.offsets, static_cast<int>(V + 1), st)) !=
         hipSuccess)
         return e;
-    k_iota<<<grid_for(n), kBlock, 0, st>>>(c.iota, n);
+    k_place_corners<<<grid_for(n), kBlock, 0, st>>>(idx, n, V, c.offsets, c.counts, c.corners, c.slot_of);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    int bits = 1;
-    while (bits < 32 && (1ull << bits) < V) ++bits;
-    bytes = c.temp_bytes;
-    return hipcub::DeviceRadixSort::SortPairs(c.temp, bytes, idx, c.keys_out, c.iota, c.corners, static_cast<int>(n), 0,
-                                              bits, st);
+    k_sort_corner_lists<<<grid_for(V), kBlock, 0, st>>>(V, c.offsets, c.corners, c.slot_of);
+    return hipGetLastError();
 }
 
 hipError_t corner_gradients(const ViewTable& vt, const uint32_t* idx, uint32_t T, uint32_t V, const RasterScratch& s,
@@ -1050,17 +1087,18 @@ hipError_t corner_gradients(const ViewTable& vt, const uint32_t* idx, uint32_t T
     if (e != hipSuccess) return e;
     const uint32_t bsz = tri_batch(T, vt.n, tp.bwd_waves);
     k_corner_small<<<dim3(tri_grid(T, bsz), vt.n), kBlock, 0, st>>>(vt, idx, T, V, s.sx, s.sy, s.cid, s.view_flags,
-                                                                    corner, s.queue, s.counters, bsz, tp.flat_work);
+                                                                    s.slot_of, corner, s.queue, s.counters, bsz,
+                                                                    tp.flat_work);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    k_corner_large<<<kLargeBlocks, kBlock, 0, st>>>(vt, idx, T, V, s.sx, s.sy, s.cid, s.view_flags, corner, s.queue,
-                                                    s.counters);
+    k_corner_large<<<kLargeBlocks, kBlock, 0, st>>>(vt, idx, T, V, s.sx, s.sy, s.cid, s.view_flags, s.slot_of, corner,
+                                                    s.queue, s.counters);
     return hipGetLastError();
 }
 
 hipError_t vertex_reduce(const ViewTable& vt, uint32_t T, uint32_t V, const VertexCsr& csr, const float4* corner,
                          hipStream_t st) {
     k_vertex_reduce<<<dim3(grid_for(V) < 4096 ? grid_for(V) : 4096, vt.n), kBlock, 0, st>>>(vt, T, V, csr.offsets,
-                                                                                         csr.corners, corner);
+                                                                                         corner);
     return hipGetLastError();
 }
 
