@@ -42,10 +42,20 @@ __device__ __forceinline__ void relax(int32_t& cx, int32_t& cy, uint32_t nb, int
     }
 }
 
-__device__ __forceinline__ void count(unsigned long long* counter, bool pred) {
-    const unsigned long long mask = __ballot(pred);
-    if (mask && (threadIdx.x & 63u) == static_cast<uint32_t>(__ffsll(static_cast<long long>(mask)) - 1))
-        atomicAdd(counter, static_cast<unsigned long long>(__popcll(mask)));
+// Adds a lane count to *counter once per wave (callers loop over a bounded grid,
+// so few atomics meet on the counter).
+__device__ __forceinline__ void count_total(unsigned long long* counter, uint32_t n) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) n += __shfl_xor(n, m, 64);
+    if ((threadIdx.x & 63u) == 0 && n) atomicAdd(counter, static_cast<unsigned long long>(n));
+}
+
+// Step barrier of the sweep: LDS writes complete, then the workgroup barrier.
+// Global stores of earlier steps may stay in flight (no step reads them).
+__device__ __forceinline__ void lds_step_barrier() {
+    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
 }
 
 // Sources (:259-276): valid texels and earlier covered fills; other texels lose their fill flag.
@@ -53,54 +63,85 @@ __device__ __forceinline__ void count(unsigned long long* counter, bool pred) {
 __global__ void k_pad_prepare(uint32_t P, const float* __restrict__ valid, const float* __restrict__ coverage,
                               float* __restrict__ filled, float threshold, uint8_t* __restrict__ source,
                               unsigned long long* __restrict__ counters) {
-    const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
-    bool is_valid = false, refill = false;
-    if (p < P) {
-        is_valid = valid[p] >= threshold;
-        refill = !is_valid && filled && filled[p] > 0.5f && coverage && coverage[p] > 0.5f;
+    uint32_t nvalid = 0, nrefill = 0;
+    for (uint32_t p = blockIdx.x * kBlock + threadIdx.x; p < P; p += gridDim.x * kBlock) {
+        const bool is_valid = valid[p] >= threshold;
+        const bool refill = !is_valid && filled && filled[p] > 0.5f && coverage && coverage[p] > 0.5f;
         if (!is_valid && !refill && filled) filled[p] = 0.0f;
         source[p] = is_valid || refill ? 1 : 0;
+        nvalid += is_valid ? 1u : 0u;
+        nrefill += refill ? 1u : 0u;
     }
-    count(counters + 0, is_valid);
-    count(counters + 2, refill);
-    count(counters + 4, refill);
+    count_total(counters + 0, nvalid);
+    count_total(counters + 2, nrefill);
+    count_total(counters + 4, nrefill);
 }
 
 // One 8SSEDT sweep as a wavefront; mirror = the backward sweep.  field holds
 // packed int16 offsets: read as the initial state of the backward sweep,
-// written by both.  ring: R x 4 settled offsets (dynamic LDS).
+// written by both.  ring: R x 4 settled offsets (dynamic LDS).  Each lane
+// walks up to kClasses row classes c = lane + j * kSweepThreads; a class
+// settles (x, y) at step 2 c + x + 2 R k for its k-th row y = c + k R, and the
+// lane loads the input of a class's next texel one step ahead.
+constexpr int kClasses = 8;  // R <= kClasses * kSweepThreads (W <= 16384)
+
+__device__ __forceinline__ uint32_t sweep_input(const uint8_t* __restrict__ source, const uint32_t* __restrict__ field,
+                                                uint32_t W, uint32_t H, uint32_t x, uint32_t y, int mirror) {
+    const uint32_t rx = mirror ? W - 1u - x : x, ry = mirror ? H - 1u - y : y;
+    const size_t idx = static_cast<size_t>(ry) * W + rx;
+    return mirror ? field[idx] : static_cast<uint32_t>(source[idx]);
+}
+
 __global__ __launch_bounds__(kSweepThreads) void k_pad_sweep(uint32_t W, uint32_t H, const uint8_t* __restrict__ source,
                                                               uint32_t* __restrict__ field, int mirror) {
     extern __shared__ uint32_t ring[];
     const uint32_t R = (W + 1u) / 2u;
     const uint32_t steps = W + 2u * (H - 1u);
+    uint32_t px[kClasses], py[kClasses], next[kClasses];
+#pragma unroll
+    for (int j = 0; j < kClasses; ++j) {
+        const uint32_t c = threadIdx.x + static_cast<uint32_t>(j) * kSweepThreads;
+        px[j] = 0;
+        py[j] = c;
+        next[j] = c < R && c < H ? sweep_input(source, field, W, H, 0, c, mirror) : 0u;
+    }
     for (uint32_t t = 0; t < steps; ++t) {
-        for (uint32_t c = threadIdx.x; c < R; c += kSweepThreads) {
-            if (t < 2u * c) continue;
-            const uint32_t k = (t - 2u * c) / (2u * R);
-            const uint32_t x = t - 2u * c - 2u * R * k, y = c + k * R;
-            if (x >= W || y >= H) continue;
-            const uint32_t rx = mirror ? W - 1u - x : x, ry = mirror ? H - 1u - y : y;
-            const size_t idx = static_cast<size_t>(ry) * W + rx;
-            int32_t cx, cy;
-            if (mirror) {
-                const uint32_t v = field[idx];
-                cx = -off_x(v);
-                cy = -off_y(v);
-            } else {
-                cx = cy = source[idx] ? 0 : kFar;
+#pragma unroll
+        for (int j = 0; j < kClasses; ++j) {
+            const uint32_t c = threadIdx.x + static_cast<uint32_t>(j) * kSweepThreads;
+            if (c >= R || t < 2u * c) continue;
+            const uint32_t x = px[j], y = py[j];
+            if (x < W && y < H) {
+                const uint32_t rx = mirror ? W - 1u - x : x, ry = mirror ? H - 1u - y : y;
+                const size_t idx = static_cast<size_t>(ry) * W + rx;
+                int32_t cx, cy;
+                if (mirror) {
+                    cx = -off_x(next[j]);
+                    cy = -off_y(next[j]);
+                } else {
+                    cx = cy = next[j] ? 0 : kFar;
+                }
+                if (x > 0) relax(cx, cy, ring[c * 4u + ((t - 1u) & 3u)], -1, 0);
+                if (y > 0) {
+                    const uint32_t pc = c == 0 ? R - 1u : c - 1u;
+                    relax(cx, cy, ring[pc * 4u + ((t - 2u) & 3u)], 0, -1);
+                    if (x > 0) relax(cx, cy, ring[pc * 4u + ((t - 3u) & 3u)], -1, -1);
+                    if (x + 1u < W) relax(cx, cy, ring[pc * 4u + ((t - 1u) & 3u)], 1, -1);
+                }
+                ring[c * 4u + (t & 3u)] = pack(cx, cy);
+                field[idx] = mirror ? pack(-cx, -cy) : pack(cx, cy);
             }
-            if (x > 0) relax(cx, cy, ring[c * 4u + ((t - 1u) & 3u)], -1, 0);
-            if (y > 0) {
-                const uint32_t pc = c == 0 ? R - 1u : c - 1u;
-                relax(cx, cy, ring[pc * 4u + ((t - 2u) & 3u)], 0, -1);
-                if (x > 0) relax(cx, cy, ring[pc * 4u + ((t - 3u) & 3u)], -1, -1);
-                if (x + 1u < W) relax(cx, cy, ring[pc * 4u + ((t - 1u) & 3u)], 1, -1);
+            // the class's next texel, and its input loaded ahead
+            uint32_t nx = x + 1u, ny = y;
+            if (nx == 2u * R) {
+                nx = 0;
+                ny = y + R;
             }
-            ring[c * 4u + (t & 3u)] = pack(cx, cy);
-            field[idx] = mirror ? pack(-cx, -cy) : pack(cx, cy);
+            px[j] = nx;
+            py[j] = ny;
+            if (nx < W && ny < H) next[j] = sweep_input(source, field, W, H, nx, ny, mirror);
         }
-        __syncthreads();
+        lds_step_barrier();
     }
 }
 
@@ -112,35 +153,37 @@ __global__ void k_pad_fill(uint32_t W, uint32_t H, uint32_t C, const uint32_t* _
                            unsigned long long margin2, unsigned long long limit2, int fill_unobserved,
                            const float* src_color, float* color, float* __restrict__ filled,
                            unsigned long long* __restrict__ counters) {
-    const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
-    bool gutter = false, unobserved = false, remaining = false;
-    if (p < W * H && !source[p]) {
+    uint32_t ngutter = 0, nunobserved = 0, nremaining = 0;
+    for (uint32_t p = blockIdx.x * kBlock + threadIdx.x; p < W * H; p += gridDim.x * kBlock) {
+        if (source[p]) continue;
         const uint32_t v = field[p];
         const int64_t ox = off_x(v), oy = off_y(v);
         const unsigned long long d2 = static_cast<unsigned long long>(ox * ox + oy * oy);
         const int64_t sx = static_cast<int64_t>(p % W) + ox, sy = static_cast<int64_t>(p / W) + oy;
-        if (d2 < 2ull * kFar * kFar && sx >= 0 && sy >= 0 && sx < W && sy < H) {
-            const bool covered = coverage && coverage[p] > 0.5f;
-            const bool fill = covered ? (fill_unobserved && d2 <= limit2) : (margin2 > 0 && d2 <= margin2);
-            if (fill) {
-                const size_t s = static_cast<size_t>(sy) * W + static_cast<size_t>(sx);
-                for (uint32_t c = 0; c < C; ++c) color[static_cast<size_t>(p) * C + c] = src_color[s * C + c];
-                if (C == 4) color[static_cast<size_t>(p) * C + 3] = 1.0f;
-                if (filled) filled[p] = 1.0f;
-                gutter = !covered;
-                unobserved = covered;
-            } else {
-                remaining = covered;
-            }
+        if (d2 >= 2ull * kFar * kFar || sx < 0 || sy < 0 || sx >= W || sy >= H) continue;
+        const bool covered = coverage && coverage[p] > 0.5f;
+        const bool fill = covered ? (fill_unobserved && d2 <= limit2) : (margin2 > 0 && d2 <= margin2);
+        if (!fill) {
+            nremaining += covered ? 1u : 0u;
+            continue;
         }
+        const size_t s = static_cast<size_t>(sy) * W + static_cast<size_t>(sx);
+        for (uint32_t c = 0; c < C; ++c) color[static_cast<size_t>(p) * C + c] = src_color[s * C + c];
+        if (C == 4) color[static_cast<size_t>(p) * C + 3] = 1.0f;
+        if (filled) filled[p] = 1.0f;
+        ngutter += covered ? 0u : 1u;
+        nunobserved += covered ? 1u : 0u;
     }
-    count(counters + 1, gutter);
-    count(counters + 2, unobserved);
-    count(counters + 3, remaining);
-    count(counters + 4, gutter || unobserved);
+    count_total(counters + 1, ngutter);
+    count_total(counters + 2, nunobserved);
+    count_total(counters + 3, nremaining);
+    count_total(counters + 4, ngutter + nunobserved);
 }
 
-uint32_t blocks_of(uint64_t n) { return static_cast<uint32_t>((n + kBlock - 1) / kBlock); }
+uint32_t blocks_of(uint64_t n) {
+    const uint64_t b = (n + kBlock - 1) / kBlock;
+    return static_cast<uint32_t>(b < 2048 ? (b ? b : 1) : 2048);
+}
 
 }  // namespace
 

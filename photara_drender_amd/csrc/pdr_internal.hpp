@@ -132,15 +132,16 @@ struct RefineStepTables {
     const uint32_t* views;  // [steps * batch]
 };
 
+// One view of the refinement, reduced to its pixels with mask > 0 ("active"),
+// in 8x8-tile order.
 struct RefineView {
-    const float* uv;      // [P,2]
-    const float* target;  // [P,3]
-    const float* mask;    // [P]
-    uint32_t pixels;
-    // bilinear taps of the pixels with mask > 0, grouped by texel (ascending):
+    const float4* uvm;  // [active] (u, v, mask, 0)
+    const float4* rgb;  // [active] target (r, g, b, 0)
+    uint32_t active;
+    // bilinear taps of the active pixels, grouped by texel (ascending):
     const uint32_t* texels;       // [ntexels] texels the view samples
     const uint32_t* tap_offsets;  // [ntexels + 1] entry range of each texel
-    const uint32_t* taps;         // [entries] pixel * 4 + tap, ascending within a texel
+    const uint32_t* taps;         // [entries] active pixel * 4 + tap, ascending within a texel
     const float* tap_weights;     // [entries] bilinear weight of the tap
     uint32_t ntexels;
 };
@@ -150,13 +151,24 @@ hipError_t refine_view_inputs(const float* image, uint32_t channels, const float
                               float* target3, float* mask, double* partials, uint32_t npartials, hipStream_t st);
 uint32_t refine_partial_count(uint32_t P);
 hipError_t sum_partials(const double* partials, uint32_t n, double* out, hipStream_t st);
-hipError_t photometric_step(const RefineView* views_dev, uint32_t batch, uint32_t max_pixels, uint32_t max_texels,
+// Per-block loss partials written by the residual pass (photometric_partials of them).
+uint32_t photometric_partials(uint32_t batch, uint32_t max_active);
+hipError_t photometric_step(const RefineView* views_dev, uint32_t batch, uint32_t max_active, uint32_t max_texels,
                             const float* tex, uint32_t tw, uint32_t th, float eps, const RefineStepTables& tables,
-                            const uint32_t* step_counter, float4* resid, float* grad, float* loss, hipStream_t st);
+                            const uint32_t* step_counter, float4* resid, float* grad, float* loss_part,
+                            hipStream_t st);
 // Precompute of the gather form: tap keys / values of one view, weights of the sorted entries, last offset.
-hipError_t photo_taps(const float* uv, const float* mask, uint32_t P, uint32_t tw, uint32_t th, uint32_t* keys,
-                      uint32_t* vals, uint32_t* count, hipStream_t st);
-hipError_t photo_tap_weights(const uint32_t* vals, uint32_t E, const float* uv, uint32_t tw, uint32_t th, float* w,
+// Active pixels of one view in 8x8-tile order: flags over tile-order slots,
+// then (after an exclusive scan of the flags) the packed records.
+uint32_t tile_slots(uint32_t W, uint32_t H);
+hipError_t active_flags(const float* mask, uint32_t W, uint32_t H, uint32_t* flags, hipStream_t st);
+hipError_t active_pack(const float* uv, const float* target, const float* mask, uint32_t W, uint32_t H,
+                       const uint32_t* flags, const uint32_t* pos, float4* uvm, float4* rgb, hipStream_t st);
+hipError_t exclusive_scan(const uint32_t* in, uint32_t* out, uint32_t n, void* temp, size_t temp_bytes, hipStream_t st);
+size_t exclusive_scan_temp_bytes(uint32_t n);
+hipError_t photo_taps(const float4* uvm, uint32_t n, uint32_t tw, uint32_t th, uint32_t* keys, uint32_t* vals,
+                      hipStream_t st);
+hipError_t photo_tap_weights(const uint32_t* vals, uint32_t E, const float4* uvm, uint32_t tw, uint32_t th, float* w,
                              hipStream_t st);
 size_t photo_group_temp_bytes(uint32_t n);
 hipError_t photo_group_taps(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out, uint32_t* vals_out,
@@ -182,6 +194,8 @@ hipError_t adam_tv_step(const float* tex, float* tex_out, float* grad, float* m,
                         const float* bc1_table, const float* bc2_table, float b1, float b2, float eps, float cmin,
                         float cmax, const uint32_t* step_counter, hipStream_t st);
 hipError_t advance_step(uint32_t* step_counter, hipStream_t st);
+// loss[step] += the sum of the photometric partials (fixed order), then step += 1.
+hipError_t finish_step(const float* loss_part, uint32_t n, float* loss, uint32_t* step_counter, hipStream_t st);
 hipError_t tv_gradient(const float* tex, uint32_t tw, uint32_t th, const int32_t* chart, float eps, float scale,
                        float* grad, hipStream_t st);
 hipError_t adam_step(float* tex, float* grad, float* m, float* v, uint32_t n, const float* lr_table,

@@ -120,12 +120,21 @@ pdr_status pdr_refine(pdr_context ctx, const float* initial_texture, const float
         uint32_t* group_texels = bufs.alloc<uint32_t>(ntaps);
         uint32_t* group_runlen = bufs.alloc<uint32_t>(ntaps);
         uint32_t* group_offsets = bufs.alloc<uint32_t>(ntaps + 1);
-        uint32_t* group_counts = bufs.alloc<uint32_t>(2);  // masked pixels, runs
+        uint32_t* group_counts = bufs.alloc<uint32_t>(2);  // [1]: runs
         const size_t group_temp_bytes = pdr::photo_group_temp_bytes(ntaps);
         void* group_temp = bufs.alloc<unsigned char>(group_temp_bytes);
         int key_bits = 1;
         while (key_bits < 32 && (1ull << key_bits) <= static_cast<unsigned long long>(o.width) * o.height) ++key_bits;
-        uint32_t max_texels = 0;
+        uint32_t max_texels = 0, max_active = 0, max_slots = 0;
+        for (uint32_t k = 0; k < nviews; ++k)
+            max_slots = std::max(max_slots, pdr::tile_slots(views[k].width, views[k].height));
+        uint32_t* act_flags = bufs.alloc<uint32_t>(max_slots);
+        uint32_t* act_pos = bufs.alloc<uint32_t>(max_slots);
+        const size_t scan_temp_bytes = pdr::exclusive_scan_temp_bytes(max_slots);
+        void* scan_temp = bufs.alloc<unsigned char>(scan_temp_bytes);
+        float* vuv = bufs.alloc<float>(static_cast<size_t>(max_pixels) * 2);
+        float* target = bufs.alloc<float>(static_cast<size_t>(max_pixels) * 3);
+        float* mask = bufs.alloc<float>(max_pixels);
         for (uint32_t k = 0; k < nviews; ++k) {
             const pdr_refine_view& v = views[k];
             const uint32_t P = v.width * v.height;
@@ -140,9 +149,6 @@ pdr_status pdr_refine(pdr_context ctx, const float* initial_texture, const float
             vt.v[0].raster = raster;
             finalize_table(vt);
             run_raster_forward(ctx, vt, didx, T, V, 0, 0, st);
-            float* vuv = bufs.alloc<float>(static_cast<size_t>(P) * 2);
-            float* target = bufs.alloc<float>(static_cast<size_t>(P) * 3);
-            float* mask = bufs.alloc<float>(P);
             const pdr::PixelTable pt = pixel_table_one(raster, nullptr, nullptr, vuv, nullptr, v.width, v.height);
             hip_check(pdr::interp_forward(pt, duv, V, 2, didx, T, st), "interp_forward");
             const uint32_t np = pdr::refine_partial_count(P);
@@ -153,15 +159,24 @@ pdr_status pdr_refine(pdr_context ctx, const float* initial_texture, const float
             hip_check(hipMemcpyAsync(&w, partials + np, sizeof(double), hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
             sync(st);
             valid_weight[k] = w;
-            hip_check(pdr::fill_u32(group_counts, 0u, 2, st), "fill");
-            hip_check(pdr::photo_taps(vuv, mask, P, o.width, o.height, tap_keys, tap_vals, group_counts, st),
-                      "photo_taps");
-            uint32_t masked = 0;
-            hip_check(hipMemcpyAsync(&masked, group_counts, sizeof(masked), hipMemcpyDeviceToHost, st),
-                      "hipMemcpyAsync");
+            // the pixels with mask > 0, packed in 8x8-tile order
+            const uint32_t slots = pdr::tile_slots(v.width, v.height);
+            hip_check(pdr::active_flags(mask, v.width, v.height, act_flags, st), "active_flags");
+            hip_check(pdr::exclusive_scan(act_flags, act_pos, slots, scan_temp, scan_temp_bytes, st),
+                      "exclusive_scan");
+            uint32_t last[2] = {0, 0};
+            hip_check(hipMemcpyAsync(&last[0], act_flags + slots - 1, 4, hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
+            hip_check(hipMemcpyAsync(&last[1], act_pos + slots - 1, 4, hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
             sync(st);
-            const uint32_t E = masked * 4u;
-            hip_check(pdr::photo_group_taps(tap_keys, tap_vals, tap_keys_sorted, tap_vals_sorted, P * 4u, key_bits, E,
+            const uint32_t active = last[0] + last[1];
+            float4* uvm = bufs.alloc<float4>(active);
+            float4* rgb = bufs.alloc<float4>(active);
+            hip_check(pdr::active_pack(vuv, target, mask, v.width, v.height, act_flags, act_pos, uvm, rgb, st),
+                      "active_pack");
+            // their bilinear taps grouped by texel
+            const uint32_t E = active * 4u;
+            hip_check(pdr::photo_taps(uvm, active, o.width, o.height, tap_keys, tap_vals, st), "photo_taps");
+            hip_check(pdr::photo_group_taps(tap_keys, tap_vals, tap_keys_sorted, tap_vals_sorted, E, key_bits, E,
                                             group_texels, group_runlen, group_offsets, group_counts + 1, group_temp,
                                             group_temp_bytes, st),
                       "photo_group_taps");
@@ -179,11 +194,12 @@ pdr_status pdr_refine(pdr_context ctx, const float* initial_texture, const float
                 copy_any(view_texels, group_texels, static_cast<size_t>(runs) * 4, st);
                 copy_any(view_offsets, group_offsets, (static_cast<size_t>(runs) + 1) * 4, st);
                 copy_any(taps, tap_vals_sorted, static_cast<size_t>(E) * 4, st);
-                hip_check(pdr::photo_tap_weights(taps, E, vuv, o.width, o.height, tap_w, st), "photo_tap_weights");
+                hip_check(pdr::photo_tap_weights(taps, E, uvm, o.width, o.height, tap_w, st), "photo_tap_weights");
             }
             max_texels = std::max(max_texels, runs);
-            rviews[k] = pdr::RefineView{vuv, target, mask, P, view_texels, view_offsets, taps, tap_w, runs};
-            sync(st);  // the grouping scratch is reused by the next view
+            max_active = std::max(max_active, active);
+            rviews[k] = pdr::RefineView{uvm, rgb, active, view_texels, view_offsets, taps, tap_w, runs};
+            sync(st);  // the per-view scratch is reused by the next view
         }
         pdr::RefineView* dviews = bufs.alloc<pdr::RefineView>(nviews);
         hip_check(hipMemcpyAsync(dviews, rviews.data(), nviews * sizeof(pdr::RefineView), hipMemcpyHostToDevice, st),
@@ -199,7 +215,9 @@ pdr_status pdr_refine(pdr_context ctx, const float* initial_texture, const float
         float* m1 = bufs.alloc<float>(n);
         float* m2 = bufs.alloc<float>(n);
         float* loss = bufs.alloc<float>(o.steps + seam_steps);
-        float4* resid = bufs.alloc<float4>(static_cast<size_t>(o.batch_size) * max_pixels);
+        float4* resid = bufs.alloc<float4>(static_cast<size_t>(o.batch_size) * max_active);
+        const uint32_t nparts = pdr::photometric_partials(o.batch_size, max_active);
+        float* loss_part = bufs.alloc<float>(nparts);
         float4* seam_resid = bufs.alloc<float4>(npairs);
         uint32_t* counters = bufs.alloc<uint32_t>(4);  // step, done
         copy_any(tex, initial_texture, static_cast<size_t>(n) * 4, st);
@@ -291,8 +309,8 @@ pdr_status pdr_refine(pdr_context ctx, const float* initial_texture, const float
         float* texbuf[2] = {tex, fused_tv ? bufs.alloc<float>(n) : tex};
         const auto t2 = std::chrono::steady_clock::now();
         auto one_step = [&](hipStream_t s, uint32_t c) {
-            hip_check(pdr::photometric_step(dviews, o.batch_size, max_pixels, max_texels, texbuf[c], o.width,
-                                            o.height, o.photometric_epsilon, tables, counters, resid, grad, loss, s),
+            hip_check(pdr::photometric_step(dviews, o.batch_size, max_active, max_texels, texbuf[c], o.width, o.height,
+                                            o.photometric_epsilon, tables, counters, resid, grad, loss_part, s),
                       "photometric_step");
             if (o.joint_seam_weight != 0.0f && npairs)
                 hip_check(pdr::seam_step(texbuf[c], o.width, o.height, dpairs, npairs, o.seam_epsilon, seam_norm,
@@ -307,7 +325,7 @@ pdr_status pdr_refine(pdr_context ctx, const float* initial_texture, const float
                 hip_check(pdr::adam_step(texbuf[c], grad, m1, m2, n, d_lr, d_bc1, d_bc2, o.adam_beta1, o.adam_beta2,
                                          o.adam_epsilon, o.clamp_min, o.clamp_max, counters, s),
                           "adam_step");
-            hip_check(pdr::advance_step(counters, s), "advance_step");
+            hip_check(pdr::finish_step(loss_part, nparts, loss, counters, s), "finish_step");
             return fused_tv ? c ^ 1u : c;
         };
         const uint32_t block = o.checkpoint_every ? o.checkpoint_every : std::min<uint32_t>(o.steps, 50);

@@ -5,7 +5,9 @@
 //                     (texture_refiner.cpp:228-242)
 //  k_photo_residual   atlas_photometric_gradient.hlsl:45-108 for all views of a
 //                     step in one launch, up to the per-pixel output gradient
-//                     d / r * m * norm (and the loss); per-step scalars come
+//                     d / r * m * norm (and the loss), over each view's pixels
+//                     with mask > 0 packed in 8x8-tile order at precompute
+//                     (k_active_flags, a scan, k_active_pack); per-step scalars come
 //                     from device tables indexed by a device step counter so
 //                     the step loop can be captured once and replayed
 //  k_photo_gather     the texel gradient as a gather: each view's bilinear taps
@@ -24,8 +26,8 @@
 //  k_advance_step     moves the device step counter on after a step
 //
 // No gradient uses float atomics (the reference emulates them with
-// compare-exchange loops); per-64-pixel loss partials use the shaders' tree
-// order exactly.
+// compare-exchange loops); loss partials are summed per 64-lane wave with the
+// shaders' tree order (over active pixels for the photometric term).
 #include <hipcub/hipcub.hpp>
 
 #include "pdr_internal.hpp"
@@ -122,36 +124,43 @@ __global__ __launch_bounds__(kBlock) void k_photo_residual(const RefineView* __r
                                                            const float* __restrict__ tex, uint32_t tw, uint32_t th,
                                                            float eps, RefineStepTables tables,
                                                            const uint32_t* __restrict__ step_counter,
-                                                           float4* __restrict__ resid, uint32_t max_pixels,
-                                                           float* __restrict__ loss) {
+                                                           float4* __restrict__ resid, uint32_t max_active,
+                                                           float* __restrict__ loss_part) {
+    __shared__ float wsum[kBlock / 64];
     const uint32_t step = *step_counter;
     const uint32_t b = blockIdx.y;
     const RefineView& rv = views[tables.views[static_cast<size_t>(step) * batch + b]];
     const float norm = tables.norm[step];
-    const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
-    if (blockIdx.x * kBlock >= rv.pixels) return;  // whole block past this view (uniform)
+    const uint32_t a = blockIdx.x * kBlock + threadIdx.x;
+    float* part = loss_part + static_cast<size_t>(b) * gridDim.x + blockIdx.x;
+    if (blockIdx.x * kBlock >= rv.active) {  // whole block past this view (uniform)
+        if (threadIdx.x == 0) *part = 0.0f;
+        return;
+    }
     float local = 0.0f;
-    if (p < rv.pixels) {
-        const float m = rv.mask[p];
-        if (m > 0.0f) {
-            const float2 uv = reinterpret_cast<const float2*>(rv.uv)[p];
-            const AtlasTap a = atlas_tap(uv.x, uv.y, tw, th);
-            float og[3];
+    if (a < rv.active) {
+        const float4 q = rv.uvm[a];
+        const float4 target = rv.rgb[a];
+        const float m = q.z;
+        const AtlasTap at = atlas_tap(q.x, q.y, tw, th);
+        const float tgt[3] = {target.x, target.y, target.z};
+        float og[3];
 #pragma unroll
-            for (int ch = 0; ch < 3; ++ch) {
-                float rendered = 0.0f;
+        for (int ch = 0; ch < 3; ++ch) {
+            float rendered = 0.0f;
 #pragma unroll
-                for (int k = 0; k < 4; ++k) rendered += tex[static_cast<size_t>(a.t[k]) * 3 + ch] * a.w[k];
-                const float d = rendered - rv.target[static_cast<size_t>(p) * 3 + ch];
-                const float r = sqrtf(d * d + eps * eps);
-                og[ch] = d / r * m * norm;
-                local += r * m * norm;
-            }
-            resid[static_cast<size_t>(b) * max_pixels + p] = make_float4(og[0], og[1], og[2], 0.0f);
+            for (int k = 0; k < 4; ++k) rendered += tex[static_cast<size_t>(at.t[k]) * 3 + ch] * at.w[k];
+            const float d = rendered - tgt[ch];
+            const float r = sqrtf(d * d + eps * eps);
+            og[ch] = d / r * m * norm;
+            local += r * m * norm;
         }
+        resid[static_cast<size_t>(b) * max_active + a] = make_float4(og[0], og[1], og[2], 0.0f);
     }
     const float s = wave_tree_sum(local);
-    if ((threadIdx.x & 63u) == 0 && s != 0.0f) unsafeAtomicAdd(&loss[step], s);
+    if ((threadIdx.x & 63u) == 0) wsum[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) *part = (wsum[0] + wsum[1]) + (wsum[2] + wsum[3]);
 }
 
 // Texel gradient of batch slot b: one lane per texel the slot's view samples.
@@ -178,41 +187,60 @@ __global__ __launch_bounds__(kBlock) void k_photo_gather(const RefineView* __res
     grad[t + 2] += a2;
 }
 
-// Tap entries of one view: for every pixel with mask > 0 its 4 bilinear taps
-// (key = texel, value = pixel * 4 + tap); other pixels get key tw * th, which
-// sorts past every texel.  count += pixels with mask > 0.
-__global__ void k_photo_taps(const float* __restrict__ uv, const float* __restrict__ mask, uint32_t P, uint32_t tw,
-                             uint32_t th, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
-                             uint32_t* __restrict__ count) {
-    const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
-    bool on = false;
-    if (p < P) {
-        on = mask[p] > 0.0f;
-        AtlasTap a{};
-        if (on) {
-            const float2 c = reinterpret_cast<const float2*>(uv)[p];
-            a = atlas_tap(c.x, c.y, tw, th);
-        }
+// Tile-order slot q of a W x H view: tile q / 64 (row-major over 8x8 tiles),
+// pixel (q % 64) row-major inside the tile.
+__device__ __forceinline__ bool tile_pixel(uint32_t q, uint32_t W, uint32_t H, uint32_t& p) {
+    const uint32_t tiles_x = (W + 7u) / 8u, tile = q >> 6, lane = q & 63u;
+    const uint32_t x = (tile % tiles_x) * 8u + (lane & 7u), y = (tile / tiles_x) * 8u + (lane >> 3);
+    p = y * W + x;
+    return x < W && y < H;
+}
+
+__global__ void k_active_flags(const float* __restrict__ mask, uint32_t W, uint32_t H, uint32_t n,
+                               uint32_t* __restrict__ flags) {
+    const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
+    if (q >= n) return;
+    uint32_t p = 0;
+    flags[q] = tile_pixel(q, W, H, p) && mask[p] > 0.0f ? 1u : 0u;
+}
+
+__global__ void k_active_pack(const float* __restrict__ uv, const float* __restrict__ target,
+                              const float* __restrict__ mask, uint32_t W, uint32_t H, uint32_t n,
+                              const uint32_t* __restrict__ flags, const uint32_t* __restrict__ pos,
+                              float4* __restrict__ uvm, float4* __restrict__ rgb) {
+    const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
+    if (q >= n || !flags[q]) return;
+    uint32_t p = 0;
+    tile_pixel(q, W, H, p);
+    const uint32_t a = pos[q];
+    uvm[a] = make_float4(uv[static_cast<size_t>(p) * 2], uv[static_cast<size_t>(p) * 2 + 1], mask[p], 0.0f);
+    rgb[a] = make_float4(target[static_cast<size_t>(p) * 3], target[static_cast<size_t>(p) * 3 + 1],
+                         target[static_cast<size_t>(p) * 3 + 2], 0.0f);
+}
+
+// Tap entries of one view's active pixels: key = texel, value = active pixel * 4 + tap.
+__global__ void k_photo_taps(const float4* __restrict__ uvm, uint32_t n, uint32_t tw, uint32_t th,
+                             uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    const uint32_t a = blockIdx.x * kBlock + threadIdx.x;
+    if (a >= n) return;
+    const float4 q = uvm[a];
+    const AtlasTap at = atlas_tap(q.x, q.y, tw, th);
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            keys[static_cast<size_t>(p) * 4 + k] = on ? a.t[k] : tw * th;
-            vals[static_cast<size_t>(p) * 4 + k] = p * 4u + static_cast<uint32_t>(k);
-        }
+    for (int k = 0; k < 4; ++k) {
+        keys[static_cast<size_t>(a) * 4 + k] = at.t[k];
+        vals[static_cast<size_t>(a) * 4 + k] = a * 4u + static_cast<uint32_t>(k);
     }
-    const unsigned long long mk = __ballot(on);
-    if (mk && (threadIdx.x & 63u) == static_cast<uint32_t>(__ffsll(static_cast<long long>(mk)) - 1))
-        atomicAdd(count, static_cast<uint32_t>(__popcll(mk)));
 }
 
 // Bilinear weight of every sorted tap entry (the same expression as the scatter's).
-__global__ void k_photo_tap_weights(const uint32_t* __restrict__ vals, uint32_t E, const float* __restrict__ uv,
+__global__ void k_photo_tap_weights(const uint32_t* __restrict__ vals, uint32_t E, const float4* __restrict__ uvm,
                                     uint32_t tw, uint32_t th, float* __restrict__ w) {
     const uint32_t e = blockIdx.x * kBlock + threadIdx.x;
     if (e >= E) return;
     const uint32_t v = vals[e];
-    const float2 c = reinterpret_cast<const float2*>(uv)[v >> 2];
-    const AtlasTap a = atlas_tap(c.x, c.y, tw, th);
-    w[e] = a.w[v & 3u];
+    const float4 q = uvm[v >> 2];
+    const AtlasTap at = atlas_tap(q.x, q.y, tw, th);
+    w[e] = at.w[v & 3u];
 }
 
 // offsets[U] = E (the run starts come from an exclusive scan of the run lengths).
@@ -405,6 +433,27 @@ __global__ void k_advance_step(uint32_t* step_counter) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *step_counter += 1u;
 }
 
+// The step's photometric loss from the residual pass's block partials (fixed
+// order: contiguous chunks per lane, then the lanes in order), then the step
+// counter moves on.  One block of kBlock lanes.
+__global__ __launch_bounds__(kBlock) void k_finish_step(const float* __restrict__ loss_part, uint32_t n,
+                                                        float* __restrict__ loss, uint32_t* step_counter) {
+    __shared__ float chunk[kBlock];
+    const uint32_t per = (n + kBlock - 1) / kBlock;
+    const uint32_t b = threadIdx.x * per, e = b + per < n ? b + per : n;
+    float s = 0.0f;
+    for (uint32_t i = b; i < e; ++i) s += loss_part[i];
+    chunk[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float t = 0.0f;
+        for (uint32_t i = 0; i < kBlock; ++i) t += chunk[i];
+        const uint32_t step = *step_counter;
+        loss[step] += t;
+        *step_counter = step + 1u;
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void k_adam(float* __restrict__ tex, float* __restrict__ grad,
                                                  float* __restrict__ mom1, float* __restrict__ mom2, uint32_t n,
                                                  const float* __restrict__ lr_t, const float* __restrict__ bc1_t,
@@ -457,29 +506,64 @@ hipError_t sum_partials(const double* partials, uint32_t n, double* out, hipStre
     return hipGetLastError();
 }
 
-hipError_t photometric_step(const RefineView* views_dev, uint32_t batch, uint32_t max_pixels, uint32_t max_texels,
+uint32_t photometric_partials(uint32_t batch, uint32_t max_active) {
+    const uint32_t blocks = (max_active + kBlock - 1) / kBlock;
+    return (blocks ? blocks : 1) * batch;
+}
+
+hipError_t photometric_step(const RefineView* views_dev, uint32_t batch, uint32_t max_active, uint32_t max_texels,
                             const float* tex, uint32_t tw, uint32_t th, float eps, const RefineStepTables& tables,
-                            const uint32_t* step_counter, float4* resid, float* grad, float* loss, hipStream_t st) {
-    const dim3 grid((max_pixels + kBlock - 1) / kBlock, batch);
+                            const uint32_t* step_counter, float4* resid, float* grad, float* loss_part,
+                            hipStream_t st) {
+    const uint32_t blocks = (max_active + kBlock - 1) / kBlock;
+    const dim3 grid(blocks ? blocks : 1, batch);
     k_photo_residual<<<grid, kBlock, 0, st>>>(views_dev, batch, tex, tw, th, eps, tables, step_counter, resid,
-                                              max_pixels, loss);
+                                              max_active, loss_part);
     const uint32_t gblocks = (max_texels + kBlock - 1) / kBlock;
     for (uint32_t b = 0; b < batch; ++b)
         k_photo_gather<<<gblocks ? gblocks : 1, kBlock, 0, st>>>(views_dev, batch, b, tables, step_counter,
-                                                                 resid + static_cast<size_t>(b) * max_pixels, grad);
+                                                                 resid + static_cast<size_t>(b) * max_active, grad);
     return hipGetLastError();
 }
 
-hipError_t photo_taps(const float* uv, const float* mask, uint32_t P, uint32_t tw, uint32_t th, uint32_t* keys,
-                      uint32_t* vals, uint32_t* count, hipStream_t st) {
-    k_photo_taps<<<(P + kBlock - 1) / kBlock, kBlock, 0, st>>>(uv, mask, P, tw, th, keys, vals, count);
+uint32_t tile_slots(uint32_t W, uint32_t H) { return ((W + 7u) / 8u) * ((H + 7u) / 8u) * 64u; }
+
+hipError_t active_flags(const float* mask, uint32_t W, uint32_t H, uint32_t* flags, hipStream_t st) {
+    const uint32_t n = tile_slots(W, H);
+    k_active_flags<<<(n + kBlock - 1) / kBlock, kBlock, 0, st>>>(mask, W, H, n, flags);
     return hipGetLastError();
 }
 
-hipError_t photo_tap_weights(const uint32_t* vals, uint32_t E, const float* uv, uint32_t tw, uint32_t th, float* w,
+hipError_t active_pack(const float* uv, const float* target, const float* mask, uint32_t W, uint32_t H,
+                       const uint32_t* flags, const uint32_t* pos, float4* uvm, float4* rgb, hipStream_t st) {
+    const uint32_t n = tile_slots(W, H);
+    k_active_pack<<<(n + kBlock - 1) / kBlock, kBlock, 0, st>>>(uv, target, mask, W, H, n, flags, pos, uvm, rgb);
+    return hipGetLastError();
+}
+
+size_t exclusive_scan_temp_bytes(uint32_t n) {
+    size_t a = 0;
+    (void)hipcub::DeviceScan::ExclusiveSum(nullptr, a, static_cast<const uint32_t*>(nullptr),
+                                           static_cast<uint32_t*>(nullptr), static_cast<int>(n));
+    return a;
+}
+
+hipError_t exclusive_scan(const uint32_t* in, uint32_t* out, uint32_t n, void* temp, size_t temp_bytes, hipStream_t st) {
+    size_t bytes = temp_bytes;
+    return hipcub::DeviceScan::ExclusiveSum(temp, bytes, in, out, static_cast<int>(n), st);
+}
+
+hipError_t photo_taps(const float4* uvm, uint32_t n, uint32_t tw, uint32_t th, uint32_t* keys, uint32_t* vals,
+                      hipStream_t st) {
+    if (!n) return hipSuccess;
+    k_photo_taps<<<(n + kBlock - 1) / kBlock, kBlock, 0, st>>>(uvm, n, tw, th, keys, vals);
+    return hipGetLastError();
+}
+
+hipError_t photo_tap_weights(const uint32_t* vals, uint32_t E, const float4* uvm, uint32_t tw, uint32_t th, float* w,
                              hipStream_t st) {
     if (!E) return hipSuccess;
-    k_photo_tap_weights<<<(E + kBlock - 1) / kBlock, kBlock, 0, st>>>(vals, E, uv, tw, th, w);
+    k_photo_tap_weights<<<(E + kBlock - 1) / kBlock, kBlock, 0, st>>>(vals, E, uvm, tw, th, w);
     return hipGetLastError();
 }
 
@@ -556,6 +640,11 @@ hipError_t adam_tv_step(const float* tex, float* tex_out, float* grad, float* m,
 hipError_t tv_gradient(const float* tex, uint32_t tw, uint32_t th, const int32_t* chart, float eps, float scale,
                        float* grad, hipStream_t st) {
     k_tv_gradient<<<(tw * th + kBlock - 1) / kBlock, kBlock, 0, st>>>(tex, tw, th, chart, eps, scale, grad);
+    return hipGetLastError();
+}
+
+hipError_t finish_step(const float* loss_part, uint32_t n, float* loss, uint32_t* step_counter, hipStream_t st) {
+    k_finish_step<<<1, kBlock, 0, st>>>(loss_part, n, loss, step_counter);
     return hipGetLastError();
 }
 
