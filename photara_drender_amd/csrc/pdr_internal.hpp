@@ -135,15 +135,15 @@ struct RefineStepTables {
 // One view of the refinement, reduced to its pixels with mask > 0 ("active"),
 // in 8x8-tile order.
 struct RefineView {
-    const float4* uvm;  // [active] (u, v, mask, 0)
-    const float4* rgb;  // [active] target (r, g, b, 0)
-    uint32_t active;
-    // bilinear taps of the active pixels, grouped by texel (ascending):
-    const uint32_t* texels;       // [ntexels] texels the view samples
-    const uint32_t* tap_offsets;  // [ntexels + 1] entry range of each texel
+    const float4* uvm;       // [active] (u, v, mask, 0)
+    const float4* rgb;       // [active] target (r, g, b, 0)
+    const uint32_t* counts;  // device: [0] active pixels, [1] texel runs (the last may be the past-the-atlas run)
+    // bilinear taps of the active pixels, grouped by texel (ascending; taps of
+    // the unused pixel slots form a last run keyed past the atlas):
+    const uint32_t* texels;       // [runs] texels the view samples
+    const uint32_t* tap_offsets;  // [runs + 1] entry range of each texel
     const uint32_t* taps;         // [entries] active pixel * 4 + tap, ascending within a texel
     const float* tap_weights;     // [entries] bilinear weight of the tap
-    uint32_t ntexels;
 };
 
 hipError_t transform_positions(const float* pos3, uint32_t V, const float* m16_host, float* out4, hipStream_t st);
@@ -166,8 +166,12 @@ hipError_t active_pack(const float* uv, const float* target, const float* mask, 
                        const uint32_t* flags, const uint32_t* pos, float4* uvm, float4* rgb, hipStream_t st);
 hipError_t exclusive_scan(const uint32_t* in, uint32_t* out, uint32_t n, void* temp, size_t temp_bytes, hipStream_t st);
 size_t exclusive_scan_temp_bytes(uint32_t n);
-hipError_t photo_taps(const float4* uvm, uint32_t n, uint32_t tw, uint32_t th, uint32_t* keys, uint32_t* vals,
-                      hipStream_t st);
+hipError_t active_count(const uint32_t* flags, const uint32_t* pos, uint32_t n, uint32_t* count, hipStream_t st);
+hipError_t photo_taps(const float4* uvm, const uint32_t* count, uint32_t slots, uint32_t tw, uint32_t th,
+                      uint32_t* keys, uint32_t* vals, hipStream_t st);
+// norm[s] = 1 / (3 * sum of the step's views' mask sums) (0 when that sum is 0)
+hipError_t norm_table(const double* view_weight, const uint32_t* view_index, uint32_t steps, uint32_t batch,
+                      float* norm, hipStream_t st);
 hipError_t photo_tap_weights(const uint32_t* vals, uint32_t E, const float4* uvm, uint32_t tw, uint32_t th, float* w,
                              hipStream_t st);
 size_t photo_group_temp_bytes(uint32_t n);

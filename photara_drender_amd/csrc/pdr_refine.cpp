@@ -109,23 +109,22 @@ pdr_status pdr_refine(pdr_context ctx, const float* initial_texture, const float
         for (uint32_t k = 0; k < nviews; ++k) max_pixels = std::max(max_pixels, views[k].width * views[k].height);
         float* raster = bufs.alloc<float>(static_cast<size_t>(max_pixels) * 4);
         double* partials = bufs.alloc<double>(pdr::refine_partial_count(max_pixels) + 1);
+        double* view_weight = bufs.alloc<double>(nviews);
+        uint32_t* view_counts = bufs.alloc<uint32_t>(static_cast<size_t>(nviews) * 2);  // active pixels, texel runs
         std::vector<pdr::RefineView> rviews(nviews);
-        std::vector<double> valid_weight(nviews);
-        // gather form of the photometric gradient: per view, the bilinear taps grouped by texel
+        // gather form of the photometric gradient: per view, the bilinear taps of
+        // its pixels with mask > 0 grouped by texel.  Per-view arrays are sized for
+        // every pixel, so the loop needs no host readback.
         const uint32_t ntaps = max_pixels * 4u;
         uint32_t* tap_keys = bufs.alloc<uint32_t>(ntaps);
         uint32_t* tap_vals = bufs.alloc<uint32_t>(ntaps);
         uint32_t* tap_keys_sorted = bufs.alloc<uint32_t>(ntaps);
-        uint32_t* tap_vals_sorted = bufs.alloc<uint32_t>(ntaps);
-        uint32_t* group_texels = bufs.alloc<uint32_t>(ntaps);
         uint32_t* group_runlen = bufs.alloc<uint32_t>(ntaps);
-        uint32_t* group_offsets = bufs.alloc<uint32_t>(ntaps + 1);
-        uint32_t* group_counts = bufs.alloc<uint32_t>(2);  // [1]: runs
         const size_t group_temp_bytes = pdr::photo_group_temp_bytes(ntaps);
         void* group_temp = bufs.alloc<unsigned char>(group_temp_bytes);
         int key_bits = 1;
         while (key_bits < 32 && (1ull << key_bits) <= static_cast<unsigned long long>(o.width) * o.height) ++key_bits;
-        uint32_t max_texels = 0, max_active = 0, max_slots = 0;
+        uint32_t max_slots = 0;
         for (uint32_t k = 0; k < nviews; ++k)
             max_slots = std::max(max_slots, pdr::tile_slots(views[k].width, views[k].height));
         uint32_t* act_flags = bufs.alloc<uint32_t>(max_slots);
@@ -135,6 +134,21 @@ pdr_status pdr_refine(pdr_context ctx, const float* initial_texture, const float
         float* vuv = bufs.alloc<float>(static_cast<size_t>(max_pixels) * 2);
         float* target = bufs.alloc<float>(static_cast<size_t>(max_pixels) * 3);
         float* mask = bufs.alloc<float>(max_pixels);
+        // every view's packed pixels and tap lists in one allocation (no allocation inside the loop)
+        auto up = [](size_t b) { return (b + 255) & ~static_cast<size_t>(255); };
+        auto view_bytes = [&](uint32_t P) {
+            const size_t E = static_cast<size_t>(P) * 4;
+            return 2 * up(P * sizeof(float4)) + up(E * 4) + up((E + 1) * 4) + up(E * 4) + up(E * 4);
+        };
+        size_t total_bytes = 0;
+        for (uint32_t k = 0; k < nviews; ++k) total_bytes += view_bytes(views[k].width * views[k].height);
+        unsigned char* view_arena = bufs.alloc<unsigned char>(total_bytes);
+        size_t arena_off = 0;
+        auto carve = [&](size_t bytes) {
+            void* p = view_arena + arena_off;
+            arena_off += up(bytes);
+            return p;
+        };
         for (uint32_t k = 0; k < nviews; ++k) {
             const pdr_refine_view& v = views[k];
             const uint32_t P = v.width * v.height;
@@ -154,52 +168,39 @@ pdr_status pdr_refine(pdr_context ctx, const float* initial_texture, const float
             const uint32_t np = pdr::refine_partial_count(P);
             hip_check(pdr::refine_view_inputs(img, v.channels, vis, raster, P, target, mask, partials, np, st),
                       "refine_view_inputs");
-            hip_check(pdr::sum_partials(partials, np, partials + np, st), "sum_partials");
-            double w = 0.0;
-            hip_check(hipMemcpyAsync(&w, partials + np, sizeof(double), hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
-            sync(st);
-            valid_weight[k] = w;
+            hip_check(pdr::sum_partials(partials, np, view_weight + k, st), "sum_partials");
             // the pixels with mask > 0, packed in 8x8-tile order
+            uint32_t* counts = view_counts + static_cast<size_t>(k) * 2;
             const uint32_t slots = pdr::tile_slots(v.width, v.height);
             hip_check(pdr::active_flags(mask, v.width, v.height, act_flags, st), "active_flags");
             hip_check(pdr::exclusive_scan(act_flags, act_pos, slots, scan_temp, scan_temp_bytes, st),
                       "exclusive_scan");
-            uint32_t last[2] = {0, 0};
-            hip_check(hipMemcpyAsync(&last[0], act_flags + slots - 1, 4, hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
-            hip_check(hipMemcpyAsync(&last[1], act_pos + slots - 1, 4, hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
-            sync(st);
-            const uint32_t active = last[0] + last[1];
-            float4* uvm = bufs.alloc<float4>(active);
-            float4* rgb = bufs.alloc<float4>(active);
+            hip_check(pdr::active_count(act_flags, act_pos, slots, counts, st), "active_count");
+            float4* uvm = static_cast<float4*>(carve(static_cast<size_t>(P) * sizeof(float4)));
+            float4* rgb = static_cast<float4*>(carve(static_cast<size_t>(P) * sizeof(float4)));
             hip_check(pdr::active_pack(vuv, target, mask, v.width, v.height, act_flags, act_pos, uvm, rgb, st),
                       "active_pack");
             // their bilinear taps grouped by texel
-            const uint32_t E = active * 4u;
-            hip_check(pdr::photo_taps(uvm, active, o.width, o.height, tap_keys, tap_vals, st), "photo_taps");
-            hip_check(pdr::photo_group_taps(tap_keys, tap_vals, tap_keys_sorted, tap_vals_sorted, E, key_bits, E,
-                                            group_texels, group_runlen, group_offsets, group_counts + 1, group_temp,
-                                            group_temp_bytes, st),
+            const uint32_t E = P * 4u;
+            uint32_t* view_texels = static_cast<uint32_t*>(carve(static_cast<size_t>(E) * 4));
+            uint32_t* view_offsets = static_cast<uint32_t*>(carve((static_cast<size_t>(E) + 1) * 4));
+            uint32_t* taps = static_cast<uint32_t*>(carve(static_cast<size_t>(E) * 4));
+            float* tap_w = static_cast<float*>(carve(static_cast<size_t>(E) * 4));
+            hip_check(pdr::photo_taps(uvm, counts, P, o.width, o.height, tap_keys, tap_vals, st), "photo_taps");
+            hip_check(pdr::photo_group_taps(tap_keys, tap_vals, tap_keys_sorted, taps, E, key_bits, E, view_texels,
+                                            group_runlen, view_offsets, counts + 1, group_temp, group_temp_bytes, st),
                       "photo_group_taps");
-            uint32_t runs = 0;
-            if (E) {
-                hip_check(hipMemcpyAsync(&runs, group_counts + 1, sizeof(runs), hipMemcpyDeviceToHost, st),
-                          "hipMemcpyAsync");
-                sync(st);
-            }
-            uint32_t* view_texels = bufs.alloc<uint32_t>(runs);
-            uint32_t* view_offsets = bufs.alloc<uint32_t>(static_cast<size_t>(runs) + 1);
-            uint32_t* taps = bufs.alloc<uint32_t>(E);
-            float* tap_w = bufs.alloc<float>(E);
-            if (E) {
-                copy_any(view_texels, group_texels, static_cast<size_t>(runs) * 4, st);
-                copy_any(view_offsets, group_offsets, (static_cast<size_t>(runs) + 1) * 4, st);
-                copy_any(taps, tap_vals_sorted, static_cast<size_t>(E) * 4, st);
-                hip_check(pdr::photo_tap_weights(taps, E, uvm, o.width, o.height, tap_w, st), "photo_tap_weights");
-            }
-            max_texels = std::max(max_texels, runs);
-            max_active = std::max(max_active, active);
-            rviews[k] = pdr::RefineView{uvm, rgb, active, view_texels, view_offsets, taps, tap_w, runs};
-            sync(st);  // the per-view scratch is reused by the next view
+            hip_check(pdr::photo_tap_weights(taps, E, uvm, o.width, o.height, tap_w, st), "photo_tap_weights");
+            rviews[k] = pdr::RefineView{uvm, rgb, counts, view_texels, view_offsets, taps, tap_w};
+        }
+        std::vector<uint32_t> host_counts(static_cast<size_t>(nviews) * 2);
+        hip_check(hipMemcpyAsync(host_counts.data(), view_counts, host_counts.size() * 4, hipMemcpyDeviceToHost, st),
+                  "hipMemcpyAsync");
+        sync(st);
+        uint32_t max_active = 0, max_texels = 0;
+        for (uint32_t k = 0; k < nviews; ++k) {
+            max_active = std::max(max_active, host_counts[k * 2]);
+            max_texels = std::max(max_texels, host_counts[k * 2 + 1]);
         }
         pdr::RefineView* dviews = bufs.alloc<pdr::RefineView>(nviews);
         hip_check(hipMemcpyAsync(dviews, rviews.data(), nviews * sizeof(pdr::RefineView), hipMemcpyHostToDevice, st),
@@ -254,18 +255,15 @@ pdr_status pdr_refine(pdr_context ctx, const float* initial_texture, const float
             seam_groups = pdr::TapGroups{texels_tmp, offsets, vals_sorted, weights, runs};
         }
 
-        // src/texture_refiner.cpp:315-356
-        std::vector<float> lr(o.steps), bc1(o.steps), bc2(o.steps), norm(o.steps);
+        // src/texture_refiner.cpp:315-356 (the normalisation from the mask sums on the device)
+        std::vector<float> lr(o.steps), bc1(o.steps), bc2(o.steps);
         std::vector<uint32_t> vidx(static_cast<size_t>(o.steps) * o.batch_size);
         const float pi = std::acos(-1.0F);
         for (uint32_t s = 0; s < o.steps; ++s) {
-            double bw = 0.0;
             for (uint32_t b = 0; b < o.batch_size; ++b) {
                 const size_t vi = (static_cast<size_t>(s) * o.batch_size + b) % nviews;
-                bw += valid_weight[vi];
                 vidx[static_cast<size_t>(s) * o.batch_size + b] = static_cast<uint32_t>(vi);
             }
-            norm[s] = bw > 0.0 ? static_cast<float>(1.0 / (bw * 3.0)) : 0.0F;
             const float phase = static_cast<float>(s) / static_cast<float>(o.steps);
             lr[s] = o.minimum_learning_rate +
                     0.5F * (o.learning_rate - o.minimum_learning_rate) * (1.0F + std::cos(pi * phase));
@@ -280,8 +278,8 @@ pdr_status pdr_refine(pdr_context ctx, const float* initial_texture, const float
         copy_any(d_lr, lr.data(), lr.size() * 4, st);
         copy_any(d_bc1, bc1.data(), bc1.size() * 4, st);
         copy_any(d_bc2, bc2.data(), bc2.size() * 4, st);
-        copy_any(d_norm, norm.data(), norm.size() * 4, st);
         copy_any(d_vidx, vidx.data(), vidx.size() * 4, st);
+        hip_check(pdr::norm_table(view_weight, d_vidx, o.steps, o.batch_size, d_norm, st), "norm_table");
         pdr::RefineStepTables tables{d_lr, d_bc1, d_bc2, d_norm, d_vidx};
 
         // extended objective (python/photara_drender/atlas.py:84-144)

@@ -132,13 +132,14 @@ __global__ __launch_bounds__(kBlock) void k_photo_residual(const RefineView* __r
     const RefineView& rv = views[tables.views[static_cast<size_t>(step) * batch + b]];
     const float norm = tables.norm[step];
     const uint32_t a = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t active = rv.counts[0];
     float* part = loss_part + static_cast<size_t>(b) * gridDim.x + blockIdx.x;
-    if (blockIdx.x * kBlock >= rv.active) {  // whole block past this view (uniform)
+    if (blockIdx.x * kBlock >= active) {  // whole block past this view (uniform)
         if (threadIdx.x == 0) *part = 0.0f;
         return;
     }
     float local = 0.0f;
-    if (a < rv.active) {
+    if (a < active) {
         const float4 q = rv.uvm[a];
         const float4 target = rv.rgb[a];
         const float m = q.z;
@@ -165,13 +166,15 @@ __global__ __launch_bounds__(kBlock) void k_photo_residual(const RefineView* __r
 
 // Texel gradient of batch slot b: one lane per texel the slot's view samples.
 __global__ __launch_bounds__(kBlock) void k_photo_gather(const RefineView* __restrict__ views, uint32_t batch,
-                                                         uint32_t b, RefineStepTables tables,
+                                                         uint32_t b, uint32_t texels_total, RefineStepTables tables,
                                                          const uint32_t* __restrict__ step_counter,
                                                          const float4* __restrict__ resid, float* __restrict__ grad) {
     const uint32_t step = *step_counter;
     const RefineView& rv = views[tables.views[static_cast<size_t>(step) * batch + b]];
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= rv.ntexels) return;
+    if (i >= rv.counts[1]) return;
+    const uint32_t texel = rv.texels[i];
+    if (texel >= texels_total) return;  // the run of unused pixel slots
     const uint32_t e1 = rv.tap_offsets[i + 1];
     float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
     for (uint32_t e = rv.tap_offsets[i]; e < e1; ++e) {
@@ -181,7 +184,7 @@ __global__ __launch_bounds__(kBlock) void k_photo_gather(const RefineView* __res
         a1 += r.y * w;
         a2 += r.z * w;
     }
-    const size_t t = static_cast<size_t>(rv.texels[i]) * 3;
+    const size_t t = static_cast<size_t>(texel) * 3;
     grad[t] += a0;
     grad[t + 1] += a1;
     grad[t + 2] += a2;
@@ -218,18 +221,38 @@ __global__ void k_active_pack(const float* __restrict__ uv, const float* __restr
                          target[static_cast<size_t>(p) * 3 + 2], 0.0f);
 }
 
-// Tap entries of one view's active pixels: key = texel, value = active pixel * 4 + tap.
-__global__ void k_photo_taps(const float4* __restrict__ uvm, uint32_t n, uint32_t tw, uint32_t th,
-                             uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+__global__ void k_active_count(const uint32_t* __restrict__ flags, const uint32_t* __restrict__ pos, uint32_t n,
+                               uint32_t* __restrict__ count) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *count = n ? flags[n - 1] + pos[n - 1] : 0u;
+}
+
+// Tap entries of one view's pixel slots: key = texel, value = slot * 4 + tap;
+// slots past the active count get key tw * th (sorted past every texel).
+__global__ void k_photo_taps(const float4* __restrict__ uvm, const uint32_t* __restrict__ count, uint32_t slots,
+                             uint32_t tw, uint32_t th, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
     const uint32_t a = blockIdx.x * kBlock + threadIdx.x;
-    if (a >= n) return;
-    const float4 q = uvm[a];
-    const AtlasTap at = atlas_tap(q.x, q.y, tw, th);
+    if (a >= slots) return;
+    const bool on = a < *count;
+    AtlasTap at{};
+    if (on) {
+        const float4 q = uvm[a];
+        at = atlas_tap(q.x, q.y, tw, th);
+    }
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        keys[static_cast<size_t>(a) * 4 + k] = at.t[k];
+        keys[static_cast<size_t>(a) * 4 + k] = on ? at.t[k] : tw * th;
         vals[static_cast<size_t>(a) * 4 + k] = a * 4u + static_cast<uint32_t>(k);
     }
+}
+
+// src/texture_refiner.cpp:315-330: the per-step normalisation from the views' mask sums.
+__global__ void k_norm_table(const double* __restrict__ view_weight, const uint32_t* __restrict__ view_index,
+                             uint32_t steps, uint32_t batch, float* __restrict__ norm) {
+    const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+    if (s >= steps) return;
+    double bw = 0.0;
+    for (uint32_t b = 0; b < batch; ++b) bw += view_weight[view_index[static_cast<size_t>(s) * batch + b]];
+    norm[s] = bw > 0.0 ? static_cast<float>(1.0 / (bw * 3.0)) : 0.0f;
 }
 
 // Bilinear weight of every sorted tap entry (the same expression as the scatter's).
@@ -521,7 +544,7 @@ hipError_t photometric_step(const RefineView* views_dev, uint32_t batch, uint32_
                                               max_active, loss_part);
     const uint32_t gblocks = (max_texels + kBlock - 1) / kBlock;
     for (uint32_t b = 0; b < batch; ++b)
-        k_photo_gather<<<gblocks ? gblocks : 1, kBlock, 0, st>>>(views_dev, batch, b, tables, step_counter,
+        k_photo_gather<<<gblocks ? gblocks : 1, kBlock, 0, st>>>(views_dev, batch, b, tw * th, tables, step_counter,
                                                                  resid + static_cast<size_t>(b) * max_active, grad);
     return hipGetLastError();
 }
@@ -553,10 +576,22 @@ hipError_t exclusive_scan(const uint32_t* in, uint32_t* out, uint32_t n, void* t
     return hipcub::DeviceScan::ExclusiveSum(temp, bytes, in, out, static_cast<int>(n), st);
 }
 
-hipError_t photo_taps(const float4* uvm, uint32_t n, uint32_t tw, uint32_t th, uint32_t* keys, uint32_t* vals,
-                      hipStream_t st) {
-    if (!n) return hipSuccess;
-    k_photo_taps<<<(n + kBlock - 1) / kBlock, kBlock, 0, st>>>(uvm, n, tw, th, keys, vals);
+hipError_t active_count(const uint32_t* flags, const uint32_t* pos, uint32_t n, uint32_t* count, hipStream_t st) {
+    k_active_count<<<1, 64, 0, st>>>(flags, pos, n, count);
+    return hipGetLastError();
+}
+
+hipError_t photo_taps(const float4* uvm, const uint32_t* count, uint32_t slots, uint32_t tw, uint32_t th,
+                      uint32_t* keys, uint32_t* vals, hipStream_t st) {
+    if (!slots) return hipSuccess;
+    k_photo_taps<<<(slots + kBlock - 1) / kBlock, kBlock, 0, st>>>(uvm, count, slots, tw, th, keys, vals);
+    return hipGetLastError();
+}
+
+hipError_t norm_table(const double* view_weight, const uint32_t* view_index, uint32_t steps, uint32_t batch,
+                      float* norm, hipStream_t st) {
+    if (!steps) return hipSuccess;
+    k_norm_table<<<(steps + kBlock - 1) / kBlock, kBlock, 0, st>>>(view_weight, view_index, steps, batch, norm);
     return hipGetLastError();
 }
 
