@@ -137,13 +137,18 @@ struct RefineStepTables {
 struct RefineView {
     const float4* uvm;       // [active] (u, v, mask, 0)
     const float4* rgb;       // [active] target (r, g, b, 0)
-    const uint32_t* counts;  // device: [0] active pixels, [1] texel runs (the last may be the past-the-atlas run)
-    // bilinear taps of the active pixels, grouped by texel (ascending; taps of
-    // the unused pixel slots form a last run keyed past the atlas):
-    const uint32_t* texels;       // [runs] texels the view samples
+    const uint32_t* counts;  // device: [0] active pixels
+};
+
+// The bilinear taps of one recurring step batch (the views of its batch slots)
+// merged and grouped by texel, ascending.  Entry = (slot * stride + active
+// pixel) * 4 + tap; within a texel the entries keep slot-then-pixel order.
+struct RefineBatchTaps {
+    const uint32_t* texels;       // [runs] texels the batch samples
     const uint32_t* tap_offsets;  // [runs + 1] entry range of each texel
-    const uint32_t* taps;         // [entries] active pixel * 4 + tap, ascending within a texel
+    const uint32_t* taps;         // [entries], ascending within a texel
     const float* tap_weights;     // [entries] bilinear weight of the tap
+    const uint32_t* runs;         // device count of runs
 };
 
 hipError_t transform_positions(const float* pos3, uint32_t V, const float* m16_host, float* out4, hipStream_t st);
@@ -153,11 +158,13 @@ uint32_t refine_partial_count(uint32_t P);
 hipError_t sum_partials(const double* partials, uint32_t n, double* out, hipStream_t st);
 // Per-block loss partials written by the residual pass (photometric_partials of them).
 uint32_t photometric_partials(uint32_t batch, uint32_t max_active);
-hipError_t photometric_step(const RefineView* views_dev, uint32_t batch, uint32_t max_active, uint32_t max_texels,
+// Residual pass over the step's views, then the texel gather of the step's
+// batch (batches[step % nbatches]); resid holds batch slots of `stride` entries.
+hipError_t photometric_step(const RefineView* views_dev, uint32_t batch, uint32_t max_active,
+                            const RefineBatchTaps* batches_dev, uint32_t nbatches, uint32_t max_runs,
                             const float* tex, uint32_t tw, uint32_t th, float eps, const RefineStepTables& tables,
-                            const uint32_t* step_counter, float4* resid, float* grad, float* loss_part,
-                            hipStream_t st);
-// Precompute of the gather form: tap keys / values of one view, weights of the sorted entries, last offset.
+                            const uint32_t* step_counter, float4* resid, uint32_t stride, float* grad,
+                            float* loss_part, hipStream_t st);
 // Active pixels of one view in 8x8-tile order: flags over tile-order slots,
 // then (after an exclusive scan of the flags) the packed records.
 uint32_t tile_slots(uint32_t W, uint32_t H);
@@ -167,18 +174,22 @@ hipError_t active_pack(const float* uv, const float* target, const float* mask, 
 hipError_t exclusive_scan(const uint32_t* in, uint32_t* out, uint32_t n, void* temp, size_t temp_bytes, hipStream_t st);
 size_t exclusive_scan_temp_bytes(uint32_t n);
 hipError_t active_count(const uint32_t* flags, const uint32_t* pos, uint32_t n, uint32_t* count, hipStream_t st);
-hipError_t photo_taps(const float4* uvm, const uint32_t* count, uint32_t slots, uint32_t tw, uint32_t th,
-                      uint32_t* keys, uint32_t* vals, hipStream_t st);
 // norm[s] = 1 / (3 * sum of the step's views' mask sums) (0 when that sum is 0)
 hipError_t norm_table(const double* view_weight, const uint32_t* view_index, uint32_t steps, uint32_t batch,
                       float* norm, hipStream_t st);
-hipError_t photo_tap_weights(const uint32_t* vals, uint32_t E, const float4* uvm, uint32_t tw, uint32_t th, float* w,
-                             hipStream_t st);
+// Tap entries of batch slot `slot` over its view's `count` packed pixels uvm.
+hipError_t batch_taps(const float4* uvm, uint32_t count, uint32_t slot, uint32_t stride, uint32_t tw, uint32_t th,
+                      uint32_t* keys, uint32_t* vals, hipStream_t st);
+// Weights of sorted batch tap entries; slot_uvm[slot] = that slot's packed pixels (device array).
+hipError_t batch_tap_weights(const uint32_t* vals, uint32_t E, const float4* const* slot_uvm, uint32_t stride,
+                             uint32_t tw, uint32_t th, float* w, hipStream_t st);
+// Sorts n (texel, entry) pairs by texel (stably) and run-length encodes the
+// first E of them: texels, offsets ([runs + 1]) and the device run count.
 size_t photo_group_temp_bytes(uint32_t n);
 hipError_t photo_group_taps(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out, uint32_t* vals_out,
                             uint32_t n, int end_bit, uint32_t E, uint32_t* texels, uint32_t* runlen,
                             uint32_t* offsets, uint32_t* runs, void* temp, size_t temp_bytes, hipStream_t st);
-// Taps of fixed sample points grouped by texel (see RefineView).
+// Taps of fixed sample points grouped by texel (as in RefineBatchTaps).
 struct TapGroups {
     const uint32_t* texels;
     const uint32_t* offsets;

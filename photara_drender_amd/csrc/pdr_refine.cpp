@@ -7,7 +7,9 @@
 // host with the same float library calls, seam polish with a reset Adam) but
 // reads every per-step scalar from device tables indexed by a device step
 // counter, so a block of steps is captured once as a hipGraph and replayed.
+#include <algorithm>
 #include <chrono>
+#include <numeric>
 
 #include "pdr_host.hpp"
 
@@ -110,20 +112,8 @@ pdr_status pdr_refine(pdr_context ctx, const float* initial_texture, const float
         float* raster = bufs.alloc<float>(static_cast<size_t>(max_pixels) * 4);
         double* partials = bufs.alloc<double>(pdr::refine_partial_count(max_pixels) + 1);
         double* view_weight = bufs.alloc<double>(nviews);
-        uint32_t* view_counts = bufs.alloc<uint32_t>(static_cast<size_t>(nviews) * 2);  // active pixels, texel runs
+        uint32_t* view_counts = bufs.alloc<uint32_t>(nviews);  // active pixels
         std::vector<pdr::RefineView> rviews(nviews);
-        // gather form of the photometric gradient: per view, the bilinear taps of
-        // its pixels with mask > 0 grouped by texel.  Per-view arrays are sized for
-        // every pixel, so the loop needs no host readback.
-        const uint32_t ntaps = max_pixels * 4u;
-        uint32_t* tap_keys = bufs.alloc<uint32_t>(ntaps);
-        uint32_t* tap_vals = bufs.alloc<uint32_t>(ntaps);
-        uint32_t* tap_keys_sorted = bufs.alloc<uint32_t>(ntaps);
-        uint32_t* group_runlen = bufs.alloc<uint32_t>(ntaps);
-        const size_t group_temp_bytes = pdr::photo_group_temp_bytes(ntaps);
-        void* group_temp = bufs.alloc<unsigned char>(group_temp_bytes);
-        int key_bits = 1;
-        while (key_bits < 32 && (1ull << key_bits) <= static_cast<unsigned long long>(o.width) * o.height) ++key_bits;
         uint32_t max_slots = 0;
         for (uint32_t k = 0; k < nviews; ++k)
             max_slots = std::max(max_slots, pdr::tile_slots(views[k].width, views[k].height));
@@ -134,21 +124,13 @@ pdr_status pdr_refine(pdr_context ctx, const float* initial_texture, const float
         float* vuv = bufs.alloc<float>(static_cast<size_t>(max_pixels) * 2);
         float* target = bufs.alloc<float>(static_cast<size_t>(max_pixels) * 3);
         float* mask = bufs.alloc<float>(max_pixels);
-        // every view's packed pixels and tap lists in one allocation (no allocation inside the loop)
+        // every view's packed pixels in one allocation, sized for every pixel so
+        // the per-view passes need no host readback
         auto up = [](size_t b) { return (b + 255) & ~static_cast<size_t>(255); };
-        auto view_bytes = [&](uint32_t P) {
-            const size_t E = static_cast<size_t>(P) * 4;
-            return 2 * up(P * sizeof(float4)) + up(E * 4) + up((E + 1) * 4) + up(E * 4) + up(E * 4);
-        };
         size_t total_bytes = 0;
-        for (uint32_t k = 0; k < nviews; ++k) total_bytes += view_bytes(views[k].width * views[k].height);
-        unsigned char* view_arena = bufs.alloc<unsigned char>(total_bytes);
+        for (uint32_t k = 0; k < nviews; ++k) total_bytes += 2 * up(views[k].width * views[k].height * sizeof(float4));
+        unsigned char* view_arena = bufs.alloc<unsigned char>(std::max<size_t>(total_bytes, 256));
         size_t arena_off = 0;
-        auto carve = [&](size_t bytes) {
-            void* p = view_arena + arena_off;
-            arena_off += up(bytes);
-            return p;
-        };
         for (uint32_t k = 0; k < nviews; ++k) {
             const pdr_refine_view& v = views[k];
             const uint32_t P = v.width * v.height;
@@ -170,42 +152,106 @@ pdr_status pdr_refine(pdr_context ctx, const float* initial_texture, const float
                       "refine_view_inputs");
             hip_check(pdr::sum_partials(partials, np, view_weight + k, st), "sum_partials");
             // the pixels with mask > 0, packed in 8x8-tile order
-            uint32_t* counts = view_counts + static_cast<size_t>(k) * 2;
             const uint32_t slots = pdr::tile_slots(v.width, v.height);
             hip_check(pdr::active_flags(mask, v.width, v.height, act_flags, st), "active_flags");
             hip_check(pdr::exclusive_scan(act_flags, act_pos, slots, scan_temp, scan_temp_bytes, st),
                       "exclusive_scan");
-            hip_check(pdr::active_count(act_flags, act_pos, slots, counts, st), "active_count");
-            float4* uvm = static_cast<float4*>(carve(static_cast<size_t>(P) * sizeof(float4)));
-            float4* rgb = static_cast<float4*>(carve(static_cast<size_t>(P) * sizeof(float4)));
+            hip_check(pdr::active_count(act_flags, act_pos, slots, view_counts + k, st), "active_count");
+            float4* uvm = reinterpret_cast<float4*>(view_arena + arena_off);
+            arena_off += up(static_cast<size_t>(P) * sizeof(float4));
+            float4* rgb = reinterpret_cast<float4*>(view_arena + arena_off);
+            arena_off += up(static_cast<size_t>(P) * sizeof(float4));
             hip_check(pdr::active_pack(vuv, target, mask, v.width, v.height, act_flags, act_pos, uvm, rgb, st),
                       "active_pack");
-            // their bilinear taps grouped by texel
-            const uint32_t E = P * 4u;
-            uint32_t* view_texels = static_cast<uint32_t*>(carve(static_cast<size_t>(E) * 4));
-            uint32_t* view_offsets = static_cast<uint32_t*>(carve((static_cast<size_t>(E) + 1) * 4));
-            uint32_t* taps = static_cast<uint32_t*>(carve(static_cast<size_t>(E) * 4));
-            float* tap_w = static_cast<float*>(carve(static_cast<size_t>(E) * 4));
-            hip_check(pdr::photo_taps(uvm, counts, P, o.width, o.height, tap_keys, tap_vals, st), "photo_taps");
-            hip_check(pdr::photo_group_taps(tap_keys, tap_vals, tap_keys_sorted, taps, E, key_bits, E, view_texels,
-                                            group_runlen, view_offsets, counts + 1, group_temp, group_temp_bytes, st),
-                      "photo_group_taps");
-            hip_check(pdr::photo_tap_weights(taps, E, uvm, o.width, o.height, tap_w, st), "photo_tap_weights");
-            rviews[k] = pdr::RefineView{uvm, rgb, counts, view_texels, view_offsets, taps, tap_w};
+            rviews[k] = pdr::RefineView{uvm, rgb, view_counts + k};
         }
-        std::vector<uint32_t> host_counts(static_cast<size_t>(nviews) * 2);
+        std::vector<uint32_t> host_counts(nviews);
         hip_check(hipMemcpyAsync(host_counts.data(), view_counts, host_counts.size() * 4, hipMemcpyDeviceToHost, st),
                   "hipMemcpyAsync");
         sync(st);
-        uint32_t max_active = 0, max_texels = 0;
-        for (uint32_t k = 0; k < nviews; ++k) {
-            max_active = std::max(max_active, host_counts[k * 2]);
-            max_texels = std::max(max_texels, host_counts[k * 2 + 1]);
-        }
+        uint32_t max_active = 0;
+        for (uint32_t k = 0; k < nviews; ++k) max_active = std::max(max_active, host_counts[k]);
+        if (static_cast<uint64_t>(o.batch_size) * max_active * 4u > 0x7fffffffull)
+            raise(PDR_OVERFLOW, "texture refinement batch exceeds 2^29 active pixels");
         pdr::RefineView* dviews = bufs.alloc<pdr::RefineView>(nviews);
         hip_check(hipMemcpyAsync(dviews, rviews.data(), nviews * sizeof(pdr::RefineView), hipMemcpyHostToDevice, st),
                   "hipMemcpyAsync");
+
+        // Gather form of the photometric gradient.  Step s samples views
+        // (s * batch + b) % nviews, so the step batches recur with period
+        // nviews / gcd(nviews, batch): each recurring batch's bilinear taps are
+        // merged and grouped by texel once here (a stable radix sort by texel).
+        uint32_t nbatches = nviews / std::gcd(nviews, o.batch_size);
+        nbatches = std::max<uint32_t>(1, std::min(nbatches, o.steps));
+        auto slot_view = [&](uint32_t c, uint32_t b) {
+            return static_cast<uint32_t>((static_cast<uint64_t>(c) * o.batch_size + b) % nviews);
+        };
+        std::vector<uint32_t> batch_entries(nbatches, 0);
+        uint32_t max_entries = 0;
+        size_t tap_bytes = 0;
+        for (uint32_t c = 0; c < nbatches; ++c) {
+            for (uint32_t b = 0; b < o.batch_size; ++b) batch_entries[c] += host_counts[slot_view(c, b)] * 4u;
+            max_entries = std::max(max_entries, batch_entries[c]);
+            const size_t E = batch_entries[c];
+            tap_bytes += 3 * up(E * 4) + up((E + 1) * 4);
+        }
+        int key_bits = 1;
+        while (key_bits < 32 && (1ull << key_bits) < static_cast<unsigned long long>(o.width) * o.height) ++key_bits;
+        uint32_t* tap_keys = bufs.alloc<uint32_t>(std::max<uint32_t>(max_entries, 1));
+        uint32_t* tap_vals = bufs.alloc<uint32_t>(std::max<uint32_t>(max_entries, 1));
+        uint32_t* tap_keys_sorted = bufs.alloc<uint32_t>(std::max<uint32_t>(max_entries, 1));
+        uint32_t* group_runlen = bufs.alloc<uint32_t>(std::max<uint32_t>(max_entries, 1));
+        const size_t group_temp_bytes = pdr::photo_group_temp_bytes(max_entries);
+        void* group_temp = bufs.alloc<unsigned char>(std::max<size_t>(group_temp_bytes, 1));
+        unsigned char* tap_arena = bufs.alloc<unsigned char>(std::max<size_t>(tap_bytes, 256));
+        uint32_t* batch_runs = bufs.alloc<uint32_t>(nbatches);
+        const float4** slot_uvm = bufs.alloc<const float4*>(static_cast<size_t>(nbatches) * o.batch_size);
+        std::vector<const float4*> host_slot_uvm(static_cast<size_t>(nbatches) * o.batch_size);
+        for (uint32_t c = 0; c < nbatches; ++c)
+            for (uint32_t b = 0; b < o.batch_size; ++b)
+                host_slot_uvm[static_cast<size_t>(c) * o.batch_size + b] = rviews[slot_view(c, b)].uvm;
+        hip_check(hipMemcpyAsync(slot_uvm, host_slot_uvm.data(), host_slot_uvm.size() * sizeof(const float4*),
+                                 hipMemcpyHostToDevice, st),
+                  "hipMemcpyAsync");
+        hip_check(pdr::fill_u32(batch_runs, 0u, nbatches, st), "fill");
+        std::vector<pdr::RefineBatchTaps> rbatches(nbatches);
+        size_t tap_off = 0;
+        auto carve = [&](size_t bytes) {
+            void* p = tap_arena + tap_off;
+            tap_off += up(bytes);
+            return p;
+        };
+        for (uint32_t c = 0; c < nbatches; ++c) {
+            const uint32_t E = batch_entries[c];
+            uint32_t off = 0;
+            for (uint32_t b = 0; b < o.batch_size; ++b) {
+                const uint32_t k = slot_view(c, b);
+                hip_check(pdr::batch_taps(rviews[k].uvm, host_counts[k], b, max_active, o.width, o.height,
+                                          tap_keys + off, tap_vals + off, st),
+                          "batch_taps");
+                off += host_counts[k] * 4u;
+            }
+            uint32_t* texels_c = static_cast<uint32_t*>(carve(static_cast<size_t>(E) * 4));
+            uint32_t* offsets_c = static_cast<uint32_t*>(carve((static_cast<size_t>(E) + 1) * 4));
+            uint32_t* taps_c = static_cast<uint32_t*>(carve(static_cast<size_t>(E) * 4));
+            float* weights_c = static_cast<float*>(carve(static_cast<size_t>(E) * 4));
+            hip_check(pdr::photo_group_taps(tap_keys, tap_vals, tap_keys_sorted, taps_c, E, key_bits, E, texels_c,
+                                            group_runlen, offsets_c, batch_runs + c, group_temp, group_temp_bytes, st),
+                      "photo_group_taps");
+            hip_check(pdr::batch_tap_weights(taps_c, E, slot_uvm + static_cast<size_t>(c) * o.batch_size, max_active,
+                                             o.width, o.height, weights_c, st),
+                      "batch_tap_weights");
+            rbatches[c] = pdr::RefineBatchTaps{texels_c, offsets_c, taps_c, weights_c, batch_runs + c};
+        }
+        pdr::RefineBatchTaps* dbatches = bufs.alloc<pdr::RefineBatchTaps>(nbatches);
+        hip_check(hipMemcpyAsync(dbatches, rbatches.data(), nbatches * sizeof(pdr::RefineBatchTaps),
+                                 hipMemcpyHostToDevice, st),
+                  "hipMemcpyAsync");
+        std::vector<uint32_t> host_runs(nbatches);
+        hip_check(hipMemcpyAsync(host_runs.data(), batch_runs, nbatches * 4, hipMemcpyDeviceToHost, st),
+                  "hipMemcpyAsync");
         sync(st);
+        const uint32_t max_runs = *std::max_element(host_runs.begin(), host_runs.end());
         const auto t1 = std::chrono::steady_clock::now();
 
         // ---------------------------------------------------------- state + schedule
@@ -307,8 +353,9 @@ pdr_status pdr_refine(pdr_context ctx, const float* initial_texture, const float
         float* texbuf[2] = {tex, fused_tv ? bufs.alloc<float>(n) : tex};
         const auto t2 = std::chrono::steady_clock::now();
         auto one_step = [&](hipStream_t s, uint32_t c) {
-            hip_check(pdr::photometric_step(dviews, o.batch_size, max_active, max_texels, texbuf[c], o.width, o.height,
-                                            o.photometric_epsilon, tables, counters, resid, grad, loss_part, s),
+            hip_check(pdr::photometric_step(dviews, o.batch_size, max_active, dbatches, nbatches, max_runs, texbuf[c],
+                                            o.width, o.height, o.photometric_epsilon, tables, counters, resid,
+                                            max_active, grad, loss_part, s),
                       "photometric_step");
             if (o.joint_seam_weight != 0.0f && npairs)
                 hip_check(pdr::seam_step(texbuf[c], o.width, o.height, dpairs, npairs, o.seam_epsilon, seam_norm,

@@ -10,13 +10,14 @@
 //                     (k_active_flags, a scan, k_active_pack); per-step scalars come
 //                     from device tables indexed by a device step counter so
 //                     the step loop can be captured once and replayed
-//  k_photo_gather     the texel gradient as a gather: each view's bilinear taps
-//                     are inverted once at precompute (k_photo_taps, a radix
-//                     sort by texel, run-length offsets, k_photo_tap_weights),
-//                     so a step sums sum_taps og * w per texel of a view with a
-//                     plain read-modify-write, one batch slot after another -
-//                     the same terms as the reference's per-pixel atomic
-//                     scatter, in a fixed order and without atomics
+//  k_photo_gather     the texel gradient as a gather: the step batches recur
+//                     (step s uses batch s % L), so each batch's bilinear taps
+//                     are inverted once at precompute (k_batch_taps, a radix
+//                     sort by texel, run-length offsets, k_batch_tap_weights),
+//                     and a step sums og * w over each texel's taps from all
+//                     its views with one plain read-modify-write - the same
+//                     terms as the reference's per-pixel atomic scatter, in a
+//                     fixed order and without atomics
 //  k_seam_residual /  atlas_seam_gradient.hlsl:50-100, in the same residual +
 //  k_seam_gather      gather form over the (fixed) seam pairs' taps
 //  k_adam             atlas_adam_update.hlsl:20-43, float4-vectorised
@@ -164,22 +165,20 @@ __global__ __launch_bounds__(kBlock) void k_photo_residual(const RefineView* __r
     if (threadIdx.x == 0) *part = (wsum[0] + wsum[1]) + (wsum[2] + wsum[3]);
 }
 
-// Texel gradient of batch slot b: one lane per texel the slot's view samples.
-__global__ __launch_bounds__(kBlock) void k_photo_gather(const RefineView* __restrict__ views, uint32_t batch,
-                                                         uint32_t b, uint32_t texels_total, RefineStepTables tables,
+// Texel gradient of the step's batch: one lane per texel the batch samples,
+// its taps from all slots added in list order.
+__global__ __launch_bounds__(kBlock) void k_photo_gather(const RefineBatchTaps* __restrict__ batches, uint32_t nbatches,
                                                          const uint32_t* __restrict__ step_counter,
                                                          const float4* __restrict__ resid, float* __restrict__ grad) {
-    const uint32_t step = *step_counter;
-    const RefineView& rv = views[tables.views[static_cast<size_t>(step) * batch + b]];
+    const RefineBatchTaps& bt = batches[*step_counter % nbatches];
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= rv.counts[1]) return;
-    const uint32_t texel = rv.texels[i];
-    if (texel >= texels_total) return;  // the run of unused pixel slots
-    const uint32_t e1 = rv.tap_offsets[i + 1];
+    if (i >= *bt.runs) return;
+    const uint32_t texel = bt.texels[i];
+    const uint32_t e1 = bt.tap_offsets[i + 1];
     float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
-    for (uint32_t e = rv.tap_offsets[i]; e < e1; ++e) {
-        const float w = rv.tap_weights[e];
-        const float4 r = resid[rv.taps[e] >> 2];
+    for (uint32_t e = bt.tap_offsets[i]; e < e1; ++e) {
+        const float w = bt.tap_weights[e];
+        const float4 r = resid[bt.taps[e] >> 2];
         a0 += r.x * w;
         a1 += r.y * w;
         a2 += r.z * w;
@@ -226,23 +225,32 @@ __global__ void k_active_count(const uint32_t* __restrict__ flags, const uint32_
     if (threadIdx.x == 0 && blockIdx.x == 0) *count = n ? flags[n - 1] + pos[n - 1] : 0u;
 }
 
-// Tap entries of one view's pixel slots: key = texel, value = slot * 4 + tap;
-// slots past the active count get key tw * th (sorted past every texel).
-__global__ void k_photo_taps(const float4* __restrict__ uvm, const uint32_t* __restrict__ count, uint32_t slots,
+// Tap entries of batch slot `slot` over its view's `count` packed pixels:
+// key = texel, value = (slot * stride + a) * 4 + tap.
+__global__ void k_batch_taps(const float4* __restrict__ uvm, uint32_t count, uint32_t slot, uint32_t stride,
                              uint32_t tw, uint32_t th, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
     const uint32_t a = blockIdx.x * kBlock + threadIdx.x;
-    if (a >= slots) return;
-    const bool on = a < *count;
-    AtlasTap at{};
-    if (on) {
-        const float4 q = uvm[a];
-        at = atlas_tap(q.x, q.y, tw, th);
-    }
+    if (a >= count) return;
+    const float4 q = uvm[a];
+    const AtlasTap at = atlas_tap(q.x, q.y, tw, th);
+    const uint32_t base = (slot * stride + a) * 4u;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        keys[static_cast<size_t>(a) * 4 + k] = on ? at.t[k] : tw * th;
-        vals[static_cast<size_t>(a) * 4 + k] = a * 4u + static_cast<uint32_t>(k);
+        keys[static_cast<size_t>(a) * 4 + k] = at.t[k];
+        vals[static_cast<size_t>(a) * 4 + k] = base + static_cast<uint32_t>(k);
     }
+}
+
+// Bilinear weight of every sorted batch tap entry (the same expression as the scatter's).
+__global__ void k_batch_tap_weights(const uint32_t* __restrict__ vals, uint32_t E,
+                                    const float4* const* __restrict__ slot_uvm, uint32_t stride, uint32_t tw,
+                                    uint32_t th, float* __restrict__ w) {
+    const uint32_t e = blockIdx.x * kBlock + threadIdx.x;
+    if (e >= E) return;
+    const uint32_t v = vals[e], r = v >> 2;
+    const float4 q = slot_uvm[r / stride][r % stride];
+    const AtlasTap at = atlas_tap(q.x, q.y, tw, th);
+    w[e] = at.w[v & 3u];
 }
 
 // src/texture_refiner.cpp:315-330: the per-step normalisation from the views' mask sums.
@@ -253,17 +261,6 @@ __global__ void k_norm_table(const double* __restrict__ view_weight, const uint3
     double bw = 0.0;
     for (uint32_t b = 0; b < batch; ++b) bw += view_weight[view_index[static_cast<size_t>(s) * batch + b]];
     norm[s] = bw > 0.0 ? static_cast<float>(1.0 / (bw * 3.0)) : 0.0f;
-}
-
-// Bilinear weight of every sorted tap entry (the same expression as the scatter's).
-__global__ void k_photo_tap_weights(const uint32_t* __restrict__ vals, uint32_t E, const float4* __restrict__ uvm,
-                                    uint32_t tw, uint32_t th, float* __restrict__ w) {
-    const uint32_t e = blockIdx.x * kBlock + threadIdx.x;
-    if (e >= E) return;
-    const uint32_t v = vals[e];
-    const float4 q = uvm[v >> 2];
-    const AtlasTap at = atlas_tap(q.x, q.y, tw, th);
-    w[e] = at.w[v & 3u];
 }
 
 // offsets[U] = E (the run starts come from an exclusive scan of the run lengths).
@@ -534,18 +531,17 @@ uint32_t photometric_partials(uint32_t batch, uint32_t max_active) {
     return (blocks ? blocks : 1) * batch;
 }
 
-hipError_t photometric_step(const RefineView* views_dev, uint32_t batch, uint32_t max_active, uint32_t max_texels,
+hipError_t photometric_step(const RefineView* views_dev, uint32_t batch, uint32_t max_active,
+                            const RefineBatchTaps* batches_dev, uint32_t nbatches, uint32_t max_runs,
                             const float* tex, uint32_t tw, uint32_t th, float eps, const RefineStepTables& tables,
-                            const uint32_t* step_counter, float4* resid, float* grad, float* loss_part,
-                            hipStream_t st) {
+                            const uint32_t* step_counter, float4* resid, uint32_t stride, float* grad,
+                            float* loss_part, hipStream_t st) {
     const uint32_t blocks = (max_active + kBlock - 1) / kBlock;
     const dim3 grid(blocks ? blocks : 1, batch);
-    k_photo_residual<<<grid, kBlock, 0, st>>>(views_dev, batch, tex, tw, th, eps, tables, step_counter, resid,
-                                              max_active, loss_part);
-    const uint32_t gblocks = (max_texels + kBlock - 1) / kBlock;
-    for (uint32_t b = 0; b < batch; ++b)
-        k_photo_gather<<<gblocks ? gblocks : 1, kBlock, 0, st>>>(views_dev, batch, b, tw * th, tables, step_counter,
-                                                                 resid + static_cast<size_t>(b) * max_active, grad);
+    k_photo_residual<<<grid, kBlock, 0, st>>>(views_dev, batch, tex, tw, th, eps, tables, step_counter, resid, stride,
+                                              loss_part);
+    const uint32_t gblocks = (max_runs + kBlock - 1) / kBlock;
+    k_photo_gather<<<gblocks ? gblocks : 1, kBlock, 0, st>>>(batches_dev, nbatches, step_counter, resid, grad);
     return hipGetLastError();
 }
 
@@ -581,10 +577,17 @@ hipError_t active_count(const uint32_t* flags, const uint32_t* pos, uint32_t n, 
     return hipGetLastError();
 }
 
-hipError_t photo_taps(const float4* uvm, const uint32_t* count, uint32_t slots, uint32_t tw, uint32_t th,
+hipError_t batch_taps(const float4* uvm, uint32_t count, uint32_t slot, uint32_t stride, uint32_t tw, uint32_t th,
                       uint32_t* keys, uint32_t* vals, hipStream_t st) {
-    if (!slots) return hipSuccess;
-    k_photo_taps<<<(slots + kBlock - 1) / kBlock, kBlock, 0, st>>>(uvm, count, slots, tw, th, keys, vals);
+    if (!count) return hipSuccess;
+    k_batch_taps<<<(count + kBlock - 1) / kBlock, kBlock, 0, st>>>(uvm, count, slot, stride, tw, th, keys, vals);
+    return hipGetLastError();
+}
+
+hipError_t batch_tap_weights(const uint32_t* vals, uint32_t E, const float4* const* slot_uvm, uint32_t stride,
+                             uint32_t tw, uint32_t th, float* w, hipStream_t st) {
+    if (!E) return hipSuccess;
+    k_batch_tap_weights<<<(E + kBlock - 1) / kBlock, kBlock, 0, st>>>(vals, E, slot_uvm, stride, tw, th, w);
     return hipGetLastError();
 }
 
@@ -592,13 +595,6 @@ hipError_t norm_table(const double* view_weight, const uint32_t* view_index, uin
                       float* norm, hipStream_t st) {
     if (!steps) return hipSuccess;
     k_norm_table<<<(steps + kBlock - 1) / kBlock, kBlock, 0, st>>>(view_weight, view_index, steps, batch, norm);
-    return hipGetLastError();
-}
-
-hipError_t photo_tap_weights(const uint32_t* vals, uint32_t E, const float4* uvm, uint32_t tw, uint32_t th, float* w,
-                             hipStream_t st) {
-    if (!E) return hipSuccess;
-    k_photo_tap_weights<<<(E + kBlock - 1) / kBlock, kBlock, 0, st>>>(vals, E, uvm, tw, th, w);
     return hipGetLastError();
 }
 
