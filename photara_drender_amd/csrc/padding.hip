@@ -77,72 +77,103 @@ __global__ void k_pad_prepare(uint32_t P, const float* __restrict__ valid, const
     count_total(counters + 4, nrefill);
 }
 
-// One 8SSEDT sweep as a wavefront; mirror = the backward sweep.  field holds
-// packed int16 offsets: read as the initial state of the backward sweep,
-// written by both.  ring: R x 4 settled offsets (dynamic LDS).  Each lane
-// walks up to kClasses row classes c = lane + j * kSweepThreads; a class
-// settles (x, y) at step 2 c + x + 2 R k for its k-th row y = c + k R, and the
-// lane loads the input of a class's next texel one step ahead.
-constexpr int kClasses = 8;  // R <= kClasses * kSweepThreads (W <= 16384)
-
+// One 8SSEDT sweep as a wavefront; kMirror = the backward sweep.  field
+// holds packed int16 offsets: read as the initial state of the backward
+// sweep, written by both; field[W * H] is a discard slot.  ring: R x 4
+// settled offsets (dynamic LDS).  Each lane walks kC row classes
+// c = lane + j * blockDim.x; a class settles (x, y) at step 2 c + x + 2 R k
+// for its k-th row y = c + k R.  A step's only global load is the input of a
+// texel, which does not depend on the sweep: each class loads it kD steps
+// ahead into a register ring indexed by the step (static slots: the step loop
+// is unrolled by kD).  Every lane issues the same global loads and stores each
+// step (clamped addresses, the discard slot), so the waits for a step's
+// input cover only that load and not the ones issued after it.
+template <bool kMirror>
 __device__ __forceinline__ uint32_t sweep_input(const uint8_t* __restrict__ source, const uint32_t* __restrict__ field,
-                                                uint32_t W, uint32_t H, uint32_t x, uint32_t y, int mirror) {
-    const uint32_t rx = mirror ? W - 1u - x : x, ry = mirror ? H - 1u - y : y;
+                                                uint32_t W, uint32_t H, uint32_t x, uint32_t y) {
+    x = x < W ? x : W - 1u;
+    y = y < H ? y : H - 1u;
+    const uint32_t rx = kMirror ? W - 1u - x : x, ry = kMirror ? H - 1u - y : y;
     const size_t idx = static_cast<size_t>(ry) * W + rx;
-    return mirror ? field[idx] : static_cast<uint32_t>(source[idx]);
+    if constexpr (kMirror)
+        return field[idx];
+    else
+        return static_cast<uint32_t>(source[idx]);
 }
 
+// Next texel of a class walk: along the row, then R rows down.
+__device__ __forceinline__ void class_advance(uint32_t& x, uint32_t& y, uint32_t R) {
+    const bool wrap = x + 1u == 2u * R;
+    x = wrap ? 0u : x + 1u;
+    y = wrap ? y + R : y;
+}
+
+template <int kC, int kD, bool kMirror>
 __global__ __launch_bounds__(kSweepThreads) void k_pad_sweep(uint32_t W, uint32_t H, const uint8_t* __restrict__ source,
-                                                              uint32_t* __restrict__ field, int mirror) {
+                                                              uint32_t* __restrict__ field) {
     extern __shared__ uint32_t ring[];
     const uint32_t R = (W + 1u) / 2u;
     const uint32_t steps = W + 2u * (H - 1u);
-    uint32_t px[kClasses], py[kClasses], next[kClasses];
+    const size_t discard = static_cast<size_t>(W) * H;
+    uint32_t px[kC], py[kC];  // texel the class settles next
+    uint32_t fx[kC], fy[kC];  // texel whose input the class loads next
+    uint32_t in[kC][kD];      // in[j][t % kD]: input of class j's texel of step t
 #pragma unroll
-    for (int j = 0; j < kClasses; ++j) {
-        const uint32_t c = threadIdx.x + static_cast<uint32_t>(j) * kSweepThreads;
-        px[j] = 0;
-        py[j] = c;
-        next[j] = c < R && c < H ? sweep_input(source, field, W, H, 0, c, mirror) : 0u;
-    }
-    for (uint32_t t = 0; t < steps; ++t) {
+    for (int j = 0; j < kC; ++j) {
+        const uint32_t c = threadIdx.x + static_cast<uint32_t>(j) * blockDim.x;
+        px[j] = fx[j] = 0;
+        py[j] = fy[j] = c;
 #pragma unroll
-        for (int j = 0; j < kClasses; ++j) {
-            const uint32_t c = threadIdx.x + static_cast<uint32_t>(j) * kSweepThreads;
-            if (c >= R || t < 2u * c) continue;
-            const uint32_t x = px[j], y = py[j];
-            if (x < W && y < H) {
-                const uint32_t rx = mirror ? W - 1u - x : x, ry = mirror ? H - 1u - y : y;
-                const size_t idx = static_cast<size_t>(ry) * W + rx;
-                int32_t cx, cy;
-                if (mirror) {
-                    cx = -off_x(next[j]);
-                    cy = -off_y(next[j]);
-                } else {
-                    cx = cy = next[j] ? 0 : kFar;
-                }
-                if (x > 0) relax(cx, cy, ring[c * 4u + ((t - 1u) & 3u)], -1, 0);
-                if (y > 0) {
-                    const uint32_t pc = c == 0 ? R - 1u : c - 1u;
-                    relax(cx, cy, ring[pc * 4u + ((t - 2u) & 3u)], 0, -1);
-                    if (x > 0) relax(cx, cy, ring[pc * 4u + ((t - 3u) & 3u)], -1, -1);
-                    if (x + 1u < W) relax(cx, cy, ring[pc * 4u + ((t - 1u) & 3u)], 1, -1);
-                }
-                ring[c * 4u + (t & 3u)] = pack(cx, cy);
-                field[idx] = mirror ? pack(-cx, -cy) : pack(cx, cy);
-            }
-            // the class's next texel, and its input loaded ahead
-            uint32_t nx = x + 1u, ny = y;
-            if (nx == 2u * R) {
-                nx = 0;
-                ny = y + R;
-            }
-            px[j] = nx;
-            py[j] = ny;
-            if (nx < W && ny < H) next[j] = sweep_input(source, field, W, H, nx, ny, mirror);
+        for (int d = 0; d < kD; ++d) {
+            in[j][d] = sweep_input<kMirror>(source, field, W, H, fx[j], fy[j]);
+            if (c < R && static_cast<uint32_t>(d) >= 2u * c) class_advance(fx[j], fy[j], R);
         }
-        lds_step_barrier();
     }
+    for (uint32_t t0 = 0; t0 < steps; t0 += kD) {
+#pragma unroll
+        for (int s = 0; s < kD; ++s) {
+            // steps past the last one (up to kD - 1) settle nothing: stores go to the discard slot
+            const uint32_t t = t0 + static_cast<uint32_t>(s);
+#pragma unroll
+            for (int j = 0; j < kC; ++j) {
+                const uint32_t c = threadIdx.x + static_cast<uint32_t>(j) * blockDim.x;
+                const bool live = c < R && t >= 2u * c;
+                const uint32_t x = px[j], y = py[j];
+                const bool settle = live && x < W && y < H;
+                int32_t cx, cy;
+                if (kMirror) {
+                    cx = -off_x(in[j][s]);
+                    cy = -off_y(in[j][s]);
+                } else {
+                    cx = cy = in[j][s] ? 0 : kFar;
+                }
+                if (settle) {
+                    if (x > 0) relax(cx, cy, ring[c * 4u + ((t - 1u) & 3u)], -1, 0);
+                    if (y > 0) {
+                        const uint32_t pc = c == 0 ? R - 1u : c - 1u;
+                        relax(cx, cy, ring[pc * 4u + ((t - 2u) & 3u)], 0, -1);
+                        if (x > 0) relax(cx, cy, ring[pc * 4u + ((t - 3u) & 3u)], -1, -1);
+                        if (x + 1u < W) relax(cx, cy, ring[pc * 4u + ((t - 1u) & 3u)], 1, -1);
+                    }
+                    ring[c * 4u + (t & 3u)] = pack(cx, cy);
+                }
+                const uint32_t rx = kMirror ? W - 1u - x : x, ry = kMirror ? H - 1u - y : y;
+                field[settle ? static_cast<size_t>(ry) * W + rx : discard] = kMirror ? pack(-cx, -cy) : pack(cx, cy);
+                if (live) class_advance(px[j], py[j], R);
+                // the input of the class's texel at step t + kD
+                in[j][s] = sweep_input<kMirror>(source, field, W, H, fx[j], fy[j]);
+                if (c < R && t + static_cast<uint32_t>(kD) >= 2u * c) class_advance(fx[j], fy[j], R);
+            }
+            lds_step_barrier();
+        }
+    }
+}
+
+// Row classes per lane and lanes of the sweep for a W-wide grid.
+void sweep_shape(uint32_t W, int& classes, uint32_t& threads) {
+    const uint32_t R = (W + 1u) / 2u;
+    classes = R <= kSweepThreads ? 1 : (R <= 2 * kSweepThreads ? 2 : (R <= 4 * kSweepThreads ? 4 : 8));
+    threads = classes == 1 ? (R + 63u) / 64u * 64u : kSweepThreads;
 }
 
 // Fills (:289-348).  Sources are never written; colours are read from `src_color`
@@ -193,16 +224,31 @@ hipError_t pad_prepare(uint32_t P, const float* valid, const float* coverage, fl
     return hipGetLastError();
 }
 
+template <int kC, int kD>
+static hipError_t launch_sweeps(uint32_t W, uint32_t H, const uint8_t* source, uint32_t* field, uint32_t threads,
+                                size_t lds, hipStream_t st) {
+    if (lds > 64u * 1024u) {
+        for (const void* k : {reinterpret_cast<const void*>(&k_pad_sweep<kC, kD, false>),
+                              reinterpret_cast<const void*>(&k_pad_sweep<kC, kD, true>)}) {
+            const hipError_t e =
+                hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+            if (e != hipSuccess) return e;
+        }
+    }
+    k_pad_sweep<kC, kD, false><<<1, threads, lds, st>>>(W, H, source, field);
+    k_pad_sweep<kC, kD, true><<<1, threads, lds, st>>>(W, H, source, field);
+    return hipGetLastError();
+}
+
 hipError_t pad_field(uint32_t W, uint32_t H, const uint8_t* source, uint32_t* field, hipStream_t st) {
     const size_t lds = static_cast<size_t>((W + 1u) / 2u) * 4u * sizeof(uint32_t);
-    if (lds > 64u * 1024u) {
-        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_pad_sweep),
-                                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
-        if (e != hipSuccess) return e;
-    }
-    k_pad_sweep<<<1, kSweepThreads, lds, st>>>(W, H, source, field, 0);
-    k_pad_sweep<<<1, kSweepThreads, lds, st>>>(W, H, source, field, 1);
-    return hipGetLastError();
+    int classes = 1;
+    uint32_t threads = 64;
+    sweep_shape(W, classes, threads);
+    if (classes == 1) return launch_sweeps<1, 8>(W, H, source, field, threads, lds, st);
+    if (classes == 2) return launch_sweeps<2, 8>(W, H, source, field, threads, lds, st);
+    if (classes == 4) return launch_sweeps<4, 8>(W, H, source, field, threads, lds, st);
+    return launch_sweeps<8, 4>(W, H, source, field, threads, lds, st);
 }
 
 hipError_t pad_fill(uint32_t W, uint32_t H, uint32_t C, const uint32_t* field, const uint8_t* source,

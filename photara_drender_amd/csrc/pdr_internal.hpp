@@ -261,6 +261,7 @@ hipError_t mlp_reduce(const float* part, uint32_t waves, float* grad, float* los
 // counters: valid, gutter, unobserved, remaining, filled (zeroed by the caller).
 hipError_t pad_prepare(uint32_t P, const float* valid, const float* coverage, float* filled, float threshold,
                        uint8_t* source, unsigned long long* counters, hipStream_t st);
+// field: W * H + 1 entries (the last is written as a discard slot)
 hipError_t pad_field(uint32_t W, uint32_t H, const uint8_t* source, uint32_t* field, hipStream_t st);
 hipError_t pad_fill(uint32_t W, uint32_t H, uint32_t C, const uint32_t* field, const uint8_t* source,
                     const float* coverage, uint32_t margin, int fill_unobserved, uint32_t max_fill,

@@ -53,7 +53,7 @@ pdr_status pdr_pad_texture_atlas(pdr_context ctx, float* color, uint32_t width, 
             dfilled = const_cast<float*>(stage_in(ctx, "pad_filled", static_cast<const float*>(filled_mask), P, st));
         }
         uint8_t* source = ctx->get_t<uint8_t>("pad_source", P);
-        uint32_t* field = ctx->get_t<uint32_t>("pad_field", P);
+        uint32_t* field = ctx->get_t<uint32_t>("pad_field", P + 1);  // + the sweeps' discard slot
         unsigned long long* counters = ctx->get_t<unsigned long long>("pad_counters", 5);
         hip_check(pdr::fill_u64(counters, 0ull, 5, st), "fill_u64");
         ctx->timed("pad_prepare", st, [&] {
