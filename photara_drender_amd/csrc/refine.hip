@@ -165,6 +165,35 @@ __global__ __launch_bounds__(kBlock) void k_photo_residual(const RefineView* __r
     if (threadIdx.x == 0) *part = (wsum[0] + wsum[1]) + (wsum[2] + wsum[3]);
 }
 
+// acc += sum over entries [e0, e1) of resid[tap >> kShift] * weight, in entry
+// order; the loads of four entries are issued together.
+template <int kShift>
+__device__ __forceinline__ void gather_taps(uint32_t e0, uint32_t e1, const uint32_t* __restrict__ taps,
+                                            const float* __restrict__ weights, const float4* __restrict__ resid,
+                                            float acc[3]) {
+    for (uint32_t e = e0; e < e1; e += 4) {
+        uint32_t tp[4];
+        float w[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const bool on = e + k < e1;
+            tp[k] = on ? taps[e + k] : 0u;
+            w[k] = on ? weights[e + k] : 0.0f;
+        }
+        float4 r[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            r[k] = e + k < e1 ? resid[tp[k] >> kShift] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (e + k >= e1) break;
+            acc[0] += r[k].x * w[k];
+            acc[1] += r[k].y * w[k];
+            acc[2] += r[k].z * w[k];
+        }
+    }
+}
+
 // Texel gradient of the step's batch: one lane per texel the batch samples,
 // its taps from all slots added in list order.
 __global__ __launch_bounds__(kBlock) void k_photo_gather(const RefineBatchTaps* __restrict__ batches, uint32_t nbatches,
@@ -174,19 +203,12 @@ __global__ __launch_bounds__(kBlock) void k_photo_gather(const RefineBatchTaps* 
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= *bt.runs) return;
     const uint32_t texel = bt.texels[i];
-    const uint32_t e1 = bt.tap_offsets[i + 1];
-    float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
-    for (uint32_t e = bt.tap_offsets[i]; e < e1; ++e) {
-        const float w = bt.tap_weights[e];
-        const float4 r = resid[bt.taps[e] >> 2];
-        a0 += r.x * w;
-        a1 += r.y * w;
-        a2 += r.z * w;
-    }
+    float a[3] = {0.0f, 0.0f, 0.0f};
+    gather_taps<2>(bt.tap_offsets[i], bt.tap_offsets[i + 1], bt.taps, bt.tap_weights, resid, a);
     const size_t t = static_cast<size_t>(texel) * 3;
-    grad[t] += a0;
-    grad[t + 1] += a1;
-    grad[t + 2] += a2;
+    grad[t] += a[0];
+    grad[t + 1] += a[1];
+    grad[t + 2] += a[2];
 }
 
 // Tile-order slot q of a W x H view: tile q / 64 (row-major over 8x8 tiles),
@@ -314,19 +336,12 @@ __global__ __launch_bounds__(kBlock) void k_seam_gather(const uint32_t* __restri
                                                         const float4* __restrict__ resid, float* __restrict__ grad) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= ntexels) return;
-    const uint32_t e1 = offsets[i + 1];
-    float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
-    for (uint32_t e = offsets[i]; e < e1; ++e) {
-        const float w = weights[e];
-        const float4 r = resid[taps[e] >> 3];
-        a0 += r.x * w;
-        a1 += r.y * w;
-        a2 += r.z * w;
-    }
+    float a[3] = {0.0f, 0.0f, 0.0f};
+    gather_taps<3>(offsets[i], offsets[i + 1], taps, weights, resid, a);
     const size_t t = static_cast<size_t>(texels[i]) * 3;
-    grad[t] += a0;
-    grad[t + 1] += a1;
-    grad[t + 2] += a2;
+    grad[t] += a[0];
+    grad[t + 1] += a[1];
+    grad[t + 2] += a[2];
 }
 
 __global__ void k_seam_taps(const float* __restrict__ pairs, uint32_t npairs, uint32_t tw, uint32_t th,
