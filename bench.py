@@ -102,6 +102,39 @@ def traffic_summary(fetch_dir, write_dir, out_path, commands):
     return out_path
 
 
+def trace_summary(directory, limit=40):
+    """Per-kernel table of a rocprofv3 --kernel-trace --output-format csv folder: dispatches grouped by kernel
+    and grid size (x in work-items, y), with the mean and total of (end - start)."""
+    import csv
+    import glob
+    import re
+
+    groups = {}
+    for path in glob.glob(os.path.join(directory, "**", "*kernel_trace.csv"), recursive=True):
+        with open(path) as f:
+            for r in csv.DictReader(f):
+                name = r["Kernel_Name"]
+                m = re.search(r"(k_[a-z0-9_]+(<[^>(]*>)?)", name)
+                if m:
+                    short = m.group(1)
+                elif "at::native" in name:
+                    short = "torch " + re.search(r"at::native::(\w+)", name).group(1)
+                elif "rocprim" in name or "hipcub" in name:
+                    short = "rocprim scan/sort"
+                elif name.startswith("Cijk"):
+                    short = "hipBLASLt GEMM"
+                else:
+                    short = name.split("(")[0][:40]
+                key = (short, int(r["Grid_Size_X"]), int(r.get("Grid_Size_Y", 1) or 1))
+                groups.setdefault(key, []).append(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]))
+    rows = sorted(groups.items(), key=lambda kv: -sum(kv[1]))[:limit]
+    lines = []
+    for (short, gx, gy), d in rows:
+        lines.append(f"{short:34s} grid {gx:>9d}x{gy:<3d} calls {len(d):5d}  avg {sum(d) / len(d) / 1e3:9.1f} us  "
+                     f"total {sum(d) / 1e3:10.1f} us")
+    return "\n".join(lines)
+
+
 def load_traffic(kernel):
     """Calibrated HBM bytes per launch of ``kernel`` from the committed --pmc summary (None if absent)."""
     try:
@@ -399,6 +432,10 @@ def main():
     ap.add_argument("--cpu-views", type=int, default=16)
     ap.add_argument("--no-refine", action="store_true", help="skip the atlas-refinement workload")
     ap.add_argument("--no-chain", action="store_true", help="skip the single-view textured chain workload")
+    ap.add_argument("--trace-summary", metavar="TRACE_DIR",
+                    help="print the per-kernel table of a rocprofv3 --kernel-trace --output-format csv folder")
+    ap.add_argument("--refine-only", action="store_true",
+                    help="run only the atlas-refinement workload and print its result (for profiling runs)")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="run 3 steps of the headline workload and a 1 GiB device copy (for rocprofv3 --pmc runs)")
     ap.add_argument("--traffic-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
@@ -412,10 +449,16 @@ def main():
                 f"python3 bench.py --traffic-summary {args.traffic_summary[0]} {args.traffic_summary[1]}"]
         print(traffic_summary(args.traffic_summary[0], args.traffic_summary[1], TRAFFIC_PROFILE, cmds))
         return
+    if args.trace_summary:
+        print(trace_summary(args.trace_summary))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU layout "
               "(views sharded across GPUs, one gradient all-reduce per step) is design-only.", file=sys.stderr)
         sys.exit(2)
+    if args.refine_only:
+        print(json.dumps({"refine": refine_benchmark(cpu=not args.no_cpu_baseline)}))
+        return
 
     import ctypes
 

@@ -31,15 +31,28 @@ __device__ __forceinline__ int32_t off_y(uint32_t v) { return static_cast<int16_
 
 __device__ __forceinline__ int32_t clamp_side(int32_t v) { return v < -kSide ? -kSide : (v > kSide ? kSide : v); }
 
-// cur <- neighbour's offset + (dx, dy) when strictly closer (squares fit in uint32: |offset| <= kSide + 1).
-__device__ __forceinline__ void relax(int32_t& cx, int32_t& cy, uint32_t nb, int32_t dx, int32_t dy) {
-    const int32_t x = off_x(nb) + dx, y = off_y(nb) + dy;
-    const uint32_t cand = static_cast<uint32_t>(x * x) + static_cast<uint32_t>(y * y);
-    const uint32_t cur = static_cast<uint32_t>(cx * cx) + static_cast<uint32_t>(cy * cy);
-    if (cand < cur) {
-        cx = clamp_side(x);
-        cy = clamp_side(y);
-    }
+// An offset (x, y) as two int16 lanes: the same bits as pack(x, y).
+typedef int16_t Offset __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ Offset as_offset(uint32_t v) { return __builtin_bit_cast(Offset, v); }
+__device__ __forceinline__ uint32_t as_word(Offset v) { return __builtin_bit_cast(uint32_t, v); }
+// x^2 + y^2 (at most 2 * 30001^2 here, so the int32 dot product is exact)
+__device__ __forceinline__ int32_t len2(Offset v) { return __builtin_amdgcn_sdot2(v, v, 0, false); }
+
+// A neighbour that does not exist: a candidate built from it is never closer
+// than any offset the sweep holds (|components| <= kSide).
+constexpr int16_t kNone16 = 30000;
+
+// cur <- neighbour's offset + d when strictly closer (squared lengths), clamped
+// to +-kSide; cur2 = |cur|^2.  Components stay within int16 (|nb| <= kNone16).
+__device__ __forceinline__ void relax(Offset& cur, int32_t& cur2, Offset nb, Offset d) {
+    const Offset cand = nb + d;
+    const int32_t c2 = len2(cand);
+    const Offset side = {static_cast<int16_t>(kSide), static_cast<int16_t>(kSide)};
+    const Offset clamped = __builtin_elementwise_max(__builtin_elementwise_min(cand, side), -side);
+    const bool take = c2 < cur2;
+    cur = take ? clamped : cur;
+    cur2 = take ? len2(clamped) : cur2;
 }
 
 // Adds a lane count to *counter once per wave (callers loop over a bounded grid,
@@ -88,17 +101,32 @@ __global__ void k_pad_prepare(uint32_t P, const float* __restrict__ valid, const
 // is unrolled by kD).  Every lane issues the same global loads and stores each
 // step (clamped addresses, the discard slot), so the waits for a step's
 // input cover only that load and not the ones issued after it.
+// Word i of an array below 4 GiB, addressed by a 32-bit byte offset.
+__device__ __forceinline__ uint32_t* word_at(uint32_t* base, uint32_t i) {
+    return reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(base) + (i << 2));
+}
+
 template <bool kMirror>
 __device__ __forceinline__ uint32_t sweep_input(const uint8_t* __restrict__ source, const uint32_t* __restrict__ field,
                                                 uint32_t W, uint32_t H, uint32_t x, uint32_t y) {
     x = x < W ? x : W - 1u;
     y = y < H ? y : H - 1u;
     const uint32_t rx = kMirror ? W - 1u - x : x, ry = kMirror ? H - 1u - y : y;
-    const size_t idx = static_cast<size_t>(ry) * W + rx;
+    const uint32_t idx = ry * W + rx;
     if constexpr (kMirror)
-        return field[idx];
+        return *word_at(const_cast<uint32_t*>(field), idx);
     else
         return static_cast<uint32_t>(source[idx]);
+}
+
+// Slot k & 3 of a ring row (k static after unrolling).
+__device__ __forceinline__ uint32_t ring_slot(const uint4& row, int k) {
+    switch (k & 3) {
+        case 0: return row.x;
+        case 1: return row.y;
+        case 2: return row.z;
+        default: return row.w;
+    }
 }
 
 // Next texel of a class walk: along the row, then R rows down.
@@ -111,10 +139,12 @@ __device__ __forceinline__ void class_advance(uint32_t& x, uint32_t& y, uint32_t
 template <int kC, int kD, bool kMirror>
 __global__ __launch_bounds__(kSweepThreads) void k_pad_sweep(uint32_t W, uint32_t H, const uint8_t* __restrict__ source,
                                                               uint32_t* __restrict__ field) {
-    extern __shared__ uint32_t ring[];
+    extern __shared__ uint4 ring_rows[];  // R rows of 4 slots, then a discard word
+    uint32_t* ring = reinterpret_cast<uint32_t*>(ring_rows);
     const uint32_t R = (W + 1u) / 2u;
     const uint32_t steps = W + 2u * (H - 1u);
-    const size_t discard = static_cast<size_t>(W) * H;
+    const uint32_t discard = W * H;
+    const Offset none = {kNone16, kNone16};
     uint32_t px[kC], py[kC];  // texel the class settles next
     uint32_t fx[kC], fy[kC];  // texel whose input the class loads next
     uint32_t in[kC][kD];      // in[j][t % kD]: input of class j's texel of step t
@@ -129,10 +159,11 @@ __global__ __launch_bounds__(kSweepThreads) void k_pad_sweep(uint32_t W, uint32_
             if (c < R && static_cast<uint32_t>(d) >= 2u * c) class_advance(fx[j], fy[j], R);
         }
     }
+    static_assert(kD % 4 == 0, "ring slots are static only for kD a multiple of 4");
     for (uint32_t t0 = 0; t0 < steps; t0 += kD) {
 #pragma unroll
         for (int s = 0; s < kD; ++s) {
-            // steps past the last one (up to kD - 1) settle nothing: stores go to the discard slot
+            // steps past the last one (up to kD - 1) settle nothing: stores go to the discard slots
             const uint32_t t = t0 + static_cast<uint32_t>(s);
 #pragma unroll
             for (int j = 0; j < kC; ++j) {
@@ -140,25 +171,30 @@ __global__ __launch_bounds__(kSweepThreads) void k_pad_sweep(uint32_t W, uint32_
                 const bool live = c < R && t >= 2u * c;
                 const uint32_t x = px[j], y = py[j];
                 const bool settle = live && x < W && y < H;
-                int32_t cx, cy;
-                if (kMirror) {
-                    cx = -off_x(in[j][s]);
-                    cy = -off_y(in[j][s]);
+                // ring rows of this class and of the row above (clamped for idle lanes),
+                // slot k = step k mod 4 (t0 is a multiple of 4, so the slots are static)
+                const uint4 own = ring_rows[c < R ? c : R - 1u];
+                const uint4 up = ring_rows[c == 0 ? R - 1u : (c - 1u < R ? c - 1u : R - 1u)];
+                const Offset left = x > 0 ? as_offset(ring_slot(own, s + 3)) : none;
+                const Offset above = y > 0 ? as_offset(ring_slot(up, s + 2)) : none;
+                const Offset above_left = y > 0 && x > 0 ? as_offset(ring_slot(up, s + 1)) : none;
+                const Offset above_right = y > 0 && x + 1u < W ? as_offset(ring_slot(up, s + 3)) : none;
+                Offset cur;
+                if constexpr (kMirror) {
+                    cur = -as_offset(in[j][s]);
                 } else {
-                    cx = cy = in[j][s] ? 0 : kFar;
+                    const Offset far = {static_cast<int16_t>(kFar), static_cast<int16_t>(kFar)};
+                    const Offset zero = {0, 0};
+                    cur = in[j][s] ? zero : far;
                 }
-                if (settle) {
-                    if (x > 0) relax(cx, cy, ring[c * 4u + ((t - 1u) & 3u)], -1, 0);
-                    if (y > 0) {
-                        const uint32_t pc = c == 0 ? R - 1u : c - 1u;
-                        relax(cx, cy, ring[pc * 4u + ((t - 2u) & 3u)], 0, -1);
-                        if (x > 0) relax(cx, cy, ring[pc * 4u + ((t - 3u) & 3u)], -1, -1);
-                        if (x + 1u < W) relax(cx, cy, ring[pc * 4u + ((t - 1u) & 3u)], 1, -1);
-                    }
-                    ring[c * 4u + (t & 3u)] = pack(cx, cy);
-                }
+                int32_t cur2 = len2(cur);
+                relax(cur, cur2, left, Offset{-1, 0});
+                relax(cur, cur2, above, Offset{0, -1});
+                relax(cur, cur2, above_left, Offset{-1, -1});
+                relax(cur, cur2, above_right, Offset{1, -1});
+                ring[settle ? c * 4u + static_cast<uint32_t>(s & 3) : R * 4u] = as_word(cur);
                 const uint32_t rx = kMirror ? W - 1u - x : x, ry = kMirror ? H - 1u - y : y;
-                field[settle ? static_cast<size_t>(ry) * W + rx : discard] = kMirror ? pack(-cx, -cy) : pack(cx, cy);
+                *word_at(field, settle ? ry * W + rx : discard) = as_word(kMirror ? -cur : cur);
                 if (live) class_advance(px[j], py[j], R);
                 // the input of the class's texel at step t + kD
                 in[j][s] = sweep_input<kMirror>(source, field, W, H, fx[j], fy[j]);
@@ -241,7 +277,7 @@ static hipError_t launch_sweeps(uint32_t W, uint32_t H, const uint8_t* source, u
 }
 
 hipError_t pad_field(uint32_t W, uint32_t H, const uint8_t* source, uint32_t* field, hipStream_t st) {
-    const size_t lds = static_cast<size_t>((W + 1u) / 2u) * 4u * sizeof(uint32_t);
+    const size_t lds = (static_cast<size_t>((W + 1u) / 2u) * 4u + 1u) * sizeof(uint32_t);
     int classes = 1;
     uint32_t threads = 64;
     sweep_shape(W, classes, threads);

@@ -410,7 +410,9 @@ __global__ void k_tv_pairs(const int32_t* __restrict__ chart, uint32_t tw, uint3
             if (y + 1 < th && (chart ? chart[i + tw] : 0) == c) ++n;
         }
     }
-    if (n) atomicAdd(out, n);
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) n += __shfl_xor(n, m, 64);
+    if ((threadIdx.x & 63u) == 0 && n) atomicAdd(out, n);
 }
 
 __device__ __forceinline__ float adam_one(float p, float& g, float& m, float& v, float lr, float b1, float b2,
@@ -469,20 +471,22 @@ __global__ void k_advance_step(uint32_t* step_counter) {
 }
 
 // The step's photometric loss from the residual pass's block partials (fixed
-// order: contiguous chunks per lane, then the lanes in order), then the step
-// counter moves on.  One block of kBlock lanes.
-__global__ __launch_bounds__(kBlock) void k_finish_step(const float* __restrict__ loss_part, uint32_t n,
-                                                        float* __restrict__ loss, uint32_t* step_counter) {
-    __shared__ float chunk[kBlock];
-    const uint32_t per = (n + kBlock - 1) / kBlock;
+// order: contiguous chunks per lane, the 64-lane tree per wave, then the waves
+// in order), then the step counter moves on.  One block of kFinishThreads.
+constexpr uint32_t kFinishThreads = 1024;
+__global__ __launch_bounds__(kFinishThreads) void k_finish_step(const float* __restrict__ loss_part, uint32_t n,
+                                                                float* __restrict__ loss, uint32_t* step_counter) {
+    __shared__ float wsum[kFinishThreads / 64];
+    const uint32_t per = (n + kFinishThreads - 1) / kFinishThreads;
     const uint32_t b = threadIdx.x * per, e = b + per < n ? b + per : n;
     float s = 0.0f;
     for (uint32_t i = b; i < e; ++i) s += loss_part[i];
-    chunk[threadIdx.x] = s;
+    s = wave_tree_sum(s);
+    if ((threadIdx.x & 63u) == 0) wsum[threadIdx.x >> 6] = s;
     __syncthreads();
     if (threadIdx.x == 0) {
         float t = 0.0f;
-        for (uint32_t i = 0; i < kBlock; ++i) t += chunk[i];
+        for (uint32_t w = 0; w < kFinishThreads / 64; ++w) t += wsum[w];
         const uint32_t step = *step_counter;
         loss[step] += t;
         *step_counter = step + 1u;
@@ -690,7 +694,7 @@ hipError_t tv_gradient(const float* tex, uint32_t tw, uint32_t th, const int32_t
 }
 
 hipError_t finish_step(const float* loss_part, uint32_t n, float* loss, uint32_t* step_counter, hipStream_t st) {
-    k_finish_step<<<1, kBlock, 0, st>>>(loss_part, n, loss, step_counter);
+    k_finish_step<<<1, kFinishThreads, 0, st>>>(loss_part, n, loss, step_counter);
     return hipGetLastError();
 }
 
