@@ -79,38 +79,58 @@ __global__ __launch_bounds__(kBlock) void k_interp_fwd(PixelTable pt, const floa
 constexpr uint32_t kVertexSlots = 1024;  // vertex accumulators per tile
 constexpr uint32_t kMaxHashedAttrs = 8;  // attributes per accumulator
 
-// Slot of vertex key (index + 1) in the tile's table (linear probing, at most
-// 32 probes), or kVertexSlots when none is free along the probe sequence.
-__device__ __forceinline__ uint32_t vertex_slot(uint32_t* keys, uint32_t key) {
-    uint32_t h = (key * 2654435761u) >> 22;
-#pragma unroll 1
-    for (int probe = 0; probe < 32; ++probe) {
-        const uint32_t cur = __hip_atomic_load(&keys[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        if (cur == key) return h;
-        if (cur == 0u) {
-            const uint32_t old = atomicCAS(&keys[h], 0u, key);
-            if (old == 0u || old == key) return h;
-        }
-        h = (h + 1) & (kVertexSlots - 1);
+constexpr uint32_t kBucketSlots = 8;  // the table is 128 buckets of 8 slots
+
+// First slot of the bucket of vertex key (index + 1).
+__device__ __forceinline__ uint32_t vertex_bucket(uint32_t key) {
+    return ((key * 2654435761u) >> 25) * kBucketSlots;
+}
+
+// Slot of `key` in its bucket given a snapshot of the bucket's keys (slots
+// only ever go from 0 to a key): the slot holding it, else the first empty
+// slot it wins (or finds taken by the same key), else kVertexSlots.
+__device__ __forceinline__ uint32_t vertex_slot(uint32_t* keys, uint32_t key, uint32_t b, uint4 lo, uint4 hi) {
+    uint32_t k[kBucketSlots] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+#pragma unroll
+    for (uint32_t i = 0; i < kBucketSlots; ++i)
+        if (k[i] == key) return b + i;
+#pragma unroll
+    for (uint32_t i = 0; i < kBucketSlots; ++i) {
+        if (k[i] != 0u) continue;
+        const uint32_t old = atomicCAS(&keys[b + i], 0u, key);
+        if (old == 0u || old == key) return b + i;
     }
     return kVertexSlots;
 }
 
-// Adds a vertex's accumulated terms into the tile table (or global memory when
-// the table has no free slot on its probe sequence).
-__device__ __forceinline__ void flush_vertex(uint32_t* keys, float* acc, float* __restrict__ grad_attr, uint32_t A,
-                                             uint32_t v, const float* val) {
-    const uint32_t slot = vertex_slot(keys, v + 1u);
-    if (slot < kVertexSlots) {
-        float* dst = acc + slot * A;
+// Adds the accumulated terms of up to three vertices (v[e] with on[e]) into
+// the tile table, acc[a * kVertexSlots + slot] (or into global memory for a
+// vertex whose bucket is full).  The three buckets are read together.
+__device__ __forceinline__ void flush_vertices(uint32_t* keys, float* acc, float* __restrict__ grad_attr, uint32_t A,
+                                               const uint32_t (&v)[3], const bool (&on)[3],
+                                               const float (&val)[3][kMaxHashedAttrs]) {
+    uint32_t b[3];
+    uint4 lo[3], hi[3];
 #pragma unroll
-        for (uint32_t a = 0; a < kMaxHashedAttrs; ++a)
-            if (a < A) atomicAdd(&dst[a], val[a]);
-    } else {
-        float* dst = grad_attr + static_cast<size_t>(v) * A;
+    for (int e = 0; e < 3; ++e) {
+        b[e] = vertex_bucket(on[e] ? v[e] + 1u : 1u);
+        lo[e] = *reinterpret_cast<const uint4*>(&keys[b[e]]);
+        hi[e] = *reinterpret_cast<const uint4*>(&keys[b[e] + 4]);
+    }
 #pragma unroll
-        for (uint32_t a = 0; a < kMaxHashedAttrs; ++a)
-            if (a < A) unsafeAtomicAdd(&dst[a], val[a]);
+    for (int e = 0; e < 3; ++e) {
+        if (!on[e]) continue;
+        const uint32_t slot = vertex_slot(keys, v[e] + 1u, b[e], lo[e], hi[e]);
+        if (slot < kVertexSlots) {
+#pragma unroll
+            for (uint32_t a = 0; a < kMaxHashedAttrs; ++a)
+                if (a < A) atomicAdd(&acc[a * kVertexSlots + slot], val[e][a]);
+        } else {
+            float* dst = grad_attr + static_cast<size_t>(v[e]) * A;
+#pragma unroll
+            for (uint32_t a = 0; a < kMaxHashedAttrs; ++a)
+                if (a < A) unsafeAtomicAdd(&dst[a], val[e][a]);
+        }
     }
 }
 
@@ -130,7 +150,7 @@ __global__ __launch_bounds__(kBlock) void k_interp_bwd(PixelTable pt, const floa
                                                        float* __restrict__ grad_attr) {
     constexpr uint32_t kNone = 0xFFFFFFFFu;
     constexpr uint32_t kM = kMaxHashedAttrs;
-    __shared__ uint32_t keys[kVertexSlots];
+    __shared__ __align__(16) uint32_t keys[kVertexSlots];
     __shared__ float acc[kVertexSlots * kMaxHashedAttrs];
     const uint32_t A = kA > 0 ? static_cast<uint32_t>(kA) : A_rt;
     for (uint32_t s = threadIdx.x; s < kVertexSlots; s += kBlock) keys[s] = 0u;
@@ -175,9 +195,9 @@ __global__ __launch_bounds__(kBlock) void k_interp_bwd(PixelTable pt, const floa
                             src[c] = e;
                             used |= 1u << e;
                         }
-#pragma unroll
-                for (int e = 0; e < 3; ++e)
-                    if (!((used >> e) & 1u) && ev[e] != kNone) flush_vertex(keys, acc, grad_attr, A, ev[e], ea[e]);
+                const bool evict[3] = {!(used & 1u) && ev[0] != kNone, !((used >> 1) & 1u) && ev[1] != kNone,
+                                       !((used >> 2) & 1u) && ev[2] != kNone};
+                flush_vertices(keys, acc, grad_attr, A, ev, evict, ea);
                 float na[3][kM];
 #pragma unroll
                 for (int c = 0; c < 3; ++c)
@@ -236,15 +256,14 @@ __global__ __launch_bounds__(kBlock) void k_interp_bwd(PixelTable pt, const floa
         }
         reinterpret_cast<float4*>(pv.out1)[p] = make_float4(gx, gy, 0.0f, 0.0f);
     }
-#pragma unroll
-    for (int e = 0; e < 3; ++e)
-        if (ev[e] != kNone) flush_vertex(keys, acc, grad_attr, A, ev[e], ea[e]);
+    const bool held[3] = {ev[0] != kNone, ev[1] != kNone, ev[2] != kNone};
+    flush_vertices(keys, acc, grad_attr, A, ev, held, ea);
     __syncthreads();
     for (uint32_t s = threadIdx.x; s < kVertexSlots; s += kBlock) {
         const uint32_t key = keys[s];
         if (!key) continue;
         float* dst = grad_attr + static_cast<size_t>(key - 1u) * A;
-        for (uint32_t a = 0; a < A; ++a) unsafeAtomicAdd(&dst[a], acc[s * A + a]);
+        for (uint32_t a = 0; a < A; ++a) unsafeAtomicAdd(&dst[a], acc[a * kVertexSlots + s]);
     }
 }
 
