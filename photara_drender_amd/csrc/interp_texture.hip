@@ -41,39 +41,46 @@ __global__ __launch_bounds__(kBlock) void k_interp_fwd(PixelTable pt, const floa
     const uint32_t A = kA > 0 ? static_cast<uint32_t>(kA) : A_rt;
     const uint32_t gp = blockIdx.x * kBlock + threadIdx.x;
     if (gp >= pt.total_pixels) return;
-    const PixelView& pv = pt.v[find_pixel_view(pt, gp)];
-    const uint32_t p = gp - pv.pixel_base;
-    const float4 r = reinterpret_cast<const float4*>(pv.raster)[p];
-    float* out = pv.out0 + static_cast<size_t>(p) * A;
-    const uint32_t id = decode_id(r.w);
-    uint32_t i0 = 0, i1 = 0, i2 = 0;
-    bool ok = id != 0 && id <= T;
-    if (ok) {
-        i0 = idx[(id - 1) * 3 + 0];
-        i1 = idx[(id - 1) * 3 + 1];
-        i2 = idx[(id - 1) * 3 + 2];
-        ok = i0 < V && i1 < V && i2 < V;
-    }
-    if (!ok) {
-        for (uint32_t a = 0; a < A; ++a) out[a] = 0.0f;
-        return;
-    }
-    const float u = r.x, v = r.y;
-    const float w = 1.0f - u - v;
-    const float* a0 = attr + static_cast<size_t>(i0) * A;
-    const float* a1 = attr + static_cast<size_t>(i1) * A;
-    const float* a2 = attr + static_cast<size_t>(i2) * A;
-    if constexpr (kA == 8) {
-        float o[8];
-#pragma unroll
-        for (int a = 0; a < 8; ++a) o[a] = u * a0[a] + v * a1[a] + w * a2[a];
-        reinterpret_cast<float4*>(out)[0] = make_float4(o[0], o[1], o[2], o[3]);
-        reinterpret_cast<float4*>(out)[1] = make_float4(o[4], o[5], o[6], o[7]);
-    } else if constexpr (kA == 2) {
-        reinterpret_cast<float2*>(out)[0] = make_float2(u * a0[0] + v * a1[0] + w * a2[0], u * a0[1] + v * a1[1] + w * a2[1]);
-    } else {
-        for (uint32_t a = 0; a < A; ++a) out[a] = u * a0[a] + v * a1[a] + w * a2[a];
-    }
+    bool split;
+    const uint32_t kv = wave_view(pt, gp, split);
+    auto body = [&](const PixelView& pv) {
+        const uint32_t p = gp - pv.pixel_base;
+        const float4 r = reinterpret_cast<const float4*>(pv.raster)[p];
+        float* out = pv.out0 + static_cast<size_t>(p) * A;
+        const uint32_t id = decode_id(r.w);
+        uint32_t i0 = 0, i1 = 0, i2 = 0;
+        bool ok = id != 0 && id <= T;
+        if (ok) {
+            i0 = idx[(id - 1) * 3 + 0];
+            i1 = idx[(id - 1) * 3 + 1];
+            i2 = idx[(id - 1) * 3 + 2];
+            ok = i0 < V && i1 < V && i2 < V;
+        }
+        if (!ok) {
+            for (uint32_t a = 0; a < A; ++a) out[a] = 0.0f;
+            return;
+        }
+        const float u = r.x, v = r.y;
+        const float w = 1.0f - u - v;
+        const float* a0 = attr + static_cast<size_t>(i0) * A;
+        const float* a1 = attr + static_cast<size_t>(i1) * A;
+        const float* a2 = attr + static_cast<size_t>(i2) * A;
+        if constexpr (kA == 8) {
+            float o[8];
+    #pragma unroll
+            for (int a = 0; a < 8; ++a) o[a] = u * a0[a] + v * a1[a] + w * a2[a];
+            reinterpret_cast<float4*>(out)[0] = make_float4(o[0], o[1], o[2], o[3]);
+            reinterpret_cast<float4*>(out)[1] = make_float4(o[4], o[5], o[6], o[7]);
+        } else if constexpr (kA == 2) {
+            reinterpret_cast<float2*>(out)[0] = make_float2(u * a0[0] + v * a1[0] + w * a2[0], u * a0[1] + v * a1[1] + w * a2[1]);
+        } else {
+            for (uint32_t a = 0; a < A; ++a) out[a] = u * a0[a] + v * a1[a] + w * a2[a];
+        }
+    };
+    if (!split)
+        body(pt.v[kv]);
+    else
+        body(pt.v[find_pixel_view(pt, gp)]);
 }
 
 constexpr uint32_t kVertexSlots = 1024;  // vertex accumulators per tile
@@ -273,32 +280,39 @@ __global__ __launch_bounds__(kBlock) void k_interp_bwd_direct(PixelTable pt, con
                                                               uint32_t T, float* __restrict__ grad_attr) {
     const uint32_t gp = blockIdx.x * kBlock + threadIdx.x;
     if (gp >= pt.total_pixels) return;
-    const PixelView& pv = pt.v[find_pixel_view(pt, gp)];
-    const uint32_t p = gp - pv.pixel_base;
-    const float4 r = reinterpret_cast<const float4*>(pv.raster)[p];
-    const uint32_t id = decode_id(r.w);
-    const float* gin = pv.in0 + static_cast<size_t>(p) * A;
-    float gx = 0.0f, gy = 0.0f;
-    if (id != 0 && id <= T) {
-        const uint32_t i0 = idx[(id - 1) * 3 + 0], i1 = idx[(id - 1) * 3 + 1], i2 = idx[(id - 1) * 3 + 2];
-        const bool in_range = i0 < V && i1 < V && i2 < V;
-        const float u = r.x, v = r.y;
-        const float w = 1.0f - u - v;
-        for (uint32_t a = 0; a < A; ++a) {
-            const float og = gin[a];
-            if (in_range) {
-                const float a0 = attr[static_cast<size_t>(i0) * A + a];
-                const float a1 = attr[static_cast<size_t>(i1) * A + a];
-                const float a2 = attr[static_cast<size_t>(i2) * A + a];
-                gx += og * (a0 - a2);
-                gy += og * (a1 - a2);
+    bool split;
+    const uint32_t kv = wave_view(pt, gp, split);
+    auto body = [&](const PixelView& pv) {
+        const uint32_t p = gp - pv.pixel_base;
+        const float4 r = reinterpret_cast<const float4*>(pv.raster)[p];
+        const uint32_t id = decode_id(r.w);
+        const float* gin = pv.in0 + static_cast<size_t>(p) * A;
+        float gx = 0.0f, gy = 0.0f;
+        if (id != 0 && id <= T) {
+            const uint32_t i0 = idx[(id - 1) * 3 + 0], i1 = idx[(id - 1) * 3 + 1], i2 = idx[(id - 1) * 3 + 2];
+            const bool in_range = i0 < V && i1 < V && i2 < V;
+            const float u = r.x, v = r.y;
+            const float w = 1.0f - u - v;
+            for (uint32_t a = 0; a < A; ++a) {
+                const float og = gin[a];
+                if (in_range) {
+                    const float a0 = attr[static_cast<size_t>(i0) * A + a];
+                    const float a1 = attr[static_cast<size_t>(i1) * A + a];
+                    const float a2 = attr[static_cast<size_t>(i2) * A + a];
+                    gx += og * (a0 - a2);
+                    gy += og * (a1 - a2);
+                }
+                if (i0 < V) unsafeAtomicAdd(&grad_attr[static_cast<size_t>(i0) * A + a], u * og);
+                if (i1 < V) unsafeAtomicAdd(&grad_attr[static_cast<size_t>(i1) * A + a], v * og);
+                if (i2 < V) unsafeAtomicAdd(&grad_attr[static_cast<size_t>(i2) * A + a], w * og);
             }
-            if (i0 < V) unsafeAtomicAdd(&grad_attr[static_cast<size_t>(i0) * A + a], u * og);
-            if (i1 < V) unsafeAtomicAdd(&grad_attr[static_cast<size_t>(i1) * A + a], v * og);
-            if (i2 < V) unsafeAtomicAdd(&grad_attr[static_cast<size_t>(i2) * A + a], w * og);
         }
-    }
-    reinterpret_cast<float4*>(pv.out1)[p] = make_float4(gx, gy, 0.0f, 0.0f);
+        reinterpret_cast<float4*>(pv.out1)[p] = make_float4(gx, gy, 0.0f, 0.0f);
+    };
+    if (!split)
+        body(pt.v[kv]);
+    else
+        body(pt.v[find_pixel_view(pt, gp)]);
 }
 
 struct Bilinear {
@@ -357,32 +371,39 @@ __global__ __launch_bounds__(kBlock) void k_texture_fwd(PixelTable pt, const flo
     const uint32_t C = kC > 0 ? static_cast<uint32_t>(kC) : C_rt;
     const uint32_t gp = blockIdx.x * kBlock + threadIdx.x;
     if (gp >= pt.total_pixels) return;
-    const PixelView& pv = pt.v[find_pixel_view(pt, gp)];
-    const uint32_t p = gp - pv.pixel_base;
-    float* out = pv.out0 + static_cast<size_t>(p) * C;
-    if (decode_id(pv.raster[static_cast<size_t>(p) * 4 + 3]) == 0) {
-        if constexpr (kC == 4) {
-            reinterpret_cast<float4*>(out)[0] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        } else {
-            for (uint32_t c = 0; c < C; ++c) out[c] = 0.0f;
+    bool split;
+    const uint32_t kv = wave_view(pt, gp, split);
+    auto body = [&](const PixelView& pv) {
+        const uint32_t p = gp - pv.pixel_base;
+        float* out = pv.out0 + static_cast<size_t>(p) * C;
+        if (decode_id(pv.raster[static_cast<size_t>(p) * 4 + 3]) == 0) {
+            if constexpr (kC == 4) {
+                reinterpret_cast<float4*>(out)[0] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            } else {
+                for (uint32_t c = 0; c < C; ++c) out[c] = 0.0f;
+            }
+            return;
         }
-        return;
-    }
-    const float2 uv = reinterpret_cast<const float2*>(pv.in0)[p];
-    const Bilinear b = bilinear_f32(uv.x, uv.y, tw, th, mode);
-    if constexpr (kC == 4) {
-        const float4* T4 = reinterpret_cast<const float4*>(tex);
-        const float4 c00 = T4[b.o00], c10 = T4[b.o10], c01 = T4[b.o01], c11 = T4[b.o11];
-        reinterpret_cast<float4*>(out)[0] =
-            make_float4(c00.x * b.w0 + c10.x * b.w1 + c01.x * b.w2 + c11.x * b.w3,
-                        c00.y * b.w0 + c10.y * b.w1 + c01.y * b.w2 + c11.y * b.w3,
-                        c00.z * b.w0 + c10.z * b.w1 + c01.z * b.w2 + c11.z * b.w3,
-                        c00.w * b.w0 + c10.w * b.w1 + c01.w * b.w2 + c11.w * b.w3);
-    } else {
-        for (uint32_t c = 0; c < C; ++c)
-            out[c] = tex[static_cast<size_t>(b.o00) * C + c] * b.w0 + tex[static_cast<size_t>(b.o10) * C + c] * b.w1 +
-                     tex[static_cast<size_t>(b.o01) * C + c] * b.w2 + tex[static_cast<size_t>(b.o11) * C + c] * b.w3;
-    }
+        const float2 uv = reinterpret_cast<const float2*>(pv.in0)[p];
+        const Bilinear b = bilinear_f32(uv.x, uv.y, tw, th, mode);
+        if constexpr (kC == 4) {
+            const float4* T4 = reinterpret_cast<const float4*>(tex);
+            const float4 c00 = T4[b.o00], c10 = T4[b.o10], c01 = T4[b.o01], c11 = T4[b.o11];
+            reinterpret_cast<float4*>(out)[0] =
+                make_float4(c00.x * b.w0 + c10.x * b.w1 + c01.x * b.w2 + c11.x * b.w3,
+                            c00.y * b.w0 + c10.y * b.w1 + c01.y * b.w2 + c11.y * b.w3,
+                            c00.z * b.w0 + c10.z * b.w1 + c01.z * b.w2 + c11.z * b.w3,
+                            c00.w * b.w0 + c10.w * b.w1 + c01.w * b.w2 + c11.w * b.w3);
+        } else {
+            for (uint32_t c = 0; c < C; ++c)
+                out[c] = tex[static_cast<size_t>(b.o00) * C + c] * b.w0 + tex[static_cast<size_t>(b.o10) * C + c] * b.w1 +
+                         tex[static_cast<size_t>(b.o01) * C + c] * b.w2 + tex[static_cast<size_t>(b.o11) * C + c] * b.w3;
+        }
+    };
+    if (!split)
+        body(pt.v[kv]);
+    else
+        body(pt.v[find_pixel_view(pt, gp)]);
 }
 
 // texture_grad_texture (scatter form) + texture_grad_uv for one view.
@@ -393,39 +414,46 @@ __global__ __launch_bounds__(kBlock) void k_texture_bwd(PixelTable pt, const flo
     const uint32_t C = kC > 0 ? static_cast<uint32_t>(kC) : C_rt;
     const uint32_t gp = blockIdx.x * kBlock + threadIdx.x;
     if (gp >= pt.total_pixels) return;
-    const PixelView& pv = pt.v[find_pixel_view(pt, gp)];
-    const uint32_t p = gp - pv.pixel_base;
-    const float idf = pv.raster[static_cast<size_t>(p) * 4 + 3];
-    const float2 uv = reinterpret_cast<const float2*>(pv.in0)[p];
-    const float* g = pv.in1 + static_cast<size_t>(p) * C;
-    const bool finite = __builtin_isfinite(uv.x) && __builtin_isfinite(uv.y);
-    if (idf != 0.0f && finite) {
-        const Bilinear b = bilinear_f64(uv.x, uv.y, tw, th, mode);
-        for (uint32_t c = 0; c < C; ++c) {
-            const float gv = g[c];
-            unsafeAtomicAdd(&grad_tex[static_cast<size_t>(b.o00) * C + c], b.w0 * gv);
-            unsafeAtomicAdd(&grad_tex[static_cast<size_t>(b.o10) * C + c], b.w1 * gv);
-            unsafeAtomicAdd(&grad_tex[static_cast<size_t>(b.o01) * C + c], b.w2 * gv);
-            unsafeAtomicAdd(&grad_tex[static_cast<size_t>(b.o11) * C + c], b.w3 * gv);
+    bool split;
+    const uint32_t kv = wave_view(pt, gp, split);
+    auto body = [&](const PixelView& pv) {
+        const uint32_t p = gp - pv.pixel_base;
+        const float idf = pv.raster[static_cast<size_t>(p) * 4 + 3];
+        const float2 uv = reinterpret_cast<const float2*>(pv.in0)[p];
+        const float* g = pv.in1 + static_cast<size_t>(p) * C;
+        const bool finite = __builtin_isfinite(uv.x) && __builtin_isfinite(uv.y);
+        if (idf != 0.0f && finite) {
+            const Bilinear b = bilinear_f64(uv.x, uv.y, tw, th, mode);
+            for (uint32_t c = 0; c < C; ++c) {
+                const float gv = g[c];
+                unsafeAtomicAdd(&grad_tex[static_cast<size_t>(b.o00) * C + c], b.w0 * gv);
+                unsafeAtomicAdd(&grad_tex[static_cast<size_t>(b.o10) * C + c], b.w1 * gv);
+                unsafeAtomicAdd(&grad_tex[static_cast<size_t>(b.o01) * C + c], b.w2 * gv);
+                unsafeAtomicAdd(&grad_tex[static_cast<size_t>(b.o11) * C + c], b.w3 * gv);
+            }
         }
-    }
-    if (!compute_uv) return;
-    float2 gu = make_float2(0.0f, 0.0f);
-    if (decode_id(idf) != 0) {
-        const Bilinear b = bilinear_f32(uv.x, uv.y, tw, th, mode);
-        for (uint32_t c = 0; c < C; ++c) {
-            const float c00 = tex[static_cast<size_t>(b.o00) * C + c];
-            const float c10 = tex[static_cast<size_t>(b.o10) * C + c];
-            const float c01 = tex[static_cast<size_t>(b.o01) * C + c];
-            const float c11 = tex[static_cast<size_t>(b.o11) * C + c];
-            const float gv = g[c];
-            const float dx = (1.0f - b.fy) * (c10 - c00) + b.fy * (c11 - c01);
-            const float dy = (1.0f - b.fx) * (c01 - c00) + b.fx * (c11 - c10);
-            gu.x += gv * (dx * static_cast<float>(tw));
-            gu.y += gv * (dy * static_cast<float>(th));
+        if (!compute_uv) return;
+        float2 gu = make_float2(0.0f, 0.0f);
+        if (decode_id(idf) != 0) {
+            const Bilinear b = bilinear_f32(uv.x, uv.y, tw, th, mode);
+            for (uint32_t c = 0; c < C; ++c) {
+                const float c00 = tex[static_cast<size_t>(b.o00) * C + c];
+                const float c10 = tex[static_cast<size_t>(b.o10) * C + c];
+                const float c01 = tex[static_cast<size_t>(b.o01) * C + c];
+                const float c11 = tex[static_cast<size_t>(b.o11) * C + c];
+                const float gv = g[c];
+                const float dx = (1.0f - b.fy) * (c10 - c00) + b.fy * (c11 - c01);
+                const float dy = (1.0f - b.fx) * (c01 - c00) + b.fx * (c11 - c10);
+                gu.x += gv * (dx * static_cast<float>(tw));
+                gu.y += gv * (dy * static_cast<float>(th));
+            }
         }
-    }
-    reinterpret_cast<float2*>(pv.out1)[p] = gu;
+        reinterpret_cast<float2*>(pv.out1)[p] = gu;
+    };
+    if (!split)
+        body(pt.v[kv]);
+    else
+        body(pt.v[find_pixel_view(pt, gp)]);
 }
 
 // src/rasterizer.cpp:198-211 checks, reported as the first offending pixel of
@@ -434,16 +462,23 @@ __global__ __launch_bounds__(kBlock) void k_texture_check_uv(PixelTable pt, uint
                                                              uint32_t* flags) {
     const uint32_t gp = blockIdx.x * kBlock + threadIdx.x;
     if (gp >= pt.total_pixels) return;
-    const PixelView& pv = pt.v[find_pixel_view(pt, gp)];
-    const uint32_t p = gp - pv.pixel_base;
-    if (pv.raster[static_cast<size_t>(p) * 4 + 3] == 0.0f) return;
-    const float2 uv = reinterpret_cast<const float2*>(pv.in0)[p];
-    if (!__builtin_isfinite(uv.x) || !__builtin_isfinite(uv.y)) {
-        atomicMin(&flags[0], gp);
-        return;
-    }
-    const double tx = static_cast<double>(uv.x) * tw - 0.5, ty = static_cast<double>(uv.y) * th - 0.5;
-    if (fabs(tx) > 1.0e9 || fabs(ty) > 1.0e9) atomicMin(&flags[1], gp);
+    bool split;
+    const uint32_t kv = wave_view(pt, gp, split);
+    auto body = [&](const PixelView& pv) {
+        const uint32_t p = gp - pv.pixel_base;
+        if (pv.raster[static_cast<size_t>(p) * 4 + 3] == 0.0f) return;
+        const float2 uv = reinterpret_cast<const float2*>(pv.in0)[p];
+        if (!__builtin_isfinite(uv.x) || !__builtin_isfinite(uv.y)) {
+            atomicMin(&flags[0], gp);
+            return;
+        }
+        const double tx = static_cast<double>(uv.x) * tw - 0.5, ty = static_cast<double>(uv.y) * th - 0.5;
+        if (fabs(tx) > 1.0e9 || fabs(ty) > 1.0e9) atomicMin(&flags[1], gp);
+    };
+    if (!split)
+        body(pt.v[kv]);
+    else
+        body(pt.v[find_pixel_view(pt, gp)]);
 }
 
 uint32_t pixel_grid(uint32_t n) { return (n + kBlock - 1) / kBlock; }

@@ -81,4 +81,17 @@ __device__ __forceinline__ float wave_tree_sum(float v) {
     return v;
 }
 
+// View (index into table.v, ordered by pixel_base) of the first pixel of the
+// calling wave, looked up once per wave with scalar loads; `split` is set when
+// the wave's 64 pixels may reach into a later view (the callers then look the
+// view up per lane).
+template <class Table>
+__device__ __forceinline__ uint32_t wave_view(const Table& t, uint32_t gp, bool& split) {
+    const uint32_t first = __builtin_amdgcn_readfirstlane(gp);
+    uint32_t k = 0;
+    while (k + 1 < t.n && first >= t.v[k + 1].pixel_base) ++k;
+    split = k + 1 < t.n && first + 63u >= t.v[k + 1].pixel_base;
+    return k;
+}
+
 }  // namespace pdr

@@ -450,46 +450,53 @@ __global__ __launch_bounds__(kBlock) void k_raster_resolve(ViewTable vt, const u
                                                            unsigned long long* __restrict__ keys) {
     const uint32_t gp = blockIdx.x * kBlock + threadIdx.x;
     if (gp >= vt.total_pixels) return;
-    const ViewDesc& vd = vt.v[find_view(vt, gp)];
-    const uint32_t p = gp - vd.pixel_base;
-    const unsigned long long key = keys[gp];
-    keys[gp] = ~0ull;
-    float4 r = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    float4 d = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    if (key != ~0ull) {
-        const uint32_t t = static_cast<uint32_t>(key & 0xFFFFFFFFull) - 1u;
-        const float sx = sxt[vd.col_base + p % vd.width], sy = syt[vd.row_base + p / vd.width];
-        const float4* P = reinterpret_cast<const float4*>(vd.pos);
-        const float4 p0 = P[idx[t * 3 + 0]], p1 = P[idx[t * 3 + 1]], p2 = P[idx[t * 3 + 2]];
-        const float q0x = p0.x - sx * p0.w, q0y = p0.y - sy * p0.w;
-        const float q1x = p1.x - sx * p1.w, q1y = p1.y - sy * p1.w;
-        const float q2x = p2.x - sx * p2.w, q2y = p2.y - sy * p2.w;
-        const float a0 = cross2(q1x, q1y, q2x, q2y);
-        const float a1 = cross2(q2x, q2y, q0x, q0y);
-        const float a2 = cross2(q0x, q0y, q1x, q1y);
-        const float inv = 1.0f / (a0 + a1 + a2);
-        const float b0 = clampf(a0 * inv, 0.0f, 1.0f);
-        const float b1 = clampf(a1 * inv, 0.0f, 1.0f);
-        const float iz = p0.z * a0 + p1.z * a1 + p2.z * a2;
-        const float iw = p0.w * a0 + p1.w * a1 + p2.w * a2;
-        r = make_float4(b0, b1, clampf(iz / iw, -1.0f, 1.0f), static_cast<float>(t + 1u));
-        if (want_deriv) {
-            const float da0_dx = -p1.w * p2.y + p2.w * p1.y;
-            const float da0_dy = -p1.x * p2.w + p1.w * p2.x;
-            const float da1_dx = -p2.w * p0.y + p2.y * p0.w;
-            const float da1_dy = -p2.x * p0.w + p2.w * p0.x;
-            const float da2_dx = -p0.w * p1.y + p0.y * p1.w;
-            const float da2_dy = -p0.x * p1.w + p0.w * p1.x;
-            const float dA_dx = da0_dx + da1_dx + da2_dx;
-            const float dA_dy = da0_dy + da1_dy + da2_dy;
-            d.x = (da0_dx - b0 * dA_dx) * inv * 2.0f / vd.vw;
-            d.z = (da1_dx - b1 * dA_dx) * inv * 2.0f / vd.vw;
-            d.y = (da0_dy - b0 * dA_dy) * inv * 2.0f / vd.vh;
-            d.w = (da1_dy - b1 * dA_dy) * inv * 2.0f / vd.vh;
+    bool split;
+    const uint32_t kv = wave_view(vt, gp, split);
+    auto body = [&](const ViewDesc& vd) {
+        const uint32_t p = gp - vd.pixel_base;
+        const unsigned long long key = keys[gp];
+        keys[gp] = ~0ull;
+        float4 r = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        float4 d = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (key != ~0ull) {
+            const uint32_t t = static_cast<uint32_t>(key & 0xFFFFFFFFull) - 1u;
+            const float sx = sxt[vd.col_base + p % vd.width], sy = syt[vd.row_base + p / vd.width];
+            const float4* P = reinterpret_cast<const float4*>(vd.pos);
+            const float4 p0 = P[idx[t * 3 + 0]], p1 = P[idx[t * 3 + 1]], p2 = P[idx[t * 3 + 2]];
+            const float q0x = p0.x - sx * p0.w, q0y = p0.y - sy * p0.w;
+            const float q1x = p1.x - sx * p1.w, q1y = p1.y - sy * p1.w;
+            const float q2x = p2.x - sx * p2.w, q2y = p2.y - sy * p2.w;
+            const float a0 = cross2(q1x, q1y, q2x, q2y);
+            const float a1 = cross2(q2x, q2y, q0x, q0y);
+            const float a2 = cross2(q0x, q0y, q1x, q1y);
+            const float inv = 1.0f / (a0 + a1 + a2);
+            const float b0 = clampf(a0 * inv, 0.0f, 1.0f);
+            const float b1 = clampf(a1 * inv, 0.0f, 1.0f);
+            const float iz = p0.z * a0 + p1.z * a1 + p2.z * a2;
+            const float iw = p0.w * a0 + p1.w * a1 + p2.w * a2;
+            r = make_float4(b0, b1, clampf(iz / iw, -1.0f, 1.0f), static_cast<float>(t + 1u));
+            if (want_deriv) {
+                const float da0_dx = -p1.w * p2.y + p2.w * p1.y;
+                const float da0_dy = -p1.x * p2.w + p1.w * p2.x;
+                const float da1_dx = -p2.w * p0.y + p2.y * p0.w;
+                const float da1_dy = -p2.x * p0.w + p2.w * p0.x;
+                const float da2_dx = -p0.w * p1.y + p0.y * p1.w;
+                const float da2_dy = -p0.x * p1.w + p0.w * p1.x;
+                const float dA_dx = da0_dx + da1_dx + da2_dx;
+                const float dA_dy = da0_dy + da1_dy + da2_dy;
+                d.x = (da0_dx - b0 * dA_dx) * inv * 2.0f / vd.vw;
+                d.z = (da1_dx - b1 * dA_dx) * inv * 2.0f / vd.vw;
+                d.y = (da0_dy - b0 * dA_dy) * inv * 2.0f / vd.vh;
+                d.w = (da1_dy - b1 * dA_dy) * inv * 2.0f / vd.vh;
+            }
         }
-    }
-    reinterpret_cast<float4*>(vd.raster)[p] = r;
-    if (vd.deriv) reinterpret_cast<float4*>(vd.deriv)[p] = d;
+        reinterpret_cast<float4*>(vd.raster)[p] = r;
+        if (vd.deriv) reinterpret_cast<float4*>(vd.deriv)[p] = d;
+    };
+    if (!split)
+        body(vt.v[kv]);
+    else
+        body(vt.v[find_view(vt, gp)]);
 }
 
 // ----------------------------------------------------------------- backward
@@ -509,35 +516,42 @@ __global__ __launch_bounds__(kBlock) void k_bwd_prepass(ViewTable vt, const uint
                                                         uint32_t* __restrict__ view_flags) {
     const uint32_t gp = blockIdx.x * kBlock + threadIdx.x;
     if (gp >= vt.total_pixels) return;
-    const uint32_t view = find_view(vt, gp);
-    const ViewDesc& vd = vt.v[view];
-    const uint32_t p = gp - vd.pixel_base;
-    const uint32_t id = decode_id(vd.raster[static_cast<size_t>(p) * 4 + 3]);
-    const float2 lg = reinterpret_cast<const float2*>(vd.grad_raster)[static_cast<size_t>(p) * 2];
-    uint32_t c = 0;
-    if (id != 0u && id <= T && !(lg.x == 0.0f && lg.y == 0.0f)) {
-        c = id;
-        float4 p0, p1, p2;
-        int32_t bx0, bx1, by0, by1;
-        const int32_t x = static_cast<int32_t>(p % vd.width), y = static_cast<int32_t>(p / vd.width);
-        if (load_corners(vd, idx, id - 1u, V, p0, p1, p2) && backward_box(vd, p0, p1, p2, &bx0, &bx1, &by0, &by1) &&
-            x >= bx0 && x < bx1 && y >= by0 && y < by1) {
-            const float sx = sxt[vd.col_base + x], sy = syt[vd.row_base + y];
-            ScreenTri r;
-            float bb[4];
-            bool inside = screen_coords(p0, p1, p2, 0u, r, bb) && edge_pass(r, sx, sy);
-            if (!inside) {
-                LineSpans sp;
-                if (setup_screen(vd, p0, p1, p2, 0u, false, r, sp) && y >= r.y0 && y <= r.y1) {
-                    int32_t lo, hi;
-                    line_span(sp, sy, span_alpha(vd, 0), span_beta(vd, 0), r.x0, r.x1, &lo, &hi);
-                    inside = x >= lo && x <= hi;
+    bool split;
+    const uint32_t kv = wave_view(vt, gp, split);
+    auto body = [&](uint32_t view) {
+        const ViewDesc& vd = vt.v[view];
+        const uint32_t p = gp - vd.pixel_base;
+        const uint32_t id = decode_id(vd.raster[static_cast<size_t>(p) * 4 + 3]);
+        const float2 lg = reinterpret_cast<const float2*>(vd.grad_raster)[static_cast<size_t>(p) * 2];
+        uint32_t c = 0;
+        if (id != 0u && id <= T && !(lg.x == 0.0f && lg.y == 0.0f)) {
+            c = id;
+            float4 p0, p1, p2;
+            int32_t bx0, bx1, by0, by1;
+            const int32_t x = static_cast<int32_t>(p % vd.width), y = static_cast<int32_t>(p / vd.width);
+            if (load_corners(vd, idx, id - 1u, V, p0, p1, p2) && backward_box(vd, p0, p1, p2, &bx0, &bx1, &by0, &by1) &&
+                x >= bx0 && x < bx1 && y >= by0 && y < by1) {
+                const float sx = sxt[vd.col_base + x], sy = syt[vd.row_base + y];
+                ScreenTri r;
+                float bb[4];
+                bool inside = screen_coords(p0, p1, p2, 0u, r, bb) && edge_pass(r, sx, sy);
+                if (!inside) {
+                    LineSpans sp;
+                    if (setup_screen(vd, p0, p1, p2, 0u, false, r, sp) && y >= r.y0 && y <= r.y1) {
+                        int32_t lo, hi;
+                        line_span(sp, sy, span_alpha(vd, 0), span_beta(vd, 0), r.x0, r.x1, &lo, &hi);
+                        inside = x >= lo && x <= hi;
+                    }
                 }
+                if (!inside) atomicOr(&view_flags[view], 1u);
             }
-            if (!inside) atomicOr(&view_flags[view], 1u);
         }
-    }
-    cid[gp] = c;
+        cid[gp] = c;
+    };
+    if (!split)
+        body(kv);
+    else
+        body(find_view(vt, gp));
 }
 
 // Terms of one pixel (rasterize_backward.hlsl:101-151): false when the pixel's
