@@ -458,6 +458,8 @@ inline void run_raster_backward(pdr_context ctx, const pdr::ViewTable& vt, const
     pdr::RasterScratch sb = s;
     sb.cid = ctx->get_t<uint32_t>("backward_ids", vt.total_pixels);
     sb.view_flags = ctx->get_t<uint32_t>("backward_view_flags", pdr::kMaxViews);
+    sb.won = reinterpret_cast<uint8_t*>(
+        ctx->get_t<uint32_t>("backward_won", (static_cast<size_t>(vt.n) * T + 3) / 4));
     sb.slot_of = csr.slot_of;
     ctx->timed("vertex_csr", st, [&] { hip_check(pdr::build_vertex_csr(idx, T, V, csr, st), "build_vertex_csr"); });
     ctx->timed("bwd_prepass", st,

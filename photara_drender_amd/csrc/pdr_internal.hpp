@@ -61,6 +61,7 @@ struct RasterScratch {
     uint32_t* counters = nullptr;        // [4] queue length, next queue entry
     uint32_t* cid = nullptr;             // [total_pixels] backward: compared id per pixel
     uint32_t* view_flags = nullptr;      // [kMaxViews] backward: view walked over whole boxes
+    uint8_t* won = nullptr;              // [views * T, padded to words] backward: 1 where a pixel's compared id names the triangle
     const uint32_t* slot_of = nullptr;   // [3T] backward: where each corner's gradient is written (VertexCsr)
 };
 

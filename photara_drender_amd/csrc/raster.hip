@@ -31,7 +31,10 @@
 //                     pixel's grad_raster.xy is zero) and a per-view flag when
 //                     a pixel names a triangle whose candidate region does not
 //                     hold it (only possible for a raster not produced by the
-//                     forward; such views are walked over whole boxes);
+//                     forward; such views are walked over whole boxes), and
+//                     a per-(view, triangle) mark of the triangles some pixel
+//                     names (most of an overdrawn view's triangles name none:
+//                     their corner sums are zero and are written without a walk);
 //   k_corner_small    per (view, triangle): the corner gradients summed over
 //                     the reference's screen bounding box (restricted to the
 //                     candidate region, outside which no pixel names the
@@ -513,7 +516,7 @@ __global__ __launch_bounds__(kBlock) void k_raster_resolve(ViewTable vt, const u
 __global__ __launch_bounds__(kBlock) void k_bwd_prepass(ViewTable vt, const uint32_t* __restrict__ idx, uint32_t T,
                                                         uint32_t V, const float* __restrict__ sxt,
                                                         const float* __restrict__ syt, uint32_t* __restrict__ cid,
-                                                        uint32_t* __restrict__ view_flags) {
+                                                        uint32_t* __restrict__ view_flags, uint8_t* __restrict__ won) {
     const uint32_t gp = blockIdx.x * kBlock + threadIdx.x;
     if (gp >= vt.total_pixels) return;
     bool split;
@@ -547,6 +550,10 @@ __global__ __launch_bounds__(kBlock) void k_bwd_prepass(ViewTable vt, const uint
             }
         }
         cid[gp] = c;
+        // mark the named triangle once per run of equal neighbours in the wave
+        const uint32_t mark = c ? view * T + (c - 1u) : 0xFFFFFFFFu;
+        const uint32_t prev = __shfl_up(mark, 1, 64);
+        if (c && ((threadIdx.x & 63u) == 0 || prev != mark)) won[mark] = 1;
     };
     if (!split)
         body(kv);
@@ -679,6 +686,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_corner_small(ViewTable vt, const 
                                                          const float* __restrict__ syt,
                                                          const uint32_t* __restrict__ cid,
                                                          const uint32_t* __restrict__ view_flags,
+                                                         const uint8_t* __restrict__ won,
                                                          const uint32_t* __restrict__ slot_of,
                                                          float4* __restrict__ corner, uint4* __restrict__ queue,
                                                          uint32_t* __restrict__ counters, uint32_t bsz,
@@ -704,7 +712,8 @@ __global__ __launch_bounds__(kBlock, 4) void k_corner_small(ViewTable vt, const 
         int32_t rows = 0;
         {
             BwdRegion g;
-            const bool ok = mine && setup_backward(vd, idx, t, V, full, g);
+            // a triangle no pixel names adds nothing: its corners are zero without a walk
+            const bool ok = mine && won[static_cast<size_t>(view) * T + t] && setup_backward(vd, idx, t, V, full, g);
             bool deferred = false;
             if (mine && !ok) {
 #pragma unroll
@@ -1064,8 +1073,10 @@ hipError_t backward_prepass(const ViewTable& vt, const uint32_t* idx, uint32_t T
                             hipStream_t st) {
     hipError_t e = fill_u32(s.view_flags, 0u, kMaxViews, st);
     if (e != hipSuccess) return e;
+    e = fill_u32(reinterpret_cast<uint32_t*>(s.won), 0u, (static_cast<size_t>(vt.n) * T + 3) / 4, st);
+    if (e != hipSuccess) return e;
     k_bwd_prepass<<<(vt.total_pixels + kBlock - 1) / kBlock, kBlock, 0, st>>>(vt, idx, T, V, s.sx, s.sy, s.cid,
-                                                                             s.view_flags);
+                                                                             s.view_flags, s.won);
     return hipGetLastError();
 }
 
@@ -1101,8 +1112,8 @@ hipError_t corner_gradients(const ViewTable& vt, const uint32_t* idx, uint32_t T
     if (e != hipSuccess) return e;
     const uint32_t bsz = tri_batch(T, vt.n, tp.bwd_waves);
     k_corner_small<<<dim3(tri_grid(T, bsz), vt.n), kBlock, 0, st>>>(vt, idx, T, V, s.sx, s.sy, s.cid, s.view_flags,
-                                                                    s.slot_of, corner, s.queue, s.counters, bsz,
-                                                                    tp.flat_work);
+                                                                    s.won, s.slot_of, corner, s.queue, s.counters,
+                                                                    bsz, tp.flat_work);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     k_corner_large<<<kLargeBlocks, kBlock, 0, st>>>(vt, idx, T, V, s.sx, s.sy, s.cid, s.view_flags, s.slot_of, corner,
                                                     s.queue, s.counters);
