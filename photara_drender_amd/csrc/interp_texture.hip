@@ -266,11 +266,13 @@ __global__ __launch_bounds__(kBlock) void k_interp_bwd(PixelTable pt, const floa
     const bool held[3] = {ev[0] != kNone, ev[1] != kNone, ev[2] != kNone};
     flush_vertices(keys, acc, grad_attr, A, ev, held, ea);
     __syncthreads();
-    for (uint32_t s = threadIdx.x; s < kVertexSlots; s += kBlock) {
+    // one (vertex, attribute) per lane, a vertex's attributes on neighbouring
+    // lanes: each atomic instruction touches 8 vertex rows rather than 64
+    for (uint32_t i = threadIdx.x; i < kVertexSlots * kMaxHashedAttrs; i += kBlock) {
+        const uint32_t s = i / kMaxHashedAttrs, a = i % kMaxHashedAttrs;
         const uint32_t key = keys[s];
-        if (!key) continue;
-        float* dst = grad_attr + static_cast<size_t>(key - 1u) * A;
-        for (uint32_t a = 0; a < A; ++a) unsafeAtomicAdd(&dst[a], acc[a * kVertexSlots + s]);
+        if (!key || a >= A) continue;
+        unsafeAtomicAdd(&grad_attr[static_cast<size_t>(key - 1u) * A + a], acc[a * kVertexSlots + s]);
     }
 }
 
