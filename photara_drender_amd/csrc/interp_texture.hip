@@ -141,8 +141,8 @@ __device__ __forceinline__ void flush_vertices(uint32_t* keys, float* acc, float
     }
 }
 
-// One kGradTile x kGradTile tile per workgroup (A <= kMaxHashedAttrs); each
-// wave walks a 16-row band, one column per lane, top to bottom.
+// One tile of kGradTile columns x 4 bands per workgroup (A <= kMaxHashedAttrs);
+// each wave walks a band of pt.band_rows rows, one column per lane, top to bottom.
 // grad_raster is written per pixel (interpolate_grad_raster.hlsl, exact).  The
 // attribute-gradient terms b_c * g[p, a] (interpolate_grad_attributes.hlsl:
 // 97-110) accumulate in registers per vertex of the lane's current triangle;
@@ -166,8 +166,9 @@ __global__ __launch_bounds__(kBlock) void k_interp_bwd(PixelTable pt, const floa
     const uint32_t g = xcd_remap(blockIdx.x, gridDim.x);
     const PixelView& pv = pt.v[find_tile_view(pt, g)];
     const uint32_t local = g - pv.tile_base;
+    const uint32_t rows = pt.band_rows;
     const uint32_t px = (local % pv.tiles_x) * kGradTile + (threadIdx.x & 63u);
-    const uint32_t band = (local / pv.tiles_x) * kGradTile + (threadIdx.x >> 6) * (kGradTile / (kBlock / 64));
+    const uint32_t band = (local / pv.tiles_x) * (rows * (kBlock / 64)) + (threadIdx.x >> 6) * rows;
     uint32_t cur = 0;                   // current triangle id (raster encoding), 0: none
     uint32_t ev[3] = {kNone, kNone, kNone};  // vertex of each accumulator = corner of cur
     float ea[3][kM];                    // per-corner accumulated terms
@@ -177,7 +178,7 @@ __global__ __launch_bounds__(kBlock) void k_interp_bwd(PixelTable pt, const floa
     for (int c = 0; c < 3; ++c)
 #pragma unroll
         for (uint32_t a = 0; a < kM; ++a) ea[c][a] = av[c][a] = 0.0f;
-    for (uint32_t row = 0; row < kGradTile / (kBlock / 64); ++row) {
+    for (uint32_t row = 0; row < rows; ++row) {
         const uint32_t py = band + row;
         if (px >= pv.width || py >= pv.height) break;
         const size_t p = static_cast<size_t>(py) * pv.width + px;
@@ -502,9 +503,28 @@ hipError_t interp_forward(const PixelTable& pt, const float* attr, uint32_t V, u
     return hipGetLastError();
 }
 
-hipError_t interp_backward(const PixelTable& pt, const float* attr, uint32_t V, uint32_t A, const uint32_t* idx,
+hipError_t interp_backward(const PixelTable& table, const float* attr, uint32_t V, uint32_t A, const uint32_t* idx,
                            uint32_t T, float* grad_attr, hipStream_t st) {
-    if (!pt.total_tiles) return hipSuccess;
+    if (!table.total_tiles) return hipSuccess;
+    // Tiles of kGradTile columns x 4 bands: 16-row bands, halved (down to 2
+    // rows) while that leaves fewer than 2048 tiles, so small views still
+    // spread over the chip.
+    PixelTable pt = table;
+    auto tiles_for = [&](uint32_t rows) {
+        uint64_t n = 0;
+        for (uint32_t k = 0; k < pt.n; ++k)
+            n += static_cast<uint64_t>(pt.v[k].tiles_x) * ((pt.v[k].height + rows * 4 - 1) / (rows * 4));
+        return n;
+    };
+    uint32_t rows = kGradTile / (kBlock / 64);
+    while (rows > 2 && tiles_for(rows) < 2048) rows /= 2;
+    pt.band_rows = rows;
+    uint32_t base = 0;
+    for (uint32_t k = 0; k < pt.n; ++k) {
+        pt.v[k].tile_base = base;
+        base += pt.v[k].tiles_x * ((pt.v[k].height + rows * 4 - 1) / (rows * 4));
+    }
+    pt.total_tiles = base;
     bool aligned = (reinterpret_cast<uintptr_t>(attr) % 16) == 0;
     for (uint32_t k = 0; k < pt.n; ++k) aligned &= (reinterpret_cast<uintptr_t>(pt.v[k].in0) % 16) == 0;
     if (A == 8 && aligned)

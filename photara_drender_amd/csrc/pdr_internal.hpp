@@ -40,7 +40,7 @@ struct PixelView {
     float* out0;           // kernel-specific output [H,W,*]
     float* out1;           // kernel-specific output [H,W,*]
     uint32_t width, height;
-    uint32_t tiles_x;     // kGradTile x kGradTile tiles per row (interpolation backward)
+    uint32_t tiles_x;     // kGradTile-wide tiles per row (interpolation backward)
     uint32_t tile_base;   // first such tile of this view
     uint32_t pixel_base;
 };
@@ -50,6 +50,7 @@ struct PixelTable {
     uint32_t n;
     uint32_t total_tiles;
     uint32_t total_pixels;
+    uint32_t band_rows;  // interpolation backward: rows per wave band (tiles are kGradTile x 4 bands; set by the launcher)
 };
 
 // Scratch of the triangle-parallel raster (owned by the context).
