@@ -409,14 +409,18 @@ __global__ __launch_bounds__(kBlock) void k_texture_fwd(PixelTable pt, const flo
         body(pt.v[find_pixel_view(pt, gp)]);
 }
 
-// texture_grad_texture (scatter form) + texture_grad_uv for one view.
+// texture_grad_texture (scatter form) + texture_grad_uv for one view.  For
+// C = 4 the texel atomics are issued in (pixel, channel) lane order, 16 pixels
+// per instruction: a texel's four channels sit on neighbouring lanes, so an
+// instruction touches a few texel rows instead of one channel of 64 texels.
 template <int kC>
 __global__ __launch_bounds__(kBlock) void k_texture_bwd(PixelTable pt, const float* __restrict__ tex, uint32_t tw,
                                                         uint32_t th, uint32_t C_rt, uint32_t mode, int compute_uv,
                                                         float* __restrict__ grad_tex) {
     const uint32_t C = kC > 0 ? static_cast<uint32_t>(kC) : C_rt;
-    const uint32_t gp = blockIdx.x * kBlock + threadIdx.x;
-    if (gp >= pt.total_pixels) return;
+    const uint32_t gp0 = blockIdx.x * kBlock + threadIdx.x;
+    const bool live = gp0 < pt.total_pixels;  // every lane runs (the C = 4 scatter exchanges across lanes)
+    const uint32_t gp = live ? gp0 : pt.total_pixels - 1u;
     bool split;
     const uint32_t kv = wave_view(pt, gp, split);
     auto body = [&](const PixelView& pv) {
@@ -425,7 +429,34 @@ __global__ __launch_bounds__(kBlock) void k_texture_bwd(PixelTable pt, const flo
         const float2 uv = reinterpret_cast<const float2*>(pv.in0)[p];
         const float* g = pv.in1 + static_cast<size_t>(p) * C;
         const bool finite = __builtin_isfinite(uv.x) && __builtin_isfinite(uv.y);
-        if (idf != 0.0f && finite) {
+        const bool scatter = live && idf != 0.0f && finite;
+        if constexpr (kC == 4) {
+            Bilinear b{};
+            float4 g4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (scatter) {
+                b = bilinear_f64(uv.x, uv.y, tw, th, mode);
+                g4 = make_float4(g[0], g[1], g[2], g[3]);
+            }
+            const uint32_t lane = threadIdx.x & 63u, c = lane & 3u;
+#pragma unroll
+            for (uint32_t q = 0; q < 4; ++q) {
+                const int src = static_cast<int>(q * 16u + (lane >> 2));
+                const int on = __shfl(scatter ? 1 : 0, src, 64);
+                const uint32_t o[4] = {static_cast<uint32_t>(__shfl(static_cast<int>(b.o00), src, 64)),
+                                       static_cast<uint32_t>(__shfl(static_cast<int>(b.o10), src, 64)),
+                                       static_cast<uint32_t>(__shfl(static_cast<int>(b.o01), src, 64)),
+                                       static_cast<uint32_t>(__shfl(static_cast<int>(b.o11), src, 64))};
+                const float w[4] = {__shfl(b.w0, src, 64), __shfl(b.w1, src, 64), __shfl(b.w2, src, 64),
+                                    __shfl(b.w3, src, 64)};
+                const float gx = __shfl(g4.x, src, 64), gy = __shfl(g4.y, src, 64);
+                const float gz = __shfl(g4.z, src, 64), gw = __shfl(g4.w, src, 64);
+                const float gv = c == 0 ? gx : (c == 1 ? gy : (c == 2 ? gz : gw));
+                if (on) {
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) unsafeAtomicAdd(&grad_tex[static_cast<size_t>(o[k]) * 4 + c], w[k] * gv);
+                }
+            }
+        } else if (scatter) {
             const Bilinear b = bilinear_f64(uv.x, uv.y, tw, th, mode);
             for (uint32_t c = 0; c < C; ++c) {
                 const float gv = g[c];
@@ -435,7 +466,7 @@ __global__ __launch_bounds__(kBlock) void k_texture_bwd(PixelTable pt, const flo
                 unsafeAtomicAdd(&grad_tex[static_cast<size_t>(b.o11) * C + c], b.w3 * gv);
             }
         }
-        if (!compute_uv) return;
+        if (!compute_uv || !live) return;
         float2 gu = make_float2(0.0f, 0.0f);
         if (decode_id(idf) != 0) {
             const Bilinear b = bilinear_f32(uv.x, uv.y, tw, th, mode);
