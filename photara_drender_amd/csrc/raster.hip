@@ -882,6 +882,7 @@ __global__ __launch_bounds__(kBlock) void k_corner_large(ViewTable vt, const uin
     __shared__ uint32_t entry;
     __shared__ float red[9][kBlock];
     const uint32_t count = counters[0];
+    if (blockIdx.x >= count) return;  // more workgroups than queued boxes
     while (true) {
         if (threadIdx.x == 0) entry = atomicAdd(&counters[1], 1u);
         __syncthreads();
