@@ -5,11 +5,9 @@
 // upper-left and upper-right neighbours, then the mirrored backward sweep.
 // A texel's forward value depends on its left neighbour (same sweep) and on
 // three texels of the row above, so texel (x, y) can be settled at step
-// t = x + 2 y: the sweep runs as a wavefront of W + 2 (H - 1) steps in one
-// workgroup, one row per lane and step (rows y, y + R, y + 2 R, ... share a
-// lane, R = ceil(W / 2), and are never active together).  Each row keeps its
-// last four settled texels in an LDS ring, which is everything the next row
-// reads.  The backward sweep is the forward sweep on the mirrored grid with
+// t = x + 2 y: the sweep runs as a wavefront.  Rows go in bands of 64 to
+// single-wave workgroups (lane = row), each band pipelined behind the band
+// above.  The backward sweep is the forward sweep on the mirrored grid with
 // negated offsets (same relaxation order, same squared lengths), so results
 // equal the sequential sweeps exactly.  Preparation and the fill are one
 // texel per lane over the whole grid.
@@ -20,18 +18,12 @@ namespace {
 
 constexpr int32_t kFar = 12000;   // offset of "no source yet"
 constexpr int32_t kSide = 16384;  // offsets are clamped to +-kSide (int16 storage)
-constexpr uint32_t kSweepThreads = 1024;
 constexpr uint32_t kBlock = 256;
 
-__device__ __forceinline__ uint32_t pack(int32_t x, int32_t y) {
-    return (static_cast<uint32_t>(x) & 0xFFFFu) | (static_cast<uint32_t>(y) << 16);
-}
 __device__ __forceinline__ int32_t off_x(uint32_t v) { return static_cast<int16_t>(v & 0xFFFFu); }
 __device__ __forceinline__ int32_t off_y(uint32_t v) { return static_cast<int16_t>(v >> 16); }
 
-__device__ __forceinline__ int32_t clamp_side(int32_t v) { return v < -kSide ? -kSide : (v > kSide ? kSide : v); }
-
-// An offset (x, y) as two int16 lanes: the same bits as pack(x, y).
+// An offset (x, y) as two int16 lanes: x in the low half of the word, y in the high half.
 typedef int16_t Offset __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ Offset as_offset(uint32_t v) { return __builtin_bit_cast(Offset, v); }
@@ -63,14 +55,6 @@ __device__ __forceinline__ void count_total(unsigned long long* counter, uint32_
     if ((threadIdx.x & 63u) == 0 && n) atomicAdd(counter, static_cast<unsigned long long>(n));
 }
 
-// Step barrier of the sweep: LDS writes complete, then the workgroup barrier.
-// Global stores of earlier steps may stay in flight (no step reads them).
-__device__ __forceinline__ void lds_step_barrier() {
-    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
-    __builtin_amdgcn_s_barrier();
-    asm volatile("" ::: "memory");
-}
-
 // Sources (:259-276): valid texels and earlier covered fills; other texels lose their fill flag.
 // counters: valid, gutter, unobserved, remaining, filled.
 __global__ void k_pad_prepare(uint32_t P, const float* __restrict__ valid, const float* __restrict__ coverage,
@@ -90,126 +74,121 @@ __global__ void k_pad_prepare(uint32_t P, const float* __restrict__ valid, const
     count_total(counters + 4, nrefill);
 }
 
-// One 8SSEDT sweep as a wavefront; kMirror = the backward sweep.  field
-// holds packed int16 offsets: read as the initial state of the backward
-// sweep, written by both; field[W * H] is a discard slot.  ring: R x 4
-// settled offsets (dynamic LDS).  Each lane walks kC row classes
-// c = lane + j * blockDim.x; a class settles (x, y) at step 2 c + x + 2 R k
-// for its k-th row y = c + k R.  A step's only global load is the input of a
-// texel, which does not depend on the sweep: each class loads it kD steps
-// ahead into a register ring indexed by the step (static slots: the step loop
-// is unrolled by kD).  Every lane issues the same global loads and stores each
-// step (clamped addresses, the discard slot), so the waits for a step's
-// input cover only that load and not the ones issued after it.
 // Word i of an array below 4 GiB, addressed by a 32-bit byte offset.
 __device__ __forceinline__ uint32_t* word_at(uint32_t* base, uint32_t i) {
     return reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(base) + (i << 2));
 }
 
+// One 8SSEDT sweep; kMirror = the backward sweep.  field holds packed int16
+// offsets: read as the initial state of the backward sweep, written by both;
+// field[W * H] is a discard slot.  Rows in bands of 64, one single-wave
+// workgroup per band, lane i = row 64 b + i settling column x = t - 2 i at its
+// step t.  Inside a band the row above is the neighbouring lane: its last
+// three results come through one shift by a lane per step (no barrier).  Lane
+// 0 reads the row above the band from `edge`, which the band above publishes
+// every kChunk steps: its last row's results (agent-scope stores), then the
+// number of columns done (a release store of progress[b]); the band below
+// waits with acquire loads before each chunk.  A wait gives up after a bounded
+// number of polls (status[0] = 1, and every later wait returns at once), so a
+// launch always ends.  Inputs are loaded kChunk steps ahead into a register
+// ring; every lane issues the same load and store each step (clamped
+// addresses, the discard slot).
+constexpr uint32_t kBandRows = 64;
+constexpr uint32_t kChunk = 32;
+constexpr uint32_t kMaxPolls = 1u << 20;
+
 template <bool kMirror>
-__device__ __forceinline__ uint32_t sweep_input(const uint8_t* __restrict__ source, const uint32_t* __restrict__ field,
-                                                uint32_t W, uint32_t H, uint32_t x, uint32_t y) {
-    x = x < W ? x : W - 1u;
-    y = y < H ? y : H - 1u;
-    const uint32_t rx = kMirror ? W - 1u - x : x, ry = kMirror ? H - 1u - y : y;
-    const uint32_t idx = ry * W + rx;
-    if constexpr (kMirror)
-        return *word_at(const_cast<uint32_t*>(field), idx);
-    else
-        return static_cast<uint32_t>(source[idx]);
-}
-
-// Slot k & 3 of a ring row (k static after unrolling).
-__device__ __forceinline__ uint32_t ring_slot(const uint4& row, int k) {
-    switch (k & 3) {
-        case 0: return row.x;
-        case 1: return row.y;
-        case 2: return row.z;
-        default: return row.w;
-    }
-}
-
-// Next texel of a class walk: along the row, then R rows down.
-__device__ __forceinline__ void class_advance(uint32_t& x, uint32_t& y, uint32_t R) {
-    const bool wrap = x + 1u == 2u * R;
-    x = wrap ? 0u : x + 1u;
-    y = wrap ? y + R : y;
-}
-
-template <int kC, int kD, bool kMirror>
-__global__ __launch_bounds__(kSweepThreads) void k_pad_sweep(uint32_t W, uint32_t H, const uint8_t* __restrict__ source,
-                                                              uint32_t* __restrict__ field) {
-    extern __shared__ uint4 ring_rows[];  // R rows of 4 slots, then a discard word
-    uint32_t* ring = reinterpret_cast<uint32_t*>(ring_rows);
-    const uint32_t R = (W + 1u) / 2u;
-    const uint32_t steps = W + 2u * (H - 1u);
-    const uint32_t discard = W * H;
+__global__ __launch_bounds__(64) void k_pad_band_sweep(uint32_t W, uint32_t H, const uint8_t* __restrict__ source,
+                                                        uint32_t* __restrict__ field, uint32_t* progress,
+                                                        uint32_t* edge, uint32_t* status) {
+    const uint32_t band = blockIdx.x, lane = threadIdx.x;
+    const uint32_t y = band * kBandRows + lane;  // row in sweep (possibly mirrored) coordinates
+    const uint32_t rows = H - band * kBandRows < kBandRows ? H - band * kBandRows : kBandRows;
+    const uint32_t steps = W + 2u * (rows - 1u);
+    const bool publish = band * kBandRows + kBandRows < H;  // a band below reads this band's last row
+    const uint32_t ry = kMirror ? H - 1u - (y < H ? y : H - 1u) : (y < H ? y : H - 1u);
     const Offset none = {kNone16, kNone16};
-    uint32_t px[kC], py[kC];  // texel the class settles next
-    uint32_t fx[kC], fy[kC];  // texel whose input the class loads next
-    uint32_t in[kC][kD];      // in[j][t % kD]: input of class j's texel of step t
+    uint32_t* my_edge = edge + static_cast<size_t>(band) * W;
+    const uint32_t* up_edge = edge + static_cast<size_t>(band ? band - 1u : 0u) * W;
+    // inputs of this lane's texels at the steps of the current and the next chunk
+    uint32_t in[kChunk];
+    auto load_input = [&](uint32_t x) {
+        const uint32_t cx = x < W ? x : W - 1u;
+        const uint32_t rx = kMirror ? W - 1u - cx : cx;
+        const uint32_t idx = ry * W + rx;
+        return kMirror ? *word_at(field, idx) : static_cast<uint32_t>(source[idx]);
+    };
 #pragma unroll
-    for (int j = 0; j < kC; ++j) {
-        const uint32_t c = threadIdx.x + static_cast<uint32_t>(j) * blockDim.x;
-        px[j] = fx[j] = 0;
-        py[j] = fy[j] = c;
-#pragma unroll
-        for (int d = 0; d < kD; ++d) {
-            in[j][d] = sweep_input<kMirror>(source, field, W, H, fx[j], fy[j]);
-            if (c < R && static_cast<uint32_t>(d) >= 2u * c) class_advance(fx[j], fy[j], R);
-        }
-    }
-    static_assert(kD % 4 == 0, "ring slots are static only for kD a multiple of 4");
-    for (uint32_t t0 = 0; t0 < steps; t0 += kD) {
-#pragma unroll
-        for (int s = 0; s < kD; ++s) {
-            // steps past the last one (up to kD - 1) settle nothing: stores go to the discard slots
-            const uint32_t t = t0 + static_cast<uint32_t>(s);
-#pragma unroll
-            for (int j = 0; j < kC; ++j) {
-                const uint32_t c = threadIdx.x + static_cast<uint32_t>(j) * blockDim.x;
-                const bool live = c < R && t >= 2u * c;
-                const uint32_t x = px[j], y = py[j];
-                const bool settle = live && x < W && y < H;
-                // ring rows of this class and of the row above (clamped for idle lanes),
-                // slot k = step k mod 4 (t0 is a multiple of 4, so the slots are static)
-                const uint4 own = ring_rows[c < R ? c : R - 1u];
-                const uint4 up = ring_rows[c == 0 ? R - 1u : (c - 1u < R ? c - 1u : R - 1u)];
-                const Offset left = x > 0 ? as_offset(ring_slot(own, s + 3)) : none;
-                const Offset above = y > 0 ? as_offset(ring_slot(up, s + 2)) : none;
-                const Offset above_left = y > 0 && x > 0 ? as_offset(ring_slot(up, s + 1)) : none;
-                const Offset above_right = y > 0 && x + 1u < W ? as_offset(ring_slot(up, s + 3)) : none;
-                Offset cur;
-                if constexpr (kMirror) {
-                    cur = -as_offset(in[j][s]);
-                } else {
-                    const Offset far = {static_cast<int16_t>(kFar), static_cast<int16_t>(kFar)};
-                    const Offset zero = {0, 0};
-                    cur = in[j][s] ? zero : far;
+    for (uint32_t k = 0; k < kChunk; ++k) in[k] = load_input(k - 2u * lane);  // x of step k (wraps: clamped)
+    Offset prev = none;                      // this row's previous result (left neighbour)
+    Offset up1 = none, up2 = none, up3 = none;  // row above at columns x + 1, x, x - 1
+    uint32_t edge_vals = 0u;                 // lanes 0..kChunk+1: row above the band, columns t0 - 1 + lane
+    for (uint32_t t0 = 0; t0 < steps; t0 += kChunk) {
+        if (band > 0) {
+            // columns up to t0 + kChunk of the row above are needed in this chunk
+            const uint32_t need = t0 + kChunk + 1u < W ? t0 + kChunk + 1u : W;
+            if (lane == 0) {
+                uint32_t polls = 0;
+                while (__hip_atomic_load(&progress[band - 1u], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < need) {
+                    if (++polls >= kMaxPolls ||
+                        __hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
+                        __hip_atomic_store(status, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(2);
                 }
-                int32_t cur2 = len2(cur);
-                relax(cur, cur2, left, Offset{-1, 0});
-                relax(cur, cur2, above, Offset{0, -1});
-                relax(cur, cur2, above_left, Offset{-1, -1});
-                relax(cur, cur2, above_right, Offset{1, -1});
-                ring[settle ? c * 4u + static_cast<uint32_t>(s & 3) : R * 4u] = as_word(cur);
-                const uint32_t rx = kMirror ? W - 1u - x : x, ry = kMirror ? H - 1u - y : y;
-                *word_at(field, settle ? ry * W + rx : discard) = as_word(kMirror ? -cur : cur);
-                if (live) class_advance(px[j], py[j], R);
-                // the input of the class's texel at step t + kD
-                in[j][s] = sweep_input<kMirror>(source, field, W, H, fx[j], fy[j]);
-                if (c < R && t + static_cast<uint32_t>(kD) >= 2u * c) class_advance(fx[j], fy[j], R);
             }
-            lds_step_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            const int32_t col = static_cast<int32_t>(t0) - 1 + static_cast<int32_t>(lane);
+            edge_vals = col >= 0 && col < static_cast<int32_t>(W) && lane < kChunk + 2u
+                            ? __hip_atomic_load(&up_edge[col], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                            : as_word(none);
+        }
+#pragma unroll
+        for (uint32_t s = 0; s < kChunk; ++s) {
+            const uint32_t t = t0 + s;
+            const int32_t x = static_cast<int32_t>(t) - 2 * static_cast<int32_t>(lane);
+            const bool settle = x >= 0 && x < static_cast<int32_t>(W) && y < H;
+            if (lane == 0 && band > 0) {
+                // row above the band at columns x + 1, x, x - 1 (x = t for lane 0)
+                up1 = as_offset(__builtin_amdgcn_readlane(edge_vals, static_cast<int>(s + 2)));
+                up2 = as_offset(__builtin_amdgcn_readlane(edge_vals, static_cast<int>(s + 1)));
+                up3 = as_offset(__builtin_amdgcn_readlane(edge_vals, static_cast<int>(s)));
+            }
+            Offset cur;
+            if constexpr (kMirror) {
+                cur = -as_offset(in[s]);
+            } else {
+                const Offset far = {static_cast<int16_t>(kFar), static_cast<int16_t>(kFar)};
+                const Offset zero = {0, 0};
+                cur = in[s] ? zero : far;
+            }
+            const bool has_up = y > 0;
+            int32_t cur2 = len2(cur);
+            relax(cur, cur2, x > 0 ? prev : none, Offset{-1, 0});
+            relax(cur, cur2, has_up ? up2 : none, Offset{0, -1});
+            relax(cur, cur2, has_up && x > 0 ? up3 : none, Offset{-1, -1});
+            relax(cur, cur2, has_up && x + 1 < static_cast<int32_t>(W) ? up1 : none, Offset{1, -1});
+            const Offset res = settle ? cur : none;
+            const uint32_t rx = kMirror ? W - 1u - static_cast<uint32_t>(x) : static_cast<uint32_t>(x);
+            *word_at(field, settle ? ry * W + rx : W * H) = as_word(kMirror ? -cur : cur);
+            if (publish && lane == kBandRows - 1u && settle)
+                __hip_atomic_store(&my_edge[x], as_word(cur), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            // the next step's neighbours: own result on the left, the lane above's results above
+            prev = res;
+            up3 = up2;
+            up2 = up1;
+            up1 = as_offset(static_cast<uint32_t>(__shfl_up(static_cast<int>(as_word(res)), 1, 64)));
+            in[s] = load_input(static_cast<uint32_t>(x + static_cast<int32_t>(kChunk)));
+        }
+        if (publish && lane == kBandRows - 1u) {
+            // columns of the last row settled so far (step t settles column t - 2 * 63)
+            const int32_t done = static_cast<int32_t>(t0 + kChunk) - 2 * static_cast<int32_t>(kBandRows - 1u);
+            const uint32_t cols = done <= 0 ? 0u : (static_cast<uint32_t>(done) < W ? static_cast<uint32_t>(done) : W);
+            __hip_atomic_store(&progress[band], cols, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
-}
-
-// Row classes per lane and lanes of the sweep for a W-wide grid.
-void sweep_shape(uint32_t W, int& classes, uint32_t& threads) {
-    const uint32_t R = (W + 1u) / 2u;
-    classes = R <= kSweepThreads ? 1 : (R <= 2 * kSweepThreads ? 2 : (R <= 4 * kSweepThreads ? 4 : 8));
-    threads = classes == 1 ? (R + 63u) / 64u * 64u : kSweepThreads;
+    if (publish && lane == kBandRows - 1u) __hip_atomic_store(&progress[band], W, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Fills (:289-348).  Sources are never written; colours are read from `src_color`
@@ -260,31 +239,22 @@ hipError_t pad_prepare(uint32_t P, const float* valid, const float* coverage, fl
     return hipGetLastError();
 }
 
-template <int kC, int kD>
-static hipError_t launch_sweeps(uint32_t W, uint32_t H, const uint8_t* source, uint32_t* field, uint32_t threads,
-                                size_t lds, hipStream_t st) {
-    if (lds > 64u * 1024u) {
-        for (const void* k : {reinterpret_cast<const void*>(&k_pad_sweep<kC, kD, false>),
-                              reinterpret_cast<const void*>(&k_pad_sweep<kC, kD, true>)}) {
-            const hipError_t e =
-                hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
-            if (e != hipSuccess) return e;
-        }
-    }
-    k_pad_sweep<kC, kD, false><<<1, threads, lds, st>>>(W, H, source, field);
-    k_pad_sweep<kC, kD, true><<<1, threads, lds, st>>>(W, H, source, field);
-    return hipGetLastError();
-}
+uint32_t pad_band_count(uint32_t H) { return (H + kBandRows - 1u) / kBandRows; }
 
-hipError_t pad_field(uint32_t W, uint32_t H, const uint8_t* source, uint32_t* field, hipStream_t st) {
-    const size_t lds = (static_cast<size_t>((W + 1u) / 2u) * 4u + 1u) * sizeof(uint32_t);
-    int classes = 1;
-    uint32_t threads = 64;
-    sweep_shape(W, classes, threads);
-    if (classes == 1) return launch_sweeps<1, 8>(W, H, source, field, threads, lds, st);
-    if (classes == 2) return launch_sweeps<2, 8>(W, H, source, field, threads, lds, st);
-    if (classes == 4) return launch_sweeps<4, 8>(W, H, source, field, threads, lds, st);
-    return launch_sweeps<8, 4>(W, H, source, field, threads, lds, st);
+hipError_t pad_field_bands(uint32_t W, uint32_t H, const uint8_t* source, uint32_t* field, uint32_t* progress,
+                           uint32_t* edge, uint32_t* status, hipStream_t st) {
+    const uint32_t bands = pad_band_count(H);
+    hipError_t e = fill_u32(status, 0u, 1, st);
+    if (e != hipSuccess) return e;
+    for (int mirror = 0; mirror < 2; ++mirror) {
+        if ((e = fill_u32(progress, 0u, bands, st)) != hipSuccess) return e;
+        if (mirror)
+            k_pad_band_sweep<true><<<bands, 64, 0, st>>>(W, H, source, field, progress, edge, status);
+        else
+            k_pad_band_sweep<false><<<bands, 64, 0, st>>>(W, H, source, field, progress, edge, status);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 hipError_t pad_fill(uint32_t W, uint32_t H, uint32_t C, const uint32_t* field, const uint8_t* source,

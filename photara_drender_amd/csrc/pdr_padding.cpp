@@ -65,7 +65,18 @@ pdr_status pdr_pad_texture_atlas(pdr_context ctx, float* color, uint32_t width, 
         hip_check(hipMemcpyAsync(c, counters, sizeof(c), hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
         sync(st);
         if (c[0] != 0 || c[4] != 0) {  // :277-281: nothing to extrapolate from otherwise
-            ctx->timed("pad_field", st, [&] { hip_check(pdr::pad_field(W, H, source, field, st), "pad_field"); });
+            // 8SSEDT field: rows in bands of 64 on separate workgroups, each waiting for the band above
+            const uint32_t bands = pdr::pad_band_count(H);
+            uint32_t* progress = ctx->get_t<uint32_t>("pad_band_progress", bands);
+            uint32_t* edge = ctx->get_t<uint32_t>("pad_band_edge", static_cast<size_t>(bands) * W);
+            uint32_t* status = ctx->get_t<uint32_t>("pad_band_status", 1);
+            ctx->timed("pad_field", st, [&] {
+                hip_check(pdr::pad_field_bands(W, H, source, field, progress, edge, status, st), "pad_field_bands");
+            });
+            uint32_t gave_up = 0;
+            hip_check(hipMemcpyAsync(&gave_up, status, sizeof(gave_up), hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
+            sync(st);
+            if (gave_up) raise(PDR_RUNTIME, "texture padding: a row band gave up waiting for the band above");
             // A far sentinel offset can land inside atlases wider or taller than it: read colours from a copy then.
             const float* src_color = dcolor;
             if (W > 12000u || H > 12000u) {
