@@ -93,13 +93,15 @@ __device__ __forceinline__ bool bake_sample(const BakeView& v, float x, float y,
     const float px = v.vp[0] + (ndx * 0.5f + 0.5f) * v.vp[2];
     const float py = v.vp[1] + (ndy * 0.5f + 0.5f) * v.vp[3];
     if (px < v.vp[0] || py < v.vp[1] || px >= v.vp[0] + v.vp[2] || py >= v.vp[1] + v.vp[3]) return false;
-    float vis = bake_shadow(v, px, py, ndz, pcf);
-    if (v.mask) vis *= clampf(bake_mask(v, px, py), 0.0f, 1.0f);
+    // the view-angle test first: a texel facing away needs no shadow or mask reads
     const float dx = v.cam[0] - x, dy = v.cam[1] - y, dz = v.cam[2] - z;
     const float dl = sqrtf(dx * dx + dy * dy + dz * dz);
     const float vx = dx / dl, vy = dy / dl, vz = dz / dl;
     const float cosv = nx * vx + ny * vy + nz * vz;
-    if (cosv < v.min_cos || vis <= 0.0f) return false;
+    if (cosv < v.min_cos) return false;
+    float vis = bake_shadow(v, px, py, ndz, pcf);
+    if (v.mask) vis *= clampf(bake_mask(v, px, py), 0.0f, 1.0f);
+    if (vis <= 0.0f) return false;
     s.vis = vis;
     s.cosv = cosv;
     s.dl = dl;
