@@ -24,20 +24,37 @@ __global__ void k_atlas_clip(const float* __restrict__ uv, uint32_t V, float4* _
 // lerp(a, b, t) = a + t * (b - a)
 __device__ __forceinline__ float bake_lerp(float a, float b, float t) { return a + t * (b - a); }
 
-// project_texture.hlsl:36-73: bilinear sample, coordinates clamped to the image.
-__device__ __forceinline__ float bake_photo(const BakeView& v, float px, float py, uint32_t ch) {
+// project_texture.hlsl:36-73: bilinear sample, coordinates clamped to the image,
+// of every channel at once (a tap's channels are adjacent in memory and are
+// loaded together); alpha = 1 for a 3-channel photo.
+__device__ __forceinline__ void bake_photo_all(const BakeView& v, float px, float py, float out[4]) {
     const float tx = px - 0.5f, ty = py - 0.5f;
     const float flx = floorf(tx), fly = floorf(ty);
     const int32_t bx = f2i_sat(flx), by = f2i_sat(fly);
     const float fx = tx - flx, fy = ty - fly;
     const int32_t wm = static_cast<int32_t>(v.width) - 1, hm = static_cast<int32_t>(v.height) - 1;
-    auto at = [&](int32_t x, int32_t y) {
+    auto tap = [&](int32_t x, int32_t y) {
         x = x < 0 ? 0 : (x > wm ? wm : x);
         y = y < 0 ? 0 : (y > hm ? hm : y);
-        return v.photo[(static_cast<size_t>(y) * v.width + x) * v.channels + ch];
+        return v.photo + (static_cast<size_t>(y) * v.width + x) * v.channels;
     };
-    return bake_lerp(bake_lerp(at(bx, by), at(add1(bx), by), fx), bake_lerp(at(bx, add1(by)), at(add1(bx), add1(by)), fx),
-                     fy);
+    const float* t00 = tap(bx, by);
+    const float* t10 = tap(add1(bx), by);
+    const float* t01 = tap(bx, add1(by));
+    const float* t11 = tap(add1(bx), add1(by));
+    const uint32_t nc = v.channels == 4 ? 4u : 3u;
+    float a[4][4];
+#pragma unroll
+    for (uint32_t c = 0; c < 4; ++c) {
+        if (c >= nc) break;
+        a[0][c] = t00[c];
+        a[1][c] = t10[c];
+        a[2][c] = t01[c];
+        a[3][c] = t11[c];
+    }
+#pragma unroll
+    for (uint32_t c = 0; c < 4; ++c)
+        out[c] = c < nc ? bake_lerp(bake_lerp(a[0][c], a[1][c], fx), bake_lerp(a[2][c], a[3][c], fx), fy) : 1.0f;
 }
 
 __device__ __forceinline__ float bake_mask(const BakeView& v, float px, float py) {
@@ -65,9 +82,9 @@ __device__ __forceinline__ float bake_shadow(const BakeView& v, float px, float 
         for (int32_t x = -r; x <= r; ++x) {
             const int32_t qx = cx + x, qy = cy + y;
             if (qx < 0 || qy < 0 || qx >= static_cast<int32_t>(v.width) || qy >= static_cast<int32_t>(v.height)) continue;
-            const float4 s = reinterpret_cast<const float4*>(v.shadow)[static_cast<size_t>(qy) * v.width + qx];
-            if (decode_id(s.w) != 0u) {
-                visible += depth <= s.z + v.depth_bias ? 1.0f : 0.0f;
+            const float2 s = reinterpret_cast<const float2*>(v.shadow)[(static_cast<size_t>(qy) * v.width + qx) * 2 + 1];
+            if (decode_id(s.y) != 0u) {
+                visible += depth <= s.x + v.depth_bias ? 1.0f : 0.0f;
                 count += 1.0f;
             }
         }
@@ -105,10 +122,12 @@ __device__ __forceinline__ bool bake_sample(const BakeView& v, float x, float y,
     s.vis = vis;
     s.cosv = cosv;
     s.dl = dl;
-    s.cr = bake_photo(v, px, py, 0);
-    s.cg = bake_photo(v, px, py, 1);
-    s.cb = bake_photo(v, px, py, 2);
-    s.alpha = v.channels == 4 ? bake_photo(v, px, py, 3) : 1.0f;
+    float col[4];
+    bake_photo_all(v, px, py, col);
+    s.cr = col[0];
+    s.cg = col[1];
+    s.cb = col[2];
+    s.alpha = col[3];
     return true;
 }
 
