@@ -398,11 +398,11 @@ __device__ __forceinline__ void tv_texel(const float* __restrict__ tex, uint32_t
     if (y + 1 < th) pair(i + tw, -1.0f);
 }
 
-__global__ void k_tv_pairs(const int32_t* __restrict__ chart, uint32_t tw, uint32_t th,
-                           unsigned long long* __restrict__ out) {
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+__global__ __launch_bounds__(kBlock) void k_tv_pairs(const int32_t* __restrict__ chart, uint32_t tw, uint32_t th,
+                                                     unsigned long long* __restrict__ out) {
+    __shared__ unsigned long long wsum[kBlock / 64];
     unsigned long long n = 0;
-    if (i < tw * th) {
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < tw * th; i += gridDim.x * kBlock) {
         const uint32_t x = i % tw, y = i / tw;
         const int32_t c = chart ? chart[i] : 0;
         if (c >= 0) {
@@ -412,7 +412,13 @@ __global__ void k_tv_pairs(const int32_t* __restrict__ chart, uint32_t tw, uint3
     }
 #pragma unroll
     for (int m = 32; m >= 1; m >>= 1) n += __shfl_xor(n, m, 64);
-    if ((threadIdx.x & 63u) == 0 && n) atomicAdd(out, n);
+    if ((threadIdx.x & 63u) == 0) wsum[threadIdx.x >> 6] = n;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (uint32_t w = 0; w < kBlock / 64; ++w) t += wsum[w];
+        if (t) atomicAdd(out, t);  // one atomic per workgroup (the grid is capped)
+    }
 }
 
 __device__ __forceinline__ float adam_one(float p, float& g, float& m, float& v, float lr, float b1, float b2,
@@ -704,7 +710,9 @@ hipError_t advance_step(uint32_t* step_counter, hipStream_t st) {
 }
 
 hipError_t tv_pair_count(const int32_t* chart, uint32_t tw, uint32_t th, unsigned long long* out, hipStream_t st) {
-    k_tv_pairs<<<(tw * th + kBlock - 1) / kBlock, kBlock, 0, st>>>(chart, tw, th, out);
+    uint32_t grid = (tw * th + kBlock - 1) / kBlock;
+    if (grid > 512) grid = 512;
+    k_tv_pairs<<<grid ? grid : 1, kBlock, 0, st>>>(chart, tw, th, out);
     return hipGetLastError();
 }
 

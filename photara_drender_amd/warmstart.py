@@ -313,6 +313,8 @@ def main(argv=None) -> int:
                     help="compare plain bake + refinement with the warm start on the 2048^2 / 64-view scene")
     ap.add_argument("--warm-lr", type=float, nargs="*", default=[2e-3],
                     help="starting learning rates of the warm-started refinement to evaluate")
+    ap.add_argument("--warm-steps", type=int, nargs="*", default=[1000],
+                    help="schedule lengths (Adam steps) of the warm-started refinement to evaluate")
     args = ap.parse_args(argv)
     if not (args.train or args.evaluate):
         ap.print_help()
@@ -328,7 +330,8 @@ def main(argv=None) -> int:
         mlp = WarmStartMLP.load(ctx, args.out)
         prob = workloads.refine_problem(ctx)
         for lr in args.warm_lr:
-            evaluate(ctx, prob, mlp, warm=dict(learning_rate=lr), log=log)
+            for steps in args.warm_steps:
+                evaluate(ctx, prob, mlp, warm=dict(learning_rate=lr, steps=steps), log=log)
         return 0
     data = build_training_set(ctx, args.scenes, log=log)
     log(f"training set: {data.features.shape[0]} texels from {data.scenes} scenes in {data.seconds:.1f} s")
