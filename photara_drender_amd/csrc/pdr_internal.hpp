@@ -179,6 +179,10 @@ hipError_t active_count(const uint32_t* flags, const uint32_t* pos, uint32_t n, 
 // norm[s] = 1 / (3 * sum of the step's views' mask sums) (0 when that sum is 0)
 hipError_t norm_table(const double* view_weight, const uint32_t* view_index, uint32_t steps, uint32_t batch,
                       float* norm, hipStream_t st);
+// The batch tap lists key texels in 8x8-tile order when the key space fits 32
+// bits: tiles per row (0: linear texel keys) and the number of keys.
+uint32_t texel_key_tiles(uint32_t tw, uint32_t th);
+uint64_t texel_key_range(uint32_t tw, uint32_t th);
 // Tap entries of batch slot `slot` over its view's `count` packed pixels uvm.
 hipError_t batch_taps(const float4* uvm, uint32_t count, uint32_t slot, uint32_t stride, uint32_t tw, uint32_t th,
                       uint32_t* keys, uint32_t* vals, hipStream_t st);

@@ -196,7 +196,10 @@ pdr_status pdr_refine(pdr_context ctx, const float* initial_texture, const float
             tap_bytes += 3 * up(E * 4) + up((E + 1) * 4);
         }
         int key_bits = 1;
-        while (key_bits < 32 && (1ull << key_bits) < static_cast<unsigned long long>(o.width) * o.height) ++key_bits;
+        while (key_bits < 32 && (1ull << key_bits) < pdr::texel_key_range(o.width, o.height)) ++key_bits;
+        int seam_key_bits = 1;
+        while (seam_key_bits < 32 && (1ull << seam_key_bits) < static_cast<unsigned long long>(o.width) * o.height)
+            ++seam_key_bits;
         uint32_t* tap_keys = bufs.alloc<uint32_t>(std::max<uint32_t>(max_entries, 1));
         uint32_t* tap_vals = bufs.alloc<uint32_t>(std::max<uint32_t>(max_entries, 1));
         uint32_t* tap_keys_sorted = bufs.alloc<uint32_t>(std::max<uint32_t>(max_entries, 1));
@@ -289,8 +292,8 @@ pdr_status pdr_refine(pdr_context ctx, const float* initial_texture, const float
             const size_t temp_bytes = pdr::photo_group_temp_bytes(E);
             void* temp = bufs.alloc<unsigned char>(temp_bytes);
             hip_check(pdr::seam_taps(dpairs, npairs, o.width, o.height, keys, vals, st), "seam_taps");
-            hip_check(pdr::photo_group_taps(keys, vals, keys_sorted, vals_sorted, E, key_bits, E, texels_tmp, runlen,
-                                            offsets, runs_dev, temp, temp_bytes, st),
+            hip_check(pdr::photo_group_taps(keys, vals, keys_sorted, vals_sorted, E, seam_key_bits, E, texels_tmp,
+                                            runlen, offsets, runs_dev, temp, temp_bytes, st),
                       "photo_group_taps");
             uint32_t runs = 0;
             hip_check(hipMemcpyAsync(&runs, runs_dev, sizeof(runs), hipMemcpyDeviceToHost, st), "hipMemcpyAsync");

@@ -194,15 +194,30 @@ __device__ __forceinline__ void gather_taps(uint32_t e0, uint32_t e1, const uint
     }
 }
 
+// Texel t of a tw-wide atlas as a key in 8x8-tile order (tiles_x = tiles per
+// row; 0: the key is t itself).  Sorting taps by it makes the gather's 64 lanes
+// take an 8x8 texel block, whose taps come from a compact image region.
+__device__ __forceinline__ uint32_t texel_key(uint32_t t, uint32_t tw, uint32_t tiles_x) {
+    if (!tiles_x) return t;
+    const uint32_t x = t % tw, y = t / tw;
+    return ((y >> 3) * tiles_x + (x >> 3)) * 64u + ((y & 7u) << 3) + (x & 7u);
+}
+__device__ __forceinline__ uint32_t key_texel(uint32_t k, uint32_t tw, uint32_t tiles_x) {
+    if (!tiles_x) return k;
+    const uint32_t tile = k >> 6;
+    return ((tile / tiles_x) * 8u + ((k >> 3) & 7u)) * tw + (tile % tiles_x) * 8u + (k & 7u);
+}
+
 // Texel gradient of the step's batch: one lane per texel the batch samples,
 // its taps from all slots added in list order.
 __global__ __launch_bounds__(kBlock) void k_photo_gather(const RefineBatchTaps* __restrict__ batches, uint32_t nbatches,
+                                                         uint32_t tw, uint32_t tiles_x,
                                                          const uint32_t* __restrict__ step_counter,
                                                          const float4* __restrict__ resid, float* __restrict__ grad) {
     const RefineBatchTaps& bt = batches[*step_counter % nbatches];
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= *bt.runs) return;
-    const uint32_t texel = bt.texels[i];
+    const uint32_t texel = key_texel(bt.texels[i], tw, tiles_x);
     float a[3] = {0.0f, 0.0f, 0.0f};
     gather_taps<2>(bt.tap_offsets[i], bt.tap_offsets[i + 1], bt.taps, bt.tap_weights, resid, a);
     const size_t t = static_cast<size_t>(texel) * 3;
@@ -250,7 +265,8 @@ __global__ void k_active_count(const uint32_t* __restrict__ flags, const uint32_
 // Tap entries of batch slot `slot` over its view's `count` packed pixels:
 // key = texel, value = (slot * stride + a) * 4 + tap.
 __global__ void k_batch_taps(const float4* __restrict__ uvm, uint32_t count, uint32_t slot, uint32_t stride,
-                             uint32_t tw, uint32_t th, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+                             uint32_t tw, uint32_t th, uint32_t tiles_x, uint32_t* __restrict__ keys,
+                             uint32_t* __restrict__ vals) {
     const uint32_t a = blockIdx.x * kBlock + threadIdx.x;
     if (a >= count) return;
     const float4 q = uvm[a];
@@ -258,7 +274,7 @@ __global__ void k_batch_taps(const float4* __restrict__ uvm, uint32_t count, uin
     const uint32_t base = (slot * stride + a) * 4u;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        keys[static_cast<size_t>(a) * 4 + k] = at.t[k];
+        keys[static_cast<size_t>(a) * 4 + k] = texel_key(at.t[k], tw, tiles_x);
         vals[static_cast<size_t>(a) * 4 + k] = base + static_cast<uint32_t>(k);
     }
 }
@@ -556,6 +572,16 @@ uint32_t photometric_partials(uint32_t batch, uint32_t max_active) {
     return (blocks ? blocks : 1) * batch;
 }
 
+uint32_t texel_key_tiles(uint32_t tw, uint32_t th) {
+    const uint64_t tx = (tw + 7u) / 8u, ty = (th + 7u) / 8u;
+    return tx * ty * 64u <= 0xFFFFFFFFull ? static_cast<uint32_t>(tx) : 0u;
+}
+
+uint64_t texel_key_range(uint32_t tw, uint32_t th) {
+    const uint32_t tx = texel_key_tiles(tw, th);
+    return tx ? static_cast<uint64_t>(tx) * ((th + 7u) / 8u) * 64u : static_cast<uint64_t>(tw) * th;
+}
+
 hipError_t photometric_step(const RefineView* views_dev, uint32_t batch, uint32_t max_active,
                             const RefineBatchTaps* batches_dev, uint32_t nbatches, uint32_t max_runs,
                             const float* tex, uint32_t tw, uint32_t th, float eps, const RefineStepTables& tables,
@@ -566,7 +592,8 @@ hipError_t photometric_step(const RefineView* views_dev, uint32_t batch, uint32_
     k_photo_residual<<<grid, kBlock, 0, st>>>(views_dev, batch, tex, tw, th, eps, tables, step_counter, resid, stride,
                                               loss_part);
     const uint32_t gblocks = (max_runs + kBlock - 1) / kBlock;
-    k_photo_gather<<<gblocks ? gblocks : 1, kBlock, 0, st>>>(batches_dev, nbatches, step_counter, resid, grad);
+    k_photo_gather<<<gblocks ? gblocks : 1, kBlock, 0, st>>>(batches_dev, nbatches, tw, texel_key_tiles(tw, th),
+                                                             step_counter, resid, grad);
     return hipGetLastError();
 }
 
@@ -605,7 +632,8 @@ hipError_t active_count(const uint32_t* flags, const uint32_t* pos, uint32_t n, 
 hipError_t batch_taps(const float4* uvm, uint32_t count, uint32_t slot, uint32_t stride, uint32_t tw, uint32_t th,
                       uint32_t* keys, uint32_t* vals, hipStream_t st) {
     if (!count) return hipSuccess;
-    k_batch_taps<<<(count + kBlock - 1) / kBlock, kBlock, 0, st>>>(uvm, count, slot, stride, tw, th, keys, vals);
+    k_batch_taps<<<(count + kBlock - 1) / kBlock, kBlock, 0, st>>>(uvm, count, slot, stride, tw, th,
+                                                                   texel_key_tiles(tw, th), keys, vals);
     return hipGetLastError();
 }
 
