@@ -106,10 +106,12 @@ pdr_status pdr_refine(pdr_context ctx, const float* initial_texture, const float
         const float* dpos = device_view(bufs, positions, static_cast<size_t>(V) * 3, st);
         const float* duv = device_view(bufs, uv, static_cast<size_t>(V) * 2, st);
         const uint32_t* didx = device_view(bufs, idx, static_cast<size_t>(T) * 3, st);
-        float* clip = bufs.alloc<float>(static_cast<size_t>(V) * 4);
         uint32_t max_pixels = 0;
         for (uint32_t k = 0; k < nviews; ++k) max_pixels = std::max(max_pixels, views[k].width * views[k].height);
-        float* raster = bufs.alloc<float>(static_cast<size_t>(max_pixels) * 4);
+        // views are rasterised in groups of up to kMaxViews per launch
+        const uint32_t group = std::min<uint32_t>(nviews, pdr::kMaxViews);
+        float* clips = bufs.alloc<float>(static_cast<size_t>(group) * V * 4);
+        float* rasters = bufs.alloc<float>(static_cast<size_t>(group) * max_pixels * 4);
         double* partials = bufs.alloc<double>(pdr::refine_partial_count(max_pixels) + 1);
         double* view_weight = bufs.alloc<double>(nviews);
         uint32_t* view_counts = bufs.alloc<uint32_t>(nviews);  // active pixels
@@ -131,39 +133,48 @@ pdr_status pdr_refine(pdr_context ctx, const float* initial_texture, const float
         for (uint32_t k = 0; k < nviews; ++k) total_bytes += 2 * up(views[k].width * views[k].height * sizeof(float4));
         unsigned char* view_arena = bufs.alloc<unsigned char>(std::max<size_t>(total_bytes, 256));
         size_t arena_off = 0;
-        for (uint32_t k = 0; k < nviews; ++k) {
-            const pdr_refine_view& v = views[k];
-            const uint32_t P = v.width * v.height;
-            const float* img = device_view(bufs, v.image, static_cast<size_t>(P) * v.channels, st);
-            const float* vis = device_view(bufs, v.visibility_mask, P, st);
-            hip_check(pdr::transform_positions(dpos, V, v.world_to_clip, clip, st), "transform_positions");
-            float vp[4];
-            resolve_viewport(v.width, v.height, v.has_viewport ? v.viewport : nullptr, vp);
+        for (uint32_t g0 = 0; g0 < nviews; g0 += group) {
+            const uint32_t gn = std::min(group, nviews - g0);
             pdr::ViewTable vt{};
-            vt.n = 1;
-            vt.v[0] = make_view(clip, v.width, v.height, vp);
-            vt.v[0].raster = raster;
+            vt.n = gn;
+            for (uint32_t b = 0; b < gn; ++b) {
+                const pdr_refine_view& v = views[g0 + b];
+                float* clip = clips + static_cast<size_t>(b) * V * 4;
+                hip_check(pdr::transform_positions(dpos, V, v.world_to_clip, clip, st), "transform_positions");
+                float vp[4];
+                resolve_viewport(v.width, v.height, v.has_viewport ? v.viewport : nullptr, vp);
+                vt.v[b] = make_view(clip, v.width, v.height, vp);
+                vt.v[b].raster = rasters + static_cast<size_t>(b) * max_pixels * 4;
+            }
             finalize_table(vt);
             run_raster_forward(ctx, vt, didx, T, V, 0, 0, st);
-            const pdr::PixelTable pt = pixel_table_one(raster, nullptr, nullptr, vuv, nullptr, v.width, v.height);
-            hip_check(pdr::interp_forward(pt, duv, V, 2, didx, T, st), "interp_forward");
-            const uint32_t np = pdr::refine_partial_count(P);
-            hip_check(pdr::refine_view_inputs(img, v.channels, vis, raster, P, target, mask, partials, np, st),
-                      "refine_view_inputs");
-            hip_check(pdr::sum_partials(partials, np, view_weight + k, st), "sum_partials");
-            // the pixels with mask > 0, packed in 8x8-tile order
-            const uint32_t slots = pdr::tile_slots(v.width, v.height);
-            hip_check(pdr::active_flags(mask, v.width, v.height, act_flags, st), "active_flags");
-            hip_check(pdr::exclusive_scan(act_flags, act_pos, slots, scan_temp, scan_temp_bytes, st),
-                      "exclusive_scan");
-            hip_check(pdr::active_count(act_flags, act_pos, slots, view_counts + k, st), "active_count");
-            float4* uvm = reinterpret_cast<float4*>(view_arena + arena_off);
-            arena_off += up(static_cast<size_t>(P) * sizeof(float4));
-            float4* rgb = reinterpret_cast<float4*>(view_arena + arena_off);
-            arena_off += up(static_cast<size_t>(P) * sizeof(float4));
-            hip_check(pdr::active_pack(vuv, target, mask, v.width, v.height, act_flags, act_pos, uvm, rgb, st),
-                      "active_pack");
-            rviews[k] = pdr::RefineView{uvm, rgb, view_counts + k};
+            for (uint32_t b = 0; b < gn; ++b) {
+                const uint32_t k = g0 + b;
+                const pdr_refine_view& v = views[k];
+                const uint32_t P = v.width * v.height;
+                const float* raster = rasters + static_cast<size_t>(b) * max_pixels * 4;
+                const float* img = device_view(bufs, v.image, static_cast<size_t>(P) * v.channels, st);
+                const float* vis = device_view(bufs, v.visibility_mask, P, st);
+                const pdr::PixelTable pt = pixel_table_one(raster, nullptr, nullptr, vuv, nullptr, v.width, v.height);
+                hip_check(pdr::interp_forward(pt, duv, V, 2, didx, T, st), "interp_forward");
+                const uint32_t np = pdr::refine_partial_count(P);
+                hip_check(pdr::refine_view_inputs(img, v.channels, vis, raster, P, target, mask, partials, np, st),
+                          "refine_view_inputs");
+                hip_check(pdr::sum_partials(partials, np, view_weight + k, st), "sum_partials");
+                // the pixels with mask > 0, packed in 8x8-tile order
+                const uint32_t slots = pdr::tile_slots(v.width, v.height);
+                hip_check(pdr::active_flags(mask, v.width, v.height, act_flags, st), "active_flags");
+                hip_check(pdr::exclusive_scan(act_flags, act_pos, slots, scan_temp, scan_temp_bytes, st),
+                          "exclusive_scan");
+                hip_check(pdr::active_count(act_flags, act_pos, slots, view_counts + k, st), "active_count");
+                float4* uvm = reinterpret_cast<float4*>(view_arena + arena_off);
+                arena_off += up(static_cast<size_t>(P) * sizeof(float4));
+                float4* rgb = reinterpret_cast<float4*>(view_arena + arena_off);
+                arena_off += up(static_cast<size_t>(P) * sizeof(float4));
+                hip_check(pdr::active_pack(vuv, target, mask, v.width, v.height, act_flags, act_pos, uvm, rgb, st),
+                          "active_pack");
+                rviews[k] = pdr::RefineView{uvm, rgb, view_counts + k};
+            }
         }
         std::vector<uint32_t> host_counts(nviews);
         hip_check(hipMemcpyAsync(host_counts.data(), view_counts, host_counts.size() * 4, hipMemcpyDeviceToHost, st),

@@ -150,7 +150,8 @@ def _gpu_pad(ctx, col, val, cov, filled, margin, fill_unobserved, max_fill):
     return c.cpu().numpy(), None if f is None else f.cpu().numpy(), stats
 
 
-SHAPES = [(1, 1), (1, 37), (37, 1), (48, 64), (131, 97), (64, 256), (300, 301), (8, 12288), (4, 16384)]
+SHAPES = [(1, 1), (1, 37), (37, 1), (48, 64), (131, 97), (64, 256), (300, 301), (8, 12288), (4, 16384), (1000, 3),
+          (517, 2100)]
 
 
 @pytest.mark.gpu
