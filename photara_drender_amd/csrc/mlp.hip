@@ -46,13 +46,26 @@ __device__ __forceinline__ void wave_sync() {
 
 constexpr float kGeluC = 0.7978845608028654f;  // sqrt(2 / pi)
 
+// GELU, tanh form: 0.5 z (1 + tanh u) with u = c (z + 0.044715 z^3), evaluated as
+// z * s with s = (1 + tanh u) / 2 = 1 / (1 + e^(-2u)) (hardware exp and
+// reciprocal, no cancellation); its derivative with 1 - tanh^2 u = 4 s (1 - s).
+__device__ __forceinline__ float gelu_sigmoid(float z, float& one_minus_s) {
+    const float u = kGeluC * (z + 0.044715f * z * z * z);
+    const float e = __expf(-2.0f * u);
+    const float s = __builtin_amdgcn_rcpf(1.0f + e);
+    one_minus_s = s < 0.5f ? 1.0f - s : e * s;  // e * s = e / (1 + e), exact where s is near 1
+    return s;
+}
+
 __device__ __forceinline__ float gelu(float z) {
-    return 0.5f * z * (1.0f + tanhf(kGeluC * (z + 0.044715f * z * z * z)));
+    float oms;
+    return z * gelu_sigmoid(z, oms);
 }
 
 __device__ __forceinline__ float gelu_grad(float z) {
-    const float t = tanhf(kGeluC * (z + 0.044715f * z * z * z));
-    return 0.5f * (1.0f + t) + 0.5f * z * (1.0f - t * t) * kGeluC * (1.0f + 3.0f * 0.044715f * z * z);
+    float oms;
+    const float s = gelu_sigmoid(z, oms);
+    return s + 2.0f * z * s * oms * kGeluC * (1.0f + 3.0f * 0.044715f * z * z);
 }
 
 __device__ __forceinline__ int unit(int ks, int q, int j) { return 16 * (2 * ks + j / 4) + 4 * q + (j % 4); }
