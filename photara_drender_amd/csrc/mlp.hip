@@ -410,16 +410,34 @@ __global__ __launch_bounds__(64) void k_mlp_grad(const float* __restrict__ x, co
 // grad [kParams] and loss_count [2] = (loss, valid count).  With params set,
 // also one Adam step on the float32 parameters (bias corrections
 // b1c = 1 - beta1^t, b2c = 1 - beta2^t).
-__global__ void k_mlp_reduce(const float* __restrict__ part, uint32_t waves, float* __restrict__ grad,
-                             float* __restrict__ loss_count, float* __restrict__ params, float* __restrict__ m1,
-                             float* __restrict__ m2, float lr, float beta1, float beta2, float eps, float b1c,
-                             float b2c) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= static_cast<uint32_t>(kPart)) return;
+// Sums the per-wave partials (fixed order: kReduceSegments contiguous wave
+// ranges, each in order, then the ranges in order), then Adam.  A block takes
+// 64 consecutive entries of the partial rows.
+constexpr uint32_t kReduceSegments = 16;
+__global__ __launch_bounds__(64 * kReduceSegments) void k_mlp_reduce(
+    const float* __restrict__ part, uint32_t waves, float* __restrict__ grad, float* __restrict__ loss_count,
+    float* __restrict__ params, float* __restrict__ m1, float* __restrict__ m2, float lr, float beta1, float beta2,
+    float eps, float b1c, float b2c) {
+    __shared__ float seg_s[kReduceSegments][64], seg_c[kReduceSegments][64];
+    const uint32_t lane = threadIdx.x & 63u, seg = threadIdx.x >> 6;
+    const uint32_t i = blockIdx.x * 64u + lane;
+    const uint32_t per = (waves + kReduceSegments - 1) / kReduceSegments;
+    const uint32_t k0 = seg * per, k1 = k0 + per < waves ? k0 + per : waves;
     float s = 0.0f, c = 0.0f;
-    for (uint32_t k = 0; k < waves; ++k) {
-        s += part[static_cast<size_t>(k) * kPart + i];
-        c += part[static_cast<size_t>(k) * kPart + kParams + 1];
+    if (i < static_cast<uint32_t>(kPart))
+        for (uint32_t k = k0; k < k1; ++k) {
+            s += part[static_cast<size_t>(k) * kPart + i];
+            c += part[static_cast<size_t>(k) * kPart + kParams + 1];
+        }
+    seg_s[seg][lane] = s;
+    seg_c[seg][lane] = c;
+    __syncthreads();
+    if (seg != 0 || i >= static_cast<uint32_t>(kPart)) return;
+    s = 0.0f;
+    c = 0.0f;
+    for (uint32_t g = 0; g < kReduceSegments; ++g) {
+        s += seg_s[g][lane];
+        c += seg_c[g][lane];
     }
     const float inv = 1.0f / (3.0f * (c > 1.0f ? c : 1.0f));
     if (i >= static_cast<uint32_t>(kParams)) {
@@ -460,8 +478,8 @@ hipError_t mlp_grad(const float* x, const float* target, const float* valid, uin
 
 hipError_t mlp_reduce(const float* part, uint32_t waves, float* grad, float* loss_count, float* params, float* m1,
                       float* m2, float lr, float beta1, float beta2, float eps, float b1c, float b2c, hipStream_t st) {
-    k_mlp_reduce<<<(kPart + 255) / 256, 256, 0, st>>>(part, waves, grad, loss_count, params, m1, m2, lr, beta1, beta2,
-                                                      eps, b1c, b2c);
+    k_mlp_reduce<<<(kPart + 63) / 64, 64 * kReduceSegments, 0, st>>>(part, waves, grad, loss_count, params, m1, m2, lr,
+                                                                     beta1, beta2, eps, b1c, b2c);
     return hipGetLastError();
 }
 
