@@ -192,11 +192,30 @@ __global__ __launch_bounds__(kInferBlock) void k_mlp_infer(const float* __restri
     const float b3[3] = {w[kB3], w[kB3 + 1], w[kB3 + 2]};
     const uint32_t waves = gridDim.x * (kInferBlock / 64);
     const uint32_t tiles = (n + 15u) / 16u;
-    for (uint32_t tile = blockIdx.x * (kInferBlock / 64) + (threadIdx.x >> 6); tile < tiles; tile += waves) {
+    // the lane's share of a tile's features (16 texels x kIn / 4 float4), loaded a tile ahead
+    constexpr int kShare = (16 * (kIn / 4) + 63) / 64;
+    auto load_share = [&](uint32_t t0, float4 (&v)[kShare]) {
+#pragma unroll
+        for (int j = 0; j < kShare; ++j) {
+            const int e = lane + 64 * j, tt = e / (kIn / 4), c = e % (kIn / 4);
+            v[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (e < 16 * (kIn / 4) && t0 + static_cast<uint32_t>(tt) < n)
+                v[j] = reinterpret_cast<const float4*>(x + static_cast<size_t>(t0 + tt) * kIn)[c];
+        }
+    };
+    float4 next[kShare];
+    uint32_t tile = blockIdx.x * (kInferBlock / 64) + (threadIdx.x >> 6);
+    if (tile < tiles) load_share(tile * 16u, next);
+    for (; tile < tiles; tile += waves) {
         const uint32_t t0 = tile * 16u, t = t0 + static_cast<uint32_t>(r);
         wave_sync();  // the previous tile's LDS reads are done
-        stage_features(x, n, t0, lane, feat);
+#pragma unroll
+        for (int j = 0; j < kShare; ++j) {
+            const int e = lane + 64 * j;
+            if (e < 16 * (kIn / 4)) reinterpret_cast<float4*>(feat[e / (kIn / 4)])[e % (kIn / 4)] = next[j];
+        }
         wave_sync();
+        if (tile + waves < tiles) load_share((tile + waves) * 16u, next);
         bf16x8 xb[2];
         feature_operands(feat, q, r, xb);
         float base[3];
