@@ -65,13 +65,7 @@ __global__ __launch_bounds__(kBlock) void k_interp_fwd(PixelTable pt, const floa
         const float* a0 = attr + static_cast<size_t>(i0) * A;
         const float* a1 = attr + static_cast<size_t>(i1) * A;
         const float* a2 = attr + static_cast<size_t>(i2) * A;
-        if constexpr (kA == 8) {
-            float o[8];
-    #pragma unroll
-            for (int a = 0; a < 8; ++a) o[a] = u * a0[a] + v * a1[a] + w * a2[a];
-            reinterpret_cast<float4*>(out)[0] = make_float4(o[0], o[1], o[2], o[3]);
-            reinterpret_cast<float4*>(out)[1] = make_float4(o[4], o[5], o[6], o[7]);
-        } else if constexpr (kA == 2) {
+        if constexpr (kA == 2) {
             reinterpret_cast<float2*>(out)[0] = make_float2(u * a0[0] + v * a1[0] + w * a2[0], u * a0[1] + v * a1[1] + w * a2[1]);
         } else {
             for (uint32_t a = 0; a < A; ++a) out[a] = u * a0[a] + v * a1[a] + w * a2[a];
@@ -81,6 +75,86 @@ __global__ __launch_bounds__(kBlock) void k_interp_fwd(PixelTable pt, const floa
         body(pt.v[kv]);
     else
         body(pt.v[find_pixel_view(pt, gp)]);
+}
+
+// A = 8 (16-byte aligned rows): two pixels per lane, gp and gp + half, their
+// dependent loads (raster -> triangle -> attribute rows) issued side by side so
+// twice as many are in flight; the values are k_interp_fwd's.
+__global__ __launch_bounds__(kBlock) void k_interp_fwd8_pair(PixelTable pt, const float* __restrict__ attr,
+                                                             uint32_t V, const uint32_t* __restrict__ idx, uint32_t T,
+                                                             uint32_t half) {
+    const uint32_t g0 = blockIdx.x * kBlock + threadIdx.x;
+    if (g0 >= half) return;
+    const uint32_t gps[2] = {g0, g0 + half};
+    bool live[2];
+    const PixelView* pv[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        live[h] = gps[h] < pt.total_pixels;
+        const uint32_t g = live[h] ? gps[h] : pt.total_pixels - 1u;
+        bool split;
+        const uint32_t kv = wave_view(pt, g, split);
+        pv[h] = &pt.v[split ? find_pixel_view(pt, g) : kv];
+    }
+    float4 r[2];
+    uint32_t p[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const uint32_t g = live[h] ? gps[h] : pt.total_pixels - 1u;
+        p[h] = g - pv[h]->pixel_base;
+        r[h] = reinterpret_cast<const float4*>(pv[h]->raster)[p[h]];
+    }
+    uint32_t iv[2][3];
+    bool ok[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const uint32_t id = decode_id(r[h].w);
+        ok[h] = live[h] && id != 0 && id <= T;
+        iv[h][0] = iv[h][1] = iv[h][2] = 0;
+        if (ok[h]) {
+            iv[h][0] = idx[(id - 1) * 3 + 0];
+            iv[h][1] = idx[(id - 1) * 3 + 1];
+            iv[h][2] = idx[(id - 1) * 3 + 2];
+        }
+    }
+    float4 row[2][3][2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        ok[h] = ok[h] && iv[h][0] < V && iv[h][1] < V && iv[h][2] < V;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            row[h][c][0] = row[h][c][1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (ok[h]) {
+                const float4* a = reinterpret_cast<const float4*>(attr + static_cast<size_t>(iv[h][c]) * 8);
+                row[h][c][0] = a[0];
+                row[h][c][1] = a[1];
+            }
+        }
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        if (!live[h]) continue;
+        float4* out = reinterpret_cast<float4*>(pv[h]->out0 + static_cast<size_t>(p[h]) * 8);
+        if (!ok[h]) {
+            out[0] = out[1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            continue;
+        }
+        const float u = r[h].x, v = r[h].y;
+        const float w = 1.0f - u - v;
+        float a0[8], a1[8], a2[8];
+        auto unpack = [](const float4 (&q)[2], float (&o)[8]) {
+            o[0] = q[0].x; o[1] = q[0].y; o[2] = q[0].z; o[3] = q[0].w;
+            o[4] = q[1].x; o[5] = q[1].y; o[6] = q[1].z; o[7] = q[1].w;
+        };
+        unpack(row[h][0], a0);
+        unpack(row[h][1], a1);
+        unpack(row[h][2], a2);
+        float o[8];
+#pragma unroll
+        for (int a = 0; a < 8; ++a) o[a] = u * a0[a] + v * a1[a] + w * a2[a];
+        out[0] = make_float4(o[0], o[1], o[2], o[3]);
+        out[1] = make_float4(o[4], o[5], o[6], o[7]);
+    }
 }
 
 constexpr uint32_t kVertexSlots = 1024;  // vertex accumulators per tile
@@ -525,9 +599,10 @@ hipError_t interp_forward(const PixelTable& pt, const float* attr, uint32_t V, u
     const uint32_t grid = pixel_grid(pt.total_pixels);
     bool aligned = (reinterpret_cast<uintptr_t>(attr) % 16) == 0;
     for (uint32_t k = 0; k < pt.n; ++k) aligned &= (reinterpret_cast<uintptr_t>(pt.v[k].out0) % 16) == 0;
-    if (A == 8 && aligned)
-        k_interp_fwd<8><<<grid, kBlock, 0, st>>>(pt, attr, V, A, idx, T);
-    else if (A == 2 && aligned)
+    if (A == 8 && aligned) {
+        const uint32_t half = (pt.total_pixels + 1u) / 2u;
+        k_interp_fwd8_pair<<<pixel_grid(half), kBlock, 0, st>>>(pt, attr, V, idx, T, half);
+    } else if (A == 2 && aligned)
         k_interp_fwd<2><<<grid, kBlock, 0, st>>>(pt, attr, V, A, idx, T);
     else
         k_interp_fwd<0><<<grid, kBlock, 0, st>>>(pt, attr, V, A, idx, T);
