@@ -211,12 +211,14 @@ hipError_t seam_taps(const float* pairs, uint32_t npairs, uint32_t tw, uint32_t 
 hipError_t seam_tap_weights(const uint32_t* vals, uint32_t E, const float* pairs, uint32_t tw, uint32_t th, float* w,
                             hipStream_t st);
 hipError_t adam_tv_step(const float* tex, float* tex_out, float* grad, float* m, float* v, uint32_t tw, uint32_t th,
-                        const int32_t* chart, float tv_eps, float tv_scale, const float* lr_table,
+                        const uint8_t* tv_mask, float tv_eps, float tv_scale, const float* lr_table,
                         const float* bc1_table, const float* bc2_table, float b1, float b2, float eps, float cmin,
                         float cmax, const uint32_t* step_counter, hipStream_t st);
 hipError_t advance_step(uint32_t* step_counter, hipStream_t st);
 // loss[step] += the sum of the photometric partials (fixed order), then step += 1.
 hipError_t finish_step(const float* loss_part, uint32_t n, float* loss, uint32_t* step_counter, hipStream_t st);
+// Per-texel neighbour mask of the chart-aware TV (chart may be null: one chart).
+hipError_t tv_mask(const int32_t* chart, uint32_t tw, uint32_t th, uint8_t* mask, hipStream_t st);
 hipError_t tv_gradient(const float* tex, uint32_t tw, uint32_t th, const int32_t* chart, float eps, float scale,
                        float* grad, hipStream_t st);
 hipError_t adam_step(float* tex, float* grad, float* m, float* v, uint32_t n, const float* lr_table,

@@ -344,9 +344,12 @@ pdr_status pdr_refine(pdr_context ctx, const float* initial_texture, const float
 
         // extended objective (python/photara_drender/atlas.py:84-144)
         const int32_t* dchart = nullptr;
+        uint8_t* dtv_mask = nullptr;
         float tv_scale = 0.0f;
         if (o.tv_weight != 0.0f) {
             dchart = device_view(bufs, o.chart_ids, static_cast<size_t>(texels), st);
+            dtv_mask = bufs.alloc<uint8_t>(static_cast<size_t>(texels));
+            hip_check(pdr::tv_mask(dchart, o.width, o.height, dtv_mask, st), "tv_mask");
             unsigned long long* cnt = bufs.alloc<unsigned long long>(1);
             hip_check(hipMemsetAsync(cnt, 0, sizeof(unsigned long long), st), "hipMemsetAsync");
             hip_check(pdr::tv_pair_count(dchart, o.width, o.height, cnt, st), "tv_pair_count");
@@ -376,7 +379,7 @@ pdr_status pdr_refine(pdr_context ctx, const float* initial_texture, const float
                                          o.joint_seam_weight, counters, 0, seam_groups, seam_resid, grad, nullptr, s),
                           "seam_step");
             if (fused_tv)
-                hip_check(pdr::adam_tv_step(texbuf[c], texbuf[c ^ 1u], grad, m1, m2, o.width, o.height, dchart, tv_eps,
+                hip_check(pdr::adam_tv_step(texbuf[c], texbuf[c ^ 1u], grad, m1, m2, o.width, o.height, dtv_mask, tv_eps,
                                             tv_scale, d_lr, d_bc1, d_bc2, o.adam_beta1, o.adam_beta2, o.adam_epsilon,
                                             o.clamp_min, o.clamp_max, counters, s),
                           "adam_tv_step");

@@ -388,30 +388,59 @@ __global__ void k_seam_tap_weights(const uint32_t* __restrict__ vals, uint32_t E
     w[e] = second ? -wt : wt;
 }
 
-// d/dt of weight * sum_pairs sqrt((t_b - t_a)^2 + eps^2) / (pairs * 3) over
-// horizontal and vertical neighbour pairs inside one chart, at texel i
-// (g = 0 outside every chart).
-__device__ __forceinline__ void tv_texel(const float* __restrict__ tex, uint32_t tw, uint32_t th,
-                                         const int32_t* __restrict__ chart, float eps, uint32_t i, float g[3]) {
-    g[0] = g[1] = g[2] = 0.0f;
+// Native-precision square root, reciprocal and reciprocal square root
+// (v_sqrt_f32, v_rcp_f32, v_rsq_f32: about 1 ulp, as the shaders' sqrt and
+// division compile on a GPU).  The correctly rounded library forms take about
+// 15 instructions each and left k_adam_tv bound by the VALU.  The rsq operand
+// is kept at or above FLT_MIN, where the instruction is defined for every input.
+__device__ __forceinline__ float native_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float native_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float charbonnier_slope(float d, float eps2) {  // d / sqrt(d^2 + eps^2)
+    return d * __builtin_amdgcn_rsqf(fmaxf(d * d + eps2, 1.17549435e-38f));
+}
+
+// Neighbour mask of texel i for the chart-aware TV: bit 0 the texel lies in a
+// chart (id >= 0; every texel when chart is null), bits 1-4 its left, right,
+// upper and lower neighbours exist and lie in the same chart.
+__device__ __forceinline__ uint32_t tv_mask_of(const int32_t* __restrict__ chart, uint32_t tw, uint32_t th,
+                                               uint32_t i) {
     const uint32_t x = i % tw, y = i / tw;
     const int32_t c = chart ? chart[i] : 0;
-    if (c < 0) return;
+    if (c < 0) return 0u;
+    auto same = [&](uint32_t j) { return (chart ? chart[j] : 0) == c; };
+    uint32_t m = 1u;
+    if (x > 0 && same(i - 1)) m |= 2u;
+    if (x + 1 < tw && same(i + 1)) m |= 4u;
+    if (y > 0 && same(i - tw)) m |= 8u;
+    if (y + 1 < th && same(i + tw)) m |= 16u;
+    return m;
+}
+
+// d/dt of weight * sum_pairs sqrt((t_b - t_a)^2 + eps^2) / (pairs * 3) over
+// horizontal and vertical neighbour pairs inside one chart, at texel i with
+// neighbour mask `mask` (pairs left, right, up, down in that order).
+__device__ __forceinline__ void tv_texel(const float* __restrict__ tex, uint32_t tw, uint32_t mask, float eps2,
+                                         uint32_t i, float g[3]) {
+    g[0] = g[1] = g[2] = 0.0f;
+    if (mask <= 1u) return;
     const float* t = tex + static_cast<size_t>(i) * 3;
     auto pair = [&](uint32_t j, float sign) {
-        const int32_t cj = chart ? chart[j] : 0;
-        if (cj != c) return;
         const float* u = tex + static_cast<size_t>(j) * 3;
 #pragma unroll
         for (int ch = 0; ch < 3; ++ch) {
             const float d = sign > 0.0f ? t[ch] - u[ch] : u[ch] - t[ch];  // forward difference of the pair
-            g[ch] += sign * (d / sqrtf(d * d + eps * eps));
+            g[ch] += sign * charbonnier_slope(d, eps2);
         }
     };
-    if (x > 0) pair(i - 1, 1.0f);        // pair (x-1, x): d = t - left, dL/dt = +d/r
-    if (x + 1 < tw) pair(i + 1, -1.0f);  // pair (x, x+1): d = right - t, dL/dt = -d/r
-    if (y > 0) pair(i - tw, 1.0f);
-    if (y + 1 < th) pair(i + tw, -1.0f);
+    if (mask & 2u) pair(i - 1, 1.0f);    // pair (x-1, x): d = t - left, dL/dt = +d/r
+    if (mask & 4u) pair(i + 1, -1.0f);   // pair (x, x+1): d = right - t, dL/dt = -d/r
+    if (mask & 8u) pair(i - tw, 1.0f);
+    if (mask & 16u) pair(i + tw, -1.0f);
+}
+
+__global__ void k_tv_mask(const int32_t* __restrict__ chart, uint32_t tw, uint32_t th, uint8_t* __restrict__ mask) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < tw * th) mask[i] = static_cast<uint8_t>(tv_mask_of(chart, tw, th, i));
 }
 
 __global__ __launch_bounds__(kBlock) void k_tv_pairs(const int32_t* __restrict__ chart, uint32_t tw, uint32_t th,
@@ -445,7 +474,7 @@ __device__ __forceinline__ float adam_one(float p, float& g, float& m, float& v,
     v = m2;
     g = 0.0f;
     const float c1 = m1 / bc1, c2 = m2 / bc2;
-    return clampf(p - lr * c1 / (sqrtf(c2) + eps), cmin, cmax);
+    return clampf(p - lr * c1 * native_rcp(native_sqrt(c2) + eps), cmin, cmax);
 }
 
 // Adam with the chart-TV gradient (scale = weight / (pairs * 3)) added to each
@@ -455,7 +484,7 @@ __device__ __forceinline__ float adam_one(float p, float& g, float& m, float& v,
 __global__ __launch_bounds__(kBlock) void k_adam_tv(const float* __restrict__ tex, float* __restrict__ tex_out,
                                                     float* __restrict__ grad, float* __restrict__ mom1,
                                                     float* __restrict__ mom2, uint32_t tw, uint32_t th,
-                                                    const int32_t* __restrict__ chart, float tv_eps, float tv_scale,
+                                                    const uint8_t* __restrict__ tv_mask, float tv_eps2, float tv_scale,
                                                     const float* __restrict__ lr_t, const float* __restrict__ bc1_t,
                                                     const float* __restrict__ bc2_t, float b1, float b2, float eps,
                                                     float cmin, float cmax, const uint32_t* __restrict__ step_counter) {
@@ -463,7 +492,7 @@ __global__ __launch_bounds__(kBlock) void k_adam_tv(const float* __restrict__ te
     const float lr = lr_t[step], bc1 = bc1_t[step], bc2 = bc2_t[step];
     for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < tw * th; i += gridDim.x * kBlock) {
         float g[3];
-        tv_texel(tex, tw, th, chart, tv_eps, i, g);
+        tv_texel(tex, tw, tv_mask[i], tv_eps2, i, g);
 #pragma unroll
         for (int ch = 0; ch < 3; ++ch) {
             const size_t j = static_cast<size_t>(i) * 3 + ch;
@@ -483,7 +512,7 @@ __global__ void k_tv_gradient(const float* __restrict__ tex, uint32_t tw, uint32
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= tw * th) return;
     float g[3];
-    tv_texel(tex, tw, th, chart, eps, i, g);
+    tv_texel(tex, tw, tv_mask_of(chart, tw, th, i), eps * eps, i, g);
 #pragma unroll
     for (int ch = 0; ch < 3; ++ch) grad[static_cast<size_t>(i) * 3 + ch] = g[ch] * scale;
 }
@@ -711,13 +740,18 @@ hipError_t seam_tap_weights(const uint32_t* vals, uint32_t E, const float* pairs
 }
 
 hipError_t adam_tv_step(const float* tex, float* tex_out, float* grad, float* m, float* v, uint32_t tw, uint32_t th,
-                        const int32_t* chart, float tv_eps, float tv_scale, const float* lr_table,
+                        const uint8_t* tv_mask, float tv_eps, float tv_scale, const float* lr_table,
                         const float* bc1_table, const float* bc2_table, float b1, float b2, float eps, float cmin,
                         float cmax, const uint32_t* step_counter, hipStream_t st) {
     uint32_t grid = (tw * th + kBlock - 1) / kBlock;
     if (grid > 8192) grid = 8192;
-    k_adam_tv<<<grid, kBlock, 0, st>>>(tex, tex_out, grad, m, v, tw, th, chart, tv_eps, tv_scale, lr_table, bc1_table,
+    k_adam_tv<<<grid, kBlock, 0, st>>>(tex, tex_out, grad, m, v, tw, th, tv_mask, tv_eps * tv_eps, tv_scale, lr_table, bc1_table,
                                        bc2_table, b1, b2, eps, cmin, cmax, step_counter);
+    return hipGetLastError();
+}
+
+hipError_t tv_mask(const int32_t* chart, uint32_t tw, uint32_t th, uint8_t* mask, hipStream_t st) {
+    k_tv_mask<<<(tw * th + kBlock - 1) / kBlock, kBlock, 0, st>>>(chart, tw, th, mask);
     return hipGetLastError();
 }
 
