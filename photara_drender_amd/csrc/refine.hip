@@ -208,18 +208,61 @@ __device__ __forceinline__ uint32_t key_texel(uint32_t k, uint32_t tw, uint32_t 
     return ((tile / tiles_x) * 8u + ((k >> 3) & 7u)) * tw + (tile % tiles_x) * 8u + (k & 7u);
 }
 
+// Hand-off of LDS data between lanes of one wave.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Texel gradient of the step's batch: one lane per texel the batch samples,
-// its taps from all slots added in list order.
+// its taps from all slots added in list order.  A wave owns 64 consecutive
+// texels of the list, whose tap entries are contiguous: it walks them 64 at a
+// time (coalesced tap and weight loads, one residual gather per lane, the next
+// chunk's entries loaded before this chunk's products are summed), stages the
+// products og * w in LDS, and each lane adds those of its own texel in entry
+// order - the same sums, in the same order, as one lane walking its texel's
+// entries.
 __global__ __launch_bounds__(kBlock) void k_photo_gather(const RefineBatchTaps* __restrict__ batches, uint32_t nbatches,
                                                          uint32_t tw, uint32_t tiles_x,
                                                          const uint32_t* __restrict__ step_counter,
                                                          const float4* __restrict__ resid, float* __restrict__ grad) {
+    __shared__ float4 prod[kBlock];
     const RefineBatchTaps& bt = batches[*step_counter % nbatches];
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= *bt.runs) return;
-    const uint32_t texel = key_texel(bt.texels[i], tw, tiles_x);
+    const uint32_t runs = *bt.runs;
+    const uint32_t lane = threadIdx.x & 63u, w0 = threadIdx.x & ~63u;
+    const uint32_t first = blockIdx.x * kBlock + w0;
+    if (first >= runs) return;  // wave-uniform
+    const uint32_t i = first + lane, end = first + 64u < runs ? first + 64u : runs;
+    const bool live = i < runs;
+    const uint32_t e_lo = bt.tap_offsets[live ? i : end], e_hi = bt.tap_offsets[live ? i + 1u : end];
+    const uint32_t E0 = bt.tap_offsets[first], E1 = bt.tap_offsets[end];
     float a[3] = {0.0f, 0.0f, 0.0f};
-    gather_taps<2>(bt.tap_offsets[i], bt.tap_offsets[i + 1], bt.taps, bt.tap_weights, resid, a);
+    if (E1 > E0) {
+        const uint32_t top = E1 - 1u;  // clamp: every load stays inside the wave's entries
+        uint32_t en = E0 + lane < top ? E0 + lane : top;
+        uint32_t tp = bt.taps[en];
+        float w = bt.tap_weights[en];
+        for (uint32_t c = E0; c < E1; c += 64u) {
+            const float4 r = resid[tp >> 2];
+            const float wc = c + lane < E1 ? w : 0.0f;
+            en = c + 64u + lane < top ? c + 64u + lane : top;
+            tp = bt.taps[en];
+            w = bt.tap_weights[en];
+            prod[threadIdx.x] = make_float4(r.x * wc, r.y * wc, r.z * wc, 0.0f);
+            wave_lds_sync();
+            const uint32_t s = e_lo > c ? e_lo : c, t = e_hi < c + 64u ? e_hi : c + 64u;
+            for (uint32_t k = s; k < t; ++k) {
+                const float4 q = prod[w0 + (k - c)];
+                a[0] += q.x;
+                a[1] += q.y;
+                a[2] += q.z;
+            }
+            wave_lds_sync();
+        }
+    }
+    if (!live) return;
+    const uint32_t texel = key_texel(bt.texels[i], tw, tiles_x);
     const size_t t = static_cast<size_t>(texel) * 3;
     grad[t] += a[0];
     grad[t + 1] += a[1];
@@ -418,24 +461,32 @@ __device__ __forceinline__ uint32_t tv_mask_of(const int32_t* __restrict__ chart
 
 // d/dt of weight * sum_pairs sqrt((t_b - t_a)^2 + eps^2) / (pairs * 3) over
 // horizontal and vertical neighbour pairs inside one chart, at texel i with
-// neighbour mask `mask` (pairs left, right, up, down in that order).
+// neighbour mask `mask` (pairs left, right, up, down, added in that order).
+// All five texels are loaded before any arithmetic: a neighbour outside the
+// mask reads the texel itself and adds exactly zero.
 __device__ __forceinline__ void tv_texel(const float* __restrict__ tex, uint32_t tw, uint32_t mask, float eps2,
                                          uint32_t i, float g[3]) {
+    const uint32_t nb[4] = {mask & 2u ? i - 1u : i, mask & 4u ? i + 1u : i, mask & 8u ? i - tw : i,
+                            mask & 16u ? i + tw : i};
+    float t[3], u[4][3];
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) t[ch] = tex[static_cast<size_t>(i) * 3 + ch];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int ch = 0; ch < 3; ++ch) u[q][ch] = tex[static_cast<size_t>(nb[q]) * 3 + ch];
     g[0] = g[1] = g[2] = 0.0f;
-    if (mask <= 1u) return;
-    const float* t = tex + static_cast<size_t>(i) * 3;
-    auto pair = [&](uint32_t j, float sign) {
-        const float* u = tex + static_cast<size_t>(j) * 3;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const bool on = (mask & (2u << q)) != 0u;
+        const float sign = (q & 1) ? -1.0f : 1.0f;  // left / up: d = t - u, +d/r; right / down: d = u - t, -d/r
 #pragma unroll
         for (int ch = 0; ch < 3; ++ch) {
-            const float d = sign > 0.0f ? t[ch] - u[ch] : u[ch] - t[ch];  // forward difference of the pair
-            g[ch] += sign * charbonnier_slope(d, eps2);
+            const float d = (q & 1) ? u[q][ch] - t[ch] : t[ch] - u[q][ch];
+            const float s = sign * charbonnier_slope(d, eps2);
+            g[ch] += on ? s : 0.0f;
         }
-    };
-    if (mask & 2u) pair(i - 1, 1.0f);    // pair (x-1, x): d = t - left, dL/dt = +d/r
-    if (mask & 4u) pair(i + 1, -1.0f);   // pair (x, x+1): d = right - t, dL/dt = -d/r
-    if (mask & 8u) pair(i - tw, 1.0f);
-    if (mask & 16u) pair(i + tw, -1.0f);
+    }
 }
 
 __global__ void k_tv_mask(const int32_t* __restrict__ chart, uint32_t tw, uint32_t th, uint8_t* __restrict__ mask) {
@@ -491,17 +542,23 @@ __global__ __launch_bounds__(kBlock) void k_adam_tv(const float* __restrict__ te
     const uint32_t step = *step_counter;
     const float lr = lr_t[step], bc1 = bc1_t[step], bc2 = bc2_t[step];
     for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < tw * th; i += gridDim.x * kBlock) {
+        const size_t b = static_cast<size_t>(i) * 3;
+        float gr[3], m[3], v[3];
+#pragma unroll
+        for (int ch = 0; ch < 3; ++ch) {
+            gr[ch] = grad[b + ch];
+            m[ch] = mom1[b + ch];
+            v[ch] = mom2[b + ch];
+        }
         float g[3];
         tv_texel(tex, tw, tv_mask[i], tv_eps2, i, g);
 #pragma unroll
         for (int ch = 0; ch < 3; ++ch) {
-            const size_t j = static_cast<size_t>(i) * 3 + ch;
-            float gj = grad[j] + g[ch] * tv_scale;
-            float m = mom1[j], v = mom2[j];
-            tex_out[j] = adam_one(tex[j], gj, m, v, lr, b1, b2, eps, bc1, bc2, cmin, cmax);
-            grad[j] = gj;
-            mom1[j] = m;
-            mom2[j] = v;
+            float gj = gr[ch] + g[ch] * tv_scale;
+            tex_out[b + ch] = adam_one(tex[b + ch], gj, m[ch], v[ch], lr, b1, b2, eps, bc1, bc2, cmin, cmax);
+            grad[b + ch] = gj;
+            mom1[b + ch] = m[ch];
+            mom2[b + ch] = v[ch];
         }
     }
 }
