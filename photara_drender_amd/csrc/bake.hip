@@ -74,20 +74,46 @@ __device__ __forceinline__ float bake_mask(const BakeView& v, float px, float py
 }
 
 // project_texture.hlsl:75-99: share of the PCF neighbourhood whose shadow-map
-// depth plus the view's depth bias is not in front of the sample.
+// depth plus the view's depth bias is not in front of the sample.  kR >= 0 is
+// the radius fixed at compile time: every tap is loaded (outside taps from a
+// clamped address) before any is tested.  kR < 0 takes the radius r at run time.
+template <int kR>
 __device__ __forceinline__ float bake_shadow(const BakeView& v, float px, float py, float depth, int32_t r) {
     const int32_t cx = f2i_sat(floorf(px)), cy = f2i_sat(floorf(py));
+    const int32_t W = static_cast<int32_t>(v.width), H = static_cast<int32_t>(v.height);
+    const float2* sm = reinterpret_cast<const float2*>(v.shadow);
     float visible = 0.0f, count = 0.0f;
-    for (int32_t y = -r; y <= r; ++y)
-        for (int32_t x = -r; x <= r; ++x) {
-            const int32_t qx = cx + x, qy = cy + y;
-            if (qx < 0 || qy < 0 || qx >= static_cast<int32_t>(v.width) || qy >= static_cast<int32_t>(v.height)) continue;
-            const float2 s = reinterpret_cast<const float2*>(v.shadow)[(static_cast<size_t>(qy) * v.width + qx) * 2 + 1];
-            if (decode_id(s.y) != 0u) {
-                visible += depth <= s.x + v.depth_bias ? 1.0f : 0.0f;
+    if constexpr (kR >= 0) {
+        constexpr int kD = 2 * kR + 1;
+        float2 s[kD * kD];
+        bool in[kD * kD];
+#pragma unroll
+        for (int y = 0; y < kD; ++y)
+#pragma unroll
+            for (int x = 0; x < kD; ++x) {
+                const int32_t qx = cx + x - kR, qy = cy + y - kR;
+                in[y * kD + x] = qx >= 0 && qy >= 0 && qx < W && qy < H;
+                const int32_t lx = qx < 0 ? 0 : (qx >= W ? W - 1 : qx), ly = qy < 0 ? 0 : (qy >= H ? H - 1 : qy);
+                s[y * kD + x] = sm[(static_cast<size_t>(ly) * v.width + lx) * 2 + 1];
+            }
+#pragma unroll
+        for (int k = 0; k < kD * kD; ++k)
+            if (in[k] && decode_id(s[k].y) != 0u) {
+                visible += depth <= s[k].x + v.depth_bias ? 1.0f : 0.0f;
                 count += 1.0f;
             }
-        }
+    } else {
+        for (int32_t y = -r; y <= r; ++y)
+            for (int32_t x = -r; x <= r; ++x) {
+                const int32_t qx = cx + x, qy = cy + y;
+                if (qx < 0 || qy < 0 || qx >= W || qy >= H) continue;
+                const float2 s = sm[(static_cast<size_t>(qy) * v.width + qx) * 2 + 1];
+                if (decode_id(s.y) != 0u) {
+                    visible += depth <= s.x + v.depth_bias ? 1.0f : 0.0f;
+                    count += 1.0f;
+                }
+            }
+    }
     return count > 0.0f ? visible / count : 0.0f;
 }
 
@@ -98,6 +124,7 @@ struct BakeSample {
     float vis, cosv, dl, cr, cg, cb, alpha;
 };
 
+template <int kR>
 __device__ __forceinline__ bool bake_sample(const BakeView& v, float x, float y, float z, float nx, float ny, float nz,
                                             int32_t pcf, BakeSample& s) {
     const float* m = v.m;
@@ -116,14 +143,16 @@ __device__ __forceinline__ bool bake_sample(const BakeView& v, float x, float y,
     const float vx = dx / dl, vy = dy / dl, vz = dz / dl;
     const float cosv = nx * vx + ny * vy + nz * vz;
     if (cosv < v.min_cos) return false;
-    float vis = bake_shadow(v, px, py, ndz, pcf);
+    // the photo taps (always inside the image) are read with the shadow and
+    // mask taps, before the visibility that may discard them is known
+    float col[4];
+    bake_photo_all(v, px, py, col);
+    float vis = bake_shadow<kR>(v, px, py, ndz, pcf);
     if (v.mask) vis *= clampf(bake_mask(v, px, py), 0.0f, 1.0f);
     if (vis <= 0.0f) return false;
     s.vis = vis;
     s.cosv = cosv;
     s.dl = dl;
-    float col[4];
-    bake_photo_all(v, px, py, col);
     s.cr = col[0];
     s.cg = col[1];
     s.cb = col[2];
@@ -144,6 +173,7 @@ __device__ __forceinline__ void texel_frame(const float* __restrict__ apos, cons
     nrm[2] = n2 / nl;
 }
 
+template <int kR>
 __global__ __launch_bounds__(kBlock) void k_bake_project(const float* __restrict__ araster,
                                                          const float* __restrict__ apos,
                                                          const float* __restrict__ anrm, uint32_t P,
@@ -161,7 +191,7 @@ __global__ __launch_bounds__(kBlock) void k_bake_project(const float* __restrict
     for (uint32_t k = 0; k < nviews; ++k) {
         const BakeView& v = views[k];
         BakeSample s;
-        if (!bake_sample(v, pos[0], pos[1], pos[2], nrm[0], nrm[1], nrm[2], pcf, s)) continue;
+        if (!bake_sample<kR>(v, pos[0], pos[1], pos[2], nrm[0], nrm[1], nrm[2], pcf, s)) continue;
         const float vis = s.vis, cosv = s.cosv, dl = s.dl, cr = s.cr, cg = s.cg, cb = s.cb, alpha = s.alpha;
         if (blend == 2) {  // project_texture.hlsl:176-206, online softmax
             const float dist = maxf_std(dl, 1e-6f);
@@ -220,6 +250,7 @@ __global__ __launch_bounds__(kBlock) void k_bake_project(const float* __restrict
 // cos]; unused slots are zero.  feats [P, 40] carries the list between launches.
 constexpr int kFeatureViews = 8;
 
+template <int kR>
 __global__ __launch_bounds__(kBlock) void k_bake_features(const float* __restrict__ araster,
                                                           const float* __restrict__ apos,
                                                           const float* __restrict__ anrm, uint32_t P,
@@ -242,7 +273,7 @@ __global__ __launch_bounds__(kBlock) void k_bake_features(const float* __restric
     for (uint32_t k = 0; k < nviews; ++k) {
         const BakeView& v = views[k];
         BakeSample s;
-        if (!bake_sample(v, pos[0], pos[1], pos[2], nrm[0], nrm[1], nrm[2], pcf, s)) continue;
+        if (!bake_sample<kR>(v, pos[0], pos[1], pos[2], nrm[0], nrm[1], nrm[2], pcf, s)) continue;
         const float conf = s.vis * s.alpha * s.cosv * v.weight;  // project_texture.hlsl:208
         if (conf <= 0.0f) continue;
         // insertion into the descending list: the carried entry swaps with every smaller slot
@@ -311,15 +342,34 @@ hipError_t bake_init(uint32_t P, float* acc3, float* accw, float* best, uint32_t
 hipError_t bake_project(const float* araster, const float* apos, const float* anrm, uint32_t P, const BakeView* views,
                         uint32_t nviews, uint32_t blend, uint32_t pcf, float* acc3, float* accw, float* best,
                         uint32_t* src, hipStream_t st) {
-    k_bake_project<<<grid_of(P), kBlock, 0, st>>>(araster, apos, anrm, P, views, nviews, blend,
-                                                  static_cast<int32_t>(pcf), acc3, accw, best, src);
+    auto go = [&](auto kernel) {
+        kernel<<<grid_of(P), kBlock, 0, st>>>(araster, apos, anrm, P, views, nviews, blend, static_cast<int32_t>(pcf),
+                                              acc3, accw, best, src);
+    };
+    if (pcf == 0)
+        go(k_bake_project<0>);
+    else if (pcf == 1)
+        go(k_bake_project<1>);
+    else if (pcf == 2)
+        go(k_bake_project<2>);
+    else
+        go(k_bake_project<-1>);
     return hipGetLastError();
 }
 
 hipError_t bake_features(const float* araster, const float* apos, const float* anrm, uint32_t P,
                          const BakeView* views, uint32_t nviews, uint32_t pcf, float* feats, hipStream_t st) {
-    k_bake_features<<<grid_of(P), kBlock, 0, st>>>(araster, apos, anrm, P, views, nviews, static_cast<int32_t>(pcf),
-                                                   feats);
+    auto go = [&](auto kernel) {
+        kernel<<<grid_of(P), kBlock, 0, st>>>(araster, apos, anrm, P, views, nviews, static_cast<int32_t>(pcf), feats);
+    };
+    if (pcf == 0)
+        go(k_bake_features<0>);
+    else if (pcf == 1)
+        go(k_bake_features<1>);
+    else if (pcf == 2)
+        go(k_bake_features<2>);
+    else
+        go(k_bake_features<-1>);
     return hipGetLastError();
 }
 

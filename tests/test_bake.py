@@ -145,6 +145,23 @@ def test_gpu_bake_matches_oracle(mode):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("pcf", [0, 2, 3])
+def test_gpu_bake_pcf_radii_match_oracle(pcf):
+    """Radii 0 and 2 run kernels with the radius fixed at compile time, 3 the run-time loop."""
+    import photara_drender_amd as pdr
+
+    sc, nrm, mats, cams, images, masks = bake_scene()
+    kw = dict(resolution=(64, 64), blend_mode="weighted_average", mask_floor=0.1, pcf_radius=pcf)
+    g = pdr.TextureBaker().bake(sc.positions, nrm, sc.uv, sc.indices, images, mats, cams, masks, None,
+                                visibility_mode="shadow_map", **kw)
+    r = oracle.bake(sc.positions, nrm, sc.uv, sc.indices, images, mats, cams, masks, **kw)
+    assert g[3].sum() > 100
+    np.testing.assert_array_equal(g[0].view(np.uint32), r[0].view(np.uint32))
+    np.testing.assert_array_equal(g[1].view(np.uint32), r[1].view(np.uint32))
+    np.testing.assert_array_equal(g[2], r[2])
+
+
+@pytest.mark.gpu
 def test_gpu_bake_validation_messages():
     import photara_drender_amd as pdr
 
