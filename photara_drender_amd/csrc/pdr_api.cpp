@@ -33,6 +33,12 @@ pdr_status pdr_context_create(uint32_t device_index, int enable_validation, pdr_
         ctx->device = static_cast<int>(device_index);
         hip_check(hipSetDevice(ctx->device), "hipSetDevice");
         hip_check(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "hipStreamCreate");
+        hipMemPool_t pool = nullptr;  // keep freed temporaries of bake / refine calls for reuse
+        if (hipDeviceGetDefaultMemPool(&pool, ctx->device) == hipSuccess && pool) {
+            uint64_t keep = ~0ull;
+            (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+        }
+        (void)hipGetLastError();
         hip_check(hipEventCreateWithFlags(&ctx->handoff, hipEventDisableTiming), "hipEventCreate");
         hip_check(hipHostMalloc(reinterpret_cast<void**>(&ctx->pinned), 4096, hipHostMallocDefault), "hipHostMalloc");
         *out = ctx.release();

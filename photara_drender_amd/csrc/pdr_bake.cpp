@@ -137,7 +137,7 @@ void run_bake(pdr_context ctx, const float* positions, const float* normals, con
         check_indices_device(ctx, idx, static_cast<size_t>(T) * 3, V, st, PDR_OUT_OF_RANGE,
                              "triangle_indices contains an invalid vertex index");
     hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");  // device inputs may be in flight elsewhere
-    DeviceBuffers bufs;
+    DeviceBuffers bufs(st);
     const float* dpos = device_view(bufs, positions, static_cast<size_t>(V) * 3, st);
     const float* dnrm = device_view(bufs, normals, static_cast<size_t>(V) * 3, st);
     const float* duv = device_view(bufs, uv, static_cast<size_t>(V) * 2, st);
@@ -203,7 +203,7 @@ void run_bake(pdr_context ctx, const float* positions, const float* normals, con
     float* shadows = bufs.alloc<float>(max_chunk_pixels * 4);
     for (uint32_t k0 = 0; k0 < nviews; k0 += kBakeChunk) {
         const uint32_t n = std::min(nviews - k0, kBakeChunk);
-        DeviceBuffers chunk_bufs;
+        DeviceBuffers chunk_bufs(st);
         std::vector<pdr::BakeView> hv(n);
         pdr::ViewTable vt{};
         vt.n = n;

@@ -494,15 +494,23 @@ inline uint32_t validate_texture(uint32_t tw, uint32_t th, uint32_t C, uint32_t 
 }
 
 // Call-local device allocations, freed when the call returns.
+// A call's temporary device arrays, allocated and freed in the order of the
+// call's stream from the device's default memory pool.  The context keeps that
+// pool's freed memory (its release threshold is raised at context creation),
+// so repeated calls reuse it instead of mapping and unmapping pages.
 struct DeviceBuffers {
+    explicit DeviceBuffers(hipStream_t s) : stream(s) {}
+    DeviceBuffers(const DeviceBuffers&) = delete;
+    DeviceBuffers& operator=(const DeviceBuffers&) = delete;
+    hipStream_t stream;
     std::vector<void*> ptrs;
     ~DeviceBuffers() {
-        for (void* p : ptrs) (void)hipFree(p);
+        for (void* p : ptrs) (void)hipFreeAsync(p, stream);
     }
     template <class T>
     T* alloc(size_t count) {
         void* p = nullptr;
-        hip_check(hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)), "hipMalloc");
+        hip_check(hipMallocAsync(&p, std::max<size_t>(count, 1) * sizeof(T), stream), "hipMallocAsync");
         ptrs.push_back(p);
         return static_cast<T*>(p);
     }

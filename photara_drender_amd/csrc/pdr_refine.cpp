@@ -94,7 +94,7 @@ pdr_status pdr_refine(pdr_context ctx, const float* initial_texture, const float
 
         Lock lock(ctx);
         const hipStream_t st = ctx->use_stream(false, nullptr);
-        DeviceBuffers bufs;
+        DeviceBuffers bufs(st);
         if (idx_device)
             check_indices_device(ctx, idx, static_cast<size_t>(T) * 3, V, st, PDR_INVALID_ARGUMENT,
                                  "texture refinement mesh arrays have inconsistent shapes");
