@@ -84,23 +84,26 @@ __device__ __forceinline__ uint32_t* word_at(uint32_t* base, uint32_t i) {
 // field[W * H] is a discard slot.  Rows in bands of 64, one single-wave
 // workgroup per band, lane i = row 64 b + i settling column x = t - 2 i at its
 // step t.  Inside a band the row above is the neighbouring lane: its last
-// three results come through one shift by a lane per step (no barrier).  Lane
-// 0 reads the row above the band from `edge`, which the band above publishes
-// every kChunk steps: its last row's results (agent-scope stores), then the
-// number of columns done (a release store of progress[b]); the band below
-// waits with acquire loads before each chunk.  A wait gives up after a bounded
-// number of polls (status[0] = 1, and every later wait returns at once), so a
-// launch always ends.  Inputs are loaded kChunk steps ahead into a register
-// ring; every lane issues the same load and store each step (clamped
-// addresses, the discard slot).
+// three results come through one DPP shift by a lane per step (no barrier or
+// LDS).  Lane 0 reads the row above the band from `edge`: the band above stores each
+// result of its last row there as one 64-bit word, `tag` in the high half
+// (relaxed agent-scope atomic stores, which reach every XCD).  Before each
+// chunk of kChunk steps the band below polls the words it needs until all
+// carry the tag: a word is written whole, so no fence orders data and flag,
+// and no L2 is written back or invalidated.  The caller clears `edge` (tag 0)
+// before the sweeps and gives each sweep its own non-zero tag.  A wait gives
+// up after a bounded number of polls (status[0] = 1, and every later wait
+// returns at once), so a launch always ends.  Inputs are loaded kChunk steps
+// ahead into a register ring; every lane issues the same load and store each
+// step (clamped addresses, the discard slot).
 constexpr uint32_t kBandRows = 64;
 constexpr uint32_t kChunk = 32;
 constexpr uint32_t kMaxPolls = 1u << 20;
 
 template <bool kMirror>
 __global__ __launch_bounds__(64) void k_pad_band_sweep(uint32_t W, uint32_t H, const uint8_t* __restrict__ source,
-                                                        uint32_t* __restrict__ field, uint32_t* progress,
-                                                        uint32_t* edge, uint32_t* status) {
+                                                        uint32_t* __restrict__ field, unsigned long long* edge,
+                                                        uint32_t tag, uint32_t* status) {
     const uint32_t band = blockIdx.x, lane = threadIdx.x;
     const uint32_t y = band * kBandRows + lane;  // row in sweep (possibly mirrored) coordinates
     const uint32_t rows = H - band * kBandRows < kBandRows ? H - band * kBandRows : kBandRows;
@@ -108,8 +111,9 @@ __global__ __launch_bounds__(64) void k_pad_band_sweep(uint32_t W, uint32_t H, c
     const bool publish = band * kBandRows + kBandRows < H;  // a band below reads this band's last row
     const uint32_t ry = kMirror ? H - 1u - (y < H ? y : H - 1u) : (y < H ? y : H - 1u);
     const Offset none = {kNone16, kNone16};
-    uint32_t* my_edge = edge + static_cast<size_t>(band) * W;
-    const uint32_t* up_edge = edge + static_cast<size_t>(band ? band - 1u : 0u) * W;
+    unsigned long long* my_edge = edge + static_cast<size_t>(band) * W;
+    const unsigned long long* up_edge = edge + static_cast<size_t>(band ? band - 1u : 0u) * W;
+    const unsigned long long tag_hi = static_cast<unsigned long long>(tag) << 32;
     // inputs of this lane's texels at the steps of the current and the next chunk
     uint32_t in[kChunk];
     auto load_input = [&](uint32_t x) {
@@ -125,36 +129,35 @@ __global__ __launch_bounds__(64) void k_pad_band_sweep(uint32_t W, uint32_t H, c
     uint32_t edge_vals = 0u;                 // lanes 0..kChunk+1: row above the band, columns t0 - 1 + lane
     for (uint32_t t0 = 0; t0 < steps; t0 += kChunk) {
         if (band > 0) {
-            // columns up to t0 + kChunk of the row above are needed in this chunk
-            const uint32_t need = t0 + kChunk + 1u < W ? t0 + kChunk + 1u : W;
-            if (lane == 0) {
-                uint32_t polls = 0;
-                while (__hip_atomic_load(&progress[band - 1u], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < need) {
-                    if (++polls >= kMaxPolls ||
-                        __hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
-                        __hip_atomic_store(status, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                        break;
-                    }
-                    __builtin_amdgcn_s_sleep(2);
-                }
-            }
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            // columns t0 - 1 .. t0 + kChunk of the row above are needed in this chunk
             const int32_t col = static_cast<int32_t>(t0) - 1 + static_cast<int32_t>(lane);
-            edge_vals = col >= 0 && col < static_cast<int32_t>(W) && lane < kChunk + 2u
-                            ? __hip_atomic_load(&up_edge[col], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                            : as_word(none);
+            const bool want = col >= 0 && col < static_cast<int32_t>(W) && lane < kChunk + 2u;
+            unsigned long long v = 0ull;
+            uint32_t polls = 0;
+            while (true) {
+                if (want) v = __hip_atomic_load(&up_edge[col], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (__all(!want || (v >> 32) == tag)) break;
+                const uint32_t failed = __builtin_amdgcn_readfirstlane(
+                    __hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+                if (++polls >= kMaxPolls || failed != 0u) {
+                    if (lane == 0) __hip_atomic_store(status, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(2);
+            }
+            edge_vals = want ? static_cast<uint32_t>(v) : as_word(none);
+            // lane 0: the row above the band at columns t0 + 1, t0, t0 - 1
+            const uint32_t e0 = __builtin_amdgcn_readlane(edge_vals, 0), e1 = __builtin_amdgcn_readlane(edge_vals, 1),
+                           e2 = __builtin_amdgcn_readlane(edge_vals, 2);
+            up1 = lane == 0 ? as_offset(e2) : up1;
+            up2 = lane == 0 ? as_offset(e1) : up2;
+            up3 = lane == 0 ? as_offset(e0) : up3;
         }
 #pragma unroll
         for (uint32_t s = 0; s < kChunk; ++s) {
             const uint32_t t = t0 + s;
             const int32_t x = static_cast<int32_t>(t) - 2 * static_cast<int32_t>(lane);
             const bool settle = x >= 0 && x < static_cast<int32_t>(W) && y < H;
-            if (lane == 0 && band > 0) {
-                // row above the band at columns x + 1, x, x - 1 (x = t for lane 0)
-                up1 = as_offset(__builtin_amdgcn_readlane(edge_vals, static_cast<int>(s + 2)));
-                up2 = as_offset(__builtin_amdgcn_readlane(edge_vals, static_cast<int>(s + 1)));
-                up3 = as_offset(__builtin_amdgcn_readlane(edge_vals, static_cast<int>(s)));
-            }
             Offset cur;
             if constexpr (kMirror) {
                 cur = -as_offset(in[s]);
@@ -173,22 +176,19 @@ __global__ __launch_bounds__(64) void k_pad_band_sweep(uint32_t W, uint32_t H, c
             const uint32_t rx = kMirror ? W - 1u - static_cast<uint32_t>(x) : static_cast<uint32_t>(x);
             *word_at(field, settle ? ry * W + rx : W * H) = as_word(kMirror ? -cur : cur);
             if (publish && lane == kBandRows - 1u && settle)
-                __hip_atomic_store(&my_edge[x], as_word(cur), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            // the next step's neighbours: own result on the left, the lane above's results above
+                __hip_atomic_store(&my_edge[x], tag_hi | as_word(cur), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            // the next step's neighbours: own result on the left, the lane above's
+            // results above (a DPP shift of the wave by one lane, wave_shr:1), and for
+            // lane 0 the row above the band at column x + 2 (edge_vals lane s + 3)
             prev = res;
             up3 = up2;
             up2 = up1;
-            up1 = as_offset(static_cast<uint32_t>(__shfl_up(static_cast<int>(as_word(res)), 1, 64)));
+            const uint32_t feed = __builtin_amdgcn_readlane(edge_vals, static_cast<int>(s + 3));
+            up1 = as_offset(static_cast<uint32_t>(
+                __builtin_amdgcn_update_dpp(static_cast<int>(feed), static_cast<int>(as_word(res)), 0x138, 0xf, 0xf, false)));
             in[s] = load_input(static_cast<uint32_t>(x + static_cast<int32_t>(kChunk)));
         }
-        if (publish && lane == kBandRows - 1u) {
-            // columns of the last row settled so far (step t settles column t - 2 * 63)
-            const int32_t done = static_cast<int32_t>(t0 + kChunk) - 2 * static_cast<int32_t>(kBandRows - 1u);
-            const uint32_t cols = done <= 0 ? 0u : (static_cast<uint32_t>(done) < W ? static_cast<uint32_t>(done) : W);
-            __hip_atomic_store(&progress[band], cols, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        }
     }
-    if (publish && lane == kBandRows - 1u) __hip_atomic_store(&progress[band], W, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Fills (:289-348).  Sources are never written; colours are read from `src_color`
@@ -241,20 +241,16 @@ hipError_t pad_prepare(uint32_t P, const float* valid, const float* coverage, fl
 
 uint32_t pad_band_count(uint32_t H) { return (H + kBandRows - 1u) / kBandRows; }
 
-hipError_t pad_field_bands(uint32_t W, uint32_t H, const uint8_t* source, uint32_t* field, uint32_t* progress,
-                           uint32_t* edge, uint32_t* status, hipStream_t st) {
+hipError_t pad_field_bands(uint32_t W, uint32_t H, const uint8_t* source, uint32_t* field, unsigned long long* edge,
+                           uint32_t* status, hipStream_t st) {
     const uint32_t bands = pad_band_count(H);
     hipError_t e = fill_u32(status, 0u, 1, st);
     if (e != hipSuccess) return e;
-    for (int mirror = 0; mirror < 2; ++mirror) {
-        if ((e = fill_u32(progress, 0u, bands, st)) != hipSuccess) return e;
-        if (mirror)
-            k_pad_band_sweep<true><<<bands, 64, 0, st>>>(W, H, source, field, progress, edge, status);
-        else
-            k_pad_band_sweep<false><<<bands, 64, 0, st>>>(W, H, source, field, progress, edge, status);
-        if ((e = hipGetLastError()) != hipSuccess) return e;
-    }
-    return hipSuccess;
+    if ((e = fill_u64(edge, 0ull, static_cast<size_t>(bands) * W, st)) != hipSuccess) return e;  // tag 0: nothing yet
+    k_pad_band_sweep<false><<<bands, 64, 0, st>>>(W, H, source, field, edge, 1u, status);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    k_pad_band_sweep<true><<<bands, 64, 0, st>>>(W, H, source, field, edge, 2u, status);
+    return hipGetLastError();
 }
 
 hipError_t pad_fill(uint32_t W, uint32_t H, uint32_t C, const uint32_t* field, const uint8_t* source,

@@ -270,12 +270,12 @@ hipError_t mlp_reduce(const float* part, uint32_t waves, float* grad, float* los
 hipError_t pad_prepare(uint32_t P, const float* valid, const float* coverage, float* filled, float threshold,
                        uint8_t* source, unsigned long long* counters, hipStream_t st);
 // The 8SSEDT field (field: W * H + 1 entries, the last written as a discard
-// slot), rows in bands of 64 on separate workgroups: progress
-// [pad_band_count(H)], edge [pad_band_count(H) * W], status[1] (set to 1 when a
-// band gave up waiting for the band above; the field is then invalid).
+// slot), rows in bands of 64 on separate workgroups handing their last rows on
+// through edge [pad_band_count(H) * W] (tagged 64-bit words), status[1] (set to
+// 1 when a band gave up waiting for the band above; the field is then invalid).
 uint32_t pad_band_count(uint32_t H);
-hipError_t pad_field_bands(uint32_t W, uint32_t H, const uint8_t* source, uint32_t* field, uint32_t* progress,
-                           uint32_t* edge, uint32_t* status, hipStream_t st);
+hipError_t pad_field_bands(uint32_t W, uint32_t H, const uint8_t* source, uint32_t* field, unsigned long long* edge,
+                           uint32_t* status, hipStream_t st);
 hipError_t pad_fill(uint32_t W, uint32_t H, uint32_t C, const uint32_t* field, const uint8_t* source,
                     const float* coverage, uint32_t margin, int fill_unobserved, uint32_t max_fill,
                     const float* src_color, float* color, float* filled, unsigned long long* counters, hipStream_t st);

@@ -67,11 +67,10 @@ pdr_status pdr_pad_texture_atlas(pdr_context ctx, float* color, uint32_t width, 
         if (c[0] != 0 || c[4] != 0) {  // :277-281: nothing to extrapolate from otherwise
             // 8SSEDT field: rows in bands of 64 on separate workgroups, each waiting for the band above
             const uint32_t bands = pdr::pad_band_count(H);
-            uint32_t* progress = ctx->get_t<uint32_t>("pad_band_progress", bands);
-            uint32_t* edge = ctx->get_t<uint32_t>("pad_band_edge", static_cast<size_t>(bands) * W);
+            unsigned long long* edge = ctx->get_t<unsigned long long>("pad_band_edge", static_cast<size_t>(bands) * W);
             uint32_t* status = ctx->get_t<uint32_t>("pad_band_status", 1);
             ctx->timed("pad_field", st, [&] {
-                hip_check(pdr::pad_field_bands(W, H, source, field, progress, edge, status, st), "pad_field_bands");
+                hip_check(pdr::pad_field_bands(W, H, source, field, edge, status, st), "pad_field_bands");
             });
             uint32_t gave_up = 0;
             hip_check(hipMemcpyAsync(&gave_up, status, sizeof(gave_up), hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
