@@ -162,11 +162,19 @@ hipError_t sum_partials(const double* partials, uint32_t n, double* out, hipStre
 uint32_t photometric_partials(uint32_t batch, uint32_t max_active);
 // Residual pass over the step's views, then the texel gather of the step's
 // batch (batches[step % nbatches]); resid holds batch slots of `stride` entries.
+// The joint seam term's residual (seam_step's first kernel), run as one more
+// slice of the photometric residual launch when npairs > 0; no loss is kept.
+struct SeamResidualArgs {
+    const float* pairs = nullptr;
+    uint32_t npairs = 0;
+    float eps = 0.0f, norm = 0.0f, weight = 0.0f;
+    float4* resid = nullptr;
+};
 hipError_t photometric_step(const RefineView* views_dev, uint32_t batch, uint32_t max_active,
                             const RefineBatchTaps* batches_dev, uint32_t nbatches, uint32_t max_runs,
                             const float* tex, uint32_t tw, uint32_t th, float eps, const RefineStepTables& tables,
                             const uint32_t* step_counter, float4* resid, uint32_t stride, float* grad,
-                            float* loss_part, hipStream_t st);
+                            float* loss_part, const SeamResidualArgs& seam, hipStream_t st);
 // Active pixels of one view in 8x8-tile order: flags over tile-order slots,
 // then (after an exclusive scan of the flags) the packed records.
 uint32_t tile_slots(uint32_t W, uint32_t H);
@@ -205,7 +213,7 @@ struct TapGroups {
 };
 hipError_t seam_step(const float* tex, uint32_t tw, uint32_t th, const float* pairs, uint32_t npairs, float eps,
                      float norm, float weight, const uint32_t* step_counter, uint32_t loss_base, const TapGroups& taps,
-                     float4* resid, float* grad, float* loss, hipStream_t st);
+                     float4* resid, float* grad, float* loss, bool with_residual, hipStream_t st);
 hipError_t seam_taps(const float* pairs, uint32_t npairs, uint32_t tw, uint32_t th, uint32_t* keys, uint32_t* vals,
                      hipStream_t st);
 hipError_t seam_tap_weights(const uint32_t* vals, uint32_t E, const float* pairs, uint32_t tw, uint32_t th, float* w,

@@ -369,14 +369,20 @@ pdr_status pdr_refine(pdr_context ctx, const float* initial_texture, const float
         const bool fused_tv = o.tv_weight != 0.0f;
         float* texbuf[2] = {tex, fused_tv ? bufs.alloc<float>(n) : tex};
         const auto t2 = std::chrono::steady_clock::now();
+        const bool joint_seam = o.joint_seam_weight != 0.0f && npairs;
         auto one_step = [&](hipStream_t s, uint32_t c) {
+            pdr::SeamResidualArgs seam_res;  // the joint seam residual rides in the photometric residual launch
+            if (joint_seam)
+                seam_res = pdr::SeamResidualArgs{dpairs, npairs, o.seam_epsilon, seam_norm, o.joint_seam_weight,
+                                                 seam_resid};
             hip_check(pdr::photometric_step(dviews, o.batch_size, max_active, dbatches, nbatches, max_runs, texbuf[c],
                                             o.width, o.height, o.photometric_epsilon, tables, counters, resid,
-                                            max_active, grad, loss_part, s),
+                                            max_active, grad, loss_part, seam_res, s),
                       "photometric_step");
-            if (o.joint_seam_weight != 0.0f && npairs)
+            if (joint_seam)
                 hip_check(pdr::seam_step(texbuf[c], o.width, o.height, dpairs, npairs, o.seam_epsilon, seam_norm,
-                                         o.joint_seam_weight, counters, 0, seam_groups, seam_resid, grad, nullptr, s),
+                                         o.joint_seam_weight, counters, 0, seam_groups, seam_resid, grad, nullptr,
+                                         false, s),
                           "seam_step");
             if (fused_tv)
                 hip_check(pdr::adam_tv_step(texbuf[c], texbuf[c ^ 1u], grad, m1, m2, o.width, o.height, dtv_mask, tv_eps,
@@ -441,7 +447,7 @@ pdr_status pdr_refine(pdr_context ctx, const float* initial_texture, const float
             hip_check(pdr::fill_u32(counters, 0u, 2, st), "fill");
             for (uint32_t k = 0; k < seam_steps; ++k) {
                 hip_check(pdr::seam_step(tex, o.width, o.height, dpairs, npairs, o.seam_epsilon, seam_norm, 1.0f,
-                                         counters, o.steps, seam_groups, seam_resid, grad, loss, st),
+                                         counters, o.steps, seam_groups, seam_resid, grad, loss, true, st),
                           "seam_step");
                 hip_check(pdr::adam_step(tex, grad, m1, m2, n, d_slr, d_sb1, d_sb2, o.adam_beta1, o.adam_beta2,
                                          o.adam_epsilon, o.clamp_min, o.clamp_max, counters, st),

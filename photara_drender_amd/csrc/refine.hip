@@ -121,13 +121,47 @@ __device__ __forceinline__ AtlasTap atlas_tap(float u, float v, uint32_t tw, uin
     return a;
 }
 
+// One seam pair's residual: the output gradient d / r * norm (* weight) per
+// channel into resid[q]; returns the pair's loss sum_ch r * norm.
+__device__ __forceinline__ float seam_pair_residual(uint32_t q, const float* __restrict__ tex, uint32_t tw, uint32_t th,
+                                                    const float* __restrict__ pairs, float eps, float norm,
+                                                    float weight, float4* __restrict__ resid) {
+    const float4 pr = reinterpret_cast<const float4*>(pairs)[q];
+    const AtlasTap a0 = atlas_tap(pr.x, pr.y, tw, th);
+    const AtlasTap a1 = atlas_tap(pr.z, pr.w, tw, th);
+    float g3[3];
+    float local = 0.0f;
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) {
+        float c0 = 0.0f, c1 = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            c0 += tex[static_cast<size_t>(a0.t[k]) * 3 + ch] * a0.w[k];
+            c1 += tex[static_cast<size_t>(a1.t[k]) * 3 + ch] * a1.w[k];
+        }
+        const float d = c0 - c1;
+        const float r = sqrtf(d * d + eps * eps);
+        float g = d / r * norm;
+        local += r * norm;
+        if (weight != 1.0f) g = g * weight;
+        g3[ch] = g;
+    }
+    resid[q] = make_float4(g3[0], g3[1], g3[2], 0.0f);
+    return local;
+}
+
 __global__ __launch_bounds__(kBlock) void k_photo_residual(const RefineView* __restrict__ views, uint32_t batch,
                                                            const float* __restrict__ tex, uint32_t tw, uint32_t th,
                                                            float eps, RefineStepTables tables,
                                                            const uint32_t* __restrict__ step_counter,
                                                            float4* __restrict__ resid, uint32_t max_active,
-                                                           float* __restrict__ loss_part) {
+                                                           float* __restrict__ loss_part, SeamResidualArgs seam) {
     __shared__ float wsum[kBlock / 64];
+    if (blockIdx.y == batch) {  // the joint seam term's residual, beside the views' (no loss)
+        for (uint32_t q = blockIdx.x * kBlock + threadIdx.x; q < seam.npairs; q += gridDim.x * kBlock)
+            (void)seam_pair_residual(q, tex, tw, th, seam.pairs, seam.eps, seam.norm, seam.weight, seam.resid);
+        return;
+    }
     const uint32_t step = *step_counter;
     const uint32_t b = blockIdx.y;
     const RefineView& rv = views[tables.views[static_cast<size_t>(step) * batch + b]];
@@ -359,29 +393,7 @@ __global__ __launch_bounds__(kBlock) void k_seam_residual(const float* __restric
                                                           uint32_t loss_base, float4* __restrict__ resid,
                                                           float* __restrict__ loss) {
     const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
-    float local = 0.0f;
-    if (q < npairs) {
-        const float4 pr = reinterpret_cast<const float4*>(pairs)[q];
-        const AtlasTap a0 = atlas_tap(pr.x, pr.y, tw, th);
-        const AtlasTap a1 = atlas_tap(pr.z, pr.w, tw, th);
-        float g3[3];
-#pragma unroll
-        for (int ch = 0; ch < 3; ++ch) {
-            float c0 = 0.0f, c1 = 0.0f;
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                c0 += tex[static_cast<size_t>(a0.t[k]) * 3 + ch] * a0.w[k];
-                c1 += tex[static_cast<size_t>(a1.t[k]) * 3 + ch] * a1.w[k];
-            }
-            const float d = c0 - c1;
-            const float r = sqrtf(d * d + eps * eps);
-            float g = d / r * norm;
-            local += r * norm;
-            if (weight != 1.0f) g = g * weight;
-            g3[ch] = g;
-        }
-        resid[q] = make_float4(g3[0], g3[1], g3[2], 0.0f);
-    }
+    const float local = q < npairs ? seam_pair_residual(q, tex, tw, th, pairs, eps, norm, weight, resid) : 0.0f;
     const float s = wave_tree_sum(local);
     if (loss && (threadIdx.x & 63u) == 0 && s != 0.0f) unsafeAtomicAdd(&loss[loss_base + *step_counter], s);
 }
@@ -672,11 +684,11 @@ hipError_t photometric_step(const RefineView* views_dev, uint32_t batch, uint32_
                             const RefineBatchTaps* batches_dev, uint32_t nbatches, uint32_t max_runs,
                             const float* tex, uint32_t tw, uint32_t th, float eps, const RefineStepTables& tables,
                             const uint32_t* step_counter, float4* resid, uint32_t stride, float* grad,
-                            float* loss_part, hipStream_t st) {
+                            float* loss_part, const SeamResidualArgs& seam, hipStream_t st) {
     const uint32_t blocks = (max_active + kBlock - 1) / kBlock;
-    const dim3 grid(blocks ? blocks : 1, batch);
+    const dim3 grid(blocks ? blocks : 1, batch + (seam.npairs ? 1u : 0u));
     k_photo_residual<<<grid, kBlock, 0, st>>>(views_dev, batch, tex, tw, th, eps, tables, step_counter, resid, stride,
-                                              loss_part);
+                                              loss_part, seam);
     const uint32_t gblocks = (max_runs + kBlock - 1) / kBlock;
     k_photo_gather<<<gblocks ? gblocks : 1, kBlock, 0, st>>>(batches_dev, nbatches, tw, texel_key_tiles(tw, th),
                                                              step_counter, resid, grad);
@@ -772,10 +784,11 @@ hipError_t photo_group_taps(const uint32_t* keys_in, const uint32_t* vals_in, ui
 
 hipError_t seam_step(const float* tex, uint32_t tw, uint32_t th, const float* pairs, uint32_t npairs, float eps,
                      float norm, float weight, const uint32_t* step_counter, uint32_t loss_base, const TapGroups& taps,
-                     float4* resid, float* grad, float* loss, hipStream_t st) {
+                     float4* resid, float* grad, float* loss, bool with_residual, hipStream_t st) {
     if (!npairs) return hipSuccess;
-    k_seam_residual<<<(npairs + kBlock - 1) / kBlock, kBlock, 0, st>>>(tex, tw, th, pairs, npairs, eps, norm, weight,
-                                                                      step_counter, loss_base, resid, loss);
+    if (with_residual)
+        k_seam_residual<<<(npairs + kBlock - 1) / kBlock, kBlock, 0, st>>>(tex, tw, th, pairs, npairs, eps, norm,
+                                                                          weight, step_counter, loss_base, resid, loss);
     if (taps.ntexels)
         k_seam_gather<<<(taps.ntexels + kBlock - 1) / kBlock, kBlock, 0, st>>>(
             taps.texels, taps.offsets, taps.taps, taps.weights, taps.ntexels, resid, grad);
