@@ -251,21 +251,22 @@ __device__ __forceinline__ void wave_lds_sync() {
 
 // Texel gradient of the step's batch: one lane per texel the batch samples,
 // its taps from all slots added in list order.  A wave owns 64 consecutive
-// texels of the list, whose tap entries are contiguous: it walks them 64 at a
-// time (coalesced tap and weight loads, one residual gather per lane, the next
-// chunk's entries loaded before this chunk's products are summed), stages the
-// products og * w in LDS, and each lane adds those of its own texel in entry
-// order - the same sums, in the same order, as one lane walking its texel's
-// entries.
+// texels of the list, whose tap entries are contiguous: it walks them 128 at a
+// time, two entries per lane (coalesced tap and weight loads, two residual
+// gathers in flight per lane, the next window's entries loaded before this
+// window's products are summed), stages the products og * w in LDS, and each
+// lane adds those of its own texel in entry order - the same sums, in the same
+// order, as one lane walking its texel's entries.
+constexpr uint32_t kGatherWindow = 128;
 __global__ __launch_bounds__(kBlock) void k_photo_gather(const RefineBatchTaps* __restrict__ batches, uint32_t nbatches,
                                                          uint32_t tw, uint32_t tiles_x,
                                                          const uint32_t* __restrict__ step_counter,
                                                          const float4* __restrict__ resid, float* __restrict__ grad) {
-    __shared__ float4 prod[kBlock];
+    __shared__ float4 prod[kBlock / 64][kGatherWindow];
     const RefineBatchTaps& bt = batches[*step_counter % nbatches];
     const uint32_t runs = *bt.runs;
-    const uint32_t lane = threadIdx.x & 63u, w0 = threadIdx.x & ~63u;
-    const uint32_t first = blockIdx.x * kBlock + w0;
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t first = blockIdx.x * kBlock + wave * 64u;
     if (first >= runs) return;  // wave-uniform
     const uint32_t i = first + lane, end = first + 64u < runs ? first + 64u : runs;
     const bool live = i < runs;
@@ -274,20 +275,36 @@ __global__ __launch_bounds__(kBlock) void k_photo_gather(const RefineBatchTaps* 
     float a[3] = {0.0f, 0.0f, 0.0f};
     if (E1 > E0) {
         const uint32_t top = E1 - 1u;  // clamp: every load stays inside the wave's entries
-        uint32_t en = E0 + lane < top ? E0 + lane : top;
-        uint32_t tp = bt.taps[en];
-        float w = bt.tap_weights[en];
-        for (uint32_t c = E0; c < E1; c += 64u) {
-            const float4 r = resid[tp >> 2];
-            const float wc = c + lane < E1 ? w : 0.0f;
-            en = c + 64u + lane < top ? c + 64u + lane : top;
-            tp = bt.taps[en];
-            w = bt.tap_weights[en];
-            prod[threadIdx.x] = make_float4(r.x * wc, r.y * wc, r.z * wc, 0.0f);
+        auto entry = [&](uint32_t e) { return e < top ? e : top; };
+        uint32_t tp[2], en[2];
+        float w[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            en[h] = entry(E0 + 64u * h + lane);
+            tp[h] = bt.taps[en[h]];
+            w[h] = bt.tap_weights[en[h]];
+        }
+        for (uint32_t c = E0; c < E1; c += kGatherWindow) {
+            float4 r[2];
+            float wc[2];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                r[h] = resid[tp[h] >> 2];
+                wc[h] = c + 64u * h + lane < E1 ? w[h] : 0.0f;
+            }
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                en[h] = entry(c + kGatherWindow + 64u * h + lane);
+                tp[h] = bt.taps[en[h]];
+                w[h] = bt.tap_weights[en[h]];
+            }
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+                prod[wave][64u * h + lane] = make_float4(r[h].x * wc[h], r[h].y * wc[h], r[h].z * wc[h], 0.0f);
             wave_lds_sync();
-            const uint32_t s = e_lo > c ? e_lo : c, t = e_hi < c + 64u ? e_hi : c + 64u;
+            const uint32_t s = e_lo > c ? e_lo : c, t = e_hi < c + kGatherWindow ? e_hi : c + kGatherWindow;
             for (uint32_t k = s; k < t; ++k) {
-                const float4 q = prod[w0 + (k - c)];
+                const float4 q = prod[wave][k - c];
                 a[0] += q.x;
                 a[1] += q.y;
                 a[2] += q.z;
