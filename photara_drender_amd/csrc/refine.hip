@@ -251,13 +251,14 @@ __device__ __forceinline__ void wave_lds_sync() {
 
 // Texel gradient of the step's batch: one lane per texel the batch samples,
 // its taps from all slots added in list order.  A wave owns 64 consecutive
-// texels of the list, whose tap entries are contiguous: it walks them 128 at a
-// time, two entries per lane (coalesced tap and weight loads, two residual
+// texels of the list, whose tap entries are contiguous: it walks them 256 at a
+// time, four entries per lane (coalesced tap and weight loads, four residual
 // gathers in flight per lane, the next window's entries loaded before this
 // window's products are summed), stages the products og * w in LDS, and each
 // lane adds those of its own texel in entry order - the same sums, in the same
 // order, as one lane walking its texel's entries.
-constexpr uint32_t kGatherWindow = 128;
+constexpr uint32_t kGatherPer = 4;  // entries per lane per window
+constexpr uint32_t kGatherWindow = 64u * kGatherPer;
 __global__ __launch_bounds__(kBlock) void k_photo_gather(const RefineBatchTaps* __restrict__ batches, uint32_t nbatches,
                                                          uint32_t tw, uint32_t tiles_x,
                                                          const uint32_t* __restrict__ step_counter,
@@ -276,30 +277,30 @@ __global__ __launch_bounds__(kBlock) void k_photo_gather(const RefineBatchTaps* 
     if (E1 > E0) {
         const uint32_t top = E1 - 1u;  // clamp: every load stays inside the wave's entries
         auto entry = [&](uint32_t e) { return e < top ? e : top; };
-        uint32_t tp[2], en[2];
-        float w[2];
+        uint32_t tp[kGatherPer], en[kGatherPer];
+        float w[kGatherPer];
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
+        for (uint32_t h = 0; h < kGatherPer; ++h) {
             en[h] = entry(E0 + 64u * h + lane);
             tp[h] = bt.taps[en[h]];
             w[h] = bt.tap_weights[en[h]];
         }
         for (uint32_t c = E0; c < E1; c += kGatherWindow) {
-            float4 r[2];
-            float wc[2];
+            float4 r[kGatherPer];
+            float wc[kGatherPer];
 #pragma unroll
-            for (int h = 0; h < 2; ++h) {
+            for (uint32_t h = 0; h < kGatherPer; ++h) {
                 r[h] = resid[tp[h] >> 2];
                 wc[h] = c + 64u * h + lane < E1 ? w[h] : 0.0f;
             }
 #pragma unroll
-            for (int h = 0; h < 2; ++h) {
+            for (uint32_t h = 0; h < kGatherPer; ++h) {
                 en[h] = entry(c + kGatherWindow + 64u * h + lane);
                 tp[h] = bt.taps[en[h]];
                 w[h] = bt.tap_weights[en[h]];
             }
 #pragma unroll
-            for (int h = 0; h < 2; ++h)
+            for (uint32_t h = 0; h < kGatherPer; ++h)
                 prod[wave][64u * h + lane] = make_float4(r[h].x * wc[h], r[h].y * wc[h], r[h].z * wc[h], 0.0f);
             wave_lds_sync();
             const uint32_t s = e_lo > c ? e_lo : c, t = e_hi < c + kGatherWindow ? e_hi : c + kGatherWindow;
