@@ -199,35 +199,6 @@ __global__ __launch_bounds__(kBlock) void k_photo_residual(const RefineView* __r
     if (threadIdx.x == 0) *part = (wsum[0] + wsum[1]) + (wsum[2] + wsum[3]);
 }
 
-// acc += sum over entries [e0, e1) of resid[tap >> kShift] * weight, in entry
-// order; the loads of four entries are issued together.
-template <int kShift>
-__device__ __forceinline__ void gather_taps(uint32_t e0, uint32_t e1, const uint32_t* __restrict__ taps,
-                                            const float* __restrict__ weights, const float4* __restrict__ resid,
-                                            float acc[3]) {
-    for (uint32_t e = e0; e < e1; e += 4) {
-        uint32_t tp[4];
-        float w[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const bool on = e + k < e1;
-            tp[k] = on ? taps[e + k] : 0u;
-            w[k] = on ? weights[e + k] : 0.0f;
-        }
-        float4 r[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
-            r[k] = e + k < e1 ? resid[tp[k] >> kShift] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            if (e + k >= e1) break;
-            acc[0] += r[k].x * w[k];
-            acc[1] += r[k].y * w[k];
-            acc[2] += r[k].z * w[k];
-        }
-    }
-}
-
 // Texel t of a tw-wide atlas as a key in 8x8-tile order (tiles_x = tiles per
 // row; 0: the key is t itself).  Sorting taps by it makes the gather's 64 lanes
 // take an 8x8 texel block, whose taps come from a compact image region.
@@ -259,20 +230,22 @@ __device__ __forceinline__ void wave_lds_sync() {
 // order, as one lane walking its texel's entries.
 constexpr uint32_t kGatherPer = 4;  // entries per lane per window
 constexpr uint32_t kGatherWindow = 64u * kGatherPer;
-__global__ __launch_bounds__(kBlock) void k_photo_gather(const RefineBatchTaps* __restrict__ batches, uint32_t nbatches,
-                                                         uint32_t tw, uint32_t tiles_x,
-                                                         const uint32_t* __restrict__ step_counter,
-                                                         const float4* __restrict__ resid, float* __restrict__ grad) {
-    __shared__ float4 prod[kBlock / 64][kGatherWindow];
-    const RefineBatchTaps& bt = batches[*step_counter % nbatches];
-    const uint32_t runs = *bt.runs;
+// The gather of one wave over texels [first, first + 64) of a texel list
+// (texel keys, tap offsets with one entry past the last texel, taps, tap
+// weights; tap >> kShift indexes the residuals; tiles_x = 0: keys are texels).
+template <uint32_t kShift>
+__device__ __forceinline__ void texel_gather(const uint32_t* __restrict__ texels, const uint32_t* __restrict__ offsets,
+                                             const uint32_t* __restrict__ taps, const float* __restrict__ weights,
+                                             uint32_t runs, const float4* __restrict__ resid, uint32_t tw,
+                                             uint32_t tiles_x, float* __restrict__ grad,
+                                             float4 (*__restrict__ prod)[kGatherWindow]) {
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     const uint32_t first = blockIdx.x * kBlock + wave * 64u;
     if (first >= runs) return;  // wave-uniform
     const uint32_t i = first + lane, end = first + 64u < runs ? first + 64u : runs;
     const bool live = i < runs;
-    const uint32_t e_lo = bt.tap_offsets[live ? i : end], e_hi = bt.tap_offsets[live ? i + 1u : end];
-    const uint32_t E0 = bt.tap_offsets[first], E1 = bt.tap_offsets[end];
+    const uint32_t e_lo = offsets[live ? i : end], e_hi = offsets[live ? i + 1u : end];
+    const uint32_t E0 = offsets[first], E1 = offsets[end];
     float a[3] = {0.0f, 0.0f, 0.0f};
     if (E1 > E0) {
         const uint32_t top = E1 - 1u;  // clamp: every load stays inside the wave's entries
@@ -282,22 +255,22 @@ __global__ __launch_bounds__(kBlock) void k_photo_gather(const RefineBatchTaps* 
 #pragma unroll
         for (uint32_t h = 0; h < kGatherPer; ++h) {
             en[h] = entry(E0 + 64u * h + lane);
-            tp[h] = bt.taps[en[h]];
-            w[h] = bt.tap_weights[en[h]];
+            tp[h] = taps[en[h]];
+            w[h] = weights[en[h]];
         }
         for (uint32_t c = E0; c < E1; c += kGatherWindow) {
             float4 r[kGatherPer];
             float wc[kGatherPer];
 #pragma unroll
             for (uint32_t h = 0; h < kGatherPer; ++h) {
-                r[h] = resid[tp[h] >> 2];
+                r[h] = resid[tp[h] >> kShift];
                 wc[h] = c + 64u * h + lane < E1 ? w[h] : 0.0f;
             }
 #pragma unroll
             for (uint32_t h = 0; h < kGatherPer; ++h) {
                 en[h] = entry(c + kGatherWindow + 64u * h + lane);
-                tp[h] = bt.taps[en[h]];
-                w[h] = bt.tap_weights[en[h]];
+                tp[h] = taps[en[h]];
+                w[h] = weights[en[h]];
             }
 #pragma unroll
             for (uint32_t h = 0; h < kGatherPer; ++h)
@@ -314,11 +287,19 @@ __global__ __launch_bounds__(kBlock) void k_photo_gather(const RefineBatchTaps* 
         }
     }
     if (!live) return;
-    const uint32_t texel = key_texel(bt.texels[i], tw, tiles_x);
-    const size_t t = static_cast<size_t>(texel) * 3;
+    const size_t t = static_cast<size_t>(key_texel(texels[i], tw, tiles_x)) * 3;
     grad[t] += a[0];
     grad[t + 1] += a[1];
     grad[t + 2] += a[2];
+}
+
+__global__ __launch_bounds__(kBlock) void k_photo_gather(const RefineBatchTaps* __restrict__ batches, uint32_t nbatches,
+                                                         uint32_t tw, uint32_t tiles_x,
+                                                         const uint32_t* __restrict__ step_counter,
+                                                         const float4* __restrict__ resid, float* __restrict__ grad) {
+    __shared__ float4 prod[kBlock / 64][kGatherWindow];
+    const RefineBatchTaps& bt = batches[*step_counter % nbatches];
+    texel_gather<2>(bt.texels, bt.tap_offsets, bt.taps, bt.tap_weights, *bt.runs, resid, tw, tiles_x, grad, prod);
 }
 
 // Tile-order slot q of a W x H view: tile q / 64 (row-major over 8x8 tiles),
@@ -423,14 +404,8 @@ __global__ __launch_bounds__(kBlock) void k_seam_gather(const uint32_t* __restri
                                                         const uint32_t* __restrict__ taps,
                                                         const float* __restrict__ weights, uint32_t ntexels,
                                                         const float4* __restrict__ resid, float* __restrict__ grad) {
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= ntexels) return;
-    float a[3] = {0.0f, 0.0f, 0.0f};
-    gather_taps<3>(offsets[i], offsets[i + 1], taps, weights, resid, a);
-    const size_t t = static_cast<size_t>(texels[i]) * 3;
-    grad[t] += a[0];
-    grad[t + 1] += a[1];
-    grad[t + 2] += a[2];
+    __shared__ float4 prod[kBlock / 64][kGatherWindow];
+    texel_gather<3>(texels, offsets, taps, weights, ntexels, resid, 0u, 0u, grad, prod);
 }
 
 __global__ void k_seam_taps(const float* __restrict__ pairs, uint32_t npairs, uint32_t tw, uint32_t th,
