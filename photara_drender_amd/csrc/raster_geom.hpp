@@ -93,8 +93,10 @@ __host__ __device__ inline bool halo_bounds(const ScreenTri& r, double vx, doubl
                                             int32_t* hx1, int32_t* hy0, int32_t* hy1, double a[3], double b[3],
                                             double c[3]) {
     const double S[3][2] = {{r.s0x, r.s0y}, {r.s1x, r.s1y}, {r.s2x, r.s2y}};
-    auto ndc_x = [&](double px) { return ((px - vx) + 0.5) * 2.0 / vw - 1.0; };
-    auto ndc_y = [&](double py) { return ((py - vy) + 0.5) * 2.0 / vh - 1.0; };
+    // reciprocals instead of divisions: one more rounding (2^-53 relative) is far inside the pads below
+    const double sxv = 2.0 / vw, syv = 2.0 / vh;
+    auto ndc_x = [&](double px) { return ((px - vx) + 0.5) * sxv - 1.0; };
+    auto ndc_y = [&](double py) { return ((py - vy) + 0.5) * syv - 1.0; };
     const double nx0 = ndc_x(cx0), nx1 = ndc_x(cx1), ny0 = ndc_y(cy0), ny1 = ndc_y(cy1);
     double M = 0.0, cmax = 0.0;
     for (int j = 0; j < 3; ++j) {
@@ -118,8 +120,9 @@ __host__ __device__ inline bool halo_bounds(const ScreenTri& r, double vx, doubl
         const double det = a[i] * b[j] - a[j] * b[i];
         const double scale = (fabs(a[i]) + fabs(b[i])) * (fabs(a[j]) + fabs(b[j]));
         if (!(fabs(det) > 1e-9 * scale)) return false;
-        const double x = (c[i] * b[j] - c[j] * b[i]) / det;
-        const double y = (a[i] * c[j] - a[j] * c[i]) / det;
+        const double inv = 1.0 / det;
+        const double x = (c[i] * b[j] - c[j] * b[i]) * inv;
+        const double y = (a[i] * c[j] - a[j] * c[i]) * inv;
         if (!__builtin_isfinite(x) || !__builtin_isfinite(y)) return false;
         xmin = fmin(xmin, x);
         xmax = fmax(xmax, x);
@@ -154,8 +157,9 @@ __host__ __device__ inline void make_spans(const double a[3], const double b[3],
         sp.u[i] = 0.0f;
         sp.v[i] = 0.0f;
         if (!(fabs(along) >= 1e-3 * fabs(across)) || along == 0.0) continue;
-        sp.u[i] = static_cast<float>(alpha * (c[i] / along) + beta);
-        sp.v[i] = static_cast<float>(-alpha * (across / along));
+        const double r = 1.0 / along;  // the float conversion below rounds far more than the reciprocal
+        sp.u[i] = static_cast<float>(alpha * (c[i] * r) + beta);
+        sp.v[i] = static_cast<float>(-alpha * (across * r));
         if (along > 0.0)
             sp.lower |= 1u << i;
         else
