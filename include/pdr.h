@@ -50,9 +50,13 @@ enum {
      * work (costs one stream synchronisation).  Host inputs are always checked,
      * as the reference does.  Default: on. */
     PDR_FLAG_VALIDATE_DEVICE_INPUTS = 1u << 0,
-    /* Gradient reductions in the reference's summation order (per-triangle /
-     * per-texel gathers over sorted adjacency lists) instead of atomics.
-     * Bit-reproducible; slower.  Default: off. */
+    /* The attribute gradient of pdr_interpolate_backward(_batched) and the
+     * texel gradient of pdr_texture_backward as ordered gathers instead of
+     * atomics: per vertex / texel over its pixels in ascending order (taps in
+     * order within a pixel), the reference's summation order.  Bit-reproducible;
+     * slower (a radix sort of 3 or 4 entries per pixel, at most 2^30 pixels per
+     * call).  The raster backward is deterministic with or without it.
+     * Default: off. */
     PDR_FLAG_DETERMINISTIC = 1u << 1,
     /* Autotune the raster's launch parameters (wave / queue thresholds,
      * triangles per wave): the first call of each workload class (view count,

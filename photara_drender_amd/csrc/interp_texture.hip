@@ -16,6 +16,8 @@
 //    reference's double-precision bilinear fractions.
 // Each scattered term is the reference's term exactly; only the order in which
 // terms are summed differs.
+#include <hipcub/hipcub.hpp>
+
 #include "pdr_internal.hpp"
 
 namespace pdr {
@@ -591,6 +593,140 @@ __global__ __launch_bounds__(kBlock) void k_texture_check_uv(PixelTable pt, uint
 
 uint32_t pixel_grid(uint32_t n) { return (n + kBlock - 1) / kBlock; }
 
+
+// ------------------------------------------------------ deterministic gradients
+// PDR_FLAG_DETERMINISTIC: the attribute and texel gradients as ordered gathers
+// instead of atomics.  Each covered pixel emits one (vertex, pixel) entry per
+// distinct vertex of its triangle, or one (texel, 4 * pixel + tap) entry per
+// bilinear tap, in slots ordered by pixel; unused slots carry a key past the
+// last vertex / texel.  A stable radix sort by key groups the entries with each
+// group in ascending pixel order, a binary search gives each group's start,
+// and one lane per vertex / texel adds its entries in that order: the sums of
+// the reference's per-vertex pixel loop and of its texel adjacency lists
+// (src/rasterizer.cpp:187-258, texture_grad_texture.hlsl:15-38), bit for bit.
+
+__global__ void k_det_attr_entries(PixelTable pt, const uint32_t* __restrict__ idx, uint32_t T, uint32_t V,
+                                   uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    const uint32_t gp = blockIdx.x * kBlock + threadIdx.x;
+    if (gp >= pt.total_pixels) return;
+    const PixelView& pv = pt.v[find_pixel_view(pt, gp)];
+    const uint32_t id = decode_id(pv.raster[static_cast<size_t>(gp - pv.pixel_base) * 4 + 3]);
+    uint32_t tri[3] = {V, V, V};
+    if (id != 0u && id <= T)
+        for (int c = 0; c < 3; ++c) tri[c] = idx[static_cast<size_t>(id - 1u) * 3 + c];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        bool seen = false;
+        for (int k = 0; k < c; ++k) seen |= tri[k] == tri[c];
+        keys[static_cast<size_t>(gp) * 3 + c] = !seen && tri[c] < V ? tri[c] : V;
+        vals[static_cast<size_t>(gp) * 3 + c] = gp;
+    }
+}
+
+__global__ void k_det_tex_entries(PixelTable pt, uint32_t tw, uint32_t th, uint32_t mode, uint32_t* __restrict__ keys,
+                                  uint32_t* __restrict__ vals) {
+    const uint32_t gp = blockIdx.x * kBlock + threadIdx.x;
+    if (gp >= pt.total_pixels) return;
+    const PixelView& pv = pt.v[find_pixel_view(pt, gp)];
+    const uint32_t p = gp - pv.pixel_base;
+    const float2 uv = reinterpret_cast<const float2*>(pv.in0)[p];
+    const bool on = pv.raster[static_cast<size_t>(p) * 4 + 3] != 0.0f && __builtin_isfinite(uv.x) &&
+                    __builtin_isfinite(uv.y);
+    const uint32_t none = tw * th;
+    uint32_t o[4] = {none, none, none, none};
+    if (on) {
+        const Bilinear b = bilinear_f64(uv.x, uv.y, tw, th, mode);
+        o[0] = b.o00;
+        o[1] = b.o10;
+        o[2] = b.o01;
+        o[3] = b.o11;
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < 4; ++k) {
+        keys[static_cast<size_t>(gp) * 4 + k] = o[k];
+        vals[static_cast<size_t>(gp) * 4 + k] = gp * 4u + k;
+    }
+}
+
+// starts[q] = first entry with key >= q (sorted keys), q = 0 .. n
+__global__ void k_det_starts(const uint32_t* __restrict__ keys, uint32_t E, uint32_t n, uint32_t* __restrict__ starts) {
+    const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
+    if (q > n) return;
+    uint32_t lo = 0, hi = E;
+    while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo) / 2u;
+        if (keys[mid] < q)
+            lo = mid + 1u;
+        else
+            hi = mid;
+    }
+    starts[q] = lo;
+}
+
+// orc / reference order: weight = [tri0 == v] u + [tri1 == v] w + [tri2 == v] (1 - u - w), summed over the
+// vertex's pixels in ascending order, per attribute.
+__global__ void k_det_attr_gather(PixelTable pt, const uint32_t* __restrict__ idx, uint32_t V, uint32_t A,
+                                  const uint32_t* __restrict__ starts, const uint32_t* __restrict__ vals,
+                                  float* __restrict__ grad_attr) {
+    const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
+    if (v >= V) return;
+    const uint32_t e0 = starts[v], e1 = starts[v + 1];
+    for (uint32_t a0 = 0; a0 < A; a0 += 8) {
+        float acc[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        for (uint32_t e = e0; e < e1; ++e) {
+            const uint32_t gp = vals[e];
+            const PixelView& pv = pt.v[find_pixel_view(pt, gp)];
+            const size_t p = gp - pv.pixel_base;
+            const float4 r = reinterpret_cast<const float4*>(pv.raster)[p];
+            const uint32_t* tri = idx + static_cast<size_t>(decode_id(r.w) - 1u) * 3;
+            float weight = 0.0f;
+            if (tri[0] == v) weight += r.x;
+            if (tri[1] == v) weight += r.y;
+            if (tri[2] == v) weight += 1.0f - r.x - r.y;
+            const float* g = pv.in0 + p * A;
+#pragma unroll
+            for (uint32_t j = 0; j < 8; ++j)
+                if (a0 + j < A) acc[j] += weight * g[a0 + j];
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < 8; ++j)
+            if (a0 + j < A) grad_attr[static_cast<size_t>(v) * A + a0 + j] = acc[j];
+    }
+}
+
+// texture_grad_texture.hlsl:15-38 over the texel's (pixel, tap) entries in ascending order.
+__global__ void k_det_tex_gather(PixelTable pt, uint32_t tw, uint32_t th, uint32_t C, uint32_t mode,
+                                 const uint32_t* __restrict__ starts, const uint32_t* __restrict__ vals,
+                                 float* __restrict__ grad_tex) {
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= tw * th) return;
+    const uint32_t e0 = starts[t], e1 = starts[t + 1];
+    for (uint32_t c0 = 0; c0 < C; c0 += 4) {
+        float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        for (uint32_t e = e0; e < e1; ++e) {
+            const uint32_t gp = vals[e] >> 2, k = vals[e] & 3u;
+            const PixelView& pv = pt.v[find_pixel_view(pt, gp)];
+            const size_t p = gp - pv.pixel_base;
+            const float2 uv = reinterpret_cast<const float2*>(pv.in0)[p];
+            const Bilinear b = bilinear_f64(uv.x, uv.y, tw, th, mode);
+            const float w = k == 0u ? b.w0 : (k == 1u ? b.w1 : (k == 2u ? b.w2 : b.w3));
+            const float* g = pv.in1 + p * C;
+#pragma unroll
+            for (uint32_t j = 0; j < 4; ++j)
+                if (c0 + j < C) acc[j] += w * g[c0 + j];
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < 4; ++j)
+            if (c0 + j < C) grad_tex[static_cast<size_t>(t) * C + c0 + j] = acc[j];
+    }
+}
+
+int key_bits_for(uint32_t n) {  // bits covering keys 0 .. n
+    int bits = 1;
+    while (bits < 32 && (1ull << bits) <= n) ++bits;
+    return bits;
+}
+
 }  // namespace
 
 hipError_t interp_forward(const PixelTable& pt, const float* attr, uint32_t V, uint32_t A, const uint32_t* idx,
@@ -671,6 +807,52 @@ hipError_t texture_backward(const PixelTable& pt, const float* tex, uint32_t tw,
 hipError_t texture_check_uv(const PixelTable& pt, uint32_t tw, uint32_t th, uint32_t* flags, hipStream_t st) {
     if (!pt.total_pixels) return hipSuccess;
     k_texture_check_uv<<<pixel_grid(pt.total_pixels), kBlock, 0, st>>>(pt, tw, th, flags);
+    return hipGetLastError();
+}
+
+size_t deterministic_temp_bytes(uint32_t entries) {
+    size_t a = 0;
+    (void)hipcub::DeviceRadixSort::SortPairs(nullptr, a, static_cast<const uint32_t*>(nullptr),
+                                             static_cast<uint32_t*>(nullptr), static_cast<const uint32_t*>(nullptr),
+                                             static_cast<uint32_t*>(nullptr), static_cast<int>(entries), 0, 32);
+    return a;
+}
+
+static hipError_t det_sort_starts(const DeterministicScratch& s, uint32_t E, uint32_t nkeys, hipStream_t st) {
+    size_t bytes = s.temp_bytes;
+    hipError_t e = hipcub::DeviceRadixSort::SortPairs(s.temp, bytes, s.keys_in, s.keys_out, s.vals_in, s.vals_out,
+                                                      static_cast<int>(E), 0, key_bits_for(nkeys), st);
+    if (e != hipSuccess) return e;
+    k_det_starts<<<(nkeys + 1 + kBlock - 1) / kBlock, kBlock, 0, st>>>(s.keys_out, E, nkeys, s.starts);
+    return hipGetLastError();
+}
+
+hipError_t interp_backward_deterministic(const PixelTable& pt, uint32_t V, uint32_t A, const uint32_t* idx, uint32_t T,
+                                         float* grad_attr, const DeterministicScratch& s, hipStream_t st) {
+    const uint32_t E = pt.total_pixels * 3u;
+    if (E) {
+        k_det_attr_entries<<<pixel_grid(pt.total_pixels), kBlock, 0, st>>>(pt, idx, T, V, s.keys_in, s.vals_in);
+        hipError_t e = det_sort_starts(s, E, V, st);
+        if (e != hipSuccess) return e;
+    } else {
+        k_det_starts<<<(V + 1 + kBlock - 1) / kBlock, kBlock, 0, st>>>(s.keys_out, 0, V, s.starts);
+    }
+    k_det_attr_gather<<<(V + kBlock - 1) / kBlock, kBlock, 0, st>>>(pt, idx, V, A, s.starts, s.vals_out, grad_attr);
+    return hipGetLastError();
+}
+
+hipError_t texture_backward_deterministic(const PixelTable& pt, uint32_t tw, uint32_t th, uint32_t C, uint32_t mode,
+                                          float* grad_tex, const DeterministicScratch& s, hipStream_t st) {
+    const uint32_t E = pt.total_pixels * 4u, texels = tw * th;
+    if (E) {
+        k_det_tex_entries<<<pixel_grid(pt.total_pixels), kBlock, 0, st>>>(pt, tw, th, mode, s.keys_in, s.vals_in);
+        hipError_t e = det_sort_starts(s, E, texels, st);
+        if (e != hipSuccess) return e;
+    } else {
+        k_det_starts<<<(texels + 1 + kBlock - 1) / kBlock, kBlock, 0, st>>>(s.keys_out, 0, texels, s.starts);
+    }
+    k_det_tex_gather<<<(texels + kBlock - 1) / kBlock, kBlock, 0, st>>>(pt, tw, th, C, mode, s.starts, s.vals_out,
+                                                                      grad_tex);
     return hipGetLastError();
 }
 

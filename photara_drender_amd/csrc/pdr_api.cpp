@@ -396,6 +396,23 @@ pdr_status pdr_interpolate_forward(pdr_context ctx, const float* attr, uint32_t 
     });
 }
 
+// Scratch of the ordered (PDR_FLAG_DETERMINISTIC) gradient gathers: `per` entries
+// per pixel, keys 0 .. nkeys.
+static pdr::DeterministicScratch det_scratch(pdr_context ctx, uint64_t pixels, uint32_t per, uint32_t nkeys) {
+    if (pixels * 4ull >= 0xFFFFFFFFull)
+        raise(PDR_OVERFLOW, "deterministic gradients support at most 2^30 pixels per call");
+    const uint32_t E = static_cast<uint32_t>(pixels * per);
+    pdr::DeterministicScratch s;
+    s.keys_in = ctx->get_t<uint32_t>("det_keys_in", E ? E : 1);
+    s.keys_out = ctx->get_t<uint32_t>("det_keys_out", E ? E : 1);
+    s.vals_in = ctx->get_t<uint32_t>("det_vals_in", E ? E : 1);
+    s.vals_out = ctx->get_t<uint32_t>("det_vals_out", E ? E : 1);
+    s.starts = ctx->get_t<uint32_t>("det_starts", static_cast<size_t>(nkeys) + 1);
+    s.temp_bytes = pdr::deterministic_temp_bytes(E ? E : 1);
+    s.temp = ctx->get_t<unsigned char>("det_temp", s.temp_bytes ? s.temp_bytes : 1);
+    return s;
+}
+
 pdr_status pdr_interpolate_backward(pdr_context ctx, const float* attr, uint32_t V, uint32_t A, const uint32_t* idx,
                                     uint32_t T, const float* raster, uint32_t W, uint32_t H, const float* grad,
                                     float* grad_attr, float* grad_raster, void* stream) {
@@ -436,6 +453,13 @@ pdr_status pdr_interpolate_backward(pdr_context ctx, const float* attr, uint32_t
         hip_check(pdr::fill_f32(dga, 0.0f, static_cast<size_t>(V) * A, st), "fill");
         ctx->timed("interp_bwd", st,
                    [&] { hip_check(pdr::interp_backward(pt, dattr, V, A, didx, T, dga, st), "interp_backward"); });
+        if (ctx->flags & PDR_FLAG_DETERMINISTIC) {
+            const pdr::DeterministicScratch ds = det_scratch(ctx, P, 3, V);
+            ctx->timed("interp_bwd_det", st, [&] {
+                hip_check(pdr::interp_backward_deterministic(pt, V, A, didx, T, dga, ds, st),
+                          "interp_backward_deterministic");
+            });
+        }
         if (!device) {
             copy_out(grad_attr, dga, static_cast<size_t>(V) * A * 4, st);
             copy_out(grad_raster, dgr, P * 16, st);
@@ -527,6 +551,13 @@ pdr_status pdr_interpolate_backward_batched(pdr_context ctx, const pdr_interp_vi
         ctx->timed("interp_bwd", st, [&] {
             hip_check(pdr::interp_backward(pt, attr, V, A, idx, T, grad_attr, st), "interp_backward");
         });
+        if (ctx->flags & PDR_FLAG_DETERMINISTIC) {
+            const pdr::DeterministicScratch ds = det_scratch(ctx, pt.total_pixels, 3, V);
+            ctx->timed("interp_bwd_det", st, [&] {
+                hip_check(pdr::interp_backward_deterministic(pt, V, A, idx, T, grad_attr, ds, st),
+                          "interp_backward_deterministic");
+            });
+        }
     });
 }
 
@@ -615,6 +646,13 @@ pdr_status pdr_texture_backward(pdr_context ctx, const float* tex, uint32_t tw, 
         ctx->timed("texture_bwd", st, [&] {
             hip_check(pdr::texture_backward(pt, dtex, tw, th, C, mode, compute_uv, dgt, st), "texture_backward");
         });
+        if (ctx->flags & PDR_FLAG_DETERMINISTIC) {
+            const pdr::DeterministicScratch ds = det_scratch(ctx, static_cast<uint64_t>(W) * H, 4, tw * th);
+            ctx->timed("texture_bwd_det", st, [&] {
+                hip_check(pdr::texture_backward_deterministic(pt, tw, th, C, mode, dgt, ds, st),
+                          "texture_backward_deterministic");
+            });
+        }
         if (!device) {
             copy_out(grad_tex, dgt, static_cast<size_t>(tw) * th * C * 4, st);
             copy_out(grad_uv, dgu, static_cast<size_t>(P) * 8, st);

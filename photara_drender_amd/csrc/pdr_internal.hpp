@@ -115,6 +115,21 @@ hipError_t texture_forward(const PixelTable& views, const float* tex, uint32_t t
                            uint32_t mode, hipStream_t st);
 hipError_t texture_backward(const PixelTable& views, const float* tex, uint32_t tw, uint32_t th, uint32_t C,
                             uint32_t mode, int compute_uv, float* grad_tex, hipStream_t st);
+// PDR_FLAG_DETERMINISTIC: grad_attr / grad_tex rewritten by ordered gathers
+// (bit-reproducible, the reference's summation order).  Scratch: keys and
+// values in / out [entries = 3 or 4 per pixel], starts [V + 1 or texels + 1],
+// radix-sort temp storage of deterministic_temp_bytes(entries).
+struct DeterministicScratch {
+    uint32_t *keys_in, *keys_out, *vals_in, *vals_out, *starts;
+    void* temp;
+    size_t temp_bytes;
+};
+size_t deterministic_temp_bytes(uint32_t entries);
+hipError_t interp_backward_deterministic(const PixelTable& views, uint32_t V, uint32_t A, const uint32_t* idx,
+                                         uint32_t T, float* grad_attr, const DeterministicScratch& s, hipStream_t st);
+hipError_t texture_backward_deterministic(const PixelTable& views, uint32_t tw, uint32_t th, uint32_t C,
+                                          uint32_t mode, float* grad_tex, const DeterministicScratch& s,
+                                          hipStream_t st);
 // uv validity check of texture_backward's covered pixels; flags[0] |= 1 non-finite, 2 too large
 hipError_t texture_check_uv(const PixelTable& views, uint32_t tw, uint32_t th, uint32_t* flags, hipStream_t st);
 // max index check; flags[0] = 1 when any index >= V
