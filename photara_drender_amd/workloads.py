@@ -27,7 +27,7 @@ class RefineProblem:
     masks: list  # per view [H,W] binary
     world_to_clip: np.ndarray  # [N,4,4] f32
     seam_pairs: object  # [P,2,2]
-    chart_texels: np.ndarray  # [S,S] int32
+    chart_texels: object  # [S,S] int32, device-resident like the views
     view_uv: list  # per view [H,W,2] interpolated atlas uv
     view_raster: list  # per view [H,W,4]
     atlas_size: int
@@ -112,7 +112,8 @@ def refine_problem(ctx, nx=1000, ny=500, charts=16, atlas_size=2048, views=64, s
     normals = torch.tensor(scenes.vertex_normals(sc.positions, sc.indices), device=dev)
     cams = np.stack([scenes.camera_center(m) for m in mats])
     return RefineProblem(pos, uv, idx, images, clean, [torch.tensor(m, device=dev) for m in masks_np], mats,
-                         torch.tensor(pairs, device=dev), sc.chart_texels, view_uv, view_raster, atlas_size,
+                         torch.tensor(pairs, device=dev), torch.tensor(sc.chart_texels, device=dev), view_uv,
+                         view_raster, atlas_size,
                          torch.tensor(gt_atlas, device=dev), normals, cams)
 
 
