@@ -152,9 +152,9 @@ struct RefineStepTables {
 // One view of the refinement, reduced to its pixels with mask > 0 ("active"),
 // in 8x8-tile order.
 struct RefineView {
-    const float4* uvm;       // [active] (u, v, mask, 0)
-    const float4* rgb;       // [active] target (r, g, b, 0)
-    const uint32_t* counts;  // device: [0] active pixels
+    const float4* uvm;  // [active] (u, v, mask, 0)
+    const float4* rgb;  // [active] target (r, g, b, 0)
+    uint32_t active;    // pixels with mask > 0 (known on the host once precompute has read the counts back)
 };
 
 // The bilinear taps of one recurring step batch (the views of its batch slots)

@@ -173,7 +173,7 @@ pdr_status pdr_refine(pdr_context ctx, const float* initial_texture, const float
                 arena_off += up(static_cast<size_t>(P) * sizeof(float4));
                 hip_check(pdr::active_pack(vuv, target, mask, v.width, v.height, act_flags, act_pos, uvm, rgb, st),
                           "active_pack");
-                rviews[k] = pdr::RefineView{uvm, rgb, view_counts + k};
+                rviews[k] = pdr::RefineView{uvm, rgb, 0u};  // active: set once the counts are read back
             }
         }
         std::vector<uint32_t> host_counts(nviews);
@@ -181,7 +181,10 @@ pdr_status pdr_refine(pdr_context ctx, const float* initial_texture, const float
                   "hipMemcpyAsync");
         sync(st);
         uint32_t max_active = 0;
-        for (uint32_t k = 0; k < nviews; ++k) max_active = std::max(max_active, host_counts[k]);
+        for (uint32_t k = 0; k < nviews; ++k) {
+            max_active = std::max(max_active, host_counts[k]);
+            rviews[k].active = host_counts[k];
+        }
         if (static_cast<uint64_t>(o.batch_size) * max_active * 4u > 0x7fffffffull)
             raise(PDR_OVERFLOW, "texture refinement batch exceeds 2^29 active pixels");
         pdr::RefineView* dviews = bufs.alloc<pdr::RefineView>(nviews);

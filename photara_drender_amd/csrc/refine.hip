@@ -167,7 +167,7 @@ __global__ __launch_bounds__(kBlock) void k_photo_residual(const RefineView* __r
     const RefineView& rv = views[tables.views[static_cast<size_t>(step) * batch + b]];
     const float norm = tables.norm[step];
     const uint32_t a = blockIdx.x * kBlock + threadIdx.x;
-    const uint32_t active = rv.counts[0];
+    const uint32_t active = rv.active;
     float* part = loss_part + static_cast<size_t>(b) * gridDim.x + blockIdx.x;
     if (blockIdx.x * kBlock >= active) {  // whole block past this view (uniform)
         if (threadIdx.x == 0) *part = 0.0f;
