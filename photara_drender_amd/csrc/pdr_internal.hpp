@@ -185,6 +185,7 @@ struct SeamResidualArgs {
     float eps = 0.0f, norm = 0.0f, weight = 0.0f;
     float4* resid = nullptr;
 };
+// views_dev: [steps * batch] view records, one per step slot (slot b of step s at s * batch + b).
 hipError_t photometric_step(const RefineView* views_dev, uint32_t batch, uint32_t max_active,
                             const RefineBatchTaps* batches_dev, uint32_t nbatches, uint32_t max_runs,
                             const float* tex, uint32_t tw, uint32_t th, float eps, const RefineStepTables& tables,

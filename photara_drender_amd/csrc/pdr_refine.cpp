@@ -187,9 +187,6 @@ pdr_status pdr_refine(pdr_context ctx, const float* initial_texture, const float
         }
         if (static_cast<uint64_t>(o.batch_size) * max_active * 4u > 0x7fffffffull)
             raise(PDR_OVERFLOW, "texture refinement batch exceeds 2^29 active pixels");
-        pdr::RefineView* dviews = bufs.alloc<pdr::RefineView>(nviews);
-        hip_check(hipMemcpyAsync(dviews, rviews.data(), nviews * sizeof(pdr::RefineView), hipMemcpyHostToDevice, st),
-                  "hipMemcpyAsync");
 
         // Gather form of the photometric gradient.  Step s samples views
         // (s * batch + b) % nviews, so the step batches recur with period
@@ -343,6 +340,11 @@ pdr_status pdr_refine(pdr_context ctx, const float* initial_texture, const float
         copy_any(d_bc2, bc2.data(), bc2.size() * 4, st);
         copy_any(d_vidx, vidx.data(), vidx.size() * 4, st);
         hip_check(pdr::norm_table(view_weight, d_vidx, o.steps, o.batch_size, d_norm, st), "norm_table");
+        // each step slot's view record itself (the residual pass then needs no view-index load)
+        std::vector<pdr::RefineView> step_views(vidx.size());
+        for (size_t q = 0; q < vidx.size(); ++q) step_views[q] = rviews[vidx[q]];
+        pdr::RefineView* dviews = bufs.alloc<pdr::RefineView>(step_views.size());
+        copy_any(dviews, step_views.data(), step_views.size() * sizeof(pdr::RefineView), st);
         pdr::RefineStepTables tables{d_lr, d_bc1, d_bc2, d_norm, d_vidx};
 
         // extended objective (python/photara_drender/atlas.py:84-144)

@@ -164,7 +164,7 @@ __global__ __launch_bounds__(kBlock) void k_photo_residual(const RefineView* __r
     }
     const uint32_t step = *step_counter;
     const uint32_t b = blockIdx.y;
-    const RefineView& rv = views[tables.views[static_cast<size_t>(step) * batch + b]];
+    const RefineView& rv = views[static_cast<size_t>(step) * batch + b];  // the step slot's view record
     const float norm = tables.norm[step];
     const uint32_t a = blockIdx.x * kBlock + threadIdx.x;
     const uint32_t active = rv.active;
