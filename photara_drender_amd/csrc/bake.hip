@@ -73,6 +73,18 @@ __device__ __forceinline__ float bake_mask(const BakeView& v, float px, float py
                      fy);
 }
 
+// A shadow-map texel from the view's raster key: (depth, 1) where a triangle
+// covers the pixel, (0, 0) where none does.  The key's high word is the
+// winner's depth clamp(iz / iw, -1, 1) under an order-preserving map (-0 as
+// +0), inverted here exactly - the depth the raster's resolve pass would
+// write, which the bake then needs no pass for.
+__device__ __forceinline__ float2 shadow_texel(unsigned long long key) {
+    if (key == ~0ull) return make_float2(0.0f, 0.0f);
+    const uint32_t hi = static_cast<uint32_t>(key >> 32);
+    const uint32_t u = (hi & 0x80000000u) ? (hi & 0x7FFFFFFFu) : ~hi;
+    return make_float2(__uint_as_float(u), 1.0f);
+}
+
 // project_texture.hlsl:75-99: share of the PCF neighbourhood whose shadow-map
 // depth plus the view's depth bias is not in front of the sample.  kR >= 0 is
 // the radius fixed at compile time: every tap is loaded (outside taps from a
@@ -81,7 +93,7 @@ template <int kR>
 __device__ __forceinline__ float bake_shadow(const BakeView& v, float px, float py, float depth, int32_t r) {
     const int32_t cx = f2i_sat(floorf(px)), cy = f2i_sat(floorf(py));
     const int32_t W = static_cast<int32_t>(v.width), H = static_cast<int32_t>(v.height);
-    const float2* sm = reinterpret_cast<const float2*>(v.shadow);
+    const unsigned long long* sm = v.shadow;
     float visible = 0.0f, count = 0.0f;
     if constexpr (kR >= 0) {
         constexpr int kD = 2 * kR + 1;
@@ -94,11 +106,11 @@ __device__ __forceinline__ float bake_shadow(const BakeView& v, float px, float 
                 const int32_t qx = cx + x - kR, qy = cy + y - kR;
                 in[y * kD + x] = qx >= 0 && qy >= 0 && qx < W && qy < H;
                 const int32_t lx = qx < 0 ? 0 : (qx >= W ? W - 1 : qx), ly = qy < 0 ? 0 : (qy >= H ? H - 1 : qy);
-                s[y * kD + x] = sm[(static_cast<size_t>(ly) * v.width + lx) * 2 + 1];
+                s[y * kD + x] = shadow_texel(sm[static_cast<size_t>(ly) * v.width + lx]);
             }
 #pragma unroll
         for (int k = 0; k < kD * kD; ++k)
-            if (in[k] && decode_id(s[k].y) != 0u) {
+            if (in[k] && s[k].y != 0.0f) {
                 visible += depth <= s[k].x + v.depth_bias ? 1.0f : 0.0f;
                 count += 1.0f;
             }
@@ -107,8 +119,8 @@ __device__ __forceinline__ float bake_shadow(const BakeView& v, float px, float 
             for (int32_t x = -r; x <= r; ++x) {
                 const int32_t qx = cx + x, qy = cy + y;
                 if (qx < 0 || qy < 0 || qx >= W || qy >= H) continue;
-                const float2 s = sm[(static_cast<size_t>(qy) * v.width + qx) * 2 + 1];
-                if (decode_id(s.y) != 0u) {
+                const float2 s = shadow_texel(sm[static_cast<size_t>(qy) * v.width + qx]);
+                if (s.y != 0.0f) {
                     visible += depth <= s.x + v.depth_bias ? 1.0f : 0.0f;
                     count += 1.0f;
                 }

@@ -193,21 +193,12 @@ void run_bake(pdr_context ctx, const float* positions, const float* normals, con
     }
     float* clips = bufs.alloc<float>(static_cast<size_t>(std::min(nviews, kBakeChunk)) * V * 4);
     pdr::BakeView* dviews = bufs.alloc<pdr::BakeView>(kBakeChunk);
-    size_t max_chunk_pixels = 0;
-    for (uint32_t k0 = 0; k0 < nviews; k0 += kBakeChunk) {
-        size_t px = 0;
-        for (uint32_t k = k0; k < std::min(nviews, k0 + kBakeChunk); ++k)
-            px += static_cast<size_t>(views[k].width) * views[k].height;
-        max_chunk_pixels = std::max(max_chunk_pixels, px);
-    }
-    float* shadows = bufs.alloc<float>(max_chunk_pixels * 4);
     for (uint32_t k0 = 0; k0 < nviews; k0 += kBakeChunk) {
         const uint32_t n = std::min(nviews - k0, kBakeChunk);
         DeviceBuffers chunk_bufs(st);
         std::vector<pdr::BakeView> hv(n);
         pdr::ViewTable vt{};
         vt.n = n;
-        size_t off = 0;
         for (uint32_t j = 0; j < n; ++j) {
             const pdr_bake_view& v = views[k0 + j];
             float* clip = clips + static_cast<size_t>(j) * V * 4;
@@ -215,10 +206,8 @@ void run_bake(pdr_context ctx, const float* positions, const float* normals, con
             float vp[4];
             resolve_viewport(v.width, v.height, v.has_viewport ? v.viewport : nullptr, vp);
             vt.v[j] = make_view(clip, v.width, v.height, vp);
-            vt.v[j].raster = shadows + off * 4;
             const size_t pix = static_cast<size_t>(v.width) * v.height;
             pdr::BakeView& b = hv[j];
-            b.shadow = shadows + off * 4;
             b.photo = device_view(chunk_bufs, v.image, pix * v.channels, st);
             b.mask = device_view(chunk_bufs, v.visibility_mask, pix, st);
             std::memcpy(b.m, v.world_to_clip, sizeof(b.m));
@@ -232,10 +221,11 @@ void run_bake(pdr_context ctx, const float* positions, const float* normals, con
             b.height = v.height;
             b.channels = v.channels;
             b.index = k0 + j;
-            off += pix;
         }
         finalize_table(vt);
-        run_raster_forward(ctx, vt, didx, T, V, 0, 0, st);  // shadow maps (:266-276)
+        // shadow maps (:266-276): each view's raster keys, read as (depth, covered) by the projection
+        const unsigned long long* keys = run_raster_keys(ctx, vt, didx, T, V, 0, st);
+        for (uint32_t j = 0; j < n; ++j) hv[j].shadow = keys + vt.v[j].pixel_base;
         hip_check(hipMemcpyAsync(dviews, hv.data(), n * sizeof(pdr::BakeView), hipMemcpyHostToDevice, st),
                   "hipMemcpyAsync");
         if (features)

@@ -443,6 +443,25 @@ inline void run_raster_forward(pdr_context ctx, const pdr::ViewTable& vt, const 
     ctx->keys_clean = true;
 }
 
+// The forward's candidate pass alone: each pixel's winning (depth key, id)
+// word stays in the raster scratch keys (index pixel_base + pixel), ~0 where no
+// triangle covers the pixel.  They stay marked as used, so the next raster
+// call on this context clears them first.
+inline const unsigned long long* run_raster_keys(pdr_context ctx, const pdr::ViewTable& vt, const uint32_t* idx,
+                                                 uint32_t T, uint32_t V, uint32_t cull, hipStream_t st) {
+    if (!vt.total_pixels) return nullptr;
+    const pdr::RasterScratch s = raster_scratch(ctx, vt, T, st, true);
+    ctx->keys_clean = false;
+    const pdr::TuneParams tp =
+        tuned_params(ctx, tune_key("forward", vt, T), forward_candidates(), st, [&](const pdr::TuneParams& c) {
+            hip_check(pdr::raster_candidates(vt, idx, T, V, cull, s, c, st), "raster_candidates");
+            hip_check(pdr::raster_resolve(vt, idx, 0, s, st), "raster_resolve");  // resets the keys
+        });
+    ctx->timed("raster_candidates", st,
+               [&] { hip_check(pdr::raster_candidates(vt, idx, T, V, cull, s, tp, st), "raster_candidates"); });
+    return s.keys;
+}
+
 // Per-triangle corner sums + ordered vertex reduce for one table of views.
 inline void run_raster_backward(pdr_context ctx, const pdr::ViewTable& vt, const uint32_t* idx, uint32_t T, uint32_t V,
                                 hipStream_t st) {

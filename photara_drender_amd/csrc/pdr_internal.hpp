@@ -253,7 +253,7 @@ hipError_t tv_pair_count(const int32_t* chart, uint32_t tw, uint32_t th, unsigne
 // ---- bake.hip
 // One projection view of the bake, in device memory (read by k_bake_project).
 struct BakeView {
-    const float* shadow;  // [h,w,4] raster of the view (shadow map)
+    const unsigned long long* shadow;  // [h,w] the view's raster keys (order-mapped depth << 32 | id), ~0 empty
     const float* photo;   // [h,w,channels]
     const float* mask;    // [h,w] or null
     float m[16];          // world_to_clip, row-major
