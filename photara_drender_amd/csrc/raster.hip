@@ -275,13 +275,26 @@ __global__ __launch_bounds__(kBlock) void k_raster_small(ViewTable vt, const uin
     const float ax[2] = {span_alpha(vd, 0), span_alpha(vd, 1)};
     const float bx[2] = {span_beta(vd, 0), span_beta(vd, 1)};
     const uint32_t stride = gridDim.x * (kBlock / 64) * bsz;
-    for (uint32_t batch = (blockIdx.x * (kBlock / 64) + wave) * bsz; batch < T; batch += stride) {
+    // each lane's corners are loaded one batch ahead, so the index and position
+    // loads of the next batch are in flight while this one is rasterised
+    float4 c0, c1, c2;
+    bool cvalid = false;
+    auto fetch = [&](uint32_t b) {
+        const uint32_t tt = b + lane;
+        cvalid = lane < bsz && b < T && tt < T && load_corners(vd, idx, tt, V, c0, c1, c2);
+    };
+    uint32_t batch = (blockIdx.x * (kBlock / 64) + wave) * bsz;
+    fetch(batch);
+    for (; batch < T; batch += stride) {
         const uint32_t t = batch + lane;
+        const float4 q0 = c0, q1 = c1, q2 = c2;
+        const bool have = cvalid;
+        fetch(batch + stride);
         int32_t lines = 0;
         {
             ScreenTri r;
             LineSpans sp;
-            const bool ok = lane < bsz && t < T && setup_forward_tri(vd, idx, t, V, cull, true, r, sp);
+            const bool ok = have && setup_screen(vd, q0, q1, q2, cull, true, r, sp);
             if (ok && r.work > wave_work) {
                 const uint32_t slabs = static_cast<uint32_t>(r.y1 - r.y0) / kSlabRows + 1u;
                 queue[atomicAdd(&counters[0], 1u)] = make_uint4(view, t, slabs, 0u);
@@ -1008,9 +1021,13 @@ uint32_t tri_batch(uint32_t T, uint32_t views, uint32_t waves) {
     return per >= 64 ? 64u : (per <= 8 ? 8u : static_cast<uint32_t>(per));
 }
 
-uint32_t tri_grid(uint32_t T, uint32_t bsz) {
+// Workgroups per view: one per four batches, at most 8192 per view, or with
+// `views` > 0 at most 8192 over all views (the candidate pass: each wave then
+// loops over batches, its next batch's corners in flight).
+uint32_t tri_grid(uint32_t T, uint32_t bsz, uint32_t views = 0) {
     const uint32_t g = (T + (kBlock / 64) * bsz - 1) / ((kBlock / 64) * bsz);
-    return g < 8192 ? g : 8192;
+    const uint32_t cap = views ? (8192u + views - 1u) / views : 8192u;
+    return g < cap ? (g ? g : 1u) : cap;
 }
 
 }  // namespace
@@ -1056,7 +1073,7 @@ hipError_t raster_candidates(const ViewTable& vt, const uint32_t* idx, uint32_t 
     hipError_t e = fill_u32(s.counters, 0u, 4, st);
     if (e != hipSuccess) return e;
     const uint32_t bsz = tri_batch(T, vt.n, tp.fwd_waves);
-    k_raster_small<<<dim3(tri_grid(T, bsz), vt.n), kBlock, 0, st>>>(vt, idx, T, V, cull, s.sx, s.sy, s.keys, s.queue,
+    k_raster_small<<<dim3(tri_grid(T, bsz, vt.n), vt.n), kBlock, 0, st>>>(vt, idx, T, V, cull, s.sx, s.sy, s.keys, s.queue,
                                                                     s.counters, bsz, tp.wave_work);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     k_raster_large<<<kLargeBlocks, kBlock, 0, st>>>(vt, idx, T, V, cull, s.sx, s.sy, s.keys, s.queue, s.counters);
