@@ -114,3 +114,29 @@ def test_texel_gradient_is_the_ordered_sum(det_ctx, mode, channels):
     ref_t, ref_u = oracle.texture_backward(tex, uv, raster, grad, mode, True)
     _bits_equal(gu, ref_u)
     _bits_equal(gt, ref_t)
+
+
+def test_empty_raster_gives_zero_gradients(det_ctx):
+    """No covered pixel: every entry is a sentinel, every vertex / texel sums nothing."""
+    from photara_drender_amd import _native
+
+    s = scenes.chain_scene(level=3, res=(40, 56), tex_size=16, seed=1)
+    V, A, h, w = s.clip.shape[0], 3, 40, 56
+    raster = np.zeros((h, w, 4), np.float32)
+    rng = np.random.default_rng(0)
+    attr = rng.standard_normal((V, A)).astype(np.float32)
+    grad = rng.standard_normal((h, w, A)).astype(np.float32)
+    idx = np.ascontiguousarray(s.indices, dtype=np.uint32)
+    ga = np.full_like(attr, np.nan)
+    gr = np.empty((h, w, 4), np.float32)
+    _native.check(_native.lib().pdr_interpolate_backward(det_ctx.handle, _p(attr), V, A, _p(idx), idx.shape[0],
+                                                         _p(raster), w, h, _p(grad), _p(ga), _p(gr), None))
+    assert (ga == 0).all()
+    tex = rng.uniform(0, 1, (16, 16, 3)).astype(np.float32)
+    uv = rng.uniform(0, 1, (h, w, 2)).astype(np.float32)
+    gt = np.full_like(tex, np.nan)
+    gu = np.empty((h, w, 2), np.float32)
+    g3 = rng.standard_normal((h, w, 3)).astype(np.float32)
+    _native.check(_native.lib().pdr_texture_backward(det_ctx.handle, _p(tex), 16, 16, 3, 0, _p(uv), _p(raster), w, h,
+                                                     _p(g3), 1, _p(gt), _p(gu), None))
+    assert (gt == 0).all()
