@@ -8,7 +8,7 @@ confidence-weighted mean colour, starts the Adam refinement in place of the
 weighted-average bake.  The network is trained on the GPU by the library's own
 gradient kernels (pdr_mlp_adam_step) on (features, refined atlas) pairs that
 the plain method - bake followed by the full refinement - produces on seeded
-scenes.  The plain method stays the fallback and the judge: a warm-started
+scenes.  The plain method stays the fallback and the yardstick: a warm-started
 refinement is scored by the same photometric PSNR.
 
 Run ``python -m photara_drender_amd.warmstart --train`` to regenerate the
