@@ -455,7 +455,8 @@ inline const unsigned long long* run_raster_keys(pdr_context ctx, const pdr::Vie
     const pdr::TuneParams tp =
         tuned_params(ctx, tune_key("forward", vt, T), forward_candidates(), st, [&](const pdr::TuneParams& c) {
             hip_check(pdr::raster_candidates(vt, idx, T, V, cull, s, c, st), "raster_candidates");
-            hip_check(pdr::raster_resolve(vt, idx, 0, s, st), "raster_resolve");  // resets the keys
+            // the views may have no raster output here: clear the keys instead of resolving them
+            hip_check(pdr::fill_keys(s.keys, vt.total_pixels, st), "fill_keys");
         });
     ctx->timed("raster_candidates", st,
                [&] { hip_check(pdr::raster_candidates(vt, idx, T, V, cull, s, tp, st), "raster_candidates"); });

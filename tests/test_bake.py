@@ -162,6 +162,23 @@ def test_gpu_bake_pcf_radii_match_oracle(pcf):
 
 
 @pytest.mark.gpu
+def test_gpu_bake_with_autotuned_raster_matches_oracle():
+    """The shadow maps are the raster's candidate keys; under PDR_FLAG_AUTOTUNE their
+    tuning runs clear the keys between candidates, and the bake stays bit-exact."""
+    from photara_drender_amd import _native
+    from photara_drender_amd.bake import bake_atlas
+
+    sc, nrm, mats, cams, images, masks = bake_scene()
+    ctx = _native.Context(0, flags=_native.PDR_FLAG_AUTOTUNE | _native.PDR_FLAG_VALIDATE_DEVICE_INPUTS)
+    kw = dict(resolution=(64, 64), blend_mode="weighted_average", mask_floor=0.1)
+    for _ in range(2):  # the first call tunes, the second uses the cached choice
+        g = bake_atlas(ctx, sc.positions, nrm, sc.uv, sc.indices, images, mats, cams, masks, **kw)
+        r = oracle.bake(sc.positions, nrm, sc.uv, sc.indices, images, mats, cams, masks, **kw)
+        np.testing.assert_array_equal(g.color.cpu().numpy().view(np.uint32), r[0].view(np.uint32))
+        np.testing.assert_array_equal(g.confidence.cpu().numpy().view(np.uint32), r[1].view(np.uint32))
+
+
+@pytest.mark.gpu
 def test_gpu_bake_validation_messages():
     import photara_drender_amd as pdr
 
