@@ -150,11 +150,22 @@ struct RefineStepTables {
 };
 
 // One view of the refinement, reduced to its pixels with mask > 0 ("active"),
-// in 8x8-tile order.
+// in raster order, with the packed range of every 64-pixel raster group.
 struct RefineView {
-    const float4* uvm;  // [active] (u, v, mask, 0)
-    const float4* rgb;  // [active] target (r, g, b, 0)
-    uint32_t active;    // pixels with mask > 0 (known on the host once precompute has read the counts back)
+    const float4* uvm;      // [active] (u, v, mask, raster index as bits)
+    const float4* rgb;      // [active] target (r, g, b, 0)
+    const uint32_t* gstart; // [ngroups + 1] first packed pixel of each raster group, then `active`
+    uint32_t active;        // pixels with mask > 0 (known on the host once precompute has read the counts back)
+    uint32_t ngroups;       // ceil(pixels / 64)
+};
+
+// Per-group photometric losses of the last `period` steps: step s writes row
+// s % period, slot b at b * max_groups (row = batch * max_groups floats).
+struct LossParts {
+    float* part;
+    uint32_t period;
+    uint32_t max_groups;
+    size_t row;
 };
 
 // The bilinear taps of one recurring step batch (the views of its batch slots)
@@ -173,8 +184,6 @@ hipError_t refine_view_inputs(const float* image, uint32_t channels, const float
                               float* target3, float* mask, double* partials, uint32_t npartials, hipStream_t st);
 uint32_t refine_partial_count(uint32_t P);
 hipError_t sum_partials(const double* partials, uint32_t n, double* out, hipStream_t st);
-// Per-block loss partials written by the residual pass (photometric_partials of them).
-uint32_t photometric_partials(uint32_t batch, uint32_t max_active);
 // Residual pass over the step's views, then the texel gather of the step's
 // batch (batches[step % nbatches]); resid holds batch slots of `stride` entries.
 // The joint seam term's residual (seam_step's first kernel), run as one more
@@ -186,17 +195,21 @@ struct SeamResidualArgs {
     float4* resid = nullptr;
 };
 // views_dev: [steps * batch] view records, one per step slot (slot b of step s at s * batch + b).
-hipError_t photometric_step(const RefineView* views_dev, uint32_t batch, uint32_t max_active,
-                            const RefineBatchTaps* batches_dev, uint32_t nbatches, uint32_t max_runs,
-                            const float* tex, uint32_t tw, uint32_t th, float eps, const RefineStepTables& tables,
-                            const uint32_t* step_counter, float4* resid, uint32_t stride, float* grad,
-                            float* loss_part, const SeamResidualArgs& seam, hipStream_t st);
-// Active pixels of one view in 8x8-tile order: flags over tile-order slots,
-// then (after an exclusive scan of the flags) the packed records.
-uint32_t tile_slots(uint32_t W, uint32_t H);
-hipError_t active_flags(const float* mask, uint32_t W, uint32_t H, uint32_t* flags, hipStream_t st);
-hipError_t active_pack(const float* uv, const float* target, const float* mask, uint32_t W, uint32_t H,
-                       const uint32_t* flags, const uint32_t* pos, float4* uvm, float4* rgb, hipStream_t st);
+hipError_t photometric_step(const RefineView* views_dev, uint32_t batch, const RefineBatchTaps* batches_dev,
+                            uint32_t nbatches, uint32_t max_runs, const float* tex, uint32_t tw, uint32_t th, float eps,
+                            const RefineStepTables& tables, const uint32_t* step_counter, float4* resid,
+                            uint32_t stride, float* grad, const LossParts& parts, const SeamResidualArgs& seam,
+                            hipStream_t st);
+// loss[s] for the `count` steps before the device step counter, from their loss partials.
+hipError_t loss_fold(const RefineView* views_dev, uint32_t batch, const LossParts& parts, uint32_t count,
+                     const uint32_t* step_counter, float* loss, hipStream_t st);
+// Active pixels of one view of n pixels in raster order: flags, then (after an
+// exclusive scan of the flags) the packed records and the group starts.
+hipError_t active_flags(const float* mask, uint32_t n, uint32_t* flags, hipStream_t st);
+hipError_t active_pack(const float* uv, const float* target, const float* mask, uint32_t n, const uint32_t* flags,
+                       const uint32_t* pos, float4* uvm, float4* rgb, hipStream_t st);
+uint32_t pixel_groups(uint32_t n);
+hipError_t group_starts(const uint32_t* flags, const uint32_t* pos, uint32_t n, uint32_t* gstart, hipStream_t st);
 hipError_t exclusive_scan(const uint32_t* in, uint32_t* out, uint32_t n, void* temp, size_t temp_bytes, hipStream_t st);
 size_t exclusive_scan_temp_bytes(uint32_t n);
 hipError_t active_count(const uint32_t* flags, const uint32_t* pos, uint32_t n, uint32_t* count, hipStream_t st);
@@ -227,9 +240,12 @@ struct TapGroups {
     const float* weights;
     uint32_t ntexels;
 };
+// loss_part (or null): [step][group] losses of the shader's 64-pair groups, step = device counter.
 hipError_t seam_step(const float* tex, uint32_t tw, uint32_t th, const float* pairs, uint32_t npairs, float eps,
-                     float norm, float weight, const uint32_t* step_counter, uint32_t loss_base, const TapGroups& taps,
-                     float4* resid, float* grad, float* loss, bool with_residual, hipStream_t st);
+                     float norm, float weight, const uint32_t* step_counter, const TapGroups& taps, float4* resid,
+                     float* grad, float* loss_part, bool with_residual, hipStream_t st);
+// loss[k] for seam steps k < steps from their group partials.
+hipError_t seam_loss_fold(const float* part, uint32_t npairs, uint32_t steps, float* loss, hipStream_t st);
 hipError_t seam_taps(const float* pairs, uint32_t npairs, uint32_t tw, uint32_t th, uint32_t* keys, uint32_t* vals,
                      hipStream_t st);
 hipError_t seam_tap_weights(const uint32_t* vals, uint32_t E, const float* pairs, uint32_t tw, uint32_t th, float* w,
@@ -239,8 +255,6 @@ hipError_t adam_tv_step(const float* tex, float* tex_out, float* grad, float* m,
                         const float* bc1_table, const float* bc2_table, float b1, float b2, float eps, float cmin,
                         float cmax, const uint32_t* step_counter, hipStream_t st);
 hipError_t advance_step(uint32_t* step_counter, hipStream_t st);
-// loss[step] += the sum of the photometric partials (fixed order), then step += 1.
-hipError_t finish_step(const float* loss_part, uint32_t n, float* loss, uint32_t* step_counter, hipStream_t st);
 // Per-texel neighbour mask of the chart-aware TV (chart may be null: one chart).
 hipError_t tv_mask(const int32_t* chart, uint32_t tw, uint32_t th, uint8_t* mask, hipStream_t st);
 hipError_t tv_gradient(const float* tex, uint32_t tw, uint32_t th, const int32_t* chart, float eps, float scale,

@@ -116,12 +116,9 @@ pdr_status pdr_refine(pdr_context ctx, const float* initial_texture, const float
         double* view_weight = bufs.alloc<double>(nviews);
         uint32_t* view_counts = bufs.alloc<uint32_t>(nviews);  // active pixels
         std::vector<pdr::RefineView> rviews(nviews);
-        uint32_t max_slots = 0;
-        for (uint32_t k = 0; k < nviews; ++k)
-            max_slots = std::max(max_slots, pdr::tile_slots(views[k].width, views[k].height));
-        uint32_t* act_flags = bufs.alloc<uint32_t>(max_slots);
-        uint32_t* act_pos = bufs.alloc<uint32_t>(max_slots);
-        const size_t scan_temp_bytes = pdr::exclusive_scan_temp_bytes(max_slots);
+        uint32_t* act_flags = bufs.alloc<uint32_t>(max_pixels);
+        uint32_t* act_pos = bufs.alloc<uint32_t>(max_pixels);
+        const size_t scan_temp_bytes = pdr::exclusive_scan_temp_bytes(max_pixels);
         void* scan_temp = bufs.alloc<unsigned char>(scan_temp_bytes);
         float* vuv = bufs.alloc<float>(static_cast<size_t>(max_pixels) * 2);
         float* target = bufs.alloc<float>(static_cast<size_t>(max_pixels) * 3);
@@ -130,7 +127,12 @@ pdr_status pdr_refine(pdr_context ctx, const float* initial_texture, const float
         // the per-view passes need no host readback
         auto up = [](size_t b) { return (b + 255) & ~static_cast<size_t>(255); };
         size_t total_bytes = 0;
-        for (uint32_t k = 0; k < nviews; ++k) total_bytes += 2 * up(views[k].width * views[k].height * sizeof(float4));
+        uint32_t max_groups = 0;
+        for (uint32_t k = 0; k < nviews; ++k) {
+            const uint32_t P = views[k].width * views[k].height;
+            total_bytes += 2 * up(P * sizeof(float4)) + up((pdr::pixel_groups(P) + 1u) * sizeof(uint32_t));
+            max_groups = std::max(max_groups, pdr::pixel_groups(P));
+        }
         unsigned char* view_arena = bufs.alloc<unsigned char>(std::max<size_t>(total_bytes, 256));
         size_t arena_off = 0;
         for (uint32_t g0 = 0; g0 < nviews; g0 += group) {
@@ -161,19 +163,23 @@ pdr_status pdr_refine(pdr_context ctx, const float* initial_texture, const float
                 hip_check(pdr::refine_view_inputs(img, v.channels, vis, raster, P, target, mask, partials, np, st),
                           "refine_view_inputs");
                 hip_check(pdr::sum_partials(partials, np, view_weight + k, st), "sum_partials");
-                // the pixels with mask > 0, packed in 8x8-tile order
-                const uint32_t slots = pdr::tile_slots(v.width, v.height);
-                hip_check(pdr::active_flags(mask, v.width, v.height, act_flags, st), "active_flags");
-                hip_check(pdr::exclusive_scan(act_flags, act_pos, slots, scan_temp, scan_temp_bytes, st),
+                // the pixels with mask > 0, packed in raster order (the order in
+                // which the reference's pixels add their taps), and the packed
+                // range of every 64-pixel raster group
+                hip_check(pdr::active_flags(mask, P, act_flags, st), "active_flags");
+                hip_check(pdr::exclusive_scan(act_flags, act_pos, P, scan_temp, scan_temp_bytes, st),
                           "exclusive_scan");
-                hip_check(pdr::active_count(act_flags, act_pos, slots, view_counts + k, st), "active_count");
+                hip_check(pdr::active_count(act_flags, act_pos, P, view_counts + k, st), "active_count");
                 float4* uvm = reinterpret_cast<float4*>(view_arena + arena_off);
                 arena_off += up(static_cast<size_t>(P) * sizeof(float4));
                 float4* rgb = reinterpret_cast<float4*>(view_arena + arena_off);
                 arena_off += up(static_cast<size_t>(P) * sizeof(float4));
-                hip_check(pdr::active_pack(vuv, target, mask, v.width, v.height, act_flags, act_pos, uvm, rgb, st),
-                          "active_pack");
-                rviews[k] = pdr::RefineView{uvm, rgb, 0u};  // active: set once the counts are read back
+                uint32_t* gstart = reinterpret_cast<uint32_t*>(view_arena + arena_off);
+                arena_off += up((pdr::pixel_groups(P) + 1u) * sizeof(uint32_t));
+                hip_check(pdr::active_pack(vuv, target, mask, P, act_flags, act_pos, uvm, rgb, st), "active_pack");
+                hip_check(pdr::group_starts(act_flags, act_pos, P, gstart, st), "group_starts");
+                // active: set once the counts are read back
+                rviews[k] = pdr::RefineView{uvm, rgb, gstart, 0u, pdr::pixel_groups(P)};
             }
         }
         std::vector<uint32_t> host_counts(nviews);
@@ -277,9 +283,20 @@ pdr_status pdr_refine(pdr_context ctx, const float* initial_texture, const float
         float* m2 = bufs.alloc<float>(n);
         float* loss = bufs.alloc<float>(o.steps + seam_steps);
         float4* resid = bufs.alloc<float4>(static_cast<size_t>(o.batch_size) * max_active);
-        const uint32_t nparts = pdr::photometric_partials(o.batch_size, max_active);
-        float* loss_part = bufs.alloc<float>(nparts);
+        // Steps are folded into their losses every `fold` steps (a divisor of
+        // the captured block length, so the partial rows of a replayed block
+        // line up): the group partials of `fold` steps stay in memory.
+        const uint32_t block = o.checkpoint_every ? o.checkpoint_every : std::min<uint32_t>(o.steps, 50);
+        uint32_t fold = block;
+        if (fold > 100)
+            for (fold = 100; block % fold; --fold) {
+            }
+        pdr::LossParts parts{nullptr, fold, max_groups, static_cast<size_t>(o.batch_size) * max_groups};
+        parts.part = bufs.alloc<float>(std::max<size_t>(parts.row * fold, 1));
         float4* seam_resid = bufs.alloc<float4>(npairs);
+        float* seam_part = bufs.alloc<float>(std::max<size_t>(static_cast<size_t>((npairs + 63u) / 64u) *
+                                                                  (npairs ? o.seam_polish_steps : 0u),
+                                                              1));
         uint32_t* counters = bufs.alloc<uint32_t>(4);  // step, done
         copy_any(tex, initial_texture, static_cast<size_t>(n) * 4, st);
         hip_check(pdr::fill_f32(grad, 0.0f, n, st), "fill");
@@ -380,13 +397,13 @@ pdr_status pdr_refine(pdr_context ctx, const float* initial_texture, const float
             if (joint_seam)
                 seam_res = pdr::SeamResidualArgs{dpairs, npairs, o.seam_epsilon, seam_norm, o.joint_seam_weight,
                                                  seam_resid};
-            hip_check(pdr::photometric_step(dviews, o.batch_size, max_active, dbatches, nbatches, max_runs, texbuf[c],
-                                            o.width, o.height, o.photometric_epsilon, tables, counters, resid,
-                                            max_active, grad, loss_part, seam_res, s),
+            hip_check(pdr::photometric_step(dviews, o.batch_size, dbatches, nbatches, max_runs, texbuf[c], o.width,
+                                            o.height, o.photometric_epsilon, tables, counters, resid, max_active,
+                                            grad, parts, seam_res, s),
                       "photometric_step");
             if (joint_seam)
                 hip_check(pdr::seam_step(texbuf[c], o.width, o.height, dpairs, npairs, o.seam_epsilon, seam_norm,
-                                         o.joint_seam_weight, counters, 0, seam_groups, seam_resid, grad, nullptr,
+                                         o.joint_seam_weight, counters, seam_groups, seam_resid, grad, nullptr,
                                          false, s),
                           "seam_step");
             if (fused_tv)
@@ -398,13 +415,23 @@ pdr_status pdr_refine(pdr_context ctx, const float* initial_texture, const float
                 hip_check(pdr::adam_step(texbuf[c], grad, m1, m2, n, d_lr, d_bc1, d_bc2, o.adam_beta1, o.adam_beta2,
                                          o.adam_epsilon, o.clamp_min, o.clamp_max, counters, s),
                           "adam_step");
-            hip_check(pdr::finish_step(loss_part, nparts, loss, counters, s), "finish_step");
+            hip_check(pdr::advance_step(counters, s), "advance_step");
             return fused_tv ? c ^ 1u : c;
         };
-        const uint32_t block = o.checkpoint_every ? o.checkpoint_every : std::min<uint32_t>(o.steps, 50);
+        // `count` steps from a step that is a multiple of `fold`, each run of up
+        // to `fold` of them folded into its losses
+        auto run_steps = [&](hipStream_t s, uint32_t count, uint32_t c) {
+            for (uint32_t k = 0; k < count; ++k) {
+                c = one_step(s, c);
+                if ((k + 1) % fold == 0 || k + 1 == count)
+                    hip_check(pdr::loss_fold(dviews, o.batch_size, parts, k % fold + 1, counters, loss, s),
+                              "loss_fold");
+            }
+            return c;
+        };
         auto capture = [&](uint32_t count, uint32_t c, GraphExec& g) {
             hip_check(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-            for (uint32_t k = 0; k < count; ++k) c = one_step(st, c);
+            run_steps(st, count, c);
             hip_check(hipStreamEndCapture(st, &g.graph), "hipStreamEndCapture");
             hip_check(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0), "hipGraphInstantiate");
         };
@@ -425,7 +452,7 @@ pdr_status pdr_refine(pdr_context ctx, const float* initial_texture, const float
                 hip_check(hipGraphLaunch(b < nblocks ? full[cur].exec : tail.exec, st), "hipGraphLaunch");
                 cur ^= count & flips;
             } else {
-                for (uint32_t k = 0; k < count; ++k) cur = one_step(st, cur);
+                cur = run_steps(st, count, cur);
             }
             done += count;
             if (o.checkpoint_every && o.checkpoints && done % o.checkpoint_every == 0)
@@ -452,13 +479,14 @@ pdr_status pdr_refine(pdr_context ctx, const float* initial_texture, const float
             hip_check(pdr::fill_u32(counters, 0u, 2, st), "fill");
             for (uint32_t k = 0; k < seam_steps; ++k) {
                 hip_check(pdr::seam_step(tex, o.width, o.height, dpairs, npairs, o.seam_epsilon, seam_norm, 1.0f,
-                                         counters, o.steps, seam_groups, seam_resid, grad, loss, true, st),
+                                         counters, seam_groups, seam_resid, grad, seam_part, true, st),
                           "seam_step");
                 hip_check(pdr::adam_step(tex, grad, m1, m2, n, d_slr, d_sb1, d_sb2, o.adam_beta1, o.adam_beta2,
                                          o.adam_epsilon, o.clamp_min, o.clamp_max, counters, st),
                           "adam_step");
                 hip_check(pdr::advance_step(counters, st), "advance_step");
             }
+            hip_check(pdr::seam_loss_fold(seam_part, npairs, seam_steps, loss + o.steps, st), "seam_loss_fold");
         }
         sync(st);
         const auto t3 = std::chrono::steady_clock::now();

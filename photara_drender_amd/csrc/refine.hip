@@ -5,11 +5,16 @@
 //                     (texture_refiner.cpp:228-242)
 //  k_photo_residual   atlas_photometric_gradient.hlsl:45-108 for all views of a
 //                     step in one launch, up to the per-pixel output gradient
-//                     d / r * m * norm (and the loss), over each view's pixels
-//                     with mask > 0 packed in 8x8-tile order at precompute
-//                     (k_active_flags, a scan, k_active_pack); per-step scalars come
-//                     from device tables indexed by a device step counter so
-//                     the step loop can be captured once and replayed
+//                     d / r * m * norm, over each view's pixels with mask > 0
+//                     packed in raster order at precompute (k_active_flags, a
+//                     scan, k_active_pack); one wave per 64-pixel raster group
+//                     (the shader's thread group), whose loss is the shader's
+//                     64-lane tree sum; per-step scalars come from device
+//                     tables indexed by a device step counter so the step loop
+//                     can be captured once and replayed
+//  k_loss_fold        each step's group losses added one after another in
+//                     (batch slot, group) order, as the shader's one atomic
+//                     add per group would if the groups ran in order
 //  k_photo_gather     the texel gradient as a gather: the step batches recur
 //                     (step s uses batch s % L), so each batch's bilinear taps
 //                     are inverted once at precompute (k_batch_taps, a radix
@@ -20,15 +25,20 @@
 //                     fixed order and without atomics
 //  k_seam_residual /  atlas_seam_gradient.hlsl:50-100, in the same residual +
 //  k_seam_gather      gather form over the (fixed) seam pairs' taps
-//  k_adam             atlas_adam_update.hlsl:20-43, float4-vectorised
+//  k_adam             atlas_adam_update.hlsl:20-43, float4-vectorised, the
+//                     shader's expression with correctly rounded sqrt and
+//                     division
 //  k_adam_tv          the same update with the gradient of the chart-aware
 //                     total variation of python/photara_drender/atlas.py:
-//                     84-115 (extension) added per texel first
+//                     84-115 (extension) added per texel first (hardware
+//                     square root and reciprocal)
 //  k_advance_step     moves the device step counter on after a step
 //
-// No gradient uses float atomics (the reference emulates them with
-// compare-exchange loops); loss partials are summed per 64-lane wave with the
-// shaders' tree order (over active pixels for the photometric term).
+// No gradient or loss uses float atomics (the reference emulates them with
+// compare-exchange loops): every texel's terms are added in (batch slot,
+// raster pixel, tap) order and every loss in (slot, group) order, so the
+// parity-mode refinement is deterministic and equals the CPU restatement bit
+// for bit.
 #include <hipcub/hipcub.hpp>
 
 #include "pdr_internal.hpp"
@@ -150,13 +160,25 @@ __device__ __forceinline__ float seam_pair_residual(uint32_t q, const float* __r
     return local;
 }
 
+// Hand-off of LDS data between lanes of one wave.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One wave per 64-pixel raster group g of the step slot's view (the shader's
+// thread group of pixels 64g .. 64g + 63): lane l takes the group's l-th
+// active pixel (packed range gstart[g] .. gstart[g + 1]).  The group's loss is
+// the shader's 64-lane tree sum with every pixel at its raster lane and zeros
+// elsewhere; it goes to the loss partials of the step's row, slot and group.
 __global__ __launch_bounds__(kBlock) void k_photo_residual(const RefineView* __restrict__ views, uint32_t batch,
                                                            const float* __restrict__ tex, uint32_t tw, uint32_t th,
                                                            float eps, RefineStepTables tables,
                                                            const uint32_t* __restrict__ step_counter,
                                                            float4* __restrict__ resid, uint32_t max_active,
-                                                           float* __restrict__ loss_part, SeamResidualArgs seam) {
-    __shared__ float wsum[kBlock / 64];
+                                                           LossParts parts, SeamResidualArgs seam) {
+    __shared__ float lanes[kBlock];
     if (blockIdx.y == batch) {  // the joint seam term's residual, beside the views' (no loss)
         for (uint32_t q = blockIdx.x * kBlock + threadIdx.x; q < seam.npairs; q += gridDim.x * kBlock)
             (void)seam_pair_residual(q, tex, tw, th, seam.pairs, seam.eps, seam.norm, seam.weight, seam.resid);
@@ -165,19 +187,25 @@ __global__ __launch_bounds__(kBlock) void k_photo_residual(const RefineView* __r
     const uint32_t step = *step_counter;
     const uint32_t b = blockIdx.y;
     const RefineView& rv = views[static_cast<size_t>(step) * batch + b];  // the step slot's view record
-    const float norm = tables.norm[step];
-    const uint32_t a = blockIdx.x * kBlock + threadIdx.x;
-    const uint32_t active = rv.active;
-    float* part = loss_part + static_cast<size_t>(b) * gridDim.x + blockIdx.x;
-    if (blockIdx.x * kBlock >= active) {  // whole block past this view (uniform)
-        if (threadIdx.x == 0) *part = 0.0f;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t g = blockIdx.x * (kBlock / 64u) + (threadIdx.x >> 6);
+    if (g >= rv.ngroups) return;  // wave-uniform
+    const uint32_t s = rv.gstart[g], e = rv.gstart[g + 1u];
+    float* part = parts.part + static_cast<size_t>(step % parts.period) * parts.row +
+                  static_cast<size_t>(b) * parts.max_groups + g;
+    if (s == e) {  // no active pixel: the group adds nothing
+        if (lane == 0) *part = 0.0f;
         return;
     }
+    const float norm = tables.norm[step];
+    const uint32_t a = s + lane;
     float local = 0.0f;
-    if (a < active) {
+    uint32_t at_lane = lane;
+    if (a < e) {
         const float4 q = rv.uvm[a];
         const float4 target = rv.rgb[a];
         const float m = q.z;
+        at_lane = __float_as_uint(q.w) & 63u;  // raster index of the pixel
         const AtlasTap at = atlas_tap(q.x, q.y, tw, th);
         const float tgt[3] = {target.x, target.y, target.z};
         float og[3];
@@ -193,10 +221,41 @@ __global__ __launch_bounds__(kBlock) void k_photo_residual(const RefineView* __r
         }
         resid[static_cast<size_t>(b) * max_active + a] = make_float4(og[0], og[1], og[2], 0.0f);
     }
-    const float s = wave_tree_sum(local);
-    if ((threadIdx.x & 63u) == 0) wsum[threadIdx.x >> 6] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) *part = (wsum[0] + wsum[1]) + (wsum[2] + wsum[3]);
+    float* wl = lanes + (threadIdx.x & ~63u);
+    wl[lane] = 0.0f;
+    wave_lds_sync();
+    if (a < e) wl[at_lane] = local;
+    wave_lds_sync();
+    const float t = wave_tree_sum(wl[lane]);
+    if (lane == 0) *part = t;
+}
+
+// loss[step] of the `count` steps before the device step counter: each step's
+// group losses added one after another in (batch slot, group) order, zeros
+// skipped, as the shader's one add per group lands when the groups run in that
+// order.  One lane per step.
+__global__ void k_loss_fold(const RefineView* __restrict__ views, uint32_t batch, LossParts parts, uint32_t count,
+                            const uint32_t* __restrict__ step_counter, float* __restrict__ loss) {
+    const uint32_t r = blockIdx.x * 64u + threadIdx.x;
+    if (r >= count) return;
+    const uint32_t step = *step_counter - count + r;
+    const float* row = parts.part + static_cast<size_t>(step % parts.period) * parts.row;
+    float acc = 0.0f;
+    for (uint32_t b = 0; b < batch; ++b) {
+        const uint32_t ng = views[static_cast<size_t>(step) * batch + b].ngroups;
+        const float* p = row + static_cast<size_t>(b) * parts.max_groups;
+        uint32_t g = 0;
+        for (; g + 4u <= ng; g += 4u) {  // four loads in flight, added in order
+            const float v0 = p[g], v1 = p[g + 1u], v2 = p[g + 2u], v3 = p[g + 3u];
+            if (v0 != 0.0f) acc += v0;
+            if (v1 != 0.0f) acc += v1;
+            if (v2 != 0.0f) acc += v2;
+            if (v3 != 0.0f) acc += v3;
+        }
+        for (; g < ng; ++g)
+            if (p[g] != 0.0f) acc += p[g];
+    }
+    loss[step] = acc;
 }
 
 // Texel t of a tw-wide atlas as a key in 8x8-tile order (tiles_x = tiles per
@@ -211,13 +270,6 @@ __device__ __forceinline__ uint32_t key_texel(uint32_t k, uint32_t tw, uint32_t 
     if (!tiles_x) return k;
     const uint32_t tile = k >> 6;
     return ((tile / tiles_x) * 8u + ((k >> 3) & 7u)) * tw + (tile % tiles_x) * 8u + (k & 7u);
-}
-
-// Hand-off of LDS data between lanes of one wave.
-__device__ __forceinline__ void wave_lds_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // Texel gradient of the step's batch: one lane per texel the batch samples,
@@ -302,35 +354,31 @@ __global__ __launch_bounds__(kBlock) void k_photo_gather(const RefineBatchTaps* 
     texel_gather<2>(bt.texels, bt.tap_offsets, bt.taps, bt.tap_weights, *bt.runs, resid, tw, tiles_x, grad, prod);
 }
 
-// Tile-order slot q of a W x H view: tile q / 64 (row-major over 8x8 tiles),
-// pixel (q % 64) row-major inside the tile.
-__device__ __forceinline__ bool tile_pixel(uint32_t q, uint32_t W, uint32_t H, uint32_t& p) {
-    const uint32_t tiles_x = (W + 7u) / 8u, tile = q >> 6, lane = q & 63u;
-    const uint32_t x = (tile % tiles_x) * 8u + (lane & 7u), y = (tile / tiles_x) * 8u + (lane >> 3);
-    p = y * W + x;
-    return x < W && y < H;
+__global__ void k_active_flags(const float* __restrict__ mask, uint32_t n, uint32_t* __restrict__ flags) {
+    const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+    if (p < n) flags[p] = mask[p] > 0.0f ? 1u : 0u;
 }
 
-__global__ void k_active_flags(const float* __restrict__ mask, uint32_t W, uint32_t H, uint32_t n,
-                               uint32_t* __restrict__ flags) {
-    const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
-    if (q >= n) return;
-    uint32_t p = 0;
-    flags[q] = tile_pixel(q, W, H, p) && mask[p] > 0.0f ? 1u : 0u;
-}
-
+// Packed record a = pos[p] of active pixel p: (u, v, mask, p as bits), target.
 __global__ void k_active_pack(const float* __restrict__ uv, const float* __restrict__ target,
-                              const float* __restrict__ mask, uint32_t W, uint32_t H, uint32_t n,
-                              const uint32_t* __restrict__ flags, const uint32_t* __restrict__ pos,
-                              float4* __restrict__ uvm, float4* __restrict__ rgb) {
-    const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
-    if (q >= n || !flags[q]) return;
-    uint32_t p = 0;
-    tile_pixel(q, W, H, p);
-    const uint32_t a = pos[q];
-    uvm[a] = make_float4(uv[static_cast<size_t>(p) * 2], uv[static_cast<size_t>(p) * 2 + 1], mask[p], 0.0f);
+                              const float* __restrict__ mask, uint32_t n, const uint32_t* __restrict__ flags,
+                              const uint32_t* __restrict__ pos, float4* __restrict__ uvm, float4* __restrict__ rgb) {
+    const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= n || !flags[p]) return;
+    const uint32_t a = pos[p];
+    uvm[a] = make_float4(uv[static_cast<size_t>(p) * 2], uv[static_cast<size_t>(p) * 2 + 1], mask[p],
+                         __uint_as_float(p));
     rgb[a] = make_float4(target[static_cast<size_t>(p) * 3], target[static_cast<size_t>(p) * 3 + 1],
                          target[static_cast<size_t>(p) * 3 + 2], 0.0f);
+}
+
+// First packed index of each 64-pixel raster group, and the active count after
+// the last: gstart[g] = pos[64 g] for g < groups, gstart[groups] = active.
+__global__ void k_group_starts(const uint32_t* __restrict__ flags, const uint32_t* __restrict__ pos, uint32_t n,
+                               uint32_t groups, uint32_t* __restrict__ gstart) {
+    const uint32_t g = blockIdx.x * kBlock + threadIdx.x;
+    if (g > groups) return;
+    gstart[g] = g < groups ? pos[g * 64u] : (n ? pos[n - 1] + flags[n - 1] : 0u);
 }
 
 __global__ void k_active_count(const uint32_t* __restrict__ flags, const uint32_t* __restrict__ pos, uint32_t n,
@@ -385,20 +433,35 @@ __global__ void k_set_last(uint32_t* __restrict__ offsets, const uint32_t* __res
 // Seam pairs (atlas_seam_gradient.hlsl:50-100) up to the per-pair output
 // gradient g = d / r * norm (* weight); the texel gradient is gathered by
 // k_seam_gather over the pairs' taps grouped by texel (k_seam_taps at setup).
+// With loss partials, each wave (the shader's group of 64 pairs) writes its
+// tree-summed loss to part[step * groups + group] for k_seam_loss_fold.
 __global__ __launch_bounds__(kBlock) void k_seam_residual(const float* __restrict__ tex, uint32_t tw, uint32_t th,
                                                           const float* __restrict__ pairs, uint32_t npairs, float eps,
                                                           float norm, float weight,
                                                           const uint32_t* __restrict__ step_counter,
-                                                          uint32_t loss_base, float4* __restrict__ resid,
-                                                          float* __restrict__ loss) {
+                                                          float4* __restrict__ resid, float* __restrict__ part) {
     const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
     const float local = q < npairs ? seam_pair_residual(q, tex, tw, th, pairs, eps, norm, weight, resid) : 0.0f;
     const float s = wave_tree_sum(local);
-    if (loss && (threadIdx.x & 63u) == 0 && s != 0.0f) unsafeAtomicAdd(&loss[loss_base + *step_counter], s);
+    const uint32_t groups = (npairs + 63u) / 64u;
+    if (part && (threadIdx.x & 63u) == 0) part[static_cast<size_t>(*step_counter) * groups + (q >> 6)] = s;
 }
 
-// Texel gradient of the seam term: entries (pair * 8 + side * 4 + tap) with the
-// tap weight, negated on the second side, grouped by texel.
+// loss[k] of seam step k < steps: its group losses added in group order, zeros skipped.
+__global__ void k_seam_loss_fold(const float* __restrict__ part, uint32_t groups, uint32_t steps,
+                                 float* __restrict__ loss) {
+    const uint32_t k = blockIdx.x * 64u + threadIdx.x;
+    if (k >= steps) return;
+    const float* p = part + static_cast<size_t>(k) * groups;
+    float acc = 0.0f;
+    for (uint32_t g = 0; g < groups; ++g)
+        if (p[g] != 0.0f) acc += p[g];
+    loss[k] = acc;
+}
+
+// Texel gradient of the seam term: entries (pair * 8 + tap * 2 + side) with the
+// tap weight, negated on the second side, grouped by texel.  Within a pair the
+// entries keep the shader's order: tap k of side 0, then tap k of side 1.
 __global__ __launch_bounds__(kBlock) void k_seam_gather(const uint32_t* __restrict__ texels,
                                                         const uint32_t* __restrict__ offsets,
                                                         const uint32_t* __restrict__ taps,
@@ -417,10 +480,10 @@ __global__ void k_seam_taps(const float* __restrict__ pairs, uint32_t npairs, ui
     const AtlasTap a1 = atlas_tap(pr.z, pr.w, tw, th);
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        keys[static_cast<size_t>(q) * 8 + k] = a0.t[k];
-        vals[static_cast<size_t>(q) * 8 + k] = q * 8u + static_cast<uint32_t>(k);
-        keys[static_cast<size_t>(q) * 8 + 4 + k] = a1.t[k];
-        vals[static_cast<size_t>(q) * 8 + 4 + k] = q * 8u + 4u + static_cast<uint32_t>(k);
+        keys[static_cast<size_t>(q) * 8 + 2 * k] = a0.t[k];
+        vals[static_cast<size_t>(q) * 8 + 2 * k] = q * 8u + 2u * static_cast<uint32_t>(k);
+        keys[static_cast<size_t>(q) * 8 + 2 * k + 1] = a1.t[k];
+        vals[static_cast<size_t>(q) * 8 + 2 * k + 1] = q * 8u + 2u * static_cast<uint32_t>(k) + 1u;
     }
 }
 
@@ -430,9 +493,9 @@ __global__ void k_seam_tap_weights(const uint32_t* __restrict__ vals, uint32_t E
     if (e >= E) return;
     const uint32_t v = vals[e];
     const float4 pr = reinterpret_cast<const float4*>(pairs)[v >> 3];
-    const bool second = (v & 4u) != 0u;
+    const bool second = (v & 1u) != 0u;
     const AtlasTap a = second ? atlas_tap(pr.z, pr.w, tw, th) : atlas_tap(pr.x, pr.y, tw, th);
-    const float wt = a.w[v & 3u];
+    const float wt = a.w[(v >> 1) & 3u];
     w[e] = second ? -wt : wt;
 }
 
@@ -522,8 +585,24 @@ __global__ __launch_bounds__(kBlock) void k_tv_pairs(const int32_t* __restrict__
     }
 }
 
+// atlas_adam_update.hlsl:27-42 as written: p - lr * c1 / (sqrt(c2) + eps),
+// with correctly rounded division and square root.
 __device__ __forceinline__ float adam_one(float p, float& g, float& m, float& v, float lr, float b1, float b2,
                                           float eps, float bc1, float bc2, float cmin, float cmax) {
+    const float m1 = b1 * m + (1.0f - b1) * g;
+    const float m2 = b2 * v + (1.0f - b2) * g * g;
+    m = m1;
+    v = m2;
+    g = 0.0f;
+    const float c1 = m1 / bc1, c2 = m2 / bc2;
+    return clampf(p - lr * c1 / (sqrtf(c2) + eps), cmin, cmax);
+}
+
+// The same update with the hardware square root and reciprocal (about 1 ulp
+// each) in place of the correctly rounded forms, for the extended objective's
+// fused TV + Adam pass, which they keep off the VALU bound.
+__device__ __forceinline__ float adam_one_native(float p, float& g, float& m, float& v, float lr, float b1, float b2,
+                                                 float eps, float bc1, float bc2, float cmin, float cmax) {
     const float m1 = b1 * m + (1.0f - b1) * g;
     const float m2 = b2 * v + (1.0f - b2) * g * g;
     m = m1;
@@ -560,7 +639,7 @@ __global__ __launch_bounds__(kBlock) void k_adam_tv(const float* __restrict__ te
 #pragma unroll
         for (int ch = 0; ch < 3; ++ch) {
             float gj = gr[ch] + g[ch] * tv_scale;
-            tex_out[b + ch] = adam_one(tex[b + ch], gj, m[ch], v[ch], lr, b1, b2, eps, bc1, bc2, cmin, cmax);
+            tex_out[b + ch] = adam_one_native(tex[b + ch], gj, m[ch], v[ch], lr, b1, b2, eps, bc1, bc2, cmin, cmax);
             grad[b + ch] = gj;
             mom1[b + ch] = m[ch];
             mom2[b + ch] = v[ch];
@@ -581,29 +660,6 @@ __global__ void k_tv_gradient(const float* __restrict__ tex, uint32_t tw, uint32
 
 __global__ void k_advance_step(uint32_t* step_counter) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *step_counter += 1u;
-}
-
-// The step's photometric loss from the residual pass's block partials (fixed
-// order: contiguous chunks per lane, the 64-lane tree per wave, then the waves
-// in order), then the step counter moves on.  One block of kFinishThreads.
-constexpr uint32_t kFinishThreads = 1024;
-__global__ __launch_bounds__(kFinishThreads) void k_finish_step(const float* __restrict__ loss_part, uint32_t n,
-                                                                float* __restrict__ loss, uint32_t* step_counter) {
-    __shared__ float wsum[kFinishThreads / 64];
-    const uint32_t per = (n + kFinishThreads - 1) / kFinishThreads;
-    const uint32_t b = threadIdx.x * per, e = b + per < n ? b + per : n;
-    float s = 0.0f;
-    for (uint32_t i = b; i < e; ++i) s += loss_part[i];
-    s = wave_tree_sum(s);
-    if ((threadIdx.x & 63u) == 0) wsum[threadIdx.x >> 6] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        float t = 0.0f;
-        for (uint32_t w = 0; w < kFinishThreads / 64; ++w) t += wsum[w];
-        const uint32_t step = *step_counter;
-        loss[step] += t;
-        *step_counter = step + 1u;
-    }
 }
 
 __global__ __launch_bounds__(kBlock) void k_adam(float* __restrict__ tex, float* __restrict__ grad,
@@ -658,11 +714,6 @@ hipError_t sum_partials(const double* partials, uint32_t n, double* out, hipStre
     return hipGetLastError();
 }
 
-uint32_t photometric_partials(uint32_t batch, uint32_t max_active) {
-    const uint32_t blocks = (max_active + kBlock - 1) / kBlock;
-    return (blocks ? blocks : 1) * batch;
-}
-
 uint32_t texel_key_tiles(uint32_t tw, uint32_t th) {
     const uint64_t tx = (tw + 7u) / 8u, ty = (th + 7u) / 8u;
     return tx * ty * 64u <= 0xFFFFFFFFull ? static_cast<uint32_t>(tx) : 0u;
@@ -673,33 +724,52 @@ uint64_t texel_key_range(uint32_t tw, uint32_t th) {
     return tx ? static_cast<uint64_t>(tx) * ((th + 7u) / 8u) * 64u : static_cast<uint64_t>(tw) * th;
 }
 
-hipError_t photometric_step(const RefineView* views_dev, uint32_t batch, uint32_t max_active,
-                            const RefineBatchTaps* batches_dev, uint32_t nbatches, uint32_t max_runs,
-                            const float* tex, uint32_t tw, uint32_t th, float eps, const RefineStepTables& tables,
-                            const uint32_t* step_counter, float4* resid, uint32_t stride, float* grad,
-                            float* loss_part, const SeamResidualArgs& seam, hipStream_t st) {
-    const uint32_t blocks = (max_active + kBlock - 1) / kBlock;
+hipError_t photometric_step(const RefineView* views_dev, uint32_t batch, const RefineBatchTaps* batches_dev,
+                            uint32_t nbatches, uint32_t max_runs, const float* tex, uint32_t tw, uint32_t th, float eps,
+                            const RefineStepTables& tables, const uint32_t* step_counter, float4* resid,
+                            uint32_t stride, float* grad, const LossParts& parts, const SeamResidualArgs& seam,
+                            hipStream_t st) {
+    const uint32_t blocks = (parts.max_groups + kBlock / 64u - 1u) / (kBlock / 64u);
     const dim3 grid(blocks ? blocks : 1, batch + (seam.npairs ? 1u : 0u));
     k_photo_residual<<<grid, kBlock, 0, st>>>(views_dev, batch, tex, tw, th, eps, tables, step_counter, resid, stride,
-                                              loss_part, seam);
+                                              parts, seam);
     const uint32_t gblocks = (max_runs + kBlock - 1) / kBlock;
     k_photo_gather<<<gblocks ? gblocks : 1, kBlock, 0, st>>>(batches_dev, nbatches, tw, texel_key_tiles(tw, th),
                                                              step_counter, resid, grad);
     return hipGetLastError();
 }
 
-uint32_t tile_slots(uint32_t W, uint32_t H) { return ((W + 7u) / 8u) * ((H + 7u) / 8u) * 64u; }
-
-hipError_t active_flags(const float* mask, uint32_t W, uint32_t H, uint32_t* flags, hipStream_t st) {
-    const uint32_t n = tile_slots(W, H);
-    k_active_flags<<<(n + kBlock - 1) / kBlock, kBlock, 0, st>>>(mask, W, H, n, flags);
+hipError_t active_flags(const float* mask, uint32_t n, uint32_t* flags, hipStream_t st) {
+    if (!n) return hipSuccess;
+    k_active_flags<<<(n + kBlock - 1) / kBlock, kBlock, 0, st>>>(mask, n, flags);
     return hipGetLastError();
 }
 
-hipError_t active_pack(const float* uv, const float* target, const float* mask, uint32_t W, uint32_t H,
-                       const uint32_t* flags, const uint32_t* pos, float4* uvm, float4* rgb, hipStream_t st) {
-    const uint32_t n = tile_slots(W, H);
-    k_active_pack<<<(n + kBlock - 1) / kBlock, kBlock, 0, st>>>(uv, target, mask, W, H, n, flags, pos, uvm, rgb);
+hipError_t active_pack(const float* uv, const float* target, const float* mask, uint32_t n, const uint32_t* flags,
+                       const uint32_t* pos, float4* uvm, float4* rgb, hipStream_t st) {
+    if (!n) return hipSuccess;
+    k_active_pack<<<(n + kBlock - 1) / kBlock, kBlock, 0, st>>>(uv, target, mask, n, flags, pos, uvm, rgb);
+    return hipGetLastError();
+}
+
+uint32_t pixel_groups(uint32_t n) { return (n + 63u) / 64u; }
+
+hipError_t group_starts(const uint32_t* flags, const uint32_t* pos, uint32_t n, uint32_t* gstart, hipStream_t st) {
+    const uint32_t groups = pixel_groups(n);
+    k_group_starts<<<groups / kBlock + 1u, kBlock, 0, st>>>(flags, pos, n, groups, gstart);
+    return hipGetLastError();
+}
+
+hipError_t loss_fold(const RefineView* views_dev, uint32_t batch, const LossParts& parts, uint32_t count,
+                     const uint32_t* step_counter, float* loss, hipStream_t st) {
+    if (!count) return hipSuccess;
+    k_loss_fold<<<(count + 63u) / 64u, 64, 0, st>>>(views_dev, batch, parts, count, step_counter, loss);
+    return hipGetLastError();
+}
+
+hipError_t seam_loss_fold(const float* part, uint32_t npairs, uint32_t steps, float* loss, hipStream_t st) {
+    if (!steps || !npairs) return hipSuccess;
+    k_seam_loss_fold<<<(steps + 63u) / 64u, 64, 0, st>>>(part, (npairs + 63u) / 64u, steps, loss);
     return hipGetLastError();
 }
 
@@ -776,12 +846,12 @@ hipError_t photo_group_taps(const uint32_t* keys_in, const uint32_t* vals_in, ui
 }
 
 hipError_t seam_step(const float* tex, uint32_t tw, uint32_t th, const float* pairs, uint32_t npairs, float eps,
-                     float norm, float weight, const uint32_t* step_counter, uint32_t loss_base, const TapGroups& taps,
-                     float4* resid, float* grad, float* loss, bool with_residual, hipStream_t st) {
+                     float norm, float weight, const uint32_t* step_counter, const TapGroups& taps, float4* resid,
+                     float* grad, float* loss_part, bool with_residual, hipStream_t st) {
     if (!npairs) return hipSuccess;
     if (with_residual)
         k_seam_residual<<<(npairs + kBlock - 1) / kBlock, kBlock, 0, st>>>(tex, tw, th, pairs, npairs, eps, norm,
-                                                                          weight, step_counter, loss_base, resid, loss);
+                                                                          weight, step_counter, resid, loss_part);
     if (taps.ntexels)
         k_seam_gather<<<(taps.ntexels + kBlock - 1) / kBlock, kBlock, 0, st>>>(
             taps.texels, taps.offsets, taps.taps, taps.weights, taps.ntexels, resid, grad);
@@ -821,11 +891,6 @@ hipError_t tv_mask(const int32_t* chart, uint32_t tw, uint32_t th, uint8_t* mask
 hipError_t tv_gradient(const float* tex, uint32_t tw, uint32_t th, const int32_t* chart, float eps, float scale,
                        float* grad, hipStream_t st) {
     k_tv_gradient<<<(tw * th + kBlock - 1) / kBlock, kBlock, 0, st>>>(tex, tw, th, chart, eps, scale, grad);
-    return hipGetLastError();
-}
-
-hipError_t finish_step(const float* loss_part, uint32_t n, float* loss, uint32_t* step_counter, hipStream_t st) {
-    k_finish_step<<<1, kFinishThreads, 0, st>>>(loss_part, n, loss, step_counter);
     return hipGetLastError();
 }
 
