@@ -150,17 +150,18 @@ struct RefineStepTables {
 };
 
 // One view of the refinement, reduced to its pixels with mask > 0 ("active"),
-// in raster order, with the packed range of every 64-pixel raster group.
+// in raster order.
 struct RefineView {
-    const float4* uvm;      // [active] (u, v, mask, raster index as bits)
-    const float4* rgb;      // [active] target (r, g, b, 0)
-    const uint32_t* gstart; // [ngroups + 1] first packed pixel of each raster group, then `active`
-    uint32_t active;        // pixels with mask > 0 (known on the host once precompute has read the counts back)
-    uint32_t ngroups;       // ceil(pixels / 64)
+    const float4* uvm;  // [active] (u, v, mask, raster index as bits)
+    const float4* rgb;  // [active] target (r, g, b, 0)
+    float* loss;        // [pixels] per-pixel loss term of the view's last step (0 without mask)
+    uint32_t active;    // pixels with mask > 0 (known on the host once precompute has read the counts back)
+    uint32_t pixels;
 };
 
 // Per-group photometric losses of the last `period` steps: step s writes row
-// s % period, slot b at b * max_groups (row = batch * max_groups floats).
+// s % period, slot b at b * max_groups (row = batch * max_groups floats, groups
+// past a view's last hold 0; max_groups is a multiple of 4).
 struct LossParts {
     float* part;
     uint32_t period;
@@ -195,21 +196,31 @@ struct SeamResidualArgs {
     float4* resid = nullptr;
 };
 // views_dev: [steps * batch] view records, one per step slot (slot b of step s at s * batch + b).
-hipError_t photometric_step(const RefineView* views_dev, uint32_t batch, const RefineBatchTaps* batches_dev,
-                            uint32_t nbatches, uint32_t max_runs, const float* tex, uint32_t tw, uint32_t th, float eps,
-                            const RefineStepTables& tables, const uint32_t* step_counter, float4* resid,
-                            uint32_t stride, float* grad, const LossParts& parts, const SeamResidualArgs& seam,
-                            hipStream_t st);
-// loss[s] for the `count` steps before the device step counter, from their loss partials.
-hipError_t loss_fold(const RefineView* views_dev, uint32_t batch, const LossParts& parts, uint32_t count,
-                     const uint32_t* step_counter, float* loss, hipStream_t st);
-// Active pixels of one view of n pixels in raster order: flags, then (after an
-// exclusive scan of the flags) the packed records and the group starts.
-hipError_t active_flags(const float* mask, uint32_t n, uint32_t* flags, hipStream_t st);
-hipError_t active_pack(const float* uv, const float* target, const float* mask, uint32_t n, const uint32_t* flags,
-                       const uint32_t* pos, float4* uvm, float4* rgb, hipStream_t st);
+hipError_t photometric_step(const RefineView* views_dev, uint32_t batch, uint32_t max_active,
+                            const RefineBatchTaps* batches_dev, uint32_t nbatches, uint32_t max_runs,
+                            const float* tex, uint32_t tw, uint32_t th, float eps, const RefineStepTables& tables,
+                            const uint32_t* step_counter, float4* resid, uint32_t stride, float* grad,
+                            const SeamResidualArgs& seam, hipStream_t st);
+// The step's photometric group losses, computed by extra blocks of the Adam
+// kernel that follows the step's gather (blocks = 0: none).
+struct PhotoLossArgs {
+    const RefineView* views = nullptr;
+    uint32_t batch = 0;
+    LossParts parts{};
+    uint32_t blocks = 0;
+};
+PhotoLossArgs photo_loss_args(const RefineView* views_dev, uint32_t batch, const LossParts& parts);
+// loss[s] for steps first .. first + count - 1 (count <= parts.period), from their loss partials.
+hipError_t loss_fold(const LossParts& parts, uint32_t first, uint32_t count, float* loss, hipStream_t st);
+// Active pixels of one view: flags over raster order and over 8x8-tile-order
+// slots; after exclusive scans of both, the records packed in tile order and
+// order[raster rank] = packed index.
+uint32_t tile_slots(uint32_t W, uint32_t H);
+hipError_t active_flags(const float* mask, uint32_t W, uint32_t H, bool tile_order, uint32_t* flags, hipStream_t st);
+hipError_t active_pack(const float* uv, const float* target, const float* mask, uint32_t W, uint32_t H,
+                       const uint32_t* tflags, const uint32_t* tpos, const uint32_t* rpos, float4* uvm, float4* rgb,
+                       uint32_t* order, hipStream_t st);
 uint32_t pixel_groups(uint32_t n);
-hipError_t group_starts(const uint32_t* flags, const uint32_t* pos, uint32_t n, uint32_t* gstart, hipStream_t st);
 hipError_t exclusive_scan(const uint32_t* in, uint32_t* out, uint32_t n, void* temp, size_t temp_bytes, hipStream_t st);
 size_t exclusive_scan_temp_bytes(uint32_t n);
 hipError_t active_count(const uint32_t* flags, const uint32_t* pos, uint32_t n, uint32_t* count, hipStream_t st);
@@ -220,9 +231,10 @@ hipError_t norm_table(const double* view_weight, const uint32_t* view_index, uin
 // bits: tiles per row (0: linear texel keys) and the number of keys.
 uint32_t texel_key_tiles(uint32_t tw, uint32_t th);
 uint64_t texel_key_range(uint32_t tw, uint32_t th);
-// Tap entries of batch slot `slot` over its view's `count` packed pixels uvm.
-hipError_t batch_taps(const float4* uvm, uint32_t count, uint32_t slot, uint32_t stride, uint32_t tw, uint32_t th,
-                      uint32_t* keys, uint32_t* vals, hipStream_t st);
+// Tap entries of batch slot `slot` over its view's `count` packed pixels uvm,
+// generated in raster order (order[raster rank] = packed index).
+hipError_t batch_taps(const float4* uvm, const uint32_t* order, uint32_t count, uint32_t slot, uint32_t stride,
+                      uint32_t tw, uint32_t th, uint32_t* keys, uint32_t* vals, hipStream_t st);
 // Weights of sorted batch tap entries; slot_uvm[slot] = that slot's packed pixels (device array).
 hipError_t batch_tap_weights(const uint32_t* vals, uint32_t E, const float4* const* slot_uvm, uint32_t stride,
                              uint32_t tw, uint32_t th, float* w, hipStream_t st);
@@ -253,7 +265,7 @@ hipError_t seam_tap_weights(const uint32_t* vals, uint32_t E, const float* pairs
 hipError_t adam_tv_step(const float* tex, float* tex_out, float* grad, float* m, float* v, uint32_t tw, uint32_t th,
                         const uint8_t* tv_mask, float tv_eps, float tv_scale, const float* lr_table,
                         const float* bc1_table, const float* bc2_table, float b1, float b2, float eps, float cmin,
-                        float cmax, const uint32_t* step_counter, hipStream_t st);
+                        float cmax, const uint32_t* step_counter, const PhotoLossArgs& ploss, hipStream_t st);
 hipError_t advance_step(uint32_t* step_counter, hipStream_t st);
 // Per-texel neighbour mask of the chart-aware TV (chart may be null: one chart).
 hipError_t tv_mask(const int32_t* chart, uint32_t tw, uint32_t th, uint8_t* mask, hipStream_t st);
@@ -261,7 +273,7 @@ hipError_t tv_gradient(const float* tex, uint32_t tw, uint32_t th, const int32_t
                        float* grad, hipStream_t st);
 hipError_t adam_step(float* tex, float* grad, float* m, float* v, uint32_t n, const float* lr_table,
                      const float* bc1_table, const float* bc2_table, float b1, float b2, float eps, float cmin,
-                     float cmax, const uint32_t* step_counter, hipStream_t st);
+                     float cmax, const uint32_t* step_counter, const PhotoLossArgs& ploss, hipStream_t st);
 hipError_t tv_pair_count(const int32_t* chart, uint32_t tw, uint32_t th, unsigned long long* out, hipStream_t st);
 
 // ---- bake.hip

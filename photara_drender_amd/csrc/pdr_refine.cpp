@@ -116,9 +116,15 @@ pdr_status pdr_refine(pdr_context ctx, const float* initial_texture, const float
         double* view_weight = bufs.alloc<double>(nviews);
         uint32_t* view_counts = bufs.alloc<uint32_t>(nviews);  // active pixels
         std::vector<pdr::RefineView> rviews(nviews);
-        uint32_t* act_flags = bufs.alloc<uint32_t>(max_pixels);
-        uint32_t* act_pos = bufs.alloc<uint32_t>(max_pixels);
-        const size_t scan_temp_bytes = pdr::exclusive_scan_temp_bytes(max_pixels);
+        std::vector<const uint32_t*> view_order(nviews);  // raster rank -> packed index, per view
+        uint32_t max_slots = 0;  // tile-order slots (>= pixels)
+        for (uint32_t k = 0; k < nviews; ++k)
+            max_slots = std::max(max_slots, pdr::tile_slots(views[k].width, views[k].height));
+        uint32_t* act_flags = bufs.alloc<uint32_t>(max_slots);  // tile order
+        uint32_t* act_pos = bufs.alloc<uint32_t>(max_slots);
+        uint32_t* rank_flags = bufs.alloc<uint32_t>(max_pixels);  // raster order
+        uint32_t* rank_pos = bufs.alloc<uint32_t>(max_pixels);
+        const size_t scan_temp_bytes = pdr::exclusive_scan_temp_bytes(max_slots);
         void* scan_temp = bufs.alloc<unsigned char>(scan_temp_bytes);
         float* vuv = bufs.alloc<float>(static_cast<size_t>(max_pixels) * 2);
         float* target = bufs.alloc<float>(static_cast<size_t>(max_pixels) * 3);
@@ -130,8 +136,8 @@ pdr_status pdr_refine(pdr_context ctx, const float* initial_texture, const float
         uint32_t max_groups = 0;
         for (uint32_t k = 0; k < nviews; ++k) {
             const uint32_t P = views[k].width * views[k].height;
-            total_bytes += 2 * up(P * sizeof(float4)) + up((pdr::pixel_groups(P) + 1u) * sizeof(uint32_t));
-            max_groups = std::max(max_groups, pdr::pixel_groups(P));
+            total_bytes += 2 * up(P * sizeof(float4)) + 2 * up(P * sizeof(float));
+            max_groups = std::max(max_groups, (pdr::pixel_groups(P) + 3u) & ~3u);
         }
         unsigned char* view_arena = bufs.alloc<unsigned char>(std::max<size_t>(total_bytes, 256));
         size_t arena_off = 0;
@@ -163,23 +169,32 @@ pdr_status pdr_refine(pdr_context ctx, const float* initial_texture, const float
                 hip_check(pdr::refine_view_inputs(img, v.channels, vis, raster, P, target, mask, partials, np, st),
                           "refine_view_inputs");
                 hip_check(pdr::sum_partials(partials, np, view_weight + k, st), "sum_partials");
-                // the pixels with mask > 0, packed in raster order (the order in
-                // which the reference's pixels add their taps), and the packed
-                // range of every 64-pixel raster group
-                hip_check(pdr::active_flags(mask, P, act_flags, st), "active_flags");
-                hip_check(pdr::exclusive_scan(act_flags, act_pos, P, scan_temp, scan_temp_bytes, st),
+                // the pixels with mask > 0, packed in 8x8-tile order (atlas
+                // locality of a wave's taps), their raster ranks (the order in
+                // which the reference's pixels add their taps), and the view's
+                // per-pixel loss terms (zero where the mask is)
+                const uint32_t slots = pdr::tile_slots(v.width, v.height);
+                hip_check(pdr::active_flags(mask, v.width, v.height, true, act_flags, st), "active_flags");
+                hip_check(pdr::exclusive_scan(act_flags, act_pos, slots, scan_temp, scan_temp_bytes, st),
                           "exclusive_scan");
-                hip_check(pdr::active_count(act_flags, act_pos, P, view_counts + k, st), "active_count");
+                hip_check(pdr::active_count(act_flags, act_pos, slots, view_counts + k, st), "active_count");
+                hip_check(pdr::active_flags(mask, v.width, v.height, false, rank_flags, st), "active_flags");
+                hip_check(pdr::exclusive_scan(rank_flags, rank_pos, P, scan_temp, scan_temp_bytes, st),
+                          "exclusive_scan");
                 float4* uvm = reinterpret_cast<float4*>(view_arena + arena_off);
                 arena_off += up(static_cast<size_t>(P) * sizeof(float4));
                 float4* rgb = reinterpret_cast<float4*>(view_arena + arena_off);
                 arena_off += up(static_cast<size_t>(P) * sizeof(float4));
-                uint32_t* gstart = reinterpret_cast<uint32_t*>(view_arena + arena_off);
-                arena_off += up((pdr::pixel_groups(P) + 1u) * sizeof(uint32_t));
-                hip_check(pdr::active_pack(vuv, target, mask, P, act_flags, act_pos, uvm, rgb, st), "active_pack");
-                hip_check(pdr::group_starts(act_flags, act_pos, P, gstart, st), "group_starts");
-                // active: set once the counts are read back
-                rviews[k] = pdr::RefineView{uvm, rgb, gstart, 0u, pdr::pixel_groups(P)};
+                float* lterm = reinterpret_cast<float*>(view_arena + arena_off);
+                arena_off += up(static_cast<size_t>(P) * sizeof(float));
+                uint32_t* order = reinterpret_cast<uint32_t*>(view_arena + arena_off);
+                arena_off += up(static_cast<size_t>(P) * sizeof(uint32_t));
+                view_order[k] = order;
+                hip_check(pdr::active_pack(vuv, target, mask, v.width, v.height, act_flags, act_pos, rank_pos, uvm,
+                                           rgb, order, st),
+                          "active_pack");
+                hip_check(pdr::fill_f32(lterm, 0.0f, P, st), "fill");
+                rviews[k] = pdr::RefineView{uvm, rgb, lterm, 0u, P};  // active: set once the counts are read back
             }
         }
         std::vector<uint32_t> host_counts(nviews);
@@ -246,8 +261,8 @@ pdr_status pdr_refine(pdr_context ctx, const float* initial_texture, const float
             uint32_t off = 0;
             for (uint32_t b = 0; b < o.batch_size; ++b) {
                 const uint32_t k = slot_view(c, b);
-                hip_check(pdr::batch_taps(rviews[k].uvm, host_counts[k], b, max_active, o.width, o.height,
-                                          tap_keys + off, tap_vals + off, st),
+                hip_check(pdr::batch_taps(rviews[k].uvm, view_order[k], host_counts[k], b, max_active, o.width,
+                                          o.height, tap_keys + off, tap_vals + off, st),
                           "batch_taps");
                 off += host_counts[k] * 4u;
             }
@@ -283,16 +298,19 @@ pdr_status pdr_refine(pdr_context ctx, const float* initial_texture, const float
         float* m2 = bufs.alloc<float>(n);
         float* loss = bufs.alloc<float>(o.steps + seam_steps);
         float4* resid = bufs.alloc<float4>(static_cast<size_t>(o.batch_size) * max_active);
-        // Steps are folded into their losses every `fold` steps (a divisor of
-        // the captured block length, so the partial rows of a replayed block
-        // line up): the group partials of `fold` steps stay in memory.
+        // The loss partials of `fold` steps stay in memory and are folded into
+        // their step losses between blocks (outside the captured graphs):
+        // `fold` is a multiple of the block length, up to 1,024 steps or
+        // 256 MiB of partials.
         const uint32_t block = o.checkpoint_every ? o.checkpoint_every : std::min<uint32_t>(o.steps, 50);
-        uint32_t fold = block;
-        if (fold > 100)
-            for (fold = 100; block % fold; --fold) {
-            }
-        pdr::LossParts parts{nullptr, fold, max_groups, static_cast<size_t>(o.batch_size) * max_groups};
-        parts.part = bufs.alloc<float>(std::max<size_t>(parts.row * fold, 1));
+        pdr::LossParts parts{nullptr, block, max_groups, static_cast<size_t>(o.batch_size) * max_groups};
+        {
+            const size_t block_bytes = parts.row * block * sizeof(float);
+            const size_t by_mem = block_bytes ? (size_t{256} << 20) / block_bytes : 1;
+            const size_t blocks = std::max<size_t>(1, std::min<size_t>({by_mem, 1024u / block, (o.steps + block - 1) / block}));
+            parts.period = static_cast<uint32_t>(blocks * block);
+        }
+        parts.part = bufs.alloc<float>(std::max<size_t>(parts.row * parts.period, 4));
         float4* seam_resid = bufs.alloc<float4>(npairs);
         float* seam_part = bufs.alloc<float>(std::max<size_t>(static_cast<size_t>((npairs + 63u) / 64u) *
                                                                   (npairs ? o.seam_polish_steps : 0u),
@@ -392,14 +410,15 @@ pdr_status pdr_refine(pdr_context ctx, const float* initial_texture, const float
         float* texbuf[2] = {tex, fused_tv ? bufs.alloc<float>(n) : tex};
         const auto t2 = std::chrono::steady_clock::now();
         const bool joint_seam = o.joint_seam_weight != 0.0f && npairs;
+        const pdr::PhotoLossArgs ploss = pdr::photo_loss_args(dviews, o.batch_size, parts);
         auto one_step = [&](hipStream_t s, uint32_t c) {
             pdr::SeamResidualArgs seam_res;  // the joint seam residual rides in the photometric residual launch
             if (joint_seam)
                 seam_res = pdr::SeamResidualArgs{dpairs, npairs, o.seam_epsilon, seam_norm, o.joint_seam_weight,
                                                  seam_resid};
-            hip_check(pdr::photometric_step(dviews, o.batch_size, dbatches, nbatches, max_runs, texbuf[c], o.width,
-                                            o.height, o.photometric_epsilon, tables, counters, resid, max_active,
-                                            grad, parts, seam_res, s),
+            hip_check(pdr::photometric_step(dviews, o.batch_size, max_active, dbatches, nbatches, max_runs, texbuf[c],
+                                            o.width, o.height, o.photometric_epsilon, tables, counters, resid,
+                                            max_active, grad, seam_res, s),
                       "photometric_step");
             if (joint_seam)
                 hip_check(pdr::seam_step(texbuf[c], o.width, o.height, dpairs, npairs, o.seam_epsilon, seam_norm,
@@ -409,29 +428,18 @@ pdr_status pdr_refine(pdr_context ctx, const float* initial_texture, const float
             if (fused_tv)
                 hip_check(pdr::adam_tv_step(texbuf[c], texbuf[c ^ 1u], grad, m1, m2, o.width, o.height, dtv_mask, tv_eps,
                                             tv_scale, d_lr, d_bc1, d_bc2, o.adam_beta1, o.adam_beta2, o.adam_epsilon,
-                                            o.clamp_min, o.clamp_max, counters, s),
+                                            o.clamp_min, o.clamp_max, counters, ploss, s),
                           "adam_tv_step");
             else
                 hip_check(pdr::adam_step(texbuf[c], grad, m1, m2, n, d_lr, d_bc1, d_bc2, o.adam_beta1, o.adam_beta2,
-                                         o.adam_epsilon, o.clamp_min, o.clamp_max, counters, s),
+                                         o.adam_epsilon, o.clamp_min, o.clamp_max, counters, ploss, s),
                           "adam_step");
             hip_check(pdr::advance_step(counters, s), "advance_step");
             return fused_tv ? c ^ 1u : c;
         };
-        // `count` steps from a step that is a multiple of `fold`, each run of up
-        // to `fold` of them folded into its losses
-        auto run_steps = [&](hipStream_t s, uint32_t count, uint32_t c) {
-            for (uint32_t k = 0; k < count; ++k) {
-                c = one_step(s, c);
-                if ((k + 1) % fold == 0 || k + 1 == count)
-                    hip_check(pdr::loss_fold(dviews, o.batch_size, parts, k % fold + 1, counters, loss, s),
-                              "loss_fold");
-            }
-            return c;
-        };
         auto capture = [&](uint32_t count, uint32_t c, GraphExec& g) {
             hip_check(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-            run_steps(st, count, c);
+            for (uint32_t k = 0; k < count; ++k) c = one_step(st, c);
             hip_check(hipStreamEndCapture(st, &g.graph), "hipStreamEndCapture");
             hip_check(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0), "hipGraphInstantiate");
         };
@@ -443,7 +451,7 @@ pdr_status pdr_refine(pdr_context ctx, const float* initial_texture, const float
             if (nblocks > 1 && (block & flips)) capture(block, 1u, full[1]);
             if (rest) capture(rest, (nblocks * block) & flips, tail);
         }
-        uint32_t done = 0, ckpt = 0, cur = 0;
+        uint32_t done = 0, ckpt = 0, cur = 0, folded = 0;
         const size_t atlas_bytes = static_cast<size_t>(n) * 4;
         for (uint32_t b = 0; b <= nblocks; ++b) {
             const uint32_t count = b < nblocks ? block : rest;
@@ -452,9 +460,13 @@ pdr_status pdr_refine(pdr_context ctx, const float* initial_texture, const float
                 hip_check(hipGraphLaunch(b < nblocks ? full[cur].exec : tail.exec, st), "hipGraphLaunch");
                 cur ^= count & flips;
             } else {
-                cur = run_steps(st, count, cur);
+                for (uint32_t k = 0; k < count; ++k) cur = one_step(st, cur);
             }
             done += count;
+            if (done - folded == parts.period || done == o.steps) {
+                hip_check(pdr::loss_fold(parts, folded, done - folded, loss, st), "loss_fold");
+                folded = done;
+            }
             if (o.checkpoint_every && o.checkpoints && done % o.checkpoint_every == 0)
                 copy_any(reinterpret_cast<char*>(o.checkpoints) + static_cast<size_t>(ckpt++) * atlas_bytes,
                          texbuf[cur], atlas_bytes, st);
@@ -482,7 +494,7 @@ pdr_status pdr_refine(pdr_context ctx, const float* initial_texture, const float
                                          counters, seam_groups, seam_resid, grad, seam_part, true, st),
                           "seam_step");
                 hip_check(pdr::adam_step(tex, grad, m1, m2, n, d_slr, d_sb1, d_sb2, o.adam_beta1, o.adam_beta2,
-                                         o.adam_epsilon, o.clamp_min, o.clamp_max, counters, st),
+                                         o.adam_epsilon, o.clamp_min, o.clamp_max, counters, pdr::PhotoLossArgs{}, st),
                           "adam_step");
                 hip_check(pdr::advance_step(counters, st), "advance_step");
             }

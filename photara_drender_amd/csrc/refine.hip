@@ -5,20 +5,22 @@
 //                     (texture_refiner.cpp:228-242)
 //  k_photo_residual   atlas_photometric_gradient.hlsl:45-108 for all views of a
 //                     step in one launch, up to the per-pixel output gradient
-//                     d / r * m * norm, over each view's pixels with mask > 0
-//                     packed in raster order at precompute (k_active_flags, a
-//                     scan, k_active_pack); one wave per 64-pixel raster group
-//                     (the shader's thread group), whose loss is the shader's
-//                     64-lane tree sum; per-step scalars come from device
-//                     tables indexed by a device step counter so the step loop
-//                     can be captured once and replayed
+//                     d / r * m * norm and each pixel's loss term, over each
+//                     view's pixels with mask > 0 packed in 8x8-tile order at
+//                     precompute (k_active_flags, scans, k_active_pack); the
+//                     per-step scalars come from device tables indexed by a
+//                     device step counter so the step loop can be captured
+//                     once and replayed
+//  photo_loss_groups  (in the Adam kernels' extra blocks) the shader's 64-lane
+//                     tree sum of each 64-pixel raster group's loss terms
 //  k_loss_fold        each step's group losses added one after another in
 //                     (batch slot, group) order, as the shader's one atomic
 //                     add per group would if the groups ran in order
 //  k_photo_gather     the texel gradient as a gather: the step batches recur
 //                     (step s uses batch s % L), so each batch's bilinear taps
-//                     are inverted once at precompute (k_batch_taps, a radix
-//                     sort by texel, run-length offsets, k_batch_tap_weights),
+//                     are inverted once at precompute (k_batch_taps in raster
+//                     order of the pixels, a stable radix sort by texel,
+//                     run-length offsets, k_batch_tap_weights),
 //                     and a step sums og * w over each texel's taps from all
 //                     its views with one plain read-modify-write - the same
 //                     terms as the reference's per-pixel atomic scatter, in a
@@ -167,18 +169,16 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// One wave per 64-pixel raster group g of the step slot's view (the shader's
-// thread group of pixels 64g .. 64g + 63): lane l takes the group's l-th
-// active pixel (packed range gstart[g] .. gstart[g + 1]).  The group's loss is
-// the shader's 64-lane tree sum with every pixel at its raster lane and zeros
-// elsewhere; it goes to the loss partials of the step's row, slot and group.
+// One lane per active pixel of each batch view (packed in raster order): the
+// output gradient into resid, and the pixel's loss term sum_ch r * m * norm into
+// the view's dense per-pixel loss array (zero at pixels without mask, set once
+// at precompute) for k_photo_loss.
 __global__ __launch_bounds__(kBlock) void k_photo_residual(const RefineView* __restrict__ views, uint32_t batch,
                                                            const float* __restrict__ tex, uint32_t tw, uint32_t th,
                                                            float eps, RefineStepTables tables,
                                                            const uint32_t* __restrict__ step_counter,
                                                            float4* __restrict__ resid, uint32_t max_active,
-                                                           LossParts parts, SeamResidualArgs seam) {
-    __shared__ float lanes[kBlock];
+                                                           SeamResidualArgs seam) {
     if (blockIdx.y == batch) {  // the joint seam term's residual, beside the views' (no loss)
         for (uint32_t q = blockIdx.x * kBlock + threadIdx.x; q < seam.npairs; q += gridDim.x * kBlock)
             (void)seam_pair_residual(q, tex, tw, th, seam.pairs, seam.eps, seam.norm, seam.weight, seam.resid);
@@ -187,73 +187,93 @@ __global__ __launch_bounds__(kBlock) void k_photo_residual(const RefineView* __r
     const uint32_t step = *step_counter;
     const uint32_t b = blockIdx.y;
     const RefineView& rv = views[static_cast<size_t>(step) * batch + b];  // the step slot's view record
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t g = blockIdx.x * (kBlock / 64u) + (threadIdx.x >> 6);
-    if (g >= rv.ngroups) return;  // wave-uniform
-    const uint32_t s = rv.gstart[g], e = rv.gstart[g + 1u];
-    float* part = parts.part + static_cast<size_t>(step % parts.period) * parts.row +
-                  static_cast<size_t>(b) * parts.max_groups + g;
-    if (s == e) {  // no active pixel: the group adds nothing
-        if (lane == 0) *part = 0.0f;
-        return;
-    }
+    const uint32_t a = blockIdx.x * kBlock + threadIdx.x;
+    if (a >= rv.active) return;
     const float norm = tables.norm[step];
-    const uint32_t a = s + lane;
+    const float4 q = rv.uvm[a];
+    const float4 target = rv.rgb[a];
+    const float m = q.z;
+    const AtlasTap at = atlas_tap(q.x, q.y, tw, th);
+    const float tgt[3] = {target.x, target.y, target.z};
+    float og[3];
     float local = 0.0f;
-    uint32_t at_lane = lane;
-    if (a < e) {
-        const float4 q = rv.uvm[a];
-        const float4 target = rv.rgb[a];
-        const float m = q.z;
-        at_lane = __float_as_uint(q.w) & 63u;  // raster index of the pixel
-        const AtlasTap at = atlas_tap(q.x, q.y, tw, th);
-        const float tgt[3] = {target.x, target.y, target.z};
-        float og[3];
 #pragma unroll
-        for (int ch = 0; ch < 3; ++ch) {
-            float rendered = 0.0f;
+    for (int ch = 0; ch < 3; ++ch) {
+        float rendered = 0.0f;
 #pragma unroll
-            for (int k = 0; k < 4; ++k) rendered += tex[static_cast<size_t>(at.t[k]) * 3 + ch] * at.w[k];
-            const float d = rendered - tgt[ch];
-            const float r = sqrtf(d * d + eps * eps);
-            og[ch] = d / r * m * norm;
-            local += r * m * norm;
-        }
-        resid[static_cast<size_t>(b) * max_active + a] = make_float4(og[0], og[1], og[2], 0.0f);
+        for (int k = 0; k < 4; ++k) rendered += tex[static_cast<size_t>(at.t[k]) * 3 + ch] * at.w[k];
+        const float d = rendered - tgt[ch];
+        const float r = sqrtf(d * d + eps * eps);
+        og[ch] = d / r * m * norm;
+        local += r * m * norm;
     }
-    float* wl = lanes + (threadIdx.x & ~63u);
-    wl[lane] = 0.0f;
-    wave_lds_sync();
-    if (a < e) wl[at_lane] = local;
-    wave_lds_sync();
-    const float t = wave_tree_sum(wl[lane]);
-    if (lane == 0) *part = t;
+    resid[static_cast<size_t>(b) * max_active + a] = make_float4(og[0], og[1], og[2], 0.0f);
+    rv.loss[__float_as_uint(q.w)] = local;
 }
 
-// loss[step] of the `count` steps before the device step counter: each step's
-// group losses added one after another in (batch slot, group) order, zeros
-// skipped, as the shader's one add per group lands when the groups run in that
-// order.  One lane per step.
-__global__ void k_loss_fold(const RefineView* __restrict__ views, uint32_t batch, LossParts parts, uint32_t count,
-                            const uint32_t* __restrict__ step_counter, float* __restrict__ loss) {
+// The loss of 64-pixel raster group g of each batch view (the shader's thread
+// group of pixels 64g .. 64g + 63): the shader's 64-lane tree sum of the
+// pixels' loss terms, into the step's row of loss partials.  Groups past the
+// view's last (up to the widest view of the batch) write 0.  Run by the first
+// blocks of the Adam kernels (after the residual pass has written the terms):
+// wave w of those blocks takes groups 4w .. 4w + 3 of the row (slot
+// q / max_groups, group q % max_groups; max_groups is a multiple of 4).
+constexpr uint32_t kLossGroupsPerWave = 4;
+__device__ __forceinline__ void photo_loss_groups(const RefineView* __restrict__ views, uint32_t batch, uint32_t step,
+                                                  const LossParts& parts, uint32_t w) {
+    const uint32_t q0 = w * kLossGroupsPerWave;
+    const uint32_t b = q0 / parts.max_groups, g0 = q0 % parts.max_groups;
+    if (b >= batch) return;  // wave-uniform
+    const RefineView& rv = views[static_cast<size_t>(step) * batch + b];
+    const uint32_t lane = threadIdx.x & 63u;
+    float v[kLossGroupsPerWave];
+#pragma unroll
+    for (uint32_t j = 0; j < kLossGroupsPerWave; ++j) {
+        const uint32_t p = (g0 + j) * 64u + lane;
+        v[j] = p < rv.pixels ? rv.loss[p] : 0.0f;
+    }
+    float t[kLossGroupsPerWave];
+#pragma unroll
+    for (uint32_t j = 0; j < kLossGroupsPerWave; ++j) t[j] = wave_tree_sum(v[j]);
+    if (lane == 0) {
+        float* out = parts.part + static_cast<size_t>(step % parts.period) * parts.row +
+                     static_cast<size_t>(b) * parts.max_groups + g0;
+        *reinterpret_cast<float4*>(out) = make_float4(t[0], t[1], t[2], t[3]);
+    }
+}
+
+// loss[step] of steps first .. first + count - 1: each step's group losses
+// (rows of batch * max_groups partials) added one after another in (batch slot,
+// group) order, as the shader's one add per group lands when the groups run in
+// that order.  The shader skips zero group sums; every partial is >= +0 (a sum
+// of r * m * norm terms) and so is the running sum, so adding a zero changes
+// nothing.  One lane per step, eight float4 loads in flight ahead of the adds.
+__global__ void k_loss_fold(LossParts parts, uint32_t first, uint32_t count, float* __restrict__ loss) {
     const uint32_t r = blockIdx.x * 64u + threadIdx.x;
     if (r >= count) return;
-    const uint32_t step = *step_counter - count + r;
-    const float* row = parts.part + static_cast<size_t>(step % parts.period) * parts.row;
+    const uint32_t step = first + r;
+    const float4* row = reinterpret_cast<const float4*>(parts.part + static_cast<size_t>(step % parts.period) * parts.row);
+    const uint32_t n4 = static_cast<uint32_t>(parts.row / 4u);
+    constexpr uint32_t kAhead = 8;
+    float4 buf[kAhead];
+    const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+    for (uint32_t i = 0; i < kAhead; ++i) buf[i] = i < n4 ? row[i] : zero;
     float acc = 0.0f;
-    for (uint32_t b = 0; b < batch; ++b) {
-        const uint32_t ng = views[static_cast<size_t>(step) * batch + b].ngroups;
-        const float* p = row + static_cast<size_t>(b) * parts.max_groups;
-        uint32_t g = 0;
-        for (; g + 4u <= ng; g += 4u) {  // four loads in flight, added in order
-            const float v0 = p[g], v1 = p[g + 1u], v2 = p[g + 2u], v3 = p[g + 3u];
-            if (v0 != 0.0f) acc += v0;
-            if (v1 != 0.0f) acc += v1;
-            if (v2 != 0.0f) acc += v2;
-            if (v3 != 0.0f) acc += v3;
+    for (uint32_t c = 0; c < n4; c += kAhead) {
+        float4 cur[kAhead];
+#pragma unroll
+        for (uint32_t i = 0; i < kAhead; ++i) {
+            cur[i] = buf[i];
+            buf[i] = c + kAhead + i < n4 ? row[c + kAhead + i] : zero;
         }
-        for (; g < ng; ++g)
-            if (p[g] != 0.0f) acc += p[g];
+#pragma unroll
+        for (uint32_t i = 0; i < kAhead; ++i) {
+            acc += cur[i].x;
+            acc += cur[i].y;
+            acc += cur[i].z;
+            acc += cur[i].w;
+        }
     }
     loss[step] = acc;
 }
@@ -354,52 +374,67 @@ __global__ __launch_bounds__(kBlock) void k_photo_gather(const RefineBatchTaps* 
     texel_gather<2>(bt.texels, bt.tap_offsets, bt.taps, bt.tap_weights, *bt.runs, resid, tw, tiles_x, grad, prod);
 }
 
-__global__ void k_active_flags(const float* __restrict__ mask, uint32_t n, uint32_t* __restrict__ flags) {
-    const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
-    if (p < n) flags[p] = mask[p] > 0.0f ? 1u : 0u;
+// Tile-order slot q of a W x H view: tile q / 64 (row-major over 8x8 tiles),
+// pixel (q % 64) row-major inside the tile.
+__device__ __forceinline__ bool tile_pixel(uint32_t q, uint32_t W, uint32_t H, uint32_t& p) {
+    const uint32_t tiles_x = (W + 7u) / 8u, tile = q >> 6, lane = q & 63u;
+    const uint32_t x = (tile % tiles_x) * 8u + (lane & 7u), y = (tile / tiles_x) * 8u + (lane >> 3);
+    p = y * W + x;
+    return x < W && y < H;
 }
 
-// Packed record a = pos[p] of active pixel p: (u, v, mask, p as bits), target.
+// flags over raster order (tile_order = 0: slot = pixel) or over tile-order slots
+__global__ void k_active_flags(const float* __restrict__ mask, uint32_t W, uint32_t H, uint32_t n,
+                               uint32_t tile_order, uint32_t* __restrict__ flags) {
+    const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
+    if (q >= n) return;
+    uint32_t p = q;
+    const bool in = tile_order ? tile_pixel(q, W, H, p) : true;
+    flags[q] = in && mask[p] > 0.0f ? 1u : 0u;
+}
+
+// Packed record a = tpos[q] of the active pixel p at tile-order slot q:
+// (u, v, mask, p as bits) and the target; order[raster rank of p] = a.
 __global__ void k_active_pack(const float* __restrict__ uv, const float* __restrict__ target,
-                              const float* __restrict__ mask, uint32_t n, const uint32_t* __restrict__ flags,
-                              const uint32_t* __restrict__ pos, float4* __restrict__ uvm, float4* __restrict__ rgb) {
-    const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
-    if (p >= n || !flags[p]) return;
-    const uint32_t a = pos[p];
+                              const float* __restrict__ mask, uint32_t W, uint32_t H, uint32_t n,
+                              const uint32_t* __restrict__ tflags, const uint32_t* __restrict__ tpos,
+                              const uint32_t* __restrict__ rpos, float4* __restrict__ uvm, float4* __restrict__ rgb,
+                              uint32_t* __restrict__ order) {
+    const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
+    if (q >= n || !tflags[q]) return;
+    uint32_t p = 0;
+    tile_pixel(q, W, H, p);
+    const uint32_t a = tpos[q];
     uvm[a] = make_float4(uv[static_cast<size_t>(p) * 2], uv[static_cast<size_t>(p) * 2 + 1], mask[p],
                          __uint_as_float(p));
     rgb[a] = make_float4(target[static_cast<size_t>(p) * 3], target[static_cast<size_t>(p) * 3 + 1],
                          target[static_cast<size_t>(p) * 3 + 2], 0.0f);
+    order[rpos[p]] = a;
 }
 
-// First packed index of each 64-pixel raster group, and the active count after
-// the last: gstart[g] = pos[64 g] for g < groups, gstart[groups] = active.
-__global__ void k_group_starts(const uint32_t* __restrict__ flags, const uint32_t* __restrict__ pos, uint32_t n,
-                               uint32_t groups, uint32_t* __restrict__ gstart) {
-    const uint32_t g = blockIdx.x * kBlock + threadIdx.x;
-    if (g > groups) return;
-    gstart[g] = g < groups ? pos[g * 64u] : (n ? pos[n - 1] + flags[n - 1] : 0u);
-}
 
 __global__ void k_active_count(const uint32_t* __restrict__ flags, const uint32_t* __restrict__ pos, uint32_t n,
                                uint32_t* __restrict__ count) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *count = n ? flags[n - 1] + pos[n - 1] : 0u;
 }
 
-// Tap entries of batch slot `slot` over its view's `count` packed pixels:
-// key = texel, value = (slot * stride + a) * 4 + tap.
-__global__ void k_batch_taps(const float4* __restrict__ uvm, uint32_t count, uint32_t slot, uint32_t stride,
-                             uint32_t tw, uint32_t th, uint32_t tiles_x, uint32_t* __restrict__ keys,
-                             uint32_t* __restrict__ vals) {
-    const uint32_t a = blockIdx.x * kBlock + threadIdx.x;
-    if (a >= count) return;
+// Tap entries of batch slot `slot` over its view's `count` active pixels, in
+// raster order (order[r] = packed index of the pixel of raster rank r), so that
+// a stable sort by texel leaves every texel's entries in (slot, raster pixel,
+// tap) order: key = texel, value = (slot * stride + a) * 4 + tap.
+__global__ void k_batch_taps(const float4* __restrict__ uvm, const uint32_t* __restrict__ order, uint32_t count,
+                             uint32_t slot, uint32_t stride, uint32_t tw, uint32_t th, uint32_t tiles_x,
+                             uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    const uint32_t r = blockIdx.x * kBlock + threadIdx.x;
+    if (r >= count) return;
+    const uint32_t a = order[r];
     const float4 q = uvm[a];
     const AtlasTap at = atlas_tap(q.x, q.y, tw, th);
     const uint32_t base = (slot * stride + a) * 4u;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        keys[static_cast<size_t>(a) * 4 + k] = texel_key(at.t[k], tw, tiles_x);
-        vals[static_cast<size_t>(a) * 4 + k] = base + static_cast<uint32_t>(k);
+        keys[static_cast<size_t>(r) * 4 + k] = texel_key(at.t[k], tw, tiles_x);
+        vals[static_cast<size_t>(r) * 4 + k] = base + static_cast<uint32_t>(k);
     }
 }
 
@@ -622,10 +657,16 @@ __global__ __launch_bounds__(kBlock) void k_adam_tv(const float* __restrict__ te
                                                     const uint8_t* __restrict__ tv_mask, float tv_eps2, float tv_scale,
                                                     const float* __restrict__ lr_t, const float* __restrict__ bc1_t,
                                                     const float* __restrict__ bc2_t, float b1, float b2, float eps,
-                                                    float cmin, float cmax, const uint32_t* __restrict__ step_counter) {
+                                                    float cmin, float cmax, const uint32_t* __restrict__ step_counter,
+                                                    PhotoLossArgs ploss) {
     const uint32_t step = *step_counter;
+    if (blockIdx.x < ploss.blocks) {  // the step's photometric group losses first
+        photo_loss_groups(ploss.views, ploss.batch, step, ploss.parts, blockIdx.x * (kBlock / 64u) + (threadIdx.x >> 6));
+        return;
+    }
+    const uint32_t blk = blockIdx.x - ploss.blocks, nadam = gridDim.x - ploss.blocks;
     const float lr = lr_t[step], bc1 = bc1_t[step], bc2 = bc2_t[step];
-    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < tw * th; i += gridDim.x * kBlock) {
+    for (uint32_t i = blk * kBlock + threadIdx.x; i < tw * th; i += nadam * kBlock) {
         const size_t b = static_cast<size_t>(i) * 3;
         float gr[3], m[3], v[3];
 #pragma unroll
@@ -666,11 +707,17 @@ __global__ __launch_bounds__(kBlock) void k_adam(float* __restrict__ tex, float*
                                                  float* __restrict__ mom1, float* __restrict__ mom2, uint32_t n,
                                                  const float* __restrict__ lr_t, const float* __restrict__ bc1_t,
                                                  const float* __restrict__ bc2_t, float b1, float b2, float eps,
-                                                 float cmin, float cmax, const uint32_t* __restrict__ step_counter) {
+                                                 float cmin, float cmax, const uint32_t* __restrict__ step_counter,
+                                                 PhotoLossArgs ploss) {
     const uint32_t step = *step_counter;
+    if (blockIdx.x < ploss.blocks) {  // the step's photometric group losses first
+        photo_loss_groups(ploss.views, ploss.batch, step, ploss.parts, blockIdx.x * (kBlock / 64u) + (threadIdx.x >> 6));
+        return;
+    }
+    const uint32_t blk = blockIdx.x - ploss.blocks, nadam = gridDim.x - ploss.blocks;
     const float lr = lr_t[step], bc1 = bc1_t[step], bc2 = bc2_t[step];
     const uint32_t n4 = n / 4;
-    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n4; i += gridDim.x * kBlock) {
+    for (uint32_t i = blk * kBlock + threadIdx.x; i < n4; i += nadam * kBlock) {
         float4 p = reinterpret_cast<float4*>(tex)[i];
         float4 g = reinterpret_cast<float4*>(grad)[i];
         float4 m = reinterpret_cast<float4*>(mom1)[i];
@@ -684,7 +731,7 @@ __global__ __launch_bounds__(kBlock) void k_adam(float* __restrict__ tex, float*
         reinterpret_cast<float4*>(mom1)[i] = m;
         reinterpret_cast<float4*>(mom2)[i] = v;
     }
-    if (blockIdx.x == 0) {
+    if (blk == 0) {
         for (uint32_t i = n4 * 4 + threadIdx.x; i < n; i += kBlock)
             tex[i] = adam_one(tex[i], grad[i], mom1[i], mom2[i], lr, b1, b2, eps, bc1, bc2, cmin, cmax);
     }
@@ -724,46 +771,50 @@ uint64_t texel_key_range(uint32_t tw, uint32_t th) {
     return tx ? static_cast<uint64_t>(tx) * ((th + 7u) / 8u) * 64u : static_cast<uint64_t>(tw) * th;
 }
 
-hipError_t photometric_step(const RefineView* views_dev, uint32_t batch, const RefineBatchTaps* batches_dev,
-                            uint32_t nbatches, uint32_t max_runs, const float* tex, uint32_t tw, uint32_t th, float eps,
-                            const RefineStepTables& tables, const uint32_t* step_counter, float4* resid,
-                            uint32_t stride, float* grad, const LossParts& parts, const SeamResidualArgs& seam,
-                            hipStream_t st) {
-    const uint32_t blocks = (parts.max_groups + kBlock / 64u - 1u) / (kBlock / 64u);
+hipError_t photometric_step(const RefineView* views_dev, uint32_t batch, uint32_t max_active,
+                            const RefineBatchTaps* batches_dev, uint32_t nbatches, uint32_t max_runs,
+                            const float* tex, uint32_t tw, uint32_t th, float eps, const RefineStepTables& tables,
+                            const uint32_t* step_counter, float4* resid, uint32_t stride, float* grad,
+                            const SeamResidualArgs& seam, hipStream_t st) {
+    const uint32_t blocks = (max_active + kBlock - 1) / kBlock;
     const dim3 grid(blocks ? blocks : 1, batch + (seam.npairs ? 1u : 0u));
     k_photo_residual<<<grid, kBlock, 0, st>>>(views_dev, batch, tex, tw, th, eps, tables, step_counter, resid, stride,
-                                              parts, seam);
+                                              seam);
     const uint32_t gblocks = (max_runs + kBlock - 1) / kBlock;
     k_photo_gather<<<gblocks ? gblocks : 1, kBlock, 0, st>>>(batches_dev, nbatches, tw, texel_key_tiles(tw, th),
                                                              step_counter, resid, grad);
     return hipGetLastError();
 }
 
-hipError_t active_flags(const float* mask, uint32_t n, uint32_t* flags, hipStream_t st) {
+uint32_t tile_slots(uint32_t W, uint32_t H) { return ((W + 7u) / 8u) * ((H + 7u) / 8u) * 64u; }
+
+hipError_t active_flags(const float* mask, uint32_t W, uint32_t H, bool tile_order, uint32_t* flags, hipStream_t st) {
+    const uint32_t n = tile_order ? tile_slots(W, H) : W * H;
     if (!n) return hipSuccess;
-    k_active_flags<<<(n + kBlock - 1) / kBlock, kBlock, 0, st>>>(mask, n, flags);
+    k_active_flags<<<(n + kBlock - 1) / kBlock, kBlock, 0, st>>>(mask, W, H, n, tile_order ? 1u : 0u, flags);
     return hipGetLastError();
 }
 
-hipError_t active_pack(const float* uv, const float* target, const float* mask, uint32_t n, const uint32_t* flags,
-                       const uint32_t* pos, float4* uvm, float4* rgb, hipStream_t st) {
+hipError_t active_pack(const float* uv, const float* target, const float* mask, uint32_t W, uint32_t H,
+                       const uint32_t* tflags, const uint32_t* tpos, const uint32_t* rpos, float4* uvm, float4* rgb,
+                       uint32_t* order, hipStream_t st) {
+    const uint32_t n = tile_slots(W, H);
     if (!n) return hipSuccess;
-    k_active_pack<<<(n + kBlock - 1) / kBlock, kBlock, 0, st>>>(uv, target, mask, n, flags, pos, uvm, rgb);
+    k_active_pack<<<(n + kBlock - 1) / kBlock, kBlock, 0, st>>>(uv, target, mask, W, H, n, tflags, tpos, rpos, uvm,
+                                                               rgb, order);
     return hipGetLastError();
 }
 
 uint32_t pixel_groups(uint32_t n) { return (n + 63u) / 64u; }
 
-hipError_t group_starts(const uint32_t* flags, const uint32_t* pos, uint32_t n, uint32_t* gstart, hipStream_t st) {
-    const uint32_t groups = pixel_groups(n);
-    k_group_starts<<<groups / kBlock + 1u, kBlock, 0, st>>>(flags, pos, n, groups, gstart);
-    return hipGetLastError();
+PhotoLossArgs photo_loss_args(const RefineView* views_dev, uint32_t batch, const LossParts& parts) {
+    const uint64_t waves = static_cast<uint64_t>(batch) * parts.max_groups / kLossGroupsPerWave;
+    return PhotoLossArgs{views_dev, batch, parts, static_cast<uint32_t>((waves + kBlock / 64u - 1u) / (kBlock / 64u))};
 }
 
-hipError_t loss_fold(const RefineView* views_dev, uint32_t batch, const LossParts& parts, uint32_t count,
-                     const uint32_t* step_counter, float* loss, hipStream_t st) {
+hipError_t loss_fold(const LossParts& parts, uint32_t first, uint32_t count, float* loss, hipStream_t st) {
     if (!count) return hipSuccess;
-    k_loss_fold<<<(count + 63u) / 64u, 64, 0, st>>>(views_dev, batch, parts, count, step_counter, loss);
+    k_loss_fold<<<(count + 63u) / 64u, 64, 0, st>>>(parts, first, count, loss);
     return hipGetLastError();
 }
 
@@ -790,10 +841,10 @@ hipError_t active_count(const uint32_t* flags, const uint32_t* pos, uint32_t n, 
     return hipGetLastError();
 }
 
-hipError_t batch_taps(const float4* uvm, uint32_t count, uint32_t slot, uint32_t stride, uint32_t tw, uint32_t th,
-                      uint32_t* keys, uint32_t* vals, hipStream_t st) {
+hipError_t batch_taps(const float4* uvm, const uint32_t* order, uint32_t count, uint32_t slot, uint32_t stride,
+                      uint32_t tw, uint32_t th, uint32_t* keys, uint32_t* vals, hipStream_t st) {
     if (!count) return hipSuccess;
-    k_batch_taps<<<(count + kBlock - 1) / kBlock, kBlock, 0, st>>>(uvm, count, slot, stride, tw, th,
+    k_batch_taps<<<(count + kBlock - 1) / kBlock, kBlock, 0, st>>>(uvm, order, count, slot, stride, tw, th,
                                                                    texel_key_tiles(tw, th), keys, vals);
     return hipGetLastError();
 }
@@ -875,11 +926,12 @@ hipError_t seam_tap_weights(const uint32_t* vals, uint32_t E, const float* pairs
 hipError_t adam_tv_step(const float* tex, float* tex_out, float* grad, float* m, float* v, uint32_t tw, uint32_t th,
                         const uint8_t* tv_mask, float tv_eps, float tv_scale, const float* lr_table,
                         const float* bc1_table, const float* bc2_table, float b1, float b2, float eps, float cmin,
-                        float cmax, const uint32_t* step_counter, hipStream_t st) {
+                        float cmax, const uint32_t* step_counter, const PhotoLossArgs& ploss, hipStream_t st) {
     uint32_t grid = (tw * th + kBlock - 1) / kBlock;
     if (grid > 8192) grid = 8192;
-    k_adam_tv<<<grid, kBlock, 0, st>>>(tex, tex_out, grad, m, v, tw, th, tv_mask, tv_eps * tv_eps, tv_scale, lr_table, bc1_table,
-                                       bc2_table, b1, b2, eps, cmin, cmax, step_counter);
+    k_adam_tv<<<grid + ploss.blocks, kBlock, 0, st>>>(tex, tex_out, grad, m, v, tw, th, tv_mask, tv_eps * tv_eps,
+                                                      tv_scale, lr_table, bc1_table, bc2_table, b1, b2, eps, cmin,
+                                                      cmax, step_counter, ploss);
     return hipGetLastError();
 }
 
@@ -908,13 +960,13 @@ hipError_t tv_pair_count(const int32_t* chart, uint32_t tw, uint32_t th, unsigne
 
 hipError_t adam_step(float* tex, float* grad, float* m, float* v, uint32_t n, const float* lr_table,
                      const float* bc1_table, const float* bc2_table, float b1, float b2, float eps, float cmin,
-                     float cmax, const uint32_t* step_counter, hipStream_t st) {
+                     float cmax, const uint32_t* step_counter, const PhotoLossArgs& ploss, hipStream_t st) {
     const uint32_t n4 = n / 4;
     uint32_t grid = (n4 + kBlock - 1) / kBlock;
     if (grid > 8192) grid = 8192;
     if (grid == 0) grid = 1;
-    k_adam<<<grid, kBlock, 0, st>>>(tex, grad, m, v, n, lr_table, bc1_table, bc2_table, b1, b2, eps, cmin, cmax,
-                                    step_counter);
+    k_adam<<<grid + ploss.blocks, kBlock, 0, st>>>(tex, grad, m, v, n, lr_table, bc1_table, bc2_table, b1, b2, eps,
+                                                   cmin, cmax, step_counter, ploss);
     return hipGetLastError();
 }
 
