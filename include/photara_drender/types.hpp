@@ -130,7 +130,7 @@ struct TextureBakeOptions {
     float softmax_scale = 0.0F;
 };
 
-// types.hpp:181-199 (filled_mask stays empty: pad_texture_atlas is not part of this backend)
+// types.hpp:181-199 (filled_mask stays empty until pad_texture_atlas has run on the output)
 struct TextureBakeOutput {
     std::uint32_t width = 0;
     std::uint32_t height = 0;
