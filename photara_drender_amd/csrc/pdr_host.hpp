@@ -481,15 +481,17 @@ inline void run_raster_backward(pdr_context ctx, const pdr::ViewTable& vt, const
     sb.won = reinterpret_cast<uint8_t*>(
         ctx->get_t<uint32_t>("backward_won", (static_cast<size_t>(vt.n) * T + 3) / 4));
     sb.slot_of = csr.slot_of;
+    // boxes over 65,536 pixels: the reference's row-major order under PDR_FLAG_DETERMINISTIC
+    const bool ordered = (ctx->flags & PDR_FLAG_DETERMINISTIC) != 0;
     ctx->timed("vertex_csr", st, [&] { hip_check(pdr::build_vertex_csr(idx, T, V, csr, st), "build_vertex_csr"); });
     ctx->timed("bwd_prepass", st,
                [&] { hip_check(pdr::backward_prepass(vt, idx, T, V, sb, st), "backward_prepass"); });
     const pdr::TuneParams tp =
         tuned_params(ctx, tune_key("backward", vt, T), backward_candidates(), st, [&](const pdr::TuneParams& c) {
-            hip_check(pdr::corner_gradients(vt, idx, T, V, sb, corner, c, st), "corner_gradients");
+            hip_check(pdr::corner_gradients(vt, idx, T, V, sb, corner, c, ordered, st), "corner_gradients");
         });
     ctx->timed("corner_grads", st,
-               [&] { hip_check(pdr::corner_gradients(vt, idx, T, V, sb, corner, tp, st), "corner_gradients"); });
+               [&] { hip_check(pdr::corner_gradients(vt, idx, T, V, sb, corner, tp, ordered, st), "corner_gradients"); });
     ctx->timed("vertex_reduce", st,
                [&] { hip_check(pdr::vertex_reduce(vt, T, V, csr, corner, st), "vertex_reduce"); });
 }

@@ -100,9 +100,12 @@ hipError_t build_vertex_csr(const uint32_t* idx, uint32_t T, uint32_t V, const V
 // k_bwd_prepass: per-pixel compared ids and per-view whole-box flags (raster, grad_raster of each view)
 hipError_t backward_prepass(const ViewTable& views, const uint32_t* idx, uint32_t T, uint32_t V,
                             const RasterScratch& s, hipStream_t st);
-// corner: [views * 3T] float4 per-triangle corner gradients (after backward_prepass)
+// corner: [views * 3T] float4 per-triangle corner gradients (after backward_prepass);
+// ordered: boxes over 65,536 pixels summed in the reference's row-major order
+// (else a fixed workgroup tree)
 hipError_t corner_gradients(const ViewTable& views, const uint32_t* idx, uint32_t T, uint32_t V,
-                            const RasterScratch& s, float4* corner, const TuneParams& tp, hipStream_t st);
+                            const RasterScratch& s, float4* corner, const TuneParams& tp, bool ordered,
+                            hipStream_t st);
 hipError_t vertex_reduce(const ViewTable& views, uint32_t T, uint32_t V, const VertexCsr& csr, const float4* corner,
                          hipStream_t st);
 

@@ -883,7 +883,8 @@ __global__ __launch_bounds__(kBlock, 4) void k_corner_small(ViewTable vt, const 
 
 // Boxes over kWaveBox pixels: each lane sums a strided subset of the box, then
 // a fixed-order tree over the workgroup (deterministic; not the reference's
-// summation order).
+// summation order: the same terms, within 1e-4 of the largest gradient plus
+// 1e-4 relative; k_corner_large_ordered keeps the order).
 __global__ __launch_bounds__(kBlock) void k_corner_large(ViewTable vt, const uint32_t* __restrict__ idx, uint32_t T,
                                                          uint32_t V, const float* __restrict__ sxt,
                                                          const float* __restrict__ syt,
@@ -937,6 +938,71 @@ __global__ __launch_bounds__(kBlock) void k_corner_large(ViewTable vt, const uin
                 out[slot_of[q.y * 3 + c]] = make_float4(red[c * 3][0], red[c * 3 + 1][0], 0.0f, red[c * 3 + 2][0]);
         }
         __syncthreads();
+    }
+}
+
+// Boxes over kWaveBox pixels in the reference's order (PDR_FLAG_DETERMINISTIC):
+// one wave per queued box walks its rows in order, 64 pixels at a time; the
+// chunk's contributing pixels put their 9 terms in LDS in pixel order (ballot
+// rank), and lanes 0-8 each add one term component over that list in order,
+// so every component is the left fold of rasterize_backward.hlsl:88-151 bit for bit.
+__global__ __launch_bounds__(kBlock) void k_corner_large_ordered(
+    ViewTable vt, const uint32_t* __restrict__ idx, uint32_t T, uint32_t V, const float* __restrict__ sxt,
+    const float* __restrict__ syt, const uint32_t* __restrict__ cid, const uint32_t* __restrict__ view_flags,
+    const uint32_t* __restrict__ slot_of, float4* __restrict__ corner, const uint4* __restrict__ queue,
+    uint32_t* __restrict__ counters) {
+    __shared__ float terms[kBlock / 64][64 * 9];
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    float* lt = terms[wave];
+    const uint32_t count = counters[0];
+    while (true) {
+        uint32_t e = 0;
+        if (lane == 0) e = atomicAdd(&counters[1], 1u);
+        e = __builtin_amdgcn_readfirstlane(e);
+        if (e >= count) return;  // wave-uniform
+        const uint4 q = queue[e];
+        const ViewDesc& vd = vt.v[q.x];
+        BwdRegion g;
+        const bool ok = setup_backward(vd, idx, q.y, V, view_flags[q.x] != 0u, g);
+        const float alpha = span_alpha(vd, 0), beta = span_beta(vd, 0);
+        const uint32_t* cv = cid + vd.pixel_base;
+        const float2* gr = reinterpret_cast<const float2*>(vd.grad_raster);
+        float acc = 0.0f;  // lane k < 9: term component k
+        for (int32_t y = g.ya; ok && y <= g.yb; ++y) {
+            const float sy = syt[vd.row_base + y];
+            int32_t lo, hi;
+            line_span(g.sp, sy, alpha, beta, g.xa, g.xb, &lo, &hi);
+            for (int32_t x0 = lo; x0 <= hi; x0 += 64) {
+                const int32_t x = x0 + static_cast<int32_t>(lane);
+                float t9[9];
+                bool hit = false;
+                if (x <= hi) {
+                    const size_t pix = static_cast<size_t>(y) * vd.width + static_cast<size_t>(x);
+                    hit = cv[pix] == q.y + 1u && pixel_terms(g.p, sxt[vd.col_base + x], sy, gr[pix * 2], t9);
+                }
+                const unsigned long long bal = __ballot(hit);
+                if (!bal) continue;
+                if (hit) {
+                    const uint32_t pos = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(bal >> 32),
+                                                                   __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(bal), 0u));
+#pragma unroll
+                    for (int k = 0; k < 9; ++k) lt[pos * 9 + k] = t9[k];
+                }
+                wave_lds_sync();
+                const uint32_t cnt = static_cast<uint32_t>(__popcll(bal));
+                if (lane < 9)
+                    for (uint32_t j = 0; j < cnt; ++j) acc += lt[j * 9 + lane];
+                wave_lds_sync();
+            }
+        }
+        float w[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) w[k] = bcf(acc, k);
+        if (lane == 0) {
+            float4* out = corner + static_cast<size_t>(q.x) * T * 3;
+            for (int c = 0; c < 3; ++c)
+                out[slot_of[q.y * 3 + c]] = make_float4(w[c * 3], w[c * 3 + 1], 0.0f, w[c * 3 + 2]);
+        }
     }
 }
 
@@ -1125,7 +1191,7 @@ hipError_t build_vertex_csr(const uint32_t* idx, uint32_t T, uint32_t V, const V
 }
 
 hipError_t corner_gradients(const ViewTable& vt, const uint32_t* idx, uint32_t T, uint32_t V, const RasterScratch& s,
-                            float4* corner, const TuneParams& tp, hipStream_t st) {
+                            float4* corner, const TuneParams& tp, bool ordered, hipStream_t st) {
     hipError_t e = fill_u32(s.counters, 0u, 4, st);
     if (e != hipSuccess) return e;
     const uint32_t bsz = tri_batch(T, vt.n, tp.bwd_waves);
@@ -1133,8 +1199,12 @@ hipError_t corner_gradients(const ViewTable& vt, const uint32_t* idx, uint32_t T
                                                                     s.won, s.slot_of, corner, s.queue, s.counters,
                                                                     bsz, tp.flat_work);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    k_corner_large<<<kLargeBlocks, kBlock, 0, st>>>(vt, idx, T, V, s.sx, s.sy, s.cid, s.view_flags, s.slot_of, corner,
-                                                    s.queue, s.counters);
+    if (ordered)
+        k_corner_large_ordered<<<kLargeBlocks, kBlock, 0, st>>>(vt, idx, T, V, s.sx, s.sy, s.cid, s.view_flags,
+                                                                s.slot_of, corner, s.queue, s.counters);
+    else
+        k_corner_large<<<kLargeBlocks, kBlock, 0, st>>>(vt, idx, T, V, s.sx, s.sy, s.cid, s.view_flags, s.slot_of,
+                                                        corner, s.queue, s.counters);
     return hipGetLastError();
 }
 
