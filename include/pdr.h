@@ -120,6 +120,21 @@ pdr_status pdr_rasterize_forward(pdr_context ctx, const float* pos, uint32_t V, 
                                  const float* viewport4, float* raster_out, float* deriv_out, float* viewport_out4,
                                  void* stream);
 
+/* pdr_rasterize_forward without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments and
+ * results as pdr_rasterize_forward, device memory only.  Its scratch (8 B per
+ * pixel + 16 B per triangle + 4 B per row and column) comes from a fixed
+ * 32 MiB array per GPU set up by pdr_context_create; larger calls fail with
+ * PDR_OVERFLOW.  Indices are not validated by a readback (out-of-range
+ * triangles are skipped, as by the shader) and the launch parameters are the
+ * defaults.  All calls on one GPU share the array: calls that overlap on one
+ * GPU (other streams or contexts) race. */
+pdr_status pdr_rasterize_forward_noalloc(pdr_context ctx, const float* pos, uint32_t V, const uint32_t* idx,
+                                         uint32_t T, uint32_t width, uint32_t height, uint32_t cull_mode,
+                                         int want_derivatives, const float* viewport4, float* raster_out,
+                                         float* deriv_out, float* viewport_out4, void* stream);
+
 /* One view of a batched raster: its own clip positions, size and outputs. */
 typedef struct pdr_raster_view {
     const float* positions; /* [V,4] */

@@ -93,6 +93,7 @@ _SIGNATURES = {
     "pdr_context_kernel_time": (_I, [_P, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
                                      ctypes.POINTER(ctypes.c_uint64)]),
     "pdr_rasterize_forward": (_I, [_P, _P, _U32, _P, _U32, _U32, _U32, _U32, _I, _P, _P, _P, _P, _P]),
+    "pdr_rasterize_forward_noalloc": (_I, [_P, _P, _U32, _P, _U32, _U32, _U32, _U32, _I, _P, _P, _P, _P, _P]),
     "pdr_rasterize_forward_batched": (_I, [_P, ctypes.POINTER(RasterView), _U32, _U32, _P, _U32, _U32, _I, _P]),
     "pdr_rasterize_backward": (_I, [_P, _P, _U32, _P, _U32, _P, _U32, _U32, _P, _P, _P, _P]),
     "pdr_rasterize_backward_batched": (_I, [_P, ctypes.POINTER(RasterView), _U32, _U32, _P, _U32, _P]),

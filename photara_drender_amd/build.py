@@ -23,7 +23,7 @@ BUILD = os.path.join(HERE, "_build")
 LIBPDR = os.path.join(HERE, "libpdr.so")
 
 HIP_SOURCES = ["raster.hip", "interp_texture.hip", "refine.hip", "bake.hip", "mlp.hip", "padding.hip", "pdr_api.cpp",
-               "pdr_refine.cpp", "pdr_bake.cpp", "pdr_mlp.cpp", "pdr_padding.cpp"]
+               "pdr_refine.cpp", "pdr_bake.cpp", "pdr_mlp.cpp", "pdr_padding.cpp", "pdr_noalloc.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("PDR_OFFLOAD_ARCH", "gfx950")
 

@@ -41,6 +41,7 @@ pdr_status pdr_context_create(uint32_t device_index, int enable_validation, pdr_
         (void)hipGetLastError();
         hip_check(hipEventCreateWithFlags(&ctx->handoff, hipEventDisableTiming), "hipEventCreate");
         hip_check(hipHostMalloc(reinterpret_cast<void**>(&ctx->pinned), 4096, hipHostMallocDefault), "hipHostMalloc");
+        pdr::noalloc_scratch_init(ctx->device);  // the device's no-allocation scratch (device current)
         *out = ctx.release();
     });
 }

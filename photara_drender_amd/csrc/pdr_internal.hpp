@@ -138,6 +138,11 @@ hipError_t texture_check_uv(const PixelTable& views, uint32_t tw, uint32_t th, u
 // max index check; flags[0] = 1 when any index >= V
 hipError_t check_indices(const uint32_t* idx, size_t n, uint32_t V, uint32_t* flags, hipStream_t st);
 
+// ---- pdr_noalloc.hip
+// Takes the device address of the no-allocation scratch array on `device`
+// (once per device; the device must be current).
+void noalloc_scratch_init(int device);
+
 // ---- common kernels (raster.hip)
 hipError_t fill_u32(uint32_t* p, uint32_t value, size_t n, hipStream_t st);
 hipError_t fill_u64(unsigned long long* p, unsigned long long value, size_t n, hipStream_t st);
