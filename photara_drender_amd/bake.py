@@ -125,34 +125,34 @@ def bake_features(context, positions, normals, uv, indices, images: Sequence, wo
 
 def pad_texture_atlas(color, valid_mask, padding: int, *, coverage_mask=None, fill_unobserved: bool = False,
                       maximum_fill_distance: int = 0) -> np.ndarray:
-    """Extend valid atlas texels into empty neighbours (python/photara_drender/baking.py:33-80).
+    """python/photara_drender/baking.py:33-80: a padded copy of ``color`` whose
+    valid texels are extended ``padding`` texels into their empty neighbours
+    (``valid_mask`` itself is not widened); with ``fill_unobserved`` the covered
+    texels (``coverage_mask``) that received no sample take the nearest valid
+    colour, up to ``maximum_fill_distance`` texels (0: no limit)."""
+    from ._photara_drender import pad_texture_atlas as native_pad
 
-    Returns a padded copy of ``color``; ``padding`` is the guard width in texels.
-    With ``fill_unobserved`` the covered texels (``coverage_mask``) that never
-    received a sample take the colour of the nearest valid texel."""
-    from ._photara_drender import pad_texture_atlas as _pad
-
-    color = np.asarray(color, dtype=np.float32)
+    rgba = np.ascontiguousarray(color, dtype=np.float32)
     valid = np.asarray(valid_mask, dtype=bool)
-    if color.ndim != 3 or color.shape[2] not in (3, 4):
-        raise ValueError("color must have shape [height, width, 3 or 4]")
-    if valid.shape != color.shape[:2]:
-        raise ValueError("valid_mask must match the color dimensions")
-    if padding < 0:
-        raise ValueError("padding must be non-negative")
-    if (padding == 0 and not fill_unobserved) or not valid.any():
-        return np.ascontiguousarray(color.copy())
-    coverage = None
-    if coverage_mask is not None:
-        coverage = np.ascontiguousarray(np.asarray(coverage_mask, dtype=np.float32))
-        if coverage.shape != valid.shape:
-            raise ValueError("coverage_mask must match the color dimensions")
+    # the reference's checks, in its order (each message is part of the interface)
+    checks = (
+        (lambda: rgba.ndim != 3 or rgba.shape[2] not in (3, 4), "color must have shape [height, width, 3 or 4]"),
+        (lambda: valid.shape != rgba.shape[:2], "valid_mask must match the color dimensions"),
+        (lambda: padding < 0, "padding must be non-negative"),
+    )
+    for failed, message in checks:
+        if failed():
+            raise ValueError(message)
+    if not valid.any() or (padding == 0 and not fill_unobserved):
+        return rgba.copy()  # nothing to extend from, or nothing asked for
+    coverage = None if coverage_mask is None else np.ascontiguousarray(coverage_mask, dtype=np.float32)
+    if coverage is not None and coverage.shape != valid.shape:
+        raise ValueError("coverage_mask must match the color dimensions")
     if maximum_fill_distance < 0:
         raise ValueError("maximum_fill_distance must be non-negative")
-    padded, _filled, _gutter, _unobserved, _remaining = _pad(np.ascontiguousarray(color), valid.astype(np.float32),
-                                                             int(padding), coverage, bool(fill_unobserved),
-                                                             int(maximum_fill_distance))
-    return padded
+    result = native_pad(rgba, valid.astype(np.float32), int(padding), coverage, bool(fill_unobserved),
+                        int(maximum_fill_distance))
+    return result[0]  # (colour, filled mask, gutter, unobserved, remaining): the colour
 
 
 def pad_atlas(context, color, valid_mask, margin: int = 8, *, coverage_mask=None, filled_mask=None,
