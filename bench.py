@@ -40,6 +40,7 @@ TRAFFIC_PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "prof
 COPY_BYTES = 1 << 30  # calibration copy of the traffic probe
 # library timing name -> HIP kernel whose counters it covers
 KERNEL_OF = {"interp_bwd": "k_interp_bwd", "interp_fwd": "k_interp_fwd", "raster_resolve": "k_raster_resolve",
+             "raster_tiled": "k_tile_raster",
              "corner_grads": "k_corner_small", "bwd_prepass": "k_bwd_prepass", "vertex_reduce": "k_vertex_reduce",
              "raster_candidates": "k_raster_small"}
 
@@ -157,6 +158,8 @@ def algorithmic_bytes(V, T, A, pixels):
         # forward raster: mesh per view in, raster + derivatives out (keys are scratch)
         "raster_candidates": n * mesh,
         "raster_resolve": n * mesh + 32 * P,
+        # tiled forward (bins, LDS keys, resolve in one pass): mesh per view in, raster + derivatives out
+        "raster_tiled": n * mesh + 32 * P,
         # backward raster: raster id + grad_raster.xy per pixel, mesh; per-vertex gradients out
         "bwd_prepass": 32 * P,
         "corner_grads": n * mesh + 32 * P,
@@ -263,7 +266,8 @@ def chain_benchmark(steps=50, warmup=5, cpu=True):
     torch.cuda.synchronize()
     wall_ms = (time.perf_counter() - t0) / steps * 1e3
     kernels = {}
-    for name in ("raster_candidates", "raster_resolve", "interp_fwd", "texture_fwd", "texture_bwd", "interp_bwd",
+    for name in ("raster_tiled", "raster_candidates", "raster_resolve", "interp_fwd", "texture_fwd", "texture_bwd",
+                 "interp_bwd",
                  "vertex_csr", "bwd_prepass", "corner_grads", "vertex_reduce"):
         ms, launches = ctx.kernel_time(name)
         if launches:
@@ -437,6 +441,8 @@ def main():
     ap.add_argument("--cpu-views", type=int, default=16)
     ap.add_argument("--no-refine", action="store_true", help="skip the atlas-refinement workload")
     ap.add_argument("--no-chain", action="store_true", help="skip the single-view textured chain workload")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="run the raster with its default launch parameters (no per-workload tuning)")
     ap.add_argument("--trace-summary", metavar="TRACE_DIR",
                     help="print the per-kernel table of a rocprofv3 --kernel-trace --output-format csv folder")
     ap.add_argument("--refine-only", action="store_true",
@@ -510,7 +516,7 @@ def main():
         iv.grad_raster = grad_raster[o].data_ptr()
 
     # device-input validation off (inputs are known-good); raster launch parameters autotuned per workload class
-    ctx = _native.Context(0, flags=_native.PDR_FLAG_AUTOTUNE)
+    ctx = _native.Context(0, flags=0 if args.no_autotune else _native.PDR_FLAG_AUTOTUNE)
     lib = _native.lib()
     stream = torch.cuda.current_stream().cuda_stream
 
@@ -548,7 +554,8 @@ def main():
     elapsed = time.perf_counter() - t0
     ms_per_step = elapsed / args.steps * 1e3
     kernels = {}
-    for name in ("raster_candidates", "raster_resolve", "interp_fwd", "interp_bwd", "vertex_csr", "bwd_prepass",
+    for name in ("raster_tiled", "raster_candidates", "raster_resolve", "interp_fwd", "interp_bwd", "vertex_csr",
+                 "bwd_prepass",
                  "corner_grads", "vertex_reduce"):
         ms, launches = ctx.kernel_time(name)
         if launches:

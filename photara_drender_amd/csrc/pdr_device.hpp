@@ -15,6 +15,7 @@ namespace pdr {
 constexpr uint32_t kTile = 16;         // src/rasterizer.cpp:19 RASTER_TILE_SIZE
 constexpr uint32_t kTilePixels = kTile * kTile;
 constexpr uint32_t kMaxViews = 32;     // views per batched launch (kernel-argument table)
+constexpr uint32_t kRasterTile = 32;   // tiled forward: pixels per raster tile side (LDS depth keys)
 constexpr uint32_t kGradTile = 64;     // pixels per side of the interpolation-backward tiles
 
 __host__ __device__ __forceinline__ int32_t f2i_sat(float f) {

@@ -272,8 +272,12 @@ inline pdr::ViewDesc make_view(const float* pos, uint32_t W, uint32_t H, const f
 }
 
 inline void finalize_table(pdr::ViewTable& t) {
-    uint32_t tiles = 0, pixels = 0, cols = 0, rows = 0;
+    uint32_t tiles = 0, pixels = 0, cols = 0, rows = 0, rtiles = 0;
     for (uint32_t k = 0; k < t.n; ++k) {
+        t.v[k].rtiles_x = (t.v[k].width + pdr::kRasterTile - 1) / pdr::kRasterTile;
+        t.v[k].rtiles_y = (t.v[k].height + pdr::kRasterTile - 1) / pdr::kRasterTile;
+        t.v[k].rtile_base = rtiles;
+        rtiles += t.v[k].rtiles_x * t.v[k].rtiles_y;
         t.v[k].tile_base = tiles;
         t.v[k].pixel_base = pixels;
         t.v[k].col_base = cols;
@@ -288,6 +292,7 @@ inline void finalize_table(pdr::ViewTable& t) {
     }
     t.total_tiles = tiles;
     t.total_pixels = pixels;
+    t.total_rtiles = rtiles;
 }
 
 inline pdr::PixelTable pixel_table_one(const float* raster, const float* in0, const float* in1, float* out0, float* out1,
@@ -399,17 +404,47 @@ pdr::TuneParams tuned_params(pdr_context ctx, const std::string& key, const std:
 }
 
 // Candidates; the first is the defaults.
-inline std::vector<pdr::TuneParams> forward_candidates() {
-    std::vector<pdr::TuneParams> out{pdr::TuneParams{}};
-    for (float work : {4096.0f, 16384.0f, 65536.0f})
-        for (uint32_t waves : {2048u, 4096u, 8192u}) {
-            if (work == out[0].wave_work && waves == out[0].fwd_waves) continue;
+// Forward candidates.  With outputs (tiled_choice): the tiled path at bin
+// depths of 2, 4 and 8 raster tiles per triangle, and the triangle-parallel
+// path with its walk limit and wave count; the keys-only pass (the bake's
+// shadow maps) has the triangle-parallel candidates alone.
+inline std::vector<pdr::TuneParams> forward_candidates(bool tiled_choice = false) {
+    std::vector<pdr::TuneParams> out;
+    for (float work : {16384.0f, 4096.0f, 65536.0f})
+        for (uint32_t waves : {4096u, 2048u, 8192u}) {
             pdr::TuneParams p;
+            p.tiled = 0;
             p.wave_work = work;
             p.fwd_waves = waves;
             out.push_back(p);
         }
+    if (tiled_choice)
+        for (uint32_t depth : {4u, 2u, 8u}) {
+            pdr::TuneParams p;
+            p.tiled = 1;
+            p.bin_tiles = depth;
+            out.push_back(p);
+        }
     return out;
+}
+
+// The bins of the tiled forward (grow-only context scratch): capacity
+// bin_tiles * views * T triangle entries.
+inline void tiled_scratch(pdr_context ctx, const pdr::ViewTable& vt, uint32_t T, uint32_t bin_tiles,
+                          pdr::RasterScratch& s) {
+    const size_t tiles = static_cast<size_t>(vt.total_rtiles) + 1;
+    s.bin_counts = ctx->get_t<uint32_t>("bin_counts", tiles);
+    s.bin_offsets = ctx->get_t<uint32_t>("bin_offsets", tiles);
+    s.bin_seeded = reinterpret_cast<uint8_t*>(ctx->get_t<uint32_t>("bin_seeded", (tiles + 3) / 4));
+    const uint64_t cap = static_cast<uint64_t>(bin_tiles) * vt.n * T;
+    if (cap > 0xFFFFFFFFull) raise(PDR_OVERFLOW, "tiled rasterization supports at most 2^32 binned triangles per call");
+    s.bin_list = ctx->get_t<uint32_t>("bin_list", static_cast<size_t>(cap ? cap : 1));
+    s.medium_list = ctx->get_t<uint32_t>("medium_list", static_cast<size_t>(vt.n) * T);
+    s.medium_count = ctx->get_t<uint32_t>("medium_count", pdr::kMaxViews);
+    s.tri_records = ctx->get("tri_records", static_cast<size_t>(vt.n) * T * pdr::tri_record_bytes());
+    s.tri_spans = ctx->get_t<unsigned long long>("tri_spans", static_cast<size_t>(vt.n) * T);
+    s.bin_temp_bytes = pdr::bin_scan_temp_bytes(static_cast<uint32_t>(tiles));
+    s.bin_temp = ctx->get("bin_temp", s.bin_temp_bytes);
 }
 
 inline std::vector<pdr::TuneParams> backward_candidates() {
@@ -425,21 +460,38 @@ inline std::vector<pdr::TuneParams> backward_candidates() {
     return out;
 }
 
-// Triangle-parallel raster forward of one table of views (device pointers), on stream st.
+// Raster forward of one table of views (device pointers), on stream st: the
+// triangle-parallel path (global keys, then a resolve pass) or the tiled one
+// (bins, LDS keys per raster tile), as the tuned parameters of the workload
+// class say (the triangle-parallel path by default).
 inline void run_raster_forward(pdr_context ctx, const pdr::ViewTable& vt, const uint32_t* idx, uint32_t T, uint32_t V,
                                uint32_t cull, int want_deriv, hipStream_t st) {
     if (!vt.total_pixels) return;
-    const pdr::RasterScratch s = raster_scratch(ctx, vt, T, st, true);
+    pdr::RasterScratch s = raster_scratch(ctx, vt, T, st, true);
     ctx->keys_clean = false;
-    const pdr::TuneParams tp =
-        tuned_params(ctx, tune_key("forward", vt, T), forward_candidates(), st, [&](const pdr::TuneParams& c) {
-            hip_check(pdr::raster_candidates(vt, idx, T, V, cull, s, c, st), "raster_candidates");
-            hip_check(pdr::raster_resolve(vt, idx, want_deriv, s, st), "raster_resolve");  // resets the keys
-        });
-    ctx->timed("raster_candidates", st,
-               [&] { hip_check(pdr::raster_candidates(vt, idx, T, V, cull, s, tp, st), "raster_candidates"); });
-    ctx->timed("raster_resolve", st,
-               [&] { hip_check(pdr::raster_resolve(vt, idx, want_deriv, s, st), "raster_resolve"); });
+    auto run = [&](const pdr::TuneParams& c, bool timed) {
+        if (c.tiled) {
+            tiled_scratch(ctx, vt, T, c.bin_tiles, s);
+            auto f = [&] { hip_check(pdr::raster_tiled(vt, idx, T, V, cull, want_deriv, s, c, st), "raster_tiled"); };
+            if (timed)
+                ctx->timed("raster_tiled", st, f);
+            else
+                f();
+            return;
+        }
+        auto cand = [&] { hip_check(pdr::raster_candidates(vt, idx, T, V, cull, s, c, st), "raster_candidates"); };
+        auto res = [&] { hip_check(pdr::raster_resolve(vt, idx, want_deriv, s, st), "raster_resolve"); };
+        if (timed) {
+            ctx->timed("raster_candidates", st, cand);
+            ctx->timed("raster_resolve", st, res);  // resets the keys
+        } else {
+            cand();
+            res();
+        }
+    };
+    const pdr::TuneParams tp = tuned_params(ctx, tune_key("forward_out", vt, T), forward_candidates(true), st,
+                                            [&](const pdr::TuneParams& c) { run(c, false); });
+    run(tp, true);
     ctx->keys_clean = true;
 }
 

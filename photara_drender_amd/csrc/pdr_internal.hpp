@@ -23,6 +23,8 @@ struct ViewDesc {
     uint32_t pixel_base;      // first global pixel of this view
     uint32_t col_base;        // first entry of this view in the column sample table
     uint32_t row_base;        // first entry of this view in the row sample table
+    uint32_t rtiles_x, rtiles_y;  // kRasterTile x kRasterTile tiles of the tiled forward
+    uint32_t rtile_base;          // first global raster tile of this view
 };
 
 struct ViewTable {
@@ -30,6 +32,7 @@ struct ViewTable {
     uint32_t n;
     uint32_t total_tiles;
     uint32_t total_pixels;
+    uint32_t total_rtiles;
 };
 
 // Generic per-view pixel table for interpolation / texture kernels.
@@ -56,6 +59,17 @@ struct PixelTable {
 // Scratch of the triangle-parallel raster (owned by the context).
 struct RasterScratch {
     unsigned long long* keys = nullptr;  // [total_pixels] (depth, id) keys; all ~0 between calls
+    // tiled forward (raster tiles of kRasterTile^2 pixels)
+    uint32_t* bin_counts = nullptr;      // [total_rtiles + 1] triangles binned per tile (0 between calls)
+    uint32_t* bin_offsets = nullptr;     // [total_rtiles + 1] exclusive scan of bin_counts
+    uint32_t* bin_list = nullptr;        // [bin_capacity] triangle ids, grouped by tile
+    uint8_t* bin_seeded = nullptr;       // [total_rtiles] 1 where an unbinned triangle left keys in `keys`
+    uint32_t* medium_list = nullptr;     // [views * T] per view: triangles too large to bin
+    uint32_t* medium_count = nullptr;    // [kMaxViews]
+    void* tri_records = nullptr;         // [views * T] tri_record_bytes() each: binned triangles' setup
+    unsigned long long* tri_spans = nullptr;  // [views * T] binned triangles' tile ranges (0: not binned)
+    void* bin_temp = nullptr;            // scan temporary
+    size_t bin_temp_bytes = 0;
     float* sx = nullptr;                 // [sum of widths] pixel-centre NDC x per column
     float* sy = nullptr;                 // [sum of heights] pixel-centre NDC y per row
     uint4* queue = nullptr;              // [views * T] (view, triangle, slabs, 0) of large triangles
@@ -86,12 +100,22 @@ hipError_t sample_tables(const ViewTable& views, float* sx, float* sy, hipStream
 struct TuneParams {
     float wave_work = 16384.0f;  // forward: candidate estimate walked by one wave (else queued as slabs)
     uint32_t fwd_waves = 4096;   // forward: target wave count (sets triangles per wave, 8..64)
+    uint32_t tiled = 0;          // forward with outputs: 1 = binned into raster tiles (LDS keys), 0 = global keys
+    uint32_t bin_tiles = 4;      // tiled forward: most raster tiles a binned triangle may touch (else slabs)
     float flat_work = 512.0f;    // backward: region estimate walked in the concatenated row list
     uint32_t bwd_waves = 4096;   // backward: target wave count
 };
 
 hipError_t raster_candidates(const ViewTable& views, const uint32_t* idx, uint32_t T, uint32_t V, uint32_t cull,
                              const RasterScratch& s, const TuneParams& tp, hipStream_t st);
+// Tiled forward with outputs (raster, derivatives): bins the triangles into
+// raster tiles, walks each tile's triangles with LDS keys and resolves the
+// tile; triangles touching more than tp.bin_tiles tiles take the slab path
+// through `keys` first.  Capacity of bin_list: tp.bin_tiles * views * T.
+size_t bin_scan_temp_bytes(uint32_t n);
+size_t tri_record_bytes();
+hipError_t raster_tiled(const ViewTable& views, const uint32_t* idx, uint32_t T, uint32_t V, uint32_t cull,
+                        int want_deriv, const RasterScratch& s, const TuneParams& tp, hipStream_t st);
 // k_raster_resolve: raster / derivatives from the keys; resets the keys
 hipError_t raster_resolve(const ViewTable& views, const uint32_t* idx, int want_deriv, const RasterScratch& s,
                           hipStream_t st);

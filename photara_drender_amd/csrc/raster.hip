@@ -252,21 +252,110 @@ struct RecTri {  // what test_pixel reads
     float s0x, s0y, s1x, s1y, s2x, s2y, sgn;
 };
 
-// One batch of up to 64 triangles per wave.  Each lane sets one triangle up; the lines
-// (rows, or columns for triangles taller than wide) of the batch's triangles
-// are concatenated and dealt 64 at a time to the lanes, each lane computing
-// its line's span; the spans of those 64 lines are concatenated in turn and
-// dealt 64 pixels at a time.  Every lane tests one candidate pixel per step
-// whatever the triangles' sizes.  Triangles above wave_work candidates are
-// queued in 16-row slabs for k_raster_large.
+// Stores the record of a set-up triangle (lane i of the block) whose lines
+// [line_lo, ...] carry spans inside [span_lo, span_hi].
+__device__ __forceinline__ void put_record(FwdRecs& R, uint32_t i, const ScreenTri& r, const LineSpans& sp, uint32_t t,
+                                           int32_t span_lo, int32_t span_hi, int32_t line_lo) {
+    R.p0[i] = r.p0;
+    R.p1[i] = r.p1;
+    R.p2[i] = r.p2;
+    R.sa[i] = make_float4(r.s0x, r.s0y, r.s1x, r.s1y);
+    R.sb[i] = make_float4(r.s2x, r.s2y, r.sgn, __uint_as_float(t));
+    R.su[i] = make_float4(sp.u[0], sp.u[1], sp.u[2], __uint_as_float(sp.lower));
+    R.sv[i] = make_float4(sp.v[0], sp.v[1], sp.v[2], __uint_as_float(sp.upper));
+    R.ln[i] = make_int4(span_lo, span_hi, line_lo, static_cast<int32_t>(sp.cols));
+}
+
+// The walk of one wave's batch of up to 64 triangles (records w0 .. w0 + 63 of
+// R; lane `lane` owns `lines` lines of its triangle): the lines of the batch
+// are concatenated and dealt 64 at a time to the lanes, each lane computing its
+// line's span; the spans of those 64 lines are concatenated in turn and dealt
+// 64 pixels at a time, so every lane tests one candidate pixel per step
+// whatever the triangles' sizes.  sink(x, y, key) receives every covered
+// candidate's (depth, id) key.
+template <class Sink>
+__device__ __forceinline__ void walk_batch(const ViewDesc& vd, FwdRecs& R, uint32_t w0, uint32_t lane, int32_t lines,
+                                           int32_t* slot, const float ax[2], const float bx[2], Sink&& sink) {
+    const int32_t incl_l = wave_incl_scan(lines);
+    const int32_t nlines = __builtin_amdgcn_readlane(incl_l, 63);
+    if (lines > 0) R.ln[w0 + lane].z -= incl_l - lines;  // line coordinate = base + list position
+    wave_lds_sync();
+    for (int32_t lc = 0; lc < nlines; lc += 64) {
+        const int32_t f = lc + static_cast<int32_t>(lane);
+        const int32_t o = chunk_owner(incl_l, lines, lc, lane, slot);
+        int32_t lo = 0, hi = -1, line = 0, cols = 0;
+        float sl = 0.0f;
+        if (f < nlines) {
+            const uint32_t i = w0 + static_cast<uint32_t>(o);
+            const int4 ln = R.ln[i];
+            const float4 su = R.su[i], sv = R.sv[i];
+            LineSpans sp;
+            sp.u[0] = su.x;
+            sp.u[1] = su.y;
+            sp.u[2] = su.z;
+            sp.lower = __float_as_uint(su.w);
+            sp.v[0] = sv.x;
+            sp.v[1] = sv.y;
+            sp.v[2] = sv.z;
+            sp.upper = __float_as_uint(sv.w);
+            sp.cols = static_cast<uint32_t>(ln.w);
+            cols = ln.w;
+            line = ln.z + f;
+            sl = cols ? sample_x(vd, line) : sample_y(vd, line);
+            line_span(sp, sl, ax[cols], bx[cols], ln.x, ln.y, &lo, &hi);
+        }
+        const int32_t len = hi >= lo ? hi - lo + 1 : 0;
+        const int32_t incl_p = wave_incl_scan(len);
+        const int32_t npix = __builtin_amdgcn_readlane(incl_p, 63);
+        const int32_t aoff = lo - (incl_p - len);
+        for (int32_t pc = 0; pc < npix; pc += 64) {
+            const int32_t m = pc + static_cast<int32_t>(lane);
+            const int32_t k = chunk_owner(incl_p, len, pc, lane, slot);
+            const int32_t a = __shfl(aoff, k, 64) + m;
+            const int32_t ln = __shfl(line, k, 64);
+            const int32_t oo = __shfl(o, k, 64);
+            const int32_t cc = __shfl(cols, k, 64);
+            const float sln = __shfl(sl, k, 64);
+            if (m < npix) {
+                const uint32_t i = w0 + static_cast<uint32_t>(oo);
+                RecTri q;
+                q.p0 = R.p0[i];
+                q.p1 = R.p1[i];
+                q.p2 = R.p2[i];
+                const float4 sa = R.sa[i], sb = R.sb[i];
+                q.s0x = sa.x;
+                q.s0y = sa.y;
+                q.s1x = sa.z;
+                q.s1y = sa.w;
+                q.s2x = sb.x;
+                q.s2y = sb.y;
+                q.sgn = sb.z;
+                const int32_t x = cc ? ln : a, y = cc ? a : ln;
+                const float sx = cc ? sln : sample_x(vd, x), sy = cc ? sample_y(vd, y) : sln;
+                const unsigned long long key = test_pixel(q, __float_as_uint(sb.w), sx, sy);
+                if (key != ~0ull) sink(x, y, key);
+            }
+        }
+    }
+    wave_lds_sync();
+}
+
+// One batch of up to 64 triangles per wave (walk_batch), with the keys kept
+// by a global atomicMin per pixel.  Triangles above wave_work candidates are
+// queued in 16-row slabs for k_raster_large.  With a list, the view's
+// triangles are list[view * T + i], i < list_count[view] (else every triangle).
 __global__ __launch_bounds__(kBlock) void k_raster_small(ViewTable vt, const uint32_t* __restrict__ idx, uint32_t T,
                                                          uint32_t V, uint32_t cull, const float* __restrict__ sxt,
                                                          const float* __restrict__ syt,
                                                          unsigned long long* __restrict__ keys,
                                                          uint4* __restrict__ queue, uint32_t* __restrict__ counters,
-                                                         uint32_t bsz, float wave_work) {
+                                                         uint32_t bsz, float wave_work,
+                                                         const uint32_t* __restrict__ list,
+                                                         const uint32_t* __restrict__ list_count) {
     __shared__ FwdRecs R;
     const uint32_t view = blockIdx.y;
+    const uint32_t count = list ? list_count[view] : T;
+    const uint32_t* vlist = list ? list + static_cast<size_t>(view) * T : nullptr;
     const ViewDesc& vd = vt.v[view];
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
     const uint32_t w0 = wave * 64;
@@ -279,14 +368,16 @@ __global__ __launch_bounds__(kBlock) void k_raster_small(ViewTable vt, const uin
     // loads of the next batch are in flight while this one is rasterised
     float4 c0, c1, c2;
     bool cvalid = false;
+    uint32_t ctri = 0;
     auto fetch = [&](uint32_t b) {
-        const uint32_t tt = b + lane;
-        cvalid = lane < bsz && b < T && tt < T && load_corners(vd, idx, tt, V, c0, c1, c2);
+        const uint32_t i = b + lane;
+        ctri = vlist && lane < bsz && b < count && i < count ? vlist[i] : i;
+        cvalid = lane < bsz && b < count && i < count && load_corners(vd, idx, ctri, V, c0, c1, c2);
     };
     uint32_t batch = (blockIdx.x * (kBlock / 64) + wave) * bsz;
     fetch(batch);
-    for (; batch < T; batch += stride) {
-        const uint32_t t = batch + lane;
+    for (; batch < count; batch += stride) {
+        const uint32_t t = ctri;
         const float4 q0 = c0, q1 = c1, q2 = c2;
         const bool have = cvalid;
         fetch(batch + stride);
@@ -300,80 +391,13 @@ __global__ __launch_bounds__(kBlock) void k_raster_small(ViewTable vt, const uin
                 queue[atomicAdd(&counters[0], 1u)] = make_uint4(view, t, slabs, 0u);
             } else if (ok) {
                 lines = sp.cols ? r.x1 - r.x0 + 1 : r.y1 - r.y0 + 1;
-                const uint32_t i = w0 + lane;
-                R.p0[i] = r.p0;
-                R.p1[i] = r.p1;
-                R.p2[i] = r.p2;
-                R.sa[i] = make_float4(r.s0x, r.s0y, r.s1x, r.s1y);
-                R.sb[i] = make_float4(r.s2x, r.s2y, r.sgn, __uint_as_float(t));
-                R.su[i] = make_float4(sp.u[0], sp.u[1], sp.u[2], __uint_as_float(sp.lower));
-                R.sv[i] = make_float4(sp.v[0], sp.v[1], sp.v[2], __uint_as_float(sp.upper));
-                R.ln[i] = make_int4(sp.cols ? r.y0 : r.x0, sp.cols ? r.y1 : r.x1, sp.cols ? r.x0 : r.y0,
-                                    static_cast<int32_t>(sp.cols));
+                put_record(R, w0 + lane, r, sp, t, sp.cols ? r.y0 : r.x0, sp.cols ? r.y1 : r.x1,
+                           sp.cols ? r.x0 : r.y0);
             }
         }
-        const int32_t incl_l = wave_incl_scan(lines);
-        const int32_t nlines = __builtin_amdgcn_readlane(incl_l, 63);
-        if (lines > 0) R.ln[w0 + lane].z -= incl_l - lines;  // line coordinate = base + list position
-        wave_lds_sync();
-        for (int32_t lc = 0; lc < nlines; lc += 64) {
-            const int32_t f = lc + static_cast<int32_t>(lane);
-            const int32_t o = chunk_owner(incl_l, lines, lc, lane, slot);
-            int32_t lo = 0, hi = -1, line = 0, cols = 0;
-            float sl = 0.0f;
-            if (f < nlines) {
-                const uint32_t i = w0 + static_cast<uint32_t>(o);
-                const int4 ln = R.ln[i];
-                const float4 su = R.su[i], sv = R.sv[i];
-                LineSpans sp;
-                sp.u[0] = su.x;
-                sp.u[1] = su.y;
-                sp.u[2] = su.z;
-                sp.lower = __float_as_uint(su.w);
-                sp.v[0] = sv.x;
-                sp.v[1] = sv.y;
-                sp.v[2] = sv.z;
-                sp.upper = __float_as_uint(sv.w);
-                sp.cols = static_cast<uint32_t>(ln.w);
-                cols = ln.w;
-                line = ln.z + f;
-                sl = cols ? sample_x(vd, line) : sample_y(vd, line);
-                line_span(sp, sl, ax[cols], bx[cols], ln.x, ln.y, &lo, &hi);
-            }
-            const int32_t len = hi >= lo ? hi - lo + 1 : 0;
-            const int32_t incl_p = wave_incl_scan(len);
-            const int32_t npix = __builtin_amdgcn_readlane(incl_p, 63);
-            const int32_t aoff = lo - (incl_p - len);
-            for (int32_t pc = 0; pc < npix; pc += 64) {
-                const int32_t m = pc + static_cast<int32_t>(lane);
-                const int32_t k = chunk_owner(incl_p, len, pc, lane, slot);
-                const int32_t a = __shfl(aoff, k, 64) + m;
-                const int32_t ln = __shfl(line, k, 64);
-                const int32_t oo = __shfl(o, k, 64);
-                const int32_t cc = __shfl(cols, k, 64);
-                const float sln = __shfl(sl, k, 64);
-                if (m < npix) {
-                    const uint32_t i = w0 + static_cast<uint32_t>(oo);
-                    RecTri q;
-                    q.p0 = R.p0[i];
-                    q.p1 = R.p1[i];
-                    q.p2 = R.p2[i];
-                    const float4 sa = R.sa[i], sb = R.sb[i];
-                    q.s0x = sa.x;
-                    q.s0y = sa.y;
-                    q.s1x = sa.z;
-                    q.s1y = sa.w;
-                    q.s2x = sb.x;
-                    q.s2y = sb.y;
-                    q.sgn = sb.z;
-                    const int32_t x = cc ? ln : a, y = cc ? a : ln;
-                    const float sx = cc ? sln : sample_x(vd, x), sy = cc ? sample_y(vd, y) : sln;
-                    const unsigned long long key = test_pixel(q, __float_as_uint(sb.w), sx, sy);
-                    if (key != ~0ull) key_min(&kbase[static_cast<size_t>(y) * vd.width + x], key);
-                }
-            }
-        }
-        wave_lds_sync();
+        walk_batch(vd, R, w0, lane, lines, slot, ax, bx, [&](int32_t x, int32_t y, unsigned long long key) {
+            key_min(&kbase[static_cast<size_t>(y) * vd.width + x], key);
+        });
     }
 }
 
@@ -459,24 +483,15 @@ __global__ __launch_bounds__(kBlock) void k_raster_large(ViewTable vt, const uin
     }
 }
 
-// Winner's outputs (rasterize_forward.hlsl:115-162, 165-174); resets the key.
-__global__ __launch_bounds__(kBlock) void k_raster_resolve(ViewTable vt, const uint32_t* __restrict__ idx,
-                                                           int want_deriv, const float* __restrict__ sxt,
-                                                           const float* __restrict__ syt,
-                                                           unsigned long long* __restrict__ keys) {
-    const uint32_t gp = blockIdx.x * kBlock + threadIdx.x;
-    if (gp >= vt.total_pixels) return;
-    bool split;
-    const uint32_t kv = wave_view(vt, gp, split);
-    auto body = [&](const ViewDesc& vd) {
-        const uint32_t p = gp - vd.pixel_base;
-        const unsigned long long key = keys[gp];
-        keys[gp] = ~0ull;
+// Winner's outputs of pixel p with sample (sx, sy) from its key
+// (rasterize_forward.hlsl:115-162, 165-174): raster and, with want_deriv and a
+// derivative output, the barycentric derivatives (zeros where key is ~0).
+__device__ __forceinline__ void resolve_key(const ViewDesc& vd, const uint32_t* __restrict__ idx, int want_deriv,
+                                            unsigned long long key, uint32_t p, float sx, float sy) {
         float4 r = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         float4 d = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         if (key != ~0ull) {
             const uint32_t t = static_cast<uint32_t>(key & 0xFFFFFFFFull) - 1u;
-            const float sx = sxt[vd.col_base + p % vd.width], sy = syt[vd.row_base + p / vd.width];
             const float4* P = reinterpret_cast<const float4*>(vd.pos);
             const float4 p0 = P[idx[t * 3 + 0]], p1 = P[idx[t * 3 + 1]], p2 = P[idx[t * 3 + 2]];
             const float q0x = p0.x - sx * p0.w, q0y = p0.y - sy * p0.w;
@@ -508,11 +523,297 @@ __global__ __launch_bounds__(kBlock) void k_raster_resolve(ViewTable vt, const u
         }
         reinterpret_cast<float4*>(vd.raster)[p] = r;
         if (vd.deriv) reinterpret_cast<float4*>(vd.deriv)[p] = d;
+}
+
+// Winner's outputs of every pixel from the global keys; resets the keys.
+__global__ __launch_bounds__(kBlock) void k_raster_resolve(ViewTable vt, const uint32_t* __restrict__ idx,
+                                                           int want_deriv, const float* __restrict__ sxt,
+                                                           const float* __restrict__ syt,
+                                                           unsigned long long* __restrict__ keys) {
+    const uint32_t gp = blockIdx.x * kBlock + threadIdx.x;
+    if (gp >= vt.total_pixels) return;
+    bool split;
+    const uint32_t kv = wave_view(vt, gp, split);
+    auto body = [&](const ViewDesc& vd) {
+        const uint32_t p = gp - vd.pixel_base;
+        const unsigned long long key = keys[gp];
+        keys[gp] = ~0ull;
+        resolve_key(vd, idx, want_deriv, key, p, sxt[vd.col_base + p % vd.width], syt[vd.row_base + p / vd.width]);
     };
     if (!split)
         body(vt.v[kv]);
     else
         body(vt.v[find_view(vt, gp)]);
+}
+
+// ------------------------------------------------------------ tiled forward
+//
+// k_bin_count    per (view, triangle): the forward setup; a triangle whose
+//                candidate rectangle touches at most bin_tiles raster tiles is
+//                counted in each of them, a larger one is queued in 16-row
+//                slabs for k_raster_large (global keys) and its tiles marked
+//                as seeded;
+// (scan)         bin offsets = exclusive sum of the counts;
+// k_bin_fill     the same setup, each binned triangle's id into its tiles'
+//                lists (counting the counts back down to zero);
+// k_raster_large the slab triangles' keys (global atomicMin);
+// k_tile_raster  one workgroup per raster tile: 1,024 LDS keys (seeded from
+//                the global keys where slab triangles reach the tile), the
+//                tile's triangles walked as the triangle-parallel path walks
+//                them but with LDS atomicMin and every line clamped to the
+//                tile, then every pixel of the tile resolved from LDS.
+// The candidate set of every triangle is the one of the triangle-parallel
+// path, split between tiles, and the (depth, id) minimum does not depend on
+// order: raster and derivatives are the same bit for bit.
+constexpr int32_t kRT = static_cast<int32_t>(kRasterTile);
+constexpr uint32_t kRTPix = kRasterTile * kRasterTile;
+
+__device__ __forceinline__ uint32_t rtile_count(const ScreenTri& r) {
+    return static_cast<uint32_t>(r.x1 / kRT - r.x0 / kRT + 1) * static_cast<uint32_t>(r.y1 / kRT - r.y0 / kRT + 1);
+}
+
+// Block-level aggregation of per-tile increments: the block's entries (tile
+// keys) are counted in an LDS hash table (open addressing, kBinSlots slots);
+// each slot then takes one global atomic.  Neighbouring triangles share tiles,
+// and same-address global atomics serialise.  An entry that finds no free slot
+// (more distinct tiles in a block than the table holds, which needs very large
+// triangles) goes to global memory on its own.
+constexpr uint32_t kBinSlots = 1024;
+struct BinHash {
+    uint32_t key[kBinSlots];
+    uint32_t count[kBinSlots];
+    uint32_t base[kBinSlots];
+};
+
+__device__ __forceinline__ void bin_hash_clear(BinHash& h) {
+    for (uint32_t i = threadIdx.x; i < kBinSlots; i += kBlock) {
+        h.key[i] = ~0u;
+        h.count[i] = 0u;
+    }
+}
+
+// Inserts tile `key`: returns its slot (or kBinSlots when the table is full)
+// and the entry's rank among the block's entries of that tile.
+__device__ __forceinline__ uint32_t bin_hash_add(BinHash& h, uint32_t key, uint32_t* rank) {
+    uint32_t slot = (key * 2654435761u) >> 22;  // top 10 bits: kBinSlots = 1024
+    for (uint32_t probe = 0; probe < kBinSlots; ++probe, slot = (slot + 1u) & (kBinSlots - 1u)) {
+        const uint32_t prev = atomicCAS(&h.key[slot], ~0u, key);
+        if (prev == ~0u || prev == key) {
+            *rank = atomicAdd(&h.count[slot], 1u);
+            return slot;
+        }
+    }
+    return kBinSlots;
+}
+
+
+constexpr uint32_t kMaxBinTiles = 8;  // largest bin depth (tiles per binned triangle)
+
+// Raster tile k (row-major over the triangle's candidate rectangle) of r.
+__device__ __forceinline__ uint32_t rtile_of(const ViewDesc& vd, const ScreenTri& r, uint32_t k) {
+    const uint32_t nx = static_cast<uint32_t>(r.x1 / kRT - r.x0 / kRT + 1);
+    return vd.rtile_base + (static_cast<uint32_t>(r.y0 / kRT) + k / nx) * vd.rtiles_x +
+           static_cast<uint32_t>(r.x0 / kRT) + k % nx;
+}
+
+// A binned triangle's forward setup as k_tile_raster needs it (written once by
+// k_bin_count): corners, screen coordinates and winding, spans (lower / upper
+// bits, columns flag in bit 8 of lower) and the candidate rectangle.
+struct TriRecord {
+    float4 p0, p1, p2;
+    float4 sa;  // s0x, s0y, s1x, s1y
+    float4 sb;  // s2x, s2y, sgn, 0
+    float4 su;  // u0..u2, lower | cols << 8 (bits)
+    float4 sv;  // v0..v2, upper (bits)
+    int4 rect;  // x0, x1, y0, y1
+};
+
+// Tiles of a binned triangle for the fill pass: first tile x, y (16 bits each
+// in the low word), tiles across and count (high word); 0 when not binned.
+__device__ __forceinline__ unsigned long long tile_span_word(const ScreenTri& r, uint32_t n) {
+    const uint32_t nx = static_cast<uint32_t>(r.x1 / kRT - r.x0 / kRT + 1);
+    return (static_cast<unsigned long long>((nx << 16) | n) << 32) |
+           static_cast<unsigned long long>((static_cast<uint32_t>(r.y0 / kRT) << 16) |
+                                           static_cast<uint32_t>(r.x0 / kRT));
+}
+
+// Binning pass 1 (grid.y = view; every lane of a block works in one view): the
+// forward setup of each triangle, stored as its record.  A triangle touching
+// at most bin_tiles raster tiles is counted in each (block LDS hash, then one
+// atomic per distinct tile); a larger one goes to its view's list for the
+// global-key walk (k_raster_small) and marks its tiles as seeded.
+__global__ __launch_bounds__(kBlock) void k_bin_count(ViewTable vt, const uint32_t* __restrict__ idx, uint32_t T,
+                                                      uint32_t V, uint32_t cull, uint32_t bin_tiles,
+                                                      uint32_t* __restrict__ counts, uint8_t* __restrict__ seeded,
+                                                      uint32_t* __restrict__ mlist, uint32_t* __restrict__ mcount,
+                                                      TriRecord* __restrict__ recs,
+                                                      unsigned long long* __restrict__ spans) {
+    __shared__ BinHash h;
+    __shared__ uint32_t medium_n, medium_base;
+    const uint32_t view = blockIdx.y;
+    const ViewDesc& vd = vt.v[view];
+    for (uint32_t t0 = blockIdx.x * kBlock; t0 < T; t0 += gridDim.x * kBlock) {  // block-uniform
+        bin_hash_clear(h);
+        if (threadIdx.x == 0) medium_n = 0u;
+        __syncthreads();
+        const uint32_t t = t0 + threadIdx.x;
+        ScreenTri r;
+        LineSpans sp;
+        const bool ok = t < T && setup_forward_tri(vd, idx, t, V, cull, true, r, sp);
+        const uint32_t n = ok ? rtile_count(r) : 0u;
+        const bool binned = ok && n <= bin_tiles, medium = ok && n > bin_tiles;
+        if (t < T) {
+            const size_t slot = static_cast<size_t>(view) * T + t;
+            spans[slot] = binned ? tile_span_word(r, n) : 0ull;
+            if (binned) {
+                TriRecord rec;
+                rec.p0 = r.p0;
+                rec.p1 = r.p1;
+                rec.p2 = r.p2;
+                rec.sa = make_float4(r.s0x, r.s0y, r.s1x, r.s1y);
+                rec.sb = make_float4(r.s2x, r.s2y, r.sgn, 0.0f);
+                rec.su = make_float4(sp.u[0], sp.u[1], sp.u[2], __uint_as_float(sp.lower | (sp.cols << 8)));
+                rec.sv = make_float4(sp.v[0], sp.v[1], sp.v[2], __uint_as_float(sp.upper));
+                rec.rect = make_int4(r.x0, r.x1, r.y0, r.y1);
+                recs[slot] = rec;
+            }
+        }
+        uint32_t mrank = 0;
+        if (medium) mrank = atomicAdd(&medium_n, 1u);
+        for (uint32_t k = 0; binned && k < n; ++k) {
+            const uint32_t tile = rtile_of(vd, r, k);
+            uint32_t rank;
+            if (bin_hash_add(h, tile, &rank) == kBinSlots) atomicAdd(&counts[tile], 1u);
+        }
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < kBinSlots; i += kBlock)
+            if (h.key[i] != ~0u) atomicAdd(&counts[h.key[i]], h.count[i]);
+        if (threadIdx.x == 0 && medium_n) medium_base = atomicAdd(&mcount[view], medium_n);
+        __syncthreads();
+        if (medium) {
+            mlist[static_cast<size_t>(view) * T + medium_base + mrank] = t;
+            for (uint32_t k = 0; k < n; ++k) seeded[rtile_of(vd, r, k)] = 1u;
+        }
+        __syncthreads();
+    }
+}
+
+// Binning pass 2: each binned triangle's id into its tiles' lists at
+// offsets[tile] + (a block base taken per distinct tile, counting the counts
+// back down to zero) + the entry's rank within the block.
+__global__ __launch_bounds__(kBlock) void k_bin_fill(ViewTable vt, uint32_t T, uint32_t* __restrict__ counts,
+                                                     const uint32_t* __restrict__ offsets, uint32_t* __restrict__ list,
+                                                     const unsigned long long* __restrict__ spans) {
+    __shared__ BinHash h;
+    const uint32_t view = blockIdx.y;
+    const ViewDesc& vd = vt.v[view];
+    for (uint32_t t0 = blockIdx.x * kBlock; t0 < T; t0 += gridDim.x * kBlock) {  // block-uniform
+        bin_hash_clear(h);
+        __syncthreads();
+        const uint32_t t = t0 + threadIdx.x;
+        const unsigned long long w = t < T ? spans[static_cast<size_t>(view) * T + t] : 0ull;
+        const uint32_t hi = static_cast<uint32_t>(w >> 32), lo = static_cast<uint32_t>(w);
+        const uint32_t n = hi & 0xFFFFu, nx = hi > 0xFFFFu ? hi >> 16 : 1u;
+        uint32_t slot_rank[kMaxBinTiles];  // (slot << 16) | rank of tile k, ~0 when placed on its own
+#pragma unroll
+        for (uint32_t k = 0; k < kMaxBinTiles; ++k) {
+            slot_rank[k] = ~0u;
+            if (k < n) {
+                const uint32_t tile = vd.rtile_base + ((lo >> 16) + k / nx) * vd.rtiles_x + (lo & 0xFFFFu) + k % nx;
+                uint32_t rank = 0;
+                const uint32_t slot = bin_hash_add(h, tile, &rank);
+                if (slot == kBinSlots)
+                    list[offsets[tile] + atomicSub(&counts[tile], 1u) - 1u] = t;  // table full: on its own
+                else
+                    slot_rank[k] = (slot << 16) | rank;
+            }
+        }
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < kBinSlots; i += kBlock)
+            if (h.key[i] != ~0u) h.base[i] = atomicSub(&counts[h.key[i]], h.count[i]) - h.count[i];
+        __syncthreads();
+#pragma unroll
+        for (uint32_t k = 0; k < kMaxBinTiles; ++k) {
+            const uint32_t sr = slot_rank[k];
+            if (sr == ~0u) continue;
+            const uint32_t slot = sr >> 16;
+            list[offsets[h.key[slot]] + h.base[slot] + (sr & 0xFFFFu)] = t;
+        }
+        __syncthreads();
+    }
+}
+
+// One workgroup per raster tile: LDS keys (seeded from the global keys where
+// unbinned triangles reach the tile), the tile's triangles from their records
+// walked with LDS atomicMin and every line clamped to the tile, then every
+// pixel of the tile resolved from LDS.
+__global__ __launch_bounds__(kBlock) void k_tile_raster(ViewTable vt, const uint32_t* __restrict__ idx, uint32_t T,
+                                                        int want_deriv, const uint32_t* __restrict__ offsets,
+                                                        const uint32_t* __restrict__ list,
+                                                        const uint8_t* __restrict__ seeded,
+                                                        const TriRecord* __restrict__ recs,
+                                                        unsigned long long* __restrict__ keys) {
+    __shared__ unsigned long long lk[kRTPix];
+    __shared__ FwdRecs R;
+    const uint32_t tile = blockIdx.x;
+    uint32_t view = 0;
+    while (view + 1 < vt.n && tile >= vt.v[view + 1].rtile_base) ++view;
+    const ViewDesc& vd = vt.v[view];
+    const uint32_t lt = tile - vd.rtile_base;
+    const int32_t tx0 = static_cast<int32_t>(lt % vd.rtiles_x) * kRT, ty0 = static_cast<int32_t>(lt / vd.rtiles_x) * kRT;
+    const int32_t W = static_cast<int32_t>(vd.width), H = static_cast<int32_t>(vd.height);
+    const int32_t tx1 = tx0 + kRT - 1 < W - 1 ? tx0 + kRT - 1 : W - 1;
+    const int32_t ty1 = ty0 + kRT - 1 < H - 1 ? ty0 + kRT - 1 : H - 1;
+    const bool seed = seeded[tile] != 0u;
+    unsigned long long* gk = keys + vd.pixel_base;
+    for (uint32_t i = threadIdx.x; i < kRTPix; i += kBlock) {
+        const int32_t x = tx0 + static_cast<int32_t>(i % kRasterTile), y = ty0 + static_cast<int32_t>(i / kRasterTile);
+        lk[i] = seed && x <= tx1 && y <= ty1 ? gk[static_cast<size_t>(y) * vd.width + x] : ~0ull;
+    }
+    __syncthreads();
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    const uint32_t w0 = wave * 64;
+    int32_t* slot = R.slot + w0;
+    const float ax[2] = {span_alpha(vd, 0), span_alpha(vd, 1)};
+    const float bx[2] = {span_beta(vd, 0), span_beta(vd, 1)};
+    const uint32_t e0 = offsets[tile], e1 = offsets[tile + 1];
+    for (uint32_t b = e0 + w0; b < e1; b += kBlock) {  // wave-uniform: the waves take batches of 64 entries
+        int32_t lines = 0;
+        if (b + lane < e1) {
+            const uint32_t t = list[b + lane];
+            const TriRecord& rec = recs[static_cast<size_t>(view) * T + t];
+            const int4 rc = rec.rect;
+            const int32_t xa = rc.x > tx0 ? rc.x : tx0, xb = rc.y < tx1 ? rc.y : tx1;
+            const int32_t ya = rc.z > ty0 ? rc.z : ty0, yb = rc.w < ty1 ? rc.w : ty1;
+            if (xa <= xb && ya <= yb) {
+                const float4 su = rec.su;
+                const uint32_t bits = __float_as_uint(su.w), cols = bits >> 8;
+                lines = cols ? xb - xa + 1 : yb - ya + 1;
+                const uint32_t i = w0 + lane;
+                R.p0[i] = rec.p0;
+                R.p1[i] = rec.p1;
+                R.p2[i] = rec.p2;
+                R.sa[i] = rec.sa;
+                const float4 sb = rec.sb;
+                R.sb[i] = make_float4(sb.x, sb.y, sb.z, __uint_as_float(t));
+                R.su[i] = make_float4(su.x, su.y, su.z, __uint_as_float(bits & 0xFFu));
+                R.sv[i] = rec.sv;
+                R.ln[i] = make_int4(cols ? ya : xa, cols ? yb : xb, cols ? xa : ya, static_cast<int32_t>(cols));
+            }
+        }
+        walk_batch(vd, R, w0, lane, lines, slot, ax, bx, [&](int32_t x, int32_t y, unsigned long long key) {
+            unsigned long long* k = &lk[(y - ty0) * kRT + (x - tx0)];
+            if (key < *k) atomicMin(k, key);
+        });
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < kRTPix; i += kBlock) {
+        const int32_t x = tx0 + static_cast<int32_t>(i % kRasterTile), y = ty0 + static_cast<int32_t>(i / kRasterTile);
+        if (x > tx1 || y > ty1) continue;
+        const uint32_t p = static_cast<uint32_t>(y) * vd.width + static_cast<uint32_t>(x);
+        if (seed) gk[p] = ~0ull;
+        resolve_key(vd, idx, want_deriv, lk[i], p, sample_x(vd, x), sample_y(vd, y));
+    }
 }
 
 // ----------------------------------------------------------------- backward
@@ -1140,9 +1441,50 @@ hipError_t raster_candidates(const ViewTable& vt, const uint32_t* idx, uint32_t 
     if (e != hipSuccess) return e;
     const uint32_t bsz = tri_batch(T, vt.n, tp.fwd_waves);
     k_raster_small<<<dim3(tri_grid(T, bsz, vt.n), vt.n), kBlock, 0, st>>>(vt, idx, T, V, cull, s.sx, s.sy, s.keys, s.queue,
-                                                                    s.counters, bsz, tp.wave_work);
+                                                                    s.counters, bsz, tp.wave_work, nullptr, nullptr);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     k_raster_large<<<kLargeBlocks, kBlock, 0, st>>>(vt, idx, T, V, cull, s.sx, s.sy, s.keys, s.queue, s.counters);
+    return hipGetLastError();
+}
+
+size_t tri_record_bytes() { return sizeof(TriRecord); }
+
+size_t bin_scan_temp_bytes(uint32_t n) {
+    size_t a = 0;
+    (void)hipcub::DeviceScan::ExclusiveSum(nullptr, a, static_cast<const uint32_t*>(nullptr),
+                                           static_cast<uint32_t*>(nullptr), static_cast<int>(n));
+    return a;
+}
+
+hipError_t raster_tiled(const ViewTable& vt, const uint32_t* idx, uint32_t T, uint32_t V, uint32_t cull,
+                        int want_deriv, const RasterScratch& s, const TuneParams& tp, hipStream_t st) {
+    if (tp.bin_tiles == 0 || tp.bin_tiles > kMaxBinTiles) return hipErrorInvalidValue;
+    hipError_t e = fill_u32(s.counters, 0u, 4, st);
+    if (e != hipSuccess) return e;
+    if ((e = fill_u32(s.bin_counts, 0u, static_cast<size_t>(vt.total_rtiles) + 1, st)) != hipSuccess) return e;
+    if ((e = fill_u32(reinterpret_cast<uint32_t*>(s.bin_seeded), 0u, (vt.total_rtiles + 3u) / 4u, st)) != hipSuccess)
+        return e;
+    if ((e = fill_u32(s.medium_count, 0u, kMaxViews, st)) != hipSuccess) return e;
+    const dim3 grid(grid_for(T) < 1024 ? grid_for(T) : 1024, vt.n);
+    k_bin_count<<<grid, kBlock, 0, st>>>(vt, idx, T, V, cull, tp.bin_tiles, s.bin_counts, s.bin_seeded,
+                                         s.medium_list, s.medium_count, static_cast<TriRecord*>(s.tri_records),
+                                         s.tri_spans);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    // the triangles too large to bin (global keys; the largest of them queued as slabs)
+    k_raster_small<<<dim3(64, vt.n), kBlock, 0, st>>>(vt, idx, T, V, cull, s.sx, s.sy, s.keys, s.queue, s.counters,
+                                                     64u, tp.wave_work, s.medium_list, s.medium_count);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    size_t bytes = s.bin_temp_bytes;
+    if ((e = hipcub::DeviceScan::ExclusiveSum(s.bin_temp, bytes, s.bin_counts, s.bin_offsets,
+                                              static_cast<int>(vt.total_rtiles + 1), st)) != hipSuccess)
+        return e;
+    k_bin_fill<<<grid, kBlock, 0, st>>>(vt, T, s.bin_counts, s.bin_offsets, s.bin_list, s.tri_spans);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    k_raster_large<<<kLargeBlocks, kBlock, 0, st>>>(vt, idx, T, V, cull, s.sx, s.sy, s.keys, s.queue, s.counters);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    k_tile_raster<<<vt.total_rtiles, kBlock, 0, st>>>(vt, idx, T, want_deriv, s.bin_offsets, s.bin_list,
+                                                      s.bin_seeded, static_cast<const TriRecord*>(s.tri_records),
+                                                      s.keys);
     return hipGetLastError();
 }
 

@@ -143,7 +143,7 @@ def test_seam_loss_matches_bilinear_restatement(rasterizer):
                               rasterizer=rasterizer)
     empty = seam_consistency_loss(torch.zeros(64, 64, 3, device=dev, requires_grad=True),
                                   torch.zeros(0, 2, 2, device=dev), rasterizer=rasterizer)
-    assert float(empty) == 0.0 and empty.requires_grad
+    assert float(empty.detach()) == 0.0 and empty.requires_grad
 
 
 @pytest.mark.gpu

@@ -1,5 +1,9 @@
 """Autotuner of the raster's launch parameters (PDR_FLAG_AUTOTUNE; project-specified).
 
+* The forward with outputs chooses between the tiled path (bins of 32x32
+  raster tiles, LDS keys) at bin depths of 2, 4 and 8 tiles per triangle and
+  the triangle-parallel path (global keys) with its walk limit and wave count;
+  the backward chooses its walk limit and wave count.
 * Raster forward and backward results are bit-identical under the defaults,
   under the autotuned choice, and under every candidate imported into the cache
   (the forward's winner is order-free; both backward paths sum each triangle in
@@ -16,7 +20,8 @@ from photara_drender_amd import scenes
 
 pytestmark = pytest.mark.gpu
 
-FORWARD = [(w, n) for w in (4096.0, 16384.0, 65536.0) for n in (2048, 4096, 8192)]
+FORWARD = [(1, d, 16384.0, 4096) for d in (2, 4, 8)] + \
+    [(0, 4, w, n) for w in (4096.0, 16384.0, 65536.0) for n in (2048, 4096, 8192)]
 BACKWARD = [(w, n) for w in (256.0, 512.0, 1024.0, 2048.0) for n in (2048, 4096, 8192)]
 
 
@@ -66,16 +71,17 @@ def test_autotuned_and_every_candidate_give_identical_results():
         np.testing.assert_array_equal(a, b)
     lines = tuned_ctx.autotune_export().splitlines()
     kinds = sorted(line.split()[0] for line in lines)
-    assert kinds == ["backward", "forward"], lines
+    assert kinds == ["backward", "forward_out"], lines
     keys = {line.split()[0]: " ".join(line.split()[:4]) for line in lines}
-    for (ww, fw), (flat, bw) in zip(FORWARD + FORWARD[:3], BACKWARD):
+    for (tiled, depth, ww, fw), (flat, bw) in zip(FORWARD, BACKWARD):
         ctx = _native.Context(0, flags=_native.PDR_FLAG_AUTOTUNE)
-        ctx.autotune_import(f"{keys['forward']} {ww} {fw} 512 4096\n{keys['backward']} 16384 4096 {flat} {bw}\n")
+        ctx.autotune_import(f"{keys['forward_out']} {ww} {fw} 512 4096 {tiled} {depth}\n"
+                            f"{keys['backward']} 16384 4096 {flat} {bw} 1 4\n")
         got = run_views(ctx, sc)
         for a, b in zip(ref, got):
             np.testing.assert_array_equal(a, b)
-        assert ctx.autotune_export() == f"{keys['backward']} 16384 4096 {flat:g} {bw}\n" \
-                                        f"{keys['forward']} {ww:g} {fw} 512 4096\n"  # cached classes are not re-tuned
+        assert ctx.autotune_export() == f"{keys['backward']} 16384 4096 {flat:g} {bw} 1 4\n" \
+                                        f"{keys['forward_out']} {ww:g} {fw} 512 4096 {tiled} {depth}\n"  # not re-tuned
 
 
 def test_autotune_cache_rejects_malformed_lines():
@@ -85,8 +91,10 @@ def test_autotune_cache_rejects_malformed_lines():
     with pytest.raises(ValueError, match="malformed"):
         ctx.autotune_import("forward 1 2\n")
     with pytest.raises(ValueError, match="malformed"):
-        ctx.autotune_import("sideways 1 2 3 4096 4096 512 4096\n")
-    ctx.autotune_import("forward 1 12 20 4096 2048 512 4096\n\n")
-    assert ctx.autotune_export() == "forward 1 12 20 4096 2048 512 4096\n"
+        ctx.autotune_import("sideways 1 2 3 4096 4096 512 4096 1 4\n")
+    with pytest.raises(ValueError, match="malformed"):
+        ctx.autotune_import("forward_out 1 2 3 4096 4096 512 4096 2 4\n")  # tiled is 0 or 1
+    ctx.autotune_import("forward 1 12 20 4096 2048 512 4096 0 4\n\n")
+    assert ctx.autotune_export() == "forward 1 12 20 4096 2048 512 4096 0 4\n"
     ctx.autotune_clear()
     assert ctx.autotune_export() == ""
