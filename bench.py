@@ -202,82 +202,106 @@ def cpu_baseline(scene, views, budget_s=20.0):
     }
 
 
-def chain_benchmark(steps=50, warmup=5, cpu=True):
+def texels_referenced(uvs, raster, tw, th):
+    """U: distinct texels the clamp-mode bilinear taps of covered pixels read."""
+    cov = raster[..., 3] != 0
+    tx = uvs[cov][:, 0] * np.float32(tw) - np.float32(0.5)
+    ty = uvs[cov][:, 1] * np.float32(th) - np.float32(0.5)
+    bx, by = np.floor(tx).astype(np.int64), np.floor(ty).astype(np.int64)
+    taps = [np.clip(by + dy, 0, th - 1) * tw + np.clip(bx + dx, 0, tw - 1) for dy in (0, 1) for dx in (0, 1)]
+    return int(np.unique(np.concatenate(taps)).size)
+
+
+def chain_algorithmic_bytes(V, T, P, tex_texels, U):
+    """Every API-boundary array of the chain read once / written once (A = 2
+    uv attributes, RGBA texture): raster fwd+bwd 64P + 2(16V+12T) + 16V,
+    interp fwd 24P + 8V + 12T, interp bwd 40P + 16V + 12T, texture fwd 40P +
+    16U, texture bwd 48P + 16U + 16 * texels (dense texel gradient)."""
+    mesh = 16 * V + 12 * T
+    raster = 64 * P + 2 * mesh + 16 * V
+    interp = (24 * P + 8 * V + 12 * T) + (40 * P + 16 * V + 12 * T)
+    texture = (40 * P + 16 * U) + (48 * P + 16 * U + 16 * tex_texels)
+    return raster + interp + texture
+
+
+def chain_benchmark(steps=200, warmup=5, cpu=True):
     """Configuration 1: one 512x512 view of a level-5 icosphere (20,480 tris), spherical
     UVs, 1024^2 RGBA texture; rasterize -> interpolate (uv, A=2) -> texture (clamp),
     loss sum (img - target)^2, backward to positions, uv attributes and texels.
-    Mpix/s = 262,144 / (summed library kernel time per chain; the loss op excluded)."""
-    import ctypes
-
+    value = 262,144 px / wall time per chain: the faster of eager launches and the
+    chain captured in a HIP graph and replayed (every replay checked against the
+    eager chain first); beside it the other one's wall time and the summed library
+    kernel time (event-timed regions, loss op excluded)."""
     import torch
 
-    from photara_drender_amd import _native, scenes
+    from photara_drender_amd import _native, scenes, workloads
 
     sc = scenes.chain_scene(level=5, res=(512, 512), tex_size=1024, seed=1)
-    dev = torch.device("cuda:0")
-    H, W = sc.resolution
-    C = sc.texture.shape[2]
-    th, tw = sc.texture.shape[:2]
-    V, T = sc.clip.shape[0], sc.indices.shape[0]
-    clip = torch.tensor(sc.clip, device=dev)
-    idx = torch.tensor(sc.indices.view(np.int32), device=dev)
-    uv = torch.tensor(sc.uv, device=dev)
-    tex = torch.tensor(sc.texture, device=dev)
-    target = torch.tensor(sc.target, device=dev)
-    raster = torch.empty((H, W, 4), device=dev)
-    deriv = torch.empty((H, W, 4), device=dev)
-    uvs = torch.empty((H, W, 2), device=dev)
-    color = torch.empty((H, W, C), device=dev)
-    g_color = torch.empty((H, W, C), device=dev)
-    g_tex = torch.empty((th, tw, C), device=dev)
-    g_uvs = torch.empty((H, W, 2), device=dev)
-    g_uv = torch.empty((V, 2), device=dev)
-    g_raster = torch.empty((H, W, 4), device=dev)
-    g_pos = torch.empty((V, 4), device=dev)
-    vp_out = (ctypes.c_float * 4)()
     ctx = _native.Context(0, flags=_native.PDR_FLAG_AUTOTUNE)
-    lib = _native.lib()
-    st = torch.cuda.current_stream().cuda_stream
-    ck = _native.check
-
-    def step():
-        ck(lib.pdr_rasterize_forward(ctx.handle, clip.data_ptr(), V, idx.data_ptr(), T, W, H, 0, 1, None,
-                                     raster.data_ptr(), deriv.data_ptr(), vp_out, st))
-        ck(lib.pdr_interpolate_forward(ctx.handle, uv.data_ptr(), V, 2, idx.data_ptr(), T, raster.data_ptr(), W, H,
-                                       uvs.data_ptr(), st))
-        ck(lib.pdr_texture_forward(ctx.handle, tex.data_ptr(), tw, th, C, 0, uvs.data_ptr(), raster.data_ptr(), W, H,
-                                   color.data_ptr(), st))
-        torch.sub(color, target, alpha=1.0, out=g_color).mul_(2.0)  # d/dimg of sum (img - target)^2
-        ck(lib.pdr_texture_backward(ctx.handle, tex.data_ptr(), tw, th, C, 0, uvs.data_ptr(), raster.data_ptr(), W, H,
-                                    g_color.data_ptr(), 1, g_tex.data_ptr(), g_uvs.data_ptr(), st))
-        ck(lib.pdr_interpolate_backward(ctx.handle, uv.data_ptr(), V, 2, idx.data_ptr(), T, raster.data_ptr(), W, H,
-                                        g_uvs.data_ptr(), g_uv.data_ptr(), g_raster.data_ptr(), st))
-        ck(lib.pdr_rasterize_backward(ctx.handle, clip.data_ptr(), V, idx.data_ptr(), T, raster.data_ptr(), W, H,
-                                      g_raster.data_ptr(), None, g_pos.data_ptr(), st))
-
-    step()  # autotuning pass (untimed)
+    chain = workloads.TexturedChain(ctx, sc)
+    H, W, P = chain.H, chain.W, chain.H * chain.W
+    s = torch.cuda.Stream()
+    chain.step(s)  # autotuning pass (untimed)
     for _ in range(warmup):
-        step()
+        chain.step(s)
     torch.cuda.synchronize()
+    eager = chain.outputs()
+
+    def wall(run):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            run()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / steps * 1e3
+
+    eager_ms = wall(lambda: chain.step(s))
     ctx.set_timing(True)
-    t0 = time.perf_counter()
     for _ in range(steps):
-        step()
+        chain.step(s)
     torch.cuda.synchronize()
-    wall_ms = (time.perf_counter() - t0) / steps * 1e3
     kernels = {}
     for name in ("raster_tiled", "raster_candidates", "raster_resolve", "interp_fwd", "texture_fwd", "texture_bwd",
-                 "interp_bwd",
-                 "vertex_csr", "bwd_prepass", "corner_grads", "vertex_reduce"):
+                 "interp_bwd", "vertex_csr", "bwd_prepass", "corner_grads", "vertex_reduce"):
         ms, launches = ctx.kernel_time(name)
         if launches:
             kernels[name] = ms / steps
     ctx.set_timing(False)
     chain_ms = sum(kernels.values())
+    graph, graph_ms, graph_error = None, None, None
+    try:
+        graph = chain.capture(s)
+        for _ in range(3):  # replays between eager work, each checked against the eager chain
+            graph.replay()
+            torch.cuda.synchronize()
+            got = chain.outputs()
+            for k, ref in eager.items():
+                if k in ("g_tex", "g_uv"):  # float atomics: order-dependent rounding
+                    np.testing.assert_allclose(got[k], ref, rtol=1e-4, atol=1e-6 * max(np.abs(ref).max(), 1.0))
+                else:
+                    np.testing.assert_array_equal(got[k], ref)
+            chain.step(s)
+        graph_ms = wall(graph.replay)
+    except Exception as e:  # reported, not hidden: the eager wall time then stands as the value
+        graph_error = f"{type(e).__name__}: {e}"[:300]
+    graph_faster = graph_ms is not None and graph_ms < eager_ms
+    wall_ms = graph_ms if graph_faster else eager_ms
+    U = texels_referenced(eager["uvs"], eager["raster"], chain.tw, chain.th)
+    alg = chain_algorithmic_bytes(chain.V, chain.T, P, chain.tw * chain.th, U)
+    achieved = alg / (wall_ms * 1e-3) / 1e9
     out = {"workload": "512x512 view, icosphere level 5 (20,480 tris), 1024^2x4 texture, rasterize -> interpolate "
                        "(uv) -> texture, fwd+bwd",
-           "value": H * W / (chain_ms * 1e-3) / 1e6, "unit": "Mpix/s", "chain_kernel_ms": chain_ms,
-           "wall_ms_per_chain": wall_ms, "kernel_ms": {k: round(v, 4) for k, v in kernels.items()},
+           "value": P / (wall_ms * 1e-3) / 1e6, "unit": "Mpix/s",
+           "value_basis": "wall time per chain, " + ("graph-replayed (faster than eager)" if graph_faster else
+                                                      "eager launches (no faster as a graph)" if graph_ms else
+                                                      "eager launches (graph capture failed)"),
+           "wall_ms_per_chain": wall_ms, "graph_wall_ms": graph_ms, "graph_error": graph_error,
+           "eager_wall_ms": eager_ms, "eager_wall_mpix_s": P / (eager_ms * 1e-3) / 1e6,
+           "chain_kernel_ms": chain_ms, "kernel_mpix_s": P / (chain_ms * 1e-3) / 1e6,
+           "kernel_ms": {k: round(v, 4) for k, v in kernels.items()},
+           "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": achieved / HBM_PEAK_GBS, "alg_bytes_per_chain": alg, "texels_referenced": U,
+                        "basis": "chain algorithmic bytes / wall time per chain"},
            "autotune": ctx.autotune_export().splitlines()}
     if cpu:
         import oracle
@@ -447,6 +471,8 @@ def main():
                     help="print the per-kernel table of a rocprofv3 --kernel-trace --output-format csv folder")
     ap.add_argument("--refine-only", action="store_true",
                     help="run only the atlas-refinement workload and print its result (for profiling runs)")
+    ap.add_argument("--chain-only", action="store_true",
+                    help="run only the single-view textured chain workload and print its result (for profiling runs)")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="run 3 steps of the headline workload and a 1 GiB device copy (for rocprofv3 --pmc runs)")
     ap.add_argument("--traffic-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
@@ -469,6 +495,9 @@ def main():
         sys.exit(2)
     if args.refine_only:
         print(json.dumps({"refine": refine_benchmark(cpu=not args.no_cpu_baseline)}))
+        return
+    if args.chain_only:
+        print(json.dumps({"chain_512": chain_benchmark(cpu=not args.no_cpu_baseline)}))
         return
 
     import ctypes

@@ -159,9 +159,12 @@ struct pdr_context_s {
         return st;
     }
 
-    void* get(const std::string& key, size_t bytes) {
+    // Grow-only scratch buffer `key` of at least `bytes`; *fresh (when given)
+    // tells whether it was (re)allocated, its old contents lost.
+    void* get(const std::string& key, size_t bytes, bool* fresh = nullptr) {
         Slot& s = scratch[key];
         if (bytes == 0) bytes = 16;
+        if (fresh) *fresh = s.bytes < bytes;
         if (s.bytes < bytes) {
             if (s.ptr) {
                 pdrh::hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
@@ -176,8 +179,8 @@ struct pdr_context_s {
         return s.ptr;
     }
     template <class T>
-    T* get_t(const std::string& key, size_t count) {
-        return static_cast<T*>(get(key, count * sizeof(T)));
+    T* get_t(const std::string& key, size_t count, bool* fresh = nullptr) {
+        return static_cast<T*>(get(key, count * sizeof(T), fresh));
     }
 
     ~pdr_context_s() {
@@ -519,13 +522,19 @@ inline const unsigned long long* run_raster_keys(pdr_context ctx, const pdr::Vie
 inline void run_raster_backward(pdr_context ctx, const pdr::ViewTable& vt, const uint32_t* idx, uint32_t T, uint32_t V,
                                 hipStream_t st) {
     const pdr::RasterScratch s = raster_scratch(ctx, vt, T, st, false);
+    // The adjacency persists in the context: a reallocated part invalidates it,
+    // and newly allocated counts start at zero.
     pdr::VertexCsr csr;
-    csr.counts = ctx->get_t<uint32_t>("csr_counts", static_cast<size_t>(V) + 1);
-    csr.offsets = ctx->get_t<uint32_t>("csr_offsets", static_cast<size_t>(V) + 1);
-    csr.corners = ctx->get_t<uint32_t>("csr_corners", static_cast<size_t>(T) * 3);
-    csr.slot_of = ctx->get_t<uint32_t>("csr_slot_of", static_cast<size_t>(T) * 3);
-    csr.temp_bytes = pdr::csr_temp_bytes(T, V);
-    csr.temp = ctx->get("csr_temp", csr.temp_bytes);
+    bool f[6];
+    csr.counts = ctx->get_t<uint32_t>("csr_counts", static_cast<size_t>(V) + 1, &f[0]);
+    csr.offsets = ctx->get_t<uint32_t>("csr_offsets", static_cast<size_t>(V) + 1, &f[1]);
+    csr.corners = ctx->get_t<uint32_t>("csr_corners", static_cast<size_t>(T) * 3, &f[2]);
+    csr.slot_of = ctx->get_t<uint32_t>("csr_slot_of", static_cast<size_t>(T) * 3, &f[3]);
+    csr.saved = ctx->get_t<uint32_t>("csr_saved", static_cast<size_t>(T) * 3, &f[4]);
+    csr.state = ctx->get_t<uint32_t>("csr_state", 8, &f[5]);
+    csr.totals = ctx->get_t<uint32_t>("csr_totals", pdr::csr_total_words(V));
+    if (f[0]) hip_check(pdr::fill_u32(csr.counts, 0u, ctx->scratch["csr_counts"].bytes / 4, st), "fill");
+    if (f[1] || f[2] || f[3] || f[4] || f[5]) hip_check(pdr::fill_u32(csr.state, 0u, 8, st), "fill");
     float4* corner = ctx->get_t<float4>("corner_grads", static_cast<size_t>(vt.n) * T * 3);
     pdr::RasterScratch sb = s;
     sb.cid = ctx->get_t<uint32_t>("backward_ids", vt.total_pixels);

@@ -80,14 +80,16 @@ struct RasterScratch {
     const uint32_t* slot_of = nullptr;   // [3T] backward: where each corner's gradient is written (VertexCsr)
 };
 
-// Vertex -> corner adjacency (corners ascending per vertex).
+// Vertex -> corner adjacency (corners ascending per vertex), kept across calls
+// with the indices it was built from (rebuilt on the device when they differ).
 struct VertexCsr {
-    uint32_t* counts = nullptr;   // [V+1]
+    uint32_t* counts = nullptr;   // [V+1] all zeros between calls
     uint32_t* offsets = nullptr;  // [V+1]
     uint32_t* corners = nullptr;  // [3T] corner ids, ascending per vertex
     uint32_t* slot_of = nullptr;  // [3T] position of each corner in `corners`
-    void* temp = nullptr;
-    size_t temp_bytes = 0;
+    uint32_t* saved = nullptr;    // [3T] the indices the adjacency was built from
+    uint32_t* totals = nullptr;   // [csr_total_words(V)] scan tile totals
+    uint32_t* state = nullptr;    // [8] stale, built, T, V, blocks finished
 };
 
 // ---- raster.hip
@@ -119,7 +121,7 @@ hipError_t raster_tiled(const ViewTable& views, const uint32_t* idx, uint32_t T,
 // k_raster_resolve: raster / derivatives from the keys; resets the keys
 hipError_t raster_resolve(const ViewTable& views, const uint32_t* idx, int want_deriv, const RasterScratch& s,
                           hipStream_t st);
-size_t csr_temp_bytes(uint32_t T, uint32_t V);
+size_t csr_total_words(uint32_t V);
 hipError_t build_vertex_csr(const uint32_t* idx, uint32_t T, uint32_t V, const VertexCsr& csr, hipStream_t st);
 // k_bwd_prepass: per-pixel compared ids and per-view whole-box flags (raster, grad_raster of each view)
 hipError_t backward_prepass(const ViewTable& views, const uint32_t* idx, uint32_t T, uint32_t V,

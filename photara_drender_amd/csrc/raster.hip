@@ -1307,17 +1307,183 @@ __global__ __launch_bounds__(kBlock) void k_corner_large_ordered(
     }
 }
 
-__global__ void k_count_corners(const uint32_t* __restrict__ idx, uint32_t n, uint32_t V, uint32_t* __restrict__ counts) {
+// ---- Vertex -> corner adjacency, cached across calls.
+//
+// The adjacency depends on the index buffer alone, so the context keeps it
+// with a copy of the indices it was built from.  Every call compares the
+// indices with that copy on the device, and the build kernels exit at once
+// unless a word differs, T or V changed, or nothing was built yet: a sequence
+// of calls over one mesh builds it once, with no host readback (the calls stay
+// capturable in a graph).  State words (kCsr*): the compare sets kCsrStale, the
+// build's last block clears it and records T and V.  `counts` is all zeros
+// between calls (the placement consumes each vertex's count, the last block
+// clears the invalid-corner cursor); a newly allocated `counts` or `state` is
+// zero-filled by the host.
+enum : uint32_t { kCsrStale = 0, kCsrBuilt = 1, kCsrT = 2, kCsrV = 3, kCsrDone = 4 };
+constexpr uint32_t kCsrLdsVerts = 16384;  // one-workgroup build: V + 1 counts in 64 KB of LDS
+constexpr uint32_t kCsrScanTile = 2048;   // scan: 256 threads x 8 counts
+
+// Ascending Shell sort (Ciura's gaps) of one vertex's corner list.
+__device__ __forceinline__ void sort_corner_list(uint32_t* a, uint32_t n) {
+    const uint32_t gaps[8] = {701u, 301u, 132u, 57u, 23u, 10u, 4u, 1u};
+    for (int g = 0; g < 8; ++g) {
+        const uint32_t gap = gaps[g];
+        for (uint32_t i = gap; i < n; ++i) {
+            const uint32_t x = a[i];
+            uint32_t j = i;
+            for (; j >= gap && a[j - gap] > x; j -= gap) a[j] = a[j - gap];
+            a[j] = x;
+        }
+    }
+}
+
+// Compares this block's share of the indices with the saved copy; block 0
+// also checks that an adjacency of this T and V was built.
+__device__ __forceinline__ bool csr_differs(const uint32_t* __restrict__ idx, const uint32_t* __restrict__ saved,
+                                            uint32_t n, uint32_t T, uint32_t V, const uint32_t* state) {
+    bool diff = false;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        diff |= idx[i] != saved[i];
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        diff |= !(state[kCsrBuilt] == 1u && state[kCsrT] == T && state[kCsrV] == V);
+    return diff;
+}
+
+// Meshes with V < kCsrLdsVerts: compare and build in one launch.  Every block
+// compares its share; the last block to finish builds the whole adjacency in
+// LDS when any block found a difference (counts, exclusive scan, placement
+// with the indices saved, per-vertex sort).
+__global__ __launch_bounds__(1024) void k_csr_small(const uint32_t* __restrict__ idx, uint32_t n, uint32_t T,
+                                                    uint32_t V, uint32_t* state, uint32_t* __restrict__ saved,
+                                                    uint32_t* __restrict__ offsets, uint32_t* __restrict__ corners,
+                                                    uint32_t* __restrict__ slot_of) {
+    __shared__ uint32_t cnt[kCsrLdsVerts];
+    __shared__ uint32_t last, stale;
+    using Scan = hipcub::BlockScan<uint32_t, 1024>;
+    __shared__ typename Scan::TempStorage scan_tmp;
+    const uint32_t tid = threadIdx.x;
+    if (__syncthreads_or(csr_differs(idx, saved, n, T, V, state)) && tid == 0) atomicOr(&state[kCsrStale], 1u);
+    if (tid == 0) {
+        __threadfence();
+        last = atomicAdd(&state[kCsrDone], 1u) == gridDim.x - 1;
+        if (last) {
+            __threadfence();
+            stale = atomicOr(&state[kCsrStale], 0u);
+            state[kCsrDone] = 0u;
+        }
+    }
+    __syncthreads();
+    if (!last || !stale) return;
+    for (uint32_t v = tid; v <= V; v += 1024) cnt[v] = 0u;
+    __syncthreads();
+    for (uint32_t i = tid; i < n; i += 1024) {
+        const uint32_t v = idx[i];
+        if (v < V) atomicAdd(&cnt[v], 1u);
+    }
+    __syncthreads();
+    // exclusive scan of cnt[0..V]: consecutive runs per thread, then the runs' totals
+    const uint32_t per = (V + 1 + 1023) / 1024, lo = tid * per, hi = min(lo + per, V + 1);
+    uint32_t sum = 0;
+    for (uint32_t v = lo; v < hi; ++v) sum += cnt[v];
+    uint32_t run;
+    Scan(scan_tmp).ExclusiveSum(sum, run);
+    for (uint32_t v = lo; v < hi; ++v) {
+        const uint32_t c = cnt[v];
+        cnt[v] = run;  // now the list's cursor
+        offsets[v] = run;
+        run += c;
+    }
+    __syncthreads();
+    // corners with an invalid vertex go past offsets[V], each with its slot
+    for (uint32_t i = tid; i < n; i += 1024) {
+        const uint32_t v = idx[i];
+        const uint32_t j = atomicAdd(&cnt[v < V ? v : V], 1u);
+        corners[j] = i;
+        if (v >= V) slot_of[i] = j;
+        saved[i] = v;
+    }
+    __syncthreads();
+    for (uint32_t v = tid; v < V; v += 1024) {
+        const uint32_t b = offsets[v], m = offsets[v + 1] - b;
+        uint32_t* a = corners + b;
+        sort_corner_list(a, m);
+        for (uint32_t i = 0; i < m; ++i) slot_of[a[i]] = b + i;
+    }
+    if (tid == 0) {
+        state[kCsrStale] = 0u;
+        state[kCsrBuilt] = 1u;
+        state[kCsrT] = T;
+        state[kCsrV] = V;
+    }
+}
+
+// Larger meshes: the compare, then count / scan / place / sort kernels that
+// each exit at once while kCsrStale is clear.
+__global__ void k_csr_compare(const uint32_t* __restrict__ idx, uint32_t n, uint32_t T, uint32_t V, uint32_t* state,
+                              const uint32_t* __restrict__ saved) {
+    if (__syncthreads_or(csr_differs(idx, saved, n, T, V, state)) && threadIdx.x == 0)
+        atomicOr(&state[kCsrStale], 1u);
+}
+
+__global__ void k_count_corners(const uint32_t* __restrict__ idx, uint32_t n, uint32_t V, const uint32_t* state,
+                                uint32_t* __restrict__ counts) {
+    if (!state[kCsrStale]) return;
     for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
         if (idx[i] < V) atomicAdd(&counts[idx[i]], 1u);
 }
 
+// Exclusive scan of counts[0..m) in tiles of kCsrScanTile: each tile's total,
+// then each tile's scan offset by the totals of the tiles before it.
+__global__ __launch_bounds__(256) void k_csr_scan_totals(const uint32_t* __restrict__ counts, uint32_t m,
+                                                         const uint32_t* state, uint32_t* __restrict__ totals) {
+    if (!state[kCsrStale]) return;
+    using Reduce = hipcub::BlockReduce<uint32_t, 256>;
+    __shared__ typename Reduce::TempStorage tmp;
+    const uint32_t base = blockIdx.x * kCsrScanTile + threadIdx.x * 8;
+    uint32_t s = 0;
+    for (uint32_t k = 0; k < 8; ++k)
+        if (base + k < m) s += counts[base + k];
+    s = Reduce(tmp).Sum(s);
+    if (threadIdx.x == 0) totals[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(256) void k_csr_scan_out(const uint32_t* __restrict__ counts, uint32_t m,
+                                                      const uint32_t* state, const uint32_t* __restrict__ totals,
+                                                      uint32_t* __restrict__ offsets) {
+    if (!state[kCsrStale]) return;
+    using Reduce = hipcub::BlockReduce<uint32_t, 256>;
+    using Scan = hipcub::BlockScan<uint32_t, 256>;
+    __shared__ typename Reduce::TempStorage rtmp;
+    __shared__ typename Scan::TempStorage stmp;
+    __shared__ uint32_t prefix;
+    uint32_t s = 0;
+    for (uint32_t b = threadIdx.x; b < blockIdx.x; b += 256) s += totals[b];
+    s = Reduce(rtmp).Sum(s);
+    if (threadIdx.x == 0) prefix = s;
+    const uint32_t base = blockIdx.x * kCsrScanTile + threadIdx.x * 8;
+    uint32_t c[8], t = 0;
+    for (uint32_t k = 0; k < 8; ++k) {
+        c[k] = base + k < m ? counts[base + k] : 0u;
+        t += c[k];
+    }
+    uint32_t run;
+    Scan(stmp).ExclusiveSum(t, run);
+    __syncthreads();
+    run += prefix;
+    for (uint32_t k = 0; k < 8; ++k) {
+        if (base + k < m) offsets[base + k] = run;
+        run += c[k];
+    }
+}
+
 // Places every corner in its vertex's list (order within a list fixed by
-// k_sort_corner_lists); corners with an invalid vertex go past offsets[V].
-// counts (after the scan) is consumed as a per-vertex cursor; counts[V] is 0.
-__global__ void k_place_corners(const uint32_t* __restrict__ idx, uint32_t n, uint32_t V,
+// k_sort_corner_lists) and saves the indices; corners with an invalid vertex go
+// past offsets[V].  counts (after the scan) is consumed as a per-vertex cursor.
+__global__ void k_place_corners(const uint32_t* __restrict__ idx, uint32_t n, uint32_t V, const uint32_t* state,
                                 const uint32_t* __restrict__ offsets, uint32_t* __restrict__ counts,
-                                uint32_t* __restrict__ corners, uint32_t* __restrict__ slot_of) {
+                                uint32_t* __restrict__ corners, uint32_t* __restrict__ slot_of,
+                                uint32_t* __restrict__ saved) {
+    if (!state[kCsrStale]) return;
     for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
         const uint32_t v = idx[i];
         uint32_t j;
@@ -1328,27 +1494,36 @@ __global__ void k_place_corners(const uint32_t* __restrict__ idx, uint32_t n, ui
             slot_of[i] = j;
         }
         corners[j] = i;
+        saved[i] = v;
     }
 }
 
-// One lane per vertex: its list sorted ascending (Shell sort, Ciura's gaps),
-// then the inverse map corner -> list position.
-__global__ void k_sort_corner_lists(uint32_t V, const uint32_t* __restrict__ offsets, uint32_t* __restrict__ corners,
-                                    uint32_t* __restrict__ slot_of) {
-    const uint32_t gaps[8] = {701u, 301u, 132u, 57u, 23u, 10u, 4u, 1u};
+// One lane per vertex: its list sorted ascending, then the inverse map corner
+// -> list position.  The last block to finish records the build.
+__global__ void k_sort_corner_lists(uint32_t T, uint32_t V, uint32_t* state, const uint32_t* __restrict__ offsets,
+                                    uint32_t* __restrict__ corners, uint32_t* __restrict__ slot_of,
+                                    uint32_t* __restrict__ counts) {
+    if (!state[kCsrStale]) return;
+    __shared__ bool last;
     for (uint32_t v = blockIdx.x * kBlock + threadIdx.x; v < V; v += gridDim.x * kBlock) {
         const uint32_t b = offsets[v], n = offsets[v + 1] - b;
         uint32_t* a = corners + b;
-        for (int g = 0; g < 8; ++g) {
-            const uint32_t gap = gaps[g];
-            for (uint32_t i = gap; i < n; ++i) {
-                const uint32_t x = a[i];
-                uint32_t j = i;
-                for (; j >= gap && a[j - gap] > x; j -= gap) a[j] = a[j - gap];
-                a[j] = x;
-            }
-        }
+        sort_corner_list(a, n);
         for (uint32_t i = 0; i < n; ++i) slot_of[a[i]] = b + i;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        last = atomicAdd(&state[kCsrDone], 1u) == gridDim.x - 1;
+        if (last) {
+            state[kCsrDone] = 0u;
+            counts[V] = 0u;
+            state[kCsrBuilt] = 1u;
+            state[kCsrT] = T;
+            state[kCsrV] = V;
+            __threadfence();
+            state[kCsrStale] = 0u;
+        }
     }
 }
 
@@ -1506,29 +1681,33 @@ hipError_t backward_prepass(const ViewTable& vt, const uint32_t* idx, uint32_t T
     return hipGetLastError();
 }
 
-size_t csr_temp_bytes(uint32_t T, uint32_t V) {
-    (void)T;
-    size_t a = 0;
-    (void)hipcub::DeviceScan::ExclusiveSum(nullptr, a, static_cast<const uint32_t*>(nullptr),
-                                           static_cast<uint32_t*>(nullptr), static_cast<int>(V + 1));
-    return a;
-}
+size_t csr_total_words(uint32_t V) { return (static_cast<size_t>(V) + 1 + kCsrScanTile - 1) / kCsrScanTile; }
 
 // Vertex -> corner adjacency with corners ascending per vertex, and each
-// corner's position in it.
+// corner's position in it; rebuilt only when the indices differ from the
+// ones it was last built from.
 hipError_t build_vertex_csr(const uint32_t* idx, uint32_t T, uint32_t V, const VertexCsr& c, hipStream_t st) {
     const uint32_t n = T * 3;
-    hipError_t e = fill_u32(c.counts, 0u, static_cast<size_t>(V) + 1, st);
-    if (e != hipSuccess) return e;
-    k_count_corners<<<grid_for(n), kBlock, 0, st>>>(idx, n, V, c.counts);
+    hipError_t e;
+    if (V < kCsrLdsVerts) {
+        const uint32_t blocks = (n + 8191) / 8192;
+        k_csr_small<<<blocks < 1u ? 1u : (blocks > 64u ? 64u : blocks), 1024, 0, st>>>(idx, n, T, V, c.state, c.saved, c.offsets, c.corners,
+                                                          c.slot_of);
+        return hipGetLastError();
+    }
+    k_csr_compare<<<grid_for(n), kBlock, 0, st>>>(idx, n, T, V, c.state, c.saved);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    size_t bytes = c.temp_bytes;
-    if ((e = hipcub::DeviceScan::ExclusiveSum(c.temp, bytes, c.counts, c.offsets, static_cast<int>(V + 1), st)) !=
-        hipSuccess)
-        return e;
-    k_place_corners<<<grid_for(n), kBlock, 0, st>>>(idx, n, V, c.offsets, c.counts, c.corners, c.slot_of);
+    k_count_corners<<<grid_for(n), kBlock, 0, st>>>(idx, n, V, c.state, c.counts);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    k_sort_corner_lists<<<grid_for(V), kBlock, 0, st>>>(V, c.offsets, c.corners, c.slot_of);
+    const uint32_t m = V + 1, tiles = static_cast<uint32_t>(csr_total_words(V));
+    k_csr_scan_totals<<<tiles, 256, 0, st>>>(c.counts, m, c.state, c.totals);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    k_csr_scan_out<<<tiles, 256, 0, st>>>(c.counts, m, c.state, c.totals, c.offsets);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    k_place_corners<<<grid_for(n), kBlock, 0, st>>>(idx, n, V, c.state, c.offsets, c.counts, c.corners, c.slot_of,
+                                                    c.saved);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    k_sort_corner_lists<<<grid_for(V), kBlock, 0, st>>>(T, V, c.state, c.offsets, c.corners, c.slot_of, c.counts);
     return hipGetLastError();
 }
 

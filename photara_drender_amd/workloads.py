@@ -163,3 +163,70 @@ def atlas_psnr(ctx, problem: RefineProblem, atlas, against="clean") -> float:
         count += valid.sum() * 3
     mse = float(sse / count)
     return 10.0 * math.log10(1.0 / max(mse, 1e-20))
+
+
+class TexturedChain:
+    """The textured raster chain over device tensors: rasterize -> interpolate
+    (uv) -> texture (clamp), the loss sum (color - target)^2's gradient, then
+    texture, interpolate and rasterize backward to texels, uv attributes and
+    clip positions.  ``step(stream)`` runs it eagerly on a stream;
+    ``capture(stream)`` records one step into a HIP graph (torch.cuda.CUDAGraph)
+    after the caller has run it eagerly on that stream (the context's scratch
+    is sized and its launch parameters chosen by then)."""
+
+    NAMES = ("raster", "deriv", "uvs", "color", "g_color", "g_tex", "g_uvs", "g_uv", "g_raster", "g_pos")
+
+    def __init__(self, ctx, scene, device="cuda:0"):
+        import torch
+
+        dev = torch.device(device)
+        self.ctx = ctx
+        self.H, self.W = scene.resolution
+        self.th, self.tw, self.C = scene.texture.shape
+        self.V, self.T = scene.clip.shape[0], scene.indices.shape[0]
+        self.clip = torch.tensor(scene.clip, device=dev)
+        self.idx = torch.tensor(scene.indices.view(np.int32), device=dev)
+        self.uv = torch.tensor(scene.uv, device=dev)
+        self.tex = torch.tensor(scene.texture, device=dev)
+        self.target = torch.tensor(scene.target, device=dev)
+        H, W, C = self.H, self.W, self.C
+        shapes = {"raster": (H, W, 4), "deriv": (H, W, 4), "uvs": (H, W, 2), "color": (H, W, C),
+                  "g_color": (H, W, C), "g_tex": (self.th, self.tw, C), "g_uvs": (H, W, 2), "g_uv": (self.V, 2),
+                  "g_raster": (H, W, 4), "g_pos": (self.V, 4)}
+        for k, s in shapes.items():
+            setattr(self, k, torch.empty(s, device=dev))
+        self._vp = (ctypes.c_float * 4)()
+
+    def step(self, stream) -> None:
+        import torch
+
+        lib, ck, h = _native.lib(), _native.check, self.ctx.handle
+        V, T, W, H, C, tw, th = self.V, self.T, self.W, self.H, self.C, self.tw, self.th
+        st = stream.cuda_stream
+        ck(lib.pdr_rasterize_forward(h, self.clip.data_ptr(), V, self.idx.data_ptr(), T, W, H, 0, 1, None,
+                                     self.raster.data_ptr(), self.deriv.data_ptr(), self._vp, st))
+        ck(lib.pdr_interpolate_forward(h, self.uv.data_ptr(), V, 2, self.idx.data_ptr(), T, self.raster.data_ptr(),
+                                       W, H, self.uvs.data_ptr(), st))
+        ck(lib.pdr_texture_forward(h, self.tex.data_ptr(), tw, th, C, 0, self.uvs.data_ptr(), self.raster.data_ptr(),
+                                   W, H, self.color.data_ptr(), st))
+        with torch.cuda.stream(stream):
+            torch.sub(self.color, self.target, alpha=1.0, out=self.g_color).mul_(2.0)
+        ck(lib.pdr_texture_backward(h, self.tex.data_ptr(), tw, th, C, 0, self.uvs.data_ptr(), self.raster.data_ptr(),
+                                    W, H, self.g_color.data_ptr(), 1, self.g_tex.data_ptr(), self.g_uvs.data_ptr(),
+                                    st))
+        ck(lib.pdr_interpolate_backward(h, self.uv.data_ptr(), V, 2, self.idx.data_ptr(), T, self.raster.data_ptr(),
+                                        W, H, self.g_uvs.data_ptr(), self.g_uv.data_ptr(), self.g_raster.data_ptr(),
+                                        st))
+        ck(lib.pdr_rasterize_backward(h, self.clip.data_ptr(), V, self.idx.data_ptr(), T, self.raster.data_ptr(), W,
+                                      H, self.g_raster.data_ptr(), None, self.g_pos.data_ptr(), st))
+
+    def capture(self, stream):
+        import torch
+
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph, stream=stream):
+            self.step(stream)
+        return graph
+
+    def outputs(self) -> dict:
+        return {k: getattr(self, k).cpu().numpy() for k in self.NAMES}
