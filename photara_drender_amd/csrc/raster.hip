@@ -1081,8 +1081,11 @@ __global__ __launch_bounds__(kBlock, 4) void k_corner_small(ViewTable vt, const 
                         float t9[9];
                         bool hit = false;
                         if (m < total) {
+                            // id and gradient loaded together: no dependent second load
                             const size_t pix = static_cast<size_t>(y0 + rl) * vd.width + static_cast<size_t>(x);
-                            if (cv[pix] == qt + 1u) hit = pixel_terms(q.p, sample_x(vd, x), syr, gr[pix * 2], t9);
+                            const uint32_t c = cv[pix];
+                            const float2 g = gr[pix * 2];
+                            if (c == qt + 1u) hit = pixel_terms(q.p, sample_x(vd, x), syr, g, t9);
                         }
                         unsigned long long mk = __ballot(hit);
                         while (mk) {
@@ -1143,11 +1146,14 @@ __global__ __launch_bounds__(kBlock, 4) void k_corner_small(ViewTable vt, const 
                 float t9[9];
                 bool hit = false;
                 if (m < npix) {
+                    // id and gradient loaded together: no dependent second load
                     const size_t pix = static_cast<size_t>(yy) * vd.width + static_cast<size_t>(x);
-                    if (cv[pix] == batch + static_cast<uint32_t>(oo) + 1u) {  // oo < bsz: a flat-class lane
+                    const uint32_t c = cv[pix];
+                    const float2 g = gr[pix * 2];
+                    if (c == batch + static_cast<uint32_t>(oo) + 1u) {  // oo < bsz: a flat-class lane
                         const uint32_t i = w0 + static_cast<uint32_t>(oo);
                         const float4 pp[3] = {R.p0[i], R.p1[i], R.p2[i]};
-                        hit = pixel_terms(pp, sample_x(vd, x), syy, gr[pix * 2], t9);
+                        hit = pixel_terms(pp, sample_x(vd, x), syy, g, t9);
                     }
                 }
                 const unsigned long long bal = __ballot(hit);
