@@ -38,6 +38,11 @@ HBM_PEAK_GBS = 8000.0
 TRAFFIC_PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
                                "raster_interp_16view_hbm_traffic_pmc.txt")
 COPY_BYTES = 1 << 30  # calibration copy of the traffic probe
+VALU_PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                            "raster_interp_16view_valu_pmc.txt")
+# VALU issue peak of MI355X: 256 CUs x 4 SIMDs x 2.4 GHz, one wave64 (non-packed) VALU
+# instruction per SIMD every 4 cycles (16 lanes per cycle)
+VALU_PEAK_GIPS = 256 * 4 * 2.4 / 4  # G wave-instructions per second (614.4)
 # library timing name -> HIP kernel whose counters it covers
 KERNEL_OF = {"interp_bwd": "k_interp_bwd", "interp_fwd": "k_interp_fwd", "raster_resolve": "k_raster_resolve",
              "raster_tiled": "k_tile_raster",
@@ -101,6 +106,50 @@ def traffic_summary(fetch_dir, write_dir, out_path, commands):
     with open(out_path, "w") as f:
         f.write("\n".join(lines + [""] + rows) + "\n")
     return out_path
+
+
+def valu_summary(insts_dir, busy_dir, out_path, commands):
+    """VALU work per launch from rocprofv3 --pmc SQ_INSTS_VALU / SQ_WAVES and VALUBusy /
+    VALUUtilization runs of the traffic probe (mean over dispatches)."""
+    per = {}
+    for d in (insts_dir, busy_dir):
+        for (disp, name), vals in _pmc_rows(d).items():
+            k = _short(name)
+            if k and k != "copy":
+                for c, v in vals.items():
+                    per.setdefault(k, {}).setdefault(c, []).append(v)
+    mean = lambda k, c: sum(per[k][c]) / len(per[k][c]) if per[k].get(c) else float("nan")
+    lines = ["VALU work per launch of the 16-view raster/interp fwd+bwd workload (200,000 triangles, 15,163,392 px,",
+             "A = 8), from rocprofv3 --pmc SQ_INSTS_VALU SQ_WAVES, then VALUBusy VALUUtilization (mean over",
+             "dispatches; the probe's first step is the autotuning pass, whose candidate launches do the same",
+             "work).  VALU issue peak: 256 CUs x 4 SIMDs x 2.4 GHz / 4 cycles per wave64 instruction",
+             f"= {VALU_PEAK_GIPS:.1f} G wave-instructions/s.", "", "commands:"] + ["  " + c for c in commands] + [
+             "", f"{'kernel':20s} {'dispatches':>10s} {'VALU insts':>14s} {'waves':>10s} {'VALUBusy %':>11s} "
+                 f"{'VALUUtil %':>11s}"]
+    rows = []
+    for k in sorted(per):
+        if "SQ_INSTS_VALU" not in per[k]:
+            continue
+        n = mean(k, "SQ_INSTS_VALU")
+        lines.append(f"{k:20s} {len(per[k]['SQ_INSTS_VALU']):10d} {n:14.0f} {mean(k, 'SQ_WAVES'):10.0f} "
+                     f"{mean(k, 'VALUBusy'):11.1f} {mean(k, 'VALUUtilization'):11.1f}")
+        rows.append(f"valu {k} {n:.0f}")
+    with open(out_path, "w") as f:
+        f.write("\n".join(lines + [""] + rows) + "\n")
+    return out_path
+
+
+def load_valu(kernel):
+    """VALU wave-instructions per launch of ``kernel`` from the committed --pmc summary (None if absent)."""
+    try:
+        with open(VALU_PROFILE) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 3 and parts[0] == "valu" and parts[1] == kernel:
+                    return float(parts[2])
+    except OSError:
+        pass
+    return None
 
 
 def trace_summary(directory, limit=40):
@@ -477,7 +526,17 @@ def main():
                     help="run 3 steps of the headline workload and a 1 GiB device copy (for rocprofv3 --pmc runs)")
     ap.add_argument("--traffic-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="write the HBM traffic summary from two rocprofv3 --pmc output folders")
+    ap.add_argument("--valu-summary", nargs=2, metavar=("INSTS_DIR", "BUSY_DIR"),
+                    help="write the VALU work summary from two rocprofv3 --pmc output folders")
     args = ap.parse_args()
+    if args.valu_summary:
+        cmds = [f"rocprofv3 --pmc SQ_INSTS_VALU SQ_WAVES --output-format csv -d {args.valu_summary[0]} -o pmc -- "
+                "python3 bench.py --traffic-probe",
+                f"rocprofv3 --pmc VALUBusy VALUUtilization --output-format csv -d {args.valu_summary[1]} -o pmc -- "
+                "python3 bench.py --traffic-probe",
+                f"python3 bench.py --valu-summary {args.valu_summary[0]} {args.valu_summary[1]}"]
+        print(valu_summary(args.valu_summary[0], args.valu_summary[1], VALU_PROFILE, cmds))
+        return
     if args.traffic_summary:
         cmds = [f"rocprofv3 --pmc FETCH_SIZE --output-format csv -d {args.traffic_summary[0]} -o pmc -- "
                 "python3 bench.py --traffic-probe",
@@ -594,6 +653,12 @@ def main():
     dominant = max((k for k in kernels if k in alg), key=lambda k: kernels[k])
     achieved = alg[dominant] / (kernels[dominant] * 1e-3) / 1e9
     traffic = load_traffic(KERNEL_OF.get(dominant, dominant))
+    valu = load_valu(KERNEL_OF.get(dominant, dominant))
+    # the raster's own fwd+bwd bytes (every raster boundary array once) over its summed kernels
+    raster_names = ("raster_tiled", "raster_candidates", "raster_resolve", "vertex_csr", "bwd_prepass",
+                    "corner_grads", "vertex_reduce")
+    raster_ms = sum(kernels.get(k, 0.0) for k in raster_names)
+    raster_bytes = 64 * P + n * (2 * (16 * V + 12 * T) + 16 * V)
     result = {
         "metric": "Mpix/s raster+interp fwd+bwd (16 uneven views, 200k tris, A=8)",
         "value": P / (ms_per_step * 1e-3) / 1e6,
@@ -616,7 +681,14 @@ def main():
                      "alg_bytes_per_launch": alg[dominant],
                      # measured HBM bytes per launch over the same launch time, against the same peak
                      "traffic_frac": None if traffic is None else traffic / (kernels[dominant] * 1e-3) / 1e9 /
-                     HBM_PEAK_GBS},
+                     HBM_PEAK_GBS,
+                     "raster_fwd_bwd_bytes": raster_bytes, "raster_fwd_bwd_ms": raster_ms,
+                     "raster_fwd_bwd_frac": raster_bytes / (raster_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     # the dominant kernel is VALU-bound: its VALU issue rate against the issue peak
+                     "valu": None if valu is None else {
+                         "insts_per_launch": valu, "achieved": valu / (kernels[dominant] * 1e-3) / 1e9,
+                         "peak": VALU_PEAK_GIPS, "unit": "G wave-instructions/s",
+                         "frac": valu / (kernels[dominant] * 1e-3) / 1e9 / VALU_PEAK_GIPS}},
         "cpu_baseline": None,
         "autotune": ctx.autotune_export().splitlines(),
     }
