@@ -218,12 +218,61 @@ def algorithmic_bytes(V, T, A, pixels):
     }
 
 
-def cpu_baseline(scene, views, budget_s=20.0):
-    """Oracle chain (bins + per-pixel raster, interp fwd, interp bwd with per-vertex
-    pixel lists, per-triangle raster bwd + CSR reduce) on a prefix of the views."""
+def cpu_threads():
+    """Threads for the CPU baseline: every core this process is given (OMP_NUM_THREADS
+    when the host sets it, else every core the OS reports)."""
+    return int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+
+
+def host_info():
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"host_cpus": os.cpu_count(), "cpu_model": model}
+
+
+def reference_attr_grad_sample(scene, budget_s=4.0):
+    """The reference's O(V*A*P) attribute-gradient loop (interpolate_grad_attributes.hlsl:
+    one output per (vertex, attribute) over every pixel) timed on view 0 for a slice of
+    the outputs sized to about ``budget_s``, extrapolated linearly in V*A*P to all views."""
     import oracle
 
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(os.cpu_count() or 1, 16)
+    V, A = scene.attributes.shape
+    res = scene.resolutions[0]
+    raster, _, _ = oracle.rasterize_forward(scene.clips[0], scene.indices, res, want_deriv=False)
+    out = oracle.interpolate_forward(scene.attributes, scene.indices, raster)
+    g = (out - scene.targets[0]).astype(np.float32)
+    n = 16 * A
+    while True:
+        t0 = time.perf_counter()
+        oracle.interpolate_grad_attributes_reference(V, A, scene.indices, raster, g, range(0, n))
+        dt = time.perf_counter() - t0
+        if dt > budget_s / 4 or n >= V * A:
+            break
+        n = min(V * A, int(n * max(2.0, budget_s / max(dt, 1e-3))))
+    P0 = res[0] * res[1]
+    per_iter = dt / (n * P0)
+    total_iters = V * A * sum(h * w for h, w in scene.resolutions)
+    return {"outputs_timed": n, "pixels": P0, "seconds": dt, "ns_per_iteration": per_iter * 1e9,
+            "iterations_all_views": total_iters, "extrapolated_seconds": per_iter * total_iters,
+            "sample": f"outputs 0-{n - 1} of {V * A} (vertex x attribute) over view 0 ({P0} px), extrapolated "
+                      "linearly in V*A*P to the 16 views"}
+
+
+def cpu_baseline(scene, views, budget_s=20.0):
+    """Oracle chain (bins + per-pixel raster, interp fwd, interp bwd with per-vertex
+    pixel lists, per-triangle raster bwd + CSR reduce) on a prefix of the views, and
+    beside it the reference's O(V*A*P) attribute gradient, timed on a slice and
+    extrapolated (the whole reference chain at this size is infeasible on a CPU)."""
+    import oracle
+
+    threads = cpu_threads()
     oracle.set_num_threads(threads)
     done_px, t_total, used = 0, 0.0, 0
     for k in range(len(scene.clips)):
@@ -241,6 +290,9 @@ def cpu_baseline(scene, views, budget_s=20.0):
         used += 1
         if t_total > budget_s:
             break
+    ref = reference_attr_grad_sample(scene)
+    P = sum(h * w for h, w in scene.resolutions)
+    port_all_s = t_total * P / done_px
     return {
         "value": done_px / t_total / 1e6,
         "unit": "Mpix/s",
@@ -248,6 +300,13 @@ def cpu_baseline(scene, views, budget_s=20.0):
         "kind": "port",
         "sample": f"views 0-{used - 1} of the 16-view workload ({done_px} px), full fwd+bwd chain; the attribute "
                   "gradient uses per-vertex pixel lists (the reference's sums in its order, not its O(V*A*P) loop)",
+        **host_info(),
+        # the reference's own attribute-gradient loop in place of the pixel lists: extrapolated
+        "reference_attr_grad": ref,
+        "reference_chain_extrapolated": {
+            "value": P / (port_all_s + ref["extrapolated_seconds"]) / 1e6, "unit": "Mpix/s", "extrapolated": True,
+            "basis": "the port chain over all views (scaled from the timed views) plus the extrapolated O(V*A*P) "
+                     "attribute gradient (the port's own attribute gradient is counted too: an upper bound)"},
     }
 
 
@@ -355,8 +414,7 @@ def chain_benchmark(steps=200, warmup=5, cpu=True):
     if cpu:
         import oracle
 
-        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(os.cpu_count() or 1, 16)
-        oracle.set_num_threads(threads)
+        oracle.set_num_threads(cpu_threads())
         times = []
         for _ in range(3):
             t0 = time.perf_counter()
@@ -364,12 +422,13 @@ def chain_benchmark(steps=200, warmup=5, cpu=True):
             u = oracle.interpolate_forward(sc.uv, sc.indices, r)
             c = oracle.texture_forward(sc.texture, u, r)
             gt, gu = oracle.texture_backward(sc.texture, u, r, (2.0 * (c - sc.target)).astype(np.float32))
-            _, gr = oracle.interpolate_backward(sc.uv, sc.indices, r, gu)
+            _, gr = oracle.interpolate_backward(sc.uv, sc.indices, r, gu, reference_mode=True)
             oracle.rasterize_backward(sc.clip, sc.indices, r, gr)
             times.append(time.perf_counter() - t0)
         out["cpu_baseline"] = {"value": H * W / float(np.median(times)) / 1e6, "unit": "Mpix/s",
-                               "cores": oracle.num_threads(), "kind": "port",
-                               "sample": "the whole chain, median of 3 runs (loss included)"}
+                               "cores": oracle.num_threads(), "kind": "port", **host_info(),
+                               "sample": "the whole chain, median of 3 runs (loss included), the attribute "
+                                         "gradient by the reference's O(V*A*P) loop"}
     return out
 
 
@@ -472,8 +531,7 @@ def refine_cpu_estimate(prob, steps, views=2, sample_steps=4):
     steps (batch 4) timed, extrapolated to 64 views and steps + 30 seam steps."""
     import oracle
 
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(os.cpu_count() or 1, 16)
-    oracle.set_num_threads(threads)
+    oracle.set_num_threads(cpu_threads())
     pos = prob.positions.cpu().numpy()
     uv = prob.uv.cpu().numpy()
     idx = prob.indices.cpu().numpy().view(np.uint32)
@@ -500,7 +558,7 @@ def refine_cpu_estimate(prob, steps, views=2, sample_steps=4):
                            0.0, 1.0)
     per_step = (time.perf_counter() - t0) / sample_steps
     return {"value": pre * len(prob.images) + per_step * (steps + 30), "unit": "s", "cores": oracle.num_threads(),
-            "kind": "port", "sample": f"precompute of {views} views and {sample_steps} optimisation steps "
+            "kind": "port", **host_info(), "extrapolated": True, "sample": f"precompute of {views} views and {sample_steps} optimisation steps "
                                       "(photometric gradient + Adam) timed, extrapolated to 64 views and "
                                       f"{steps}+30 steps; extension terms (TV, joint seam) not included"}
 

@@ -132,6 +132,20 @@ def interpolate_backward(attr, idx, raster, grad, reference_mode=False):
     return ga, gr
 
 
+def interpolate_grad_attributes_reference(V, A, idx, raster, grad, outputs=None):
+    """The reference's O(V*A*P) attribute-gradient loop (one output per (vertex,
+    attribute), every pixel visited) for the flattened outputs in ``outputs``
+    (a range, default all V*A); returns them as a flat float32 array."""
+    idx, raster, grad = _u32(idx), _f32(raster), _f32(grad)
+    o = range(V * A) if outputs is None else outputs
+    out = np.empty(max(len(o), 0), np.float32)
+    P = raster.shape[0] * raster.shape[1] if raster.ndim == 3 else raster.shape[0]
+    lib().orc_interpolate_grad_attributes_range(
+        ctypes.c_uint32(V), ctypes.c_uint32(A), _p(idx, _U32), ctypes.c_uint32(idx.shape[0]), _p(raster, _F32),
+        ctypes.c_uint32(P), _p(grad, _F32), ctypes.c_uint64(o.start), ctypes.c_uint64(o.stop), _p(out, _F32))
+    return out
+
+
 ADDRESS_MODES = {"clamp": 0, "wrap": 1, "mirror": 2}
 
 

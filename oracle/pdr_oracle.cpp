@@ -396,27 +396,38 @@ int orc_interpolate_forward(const float* attr, uint32_t V, uint32_t A, const uin
 }
 
 // shaders/interpolate_grad_attributes.hlsl:15-58 + interpolate_grad_raster.hlsl:16-54
+// shaders/interpolate_grad_attributes.hlsl:15-58: one invocation per output
+// o = vertex * A + attribute, looping over every pixel; outputs [o_begin, o_end)
+// only (the CPU baseline times a slice and extrapolates).
+void orc_interpolate_grad_attributes_range(uint32_t V, uint32_t A, const uint32_t* idx, uint32_t T,
+                                           const float* raster, uint32_t P, const float* grad, uint64_t o_begin,
+                                           uint64_t o_end, float* out) {
+    const uint64_t end = std::min<uint64_t>(o_end, static_cast<uint64_t>(V) * A);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t o = static_cast<int64_t>(o_begin); o < static_cast<int64_t>(end); ++o) {
+        const uint32_t v = static_cast<uint32_t>(o / A), a = static_cast<uint32_t>(o % A);
+        float g = 0.0f;
+        for (uint32_t p = 0; p < P; ++p) {
+            const uint32_t id = decode_id(raster[static_cast<size_t>(p) * 4 + 3]);
+            if (id == 0 || id > T) continue;
+            const uint32_t* tri = idx + static_cast<size_t>(id - 1) * 3;
+            const float u = raster[static_cast<size_t>(p) * 4], w = raster[static_cast<size_t>(p) * 4 + 1];
+            float weight = 0.0f;
+            if (tri[0] == v) weight += u;
+            if (tri[1] == v) weight += w;
+            if (tri[2] == v) weight += 1.0f - u - w;
+            g += weight * grad[static_cast<size_t>(p) * A + a];
+        }
+        out[o - static_cast<int64_t>(o_begin)] = g;
+    }
+}
+
 int orc_interpolate_backward(const float* attr, uint32_t V, uint32_t A, const uint32_t* idx, uint32_t T,
                              const float* raster, uint32_t P, const float* grad, float* grad_attr,
                              float* grad_raster, int reference_mode) {
     if (reference_mode) {
-#pragma omp parallel for schedule(dynamic, 1)
-        for (int64_t o = 0; o < static_cast<int64_t>(V) * A; ++o) {
-            const uint32_t v = static_cast<uint32_t>(o / A), a = static_cast<uint32_t>(o % A);
-            float g = 0.0f;
-            for (uint32_t p = 0; p < P; ++p) {
-                const uint32_t id = decode_id(raster[static_cast<size_t>(p) * 4 + 3]);
-                if (id == 0 || id > T) continue;
-                const uint32_t* tri = idx + static_cast<size_t>(id - 1) * 3;
-                const float u = raster[static_cast<size_t>(p) * 4], w = raster[static_cast<size_t>(p) * 4 + 1];
-                float weight = 0.0f;
-                if (tri[0] == v) weight += u;
-                if (tri[1] == v) weight += w;
-                if (tri[2] == v) weight += 1.0f - u - w;
-                g += weight * grad[static_cast<size_t>(p) * A + a];
-            }
-            grad_attr[o] = g;
-        }
+        orc_interpolate_grad_attributes_range(V, A, idx, T, raster, P, grad, 0, static_cast<uint64_t>(V) * A,
+                                              grad_attr);
     } else {
         // pixel lists per vertex, ascending pixel order (each pixel once per distinct vertex)
         std::vector<uint32_t> count(static_cast<size_t>(V) + 1, 0);

@@ -61,6 +61,12 @@ int orc_interpolate_backward(const float* attr, uint32_t V, uint32_t A,
                              const float* raster, uint32_t P, const float* grad,
                              float* grad_attr, float* grad_raster, int reference_mode);
 
+/* shaders/interpolate_grad_attributes.hlsl:15-58 for outputs o = vertex * A +
+ * attribute in [o_begin, o_end) only, written to out[o - o_begin]. */
+void orc_interpolate_grad_attributes_range(uint32_t V, uint32_t A, const uint32_t* idx, uint32_t T,
+                                           const float* raster, uint32_t P, const float* grad,
+                                           uint64_t o_begin, uint64_t o_end, float* out);
+
 /* src/rasterizer.cpp:685-735 + shaders/texture_forward.hlsl:16-79 */
 int orc_texture_forward(const float* tex, uint32_t tw, uint32_t th, uint32_t C,
                         uint32_t address_mode, const float* uv, const float* raster,

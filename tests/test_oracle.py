@@ -172,6 +172,12 @@ def test_interpolate_adjoint_and_reference_mode():
     fast_a, fast_r = oracle.interpolate_backward(x, s.indices, r, y, reference_mode=False)
     np.testing.assert_array_equal(ref_a, fast_a)  # same sums, same order
     np.testing.assert_array_equal(ref_r, fast_r)
+    # the ranged O(V*A*P) loop (the CPU baseline's timed slice) gives the same outputs
+    V = x.shape[0]
+    part = oracle.interpolate_grad_attributes_reference(V, A, s.indices, r, y, range(7 * A + 2, 40 * A + 3))
+    np.testing.assert_array_equal(part, ref_a.ravel()[7 * A + 2:40 * A + 3])
+    np.testing.assert_array_equal(oracle.interpolate_grad_attributes_reference(V, A, s.indices, r, y),
+                                  ref_a.ravel())
 
 
 def test_interpolate_grad_raster_finite_difference():
