@@ -18,7 +18,8 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "_build", "libpdr_oracle.so")
+# PDR_ORACLE_LIB names another build of the same source (the sanitizer build)
+_LIB_PATH = os.environ.get("PDR_ORACLE_LIB") or os.path.join(_HERE, "_build", "libpdr_oracle.so")
 _lib = None
 
 _F32 = ctypes.POINTER(ctypes.c_float)

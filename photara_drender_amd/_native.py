@@ -10,7 +10,8 @@ import ctypes
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libpdr.so")
+# PDR_LIB names another build of the same library (the host-sanitizer build)
+LIB_PATH = os.environ.get("PDR_LIB") or os.path.join(_HERE, "libpdr.so")
 
 _F = ctypes.c_float
 _U32 = ctypes.c_uint32

@@ -3,7 +3,9 @@
 * ``libpdr.so``            hipcc, gfx950: HIP kernels + the C ABI of ``include/pdr.h``;
 * ``_photara_drender*.so`` g++, pybind11: the ``_photara_drender`` Python module
   (numpy API of the reference's nanobind module) over the C++ facade in
-  ``include/photara_drender/``, linked against ``libpdr.so``.
+  ``include/photara_drender/``, linked against ``libpdr.so``;
+* ``libpdr_hostsan.so``     the same library with its host-side C++ built under
+  AddressSanitizer + UndefinedBehaviorSanitizer (for the sanitizer tests).
 
 Both land next to this file so they travel with the source tree.
 """
@@ -60,6 +62,35 @@ def build_libpdr(jobs: int = 8) -> str:
     return LIBPDR
 
 
+# Host-side AddressSanitizer + UndefinedBehaviorSanitizer build of the C-ABI
+# marshalling (the .cpp sources: argument checks, staging, refine/bake/padding
+# drivers), linked with the same kernels.  Each -fsanitize= applies to the host
+# compilation only (-Xarch_host); device code is built as in libpdr.so.  Loaded
+# by the sanitizer tests with the ASan and UBSan runtimes preloaded.
+HOST_SAN_FLAGS = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+                  "-Xarch_host", "-fno-sanitize-recover=undefined", "-Xarch_host", "-fno-omit-frame-pointer", "-g"]
+LIBPDR_HOSTSAN = os.path.join(HERE, "libpdr_hostsan.so")
+
+
+def build_hostsan(jobs: int = 8) -> str:
+    san = os.path.join(BUILD, "hostsan")
+    os.makedirs(san, exist_ok=True)
+    host_sources = [s for s in HIP_SOURCES if s.endswith(".cpp")]
+
+    def compile_one(src):
+        obj = os.path.join(san, os.path.splitext(src)[0] + ".o")
+        _run([HIPCC, *HIP_FLAGS, *HOST_SAN_FLAGS, "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj])
+        return obj
+
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        objs = list(ex.map(compile_one, host_sources))
+    kernels = [os.path.join(BUILD, os.path.splitext(s)[0] + ".o") for s in HIP_SOURCES if not s.endswith(".cpp")]
+    tmp = LIBPDR_HOSTSAN + ".tmp"
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, *kernels, "-o", tmp])
+    os.replace(tmp, LIBPDR_HOSTSAN)
+    return LIBPDR_HOSTSAN
+
+
 def build_pymodule() -> str:
     import pybind11
 
@@ -79,6 +110,7 @@ def build_pymodule() -> str:
 def build_all() -> None:
     build_libpdr()
     build_pymodule()
+    build_hostsan()
 
 
 if __name__ == "__main__":
