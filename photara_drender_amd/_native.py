@@ -132,6 +132,8 @@ EXPORTED_SYMBOLS = tuple(_SIGNATURES)
 _TESTING_SIGNATURES = {
     "pdr_testing_candidate_region": (_I, [_P, _U32, _U32, _P, _U32, _I, _P, _P]),
     "pdr_testing_tv_gradient": (_I, [_P, _U32, _U32, _P, _F, _F, _P]),
+    "pdr_testing_set_pad_poll_limit": (None, [_U32]),
+    "pdr_testing_pad_fallbacks": (ctypes.c_uint64, []),
 }
 
 _lib = None

@@ -92,8 +92,10 @@ __device__ __forceinline__ uint32_t* word_at(uint32_t* base, uint32_t i) {
 // carry the tag: a word is written whole, so no fence orders data and flag,
 // and no L2 is written back or invalidated.  The caller clears `edge` (tag 0)
 // before the sweeps and gives each sweep its own non-zero tag.  A wait gives
-// up after a bounded number of polls (status[0] = 1, and every later wait
-// returns at once), so a launch always ends.  Inputs are loaded kChunk steps
+// up after max_polls polls (status[0] = 1, and every later wait returns at
+// once), so a launch always ends; the field is then recomputed by launching
+// one band at a time in order (band0 = the band, one workgroup), where every
+// wait finds the band above complete.  Inputs are loaded kChunk steps
 // ahead into a register ring; every lane issues the same load and store each
 // step (clamped addresses, the discard slot).
 constexpr uint32_t kBandRows = 64;
@@ -103,8 +105,9 @@ constexpr uint32_t kMaxPolls = 1u << 20;
 template <bool kMirror>
 __global__ __launch_bounds__(64) void k_pad_band_sweep(uint32_t W, uint32_t H, const uint8_t* __restrict__ source,
                                                         uint32_t* __restrict__ field, unsigned long long* edge,
-                                                        uint32_t tag, uint32_t* status) {
-    const uint32_t band = blockIdx.x, lane = threadIdx.x;
+                                                        uint32_t tag, uint32_t* status, uint32_t band0,
+                                                        uint32_t max_polls) {
+    const uint32_t band = band0 + blockIdx.x, lane = threadIdx.x;
     const uint32_t y = band * kBandRows + lane;  // row in sweep (possibly mirrored) coordinates
     const uint32_t rows = H - band * kBandRows < kBandRows ? H - band * kBandRows : kBandRows;
     const uint32_t steps = W + 2u * (rows - 1u);
@@ -139,7 +142,7 @@ __global__ __launch_bounds__(64) void k_pad_band_sweep(uint32_t W, uint32_t H, c
                 if (__all(!want || (v >> 32) == tag)) break;
                 const uint32_t failed = __builtin_amdgcn_readfirstlane(
                     __hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-                if (++polls >= kMaxPolls || failed != 0u) {
+                if (++polls >= max_polls || failed != 0u) {
                     if (lane == 0) __hip_atomic_store(status, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     break;
                 }
@@ -242,15 +245,33 @@ hipError_t pad_prepare(uint32_t P, const float* valid, const float* coverage, fl
 uint32_t pad_band_count(uint32_t H) { return (H + kBandRows - 1u) / kBandRows; }
 
 hipError_t pad_field_bands(uint32_t W, uint32_t H, const uint8_t* source, uint32_t* field, unsigned long long* edge,
-                           uint32_t* status, hipStream_t st) {
+                           uint32_t* status, uint32_t max_polls, hipStream_t st) {
     const uint32_t bands = pad_band_count(H);
+    const uint32_t polls = max_polls ? max_polls : kMaxPolls;
     hipError_t e = fill_u32(status, 0u, 1, st);
     if (e != hipSuccess) return e;
     if ((e = fill_u64(edge, 0ull, static_cast<size_t>(bands) * W, st)) != hipSuccess) return e;  // tag 0: nothing yet
-    k_pad_band_sweep<false><<<bands, 64, 0, st>>>(W, H, source, field, edge, 1u, status);
+    k_pad_band_sweep<false><<<bands, 64, 0, st>>>(W, H, source, field, edge, 1u, status, 0u, polls);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    k_pad_band_sweep<true><<<bands, 64, 0, st>>>(W, H, source, field, edge, 2u, status);
+    k_pad_band_sweep<true><<<bands, 64, 0, st>>>(W, H, source, field, edge, 2u, status, 0u, polls);
     return hipGetLastError();
+}
+
+hipError_t pad_field_bands_ordered(uint32_t W, uint32_t H, const uint8_t* source, uint32_t* field,
+                                   unsigned long long* edge, uint32_t* status, hipStream_t st) {
+    const uint32_t bands = pad_band_count(H);
+    hipError_t e = fill_u32(status, 0u, 1, st);
+    if (e != hipSuccess) return e;
+    // fresh tags (3, 4): words left by an abandoned parallel sweep never pass for this one's
+    for (uint32_t b = 0; b < bands; ++b) {
+        k_pad_band_sweep<false><<<1, 64, 0, st>>>(W, H, source, field, edge, 3u, status, b, kMaxPolls);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    for (uint32_t b = 0; b < bands; ++b) {
+        k_pad_band_sweep<true><<<1, 64, 0, st>>>(W, H, source, field, edge, 4u, status, b, kMaxPolls);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 hipError_t pad_fill(uint32_t W, uint32_t H, uint32_t C, const uint32_t* field, const uint8_t* source,

@@ -356,10 +356,15 @@ hipError_t pad_prepare(uint32_t P, const float* valid, const float* coverage, fl
 // The 8SSEDT field (field: W * H + 1 entries, the last written as a discard
 // slot), rows in bands of 64 on separate workgroups handing their last rows on
 // through edge [pad_band_count(H) * W] (tagged 64-bit words), status[1] (set to
-// 1 when a band gave up waiting for the band above; the field is then invalid).
+// 1 when a band gave up waiting for the band above, after max_polls polls (0:
+// the default bound); the field is then invalid).
 uint32_t pad_band_count(uint32_t H);
 hipError_t pad_field_bands(uint32_t W, uint32_t H, const uint8_t* source, uint32_t* field, unsigned long long* edge,
-                           uint32_t* status, hipStream_t st);
+                           uint32_t* status, uint32_t max_polls, hipStream_t st);
+// The same field with one launch per band, in order (no band waits on a band
+// still running): the fallback when pad_field_bands gave up.
+hipError_t pad_field_bands_ordered(uint32_t W, uint32_t H, const uint8_t* source, uint32_t* field,
+                                   unsigned long long* edge, uint32_t* status, hipStream_t st);
 hipError_t pad_fill(uint32_t W, uint32_t H, uint32_t C, const uint32_t* field, const uint8_t* source,
                     const float* coverage, uint32_t margin, int fill_unobserved, uint32_t max_fill,
                     const float* src_color, float* color, float* filled, unsigned long long* counters, hipStream_t st);

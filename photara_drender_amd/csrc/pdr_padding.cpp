@@ -1,11 +1,19 @@
 // pdr_pad_texture_atlas: pad_texture_atlas (src/texture_padding.cpp:226-351)
 // with its TexturePaddingOptions / TexturePaddingStats
 // (include/photara_drender/texture_padding.hpp:33-57); kernels in padding.hip.
+#include <atomic>
 #include <chrono>
 
 #include "pdr_host.hpp"
 
 using namespace pdrh;
+
+namespace {
+// Test hooks: a poll bound for the parallel band sweep (0: the kernel's own
+// default) and the number of times the ordered fallback ran.
+std::atomic<uint32_t> g_pad_poll_limit{0};
+std::atomic<uint64_t> g_pad_fallbacks{0};
+}  // namespace
 
 extern "C" {
 
@@ -70,12 +78,25 @@ pdr_status pdr_pad_texture_atlas(pdr_context ctx, float* color, uint32_t width, 
             unsigned long long* edge = ctx->get_t<unsigned long long>("pad_band_edge", static_cast<size_t>(bands) * W);
             uint32_t* status = ctx->get_t<uint32_t>("pad_band_status", 1);
             ctx->timed("pad_field", st, [&] {
-                hip_check(pdr::pad_field_bands(W, H, source, field, edge, status, st), "pad_field_bands");
+                hip_check(pdr::pad_field_bands(W, H, source, field, edge, status, g_pad_poll_limit.load(), st),
+                          "pad_field_bands");
             });
             uint32_t gave_up = 0;
             hip_check(hipMemcpyAsync(&gave_up, status, sizeof(gave_up), hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
             sync(st);
-            if (gave_up) raise(PDR_RUNTIME, "texture padding: a row band gave up waiting for the band above");
+            if (gave_up) {
+                // a band waited too long (the GPU time-sliced with other work, or bands
+                // dispatched out of order): the same sweeps one band per launch, in order
+                g_pad_fallbacks.fetch_add(1);
+                ctx->timed("pad_field_ordered", st, [&] {
+                    hip_check(pdr::pad_field_bands_ordered(W, H, source, field, edge, status, st),
+                              "pad_field_bands_ordered");
+                });
+                hip_check(hipMemcpyAsync(&gave_up, status, sizeof(gave_up), hipMemcpyDeviceToHost, st),
+                          "hipMemcpyAsync");
+                sync(st);
+                if (gave_up) raise(PDR_RUNTIME, "texture padding: a row band found the band above incomplete");
+            }
             // A far sentinel offset can land inside atlases wider or taller than it: read colours from a copy then.
             const float* src_color = dcolor;
             if (W > 12000u || H > 12000u) {
@@ -106,5 +127,9 @@ pdr_status pdr_pad_texture_atlas(pdr_context ctx, float* color, uint32_t width, 
         }
     });
 }
+
+// Test hooks (outside the public header).
+void pdr_testing_set_pad_poll_limit(uint32_t polls) { g_pad_poll_limit.store(polls); }
+uint64_t pdr_testing_pad_fallbacks(void) { return g_pad_fallbacks.load(); }
 
 }  // extern "C"

@@ -14,8 +14,9 @@ CPU, pinning the oracle:
 * the unobserved fill honours maximum_fill_distance and fill_unobserved.
 GPU, through the C ABI: colour bits, filled mask and statistics equal the
 oracle on random masks of many shapes (odd widths, single rows and columns,
-atlases wider than 8192 and 12000 texels), and the package's numpy entry
-point behaves as python/photara_drender/baking.py:33-80.
+atlases wider than 8192 and 12000 texels), also when the parallel band
+sweep gives up waiting and the ordered fallback computes the field, and the
+package's numpy entry point behaves as python/photara_drender/baking.py:33-80.
 """
 import numpy as np
 import pytest
@@ -174,6 +175,37 @@ def test_gpu_padding_matches_oracle(shape):
         if filled is not None:
             np.testing.assert_array_equal(gf, rf, err_msg=f"case {k}")
         assert gs == rs, (k, gs, rs)
+
+
+@pytest.mark.gpu
+def test_gpu_padding_falls_back_when_a_band_gives_up_waiting():
+    """The parallel band sweep's bounded wait forced to give up (a poll limit of
+    1 set from the host through a test hook; every kernel still exits): the
+    field is recomputed one band per launch, in order, and colour bits, filled
+    mask and statistics equal the oracle's and the parallel path's."""
+    from photara_drender_amd import _native
+
+    lib = _native.lib()
+    ctx = _native.Context(0, flags=0)
+    rng = np.random.default_rng(77)
+    H, W = 517, 300  # nine bands of 64 rows
+    col, val, cov, filled = _case(rng, H, W, 4, 0.01)
+    rc, rf, rs = oracle.pad_texture_atlas(col, val, 6, cov, filled, True, 0)
+    pc, pf, ps = _gpu_pad(ctx, col, val, cov, filled, 6, True, 0)
+    before = lib.pdr_testing_pad_fallbacks()
+    lib.pdr_testing_set_pad_poll_limit(1)
+    try:
+        fc, ff, fs = _gpu_pad(ctx, col, val, cov, filled, 6, True, 0)
+    finally:
+        lib.pdr_testing_set_pad_poll_limit(0)
+    assert lib.pdr_testing_pad_fallbacks() == before + 1  # the wait gave up and the ordered sweep ran
+    for c, f, st in ((pc, pf, ps), (fc, ff, fs)):
+        np.testing.assert_array_equal(c.view(np.uint32), rc.view(np.uint32))
+        np.testing.assert_array_equal(f, rf)
+        assert st == rs
+    # the default bound again: no fallback
+    _gpu_pad(ctx, col, val, cov, filled, 6, True, 0)
+    assert lib.pdr_testing_pad_fallbacks() == before + 1
 
 
 @pytest.mark.gpu
