@@ -470,24 +470,59 @@ def refine_benchmark(steps=1000, checkpoint_every=25, cpu=True):
         "psnr_final_db": plain["final_psnr_db"], "psnr_final_vs_noisy_db": final_noisy,
         "psnr_checkpoints_db": plain["psnr_checkpoints_db"], "checkpoint_every": checkpoint_every,
         "steps_to_final_minus_0.1db": plain["steps_to_target"],
+        # wall time of bake + precompute + a run stopped at that checkpoint (its PSNR beside it)
         "seconds_to_final_minus_0.1db": plain["seconds_to_target"],
+        "seconds_to_final_minus_0.1db_estimate": plain["seconds_to_target_estimate"],
+        "psnr_at_stop_db": plain["psnr_at_stop_db"], "target_psnr_db": ev["target_psnr_db"],
         "problem_build_s": build_s,
+        "reference_objective": reference_objective_run(ctx, prob),
         "warm_start": {
             "method": "features (top-8 samples) -> MLP 40-64-64-3 (bf16 MFMA) -> 8-texel padding -> refinement "
                       "over a 250-step schedule started at lr 5e-3 (cosine to 2.5e-4)",
             "init_s": warm["init_s"], "psnr_initial_db": warm["init_psnr_db"], "psnr_final_db": warm["final_psnr_db"],
             "steps_to_target": warm["steps_to_target"], "seconds_to_target": warm["seconds_to_target"],
+            "seconds_to_target_estimate": warm["seconds_to_target_estimate"],
+            "psnr_at_stop_db": warm["psnr_at_stop_db"],
             "speedup_time_to_target": ev["speedup_time_to_target"],
             "psnr_checkpoints_db": warm["psnr_checkpoints_db"],
             "ablation_plain_init_same_schedule": {
                 "steps_to_target": abl["steps_to_target"], "seconds_to_target": abl["seconds_to_target"],
                 "speedup_time_to_target": ev["ablation_speedup"], "psnr_final_db": abl["final_psnr_db"]},
+            "network_part": network_part(warm, abl),
         },
         "mlp": mlp_benchmark(ctx, prob, mlp),
     }
     if cpu:
         out["cpu_baseline"] = refine_cpu_estimate(prob, steps)
     return out
+
+
+def network_part(warm, abl):
+    """What the MLP itself adds: the warm start against the same schedule from the
+    plain initial atlas (steps and wall seconds to the target)."""
+    ws, ab = warm["steps_to_target"], abl["steps_to_target"]
+    if ws is None or ab is None:
+        verdict = "unmeasured (an arm did not reach the target)"
+    elif ws < ab:
+        verdict = f"the network saves {ab - ws} steps to the target"
+    else:
+        verdict = ("unmet: the network saves no steps over the plain initial atlas under the same schedule; "
+                   "the time-to-target gain comes from the shorter, gentler schedule")
+    return {"warm_steps": ws, "ablation_steps": ab, "warm_seconds": warm["seconds_to_target"],
+            "ablation_seconds": abl["seconds_to_target"], "verdict": verdict}
+
+
+def reference_objective_run(ctx, prob, steps=1000):
+    """The reference's own objective at full size (no chart TV, no joint seam term: the
+    bit-exact parity mode, Adam as atlas_adam_update.hlsl writes it), timed, with PSNR."""
+    from photara_drender_amd import warmstart, workloads
+
+    init = workloads.bake_initial_atlas(ctx, prob)[0]
+    r = warmstart.refine_problem_atlas(ctx, prob, init, steps=steps, tv_weight=0.0, chart_ids=None,
+                                       joint_seam_weight=0.0)
+    return {"steps": steps, "precompute_s": r.precompute_seconds, "optimize_s": r.optimization_seconds,
+            "ms_per_step": r.optimization_seconds / (steps + 30) * 1e3,
+            "psnr_final_db": workloads.atlas_psnr(ctx, prob, r.color)}
 
 
 def mlp_benchmark(ctx, prob, mlp, reps=20):

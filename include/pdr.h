@@ -235,6 +235,9 @@ typedef struct pdr_refine_options {
     uint32_t use_graph;        /* capture the step loop as a hipGraph (default behaviour of pdr_refine_options_default) */
     uint32_t checkpoint_every; /* >0: copy the atlas to checkpoints every k steps */
     float* checkpoints;        /* [steps/checkpoint_every, height, width, 3] host or device, or NULL */
+    uint32_t stop_after;       /* >0: run only the first stop_after steps of the `steps`-step schedule (its
+                                  learning rates and view order unchanged), then the seam polish; loss_out
+                                  entries past it stay 0 (a run that stops at a target checkpoint) */
 } pdr_refine_options;
 
 void pdr_refine_options_default(pdr_refine_options* options);

@@ -53,7 +53,7 @@ class RefineOptions(ctypes.Structure):
                 ("adam_epsilon", _F), ("clamp_min", _F), ("clamp_max", _F), ("seam_polish_steps", _U32),
                 ("seam_learning_rate", _F), ("seam_epsilon", _F), ("tv_weight", _F), ("tv_epsilon", _F),
                 ("chart_ids", _P), ("joint_seam_weight", _F), ("use_graph", _U32), ("checkpoint_every", _U32),
-                ("checkpoints", _P)]
+                ("checkpoints", _P), ("stop_after", _U32)]
 
 
 class BakeView(ctypes.Structure):

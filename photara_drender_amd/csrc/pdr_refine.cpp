@@ -443,7 +443,8 @@ pdr_status pdr_refine(pdr_context ctx, const float* initial_texture, const float
             hip_check(hipStreamEndCapture(st, &g.graph), "hipStreamEndCapture");
             hip_check(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0), "hipGraphInstantiate");
         };
-        const uint32_t nblocks = o.steps / block, rest = o.steps % block;
+        const uint32_t run = o.stop_after && o.stop_after < o.steps ? o.stop_after : o.steps;  // steps executed
+        const uint32_t nblocks = run / block, rest = run % block;
         const uint32_t flips = fused_tv ? 1u : 0u;
         GraphExec full[2], tail;
         if (o.use_graph) {
@@ -463,7 +464,7 @@ pdr_status pdr_refine(pdr_context ctx, const float* initial_texture, const float
                 for (uint32_t k = 0; k < count; ++k) cur = one_step(st, cur);
             }
             done += count;
-            if (done - folded == parts.period || done == o.steps) {
+            if (done - folded == parts.period || done == run) {
                 hip_check(pdr::loss_fold(parts, folded, done - folded, loss, st), "loss_fold");
                 folded = done;
             }

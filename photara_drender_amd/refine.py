@@ -50,11 +50,13 @@ def refine_atlas(context, initial_texture, positions, uv, indices, images: Seque
                  learning_rate=5e-3, minimum_learning_rate=2.5e-4, photometric_epsilon=1e-3, seam_polish_steps=30,
                  seam_learning_rate=1e-3, seam_epsilon=1e-4, adam_beta1=0.9, adam_beta2=0.999, adam_epsilon=1e-8,
                  clamp_min=0.0, clamp_max=1.0, tv_weight=0.0, tv_epsilon=1e-4, chart_ids=None,
-                 joint_seam_weight=0.0, use_graph=True, checkpoint_every=0) -> RefineResult:
+                 joint_seam_weight=0.0, use_graph=True, checkpoint_every=0, stop_after=0) -> RefineResult:
     """Runs pdr_refine.  Arrays may be numpy (host) or CUDA tensors (device).
 
     ``context`` is a ``_native.Context`` or an object with ``context_handle``.
     ``chart_ids`` [H, W] int32 (< 0 = texel outside every chart) feeds the chart TV term.
+    ``stop_after`` > 0 runs only the first ``stop_after`` steps of the ``steps``-step
+    schedule (then the seam polish).
     """
     handle = context.handle if hasattr(context, "handle") else context.context_handle
     tex = _contig(initial_texture, np.float32)
@@ -97,6 +99,7 @@ def refine_atlas(context, initial_texture, positions, uv, indices, images: Seque
     o.joint_seam_weight = joint_seam_weight
     o.use_graph = int(bool(use_graph))
     o.checkpoint_every = int(checkpoint_every)
+    o.stop_after = int(stop_after)
     device = hasattr(tex, "data_ptr")
     ckpt = None
     if checkpoint_every:

@@ -113,6 +113,30 @@ def test_refine_graph_matches_eager_and_checkpoints():
     assert not torch.equal(a.checkpoints[-1], a.color)
 
 
+@pytest.mark.parametrize("tv", [0.0, 1e-3], ids=["reference", "extended"])
+def test_refine_stop_after_equals_the_full_runs_checkpoint(tv):
+    """stop_after = k runs the first k steps of the same schedule: the atlas equals
+    the full run's checkpoint after step k bit for bit, as do the first k losses
+    (the rest stay 0), in graph and eager mode, with the TV buffers' parity."""
+    torch = pytest.importorskip("torch")
+    from photara_drender_amd import _native
+    from photara_drender_amd.refine import refine_atlas
+
+    sc, mats, masks, images, pairs = small_atlas(views=4)
+    ctx = _native.Context(0)
+    init = np.full((64, 64, 3), 0.5, np.float32)
+    kw = dict(seam_uv_pairs=pairs, steps=40, batch_size=2, learning_rate=1e-2, seam_polish_steps=0, tv_weight=tv,
+              chart_ids=sc.chart_texels if tv else None, joint_seam_weight=1e-2 if tv else 0.0)
+    full = refine_atlas(ctx, init, sc.positions, sc.uv, sc.indices, images, mats, masks, checkpoint_every=10, **kw)
+    for stop, graph in ((20, True), (30, False), (13, True)):
+        part = refine_atlas(ctx, init, sc.positions, sc.uv, sc.indices, images, mats, masks, stop_after=stop,
+                            use_graph=graph, **kw)
+        assert_bits(part.loss_history[:stop], full.loss_history[:stop], f"losses to step {stop}")
+        assert not part.loss_history[stop:].any()
+        if stop % 10 == 0:
+            assert_bits(part.color, full.checkpoints[stop // 10 - 1], f"atlas after step {stop}")
+
+
 def test_refine_extended_objective_runs_and_reduces_seams():
     torch = pytest.importorskip("torch")
     from photara_drender_amd import _native

@@ -69,6 +69,7 @@ def test_refine_options_default_matches_reference_types():
     assert o.adam_epsilon == np.float32(1e-8) and (o.clamp_min, o.clamp_max) == (0.0, 1.0)
     assert o.seam_polish_steps == 30 and o.seam_learning_rate == np.float32(1e-3)
     assert o.seam_epsilon == np.float32(1e-4)
+    assert o.stop_after == 0 and o.checkpoint_every == 0 and o.use_graph == 1  # extensions off by default
 
 
 def test_workload_sizes():

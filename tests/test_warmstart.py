@@ -296,4 +296,7 @@ def test_gpu_trained_network_gain_on_a_held_out_scene():
           f"{ev['ablation_plain_init_warm_options']['steps_to_target']}; init PSNR plain "
           f"{ev['plain']['init_psnr_db']:.2f} warm {ev['warm_start']['init_psnr_db']:.2f}")
     assert loss_trained < loss_base
-    assert ev["warm_start"]["steps_to_target"] is not None
+    warm_steps = ev["warm_start"]["steps_to_target"]
+    ablation_steps = ev["ablation_plain_init_warm_options"]["steps_to_target"]
+    assert warm_steps is not None
+    assert ablation_steps is None or warm_steps <= ablation_steps
