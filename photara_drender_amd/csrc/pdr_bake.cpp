@@ -137,7 +137,7 @@ void run_bake(pdr_context ctx, const float* positions, const float* normals, con
         check_indices_device(ctx, idx, static_cast<size_t>(T) * 3, V, st, PDR_OUT_OF_RANGE,
                              "triangle_indices contains an invalid vertex index");
     hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");  // device inputs may be in flight elsewhere
-    DeviceBuffers bufs(st);
+    DeviceBuffers bufs(ctx, features ? "bake_features" : "bake");
     const float* dpos = device_view(bufs, positions, static_cast<size_t>(V) * 3, st);
     const float* dnrm = device_view(bufs, normals, static_cast<size_t>(V) * 3, st);
     const float* duv = device_view(bufs, uv, static_cast<size_t>(V) * 2, st);
@@ -195,7 +195,7 @@ void run_bake(pdr_context ctx, const float* positions, const float* normals, con
     pdr::BakeView* dviews = bufs.alloc<pdr::BakeView>(kBakeChunk);
     for (uint32_t k0 = 0; k0 < nviews; k0 += kBakeChunk) {
         const uint32_t n = std::min(nviews - k0, kBakeChunk);
-        DeviceBuffers chunk_bufs(st);
+        DeviceBuffers chunk_bufs(ctx, features ? "bake_features_chunk" : "bake_chunk");
         std::vector<pdr::BakeView> hv(n);
         pdr::ViewTable vt{};
         vt.n = n;
@@ -239,7 +239,7 @@ void run_bake(pdr_context ctx, const float* positions, const float* normals, con
                                             best, src, st),
                           "bake_project");
             });
-        sync(st);  // the chunk's staged images are released with chunk_bufs
+        sync(st);  // the next chunk stages its images into the same slots
     }
 
     if (features) {

@@ -576,26 +576,22 @@ inline uint32_t validate_texture(uint32_t tw, uint32_t th, uint32_t C, uint32_t 
     return static_cast<uint32_t>(P);
 }
 
-// Call-local device allocations, freed when the call returns.
-// A call's temporary device arrays, allocated and freed in the order of the
-// call's stream from the device's default memory pool.  The context keeps that
-// pool's freed memory (its release threshold is raised at context creation),
-// so repeated calls reuse it instead of mapping and unmapping pages.
+// A call's temporary device arrays, taken from the context's grow-only
+// scratch slots "<tag>#<n>" (the n-th array this call asks for): repeated
+// calls with the same sequence of arrays reuse the same memory, with no
+// allocation or free on the host path once the slots are large enough.  Calls
+// on one context are serialised (its lock, and each call's stream ordered
+// after the previous one's), so a call's slots are never in use by another.
 struct DeviceBuffers {
-    explicit DeviceBuffers(hipStream_t s) : stream(s) {}
+    DeviceBuffers(pdr_context c, const char* t) : ctx(c), tag(t) {}
     DeviceBuffers(const DeviceBuffers&) = delete;
     DeviceBuffers& operator=(const DeviceBuffers&) = delete;
-    hipStream_t stream;
-    std::vector<void*> ptrs;
-    ~DeviceBuffers() {
-        for (void* p : ptrs) (void)hipFreeAsync(p, stream);
-    }
+    pdr_context ctx;
+    std::string tag;
+    uint32_t count = 0;
     template <class T>
-    T* alloc(size_t count) {
-        void* p = nullptr;
-        hip_check(hipMallocAsync(&p, std::max<size_t>(count, 1) * sizeof(T), stream), "hipMallocAsync");
-        ptrs.push_back(p);
-        return static_cast<T*>(p);
+    T* alloc(size_t n) {
+        return ctx->get_t<T>(tag + "#" + std::to_string(count++), std::max<size_t>(n, 1));
     }
 };
 

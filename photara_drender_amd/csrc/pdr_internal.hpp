@@ -215,10 +215,38 @@ struct RefineBatchTaps {
 };
 
 hipError_t transform_positions(const float* pos3, uint32_t V, const float* m16_host, float* out4, hipStream_t st);
-hipError_t refine_view_inputs(const float* image, uint32_t channels, const float* vis, const float* raster, uint32_t P,
-                              float* target3, float* mask, double* partials, uint32_t npartials, hipStream_t st);
+// clips + k * V * 4 = [pos, 1] transformed by the k-th row-major matrix of m16_dev (device), k < nviews.
+hipError_t transform_views(const float* pos3, uint32_t V, const float* m16_dev, uint32_t nviews, float* clips,
+                           hipStream_t st);
 uint32_t refine_partial_count(uint32_t P);
-hipError_t sum_partials(const double* partials, uint32_t n, double* out, hipStream_t st);
+// One view of a refinement precompute group: its inputs, per-view scratch
+// and the packed outputs in the view arena.
+struct PrecomputeView {
+    const float* image;    // [P, channels]
+    const float* vis;      // [P] visibility mask or nullptr
+    const float* raster;   // [P, 4]
+    const float* uv;       // [P, 2] interpolated atlas uv
+    float* target;         // [P, 3]
+    float* mask;           // [P]
+    double* partials;      // [refine_partial_count(P)] block sums of the mask
+    double* weight;        // the view's mask sum (fixed-order double)
+    uint32_t* tflags;      // [tile slots] active flags, 8x8-tile order
+    uint32_t* tpos;        // [tile slots] their exclusive scan: packed index
+    uint32_t* rflags;      // [P] active flags, raster order
+    uint32_t* rpos;        // [P] their exclusive scan: raster rank
+    uint32_t* count;       // active pixels
+    float4* uvm;           // [P] packed (u, v, mask, pixel bits)
+    float4* rgb;           // [P] packed target
+    float* lterm;          // [P] per-pixel loss terms (cleared)
+    uint32_t* order;       // [P] raster rank -> packed index
+    uint32_t channels, width, height;
+};
+// Mask sums and active-pixel packing of every view of a group (one launch per
+// pass, views on the grid's y axis); totals: [2 * nviews * active_scan_tiles(
+// max(max_pixels, max_slots))] scan tile totals.
+uint32_t active_scan_tiles(uint32_t n);
+hipError_t precompute_views(const PrecomputeView* views_dev, uint32_t nviews, uint32_t max_pixels,
+                            uint32_t max_slots, uint32_t* totals, hipStream_t st);
 // Residual pass over the step's views, then the texel gather of the step's
 // batch (batches[step % nbatches]); resid holds batch slots of `stride` entries.
 // The joint seam term's residual (seam_step's first kernel), run as one more
@@ -246,18 +274,9 @@ struct PhotoLossArgs {
 PhotoLossArgs photo_loss_args(const RefineView* views_dev, uint32_t batch, const LossParts& parts);
 // loss[s] for steps first .. first + count - 1 (count <= parts.period), from their loss partials.
 hipError_t loss_fold(const LossParts& parts, uint32_t first, uint32_t count, float* loss, hipStream_t st);
-// Active pixels of one view: flags over raster order and over 8x8-tile-order
-// slots; after exclusive scans of both, the records packed in tile order and
-// order[raster rank] = packed index.
+// Slots of a view in 8x8-tile order (whole tiles, >= its pixels).
 uint32_t tile_slots(uint32_t W, uint32_t H);
-hipError_t active_flags(const float* mask, uint32_t W, uint32_t H, bool tile_order, uint32_t* flags, hipStream_t st);
-hipError_t active_pack(const float* uv, const float* target, const float* mask, uint32_t W, uint32_t H,
-                       const uint32_t* tflags, const uint32_t* tpos, const uint32_t* rpos, float4* uvm, float4* rgb,
-                       uint32_t* order, hipStream_t st);
 uint32_t pixel_groups(uint32_t n);
-hipError_t exclusive_scan(const uint32_t* in, uint32_t* out, uint32_t n, void* temp, size_t temp_bytes, hipStream_t st);
-size_t exclusive_scan_temp_bytes(uint32_t n);
-hipError_t active_count(const uint32_t* flags, const uint32_t* pos, uint32_t n, uint32_t* count, hipStream_t st);
 // norm[s] = 1 / (3 * sum of the step's views' mask sums) (0 when that sum is 0)
 hipError_t norm_table(const double* view_weight, const uint32_t* view_index, uint32_t steps, uint32_t batch,
                       float* norm, hipStream_t st);
@@ -265,10 +284,18 @@ hipError_t norm_table(const double* view_weight, const uint32_t* view_index, uin
 // bits: tiles per row (0: linear texel keys) and the number of keys.
 uint32_t texel_key_tiles(uint32_t tw, uint32_t th);
 uint64_t texel_key_range(uint32_t tw, uint32_t th);
-// Tap entries of batch slot `slot` over its view's `count` packed pixels uvm,
-// generated in raster order (order[raster rank] = packed index).
-hipError_t batch_taps(const float4* uvm, const uint32_t* order, uint32_t count, uint32_t slot, uint32_t stride,
-                      uint32_t tw, uint32_t th, uint32_t* keys, uint32_t* vals, hipStream_t st);
+// One slot of a recurring step batch: its view's packed pixels, raster-order
+// index and active count, and where its tap entries start in the batch's arrays.
+struct BatchSlot {
+    const float4* uvm;
+    const uint32_t* order;
+    uint32_t count;
+    uint32_t offset;
+};
+// Tap entries of every slot of one batch (slots_dev: [nslots] device array),
+// each slot's generated in raster order (order[raster rank] = packed index).
+hipError_t batch_taps(const BatchSlot* slots_dev, uint32_t nslots, uint32_t max_count, uint32_t stride, uint32_t tw,
+                      uint32_t th, uint32_t* keys, uint32_t* vals, hipStream_t st);
 // Weights of sorted batch tap entries; slot_uvm[slot] = that slot's packed pixels (device array).
 hipError_t batch_tap_weights(const uint32_t* vals, uint32_t E, const float4* const* slot_uvm, uint32_t stride,
                              uint32_t tw, uint32_t th, float* w, hipStream_t st);

@@ -94,7 +94,7 @@ pdr_status pdr_refine(pdr_context ctx, const float* initial_texture, const float
 
         Lock lock(ctx);
         const hipStream_t st = ctx->use_stream(false, nullptr);
-        DeviceBuffers bufs(st);
+        DeviceBuffers bufs(ctx, "refine");
         if (idx_device)
             check_indices_device(ctx, idx, static_cast<size_t>(T) * 3, V, st, PDR_INVALID_ARGUMENT,
                                  "texture refinement mesh arrays have inconsistent shapes");
@@ -108,27 +108,31 @@ pdr_status pdr_refine(pdr_context ctx, const float* initial_texture, const float
         const uint32_t* didx = device_view(bufs, idx, static_cast<size_t>(T) * 3, st);
         uint32_t max_pixels = 0;
         for (uint32_t k = 0; k < nviews; ++k) max_pixels = std::max(max_pixels, views[k].width * views[k].height);
-        // views are rasterised in groups of up to kMaxViews per launch
+        // views are processed in groups of up to kMaxViews, one launch per pass
         const uint32_t group = std::min<uint32_t>(nviews, pdr::kMaxViews);
+        uint32_t max_slots = 0;  // tile-order slots (>= pixels)
+        for (uint32_t k = 0; k < nviews; ++k)
+            max_slots = std::max(max_slots, pdr::tile_slots(views[k].width, views[k].height));
+        const size_t gp = static_cast<size_t>(group) * max_pixels, gs = static_cast<size_t>(group) * max_slots;
         float* clips = bufs.alloc<float>(static_cast<size_t>(group) * V * 4);
-        float* rasters = bufs.alloc<float>(static_cast<size_t>(group) * max_pixels * 4);
-        double* partials = bufs.alloc<double>(pdr::refine_partial_count(max_pixels) + 1);
+        float* mats = bufs.alloc<float>(static_cast<size_t>(group) * 16);
+        float* rasters = bufs.alloc<float>(gp * 4);
+        float* vuv = bufs.alloc<float>(gp * 2);
+        float* target = bufs.alloc<float>(gp * 3);
+        float* mask = bufs.alloc<float>(gp);
+        const uint32_t nparts = pdr::refine_partial_count(max_pixels);
+        double* partials = bufs.alloc<double>(static_cast<size_t>(group) * nparts);
+        uint32_t* act_flags = bufs.alloc<uint32_t>(gs);  // tile order
+        uint32_t* act_pos = bufs.alloc<uint32_t>(gs);
+        uint32_t* rank_flags = bufs.alloc<uint32_t>(gp);  // raster order
+        uint32_t* rank_pos = bufs.alloc<uint32_t>(gp);
+        const uint32_t scan_tiles = pdr::active_scan_tiles(std::max(max_pixels, max_slots));
+        uint32_t* scan_totals = bufs.alloc<uint32_t>(static_cast<size_t>(group) * 2 * scan_tiles);
+        pdr::PrecomputeView* dpre = bufs.alloc<pdr::PrecomputeView>(group);
         double* view_weight = bufs.alloc<double>(nviews);
         uint32_t* view_counts = bufs.alloc<uint32_t>(nviews);  // active pixels
         std::vector<pdr::RefineView> rviews(nviews);
         std::vector<const uint32_t*> view_order(nviews);  // raster rank -> packed index, per view
-        uint32_t max_slots = 0;  // tile-order slots (>= pixels)
-        for (uint32_t k = 0; k < nviews; ++k)
-            max_slots = std::max(max_slots, pdr::tile_slots(views[k].width, views[k].height));
-        uint32_t* act_flags = bufs.alloc<uint32_t>(max_slots);  // tile order
-        uint32_t* act_pos = bufs.alloc<uint32_t>(max_slots);
-        uint32_t* rank_flags = bufs.alloc<uint32_t>(max_pixels);  // raster order
-        uint32_t* rank_pos = bufs.alloc<uint32_t>(max_pixels);
-        const size_t scan_temp_bytes = pdr::exclusive_scan_temp_bytes(max_slots);
-        void* scan_temp = bufs.alloc<unsigned char>(scan_temp_bytes);
-        float* vuv = bufs.alloc<float>(static_cast<size_t>(max_pixels) * 2);
-        float* target = bufs.alloc<float>(static_cast<size_t>(max_pixels) * 3);
-        float* mask = bufs.alloc<float>(max_pixels);
         // every view's packed pixels in one allocation, sized for every pixel so
         // the per-view passes need no host readback
         auto up = [](size_t b) { return (b + 255) & ~static_cast<size_t>(255); };
@@ -141,61 +145,74 @@ pdr_status pdr_refine(pdr_context ctx, const float* initial_texture, const float
         }
         unsigned char* view_arena = bufs.alloc<unsigned char>(std::max<size_t>(total_bytes, 256));
         size_t arena_off = 0;
+        std::vector<pdr::PrecomputeView> hpre(group);
+        std::vector<float> hmats(static_cast<size_t>(group) * 16);
         for (uint32_t g0 = 0; g0 < nviews; g0 += group) {
             const uint32_t gn = std::min(group, nviews - g0);
+            for (uint32_t b = 0; b < gn; ++b) std::memcpy(&hmats[b * 16], views[g0 + b].world_to_clip, 16 * 4);
+            hip_check(hipMemcpyAsync(mats, hmats.data(), gn * 16 * 4, hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+            hip_check(pdr::transform_views(dpos, V, mats, gn, clips, st), "transform_views");
             pdr::ViewTable vt{};
             vt.n = gn;
+            pdr::PixelTable pt{};
+            pt.n = gn;
             for (uint32_t b = 0; b < gn; ++b) {
                 const pdr_refine_view& v = views[g0 + b];
-                float* clip = clips + static_cast<size_t>(b) * V * 4;
-                hip_check(pdr::transform_positions(dpos, V, v.world_to_clip, clip, st), "transform_positions");
+                const uint32_t P = v.width * v.height;
                 float vp[4];
                 resolve_viewport(v.width, v.height, v.has_viewport ? v.viewport : nullptr, vp);
-                vt.v[b] = make_view(clip, v.width, v.height, vp);
+                vt.v[b] = make_view(clips + static_cast<size_t>(b) * V * 4, v.width, v.height, vp);
                 vt.v[b].raster = rasters + static_cast<size_t>(b) * max_pixels * 4;
+                pdr::PixelView& pv = pt.v[b];
+                pv.raster = vt.v[b].raster;
+                pv.out0 = vuv + static_cast<size_t>(b) * max_pixels * 2;
+                pv.width = v.width;
+                pv.height = v.height;
+                pv.tiles_x = (v.width + pdr::kGradTile - 1) / pdr::kGradTile;
+                pv.tile_base = pt.total_tiles;
+                pv.pixel_base = pt.total_pixels;
+                pt.total_tiles += pv.tiles_x * ((v.height + pdr::kGradTile - 1) / pdr::kGradTile);
+                pt.total_pixels += P;
+                // the pixels with mask > 0 packed in 8x8-tile order (atlas locality
+                // of a wave's taps), their raster ranks (the order in which the
+                // reference's pixels add their taps), and the view's per-pixel loss
+                // terms (zero where the mask is)
+                pdr::PrecomputeView& pre = hpre[b];
+                const size_t po = static_cast<size_t>(b) * max_pixels, so = static_cast<size_t>(b) * max_slots;
+                pre.image = device_view(bufs, v.image, static_cast<size_t>(P) * v.channels, st);
+                pre.vis = device_view(bufs, v.visibility_mask, P, st);
+                pre.raster = vt.v[b].raster;
+                pre.uv = pv.out0;
+                pre.target = target + po * 3;
+                pre.mask = mask + po;
+                pre.partials = partials + static_cast<size_t>(b) * nparts;
+                pre.weight = view_weight + g0 + b;
+                pre.tflags = act_flags + so;
+                pre.tpos = act_pos + so;
+                pre.rflags = rank_flags + po;
+                pre.rpos = rank_pos + po;
+                pre.count = view_counts + g0 + b;
+                pre.uvm = reinterpret_cast<float4*>(view_arena + arena_off);
+                arena_off += up(static_cast<size_t>(P) * sizeof(float4));
+                pre.rgb = reinterpret_cast<float4*>(view_arena + arena_off);
+                arena_off += up(static_cast<size_t>(P) * sizeof(float4));
+                pre.lterm = reinterpret_cast<float*>(view_arena + arena_off);
+                arena_off += up(static_cast<size_t>(P) * sizeof(float));
+                pre.order = reinterpret_cast<uint32_t*>(view_arena + arena_off);
+                arena_off += up(static_cast<size_t>(P) * sizeof(uint32_t));
+                pre.channels = v.channels;
+                pre.width = v.width;
+                pre.height = v.height;
+                view_order[g0 + b] = pre.order;
+                rviews[g0 + b] = pdr::RefineView{pre.uvm, pre.rgb, pre.lterm, 0u, P};  // active: set once read back
             }
             finalize_table(vt);
             run_raster_forward(ctx, vt, didx, T, V, 0, 0, st);
-            for (uint32_t b = 0; b < gn; ++b) {
-                const uint32_t k = g0 + b;
-                const pdr_refine_view& v = views[k];
-                const uint32_t P = v.width * v.height;
-                const float* raster = rasters + static_cast<size_t>(b) * max_pixels * 4;
-                const float* img = device_view(bufs, v.image, static_cast<size_t>(P) * v.channels, st);
-                const float* vis = device_view(bufs, v.visibility_mask, P, st);
-                const pdr::PixelTable pt = pixel_table_one(raster, nullptr, nullptr, vuv, nullptr, v.width, v.height);
-                hip_check(pdr::interp_forward(pt, duv, V, 2, didx, T, st), "interp_forward");
-                const uint32_t np = pdr::refine_partial_count(P);
-                hip_check(pdr::refine_view_inputs(img, v.channels, vis, raster, P, target, mask, partials, np, st),
-                          "refine_view_inputs");
-                hip_check(pdr::sum_partials(partials, np, view_weight + k, st), "sum_partials");
-                // the pixels with mask > 0, packed in 8x8-tile order (atlas
-                // locality of a wave's taps), their raster ranks (the order in
-                // which the reference's pixels add their taps), and the view's
-                // per-pixel loss terms (zero where the mask is)
-                const uint32_t slots = pdr::tile_slots(v.width, v.height);
-                hip_check(pdr::active_flags(mask, v.width, v.height, true, act_flags, st), "active_flags");
-                hip_check(pdr::exclusive_scan(act_flags, act_pos, slots, scan_temp, scan_temp_bytes, st),
-                          "exclusive_scan");
-                hip_check(pdr::active_count(act_flags, act_pos, slots, view_counts + k, st), "active_count");
-                hip_check(pdr::active_flags(mask, v.width, v.height, false, rank_flags, st), "active_flags");
-                hip_check(pdr::exclusive_scan(rank_flags, rank_pos, P, scan_temp, scan_temp_bytes, st),
-                          "exclusive_scan");
-                float4* uvm = reinterpret_cast<float4*>(view_arena + arena_off);
-                arena_off += up(static_cast<size_t>(P) * sizeof(float4));
-                float4* rgb = reinterpret_cast<float4*>(view_arena + arena_off);
-                arena_off += up(static_cast<size_t>(P) * sizeof(float4));
-                float* lterm = reinterpret_cast<float*>(view_arena + arena_off);
-                arena_off += up(static_cast<size_t>(P) * sizeof(float));
-                uint32_t* order = reinterpret_cast<uint32_t*>(view_arena + arena_off);
-                arena_off += up(static_cast<size_t>(P) * sizeof(uint32_t));
-                view_order[k] = order;
-                hip_check(pdr::active_pack(vuv, target, mask, v.width, v.height, act_flags, act_pos, rank_pos, uvm,
-                                           rgb, order, st),
-                          "active_pack");
-                hip_check(pdr::fill_f32(lterm, 0.0f, P, st), "fill");
-                rviews[k] = pdr::RefineView{uvm, rgb, lterm, 0u, P};  // active: set once the counts are read back
-            }
+            hip_check(pdr::interp_forward(pt, duv, V, 2, didx, T, st), "interp_forward");
+            hip_check(hipMemcpyAsync(dpre, hpre.data(), gn * sizeof(pdr::PrecomputeView), hipMemcpyHostToDevice, st),
+                      "hipMemcpyAsync");
+            hip_check(pdr::precompute_views(dpre, gn, max_pixels, max_slots, scan_totals, st), "precompute_views");
+            if (g0 + group < nviews) sync(st);  // the next group rewrites the group's scratch and view records
         }
         std::vector<uint32_t> host_counts(nviews);
         hip_check(hipMemcpyAsync(host_counts.data(), view_counts, host_counts.size() * 4, hipMemcpyDeviceToHost, st),
@@ -256,16 +273,28 @@ pdr_status pdr_refine(pdr_context ctx, const float* initial_texture, const float
             tap_off += up(bytes);
             return p;
         };
+        // each batch's slots: view records, active counts and entry offsets
+        std::vector<pdr::BatchSlot> hslots(static_cast<size_t>(nbatches) * o.batch_size);
+        std::vector<uint32_t> slot_max(nbatches, 0);
         for (uint32_t c = 0; c < nbatches; ++c) {
-            const uint32_t E = batch_entries[c];
             uint32_t off = 0;
             for (uint32_t b = 0; b < o.batch_size; ++b) {
                 const uint32_t k = slot_view(c, b);
-                hip_check(pdr::batch_taps(rviews[k].uvm, view_order[k], host_counts[k], b, max_active, o.width,
-                                          o.height, tap_keys + off, tap_vals + off, st),
-                          "batch_taps");
+                hslots[static_cast<size_t>(c) * o.batch_size + b] =
+                    pdr::BatchSlot{rviews[k].uvm, view_order[k], host_counts[k], off};
                 off += host_counts[k] * 4u;
+                slot_max[c] = std::max(slot_max[c], host_counts[k]);
             }
+        }
+        pdr::BatchSlot* dslots = bufs.alloc<pdr::BatchSlot>(hslots.size());
+        hip_check(hipMemcpyAsync(dslots, hslots.data(), hslots.size() * sizeof(pdr::BatchSlot), hipMemcpyHostToDevice,
+                                 st),
+                  "hipMemcpyAsync");
+        for (uint32_t c = 0; c < nbatches; ++c) {
+            const uint32_t E = batch_entries[c];
+            hip_check(pdr::batch_taps(dslots + static_cast<size_t>(c) * o.batch_size, o.batch_size, slot_max[c],
+                                      max_active, o.width, o.height, tap_keys, tap_vals, st),
+                      "batch_taps");
             uint32_t* texels_c = static_cast<uint32_t*>(carve(static_cast<size_t>(E) * 4));
             uint32_t* offsets_c = static_cast<uint32_t*>(carve((static_cast<size_t>(E) + 1) * 4));
             uint32_t* taps_c = static_cast<uint32_t*>(carve(static_cast<size_t>(E) * 4));

@@ -64,21 +64,37 @@ __global__ void k_transform(const float* __restrict__ pos3, uint32_t V, M16 M, f
     out[v] = make_float4(r[0], r[1], r[2], r[3]);
 }
 
-// Block partials of the mask sum in double (fixed order: lanes, then waves).
-__global__ __launch_bounds__(kBlock) void k_view_inputs(const float* __restrict__ image, uint32_t channels,
-                                                        const float* __restrict__ vis, const float* __restrict__ raster,
-                                                        uint32_t P, float* __restrict__ target,
-                                                        float* __restrict__ mask, double* __restrict__ partials) {
+// k_transform over a group of views: view blockIdx.y with matrix mats[y], into clips + y * V.
+__global__ void k_transform_views(const float* __restrict__ pos3, uint32_t V, const M16* __restrict__ mats,
+                                  float4* __restrict__ clips) {
+    const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
+    if (v >= V) return;
+    const M16& M = mats[blockIdx.y];
+    const float x = pos3[v * 3 + 0], y = pos3[v * 3 + 1], z = pos3[v * 3 + 2];
+    float r[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) r[k] = M.m[k * 4] * x + M.m[k * 4 + 1] * y + M.m[k * 4 + 2] * z + M.m[k * 4 + 3];
+    clips[static_cast<size_t>(blockIdx.y) * V + v] = make_float4(r[0], r[1], r[2], r[3]);
+}
+
+// Per view (blockIdx.y) of a precompute group: target RGB, mask and block
+// partials of the mask sum in double (fixed order: lanes, then waves).
+__global__ __launch_bounds__(kBlock) void k_view_inputs(const PrecomputeView* __restrict__ views) {
     __shared__ double wsum[kBlock];
+    const PrecomputeView& pv = views[blockIdx.y];
+    const uint32_t P = pv.width * pv.height;
+    if (blockIdx.x * kBlock >= P) return;  // whole block past this view
     const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
     double w = 0.0;
     if (p < P) {
-        target[static_cast<size_t>(p) * 3 + 0] = image[static_cast<size_t>(p) * channels + 0];
-        target[static_cast<size_t>(p) * 3 + 1] = image[static_cast<size_t>(p) * channels + 1];
-        target[static_cast<size_t>(p) * 3 + 2] = image[static_cast<size_t>(p) * channels + 2];
+        const float* image = pv.image;
+        const uint32_t channels = pv.channels;
+        pv.target[static_cast<size_t>(p) * 3 + 0] = image[static_cast<size_t>(p) * channels + 0];
+        pv.target[static_cast<size_t>(p) * 3 + 1] = image[static_cast<size_t>(p) * channels + 1];
+        pv.target[static_cast<size_t>(p) * 3 + 2] = image[static_cast<size_t>(p) * channels + 2];
         float m = 0.0f;
-        if (raster[static_cast<size_t>(p) * 4 + 3] != 0.0f) m = vis ? clampf(vis[p], 0.0f, 1.0f) : 1.0f;
-        mask[p] = m;
+        if (pv.raster[static_cast<size_t>(p) * 4 + 3] != 0.0f) m = pv.vis ? clampf(pv.vis[p], 0.0f, 1.0f) : 1.0f;
+        pv.mask[p] = m;
         w = m;
     }
     wsum[threadIdx.x] = w;
@@ -86,24 +102,26 @@ __global__ __launch_bounds__(kBlock) void k_view_inputs(const float* __restrict_
     if (threadIdx.x == 0) {
         double s = 0.0;
         for (uint32_t i = 0; i < kBlock; ++i) s += wsum[i];
-        partials[blockIdx.x] = s;
+        pv.partials[blockIdx.x] = s;
     }
 }
 
-// Fixed-order sum: 256 lanes add contiguous chunks, lane 0 adds the chunk sums in order.
-__global__ __launch_bounds__(kBlock) void k_sum_partials(const double* __restrict__ partials, uint32_t n,
-                                                         double* __restrict__ out) {
+// Fixed-order sum of a view's block partials (one workgroup per view): 256
+// lanes add contiguous chunks, lane 0 adds the chunk sums in order.
+__global__ __launch_bounds__(kBlock) void k_sum_partials(const PrecomputeView* __restrict__ views) {
     __shared__ double chunk[kBlock];
+    const PrecomputeView& pv = views[blockIdx.x];
+    const uint32_t n = (pv.width * pv.height + kBlock - 1) / kBlock;
     const uint32_t per = (n + kBlock - 1) / kBlock;
     const uint32_t b = threadIdx.x * per, e = b + per < n ? b + per : n;
     double s = 0.0;
-    for (uint32_t i = b; i < e; ++i) s += partials[i];
+    for (uint32_t i = b; i < e; ++i) s += pv.partials[i];
     chunk[threadIdx.x] = s;
     __syncthreads();
     if (threadIdx.x == 0) {
         double t = 0.0;
         for (uint32_t i = 0; i < kBlock; ++i) t += chunk[i];
-        *out = t;
+        *pv.weight = t;
     }
 }
 
@@ -374,6 +392,11 @@ __global__ __launch_bounds__(kBlock) void k_photo_gather(const RefineBatchTaps* 
     texel_gather<2>(bt.texels, bt.tap_offsets, bt.taps, bt.tap_weights, *bt.runs, resid, tw, tiles_x, grad, prod);
 }
 
+// Tile-order slots of a W x H view (whole 8x8 tiles).
+__device__ __forceinline__ uint32_t tile_slots_of(uint32_t W, uint32_t H) {
+    return ((W + 7u) / 8u) * ((H + 7u) / 8u) * 64u;
+}
+
 // Tile-order slot q of a W x H view: tile q / 64 (row-major over 8x8 tiles),
 // pixel (q % 64) row-major inside the tile.
 __device__ __forceinline__ bool tile_pixel(uint32_t q, uint32_t W, uint32_t H, uint32_t& p) {
@@ -383,58 +406,126 @@ __device__ __forceinline__ bool tile_pixel(uint32_t q, uint32_t W, uint32_t H, u
     return x < W && y < H;
 }
 
-// flags over raster order (tile_order = 0: slot = pixel) or over tile-order slots
-__global__ void k_active_flags(const float* __restrict__ mask, uint32_t W, uint32_t H, uint32_t n,
-                               uint32_t tile_order, uint32_t* __restrict__ flags) {
+// Active flags of a view (blockIdx.y) over its 8x8-tile-order slots and over raster order.
+__global__ void k_active_flags(const PrecomputeView* __restrict__ views) {
+    const PrecomputeView& pv = views[blockIdx.y];
+    const uint32_t W = pv.width, H = pv.height;
     const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
-    if (q >= n) return;
-    uint32_t p = q;
-    const bool in = tile_order ? tile_pixel(q, W, H, p) : true;
-    flags[q] = in && mask[p] > 0.0f ? 1u : 0u;
+    if (q < tile_slots_of(W, H)) {
+        uint32_t p = 0;
+        const bool in = tile_pixel(q, W, H, p);
+        pv.tflags[q] = in && pv.mask[p] > 0.0f ? 1u : 0u;
+    }
+    if (q < W * H) pv.rflags[q] = pv.mask[q] > 0.0f ? 1u : 0u;
+}
+
+// Exclusive scans of both flag arrays of every view of the group (segment
+// 2 y + z: z = 0 tile order, with the view's active count; z = 1 raster
+// order), in tiles of kScanTile: each tile's total, then each tile's scan
+// offset by the totals of the tiles before it.
+constexpr uint32_t kScanTile = 2048;  // 256 lanes x 8
+
+__device__ __forceinline__ void scan_segment(const PrecomputeView& pv, uint32_t z, const uint32_t*& in, uint32_t*& out,
+                                             uint32_t& n) {
+    in = z ? pv.rflags : pv.tflags;
+    out = z ? pv.rpos : pv.tpos;
+    n = z ? pv.width * pv.height : tile_slots_of(pv.width, pv.height);
+}
+
+__global__ __launch_bounds__(kBlock) void k_active_scan_totals(const PrecomputeView* __restrict__ views,
+                                                               uint32_t tiles, uint32_t* __restrict__ totals) {
+    using Reduce = hipcub::BlockReduce<uint32_t, kBlock>;
+    __shared__ typename Reduce::TempStorage tmp;
+    const uint32_t* in;
+    uint32_t* out;
+    uint32_t n;
+    scan_segment(views[blockIdx.y], blockIdx.z, in, out, n);
+    const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * 8;
+    if (blockIdx.x * kScanTile >= n) return;
+    uint32_t s = 0;
+    for (uint32_t k = 0; k < 8; ++k)
+        if (base + k < n) s += in[base + k];
+    s = Reduce(tmp).Sum(s);
+    if (threadIdx.x == 0) totals[(static_cast<size_t>(blockIdx.y) * 2 + blockIdx.z) * tiles + blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(kBlock) void k_active_scan_out(const PrecomputeView* __restrict__ views, uint32_t tiles,
+                                                            const uint32_t* __restrict__ totals) {
+    using Reduce = hipcub::BlockReduce<uint32_t, kBlock>;
+    using Scan = hipcub::BlockScan<uint32_t, kBlock>;
+    __shared__ typename Reduce::TempStorage rtmp;
+    __shared__ typename Scan::TempStorage stmp;
+    __shared__ uint32_t prefix;
+    const PrecomputeView& pv = views[blockIdx.y];
+    const uint32_t* in;
+    uint32_t* out;
+    uint32_t n;
+    scan_segment(pv, blockIdx.z, in, out, n);
+    if (blockIdx.x * kScanTile >= n) return;
+    const uint32_t* tot = totals + (static_cast<size_t>(blockIdx.y) * 2 + blockIdx.z) * tiles;
+    uint32_t s = 0;
+    for (uint32_t b = threadIdx.x; b < blockIdx.x; b += kBlock) s += tot[b];
+    s = Reduce(rtmp).Sum(s);
+    if (threadIdx.x == 0) prefix = s;
+    const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * 8;
+    uint32_t c[8], t = 0;
+    for (uint32_t k = 0; k < 8; ++k) {
+        c[k] = base + k < n ? in[base + k] : 0u;
+        t += c[k];
+    }
+    uint32_t run;
+    Scan(stmp).ExclusiveSum(t, run);
+    __syncthreads();
+    run += prefix;
+    for (uint32_t k = 0; k < 8; ++k) {
+        if (base + k < n) out[base + k] = run;
+        run += c[k];
+    }
+    // the last tile's last lane holds the segment total: the view's active count
+    if (blockIdx.z == 0 && (blockIdx.x + 1) * kScanTile >= n && threadIdx.x == kBlock - 1) *pv.count = run;
 }
 
 // Packed record a = tpos[q] of the active pixel p at tile-order slot q:
-// (u, v, mask, p as bits) and the target; order[raster rank of p] = a.
-__global__ void k_active_pack(const float* __restrict__ uv, const float* __restrict__ target,
-                              const float* __restrict__ mask, uint32_t W, uint32_t H, uint32_t n,
-                              const uint32_t* __restrict__ tflags, const uint32_t* __restrict__ tpos,
-                              const uint32_t* __restrict__ rpos, float4* __restrict__ uvm, float4* __restrict__ rgb,
-                              uint32_t* __restrict__ order) {
+// (u, v, mask, p as bits) and the target; order[raster rank of p] = a; every
+// pixel's loss term cleared.
+__global__ void k_active_pack(const PrecomputeView* __restrict__ views) {
+    const PrecomputeView& pv = views[blockIdx.y];
+    const uint32_t W = pv.width, H = pv.height;
     const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
-    if (q >= n || !tflags[q]) return;
+    if (q >= tile_slots_of(W, H)) return;
     uint32_t p = 0;
-    tile_pixel(q, W, H, p);
-    const uint32_t a = tpos[q];
-    uvm[a] = make_float4(uv[static_cast<size_t>(p) * 2], uv[static_cast<size_t>(p) * 2 + 1], mask[p],
-                         __uint_as_float(p));
-    rgb[a] = make_float4(target[static_cast<size_t>(p) * 3], target[static_cast<size_t>(p) * 3 + 1],
-                         target[static_cast<size_t>(p) * 3 + 2], 0.0f);
-    order[rpos[p]] = a;
+    if (!tile_pixel(q, W, H, p)) return;
+    pv.lterm[p] = 0.0f;
+    if (!pv.tflags[q]) return;
+    const uint32_t a = pv.tpos[q];
+    const float* uv = pv.uv;
+    const float* target = pv.target;
+    pv.uvm[a] = make_float4(uv[static_cast<size_t>(p) * 2], uv[static_cast<size_t>(p) * 2 + 1], pv.mask[p],
+                            __uint_as_float(p));
+    pv.rgb[a] = make_float4(target[static_cast<size_t>(p) * 3], target[static_cast<size_t>(p) * 3 + 1],
+                            target[static_cast<size_t>(p) * 3 + 2], 0.0f);
+    pv.order[pv.rpos[p]] = a;
 }
 
-
-__global__ void k_active_count(const uint32_t* __restrict__ flags, const uint32_t* __restrict__ pos, uint32_t n,
-                               uint32_t* __restrict__ count) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *count = n ? flags[n - 1] + pos[n - 1] : 0u;
-}
-
-// Tap entries of batch slot `slot` over its view's `count` active pixels, in
-// raster order (order[r] = packed index of the pixel of raster rank r), so that
-// a stable sort by texel leaves every texel's entries in (slot, raster pixel,
-// tap) order: key = texel, value = (slot * stride + a) * 4 + tap.
-__global__ void k_batch_taps(const float4* __restrict__ uvm, const uint32_t* __restrict__ order, uint32_t count,
-                             uint32_t slot, uint32_t stride, uint32_t tw, uint32_t th, uint32_t tiles_x,
-                             uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+// Tap entries of one recurring batch, slot blockIdx.y over its view's active
+// pixels, in raster order (order[r] = packed index of the pixel of raster
+// rank r), so that a stable sort by texel leaves every texel's entries in
+// (slot, raster pixel, tap) order: key = texel, value = (slot * stride + a) *
+// 4 + tap, written from the slot's offset into the batch's arrays.
+__global__ void k_batch_taps(const BatchSlot* __restrict__ slots, uint32_t stride, uint32_t tw, uint32_t th,
+                             uint32_t tiles_x, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    const BatchSlot& bs = slots[blockIdx.y];
     const uint32_t r = blockIdx.x * kBlock + threadIdx.x;
-    if (r >= count) return;
-    const uint32_t a = order[r];
-    const float4 q = uvm[a];
+    if (r >= bs.count) return;
+    const uint32_t a = bs.order[r];
+    const float4 q = bs.uvm[a];
     const AtlasTap at = atlas_tap(q.x, q.y, tw, th);
-    const uint32_t base = (slot * stride + a) * 4u;
+    const uint32_t base = (blockIdx.y * stride + a) * 4u;
+    const size_t o = static_cast<size_t>(bs.offset) + static_cast<size_t>(r) * 4;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        keys[static_cast<size_t>(r) * 4 + k] = texel_key(at.t[k], tw, tiles_x);
-        vals[static_cast<size_t>(r) * 4 + k] = base + static_cast<uint32_t>(k);
+        keys[o + k] = texel_key(at.t[k], tw, tiles_x);
+        vals[o + k] = base + static_cast<uint32_t>(k);
     }
 }
 
@@ -748,16 +839,34 @@ hipError_t transform_positions(const float* pos3, uint32_t V, const float* m16, 
 
 uint32_t refine_partial_count(uint32_t P) { return (P + kBlock - 1) / kBlock; }
 
-hipError_t refine_view_inputs(const float* image, uint32_t channels, const float* vis, const float* raster, uint32_t P,
-                              float* target3, float* mask, double* partials, uint32_t npartials, hipStream_t st) {
-    (void)npartials;
-    k_view_inputs<<<refine_partial_count(P), kBlock, 0, st>>>(image, channels, vis, raster, P, target3, mask,
-                                                             partials);
+hipError_t transform_views(const float* pos3, uint32_t V, const float* m16_dev, uint32_t nviews, float* clips,
+                           hipStream_t st) {
+    if (!V || !nviews) return hipSuccess;
+    k_transform_views<<<dim3((V + kBlock - 1) / kBlock, nviews), kBlock, 0, st>>>(
+        pos3, V, reinterpret_cast<const M16*>(m16_dev), reinterpret_cast<float4*>(clips));
     return hipGetLastError();
 }
 
-hipError_t sum_partials(const double* partials, uint32_t n, double* out, hipStream_t st) {
-    k_sum_partials<<<1, kBlock, 0, st>>>(partials, n, out);
+uint32_t active_scan_tiles(uint32_t n) { return (n + kScanTile - 1) / kScanTile; }
+
+hipError_t precompute_views(const PrecomputeView* views_dev, uint32_t nviews, uint32_t max_pixels,
+                            uint32_t max_slots, uint32_t* totals, hipStream_t st) {
+    if (!nviews) return hipSuccess;
+    const uint32_t pix_blocks = (max_pixels + kBlock - 1) / kBlock;
+    k_view_inputs<<<dim3(pix_blocks ? pix_blocks : 1, nviews), kBlock, 0, st>>>(views_dev);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    k_sum_partials<<<nviews, kBlock, 0, st>>>(views_dev);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    const uint32_t span = max_slots > max_pixels ? max_slots : max_pixels;
+    k_active_flags<<<dim3((span + kBlock - 1) / kBlock, nviews), kBlock, 0, st>>>(views_dev);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    const uint32_t tiles = active_scan_tiles(span);
+    k_active_scan_totals<<<dim3(tiles, nviews, 2), kBlock, 0, st>>>(views_dev, tiles, totals);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    k_active_scan_out<<<dim3(tiles, nviews, 2), kBlock, 0, st>>>(views_dev, tiles, totals);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    k_active_pack<<<dim3((max_slots + kBlock - 1) / kBlock, nviews), kBlock, 0, st>>>(views_dev);
     return hipGetLastError();
 }
 
@@ -788,23 +897,6 @@ hipError_t photometric_step(const RefineView* views_dev, uint32_t batch, uint32_
 
 uint32_t tile_slots(uint32_t W, uint32_t H) { return ((W + 7u) / 8u) * ((H + 7u) / 8u) * 64u; }
 
-hipError_t active_flags(const float* mask, uint32_t W, uint32_t H, bool tile_order, uint32_t* flags, hipStream_t st) {
-    const uint32_t n = tile_order ? tile_slots(W, H) : W * H;
-    if (!n) return hipSuccess;
-    k_active_flags<<<(n + kBlock - 1) / kBlock, kBlock, 0, st>>>(mask, W, H, n, tile_order ? 1u : 0u, flags);
-    return hipGetLastError();
-}
-
-hipError_t active_pack(const float* uv, const float* target, const float* mask, uint32_t W, uint32_t H,
-                       const uint32_t* tflags, const uint32_t* tpos, const uint32_t* rpos, float4* uvm, float4* rgb,
-                       uint32_t* order, hipStream_t st) {
-    const uint32_t n = tile_slots(W, H);
-    if (!n) return hipSuccess;
-    k_active_pack<<<(n + kBlock - 1) / kBlock, kBlock, 0, st>>>(uv, target, mask, W, H, n, tflags, tpos, rpos, uvm,
-                                                               rgb, order);
-    return hipGetLastError();
-}
-
 uint32_t pixel_groups(uint32_t n) { return (n + 63u) / 64u; }
 
 PhotoLossArgs photo_loss_args(const RefineView* views_dev, uint32_t batch, const LossParts& parts) {
@@ -824,28 +916,11 @@ hipError_t seam_loss_fold(const float* part, uint32_t npairs, uint32_t steps, fl
     return hipGetLastError();
 }
 
-size_t exclusive_scan_temp_bytes(uint32_t n) {
-    size_t a = 0;
-    (void)hipcub::DeviceScan::ExclusiveSum(nullptr, a, static_cast<const uint32_t*>(nullptr),
-                                           static_cast<uint32_t*>(nullptr), static_cast<int>(n));
-    return a;
-}
-
-hipError_t exclusive_scan(const uint32_t* in, uint32_t* out, uint32_t n, void* temp, size_t temp_bytes, hipStream_t st) {
-    size_t bytes = temp_bytes;
-    return hipcub::DeviceScan::ExclusiveSum(temp, bytes, in, out, static_cast<int>(n), st);
-}
-
-hipError_t active_count(const uint32_t* flags, const uint32_t* pos, uint32_t n, uint32_t* count, hipStream_t st) {
-    k_active_count<<<1, 64, 0, st>>>(flags, pos, n, count);
-    return hipGetLastError();
-}
-
-hipError_t batch_taps(const float4* uvm, const uint32_t* order, uint32_t count, uint32_t slot, uint32_t stride,
-                      uint32_t tw, uint32_t th, uint32_t* keys, uint32_t* vals, hipStream_t st) {
-    if (!count) return hipSuccess;
-    k_batch_taps<<<(count + kBlock - 1) / kBlock, kBlock, 0, st>>>(uvm, order, count, slot, stride, tw, th,
-                                                                   texel_key_tiles(tw, th), keys, vals);
+hipError_t batch_taps(const BatchSlot* slots_dev, uint32_t nslots, uint32_t max_count, uint32_t stride, uint32_t tw,
+                      uint32_t th, uint32_t* keys, uint32_t* vals, hipStream_t st) {
+    if (!nslots || !max_count) return hipSuccess;
+    k_batch_taps<<<dim3((max_count + kBlock - 1) / kBlock, nslots), kBlock, 0, st>>>(
+        slots_dev, stride, tw, th, texel_key_tiles(tw, th), keys, vals);
     return hipGetLastError();
 }
 
