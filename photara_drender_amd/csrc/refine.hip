@@ -265,35 +265,40 @@ __device__ __forceinline__ void photo_loss_groups(const RefineView* __restrict__
 // group) order, as the shader's one add per group lands when the groups run in
 // that order.  The shader skips zero group sums; every partial is >= +0 (a sum
 // of r * m * norm terms) and so is the running sum, so adding a zero changes
-// nothing.  One lane per step, eight float4 loads in flight ahead of the adds.
-__global__ void k_loss_fold(LossParts parts, uint32_t first, uint32_t count, float* __restrict__ loss) {
-    const uint32_t r = blockIdx.x * 64u + threadIdx.x;
-    if (r >= count) return;
-    const uint32_t step = first + r;
+// nothing.  One workgroup per step: waves 1-3 stream the row through LDS in
+// chunks of kFoldChunk float4 (double-buffered) while lane 0 of wave 0 adds
+// the previous chunk in order, so the serial chain never waits on memory.
+constexpr uint32_t kFoldChunk = 1024;
+
+__global__ __launch_bounds__(kBlock) void k_loss_fold(LossParts parts, uint32_t first, float* __restrict__ loss) {
+    __shared__ float4 chunk[2][kFoldChunk];
+    const uint32_t step = first + blockIdx.x, tid = threadIdx.x;
     const float4* row = reinterpret_cast<const float4*>(parts.part + static_cast<size_t>(step % parts.period) * parts.row);
     const uint32_t n4 = static_cast<uint32_t>(parts.row / 4u);
-    constexpr uint32_t kAhead = 8;
-    float4 buf[kAhead];
-    const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-#pragma unroll
-    for (uint32_t i = 0; i < kAhead; ++i) buf[i] = i < n4 ? row[i] : zero;
+    constexpr uint32_t kLoaders = kBlock - 64u;
+    auto stage = [&](uint32_t c, uint32_t buf) {  // chunk starting at float4 c into chunk[buf] (waves 1-3)
+        if (tid < 64u) return;
+        for (uint32_t i = tid - 64u; i < kFoldChunk && c + i < n4; i += kLoaders) chunk[buf][i] = row[c + i];
+    };
+    stage(0, 0);
+    __syncthreads();
     float acc = 0.0f;
-    for (uint32_t c = 0; c < n4; c += kAhead) {
-        float4 cur[kAhead];
-#pragma unroll
-        for (uint32_t i = 0; i < kAhead; ++i) {
-            cur[i] = buf[i];
-            buf[i] = c + kAhead + i < n4 ? row[c + kAhead + i] : zero;
+    uint32_t buf = 0;
+    for (uint32_t c = 0; c < n4; c += kFoldChunk, buf ^= 1u) {
+        stage(c + kFoldChunk, buf ^ 1u);
+        if (tid == 0) {
+            const uint32_t m = n4 - c < kFoldChunk ? n4 - c : kFoldChunk;
+            for (uint32_t i = 0; i < m; ++i) {
+                const float4 v = chunk[buf][i];
+                acc += v.x;
+                acc += v.y;
+                acc += v.z;
+                acc += v.w;
+            }
         }
-#pragma unroll
-        for (uint32_t i = 0; i < kAhead; ++i) {
-            acc += cur[i].x;
-            acc += cur[i].y;
-            acc += cur[i].z;
-            acc += cur[i].w;
-        }
+        __syncthreads();
     }
-    loss[step] = acc;
+    if (tid == 0) loss[step] = acc;
 }
 
 // Texel t of a tw-wide atlas as a key in 8x8-tile order (tiles_x = tiles per
@@ -906,7 +911,7 @@ PhotoLossArgs photo_loss_args(const RefineView* views_dev, uint32_t batch, const
 
 hipError_t loss_fold(const LossParts& parts, uint32_t first, uint32_t count, float* loss, hipStream_t st) {
     if (!count) return hipSuccess;
-    k_loss_fold<<<(count + 63u) / 64u, 64, 0, st>>>(parts, first, count, loss);
+    k_loss_fold<<<count, kBlock, 0, st>>>(parts, first, loss);
     return hipGetLastError();
 }
 
