@@ -1534,23 +1534,40 @@ __global__ void k_sort_corner_lists(uint32_t T, uint32_t V, uint32_t* state, con
 }
 
 // reduce_vertex_gradients.hlsl:12-29 for every view: the corner gradients sit
-// in the adjacency's order, so each vertex sums a contiguous run.
+// in the adjacency's order, so each vertex sums a contiguous run, and a block's
+// 256 vertices sum one contiguous range.  The block streams that range through
+// LDS in chunks of kReduceChunk corners (coalesced float4 loads); each lane adds
+// the part of its run inside the chunk, in corner order, chunk after chunk.
+constexpr uint32_t kReduceChunk = 2048;
+
 __global__ __launch_bounds__(kBlock) void k_vertex_reduce(ViewTable vt, uint32_t T, uint32_t V,
                                                           const uint32_t* __restrict__ offsets,
                                                           const float4* __restrict__ corner) {
+    __shared__ float4 chunk[kReduceChunk];
     const uint32_t view = blockIdx.y;
     const ViewDesc& vd = vt.v[view];
     const float4* cg = corner + static_cast<size_t>(view) * T * 3;
-    for (uint32_t v = blockIdx.x * kBlock + threadIdx.x; v < V; v += gridDim.x * kBlock) {
+    for (uint32_t v0 = blockIdx.x * kBlock; v0 < V; v0 += gridDim.x * kBlock) {
+        const uint32_t v = v0 + threadIdx.x;
+        const uint32_t last = v0 + kBlock < V ? v0 + kBlock : V;
+        const uint32_t lo = offsets[v0], hi = offsets[last];  // the block's corner range
+        const uint32_t a0 = v < V ? offsets[v] : hi, a1 = v < V ? offsets[v + 1] : hi;
         float4 s = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        for (uint32_t a = offsets[v]; a < offsets[v + 1]; ++a) {
-            const float4 c = cg[a];
-            s.x += c.x;
-            s.y += c.y;
-            s.z += c.z;
-            s.w += c.w;
+        for (uint32_t c = lo; c < hi; c += kReduceChunk) {
+            const uint32_t n = hi - c < kReduceChunk ? hi - c : kReduceChunk;
+            __syncthreads();  // the previous chunk's reads are done
+            for (uint32_t i = threadIdx.x; i < n; i += kBlock) chunk[i] = cg[c + i];
+            __syncthreads();
+            const uint32_t b = a0 > c ? a0 : c, e = a1 < c + n ? a1 : c + n;
+            for (uint32_t a = b; a < e; ++a) {
+                const float4 q = chunk[a - c];
+                s.x += q.x;
+                s.y += q.y;
+                s.z += q.z;
+                s.w += q.w;
+            }
         }
-        reinterpret_cast<float4*>(vd.grad_pos)[v] = s;
+        if (v < V) reinterpret_cast<float4*>(vd.grad_pos)[v] = s;
     }
 }
 
