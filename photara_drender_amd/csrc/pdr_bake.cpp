@@ -193,6 +193,22 @@ void run_bake(pdr_context ctx, const float* positions, const float* normals, con
     }
     float* clips = bufs.alloc<float>(static_cast<size_t>(std::min(nviews, kBakeChunk)) * V * 4);
     pdr::BakeView* dviews = bufs.alloc<pdr::BakeView>(kBakeChunk);
+    // every view's shadow-map keys are kept in the context with the mesh they
+    // came from, for a refinement of the same views (pdr_refine)
+    auto& cache = ctx->view_keys;
+    cache = {};
+    cache.T = T;
+    cache.V = V;
+    size_t cache_pixels = 0;
+    for (uint32_t k = 0; k < nviews; ++k) cache_pixels += static_cast<size_t>(views[k].width) * views[k].height;
+    unsigned long long* cache_keys = ctx->get_t<unsigned long long>("view_keys", cache_pixels);
+    hip_check(hipMemcpyAsync(ctx->get_t<float>("view_keys_pos", static_cast<size_t>(V) * 3), dpos,
+                             static_cast<size_t>(V) * 3 * sizeof(float), hipMemcpyDeviceToDevice, st),
+              "hipMemcpyAsync");
+    hip_check(hipMemcpyAsync(ctx->get_t<uint32_t>("view_keys_idx", static_cast<size_t>(T) * 3), didx,
+                             static_cast<size_t>(T) * 3 * sizeof(uint32_t), hipMemcpyDeviceToDevice, st),
+              "hipMemcpyAsync");
+    size_t cache_off = 0;
     for (uint32_t k0 = 0; k0 < nviews; k0 += kBakeChunk) {
         const uint32_t n = std::min(nviews - k0, kBakeChunk);
         DeviceBuffers chunk_bufs(ctx, features ? "bake_features_chunk" : "bake_chunk");
@@ -225,7 +241,21 @@ void run_bake(pdr_context ctx, const float* positions, const float* normals, con
         finalize_table(vt);
         // shadow maps (:266-276): each view's raster keys, read as (depth, covered) by the projection
         const unsigned long long* keys = run_raster_keys(ctx, vt, didx, T, V, 0, st);
-        for (uint32_t j = 0; j < n; ++j) hv[j].shadow = keys + vt.v[j].pixel_base;
+        for (uint32_t j = 0; j < n; ++j) {
+            hv[j].shadow = keys + vt.v[j].pixel_base;
+            const size_t pix = static_cast<size_t>(vt.v[j].width) * vt.v[j].height;
+            hip_check(hipMemcpyAsync(cache_keys + cache_off, keys + vt.v[j].pixel_base, pix * 8,
+                                     hipMemcpyDeviceToDevice, st),
+                      "hipMemcpyAsync");
+            cache.width.push_back(vt.v[j].width);
+            cache.height.push_back(vt.v[j].height);
+            cache.viewport.push_back({vt.v[j].vx, vt.v[j].vy, vt.v[j].vw, vt.v[j].vh});
+            std::array<float, 16> m;
+            std::memcpy(m.data(), views[k0 + j].world_to_clip, sizeof(m));
+            cache.matrix.push_back(m);
+            cache.offset.push_back(cache_off);
+            cache_off += pix;
+        }
         hip_check(hipMemcpyAsync(dviews, hv.data(), n * sizeof(pdr::BakeView), hipMemcpyHostToDevice, st),
                   "hipMemcpyAsync");
         if (features)

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -94,6 +95,19 @@ struct pdr_context_s {
     std::unordered_map<std::string, Slot> scratch;
     bool keys_clean = false;  // raster key buffer is all-ones
     std::map<std::string, pdr::TuneParams> tuned;  // autotuner: chosen launch parameters per workload class
+
+    // The last bake's shadow-map keys (each view's raster candidate pass: the
+    // winning (depth, id) word per pixel) in the "view_keys" slot, with what
+    // they were computed from: the mesh in "view_keys_pos" / "view_keys_idx"
+    // and each view's size, resolved viewport and matrix.  A refinement whose
+    // view k matches view k here resolves these keys instead of rasterising.
+    struct ViewKeys {
+        uint32_t T = 0, V = 0;
+        std::vector<uint32_t> width, height;
+        std::vector<std::array<float, 4>> viewport;
+        std::vector<std::array<float, 16>> matrix;
+        std::vector<size_t> offset;  // first key of each view in "view_keys"
+    } view_keys;
 
     // Optional per-kernel HIP-event timing (pdr_context_set_timing).
     bool timing = false;
@@ -495,6 +509,27 @@ inline void run_raster_forward(pdr_context ctx, const pdr::ViewTable& vt, const 
     const pdr::TuneParams tp = tuned_params(ctx, tune_key("forward_out", vt, T), forward_candidates(true), st,
                                             [&](const pdr::TuneParams& c) { run(c, false); });
     run(tp, true);
+    ctx->keys_clean = true;
+}
+
+// The forward's resolve pass over keys taken from `src_keys` (view j's keys at
+// src_keys + offsets[j]), e.g. a bake's cached shadow-map keys of the same
+// views: the same raster as run_raster_forward (the winner is the same
+// lexicographic minimum), without the candidate pass.
+inline void run_raster_resolve_keys(pdr_context ctx, const pdr::ViewTable& vt, const uint32_t* idx, uint32_t T,
+                                    int want_deriv, const unsigned long long* src_keys,
+                                    const std::vector<size_t>& offsets, hipStream_t st) {
+    if (!vt.total_pixels) return;
+    pdr::RasterScratch s = raster_scratch(ctx, vt, T, st, true);
+    ctx->keys_clean = false;
+    for (uint32_t j = 0; j < vt.n; ++j)
+        hip_check(hipMemcpyAsync(s.keys + vt.v[j].pixel_base, src_keys + offsets[j],
+                                 static_cast<size_t>(vt.v[j].width) * vt.v[j].height * sizeof(unsigned long long),
+                                 hipMemcpyDeviceToDevice, st),
+                  "hipMemcpyAsync");
+    ctx->timed("raster_resolve", st, [&] {
+        hip_check(pdr::raster_resolve(vt, idx, want_deriv, s, st), "raster_resolve");  // resets the keys
+    });
     ctx->keys_clean = true;
 }
 

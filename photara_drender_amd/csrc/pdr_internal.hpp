@@ -122,6 +122,8 @@ hipError_t raster_tiled(const ViewTable& views, const uint32_t* idx, uint32_t T,
 hipError_t raster_resolve(const ViewTable& views, const uint32_t* idx, int want_deriv, const RasterScratch& s,
                           hipStream_t st);
 size_t csr_total_words(uint32_t V);
+// *flag |= 1 when any of the n words of a and b differ.
+hipError_t mark_if_differ(const uint32_t* a, const uint32_t* b, size_t n, uint32_t* flag, hipStream_t st);
 hipError_t build_vertex_csr(const uint32_t* idx, uint32_t T, uint32_t V, const VertexCsr& csr, hipStream_t st);
 // k_bwd_prepass: per-pixel compared ids and per-view whole-box flags (raster, grad_raster of each view)
 hipError_t backward_prepass(const ViewTable& views, const uint32_t* idx, uint32_t T, uint32_t V,

@@ -147,6 +147,38 @@ pdr_status pdr_refine(pdr_context ctx, const float* initial_texture, const float
         size_t arena_off = 0;
         std::vector<pdr::PrecomputeView> hpre(group);
         std::vector<float> hmats(static_cast<size_t>(group) * 16);
+        // views the last bake on this context rasterised for its shadow maps (same
+        // mesh, same position in the view list, size, viewport and matrix): their
+        // keys are resolved instead of rasterising the views again
+        std::vector<bool> cached(nviews, false);
+        {
+            const auto& c = ctx->view_keys;
+            bool any = false;
+            if (c.T == T && c.V == V)
+                for (uint32_t k = 0; k < nviews && k < c.width.size(); ++k) {
+                    const pdr_refine_view& v = views[k];
+                    float vp[4];
+                    resolve_viewport(v.width, v.height, v.has_viewport ? v.viewport : nullptr, vp);
+                    cached[k] = c.width[k] == v.width && c.height[k] == v.height &&
+                                std::memcmp(c.viewport[k].data(), vp, sizeof(vp)) == 0 &&
+                                std::memcmp(c.matrix[k].data(), v.world_to_clip, 16 * sizeof(float)) == 0;
+                    any |= cached[k];
+                }
+            if (any) {  // the mesh itself, compared on the device
+                uint32_t* flag = ctx->get_t<uint32_t>("view_keys_flag", 1);
+                hip_check(pdr::fill_u32(flag, 0u, 1, st), "fill");
+                hip_check(pdr::mark_if_differ(reinterpret_cast<const uint32_t*>(dpos),
+                                              ctx->get_t<uint32_t>("view_keys_pos", static_cast<size_t>(V) * 3),
+                                              static_cast<size_t>(V) * 3, flag, st),
+                          "mark_if_differ");
+                hip_check(pdr::mark_if_differ(didx, ctx->get_t<uint32_t>("view_keys_idx", static_cast<size_t>(T) * 3),
+                                              static_cast<size_t>(T) * 3, flag, st),
+                          "mark_if_differ");
+                hip_check(hipMemcpyAsync(ctx->pinned, flag, 4, hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
+                sync(st);
+                if (ctx->pinned[0]) std::fill(cached.begin(), cached.end(), false);
+            }
+        }
         for (uint32_t g0 = 0; g0 < nviews; g0 += group) {
             const uint32_t gn = std::min(group, nviews - g0);
             for (uint32_t b = 0; b < gn; ++b) std::memcpy(&hmats[b * 16], views[g0 + b].world_to_clip, 16 * 4);
@@ -207,7 +239,17 @@ pdr_status pdr_refine(pdr_context ctx, const float* initial_texture, const float
                 rviews[g0 + b] = pdr::RefineView{pre.uvm, pre.rgb, pre.lterm, 0u, P};  // active: set once read back
             }
             finalize_table(vt);
-            run_raster_forward(ctx, vt, didx, T, V, 0, 0, st);
+            bool group_cached = true;
+            std::vector<size_t> key_offsets(gn);
+            for (uint32_t b = 0; b < gn; ++b) {
+                group_cached = group_cached && cached[g0 + b];
+                if (cached[g0 + b]) key_offsets[b] = ctx->view_keys.offset[g0 + b];
+            }
+            if (group_cached)
+                run_raster_resolve_keys(ctx, vt, didx, T, 0, ctx->get_t<unsigned long long>("view_keys", 1),
+                                        key_offsets, st);
+            else
+                run_raster_forward(ctx, vt, didx, T, V, 0, 0, st);
             hip_check(pdr::interp_forward(pt, duv, V, 2, didx, T, st), "interp_forward");
             hip_check(hipMemcpyAsync(dpre, hpre.data(), gn * sizeof(pdr::PrecomputeView), hipMemcpyHostToDevice, st),
                       "hipMemcpyAsync");

@@ -1571,6 +1571,15 @@ __global__ __launch_bounds__(kBlock) void k_vertex_reduce(ViewTable vt, uint32_t
     }
 }
 
+__global__ void k_mark_if_differ(const uint32_t* __restrict__ a, const uint32_t* __restrict__ b, size_t n,
+                                 uint32_t* flag) {
+    bool diff = false;
+    for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<size_t>(gridDim.x) * blockDim.x)
+        diff |= a[i] != b[i];
+    if (__any(diff) && (threadIdx.x & 63u) == 0) atomicOr(flag, 1u);
+}
+
 __global__ void k_check_indices(const uint32_t* idx, size_t n, uint32_t V, uint32_t* flags) {
     bool bad = false;
     for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n;
@@ -1731,6 +1740,12 @@ hipError_t build_vertex_csr(const uint32_t* idx, uint32_t T, uint32_t V, const V
                                                     c.saved);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     k_sort_corner_lists<<<grid_for(V), kBlock, 0, st>>>(T, V, c.state, c.offsets, c.corners, c.slot_of, c.counts);
+    return hipGetLastError();
+}
+
+hipError_t mark_if_differ(const uint32_t* a, const uint32_t* b, size_t n, uint32_t* flag, hipStream_t st) {
+    if (!n) return hipSuccess;
+    k_mark_if_differ<<<grid_for(n), kBlock, 0, st>>>(a, b, n, flag);
     return hipGetLastError();
 }
 

@@ -137,6 +137,37 @@ def test_refine_stop_after_equals_the_full_runs_checkpoint(tv):
             assert_bits(part.color, full.checkpoints[stop // 10 - 1], f"atlas after step {stop}")
 
 
+def test_refine_after_a_bake_resolves_its_shadow_keys():
+    """After a bake on the same context, pdr_refine resolves the bake's shadow-map
+    keys of the same views (same mesh, size, viewport and matrix) instead of
+    rasterising them again (no candidate pass runs); the refinement equals one on a
+    fresh context bit for bit.  A bake of a moved mesh is not reused."""
+    torch = pytest.importorskip("torch")
+    from photara_drender_amd import _native
+    from photara_drender_amd.bake import bake_atlas
+    from photara_drender_amd.refine import refine_atlas
+
+    sc, mats, masks, images, pairs = small_atlas(views=4)
+    dev = torch.device("cuda:0")
+    t = lambda a: torch.tensor(np.ascontiguousarray(a), device=dev)
+    nrm = scenes.vertex_normals(sc.positions, sc.indices)
+    cams = np.stack([scenes.camera_center(m) for m in mats])
+    init = np.full((64, 64, 3), 0.5, np.float32)
+    kw = dict(seam_uv_pairs=pairs, steps=12, batch_size=2, learning_rate=1e-2, seam_polish_steps=2)
+    ref = refine_atlas(_native.Context(0), init, sc.positions, sc.uv, sc.indices, images, mats, masks, **kw)
+    for moved in (0.0, 0.01):
+        ctx = _native.Context(0)
+        bake_atlas(ctx, t(sc.positions + np.float32(moved)), t(nrm), t(sc.uv), t(sc.indices.view(np.int32)),
+                   [t(im) for im in images], mats, cams, [t(m) for m in masks], resolution=(64, 64))
+        ctx.set_timing(True)
+        got = refine_atlas(ctx, init, sc.positions, sc.uv, sc.indices, images, mats, masks, **kw)
+        candidates = ctx.kernel_time("raster_candidates")[1] + ctx.kernel_time("raster_tiled")[1]
+        ctx.set_timing(False)
+        assert_bits(got.color, ref.color, f"colour (mesh moved by {moved})")
+        assert_bits(got.loss_history, ref.loss_history, f"loss history (mesh moved by {moved})")
+        assert (candidates == 0) == (moved == 0.0), candidates
+
+
 def test_refine_extended_objective_runs_and_reduces_seams():
     torch = pytest.importorskip("torch")
     from photara_drender_amd import _native
