@@ -205,6 +205,15 @@ struct LossParts {
     size_t row;
 };
 
+// The step a refinement kernel works on: the device step counter plus the
+// step's position in a captured block of steps (the counter moves on once per
+// block, or once per step outside graphs).
+struct StepIndex {
+    const uint32_t* counter;
+    uint32_t add;
+};
+__device__ __forceinline__ uint32_t step_of(const StepIndex& s) { return *s.counter + s.add; }
+
 // The bilinear taps of one recurring step batch (the views of its batch slots)
 // merged and grouped by texel, ascending.  Entry = (slot * stride + active
 // pixel) * 4 + tap; within a texel the entries keep slot-then-pixel order.
@@ -263,7 +272,7 @@ struct SeamResidualArgs {
 hipError_t photometric_step(const RefineView* views_dev, uint32_t batch, uint32_t max_active,
                             const RefineBatchTaps* batches_dev, uint32_t nbatches, uint32_t max_runs,
                             const float* tex, uint32_t tw, uint32_t th, float eps, const RefineStepTables& tables,
-                            const uint32_t* step_counter, float4* resid, uint32_t stride, float* grad,
+                            StepIndex step_at, float4* resid, uint32_t stride, float* grad,
                             const SeamResidualArgs& seam, hipStream_t st);
 // The step's photometric group losses, computed by extra blocks of the Adam
 // kernel that follows the step's gather (blocks = 0: none).
@@ -317,7 +326,7 @@ struct TapGroups {
 };
 // loss_part (or null): [step][group] losses of the shader's 64-pair groups, step = device counter.
 hipError_t seam_step(const float* tex, uint32_t tw, uint32_t th, const float* pairs, uint32_t npairs, float eps,
-                     float norm, float weight, const uint32_t* step_counter, const TapGroups& taps, float4* resid,
+                     float norm, float weight, StepIndex step_at, const TapGroups& taps, float4* resid,
                      float* grad, float* loss_part, bool with_residual, hipStream_t st);
 // loss[k] for seam steps k < steps from their group partials.
 hipError_t seam_loss_fold(const float* part, uint32_t npairs, uint32_t steps, float* loss, hipStream_t st);
@@ -328,15 +337,15 @@ hipError_t seam_tap_weights(const uint32_t* vals, uint32_t E, const float* pairs
 hipError_t adam_tv_step(const float* tex, float* tex_out, float* grad, float* m, float* v, uint32_t tw, uint32_t th,
                         const uint8_t* tv_mask, float tv_eps, float tv_scale, const float* lr_table,
                         const float* bc1_table, const float* bc2_table, float b1, float b2, float eps, float cmin,
-                        float cmax, const uint32_t* step_counter, const PhotoLossArgs& ploss, hipStream_t st);
-hipError_t advance_step(uint32_t* step_counter, hipStream_t st);
+                        float cmax, StepIndex step_at, const PhotoLossArgs& ploss, hipStream_t st);
+hipError_t advance_step(uint32_t* step_counter, uint32_t count, hipStream_t st);
 // Per-texel neighbour mask of the chart-aware TV (chart may be null: one chart).
 hipError_t tv_mask(const int32_t* chart, uint32_t tw, uint32_t th, uint8_t* mask, hipStream_t st);
 hipError_t tv_gradient(const float* tex, uint32_t tw, uint32_t th, const int32_t* chart, float eps, float scale,
                        float* grad, hipStream_t st);
 hipError_t adam_step(float* tex, float* grad, float* m, float* v, uint32_t n, const float* lr_table,
                      const float* bc1_table, const float* bc2_table, float b1, float b2, float eps, float cmin,
-                     float cmax, const uint32_t* step_counter, const PhotoLossArgs& ploss, hipStream_t st);
+                     float cmax, StepIndex step_at, const PhotoLossArgs& ploss, hipStream_t st);
 hipError_t tv_pair_count(const int32_t* chart, uint32_t tw, uint32_t th, unsigned long long* out, hipStream_t st);
 
 // ---- bake.hip

@@ -482,35 +482,40 @@ pdr_status pdr_refine(pdr_context ctx, const float* initial_texture, const float
         const auto t2 = std::chrono::steady_clock::now();
         const bool joint_seam = o.joint_seam_weight != 0.0f && npairs;
         const pdr::PhotoLossArgs ploss = pdr::photo_loss_args(dviews, o.batch_size, parts);
-        auto one_step = [&](hipStream_t s, uint32_t c) {
+        // k: the step's position in a captured block (its kernels add it to the
+        // device step counter, which moves on once per block); outside graphs
+        // k = 0 and the counter moves on after every step
+        auto one_step = [&](hipStream_t s, uint32_t c, uint32_t k, bool advance) {
+            const pdr::StepIndex at{counters, k};
             pdr::SeamResidualArgs seam_res;  // the joint seam residual rides in the photometric residual launch
             if (joint_seam)
                 seam_res = pdr::SeamResidualArgs{dpairs, npairs, o.seam_epsilon, seam_norm, o.joint_seam_weight,
                                                  seam_resid};
             hip_check(pdr::photometric_step(dviews, o.batch_size, max_active, dbatches, nbatches, max_runs, texbuf[c],
-                                            o.width, o.height, o.photometric_epsilon, tables, counters, resid,
+                                            o.width, o.height, o.photometric_epsilon, tables, at, resid,
                                             max_active, grad, seam_res, s),
                       "photometric_step");
             if (joint_seam)
                 hip_check(pdr::seam_step(texbuf[c], o.width, o.height, dpairs, npairs, o.seam_epsilon, seam_norm,
-                                         o.joint_seam_weight, counters, seam_groups, seam_resid, grad, nullptr,
+                                         o.joint_seam_weight, at, seam_groups, seam_resid, grad, nullptr,
                                          false, s),
                           "seam_step");
             if (fused_tv)
                 hip_check(pdr::adam_tv_step(texbuf[c], texbuf[c ^ 1u], grad, m1, m2, o.width, o.height, dtv_mask, tv_eps,
                                             tv_scale, d_lr, d_bc1, d_bc2, o.adam_beta1, o.adam_beta2, o.adam_epsilon,
-                                            o.clamp_min, o.clamp_max, counters, ploss, s),
+                                            o.clamp_min, o.clamp_max, at, ploss, s),
                           "adam_tv_step");
             else
                 hip_check(pdr::adam_step(texbuf[c], grad, m1, m2, n, d_lr, d_bc1, d_bc2, o.adam_beta1, o.adam_beta2,
-                                         o.adam_epsilon, o.clamp_min, o.clamp_max, counters, ploss, s),
+                                         o.adam_epsilon, o.clamp_min, o.clamp_max, at, ploss, s),
                           "adam_step");
-            hip_check(pdr::advance_step(counters, s), "advance_step");
+            if (advance) hip_check(pdr::advance_step(counters, 1u, s), "advance_step");
             return fused_tv ? c ^ 1u : c;
         };
         auto capture = [&](uint32_t count, uint32_t c, GraphExec& g) {
             hip_check(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-            for (uint32_t k = 0; k < count; ++k) c = one_step(st, c);
+            for (uint32_t k = 0; k < count; ++k) c = one_step(st, c, k, false);
+            hip_check(pdr::advance_step(counters, count, st), "advance_step");
             hip_check(hipStreamEndCapture(st, &g.graph), "hipStreamEndCapture");
             hip_check(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0), "hipGraphInstantiate");
         };
@@ -532,7 +537,7 @@ pdr_status pdr_refine(pdr_context ctx, const float* initial_texture, const float
                 hip_check(hipGraphLaunch(b < nblocks ? full[cur].exec : tail.exec, st), "hipGraphLaunch");
                 cur ^= count & flips;
             } else {
-                for (uint32_t k = 0; k < count; ++k) cur = one_step(st, cur);
+                for (uint32_t k = 0; k < count; ++k) cur = one_step(st, cur, 0u, true);
             }
             done += count;
             if (done - folded == parts.period || done == run) {
@@ -563,12 +568,14 @@ pdr_status pdr_refine(pdr_context ctx, const float* initial_texture, const float
             hip_check(pdr::fill_u32(counters, 0u, 2, st), "fill");
             for (uint32_t k = 0; k < seam_steps; ++k) {
                 hip_check(pdr::seam_step(tex, o.width, o.height, dpairs, npairs, o.seam_epsilon, seam_norm, 1.0f,
-                                         counters, seam_groups, seam_resid, grad, seam_part, true, st),
+                                         pdr::StepIndex{counters, 0u}, seam_groups, seam_resid, grad, seam_part, true,
+                                         st),
                           "seam_step");
                 hip_check(pdr::adam_step(tex, grad, m1, m2, n, d_slr, d_sb1, d_sb2, o.adam_beta1, o.adam_beta2,
-                                         o.adam_epsilon, o.clamp_min, o.clamp_max, counters, pdr::PhotoLossArgs{}, st),
+                                         o.adam_epsilon, o.clamp_min, o.clamp_max, pdr::StepIndex{counters, 0u},
+                                         pdr::PhotoLossArgs{}, st),
                           "adam_step");
-                hip_check(pdr::advance_step(counters, st), "advance_step");
+                hip_check(pdr::advance_step(counters, 1u, st), "advance_step");
             }
             hip_check(pdr::seam_loss_fold(seam_part, npairs, seam_steps, loss + o.steps, st), "seam_loss_fold");
         }

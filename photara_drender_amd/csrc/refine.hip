@@ -34,7 +34,9 @@
 //                     total variation of python/photara_drender/atlas.py:
 //                     84-115 (extension) added per texel first (hardware
 //                     square root and reciprocal)
-//  k_advance_step     moves the device step counter on after a step
+//  k_advance_step     moves the device step counter on after a step, or after
+//                     a captured block of steps (each kernel of the block adds
+//                     its step's position in the block: StepIndex)
 //
 // No gradient or loss uses float atomics (the reference emulates them with
 // compare-exchange loops): every texel's terms are added in (batch slot,
@@ -194,7 +196,7 @@ __device__ __forceinline__ void wave_lds_sync() {
 __global__ __launch_bounds__(kBlock) void k_photo_residual(const RefineView* __restrict__ views, uint32_t batch,
                                                            const float* __restrict__ tex, uint32_t tw, uint32_t th,
                                                            float eps, RefineStepTables tables,
-                                                           const uint32_t* __restrict__ step_counter,
+                                                           StepIndex step_at,
                                                            float4* __restrict__ resid, uint32_t max_active,
                                                            SeamResidualArgs seam) {
     if (blockIdx.y == batch) {  // the joint seam term's residual, beside the views' (no loss)
@@ -202,7 +204,7 @@ __global__ __launch_bounds__(kBlock) void k_photo_residual(const RefineView* __r
             (void)seam_pair_residual(q, tex, tw, th, seam.pairs, seam.eps, seam.norm, seam.weight, seam.resid);
         return;
     }
-    const uint32_t step = *step_counter;
+    const uint32_t step = step_of(step_at);
     const uint32_t b = blockIdx.y;
     const RefineView& rv = views[static_cast<size_t>(step) * batch + b];  // the step slot's view record
     const uint32_t a = blockIdx.x * kBlock + threadIdx.x;
@@ -390,10 +392,10 @@ __device__ __forceinline__ void texel_gather(const uint32_t* __restrict__ texels
 
 __global__ __launch_bounds__(kBlock) void k_photo_gather(const RefineBatchTaps* __restrict__ batches, uint32_t nbatches,
                                                          uint32_t tw, uint32_t tiles_x,
-                                                         const uint32_t* __restrict__ step_counter,
+                                                         StepIndex step_at,
                                                          const float4* __restrict__ resid, float* __restrict__ grad) {
     __shared__ float4 prod[kBlock / 64][kGatherWindow];
-    const RefineBatchTaps& bt = batches[*step_counter % nbatches];
+    const RefineBatchTaps& bt = batches[step_of(step_at) % nbatches];
     texel_gather<2>(bt.texels, bt.tap_offsets, bt.taps, bt.tap_weights, *bt.runs, resid, tw, tiles_x, grad, prod);
 }
 
@@ -569,13 +571,13 @@ __global__ void k_set_last(uint32_t* __restrict__ offsets, const uint32_t* __res
 __global__ __launch_bounds__(kBlock) void k_seam_residual(const float* __restrict__ tex, uint32_t tw, uint32_t th,
                                                           const float* __restrict__ pairs, uint32_t npairs, float eps,
                                                           float norm, float weight,
-                                                          const uint32_t* __restrict__ step_counter,
+                                                          StepIndex step_at,
                                                           float4* __restrict__ resid, float* __restrict__ part) {
     const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
     const float local = q < npairs ? seam_pair_residual(q, tex, tw, th, pairs, eps, norm, weight, resid) : 0.0f;
     const float s = wave_tree_sum(local);
     const uint32_t groups = (npairs + 63u) / 64u;
-    if (part && (threadIdx.x & 63u) == 0) part[static_cast<size_t>(*step_counter) * groups + (q >> 6)] = s;
+    if (part && (threadIdx.x & 63u) == 0) part[static_cast<size_t>(step_of(step_at)) * groups + (q >> 6)] = s;
 }
 
 // loss[k] of seam step k < steps: its group losses added in group order, zeros skipped.
@@ -753,9 +755,9 @@ __global__ __launch_bounds__(kBlock) void k_adam_tv(const float* __restrict__ te
                                                     const uint8_t* __restrict__ tv_mask, float tv_eps2, float tv_scale,
                                                     const float* __restrict__ lr_t, const float* __restrict__ bc1_t,
                                                     const float* __restrict__ bc2_t, float b1, float b2, float eps,
-                                                    float cmin, float cmax, const uint32_t* __restrict__ step_counter,
+                                                    float cmin, float cmax, StepIndex step_at,
                                                     PhotoLossArgs ploss) {
-    const uint32_t step = *step_counter;
+    const uint32_t step = step_of(step_at);
     if (blockIdx.x < ploss.blocks) {  // the step's photometric group losses first
         photo_loss_groups(ploss.views, ploss.batch, step, ploss.parts, blockIdx.x * (kBlock / 64u) + (threadIdx.x >> 6));
         return;
@@ -795,17 +797,17 @@ __global__ void k_tv_gradient(const float* __restrict__ tex, uint32_t tw, uint32
     for (int ch = 0; ch < 3; ++ch) grad[static_cast<size_t>(i) * 3 + ch] = g[ch] * scale;
 }
 
-__global__ void k_advance_step(uint32_t* step_counter) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *step_counter += 1u;
+__global__ void k_advance_step(uint32_t* step_counter, uint32_t count) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *step_counter += count;
 }
 
 __global__ __launch_bounds__(kBlock) void k_adam(float* __restrict__ tex, float* __restrict__ grad,
                                                  float* __restrict__ mom1, float* __restrict__ mom2, uint32_t n,
                                                  const float* __restrict__ lr_t, const float* __restrict__ bc1_t,
                                                  const float* __restrict__ bc2_t, float b1, float b2, float eps,
-                                                 float cmin, float cmax, const uint32_t* __restrict__ step_counter,
+                                                 float cmin, float cmax, StepIndex step_at,
                                                  PhotoLossArgs ploss) {
-    const uint32_t step = *step_counter;
+    const uint32_t step = step_of(step_at);
     if (blockIdx.x < ploss.blocks) {  // the step's photometric group losses first
         photo_loss_groups(ploss.views, ploss.batch, step, ploss.parts, blockIdx.x * (kBlock / 64u) + (threadIdx.x >> 6));
         return;
@@ -888,15 +890,15 @@ uint64_t texel_key_range(uint32_t tw, uint32_t th) {
 hipError_t photometric_step(const RefineView* views_dev, uint32_t batch, uint32_t max_active,
                             const RefineBatchTaps* batches_dev, uint32_t nbatches, uint32_t max_runs,
                             const float* tex, uint32_t tw, uint32_t th, float eps, const RefineStepTables& tables,
-                            const uint32_t* step_counter, float4* resid, uint32_t stride, float* grad,
+                            StepIndex step_at, float4* resid, uint32_t stride, float* grad,
                             const SeamResidualArgs& seam, hipStream_t st) {
     const uint32_t blocks = (max_active + kBlock - 1) / kBlock;
     const dim3 grid(blocks ? blocks : 1, batch + (seam.npairs ? 1u : 0u));
-    k_photo_residual<<<grid, kBlock, 0, st>>>(views_dev, batch, tex, tw, th, eps, tables, step_counter, resid, stride,
+    k_photo_residual<<<grid, kBlock, 0, st>>>(views_dev, batch, tex, tw, th, eps, tables, step_at, resid, stride,
                                               seam);
     const uint32_t gblocks = (max_runs + kBlock - 1) / kBlock;
     k_photo_gather<<<gblocks ? gblocks : 1, kBlock, 0, st>>>(batches_dev, nbatches, tw, texel_key_tiles(tw, th),
-                                                             step_counter, resid, grad);
+                                                             step_at, resid, grad);
     return hipGetLastError();
 }
 
@@ -977,12 +979,12 @@ hipError_t photo_group_taps(const uint32_t* keys_in, const uint32_t* vals_in, ui
 }
 
 hipError_t seam_step(const float* tex, uint32_t tw, uint32_t th, const float* pairs, uint32_t npairs, float eps,
-                     float norm, float weight, const uint32_t* step_counter, const TapGroups& taps, float4* resid,
+                     float norm, float weight, StepIndex step_at, const TapGroups& taps, float4* resid,
                      float* grad, float* loss_part, bool with_residual, hipStream_t st) {
     if (!npairs) return hipSuccess;
     if (with_residual)
         k_seam_residual<<<(npairs + kBlock - 1) / kBlock, kBlock, 0, st>>>(tex, tw, th, pairs, npairs, eps, norm,
-                                                                          weight, step_counter, resid, loss_part);
+                                                                          weight, step_at, resid, loss_part);
     if (taps.ntexels)
         k_seam_gather<<<(taps.ntexels + kBlock - 1) / kBlock, kBlock, 0, st>>>(
             taps.texels, taps.offsets, taps.taps, taps.weights, taps.ntexels, resid, grad);
@@ -1006,12 +1008,12 @@ hipError_t seam_tap_weights(const uint32_t* vals, uint32_t E, const float* pairs
 hipError_t adam_tv_step(const float* tex, float* tex_out, float* grad, float* m, float* v, uint32_t tw, uint32_t th,
                         const uint8_t* tv_mask, float tv_eps, float tv_scale, const float* lr_table,
                         const float* bc1_table, const float* bc2_table, float b1, float b2, float eps, float cmin,
-                        float cmax, const uint32_t* step_counter, const PhotoLossArgs& ploss, hipStream_t st) {
+                        float cmax, StepIndex step_at, const PhotoLossArgs& ploss, hipStream_t st) {
     uint32_t grid = (tw * th + kBlock - 1) / kBlock;
     if (grid > 8192) grid = 8192;
     k_adam_tv<<<grid + ploss.blocks, kBlock, 0, st>>>(tex, tex_out, grad, m, v, tw, th, tv_mask, tv_eps * tv_eps,
                                                       tv_scale, lr_table, bc1_table, bc2_table, b1, b2, eps, cmin,
-                                                      cmax, step_counter, ploss);
+                                                      cmax, step_at, ploss);
     return hipGetLastError();
 }
 
@@ -1026,8 +1028,8 @@ hipError_t tv_gradient(const float* tex, uint32_t tw, uint32_t th, const int32_t
     return hipGetLastError();
 }
 
-hipError_t advance_step(uint32_t* step_counter, hipStream_t st) {
-    k_advance_step<<<1, 64, 0, st>>>(step_counter);
+hipError_t advance_step(uint32_t* step_counter, uint32_t count, hipStream_t st) {
+    k_advance_step<<<1, 64, 0, st>>>(step_counter, count);
     return hipGetLastError();
 }
 
@@ -1040,13 +1042,13 @@ hipError_t tv_pair_count(const int32_t* chart, uint32_t tw, uint32_t th, unsigne
 
 hipError_t adam_step(float* tex, float* grad, float* m, float* v, uint32_t n, const float* lr_table,
                      const float* bc1_table, const float* bc2_table, float b1, float b2, float eps, float cmin,
-                     float cmax, const uint32_t* step_counter, const PhotoLossArgs& ploss, hipStream_t st) {
+                     float cmax, StepIndex step_at, const PhotoLossArgs& ploss, hipStream_t st) {
     const uint32_t n4 = n / 4;
     uint32_t grid = (n4 + kBlock - 1) / kBlock;
     if (grid > 8192) grid = 8192;
     if (grid == 0) grid = 1;
     k_adam<<<grid + ploss.blocks, kBlock, 0, st>>>(tex, grad, m, v, n, lr_table, bc1_table, bc2_table, b1, b2, eps,
-                                                   cmin, cmax, step_counter, ploss);
+                                                   cmin, cmax, step_at, ploss);
     return hipGetLastError();
 }
 
