@@ -580,16 +580,28 @@ __global__ __launch_bounds__(kBlock) void k_seam_residual(const float* __restric
     if (part && (threadIdx.x & 63u) == 0) part[static_cast<size_t>(step_of(step_at)) * groups + (q >> 6)] = s;
 }
 
-// loss[k] of seam step k < steps: its group losses added in group order, zeros skipped.
-__global__ void k_seam_loss_fold(const float* __restrict__ part, uint32_t groups, uint32_t steps,
-                                 float* __restrict__ loss) {
-    const uint32_t k = blockIdx.x * 64u + threadIdx.x;
-    if (k >= steps) return;
+// loss[k] of seam step k: its group losses added in group order, zeros
+// skipped.  One workgroup per step: the row is staged through LDS in coalesced
+// chunks and lane 0 adds each chunk from LDS, so the serial chain of adds never
+// waits on a global load.
+constexpr uint32_t kSeamFoldChunk = 4096;
+
+__global__ __launch_bounds__(kBlock) void k_seam_loss_fold(const float* __restrict__ part, uint32_t groups,
+                                                           float* __restrict__ loss) {
+    __shared__ float chunk[kSeamFoldChunk];
+    const uint32_t k = blockIdx.x, tid = threadIdx.x;
     const float* p = part + static_cast<size_t>(k) * groups;
     float acc = 0.0f;
-    for (uint32_t g = 0; g < groups; ++g)
-        if (p[g] != 0.0f) acc += p[g];
-    loss[k] = acc;
+    for (uint32_t c = 0; c < groups; c += kSeamFoldChunk) {
+        const uint32_t m = groups - c < kSeamFoldChunk ? groups - c : kSeamFoldChunk;
+        for (uint32_t i = tid; i < m; i += kBlock) chunk[i] = p[c + i];
+        __syncthreads();
+        if (tid == 0)
+            for (uint32_t i = 0; i < m; ++i)
+                if (chunk[i] != 0.0f) acc += chunk[i];
+        __syncthreads();
+    }
+    if (tid == 0) loss[k] = acc;
 }
 
 // Texel gradient of the seam term: entries (pair * 8 + tap * 2 + side) with the
@@ -919,7 +931,7 @@ hipError_t loss_fold(const LossParts& parts, uint32_t first, uint32_t count, flo
 
 hipError_t seam_loss_fold(const float* part, uint32_t npairs, uint32_t steps, float* loss, hipStream_t st) {
     if (!steps || !npairs) return hipSuccess;
-    k_seam_loss_fold<<<(steps + 63u) / 64u, 64, 0, st>>>(part, (npairs + 63u) / 64u, steps, loss);
+    k_seam_loss_fold<<<steps, kBlock, 0, st>>>(part, (npairs + 63u) / 64u, loss);
     return hipGetLastError();
 }
 

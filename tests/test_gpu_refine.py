@@ -89,6 +89,28 @@ def test_refine_atlas_scene_matches_oracle():
     assert hist[-1] < hist[0]
 
 
+def test_refine_seam_fold_over_many_pairs_matches_oracle():
+    """Over 262,144 seam pairs (the pair list repeated): each seam step's loss
+    adds more than 4,096 group sums, so the fold stages its row in several
+    chunks; the history stays bit-exact."""
+    import photara_drender_amd as pdr
+
+    sc, mats, masks, images, pairs = small_atlas(views=2)
+    many = np.concatenate([pairs] * (262144 // len(pairs) + 2))
+    assert (len(many) + 63) // 64 > 4096
+    init = np.full((64, 64, 3), 0.5, np.float32)
+    kw = dict(steps=2, batch_size=2, learning_rate=1e-2, seam_polish_steps=3, seam_learning_rate=1e-3)
+    baker = pdr.TextureBaker()
+    color, hist, seam, _, _ = baker.refine_texture(init, sc.positions, sc.uv, sc.indices, images, mats,
+                                                   visibility_masks=masks, seam_uv_pairs=many, **kw)
+    ref_c, ref_h, ref_s = oracle.refine(init, sc.positions, sc.uv, sc.indices, images, mats, visibility_masks=masks,
+                                        seam_uv_pairs=many, **kw)
+    assert seam.shape == (3,) and (seam > 0).all()
+    assert_bits(seam, ref_s, "seam loss history")
+    assert_bits(hist, ref_h, "loss history")
+    assert_bits(color, ref_c, "colour")
+
+
 def test_refine_graph_matches_eager_and_checkpoints():
     torch = pytest.importorskip("torch")
     from photara_drender_amd import _native
