@@ -185,6 +185,32 @@ def trace_summary(directory, limit=40):
     return "\n".join(lines)
 
 
+def autotune_effect(step, ctx, steps, warmup):
+    """The workload's step on a second context without PDR_FLAG_AUTOTUNE (the default launch parameters)
+    against the tuned context: blocks of steps alternate between the two so clock and cache drift fall on
+    both alike; results are bit-identical either way."""
+    import torch
+
+    from photara_drender_amd import _native
+
+    plain = _native.Context(0, flags=0)
+    for _ in range(warmup + 1):
+        step(plain)
+    blocks, per = 4, max(1, steps // 4)
+    spent = {"tuned": 0.0, "default": 0.0}
+    for b in range(2 * blocks):
+        name, c = ("tuned", ctx) if b % 2 == 0 else ("default", plain)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(per):
+            step(c)
+        torch.cuda.synchronize()
+        spent[name] += time.perf_counter() - t0
+    tuned_ms, default_ms = (spent[k] / (blocks * per) * 1e3 for k in ("tuned", "default"))
+    return {"default_ms_per_step": default_ms, "tuned_ms_per_step": tuned_ms, "speedup": default_ms / tuned_ms,
+            "steps_each": blocks * per}
+
+
 def load_traffic(kernel):
     """Calibrated HBM bytes per launch of ``kernel`` from the committed --pmc summary (None if absent)."""
     try:
@@ -701,14 +727,14 @@ def main():
     lib = _native.lib()
     stream = torch.cuda.current_stream().cuda_stream
 
-    def step():
-        _native.check(lib.pdr_rasterize_forward_batched(ctx.handle, rviews, n, V, idx.data_ptr(), T, 0, 1, stream))
-        _native.check(lib.pdr_interpolate_forward_batched(ctx.handle, iviews, n, attr.data_ptr(), V, A,
+    def step(c=ctx):
+        _native.check(lib.pdr_rasterize_forward_batched(c.handle, rviews, n, V, idx.data_ptr(), T, 0, 1, stream))
+        _native.check(lib.pdr_interpolate_forward_batched(c.handle, iviews, n, attr.data_ptr(), V, A,
                                                           idx.data_ptr(), T, stream))
         torch.sub(interp, target, out=grad)  # d/dimg of 0.5 * sum (img - target)^2
-        _native.check(lib.pdr_interpolate_backward_batched(ctx.handle, iviews, n, attr.data_ptr(), V, A,
+        _native.check(lib.pdr_interpolate_backward_batched(c.handle, iviews, n, attr.data_ptr(), V, A,
                                                            idx.data_ptr(), T, grad_attr.data_ptr(), stream))
-        _native.check(lib.pdr_rasterize_backward_batched(ctx.handle, rviews, n, V, idx.data_ptr(), T, stream))
+        _native.check(lib.pdr_rasterize_backward_batched(c.handle, rviews, n, V, idx.data_ptr(), T, stream))
 
     if args.traffic_probe:
         step()
@@ -726,14 +752,18 @@ def main():
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
-    ctx.set_timing(True)
-    torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     ms_per_step = elapsed / args.steps * 1e3
+    # per-kernel times from a second pass of the same steps with the library's HIP-event timing on (the
+    # events between regions are kept out of the timed steps above)
+    ctx.set_timing(True)
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
     kernels = {}
     for name in ("raster_tiled", "raster_candidates", "raster_resolve", "interp_fwd", "interp_bwd", "vertex_csr",
                  "bwd_prepass",
@@ -785,6 +815,8 @@ def main():
         "cpu_baseline": None,
         "autotune": ctx.autotune_export().splitlines(),
     }
+    if not args.no_autotune:
+        result["autotune_effect"] = autotune_effect(step, ctx, args.steps, args.warmup)
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(scene, args.cpu_views)
     if not args.no_chain:
