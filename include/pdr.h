@@ -98,7 +98,9 @@ pdr_status pdr_context_set_timing(pdr_context ctx, int enable);
 pdr_status pdr_context_kernel_time(pdr_context ctx, const char* name, double* total_ms, uint64_t* launches);
 
 /* The autotuner's cache as text, one class per line:
- * "<kind> <views> <log2 triangles> <log2 pixels> <wave_work> <fwd_waves> <flat_work> <bwd_waves>".
+ * "<kind> <views> <log2 triangles> <log2 pixels> <wave_work> <fwd_waves> <flat_work> <bwd_waves>
+ *  <tiled> <bin depth> <order>" (order: 0 = triangle index order, 1 = each view's nearer end of the
+ * index range first; import also accepts lines without it, as order 0).
  * export: writes up to capacity bytes (NUL-terminated) and the needed size
  * (with the NUL) to *length; import: adds or replaces the listed classes;
  * clear: forgets every class. */

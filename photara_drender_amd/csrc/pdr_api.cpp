@@ -122,10 +122,10 @@ pdr_status pdr_autotune_export(pdr_context ctx, char* buffer, size_t capacity, s
             Lock lock(ctx);
             for (const auto& kv : ctx->tuned) {
                 char line[256];
-                std::snprintf(line, sizeof(line), "%s %.9g %u %.9g %u %u %u\n", kv.first.c_str(),
+                std::snprintf(line, sizeof(line), "%s %.9g %u %.9g %u %u %u %u\n", kv.first.c_str(),
                               static_cast<double>(kv.second.wave_work), kv.second.fwd_waves,
                               static_cast<double>(kv.second.flat_work), kv.second.bwd_waves, kv.second.tiled,
-                              kv.second.bin_tiles);
+                              kv.second.bin_tiles, kv.second.order);
                 text += line;
             }
         }
@@ -150,13 +150,17 @@ pdr_status pdr_autotune_import(pdr_context ctx, const char* text) {
             p = e ? e + 1 : p + line.size();
             if (line.find_first_not_of(" \t\r") == std::string::npos) continue;
             char kind[32];
-            unsigned views = 0, lt = 0, lp = 0, fw = 0, bw = 0, tiled = 0, depth = 0;
+            unsigned views = 0, lt = 0, lp = 0, fw = 0, bw = 0, tiled = 0, depth = 0, order = 0;
             double ww = 0.0, flat = 0.0;
-            if (std::sscanf(line.c_str(), "%31s %u %u %u %lf %u %lf %u %u %u", kind, &views, &lt, &lp, &ww, &fw,
-                            &flat, &bw, &tiled, &depth) != 10 ||
+            char tail = 0;
+            // ten fields, or eleven with the traversal order (cache lines written before it existed omit it)
+            const int got = std::sscanf(line.c_str(), "%31s %u %u %u %lf %u %lf %u %u %u %u %c", kind, &views, &lt,
+                                        &lp, &ww, &fw, &flat, &bw, &tiled, &depth, &order, &tail);
+            if ((got != 10 && got != 11) ||
                 (std::strcmp(kind, "forward") != 0 && std::strcmp(kind, "forward_out") != 0 &&
                  std::strcmp(kind, "backward") != 0) ||
-                !(ww > 0.0) || !(flat > 0.0) || fw == 0 || bw == 0 || tiled > 1 || depth == 0 || depth > 8)
+                !(ww > 0.0) || !(flat > 0.0) || fw == 0 || bw == 0 || tiled > 1 || depth == 0 || depth > 8 ||
+                order > 1)
                 raise(PDR_INVALID_ARGUMENT, "autotune cache line is malformed: " + line);
             pdr::TuneParams t;
             t.wave_work = static_cast<float>(ww);
@@ -165,6 +169,7 @@ pdr_status pdr_autotune_import(pdr_context ctx, const char* text) {
             t.bwd_waves = bw;
             t.tiled = tiled;
             t.bin_tiles = depth;
+            t.order = order;
             parsed[std::string(kind) + " " + std::to_string(views) + " " + std::to_string(lt) + " " +
                    std::to_string(lp)] = t;
         }

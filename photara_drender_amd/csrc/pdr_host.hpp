@@ -423,18 +423,21 @@ pdr::TuneParams tuned_params(pdr_context ctx, const std::string& key, const std:
 // Candidates; the first is the defaults.
 // Forward candidates.  With outputs (tiled_choice): the tiled path at bin
 // depths of 2, 4 and 8 raster tiles per triangle, and the triangle-parallel
-// path with its walk limit and wave count; the keys-only pass (the bake's
-// shadow maps) has the triangle-parallel candidates alone.
+// path with its walk limit, wave count and traversal order (index order, or
+// each view's nearer end first); the keys-only pass (the bake's shadow maps)
+// has the triangle-parallel candidates alone.
 inline std::vector<pdr::TuneParams> forward_candidates(bool tiled_choice = false) {
     std::vector<pdr::TuneParams> out;
-    for (float work : {16384.0f, 4096.0f, 65536.0f})
-        for (uint32_t waves : {4096u, 2048u, 8192u}) {
-            pdr::TuneParams p;
-            p.tiled = 0;
-            p.wave_work = work;
-            p.fwd_waves = waves;
-            out.push_back(p);
-        }
+    for (uint32_t order : {0u, 1u})
+        for (float work : {16384.0f, 4096.0f, 65536.0f})
+            for (uint32_t waves : {4096u, 2048u, 8192u}) {
+                pdr::TuneParams p;
+                p.tiled = 0;
+                p.wave_work = work;
+                p.fwd_waves = waves;
+                p.order = order;
+                out.push_back(p);
+            }
     if (tiled_choice)
         for (uint32_t depth : {4u, 2u, 8u}) {
             pdr::TuneParams p;

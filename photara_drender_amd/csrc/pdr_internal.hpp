@@ -106,6 +106,7 @@ struct TuneParams {
     uint32_t bin_tiles = 4;      // tiled forward: most raster tiles a binned triangle may touch (else slabs)
     float flat_work = 512.0f;    // backward: region estimate walked in the concatenated row list
     uint32_t bwd_waves = 4096;   // backward: target wave count
+    uint32_t order = 0;          // forward candidates: 0 = index order, 1 = per view, the nearer end first
 };
 
 hipError_t raster_candidates(const ViewTable& views, const uint32_t* idx, uint32_t T, uint32_t V, uint32_t cull,

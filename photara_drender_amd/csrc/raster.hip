@@ -344,6 +344,12 @@ __device__ __forceinline__ void walk_batch(const ViewDesc& vd, FwdRecs& R, uint3
 // by a global atomicMin per pixel.  Triangles above wave_work candidates are
 // queued in 16-row slabs for k_raster_large.  With a list, the view's
 // triangles are list[view * T + i], i < list_count[view] (else every triangle).
+// Traversal order (without a list): order 0 takes the triangles in index
+// order; order 1 starts from the end of the index range nearer the camera
+// (first vs last triangle's first vertex clip w) in each view, so that with a
+// mesh stored in spatial order (a grid) the near geometry lands first and the
+// key filter turns away more of the far geometry's atomics.  The winner does
+// not depend on the order.
 __global__ __launch_bounds__(kBlock) void k_raster_small(ViewTable vt, const uint32_t* __restrict__ idx, uint32_t T,
                                                          uint32_t V, uint32_t cull, const float* __restrict__ sxt,
                                                          const float* __restrict__ syt,
@@ -351,7 +357,7 @@ __global__ __launch_bounds__(kBlock) void k_raster_small(ViewTable vt, const uin
                                                          uint4* __restrict__ queue, uint32_t* __restrict__ counters,
                                                          uint32_t bsz, float wave_work,
                                                          const uint32_t* __restrict__ list,
-                                                         const uint32_t* __restrict__ list_count) {
+                                                         const uint32_t* __restrict__ list_count, uint32_t order) {
     __shared__ FwdRecs R;
     const uint32_t view = blockIdx.y;
     const uint32_t count = list ? list_count[view] : T;
@@ -369,9 +375,15 @@ __global__ __launch_bounds__(kBlock) void k_raster_small(ViewTable vt, const uin
     float4 c0, c1, c2;
     bool cvalid = false;
     uint32_t ctri = 0;
+    bool rev = false;
+    if (order == 1u && !vlist && count > 1u) {
+        const uint32_t ia = idx[0], ib = idx[static_cast<size_t>(count - 1u) * 3];
+        const float4* P = reinterpret_cast<const float4*>(vd.pos);
+        rev = ia < V && ib < V && P[ib].w < P[ia].w;
+    }
     auto fetch = [&](uint32_t b) {
         const uint32_t i = b + lane;
-        ctri = vlist && lane < bsz && b < count && i < count ? vlist[i] : i;
+        ctri = vlist && lane < bsz && b < count && i < count ? vlist[i] : (rev && i < count ? count - 1u - i : i);
         cvalid = lane < bsz && b < count && i < count && load_corners(vd, idx, ctri, V, c0, c1, c2);
     };
     uint32_t batch = (blockIdx.x * (kBlock / 64) + wave) * bsz;
@@ -1648,7 +1660,8 @@ hipError_t raster_candidates(const ViewTable& vt, const uint32_t* idx, uint32_t 
     if (e != hipSuccess) return e;
     const uint32_t bsz = tri_batch(T, vt.n, tp.fwd_waves);
     k_raster_small<<<dim3(tri_grid(T, bsz, vt.n), vt.n), kBlock, 0, st>>>(vt, idx, T, V, cull, s.sx, s.sy, s.keys, s.queue,
-                                                                    s.counters, bsz, tp.wave_work, nullptr, nullptr);
+                                                                    s.counters, bsz, tp.wave_work, nullptr, nullptr,
+                                                                    tp.order);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     k_raster_large<<<kLargeBlocks, kBlock, 0, st>>>(vt, idx, T, V, cull, s.sx, s.sy, s.keys, s.queue, s.counters);
     return hipGetLastError();
@@ -1679,7 +1692,7 @@ hipError_t raster_tiled(const ViewTable& vt, const uint32_t* idx, uint32_t T, ui
     if ((e = hipGetLastError()) != hipSuccess) return e;
     // the triangles too large to bin (global keys; the largest of them queued as slabs)
     k_raster_small<<<dim3(64, vt.n), kBlock, 0, st>>>(vt, idx, T, V, cull, s.sx, s.sy, s.keys, s.queue, s.counters,
-                                                     64u, tp.wave_work, s.medium_list, s.medium_count);
+                                                     64u, tp.wave_work, s.medium_list, s.medium_count, 0u);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     size_t bytes = s.bin_temp_bytes;
     if ((e = hipcub::DeviceScan::ExclusiveSum(s.bin_temp, bytes, s.bin_counts, s.bin_offsets,

@@ -2,8 +2,9 @@
 
 * The forward with outputs chooses between the tiled path (bins of 32x32
   raster tiles, LDS keys) at bin depths of 2, 4 and 8 tiles per triangle and
-  the triangle-parallel path (global keys) with its walk limit and wave count;
-  the backward chooses its walk limit and wave count.
+  the triangle-parallel path (global keys) with its walk limit, wave count and
+  traversal order (index order, or each view's nearer end first); the backward
+  chooses its walk limit and wave count.
 * Raster forward and backward results are bit-identical under the defaults,
   under the autotuned choice, and under every candidate imported into the cache
   (the forward's winner is order-free; both backward paths sum each triangle in
@@ -20,9 +21,9 @@ from photara_drender_amd import scenes
 
 pytestmark = pytest.mark.gpu
 
-FORWARD = [(1, d, 16384.0, 4096) for d in (2, 4, 8)] + \
-    [(0, 4, w, n) for w in (4096.0, 16384.0, 65536.0) for n in (2048, 4096, 8192)]
-BACKWARD = [(w, n) for w in (256.0, 512.0, 1024.0, 2048.0) for n in (2048, 4096, 8192)]
+FORWARD = [(1, d, 16384.0, 4096, 0) for d in (2, 4, 8)] + \
+    [(0, 4, w, n, o) for o in (0, 1) for w in (4096.0, 16384.0, 65536.0) for n in (2048, 4096, 8192)]
+BACKWARD = [(w, n) for w in (256.0, 512.0, 1024.0, 2048.0) for n in (2048, 4096, 8192)] * 2
 
 
 def run_views(ctx, sc):
@@ -73,15 +74,15 @@ def test_autotuned_and_every_candidate_give_identical_results():
     kinds = sorted(line.split()[0] for line in lines)
     assert kinds == ["backward", "forward_out"], lines
     keys = {line.split()[0]: " ".join(line.split()[:4]) for line in lines}
-    for (tiled, depth, ww, fw), (flat, bw) in zip(FORWARD, BACKWARD):
+    for (tiled, depth, ww, fw, order), (flat, bw) in zip(FORWARD, BACKWARD):
         ctx = _native.Context(0, flags=_native.PDR_FLAG_AUTOTUNE)
-        ctx.autotune_import(f"{keys['forward_out']} {ww} {fw} 512 4096 {tiled} {depth}\n"
+        ctx.autotune_import(f"{keys['forward_out']} {ww} {fw} 512 4096 {tiled} {depth} {order}\n"
                             f"{keys['backward']} 16384 4096 {flat} {bw} 1 4\n")
         got = run_views(ctx, sc)
         for a, b in zip(ref, got):
             np.testing.assert_array_equal(a, b)
-        assert ctx.autotune_export() == f"{keys['backward']} 16384 4096 {flat:g} {bw} 1 4\n" \
-                                        f"{keys['forward_out']} {ww:g} {fw} 512 4096 {tiled} {depth}\n"  # not re-tuned
+        assert ctx.autotune_export() == f"{keys['backward']} 16384 4096 {flat:g} {bw} 1 4 0\n" \
+                                        f"{keys['forward_out']} {ww:g} {fw} 512 4096 {tiled} {depth} {order}\n"  # not re-tuned
 
 
 def test_autotune_cache_rejects_malformed_lines():
@@ -94,7 +95,13 @@ def test_autotune_cache_rejects_malformed_lines():
         ctx.autotune_import("sideways 1 2 3 4096 4096 512 4096 1 4\n")
     with pytest.raises(ValueError, match="malformed"):
         ctx.autotune_import("forward_out 1 2 3 4096 4096 512 4096 2 4\n")  # tiled is 0 or 1
-    ctx.autotune_import("forward 1 12 20 4096 2048 512 4096 0 4\n\n")
-    assert ctx.autotune_export() == "forward 1 12 20 4096 2048 512 4096 0 4\n"
+    with pytest.raises(ValueError, match="malformed"):
+        ctx.autotune_import("forward_out 1 2 3 4096 4096 512 4096 0 4 2\n")  # order is 0 or 1
+    with pytest.raises(ValueError, match="malformed"):
+        ctx.autotune_import("forward_out 1 2 3 4096 4096 512 4096 0 4 1 7\n")  # one field too many
+    ctx.autotune_import("forward 1 12 20 4096 2048 512 4096 0 4\n\n")  # without the order: index order
+    assert ctx.autotune_export() == "forward 1 12 20 4096 2048 512 4096 0 4 0\n"
+    ctx.autotune_import("forward 1 12 20 4096 2048 512 4096 0 4 1\n")
+    assert ctx.autotune_export() == "forward 1 12 20 4096 2048 512 4096 0 4 1\n"
     ctx.autotune_clear()
     assert ctx.autotune_export() == ""
