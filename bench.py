@@ -75,6 +75,16 @@ def _short(name):
     return None
 
 
+PROBE_LAST = 2  # the traffic probe's steps after its autotuning step
+
+
+def _last(per):
+    """Each kernel's values of its last PROBE_LAST dispatches (the probe's tuned steps; kernels that only the
+    autotuning step launches keep their last tuning launches)."""
+    return {k: {c: [v for _, v in sorted(vs, key=lambda dv: int(dv[0]))][-PROBE_LAST:] for c, vs in cs.items()}
+            for k, cs in per.items()}
+
+
 def traffic_summary(fetch_dir, write_dir, out_path, commands):
     """HBM bytes per launch from FETCH_SIZE and WRITE_SIZE (KiB per dispatch), each scaled by the
     factor that makes the probe's 1 GiB device copy read and write exactly 1 GiB."""
@@ -83,14 +93,17 @@ def traffic_summary(fetch_dir, write_dir, out_path, commands):
         for (disp, name), vals in _pmc_rows(d).items():
             k = _short(name)
             if k and counter in vals:
-                per.setdefault(k, {}).setdefault(counter, []).append(vals[counter])
+                per.setdefault(k, {}).setdefault(counter, []).append((disp, vals[counter]))
+    per = _last(per)
     mean = lambda k, c: sum(per[k][c]) / len(per[k][c])
     copy_fetch = max(per["copy"]["FETCH_SIZE"]) * 1024.0
     copy_write = max(per["copy"]["WRITE_SIZE"]) * 1024.0
     ff, fw = COPY_BYTES / copy_fetch, COPY_BYTES / copy_write
     lines = ["HBM traffic per launch of the 16-view raster/interp fwd+bwd workload (200,000 triangles, 15,163,392 px,",
-             "A = 8), from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (KiB per dispatch, mean over dispatches),",
-             "each scaled by the factor that makes a 1 GiB device-to-device copy read and write 1 GiB.", "",
+             "A = 8), from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (KiB per dispatch, mean over each kernel's",
+             f"last {PROBE_LAST} dispatches: the probe's steps after its autotuning step, with the tuned launch",
+             "parameters; k_tile_raster is launched by the autotuning step only), each scaled by the factor that",
+             "makes a 1 GiB device-to-device copy read and write 1 GiB.", "",
              "commands:"] + ["  " + c for c in commands] + [
              "", f"calibration: copy FETCH_SIZE {copy_fetch / 1024:.0f} KiB -> factor {ff:.4f}; "
                  f"WRITE_SIZE {copy_write / 1024:.0f} KiB -> factor {fw:.4f}", "",
@@ -117,12 +130,14 @@ def valu_summary(insts_dir, busy_dir, out_path, commands):
             k = _short(name)
             if k and k != "copy":
                 for c, v in vals.items():
-                    per.setdefault(k, {}).setdefault(c, []).append(v)
+                    per.setdefault(k, {}).setdefault(c, []).append((disp, v))
+    per = _last(per)
     mean = lambda k, c: sum(per[k][c]) / len(per[k][c]) if per[k].get(c) else float("nan")
     lines = ["VALU work per launch of the 16-view raster/interp fwd+bwd workload (200,000 triangles, 15,163,392 px,",
-             "A = 8), from rocprofv3 --pmc SQ_INSTS_VALU SQ_WAVES, then VALUBusy VALUUtilization (mean over",
-             "dispatches; the probe's first step is the autotuning pass, whose candidate launches do the same",
-             "work).  VALU issue peak: 256 CUs x 4 SIMDs x 2.4 GHz / 4 cycles per wave64 instruction",
+             "A = 8), from rocprofv3 --pmc SQ_INSTS_VALU SQ_WAVES, then VALUBusy VALUUtilization (mean over each",
+             f"kernel's last {PROBE_LAST} dispatches: the probe's steps after its autotuning step, with the tuned",
+             "launch parameters; k_tile_raster is launched by the autotuning step only).  VALU issue peak:",
+             "256 CUs x 4 SIMDs x 2.4 GHz / 4 cycles per wave64 instruction",
              f"= {VALU_PEAK_GIPS:.1f} G wave-instructions/s.", "", "commands:"] + ["  " + c for c in commands] + [
              "", f"{'kernel':20s} {'dispatches':>10s} {'VALU insts':>14s} {'waves':>10s} {'VALUBusy %':>11s} "
                  f"{'VALUUtil %':>11s}"]
