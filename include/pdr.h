@@ -59,7 +59,8 @@ enum {
      * Default: off. */
     PDR_FLAG_DETERMINISTIC = 1u << 1,
     /* Autotune the raster's launch parameters (wave / queue thresholds,
-     * triangles per wave): the first call of each workload class (view count,
+     * triangles per wave, triangle traversal order, tiled or triangle-parallel
+     * forward): the first call of each workload class (view count,
      * log2 of the triangle and pixel counts) times every candidate with HIP
      * events and caches the fastest in the context; later calls use it.
      * Results do not depend on the choice.  Not applied while the stream is
@@ -126,7 +127,7 @@ pdr_status pdr_rasterize_forward(pdr_context ctx, const float* pos, uint32_t V, 
  * reference: for callers that cannot allocate (code under HIP graph capture,
  * tight serving loops, harnesses that forbid hipMalloc).  Same arguments and
  * results as pdr_rasterize_forward, device memory only.  Its scratch (8 B per
- * pixel + 16 B per triangle + 4 B per row and column) comes from a fixed
+ * pixel + 16 B per triangle, each part 256-byte aligned) comes from a fixed
  * 32 MiB array per GPU set up by pdr_context_create; larger calls fail with
  * PDR_OVERFLOW.  Indices are not validated by a readback (out-of-range
  * triangles are skipped, as by the shader) and the launch parameters are the

@@ -93,7 +93,9 @@ struct pdr_context_s {
         size_t bytes = 0;
     };
     std::unordered_map<std::string, Slot> scratch;
-    bool keys_clean = false;  // raster key buffer is all-ones
+    bool keys_clean = false;      // raster key buffer is all-ones
+    bool counters_clean = false;  // large-work queue counters are zero (the kernel after their consumer clears them)
+    bool bwd_clean = false;       // backward won marks and view flags are zero (the vertex reduce clears them)
     std::map<std::string, pdr::TuneParams> tuned;  // autotuner: chosen launch parameters per workload class
 
     // The last bake's shadow-map keys (each view's raster candidate pass: the
@@ -289,7 +291,7 @@ inline pdr::ViewDesc make_view(const float* pos, uint32_t W, uint32_t H, const f
 }
 
 inline void finalize_table(pdr::ViewTable& t) {
-    uint32_t tiles = 0, pixels = 0, cols = 0, rows = 0, rtiles = 0;
+    uint32_t tiles = 0, pixels = 0, rtiles = 0;
     for (uint32_t k = 0; k < t.n; ++k) {
         t.v[k].rtiles_x = (t.v[k].width + pdr::kRasterTile - 1) / pdr::kRasterTile;
         t.v[k].rtiles_y = (t.v[k].height + pdr::kRasterTile - 1) / pdr::kRasterTile;
@@ -297,10 +299,6 @@ inline void finalize_table(pdr::ViewTable& t) {
         rtiles += t.v[k].rtiles_x * t.v[k].rtiles_y;
         t.v[k].tile_base = tiles;
         t.v[k].pixel_base = pixels;
-        t.v[k].col_base = cols;
-        t.v[k].row_base = rows;
-        cols += t.v[k].width;
-        rows += t.v[k].height;
         const uint64_t nt = static_cast<uint64_t>(tiles) + static_cast<uint64_t>(t.v[k].tiles_x) * t.v[k].tiles_y;
         const uint64_t np = static_cast<uint64_t>(pixels) + static_cast<uint64_t>(t.v[k].width) * t.v[k].height;
         if (nt > 0x7FFFFFFFu || np > 0xFFFFFFFFu) raise(PDR_OVERFLOW, "Rasterization output size overflow");
@@ -332,29 +330,23 @@ inline pdr::PixelTable pixel_table_one(const float* raster, const float* in0, co
     return t;
 }
 
-inline uint32_t table_cols(const pdr::ViewTable& vt) {
-    return vt.v[vt.n - 1].col_base + vt.v[vt.n - 1].width;
-}
-inline uint32_t table_rows(const pdr::ViewTable& vt) {
-    return vt.v[vt.n - 1].row_base + vt.v[vt.n - 1].height;
-}
-
-// Scratch of the triangle-parallel raster for one table of views: sample
-// tables filled, key buffer guaranteed all-ones (the resolve pass restores it).
+// Scratch of the triangle-parallel raster for one table of views: key buffer
+// guaranteed all-ones (the resolve pass restores it).
 inline pdr::RasterScratch raster_scratch(pdr_context ctx, const pdr::ViewTable& vt, uint32_t T, hipStream_t st,
                                          bool with_keys) {
     pdr::RasterScratch s;
-    s.sx = ctx->get_t<float>("sample_x", table_cols(vt));
-    s.sy = ctx->get_t<float>("sample_y", table_rows(vt));
     s.queue = ctx->get_t<uint4>("large_queue", static_cast<size_t>(vt.n) * T);
-    s.counters = ctx->get_t<uint32_t>("large_counters", 4);
+    bool fresh = false;
+    s.counters = ctx->get_t<uint32_t>("large_counters", 4, &fresh);
+    // zero unless the last raster call on this context ran to the end (its queue consumers cleared them)
+    if (fresh || !ctx->counters_clean) hip_check(pdr::fill_u32(s.counters, 0u, 4, st), "fill");
+    ctx->counters_clean = false;  // set again by the caller once its kernels are enqueued
     if (with_keys) {
         const size_t before = ctx->scratch["raster_keys"].bytes;
         s.keys = ctx->get_t<unsigned long long>("raster_keys", vt.total_pixels);
         const size_t after = ctx->scratch["raster_keys"].bytes;
         if (after != before || !ctx->keys_clean) hip_check(pdr::fill_keys(s.keys, after / 8, st), "fill_keys");
     }
-    hip_check(pdr::sample_tables(vt, s.sx, s.sy, st), "sample_tables");
     return s;
 }
 
@@ -513,6 +505,7 @@ inline void run_raster_forward(pdr_context ctx, const pdr::ViewTable& vt, const 
                                             [&](const pdr::TuneParams& c) { run(c, false); });
     run(tp, true);
     ctx->keys_clean = true;
+    ctx->counters_clean = true;
 }
 
 // The forward's resolve pass over keys taken from `src_keys` (view j's keys at
@@ -534,6 +527,7 @@ inline void run_raster_resolve_keys(pdr_context ctx, const pdr::ViewTable& vt, c
         hip_check(pdr::raster_resolve(vt, idx, want_deriv, s, st), "raster_resolve");  // resets the keys
     });
     ctx->keys_clean = true;
+    ctx->counters_clean = true;
 }
 
 // The forward's candidate pass alone: each pixel's winning (depth key, id)
@@ -548,12 +542,14 @@ inline const unsigned long long* run_raster_keys(pdr_context ctx, const pdr::Vie
     const pdr::TuneParams tp =
         tuned_params(ctx, tune_key("forward", vt, T), forward_candidates(), st, [&](const pdr::TuneParams& c) {
             hip_check(pdr::raster_candidates(vt, idx, T, V, cull, s, c, st), "raster_candidates");
-            // the views may have no raster output here: clear the keys instead of resolving them
+            // the views may have no raster output here: clear the keys (and the queue counters, which a
+            // resolve pass would clear) for the next candidate
             hip_check(pdr::fill_keys(s.keys, vt.total_pixels, st), "fill_keys");
+            hip_check(pdr::fill_u32(s.counters, 0u, 4, st), "fill");
         });
     ctx->timed("raster_candidates", st,
                [&] { hip_check(pdr::raster_candidates(vt, idx, T, V, cull, s, tp, st), "raster_candidates"); });
-    return s.keys;
+    return s.keys;  // no resolve pass cleared the queue counters: the next raster call fills them
 }
 
 // Per-triangle corner sums + ordered vertex reduce for one table of views.
@@ -576,9 +572,17 @@ inline void run_raster_backward(pdr_context ctx, const pdr::ViewTable& vt, const
     float4* corner = ctx->get_t<float4>("corner_grads", static_cast<size_t>(vt.n) * T * 3);
     pdr::RasterScratch sb = s;
     sb.cid = ctx->get_t<uint32_t>("backward_ids", vt.total_pixels);
-    sb.view_flags = ctx->get_t<uint32_t>("backward_view_flags", pdr::kMaxViews);
+    bool fresh_flags = false, fresh_won = false;
+    sb.view_flags = ctx->get_t<uint32_t>("backward_view_flags", pdr::kMaxViews, &fresh_flags);
     sb.won = reinterpret_cast<uint8_t*>(
-        ctx->get_t<uint32_t>("backward_won", (static_cast<size_t>(vt.n) * T + 3) / 4));
+        ctx->get_t<uint32_t>("backward_won", (static_cast<size_t>(vt.n) * T + 3) / 4, &fresh_won));
+    // zero unless the last backward on this context ran to the end (its vertex reduce cleared them)
+    if (fresh_flags || !ctx->bwd_clean)
+        hip_check(pdr::fill_u32(sb.view_flags, 0u, ctx->scratch["backward_view_flags"].bytes / 4, st), "fill");
+    if (fresh_won || !ctx->bwd_clean)
+        hip_check(pdr::fill_u32(reinterpret_cast<uint32_t*>(sb.won), 0u, ctx->scratch["backward_won"].bytes / 4, st),
+                  "fill");
+    ctx->bwd_clean = false;
     sb.slot_of = csr.slot_of;
     // boxes over 65,536 pixels: the reference's row-major order under PDR_FLAG_DETERMINISTIC
     const bool ordered = (ctx->flags & PDR_FLAG_DETERMINISTIC) != 0;
@@ -588,11 +592,15 @@ inline void run_raster_backward(pdr_context ctx, const pdr::ViewTable& vt, const
     const pdr::TuneParams tp =
         tuned_params(ctx, tune_key("backward", vt, T), backward_candidates(), st, [&](const pdr::TuneParams& c) {
             hip_check(pdr::corner_gradients(vt, idx, T, V, sb, corner, c, ordered, st), "corner_gradients");
+            // the vertex reduce clears the queue counters after the tuned run; clear them between candidates
+            hip_check(pdr::fill_u32(sb.counters, 0u, 4, st), "fill");
         });
     ctx->timed("corner_grads", st,
                [&] { hip_check(pdr::corner_gradients(vt, idx, T, V, sb, corner, tp, ordered, st), "corner_gradients"); });
     ctx->timed("vertex_reduce", st,
-               [&] { hip_check(pdr::vertex_reduce(vt, T, V, csr, corner, st), "vertex_reduce"); });
+               [&] { hip_check(pdr::vertex_reduce(vt, T, V, csr, corner, sb.won, sb.view_flags, sb.counters, st), "vertex_reduce"); });
+    ctx->bwd_clean = true;
+    ctx->counters_clean = true;
 }
 
 inline void validate_mesh_counts(const float* pos, uint32_t V, const uint32_t* idx, uint32_t T) {

@@ -21,8 +21,6 @@ struct ViewDesc {
     float vx, vy, vw, vh;     // resolved viewport
     uint32_t tile_base;       // first global tile of this view
     uint32_t pixel_base;      // first global pixel of this view
-    uint32_t col_base;        // first entry of this view in the column sample table
-    uint32_t row_base;        // first entry of this view in the row sample table
     uint32_t rtiles_x, rtiles_y;  // kRasterTile x kRasterTile tiles of the tiled forward
     uint32_t rtile_base;          // first global raster tile of this view
 };
@@ -70,13 +68,11 @@ struct RasterScratch {
     unsigned long long* tri_spans = nullptr;  // [views * T] binned triangles' tile ranges (0: not binned)
     void* bin_temp = nullptr;            // scan temporary
     size_t bin_temp_bytes = 0;
-    float* sx = nullptr;                 // [sum of widths] pixel-centre NDC x per column
-    float* sy = nullptr;                 // [sum of heights] pixel-centre NDC y per row
     uint4* queue = nullptr;              // [views * T] (view, triangle, slabs, 0) of large triangles
-    uint32_t* counters = nullptr;        // [4] queue length, next queue entry
+    uint32_t* counters = nullptr;        // [4] queue length, next queue entry (0 between calls)
     uint32_t* cid = nullptr;             // [total_pixels] backward: compared id per pixel
-    uint32_t* view_flags = nullptr;      // [kMaxViews] backward: view walked over whole boxes
-    uint8_t* won = nullptr;              // [views * T, padded to words] backward: 1 where a pixel's compared id names the triangle
+    uint32_t* view_flags = nullptr;      // [kMaxViews] backward: view walked over whole boxes (0 between calls)
+    uint8_t* won = nullptr;              // [views * T, padded to words] backward: 1 where a pixel's compared id names the triangle (0 between calls)
     const uint32_t* slot_of = nullptr;   // [3T] backward: where each corner's gradient is written (VertexCsr)
 };
 
@@ -94,7 +90,6 @@ struct VertexCsr {
 
 // ---- raster.hip
 hipError_t fill_keys(unsigned long long* keys, size_t n, hipStream_t st);
-hipError_t sample_tables(const ViewTable& views, float* sx, float* sy, hipStream_t st);
 // k_raster_small + k_raster_large: (depth, id) keys of every covered candidate pixel
 // Launch parameters of the raster kernels, chosen per workload class by the
 // autotuner (every choice gives the same results: the forward's winner is
@@ -119,7 +114,7 @@ size_t bin_scan_temp_bytes(uint32_t n);
 size_t tri_record_bytes();
 hipError_t raster_tiled(const ViewTable& views, const uint32_t* idx, uint32_t T, uint32_t V, uint32_t cull,
                         int want_deriv, const RasterScratch& s, const TuneParams& tp, hipStream_t st);
-// k_raster_resolve: raster / derivatives from the keys; resets the keys
+// k_raster_resolve: raster / derivatives from the keys; resets the keys and the queue counters
 hipError_t raster_resolve(const ViewTable& views, const uint32_t* idx, int want_deriv, const RasterScratch& s,
                           hipStream_t st);
 size_t csr_total_words(uint32_t V);
@@ -135,8 +130,9 @@ hipError_t backward_prepass(const ViewTable& views, const uint32_t* idx, uint32_
 hipError_t corner_gradients(const ViewTable& views, const uint32_t* idx, uint32_t T, uint32_t V,
                             const RasterScratch& s, float4* corner, const TuneParams& tp, bool ordered,
                             hipStream_t st);
+// (also clears won, view_flags and the large-box queue counters for the next call)
 hipError_t vertex_reduce(const ViewTable& views, uint32_t T, uint32_t V, const VertexCsr& csr, const float4* corner,
-                         hipStream_t st);
+                         uint8_t* won, uint32_t* view_flags, uint32_t* counters, hipStream_t st);
 
 // ---- interp_texture.hip
 hipError_t interp_forward(const PixelTable& views, const float* attr, uint32_t V, uint32_t A, const uint32_t* idx,

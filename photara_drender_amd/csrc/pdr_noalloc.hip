@@ -1,10 +1,10 @@
 // pdr_rasterize_forward_noalloc: the single-view rasterize forward for callers
 // that cannot allocate device memory (code under HIP graph capture, tight
 // serving loops, harnesses that forbid hipMalloc).  It runs the same kernels as
-// pdr_rasterize_forward (raster.hip: sample tables, k_raster_small /
-// k_raster_large, k_raster_resolve) on device arrays the caller owns, with
-// their scratch - the per-pixel (depth, id) keys, the pixel-centre sample
-// tables, the large-triangle queue and its counters - carved out of one
+// pdr_rasterize_forward (raster.hip: k_raster_small / k_raster_large,
+// k_raster_resolve) on device arrays the caller owns, with their scratch -
+// the per-pixel (depth, id) keys, the large-triangle queue and its counters -
+// carved out of one
 // file-scope device array of kScratchBytes per GPU instead of the context's
 // grow-only pool.  It calls no hipMalloc / hipFree and no allocating library
 // routine.
@@ -38,20 +38,16 @@ std::array<unsigned char*, kMaxDevices> g_scratch{};  // device address of `scra
 
 inline size_t up256(size_t b) { return (b + 255) & ~static_cast<size_t>(255); }
 
-// Scratch layout of one call: keys [P] u64, sample x [W], sample y [H], large
-// queue [T] uint4, counters [4] u32, each 256-byte aligned.
+// Scratch layout of one call: keys [P] u64, large queue [T] uint4, counters
+// [4] u32, each 256-byte aligned.
 struct Layout {
-    size_t keys = 0, sx = 0, sy = 0, queue = 0, counters = 0, total = 0;
+    size_t keys = 0, queue = 0, counters = 0, total = 0;
 };
 inline Layout layout_of(uint32_t W, uint32_t H, uint32_t T) {
     Layout l;
     size_t off = 0;
     l.keys = off;
     off += up256(static_cast<size_t>(W) * H * sizeof(unsigned long long));
-    l.sx = off;
-    off += up256(static_cast<size_t>(W) * sizeof(float));
-    l.sy = off;
-    off += up256(static_cast<size_t>(H) * sizeof(float));
     l.queue = off;
     off += up256(static_cast<size_t>(T) * sizeof(uint4));
     l.counters = off;
@@ -95,7 +91,7 @@ pdr_status pdr_rasterize_forward_noalloc(pdr_context ctx, const float* pos, uint
         const Layout l = layout_of(W, H, T);
         if (l.total > kScratchBytes)
             raise(PDR_OVERFLOW, "the no-allocation rasterize forward needs " + std::to_string(l.total) +
-                                    " bytes of scratch (8 per pixel + 16 per triangle + tables); its limit is " +
+                                    " bytes of scratch (8 per pixel + 16 per triangle); its limit is " +
                                     std::to_string(kScratchBytes));
         float vp[4];
         resolve_viewport(W, H, viewport4, vp);
@@ -116,8 +112,6 @@ pdr_status pdr_rasterize_forward_noalloc(pdr_context ctx, const float* pos, uint
         finalize_table(vt);
         pdr::RasterScratch s;
         s.keys = reinterpret_cast<unsigned long long*>(base + l.keys);
-        s.sx = reinterpret_cast<float*>(base + l.sx);
-        s.sy = reinterpret_cast<float*>(base + l.sy);
         s.queue = reinterpret_cast<uint4*>(base + l.queue);
         s.counters = reinterpret_cast<uint32_t*>(base + l.counters);
         // Indices are not checked with a readback here: the kernels skip a
@@ -125,7 +119,7 @@ pdr_status pdr_rasterize_forward_noalloc(pdr_context ctx, const float* pos, uint
         // (rasterize_forward.hlsl:75-80).  Launch parameters are the defaults
         // (every choice gives the same raster).
         hip_check(pdr::fill_keys(s.keys, static_cast<size_t>(W) * H, st), "fill_keys");
-        hip_check(pdr::sample_tables(vt, s.sx, s.sy, st), "sample_tables");
+        hip_check(pdr::fill_u32(s.counters, 0u, 4, st), "fill");  // the queue's consumer clears it again
         const pdr::TuneParams tp{};
         hip_check(pdr::raster_candidates(vt, idx, T, V, cull, s, tp, st), "raster_candidates");
         hip_check(pdr::raster_resolve(vt, idx, want_deriv, s, st), "raster_resolve");

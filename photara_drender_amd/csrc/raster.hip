@@ -2,8 +2,6 @@
 //
 // Forward (shaders/rasterize_forward.hlsl:45-175; candidates from the tile bins
 // of src/rasterizer.cpp:61-138):
-//   k_sample_tables   pixel-centre NDC of every column / row of every view
-//                     (rasterize_forward.hlsl:55-57, computed once per call);
 //   k_raster_small    one lane per (view, triangle) sets the triangle up
 //                     (raster_geom.hpp): its candidate pixels are those of its
 //                     16x16 bins (the reference's pixel bounds, tile-expanded)
@@ -95,22 +93,14 @@ __device__ __forceinline__ uint32_t find_view(const ViewTable& vt, uint32_t g) {
     return k;
 }
 
-// Pixel-centre NDC of column x / row y (rasterize_forward.hlsl:55-57); the
-// kernels evaluate it inline (a correctly rounded division) rather than load it.
+// Pixel-centre NDC of column x / row y (rasterize_forward.hlsl:55-57); every
+// kernel evaluates it inline (a correctly rounded division), so no per-call
+// table launch is needed.
 __device__ __forceinline__ float sample_x(const ViewDesc& vd, int32_t x) {
     return (static_cast<float>(x) - vd.vx + 0.5f) * 2.0f / vd.vw - 1.0f;
 }
 __device__ __forceinline__ float sample_y(const ViewDesc& vd, int32_t y) {
     return (static_cast<float>(y) - vd.vy + 0.5f) * 2.0f / vd.vh - 1.0f;
-}
-
-// rasterize_forward.hlsl:55-57 for every column and row of every view.
-__global__ void k_sample_tables(ViewTable vt, float* __restrict__ sx, float* __restrict__ sy) {
-    const ViewDesc& vd = vt.v[blockIdx.y];
-    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < vd.width || i < vd.height; i += gridDim.x * kBlock) {
-        if (i < vd.width) sx[vd.col_base + i] = (static_cast<float>(i) - vd.vx + 0.5f) * 2.0f / vd.vw - 1.0f;
-        if (i < vd.height) sy[vd.row_base + i] = (static_cast<float>(i) - vd.vy + 0.5f) * 2.0f / vd.vh - 1.0f;
-    }
 }
 
 __device__ __forceinline__ uint32_t order_key(float depth) {
@@ -139,6 +129,18 @@ __device__ __forceinline__ unsigned long long test_pixel(const Tri& r, uint32_t 
 // Keys only decrease, so a stale read can only make the filter let an atomic through.
 __device__ __forceinline__ void key_min(unsigned long long* p, unsigned long long key) {
     if (key < __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) atomicMin(p, key);
+}
+
+// The large-work queue's counters (counters[0] queue length, [1] next entry)
+// are zero between calls: the producers count up from zero, and the kernel
+// that follows the queue's consumer in the same call (the resolve, the tile
+// pass or the vertex reduce) clears them from one lane, so no fill launch is
+// needed before the next call.
+__device__ __forceinline__ void clear_queue_counters(uint32_t* counters) {
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
+        counters[0] = 0u;
+        counters[1] = 0u;
+    }
 }
 
 // Intra-wave hand-off of LDS data written by other lanes of the same wave.
@@ -351,9 +353,7 @@ __device__ __forceinline__ void walk_batch(const ViewDesc& vd, FwdRecs& R, uint3
 // key filter turns away more of the far geometry's atomics.  The winner does
 // not depend on the order.
 __global__ __launch_bounds__(kBlock) void k_raster_small(ViewTable vt, const uint32_t* __restrict__ idx, uint32_t T,
-                                                         uint32_t V, uint32_t cull, const float* __restrict__ sxt,
-                                                         const float* __restrict__ syt,
-                                                         unsigned long long* __restrict__ keys,
+                                                         uint32_t V, uint32_t cull, unsigned long long* __restrict__ keys,
                                                          uint4* __restrict__ queue, uint32_t* __restrict__ counters,
                                                          uint32_t bsz, float wave_work,
                                                          const uint32_t* __restrict__ list,
@@ -416,7 +416,7 @@ __global__ __launch_bounds__(kBlock) void k_raster_small(ViewTable vt, const uin
 // Rows [ya, yb] of a wave-uniform triangle with row spans, 64 rows at a time.
 __device__ __forceinline__ void wave_forward(const ScreenTri& r, const LineSpans& sp, uint32_t t, int32_t ya,
                                              int32_t yb, const ViewDesc& vd, uint32_t lane, int32_t* slot,
-                                             const float* __restrict__ sxt, const float* __restrict__ syt,
+                                             
                                              unsigned long long* __restrict__ kbase) {
     const float alpha = span_alpha(vd, 0), beta = span_beta(vd, 0);
     for (int32_t y0 = ya; y0 <= yb; y0 += 64) {
@@ -447,9 +447,7 @@ __device__ __forceinline__ void wave_forward(const ScreenTri& r, const LineSpans
 // Queued triangles, split into 16-row slabs dealt round-robin to every wave of
 // the grid (slab k of the concatenated slab list goes to wave k mod waves).
 __global__ __launch_bounds__(kBlock) void k_raster_large(ViewTable vt, const uint32_t* __restrict__ idx, uint32_t T,
-                                                         uint32_t V, uint32_t cull, const float* __restrict__ sxt,
-                                                         const float* __restrict__ syt,
-                                                         unsigned long long* __restrict__ keys,
+                                                         uint32_t V, uint32_t cull, unsigned long long* __restrict__ keys,
                                                          const uint4* __restrict__ queue,
                                                          const uint32_t* __restrict__ counters) {
     __shared__ uint32_t nslab[kBlock];
@@ -487,7 +485,7 @@ __global__ __launch_bounds__(kBlock) void k_raster_large(ViewTable vt, const uin
                 const int32_t ya = r.y0 + static_cast<int32_t>(s * kSlabRows);
                 const int32_t yb = ya + static_cast<int32_t>(kSlabRows) - 1 < r.y1 ? ya + static_cast<int32_t>(kSlabRows) - 1
                                                                                   : r.y1;
-                wave_forward(r, sp, q.y, ya, yb, vd, lane, slots + (threadIdx.x & ~63u), sxt, syt, keys + vd.pixel_base);
+                wave_forward(r, sp, q.y, ya, yb, vd, lane, slots + (threadIdx.x & ~63u), keys + vd.pixel_base);
             }
         }
         cum += batch_total;
@@ -539,9 +537,9 @@ __device__ __forceinline__ void resolve_key(const ViewDesc& vd, const uint32_t* 
 
 // Winner's outputs of every pixel from the global keys; resets the keys.
 __global__ __launch_bounds__(kBlock) void k_raster_resolve(ViewTable vt, const uint32_t* __restrict__ idx,
-                                                           int want_deriv, const float* __restrict__ sxt,
-                                                           const float* __restrict__ syt,
-                                                           unsigned long long* __restrict__ keys) {
+                                                           int want_deriv, unsigned long long* __restrict__ keys,
+                                                           uint32_t* __restrict__ counters) {
+    clear_queue_counters(counters);  // k_raster_large, before this pass, was the queue's consumer
     const uint32_t gp = blockIdx.x * kBlock + threadIdx.x;
     if (gp >= vt.total_pixels) return;
     bool split;
@@ -550,7 +548,8 @@ __global__ __launch_bounds__(kBlock) void k_raster_resolve(ViewTable vt, const u
         const uint32_t p = gp - vd.pixel_base;
         const unsigned long long key = keys[gp];
         keys[gp] = ~0ull;
-        resolve_key(vd, idx, want_deriv, key, p, sxt[vd.col_base + p % vd.width], syt[vd.row_base + p / vd.width]);
+        resolve_key(vd, idx, want_deriv, key, p, sample_x(vd, static_cast<int32_t>(p % vd.width)),
+                    sample_y(vd, static_cast<int32_t>(p / vd.width)));
     };
     if (!split)
         body(vt.v[kv]);
@@ -764,9 +763,11 @@ __global__ __launch_bounds__(kBlock) void k_tile_raster(ViewTable vt, const uint
                                                         const uint32_t* __restrict__ list,
                                                         const uint8_t* __restrict__ seeded,
                                                         const TriRecord* __restrict__ recs,
-                                                        unsigned long long* __restrict__ keys) {
+                                                        unsigned long long* __restrict__ keys,
+                                                        uint32_t* __restrict__ counters) {
     __shared__ unsigned long long lk[kRTPix];
     __shared__ FwdRecs R;
+    clear_queue_counters(counters);  // k_raster_large, before this pass, was the queue's consumer
     const uint32_t tile = blockIdx.x;
     uint32_t view = 0;
     while (view + 1 < vt.n && tile >= vt.v[view + 1].rtile_base) ++view;
@@ -840,8 +841,7 @@ __global__ __launch_bounds__(kBlock) void k_tile_raster(ViewTable vt, const uint
 // that passes t's float edge test is inside t's region, which skips the
 // region computation for every pixel of a forward-produced raster.
 __global__ __launch_bounds__(kBlock) void k_bwd_prepass(ViewTable vt, const uint32_t* __restrict__ idx, uint32_t T,
-                                                        uint32_t V, const float* __restrict__ sxt,
-                                                        const float* __restrict__ syt, uint32_t* __restrict__ cid,
+                                                        uint32_t V,  uint32_t* __restrict__ cid,
                                                         uint32_t* __restrict__ view_flags, uint8_t* __restrict__ won) {
     const uint32_t gp = blockIdx.x * kBlock + threadIdx.x;
     if (gp >= vt.total_pixels) return;
@@ -860,7 +860,7 @@ __global__ __launch_bounds__(kBlock) void k_bwd_prepass(ViewTable vt, const uint
             const int32_t x = static_cast<int32_t>(p % vd.width), y = static_cast<int32_t>(p / vd.width);
             if (load_corners(vd, idx, id - 1u, V, p0, p1, p2) && backward_box(vd, p0, p1, p2, &bx0, &bx1, &by0, &by1) &&
                 x >= bx0 && x < bx1 && y >= by0 && y < by1) {
-                const float sx = sxt[vd.col_base + x], sy = syt[vd.row_base + y];
+                const float sx = sample_x(vd, x), sy = sample_y(vd, y);
                 ScreenTri r;
                 float bb[4];
                 bool inside = screen_coords(p0, p1, p2, 0u, r, bb) && edge_pass(r, sx, sy);
@@ -1008,9 +1008,7 @@ __device__ __forceinline__ void fold_list(const float* __restrict__ lterm, int32
 // per-wave list, which each triangle's lane then adds in order (the walk
 // order restricted to one triangle is the reference's row-major order).
 __global__ __launch_bounds__(kBlock, 4) void k_corner_small(ViewTable vt, const uint32_t* __restrict__ idx, uint32_t T,
-                                                         uint32_t V, const float* __restrict__ sxt,
-                                                         const float* __restrict__ syt,
-                                                         const uint32_t* __restrict__ cid,
+                                                         uint32_t V, const uint32_t* __restrict__ cid,
                                                          const uint32_t* __restrict__ view_flags,
                                                          const uint8_t* __restrict__ won,
                                                          const uint32_t* __restrict__ slot_of,
@@ -1205,9 +1203,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_corner_small(ViewTable vt, const 
 // summation order: the same terms, within 1e-4 of the largest gradient plus
 // 1e-4 relative; k_corner_large_ordered keeps the order).
 __global__ __launch_bounds__(kBlock) void k_corner_large(ViewTable vt, const uint32_t* __restrict__ idx, uint32_t T,
-                                                         uint32_t V, const float* __restrict__ sxt,
-                                                         const float* __restrict__ syt,
-                                                         const uint32_t* __restrict__ cid,
+                                                         uint32_t V, const uint32_t* __restrict__ cid,
                                                          const uint32_t* __restrict__ view_flags,
                                                          const uint32_t* __restrict__ slot_of,
                                                          float4* __restrict__ corner, const uint4* __restrict__ queue,
@@ -1231,13 +1227,13 @@ __global__ __launch_bounds__(kBlock) void k_corner_large(ViewTable vt, const uin
         const float2* gr = reinterpret_cast<const float2*>(vd.grad_raster);
         float s[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
         for (int32_t y = g.ya + static_cast<int32_t>(threadIdx.x >> 6); ok && y <= g.yb; y += kBlock / 64) {
-            const float sy = syt[vd.row_base + y];
+            const float sy = sample_y(vd, y);
             int32_t lo, hi;
             line_span(g.sp, sy, alpha, beta, g.xa, g.xb, &lo, &hi);
             for (int32_t x = lo + static_cast<int32_t>(threadIdx.x & 63u); x <= hi; x += 64) {
                 const size_t pix = static_cast<size_t>(y) * vd.width + static_cast<size_t>(x);
                 float t9[9];
-                if (cv[pix] == q.y + 1u && pixel_terms(g.p, sxt[vd.col_base + x], sy, gr[pix * 2], t9))
+                if (cv[pix] == q.y + 1u && pixel_terms(g.p, sample_x(vd, x), sy, gr[pix * 2], t9))
 #pragma unroll
                     for (int k = 0; k < 9; ++k) s[k] += t9[k];
             }
@@ -1266,8 +1262,7 @@ __global__ __launch_bounds__(kBlock) void k_corner_large(ViewTable vt, const uin
 // rank), and lanes 0-8 each add one term component over that list in order,
 // so every component is the left fold of rasterize_backward.hlsl:88-151 bit for bit.
 __global__ __launch_bounds__(kBlock) void k_corner_large_ordered(
-    ViewTable vt, const uint32_t* __restrict__ idx, uint32_t T, uint32_t V, const float* __restrict__ sxt,
-    const float* __restrict__ syt, const uint32_t* __restrict__ cid, const uint32_t* __restrict__ view_flags,
+    ViewTable vt, const uint32_t* __restrict__ idx, uint32_t T, uint32_t V, const uint32_t* __restrict__ cid, const uint32_t* __restrict__ view_flags,
     const uint32_t* __restrict__ slot_of, float4* __restrict__ corner, const uint4* __restrict__ queue,
     uint32_t* __restrict__ counters) {
     __shared__ float terms[kBlock / 64][64 * 9];
@@ -1288,7 +1283,7 @@ __global__ __launch_bounds__(kBlock) void k_corner_large_ordered(
         const float2* gr = reinterpret_cast<const float2*>(vd.grad_raster);
         float acc = 0.0f;  // lane k < 9: term component k
         for (int32_t y = g.ya; ok && y <= g.yb; ++y) {
-            const float sy = syt[vd.row_base + y];
+            const float sy = sample_y(vd, y);
             int32_t lo, hi;
             line_span(g.sp, sy, alpha, beta, g.xa, g.xb, &lo, &hi);
             for (int32_t x0 = lo; x0 <= hi; x0 += 64) {
@@ -1297,7 +1292,7 @@ __global__ __launch_bounds__(kBlock) void k_corner_large_ordered(
                 bool hit = false;
                 if (x <= hi) {
                     const size_t pix = static_cast<size_t>(y) * vd.width + static_cast<size_t>(x);
-                    hit = cv[pix] == q.y + 1u && pixel_terms(g.p, sxt[vd.col_base + x], sy, gr[pix * 2], t9);
+                    hit = cv[pix] == q.y + 1u && pixel_terms(g.p, sample_x(vd, x), sy, gr[pix * 2], t9);
                 }
                 const unsigned long long bal = __ballot(hit);
                 if (!bal) continue;
@@ -1552,12 +1547,21 @@ __global__ void k_sort_corner_lists(uint32_t T, uint32_t V, uint32_t* state, con
 // the part of its run inside the chunk, in corner order, chunk after chunk.
 constexpr uint32_t kReduceChunk = 2048;
 
+// The backward's last kernel: it also clears the view's won marks and whole-box
+// flag (read by the corner kernels before it) and the large-box queue's
+// counters, which are zero between calls.
 __global__ __launch_bounds__(kBlock) void k_vertex_reduce(ViewTable vt, uint32_t T, uint32_t V,
                                                           const uint32_t* __restrict__ offsets,
-                                                          const float4* __restrict__ corner) {
+                                                          const float4* __restrict__ corner,
+                                                          uint8_t* __restrict__ won, uint32_t* __restrict__ view_flags,
+                                                          uint32_t* __restrict__ counters) {
     __shared__ float4 chunk[kReduceChunk];
     const uint32_t view = blockIdx.y;
     const ViewDesc& vd = vt.v[view];
+    clear_queue_counters(counters);  // k_corner_large, before this pass, was the queue's consumer
+    for (uint32_t t = blockIdx.x * kBlock + threadIdx.x; t < T; t += gridDim.x * kBlock)
+        won[static_cast<size_t>(view) * T + t] = 0u;
+    if (blockIdx.x == 0 && threadIdx.x == 0) view_flags[view] = 0u;
     const float4* cg = corner + static_cast<size_t>(view) * T * 3;
     for (uint32_t v0 = blockIdx.x * kBlock; v0 < V; v0 += gridDim.x * kBlock) {
         const uint32_t v = v0 + threadIdx.x;
@@ -1647,23 +1651,15 @@ hipError_t check_indices(const uint32_t* idx, size_t n, uint32_t V, uint32_t* fl
     return hipGetLastError();
 }
 
-hipError_t sample_tables(const ViewTable& vt, float* sx, float* sy, hipStream_t st) {
-    uint32_t longest = 0;
-    for (uint32_t k = 0; k < vt.n; ++k) longest = max(longest, max(vt.v[k].width, vt.v[k].height));
-    k_sample_tables<<<dim3((longest + kBlock - 1) / kBlock, vt.n), kBlock, 0, st>>>(vt, sx, sy);
-    return hipGetLastError();
-}
-
 hipError_t raster_candidates(const ViewTable& vt, const uint32_t* idx, uint32_t T, uint32_t V, uint32_t cull,
                              const RasterScratch& s, const TuneParams& tp, hipStream_t st) {
-    hipError_t e = fill_u32(s.counters, 0u, 4, st);
-    if (e != hipSuccess) return e;
+    hipError_t e;
     const uint32_t bsz = tri_batch(T, vt.n, tp.fwd_waves);
-    k_raster_small<<<dim3(tri_grid(T, bsz, vt.n), vt.n), kBlock, 0, st>>>(vt, idx, T, V, cull, s.sx, s.sy, s.keys, s.queue,
+    k_raster_small<<<dim3(tri_grid(T, bsz, vt.n), vt.n), kBlock, 0, st>>>(vt, idx, T, V, cull, s.keys, s.queue,
                                                                     s.counters, bsz, tp.wave_work, nullptr, nullptr,
                                                                     tp.order);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    k_raster_large<<<kLargeBlocks, kBlock, 0, st>>>(vt, idx, T, V, cull, s.sx, s.sy, s.keys, s.queue, s.counters);
+    k_raster_large<<<kLargeBlocks, kBlock, 0, st>>>(vt, idx, T, V, cull, s.keys, s.queue, s.counters);
     return hipGetLastError();
 }
 
@@ -1679,8 +1675,7 @@ size_t bin_scan_temp_bytes(uint32_t n) {
 hipError_t raster_tiled(const ViewTable& vt, const uint32_t* idx, uint32_t T, uint32_t V, uint32_t cull,
                         int want_deriv, const RasterScratch& s, const TuneParams& tp, hipStream_t st) {
     if (tp.bin_tiles == 0 || tp.bin_tiles > kMaxBinTiles) return hipErrorInvalidValue;
-    hipError_t e = fill_u32(s.counters, 0u, 4, st);
-    if (e != hipSuccess) return e;
+    hipError_t e;
     if ((e = fill_u32(s.bin_counts, 0u, static_cast<size_t>(vt.total_rtiles) + 1, st)) != hipSuccess) return e;
     if ((e = fill_u32(reinterpret_cast<uint32_t*>(s.bin_seeded), 0u, (vt.total_rtiles + 3u) / 4u, st)) != hipSuccess)
         return e;
@@ -1691,7 +1686,7 @@ hipError_t raster_tiled(const ViewTable& vt, const uint32_t* idx, uint32_t T, ui
                                          s.tri_spans);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     // the triangles too large to bin (global keys; the largest of them queued as slabs)
-    k_raster_small<<<dim3(64, vt.n), kBlock, 0, st>>>(vt, idx, T, V, cull, s.sx, s.sy, s.keys, s.queue, s.counters,
+    k_raster_small<<<dim3(64, vt.n), kBlock, 0, st>>>(vt, idx, T, V, cull, s.keys, s.queue, s.counters,
                                                      64u, tp.wave_work, s.medium_list, s.medium_count, 0u);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     size_t bytes = s.bin_temp_bytes;
@@ -1700,28 +1695,24 @@ hipError_t raster_tiled(const ViewTable& vt, const uint32_t* idx, uint32_t T, ui
         return e;
     k_bin_fill<<<grid, kBlock, 0, st>>>(vt, T, s.bin_counts, s.bin_offsets, s.bin_list, s.tri_spans);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    k_raster_large<<<kLargeBlocks, kBlock, 0, st>>>(vt, idx, T, V, cull, s.sx, s.sy, s.keys, s.queue, s.counters);
+    k_raster_large<<<kLargeBlocks, kBlock, 0, st>>>(vt, idx, T, V, cull, s.keys, s.queue, s.counters);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     k_tile_raster<<<vt.total_rtiles, kBlock, 0, st>>>(vt, idx, T, want_deriv, s.bin_offsets, s.bin_list,
                                                       s.bin_seeded, static_cast<const TriRecord*>(s.tri_records),
-                                                      s.keys);
+                                                      s.keys, s.counters);
     return hipGetLastError();
 }
 
 hipError_t raster_resolve(const ViewTable& vt, const uint32_t* idx, int want_deriv, const RasterScratch& s,
                           hipStream_t st) {
-    k_raster_resolve<<<(vt.total_pixels + kBlock - 1) / kBlock, kBlock, 0, st>>>(vt, idx, want_deriv, s.sx, s.sy,
-                                                                                s.keys);
+    k_raster_resolve<<<(vt.total_pixels + kBlock - 1) / kBlock, kBlock, 0, st>>>(vt, idx, want_deriv, s.keys,
+                                                                                s.counters);
     return hipGetLastError();
 }
 
 hipError_t backward_prepass(const ViewTable& vt, const uint32_t* idx, uint32_t T, uint32_t V, const RasterScratch& s,
                             hipStream_t st) {
-    hipError_t e = fill_u32(s.view_flags, 0u, kMaxViews, st);
-    if (e != hipSuccess) return e;
-    e = fill_u32(reinterpret_cast<uint32_t*>(s.won), 0u, (static_cast<size_t>(vt.n) * T + 3) / 4, st);
-    if (e != hipSuccess) return e;
-    k_bwd_prepass<<<(vt.total_pixels + kBlock - 1) / kBlock, kBlock, 0, st>>>(vt, idx, T, V, s.sx, s.sy, s.cid,
+    k_bwd_prepass<<<(vt.total_pixels + kBlock - 1) / kBlock, kBlock, 0, st>>>(vt, idx, T, V, s.cid,
                                                                              s.view_flags, s.won);
     return hipGetLastError();
 }
@@ -1764,26 +1755,25 @@ hipError_t mark_if_differ(const uint32_t* a, const uint32_t* b, size_t n, uint32
 
 hipError_t corner_gradients(const ViewTable& vt, const uint32_t* idx, uint32_t T, uint32_t V, const RasterScratch& s,
                             float4* corner, const TuneParams& tp, bool ordered, hipStream_t st) {
-    hipError_t e = fill_u32(s.counters, 0u, 4, st);
-    if (e != hipSuccess) return e;
+    hipError_t e;
     const uint32_t bsz = tri_batch(T, vt.n, tp.bwd_waves);
-    k_corner_small<<<dim3(tri_grid(T, bsz), vt.n), kBlock, 0, st>>>(vt, idx, T, V, s.sx, s.sy, s.cid, s.view_flags,
+    k_corner_small<<<dim3(tri_grid(T, bsz), vt.n), kBlock, 0, st>>>(vt, idx, T, V, s.cid, s.view_flags,
                                                                     s.won, s.slot_of, corner, s.queue, s.counters,
                                                                     bsz, tp.flat_work);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (ordered)
-        k_corner_large_ordered<<<kLargeBlocks, kBlock, 0, st>>>(vt, idx, T, V, s.sx, s.sy, s.cid, s.view_flags,
+        k_corner_large_ordered<<<kLargeBlocks, kBlock, 0, st>>>(vt, idx, T, V, s.cid, s.view_flags,
                                                                 s.slot_of, corner, s.queue, s.counters);
     else
-        k_corner_large<<<kLargeBlocks, kBlock, 0, st>>>(vt, idx, T, V, s.sx, s.sy, s.cid, s.view_flags, s.slot_of,
+        k_corner_large<<<kLargeBlocks, kBlock, 0, st>>>(vt, idx, T, V, s.cid, s.view_flags, s.slot_of,
                                                         corner, s.queue, s.counters);
     return hipGetLastError();
 }
 
 hipError_t vertex_reduce(const ViewTable& vt, uint32_t T, uint32_t V, const VertexCsr& csr, const float4* corner,
-                         hipStream_t st) {
-    k_vertex_reduce<<<dim3(grid_for(V) < 4096 ? grid_for(V) : 4096, vt.n), kBlock, 0, st>>>(vt, T, V, csr.offsets,
-                                                                                         corner);
+                         uint8_t* won, uint32_t* view_flags, uint32_t* counters, hipStream_t st) {
+    k_vertex_reduce<<<dim3(grid_for(V) < 4096 ? grid_for(V) : 4096, vt.n), kBlock, 0, st>>>(
+        vt, T, V, csr.offsets, corner, won, view_flags, counters);
     return hipGetLastError();
 }
 
@@ -1819,7 +1809,7 @@ extern "C" int pdr_testing_candidate_region(const float* clip12, uint32_t width,
     const float alpha = pdr::span_alpha(vd, sp.cols), beta = pdr::span_beta(vd, sp.cols);
     const int32_t l0 = sp.cols ? r.x0 : r.y0, l1 = sp.cols ? r.x1 : r.y1;
     for (int32_t l = l0; l <= l1; ++l) {
-        // the line's sample of k_sample_tables
+        // the line's pixel-centre sample (rasterize_forward.hlsl:55-57)
         const float s = sp.cols ? (static_cast<float>(l) - vd.vx + 0.5f) * 2.0f / vd.vw - 1.0f
                                 : (static_cast<float>(l) - vd.vy + 0.5f) * 2.0f / vd.vh - 1.0f;
         int32_t lo, hi;
