@@ -462,7 +462,7 @@ inline void tiled_scratch(pdr_context ctx, const pdr::ViewTable& vt, uint32_t T,
 inline std::vector<pdr::TuneParams> backward_candidates() {
     std::vector<pdr::TuneParams> out{pdr::TuneParams{}};
     for (float work : {256.0f, 512.0f, 1024.0f, 2048.0f})
-        for (uint32_t waves : {2048u, 4096u, 8192u}) {
+        for (uint32_t waves : {2048u, 4096u, 8192u, 16384u}) {
             if (work == out[0].flat_work && waves == out[0].bwd_waves) continue;
             pdr::TuneParams p;
             p.flat_work = work;

@@ -96,7 +96,7 @@ hipError_t fill_keys(unsigned long long* keys, size_t n, hipStream_t st);
 // order-free and both backward paths keep the reference's per-triangle order).
 struct TuneParams {
     float wave_work = 16384.0f;  // forward: candidate estimate walked by one wave (else queued as slabs)
-    uint32_t fwd_waves = 4096;   // forward: target wave count (sets triangles per wave, 8..64)
+    uint32_t fwd_waves = 4096;   // forward: target wave count (sets triangles per wave, 1..64)
     uint32_t tiled = 0;          // forward with outputs: 1 = binned into raster tiles (LDS keys), 0 = global keys
     uint32_t bin_tiles = 4;      // tiled forward: most raster tiles a binned triangle may touch (else slabs)
     float flat_work = 512.0f;    // backward: region estimate walked in the concatenated row list

@@ -1608,7 +1608,7 @@ __global__ void k_check_indices(const uint32_t* idx, size_t n, uint32_t V, uint3
 // still has about `waves` waves to overlap their latencies.
 uint32_t tri_batch(uint32_t T, uint32_t views, uint32_t waves) {
     const uint64_t per = (static_cast<uint64_t>(T) * views + waves - 1) / waves;
-    return per >= 64 ? 64u : (per <= 8 ? 8u : static_cast<uint32_t>(per));
+    return per >= 64 ? 64u : (per <= 1 ? 1u : static_cast<uint32_t>(per));
 }
 
 // Workgroups per view: one per four batches, at most 8192 per view, or with

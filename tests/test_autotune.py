@@ -23,7 +23,7 @@ pytestmark = pytest.mark.gpu
 
 FORWARD = [(1, d, 16384.0, 4096, 0) for d in (2, 4, 8)] + \
     [(0, 4, w, n, o) for o in (0, 1) for w in (4096.0, 16384.0, 65536.0) for n in (2048, 4096, 8192)]
-BACKWARD = [(w, n) for w in (256.0, 512.0, 1024.0, 2048.0) for n in (2048, 4096, 8192)] * 2
+BACKWARD = [(w, n) for w in (256.0, 512.0, 1024.0, 2048.0) for n in (2048, 4096, 8192, 16384)] * 2
 
 
 def run_views(ctx, sc):
