@@ -79,37 +79,41 @@ __global__ __launch_bounds__(kBlock) void k_interp_fwd(PixelTable pt, const floa
         body(pt.v[find_pixel_view(pt, gp)]);
 }
 
-// A = 8 (16-byte aligned rows): two pixels per lane, gp and gp + half, their
+// A = 8 (16-byte aligned rows): kN pixels per lane, gp, gp + part, ..., their
 // dependent loads (raster -> triangle -> attribute rows) issued side by side so
-// twice as many are in flight; the values are k_interp_fwd's.
-__global__ __launch_bounds__(kBlock) void k_interp_fwd8_pair(PixelTable pt, const float* __restrict__ attr,
+// kN times as many are in flight; the values are k_interp_fwd's.
+constexpr uint32_t kInterpPixels = 4;
+template <int kN>
+__global__ __launch_bounds__(kBlock) void k_interp_fwd8_multi(PixelTable pt, const float* __restrict__ attr,
                                                              uint32_t V, const uint32_t* __restrict__ idx, uint32_t T,
-                                                             uint32_t half) {
+                                                             uint32_t part) {
     const uint32_t g0 = blockIdx.x * kBlock + threadIdx.x;
-    if (g0 >= half) return;
-    const uint32_t gps[2] = {g0, g0 + half};
-    bool live[2];
-    const PixelView* pv[2];
+    if (g0 >= part) return;
+    uint32_t gps[kN];
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
+    for (int h = 0; h < kN; ++h) gps[h] = g0 + static_cast<uint32_t>(h) * part;
+    bool live[kN];
+    const PixelView* pv[kN];
+#pragma unroll
+    for (int h = 0; h < kN; ++h) {
         live[h] = gps[h] < pt.total_pixels;
         const uint32_t g = live[h] ? gps[h] : pt.total_pixels - 1u;
         bool split;
         const uint32_t kv = wave_view(pt, g, split);
         pv[h] = &pt.v[split ? find_pixel_view(pt, g) : kv];
     }
-    float4 r[2];
-    uint32_t p[2];
+    float4 r[kN];
+    uint32_t p[kN];
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
+    for (int h = 0; h < kN; ++h) {
         const uint32_t g = live[h] ? gps[h] : pt.total_pixels - 1u;
         p[h] = g - pv[h]->pixel_base;
         r[h] = reinterpret_cast<const float4*>(pv[h]->raster)[p[h]];
     }
-    uint32_t iv[2][3];
-    bool ok[2];
+    uint32_t iv[kN][3];
+    bool ok[kN];
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
+    for (int h = 0; h < kN; ++h) {
         const uint32_t id = decode_id(r[h].w);
         ok[h] = live[h] && id != 0 && id <= T;
         iv[h][0] = iv[h][1] = iv[h][2] = 0;
@@ -119,9 +123,9 @@ __global__ __launch_bounds__(kBlock) void k_interp_fwd8_pair(PixelTable pt, cons
             iv[h][2] = idx[(id - 1) * 3 + 2];
         }
     }
-    float4 row[2][3][2];
+    float4 row[kN][3][2];
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
+    for (int h = 0; h < kN; ++h) {
         ok[h] = ok[h] && iv[h][0] < V && iv[h][1] < V && iv[h][2] < V;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
@@ -134,7 +138,7 @@ __global__ __launch_bounds__(kBlock) void k_interp_fwd8_pair(PixelTable pt, cons
         }
     }
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
+    for (int h = 0; h < kN; ++h) {
         if (!live[h]) continue;
         float4* out = reinterpret_cast<float4*>(pv[h]->out0 + static_cast<size_t>(p[h]) * 8);
         if (!ok[h]) {
@@ -736,8 +740,8 @@ hipError_t interp_forward(const PixelTable& pt, const float* attr, uint32_t V, u
     bool aligned = (reinterpret_cast<uintptr_t>(attr) % 16) == 0;
     for (uint32_t k = 0; k < pt.n; ++k) aligned &= (reinterpret_cast<uintptr_t>(pt.v[k].out0) % 16) == 0;
     if (A == 8 && aligned) {
-        const uint32_t half = (pt.total_pixels + 1u) / 2u;
-        k_interp_fwd8_pair<<<pixel_grid(half), kBlock, 0, st>>>(pt, attr, V, idx, T, half);
+        const uint32_t part = (pt.total_pixels + kInterpPixels - 1u) / kInterpPixels;
+        k_interp_fwd8_multi<kInterpPixels><<<pixel_grid(part), kBlock, 0, st>>>(pt, attr, V, idx, T, part);
     } else if (A == 2 && aligned)
         k_interp_fwd<2><<<grid, kBlock, 0, st>>>(pt, attr, V, A, idx, T);
     else

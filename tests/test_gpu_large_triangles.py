@@ -132,3 +132,36 @@ def test_large_triangles_backward(rasterizer, det_ctx, name):
                                                        idx_c.shape[0], _p(ref_r), w, h, _p(grad), None, _p(out),
                                                        None))
     bits_equal(out, ref, "ordered backward")
+
+
+@pytest.mark.parametrize("name", ["perspective_quad_1024", "overlapping_fan_512"])
+def test_large_work_queue_under_autotuning_and_repeated_calls(name):
+    """The large-work queue's counters are zero between calls without a fill
+    launch: the resolve pass (forward) and the vertex reduce (backward) clear them
+    after the queue's consumer.  The autotuner's first call runs every candidate
+    launch shape back to back on the same queue, and later calls reuse it: each
+    forward / backward pair must still match the oracle, call after call."""
+    from photara_drender_amd import _native
+
+    pos, idx, res, n_big = case(name)
+    h, w = res
+    ref_r, ref_d, _ = oracle.rasterize_forward(pos, idx, res)
+    rng = np.random.default_rng(11)
+    grad = rng.standard_normal(ref_r.shape).astype(np.float32)
+    ref_g = oracle.rasterize_backward(pos, idx, ref_r, grad)
+    ctx = _native.Context(0, flags=_native.PDR_FLAG_AUTOTUNE | _native.PDR_FLAG_DETERMINISTIC)
+    lib = _native.lib()
+    pos_c, idx_c = np.ascontiguousarray(pos), np.ascontiguousarray(idx)
+    for call in range(3):
+        r = np.empty((h, w, 4), np.float32)
+        d = np.empty((h, w, 4), np.float32)
+        _native.check(lib.pdr_rasterize_forward(ctx.handle, _p(pos_c), pos_c.shape[0], _p(idx_c), idx_c.shape[0], w,
+                                                h, 0, 1, None, _p(r), _p(d), None, None))
+        bits_equal(r, ref_r, f"raster, call {call}")
+        bits_equal(d, ref_d, f"derivatives, call {call}")
+        g = np.empty_like(ref_g)
+        _native.check(lib.pdr_rasterize_backward(ctx.handle, _p(pos_c), pos_c.shape[0], _p(idx_c), idx_c.shape[0],
+                                                 _p(r), w, h, _p(grad), None, _p(g), None))
+        bits_equal(g, ref_g, f"position gradients, call {call}")
+    kinds = sorted(line.split()[0] for line in ctx.autotune_export().splitlines())
+    assert kinds == ["backward", "forward_out"]
