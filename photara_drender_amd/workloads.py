@@ -167,7 +167,7 @@ def atlas_psnr(ctx, problem: RefineProblem, atlas, against="clean") -> float:
 
 class TexturedChain:
     """The textured raster chain over device tensors: rasterize -> interpolate
-    (uv) -> texture (clamp), the loss sum (color - target)^2's gradient, then
+    (uv) -> texture (clamp), the loss 0.5 * sum (color - target)^2's gradient (one op), then
     texture, interpolate and rasterize backward to texels, uv attributes and
     clip positions.  ``step(stream)`` runs it eagerly on a stream;
     ``capture(stream)`` records one step into a HIP graph (torch.cuda.CUDAGraph)
@@ -210,7 +210,7 @@ class TexturedChain:
         ck(lib.pdr_texture_forward(h, self.tex.data_ptr(), tw, th, C, 0, self.uvs.data_ptr(), self.raster.data_ptr(),
                                    W, H, self.color.data_ptr(), st))
         with torch.cuda.stream(stream):
-            torch.sub(self.color, self.target, alpha=1.0, out=self.g_color).mul_(2.0)
+            torch.sub(self.color, self.target, alpha=1.0, out=self.g_color)
         ck(lib.pdr_texture_backward(h, self.tex.data_ptr(), tw, th, C, 0, self.uvs.data_ptr(), self.raster.data_ptr(),
                                     W, H, self.g_color.data_ptr(), 1, self.g_tex.data_ptr(), self.g_uvs.data_ptr(),
                                     st))
