@@ -32,7 +32,7 @@ constexpr int kW1 = 0, kB1 = kW1 + kIn * kHid, kW2 = kB1 + kHid, kB2 = kW2 + kHi
 static_assert(kParams == static_cast<int>(kMlpParams), "parameter layout");
 constexpr int kPart = kParams + 2;  // per-wave partials: unscaled gradient, loss sum, valid count
 constexpr int kRow = 40;            // LDS row stride (bf16) of the texel-contiguous activation tiles
-constexpr uint32_t kInferBlock = 256;
+constexpr uint32_t kInferBlock = 512;
 
 __device__ __forceinline__ f32x4 mfma(const bf16x8& a, const bf16x8& b, const f32x4& c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
@@ -57,9 +57,13 @@ __device__ __forceinline__ float gelu_sigmoid(float z, float& one_minus_s) {
     return s;
 }
 
+// GELU for the forward pass: z / (1 + 2^(z (k0 + k1 z^2))) with k0 = -2 c log2 e
+// and k1 = 0.044715 k0, the same function as z * gelu_sigmoid(z) in five
+// full-rate and two quarter-rate instructions.
 __device__ __forceinline__ float gelu(float z) {
-    float oms;
-    return z * gelu_sigmoid(z, oms);
+    constexpr float k0 = -2.0f * kGeluC * 1.4426950408889634f, k1 = 0.044715f * k0;
+    const float e = __builtin_amdgcn_exp2f(z * __builtin_fmaf(k1, z * z, k0));
+    return z * __builtin_amdgcn_rcpf(1.0f + e);
 }
 
 __device__ __forceinline__ float gelu_grad(float z) {
@@ -150,34 +154,86 @@ __device__ __forceinline__ float base_color(const float (*feat)[kIn], int q, int
     return s;
 }
 
-// Forward of one tile: pre-activations z1, z2 (D layout) and the network output
-// y (lanes q = 0 hold outputs i < 3 of texel r, without b3 and base colour).
-__device__ __forceinline__ void forward_tile(const FwdOps& o, const bf16x8 xb[2], f32x4 z1[4], f32x4 z2[4],
-                                             f32x4& y) {
-    const f32x4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
+// The forward operands of every lane of a wave in LDS, lane-contiguous (one
+// 16-byte read per lane and operand, no bank conflicts): the inference kernel
+// reads them there rather than holding 104 registers of weights per lane,
+// which leaves room for more waves to keep feature loads in flight.
+struct FwdOpsLds {
+    bf16x8 a1[4][2][64], a2[4][2][64], a3[2][64];
+    f32x4 b1[4][64], b2[4][64];
+};
+
+// The values load_forward_ops gives lane `lane` (= 16 q + r), stored one
+// operand at a time so that they never all occupy registers.
+__device__ void store_forward_ops(const float* __restrict__ w, int q, int r, int lane, FwdOpsLds& ops) {
+    for (int mb = 0; mb < 4; ++mb)
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks) {
+            bf16x8 a1, a2;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int in = 32 * ks + 8 * q + j;
+                a1[j] = bf(in < kIn ? w[kW1 + in * kHid + 16 * mb + r] : 0.0f);
+                a2[j] = bf(w[kW2 + unit(ks, q, j) * kHid + 16 * mb + r]);
+            }
+            ops.a1[mb][ks][lane] = a1;
+            ops.a2[mb][ks][lane] = a2;
+        }
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks) {
+        bf16x8 a3;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) a3[j] = bf(r < kOut ? w[kW3 + unit(ks, q, j) * kOut + r] : 0.0f);
+        ops.a3[ks][lane] = a3;
+    }
 #pragma unroll
     for (int mb = 0; mb < 4; ++mb) {
-        z1[mb] = mfma(o.a1[mb][1], xb[1], mfma(o.a1[mb][0], xb[0], zero));
-#pragma unroll
-        for (int i = 0; i < 4; ++i) z1[mb][i] += o.b1[mb][i];
+        const int c = 16 * mb + 4 * q;
+        ops.b1[mb][lane] = f32x4{w[kB1 + c], w[kB1 + c + 1], w[kB1 + c + 2], w[kB1 + c + 3]};
+        ops.b2[mb][lane] = f32x4{w[kB2 + c], w[kB2 + c + 1], w[kB2 + c + 2], w[kB2 + c + 3]};
     }
+}
+
+// Operand accessors of forward_tile: registers (FwdOps) or LDS (FwdOpsLds, lane `l`).
+struct RegOps {
+    const FwdOps& o;
+    __device__ bf16x8 a1(int mb, int ks) const { return o.a1[mb][ks]; }
+    __device__ bf16x8 a2(int mb, int ks) const { return o.a2[mb][ks]; }
+    __device__ bf16x8 a3(int ks) const { return o.a3[ks]; }
+    __device__ f32x4 b1(int mb) const { return f32x4{o.b1[mb][0], o.b1[mb][1], o.b1[mb][2], o.b1[mb][3]}; }
+    __device__ f32x4 b2(int mb) const { return f32x4{o.b2[mb][0], o.b2[mb][1], o.b2[mb][2], o.b2[mb][3]}; }
+};
+struct LdsOps {
+    const FwdOpsLds& o;
+    int l;
+    __device__ bf16x8 a1(int mb, int ks) const { return o.a1[mb][ks][l]; }
+    __device__ bf16x8 a2(int mb, int ks) const { return o.a2[mb][ks][l]; }
+    __device__ bf16x8 a3(int ks) const { return o.a3[ks][l]; }
+    __device__ f32x4 b1(int mb) const { return o.b1[mb][l]; }
+    __device__ f32x4 b2(int mb) const { return o.b2[mb][l]; }
+};
+
+// Forward of one tile: pre-activations z1, z2 (D layout; the biases enter as
+// the first accumulator) and the network output y (lanes q = 0 hold outputs
+// i < 3 of texel r, without b3 and base colour).
+template <class Ops>
+__device__ __forceinline__ void forward_tile(const Ops& o, const bf16x8 xb[2], f32x4 z1[4], f32x4 z2[4], f32x4& y) {
+    const f32x4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb) z1[mb] = mfma(o.a1(mb, 1), xb[1], mfma(o.a1(mb, 0), xb[0], o.b1(mb)));
     bf16x8 h1[2];
 #pragma unroll
     for (int ks = 0; ks < 2; ++ks)
 #pragma unroll
         for (int j = 0; j < 8; ++j) h1[ks][j] = bf(gelu(z1[2 * ks + j / 4][j % 4]));
 #pragma unroll
-    for (int mb = 0; mb < 4; ++mb) {
-        z2[mb] = mfma(o.a2[mb][1], h1[1], mfma(o.a2[mb][0], h1[0], zero));
-#pragma unroll
-        for (int i = 0; i < 4; ++i) z2[mb][i] += o.b2[mb][i];
-    }
+    for (int mb = 0; mb < 4; ++mb) z2[mb] = mfma(o.a2(mb, 1), h1[1], mfma(o.a2(mb, 0), h1[0], o.b2(mb)));
     bf16x8 h2[2];
 #pragma unroll
     for (int ks = 0; ks < 2; ++ks)
 #pragma unroll
         for (int j = 0; j < 8; ++j) h2[ks][j] = bf(gelu(z2[2 * ks + j / 4][j % 4]));
-    y = mfma(o.a3[1], h2[1], mfma(o.a3[0], h2[0], zero));
+    y = mfma(o.a3(1), h2[1], mfma(o.a3(0), h2[0], zero));
 }
 
 // Inference: out [n, 3] = network + base colour for texels with samples, 0 for
@@ -185,10 +241,12 @@ __device__ __forceinline__ void forward_tile(const FwdOps& o, const bf16x8 xb[2]
 __global__ __launch_bounds__(kInferBlock) void k_mlp_infer(const float* __restrict__ x, uint32_t n,
                                                            const float* __restrict__ w, float* __restrict__ out) {
     __shared__ __align__(16) float feat_all[kInferBlock / 64][16][kIn];
+    __shared__ FwdOpsLds ops;
     const int lane = static_cast<int>(threadIdx.x & 63u), q = lane >> 4, r = lane & 15;
     float (*feat)[kIn] = feat_all[threadIdx.x >> 6];
-    FwdOps o;
-    load_forward_ops(w, q, r, o);
+    if (threadIdx.x < 64) store_forward_ops(w, q, r, lane, ops);  // the lane layout is the same in every wave
+    __syncthreads();
+    const LdsOps o{ops, lane};
     const float b3[3] = {w[kB3], w[kB3 + 1], w[kB3 + 2]};
     const uint32_t waves = gridDim.x * (kInferBlock / 64);
     const uint32_t tiles = (n + 15u) / 16u;
@@ -304,7 +362,7 @@ __global__ __launch_bounds__(64) void k_mlp_grad(const float* __restrict__ x, co
             float base[3];
             base_color(L.feat, q, r, base);
             f32x4 z1[4], z2[4], y;
-            forward_tile(o, xb, z1, z2, y);
+            forward_tile(RegOps{o}, xb, z1, z2, y);
             // output gradient 2 d (lanes q = 0 hold outputs 0..2 of texel t)
             float dyv[3] = {0.0f, 0.0f, 0.0f};
             if (q == 0 && t < n) {
@@ -484,7 +542,9 @@ uint32_t mlp_grad_waves(uint64_t n) {
 hipError_t mlp_infer(const float* x, uint32_t n, const float* w, float* out, hipStream_t st) {
     const uint32_t tiles = (n + 15u) / 16u;
     uint32_t blocks = (tiles + kInferBlock / 64 - 1) / (kInferBlock / 64);
-    blocks = blocks < 4096u ? (blocks ? blocks : 1u) : 4096u;
+    // two resident 8-wave blocks per CU (93 VGPRs, 46 KB of LDS each): the
+    // operands are staged in LDS once per block, the tiles strided over the grid
+    blocks = blocks < 512u ? (blocks ? blocks : 1u) : 512u;
     k_mlp_infer<<<blocks, kInferBlock, 0, st>>>(x, n, w, out);
     return hipGetLastError();
 }
