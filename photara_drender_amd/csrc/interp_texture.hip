@@ -140,9 +140,12 @@ __global__ __launch_bounds__(kBlock) void k_interp_fwd8_multi(PixelTable pt, con
 #pragma unroll
     for (int h = 0; h < kN; ++h) {
         if (!live[h]) continue;
-        float4* out = reinterpret_cast<float4*>(pv[h]->out0 + static_cast<size_t>(p[h]) * 8);
+        // the output is streamed (nontemporal stores: 0.239 -> 0.235 ms on 15.2 M px)
+        typedef float nt4 __attribute__((ext_vector_type(4)));
+        nt4* ot = reinterpret_cast<nt4*>(pv[h]->out0 + static_cast<size_t>(p[h]) * 8);
         if (!ok[h]) {
-            out[0] = out[1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            __builtin_nontemporal_store(nt4{0.0f, 0.0f, 0.0f, 0.0f}, ot);
+            __builtin_nontemporal_store(nt4{0.0f, 0.0f, 0.0f, 0.0f}, ot + 1);
             continue;
         }
         const float u = r[h].x, v = r[h].y;
@@ -158,8 +161,8 @@ __global__ __launch_bounds__(kBlock) void k_interp_fwd8_multi(PixelTable pt, con
         float o[8];
 #pragma unroll
         for (int a = 0; a < 8; ++a) o[a] = u * a0[a] + v * a1[a] + w * a2[a];
-        out[0] = make_float4(o[0], o[1], o[2], o[3]);
-        out[1] = make_float4(o[4], o[5], o[6], o[7]);
+        __builtin_nontemporal_store(nt4{o[0], o[1], o[2], o[3]}, ot);
+        __builtin_nontemporal_store(nt4{o[4], o[5], o[6], o[7]}, ot + 1);
     }
 }
 

@@ -531,8 +531,12 @@ __device__ __forceinline__ void resolve_key(const ViewDesc& vd, const uint32_t* 
                 d.w = (da1_dy - b1 * dA_dy) * inv * 2.0f / vd.vh;
             }
         }
-        reinterpret_cast<float4*>(vd.raster)[p] = r;
-        if (vd.deriv) reinterpret_cast<float4*>(vd.deriv)[p] = d;
+        // nontemporal stores: the outputs stream past the caches instead of
+        // displacing lines the next kernel reads (resolve + interpolate forward
+        // 0.455 -> 0.439 ms on 15.2 M px)
+        typedef float nt4 __attribute__((ext_vector_type(4)));
+        __builtin_nontemporal_store(nt4{r.x, r.y, r.z, r.w}, reinterpret_cast<nt4*>(vd.raster) + p);
+        if (vd.deriv) __builtin_nontemporal_store(nt4{d.x, d.y, d.z, d.w}, reinterpret_cast<nt4*>(vd.deriv) + p);
 }
 
 // Winner's outputs of every pixel from the global keys; resets the keys.
