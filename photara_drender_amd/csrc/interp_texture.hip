@@ -140,7 +140,7 @@ __global__ __launch_bounds__(kBlock) void k_interp_fwd8_multi(PixelTable pt, con
 #pragma unroll
     for (int h = 0; h < kN; ++h) {
         if (!live[h]) continue;
-        // the output is streamed (nontemporal stores: 0.239 -> 0.235 ms on 15.2 M px)
+        // the output is streamed with nontemporal stores
         typedef float nt4 __attribute__((ext_vector_type(4)));
         nt4* ot = reinterpret_cast<nt4*>(pv[h]->out0 + static_cast<size_t>(p[h]) * 8);
         if (!ok[h]) {
