@@ -127,7 +127,8 @@ pdr_status pdr_rasterize_forward(pdr_context ctx, const float* pos, uint32_t V, 
  * reference: for callers that cannot allocate (code under HIP graph capture,
  * tight serving loops, harnesses that forbid hipMalloc).  Same arguments and
  * results as pdr_rasterize_forward, device memory only.  Its scratch (8 B per
- * pixel + 16 B per triangle, each part 256-byte aligned) comes from a fixed
+ * pixel of the size rounded up to multiples of 4, and to 64 pixels, + 16 B per
+ * triangle, each part 256-byte aligned) comes from a fixed
  * 32 MiB array per GPU set up by pdr_context_create; larger calls fail with
  * PDR_OVERFLOW.  Indices are not validated by a readback (out-of-range
  * triangles are skipped, as by the shader) and the launch parameters are the

@@ -106,7 +106,7 @@ __device__ __forceinline__ float bake_shadow(const BakeView& v, float px, float 
                 const int32_t qx = cx + x - kR, qy = cy + y - kR;
                 in[y * kD + x] = qx >= 0 && qy >= 0 && qx < W && qy < H;
                 const int32_t lx = qx < 0 ? 0 : (qx >= W ? W - 1 : qx), ly = qy < 0 ? 0 : (qy >= H ? H - 1 : qy);
-                s[y * kD + x] = shadow_texel(sm[static_cast<size_t>(ly) * v.width + lx]);
+                s[y * kD + x] = shadow_texel(sm[key_index(lx, ly, v.key_tx)]);
             }
 #pragma unroll
         for (int k = 0; k < kD * kD; ++k)
@@ -119,7 +119,7 @@ __device__ __forceinline__ float bake_shadow(const BakeView& v, float px, float 
             for (int32_t x = -r; x <= r; ++x) {
                 const int32_t qx = cx + x, qy = cy + y;
                 if (qx < 0 || qy < 0 || qx >= W || qy >= H) continue;
-                const float2 s = shadow_texel(sm[static_cast<size_t>(qy) * v.width + qx]);
+                const float2 s = shadow_texel(sm[key_index(qx, qy, v.key_tx)]);
                 if (s.y != 0.0f) {
                     visible += depth <= s.x + v.depth_bias ? 1.0f : 0.0f;
                     count += 1.0f;

@@ -199,9 +199,9 @@ void run_bake(pdr_context ctx, const float* positions, const float* normals, con
     cache = {};
     cache.T = T;
     cache.V = V;
-    size_t cache_pixels = 0;
-    for (uint32_t k = 0; k < nviews; ++k) cache_pixels += static_cast<size_t>(views[k].width) * views[k].height;
-    unsigned long long* cache_keys = ctx->get_t<unsigned long long>("view_keys", cache_pixels);
+    size_t cache_count = 0;
+    for (uint32_t k = 0; k < nviews; ++k) cache_count += pdr::view_key_count(views[k].width, views[k].height);
+    unsigned long long* cache_keys = ctx->get_t<unsigned long long>("view_keys", cache_count);
     hip_check(hipMemcpyAsync(ctx->get_t<float>("view_keys_pos", static_cast<size_t>(V) * 3), dpos,
                              static_cast<size_t>(V) * 3 * sizeof(float), hipMemcpyDeviceToDevice, st),
               "hipMemcpyAsync");
@@ -242,9 +242,10 @@ void run_bake(pdr_context ctx, const float* positions, const float* normals, con
         // shadow maps (:266-276): each view's raster keys, read as (depth, covered) by the projection
         const unsigned long long* keys = run_raster_keys(ctx, vt, didx, T, V, 0, st);
         for (uint32_t j = 0; j < n; ++j) {
-            hv[j].shadow = keys + vt.v[j].pixel_base;
-            const size_t pix = static_cast<size_t>(vt.v[j].width) * vt.v[j].height;
-            hip_check(hipMemcpyAsync(cache_keys + cache_off, keys + vt.v[j].pixel_base, pix * 8,
+            hv[j].shadow = keys + vt.v[j].key_base;
+            hv[j].key_tx = vt.v[j].key_tx;
+            const size_t nk = pdr::view_key_count(vt.v[j].width, vt.v[j].height);
+            hip_check(hipMemcpyAsync(cache_keys + cache_off, keys + vt.v[j].key_base, nk * 8,
                                      hipMemcpyDeviceToDevice, st),
                       "hipMemcpyAsync");
             cache.width.push_back(vt.v[j].width);
@@ -254,7 +255,7 @@ void run_bake(pdr_context ctx, const float* positions, const float* normals, con
             std::memcpy(m.data(), views[k0 + j].world_to_clip, sizeof(m));
             cache.matrix.push_back(m);
             cache.offset.push_back(cache_off);
-            cache_off += pix;
+            cache_off += nk;
         }
         hip_check(hipMemcpyAsync(dviews, hv.data(), n * sizeof(pdr::BakeView), hipMemcpyHostToDevice, st),
                   "hipMemcpyAsync");

@@ -291,8 +291,13 @@ inline pdr::ViewDesc make_view(const float* pos, uint32_t W, uint32_t H, const f
 }
 
 inline void finalize_table(pdr::ViewTable& t) {
-    uint32_t tiles = 0, pixels = 0, rtiles = 0;
+    uint32_t tiles = 0, pixels = 0, rtiles = 0, keys = 0;
     for (uint32_t k = 0; k < t.n; ++k) {
+        t.v[k].key_tx = pdr::key_tiles(t.v[k].width);
+        t.v[k].key_base = keys;
+        const uint64_t nk = static_cast<uint64_t>(keys) + pdr::view_key_count(t.v[k].width, t.v[k].height);
+        if (nk > 0xFFFFFFFFu) raise(PDR_OVERFLOW, "Rasterization output size overflow");
+        keys = static_cast<uint32_t>(nk);
         t.v[k].rtiles_x = (t.v[k].width + pdr::kRasterTile - 1) / pdr::kRasterTile;
         t.v[k].rtiles_y = (t.v[k].height + pdr::kRasterTile - 1) / pdr::kRasterTile;
         t.v[k].rtile_base = rtiles;
@@ -308,6 +313,7 @@ inline void finalize_table(pdr::ViewTable& t) {
     t.total_tiles = tiles;
     t.total_pixels = pixels;
     t.total_rtiles = rtiles;
+    t.total_keys = keys;
 }
 
 inline pdr::PixelTable pixel_table_one(const float* raster, const float* in0, const float* in1, float* out0, float* out1,
@@ -343,7 +349,7 @@ inline pdr::RasterScratch raster_scratch(pdr_context ctx, const pdr::ViewTable& 
     ctx->counters_clean = false;  // set again by the caller once its kernels are enqueued
     if (with_keys) {
         const size_t before = ctx->scratch["raster_keys"].bytes;
-        s.keys = ctx->get_t<unsigned long long>("raster_keys", vt.total_pixels);
+        s.keys = ctx->get_t<unsigned long long>("raster_keys", vt.total_keys);
         const size_t after = ctx->scratch["raster_keys"].bytes;
         if (after != before || !ctx->keys_clean) hip_check(pdr::fill_keys(s.keys, after / 8, st), "fill_keys");
     }
@@ -519,8 +525,8 @@ inline void run_raster_resolve_keys(pdr_context ctx, const pdr::ViewTable& vt, c
     pdr::RasterScratch s = raster_scratch(ctx, vt, T, st, true);
     ctx->keys_clean = false;
     for (uint32_t j = 0; j < vt.n; ++j)
-        hip_check(hipMemcpyAsync(s.keys + vt.v[j].pixel_base, src_keys + offsets[j],
-                                 static_cast<size_t>(vt.v[j].width) * vt.v[j].height * sizeof(unsigned long long),
+        hip_check(hipMemcpyAsync(s.keys + vt.v[j].key_base, src_keys + offsets[j],
+                                 pdr::view_key_count(vt.v[j].width, vt.v[j].height) * sizeof(unsigned long long),
                                  hipMemcpyDeviceToDevice, st),
                   "hipMemcpyAsync");
     ctx->timed("raster_resolve", st, [&] {
@@ -531,8 +537,8 @@ inline void run_raster_resolve_keys(pdr_context ctx, const pdr::ViewTable& vt, c
 }
 
 // The forward's candidate pass alone: each pixel's winning (depth key, id)
-// word stays in the raster scratch keys (index pixel_base + pixel), ~0 where no
-// triangle covers the pixel.  They stay marked as used, so the next raster
+// word stays in the raster scratch keys (index key_base + key_index), ~0 where
+// no triangle covers the pixel.  They stay marked as used, so the next raster
 // call on this context clears them first.
 inline const unsigned long long* run_raster_keys(pdr_context ctx, const pdr::ViewTable& vt, const uint32_t* idx,
                                                  uint32_t T, uint32_t V, uint32_t cull, hipStream_t st) {
@@ -544,7 +550,7 @@ inline const unsigned long long* run_raster_keys(pdr_context ctx, const pdr::Vie
             hip_check(pdr::raster_candidates(vt, idx, T, V, cull, s, c, st), "raster_candidates");
             // the views may have no raster output here: clear the keys (and the queue counters, which a
             // resolve pass would clear) for the next candidate
-            hip_check(pdr::fill_keys(s.keys, vt.total_pixels, st), "fill_keys");
+            hip_check(pdr::fill_keys(s.keys, vt.total_keys, st), "fill_keys");
             hip_check(pdr::fill_u32(s.counters, 0u, 4, st), "fill");
         });
     ctx->timed("raster_candidates", st,

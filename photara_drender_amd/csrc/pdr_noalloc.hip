@@ -38,8 +38,9 @@ std::array<unsigned char*, kMaxDevices> g_scratch{};  // device address of `scra
 
 inline size_t up256(size_t b) { return (b + 255) & ~static_cast<size_t>(255); }
 
-// Scratch layout of one call: keys [P] u64, large queue [T] uint4, counters
-// [4] u32, each 256-byte aligned.
+// Scratch layout of one call: keys [view_key_count(W, H)] u64 (16 per 4 x 4
+// pixel tile, rounded up to 64), large queue [T] uint4, counters [4] u32, each
+// 256-byte aligned.
 struct Layout {
     size_t keys = 0, queue = 0, counters = 0, total = 0;
 };
@@ -47,7 +48,7 @@ inline Layout layout_of(uint32_t W, uint32_t H, uint32_t T) {
     Layout l;
     size_t off = 0;
     l.keys = off;
-    off += up256(static_cast<size_t>(W) * H * sizeof(unsigned long long));
+    off += up256(pdr::view_key_count(W, H) * sizeof(unsigned long long));
     l.queue = off;
     off += up256(static_cast<size_t>(T) * sizeof(uint4));
     l.counters = off;
@@ -91,7 +92,8 @@ pdr_status pdr_rasterize_forward_noalloc(pdr_context ctx, const float* pos, uint
         const Layout l = layout_of(W, H, T);
         if (l.total > kScratchBytes)
             raise(PDR_OVERFLOW, "the no-allocation rasterize forward needs " + std::to_string(l.total) +
-                                    " bytes of scratch (8 per pixel + 16 per triangle); its limit is " +
+                                    " bytes of scratch (8 per pixel of the size rounded up to multiples of 4, "
+                                    "+ 16 per triangle); its limit is " +
                                     std::to_string(kScratchBytes));
         float vp[4];
         resolve_viewport(W, H, viewport4, vp);
@@ -118,7 +120,7 @@ pdr_status pdr_rasterize_forward_noalloc(pdr_context ctx, const float* pos, uint
         // triangle whose index is out of range, as the shader does
         // (rasterize_forward.hlsl:75-80).  Launch parameters are the defaults
         // (every choice gives the same raster).
-        hip_check(pdr::fill_keys(s.keys, static_cast<size_t>(W) * H, st), "fill_keys");
+        hip_check(pdr::fill_keys(s.keys, pdr::view_key_count(W, H), st), "fill_keys");
         hip_check(pdr::fill_u32(s.counters, 0u, 4, st), "fill");  // the queue's consumer clears it again
         const pdr::TuneParams tp{};
         hip_check(pdr::raster_candidates(vt, idx, T, V, cull, s, tp, st), "raster_candidates");

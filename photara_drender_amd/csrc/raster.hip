@@ -366,7 +366,7 @@ __global__ __launch_bounds__(kBlock) void k_raster_small(ViewTable vt, const uin
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
     const uint32_t w0 = wave * 64;
     int32_t* slot = R.slot + w0;
-    unsigned long long* kbase = keys + vd.pixel_base;
+    unsigned long long* kbase = keys + vd.key_base;
     const float ax[2] = {span_alpha(vd, 0), span_alpha(vd, 1)};
     const float bx[2] = {span_beta(vd, 0), span_beta(vd, 1)};
     const uint32_t stride = gridDim.x * (kBlock / 64) * bsz;
@@ -408,7 +408,7 @@ __global__ __launch_bounds__(kBlock) void k_raster_small(ViewTable vt, const uin
             }
         }
         walk_batch(vd, R, w0, lane, lines, slot, ax, bx, [&](int32_t x, int32_t y, unsigned long long key) {
-            key_min(&kbase[static_cast<size_t>(y) * vd.width + x], key);
+            key_min(&kbase[key_index(static_cast<uint32_t>(x), static_cast<uint32_t>(y), vd.key_tx)], key);
         });
     }
 }
@@ -416,7 +416,6 @@ __global__ __launch_bounds__(kBlock) void k_raster_small(ViewTable vt, const uin
 // Rows [ya, yb] of a wave-uniform triangle with row spans, 64 rows at a time.
 __device__ __forceinline__ void wave_forward(const ScreenTri& r, const LineSpans& sp, uint32_t t, int32_t ya,
                                              int32_t yb, const ViewDesc& vd, uint32_t lane, int32_t* slot,
-                                             
                                              unsigned long long* __restrict__ kbase) {
     const float alpha = span_alpha(vd, 0), beta = span_beta(vd, 0);
     for (int32_t y0 = ya; y0 <= yb; y0 += 64) {
@@ -438,7 +437,8 @@ __device__ __forceinline__ void wave_forward(const ScreenTri& r, const LineSpans
             const float syr = __shfl(sy, rl, 64);
             if (m < total) {
                 const unsigned long long key = test_pixel(r, t, sample_x(vd, x), syr);
-                if (key != ~0ull) key_min(&kbase[static_cast<size_t>(y0 + rl) * vd.width + x], key);
+                if (key != ~0ull)
+                    key_min(&kbase[key_index(static_cast<uint32_t>(x), static_cast<uint32_t>(y0 + rl), vd.key_tx)], key);
             }
         }
     }
@@ -485,7 +485,7 @@ __global__ __launch_bounds__(kBlock) void k_raster_large(ViewTable vt, const uin
                 const int32_t ya = r.y0 + static_cast<int32_t>(s * kSlabRows);
                 const int32_t yb = ya + static_cast<int32_t>(kSlabRows) - 1 < r.y1 ? ya + static_cast<int32_t>(kSlabRows) - 1
                                                                                   : r.y1;
-                wave_forward(r, sp, q.y, ya, yb, vd, lane, slots + (threadIdx.x & ~63u), keys + vd.pixel_base);
+                wave_forward(r, sp, q.y, ya, yb, vd, lane, slots + (threadIdx.x & ~63u), keys + vd.key_base);
             }
         }
         cum += batch_total;
@@ -540,26 +540,29 @@ __device__ __forceinline__ void resolve_key(const ViewDesc& vd, const uint32_t* 
         if (vd.deriv) __builtin_nontemporal_store(nt4{d.x, d.y, d.z, d.w}, reinterpret_cast<nt4*>(vd.deriv) + p);
 }
 
-// Winner's outputs of every pixel from the global keys; resets the keys.
+// Winner's outputs of every pixel from the global keys; resets the keys.  One
+// lane per key, in key order (a wave covers four 4 x 4 tiles of one view: its
+// key reads are one contiguous 512-byte run and each output row it writes is
+// 16 pixels long); keys past the view's last row or column are skipped.
 __global__ __launch_bounds__(kBlock) void k_raster_resolve(ViewTable vt, const uint32_t* __restrict__ idx,
                                                            int want_deriv, unsigned long long* __restrict__ keys,
                                                            uint32_t* __restrict__ counters) {
     clear_queue_counters(counters);  // k_raster_large, before this pass, was the queue's consumer
-    const uint32_t gp = blockIdx.x * kBlock + threadIdx.x;
-    if (gp >= vt.total_pixels) return;
-    bool split;
-    const uint32_t kv = wave_view(vt, gp, split);
-    auto body = [&](const ViewDesc& vd) {
-        const uint32_t p = gp - vd.pixel_base;
-        const unsigned long long key = keys[gp];
-        keys[gp] = ~0ull;
-        resolve_key(vd, idx, want_deriv, key, p, sample_x(vd, static_cast<int32_t>(p % vd.width)),
-                    sample_y(vd, static_cast<int32_t>(p / vd.width)));
-    };
-    if (!split)
-        body(vt.v[kv]);
-    else
-        body(vt.v[find_view(vt, gp)]);
+    const uint32_t gk = blockIdx.x * kBlock + threadIdx.x;
+    if (gk >= vt.total_keys) return;
+    // views start at multiples of 64 keys: the wave's first key names its view
+    const uint32_t first = __builtin_amdgcn_readfirstlane(gk);
+    uint32_t view = 0;
+    while (view + 1 < vt.n && first >= vt.v[view + 1].key_base) ++view;
+    const ViewDesc& vd = vt.v[view];
+    const uint32_t k = gk - vd.key_base, tile = k >> 4;
+    const uint32_t x = (tile % vd.key_tx) * kKeyTile + (k & 3u);
+    const uint32_t y = (tile / vd.key_tx) * kKeyTile + ((k >> 2) & 3u);
+    if (x >= vd.width || y >= vd.height) return;
+    const unsigned long long key = keys[gk];
+    keys[gk] = ~0ull;
+    resolve_key(vd, idx, want_deriv, key, y * vd.width + x, sample_x(vd, static_cast<int32_t>(x)),
+                sample_y(vd, static_cast<int32_t>(y)));
 }
 
 // ------------------------------------------------------------ tiled forward
@@ -783,10 +786,12 @@ __global__ __launch_bounds__(kBlock) void k_tile_raster(ViewTable vt, const uint
     const int32_t tx1 = tx0 + kRT - 1 < W - 1 ? tx0 + kRT - 1 : W - 1;
     const int32_t ty1 = ty0 + kRT - 1 < H - 1 ? ty0 + kRT - 1 : H - 1;
     const bool seed = seeded[tile] != 0u;
-    unsigned long long* gk = keys + vd.pixel_base;
+    unsigned long long* gk = keys + vd.key_base;
     for (uint32_t i = threadIdx.x; i < kRTPix; i += kBlock) {
         const int32_t x = tx0 + static_cast<int32_t>(i % kRasterTile), y = ty0 + static_cast<int32_t>(i / kRasterTile);
-        lk[i] = seed && x <= tx1 && y <= ty1 ? gk[static_cast<size_t>(y) * vd.width + x] : ~0ull;
+        lk[i] = seed && x <= tx1 && y <= ty1
+                    ? gk[key_index(static_cast<uint32_t>(x), static_cast<uint32_t>(y), vd.key_tx)]
+                    : ~0ull;
     }
     __syncthreads();
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
@@ -829,7 +834,7 @@ __global__ __launch_bounds__(kBlock) void k_tile_raster(ViewTable vt, const uint
         const int32_t x = tx0 + static_cast<int32_t>(i % kRasterTile), y = ty0 + static_cast<int32_t>(i / kRasterTile);
         if (x > tx1 || y > ty1) continue;
         const uint32_t p = static_cast<uint32_t>(y) * vd.width + static_cast<uint32_t>(x);
-        if (seed) gk[p] = ~0ull;
+        if (seed) gk[key_index(static_cast<uint32_t>(x), static_cast<uint32_t>(y), vd.key_tx)] = ~0ull;
         resolve_key(vd, idx, want_deriv, lk[i], p, sample_x(vd, x), sample_y(vd, y));
     }
 }
@@ -1710,7 +1715,7 @@ hipError_t raster_tiled(const ViewTable& vt, const uint32_t* idx, uint32_t T, ui
 
 hipError_t raster_resolve(const ViewTable& vt, const uint32_t* idx, int want_deriv, const RasterScratch& s,
                           hipStream_t st) {
-    k_raster_resolve<<<(vt.total_pixels + kBlock - 1) / kBlock, kBlock, 0, st>>>(vt, idx, want_deriv, s.keys,
+    k_raster_resolve<<<(vt.total_keys + kBlock - 1) / kBlock, kBlock, 0, st>>>(vt, idx, want_deriv, s.keys,
                                                                                 s.counters);
     return hipGetLastError();
 }
