@@ -577,7 +577,8 @@ inline void run_raster_backward(pdr_context ctx, const pdr::ViewTable& vt, const
     if (f[1] || f[2] || f[3] || f[4] || f[5]) hip_check(pdr::fill_u32(csr.state, 0u, 8, st), "fill");
     float4* corner = ctx->get_t<float4>("corner_grads", static_cast<size_t>(vt.n) * T * 3);
     pdr::RasterScratch sb = s;
-    sb.cid = ctx->get_t<uint32_t>("backward_ids", vt.total_pixels);
+    sb.cid = ctx->get_t<uint32_t>("backward_ids", vt.total_keys);
+    sb.cgrad = ctx->get_t<float2>("backward_grads", vt.total_keys);
     bool fresh_flags = false, fresh_won = false;
     sb.view_flags = ctx->get_t<uint32_t>("backward_view_flags", pdr::kMaxViews, &fresh_flags);
     sb.won = reinterpret_cast<uint8_t*>(

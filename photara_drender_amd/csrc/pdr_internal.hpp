@@ -76,7 +76,7 @@ struct PixelTable {
 
 // Scratch of the triangle-parallel raster (owned by the context).
 struct RasterScratch {
-    unsigned long long* keys = nullptr;  // [total_pixels] (depth, id) keys; all ~0 between calls
+    unsigned long long* keys = nullptr;  // [total_keys] (depth, id) keys (key_index layout); all ~0 between calls
     // tiled forward (raster tiles of kRasterTile^2 pixels)
     uint32_t* bin_counts = nullptr;      // [total_rtiles + 1] triangles binned per tile (0 between calls)
     uint32_t* bin_offsets = nullptr;     // [total_rtiles + 1] exclusive scan of bin_counts
@@ -90,7 +90,8 @@ struct RasterScratch {
     size_t bin_temp_bytes = 0;
     uint4* queue = nullptr;              // [views * T] (view, triangle, slabs, 0) of large triangles
     uint32_t* counters = nullptr;        // [4] queue length, next queue entry (0 between calls)
-    uint32_t* cid = nullptr;             // [total_pixels] backward: compared id per pixel
+    uint32_t* cid = nullptr;             // [total_keys] backward: compared id per pixel (at key_base + key_index)
+    float2* cgrad = nullptr;             // [total_keys] backward: grad_raster.xy where cid != 0 (same layout)
     uint32_t* view_flags = nullptr;      // [kMaxViews] backward: view walked over whole boxes (0 between calls)
     uint8_t* won = nullptr;              // [views * T, padded to words] backward: 1 where a pixel's compared id names the triangle (0 between calls)
     const uint32_t* slot_of = nullptr;   // [3T] backward: where each corner's gradient is written (VertexCsr)

@@ -850,24 +850,35 @@ __global__ __launch_bounds__(kBlock) void k_tile_raster(ViewTable vt, const uint
 // and the view's triangles are then walked over their whole boxes.  A pixel
 // that passes t's float edge test is inside t's region, which skips the
 // region computation for every pixel of a forward-produced raster.
+//
+// The compared ids and the pixels' grad_raster.xy go to cid / cgrad in the
+// keys' 4 x 4-tile layout (key_base + key_index; cgrad only where the id is
+// not 0), so the corner walks, which visit a few pixels on each of a few rows
+// per triangle, load from few cache lines.  One lane per key, in key order.
 __global__ __launch_bounds__(kBlock) void k_bwd_prepass(ViewTable vt, const uint32_t* __restrict__ idx, uint32_t T,
-                                                        uint32_t V,  uint32_t* __restrict__ cid,
-                                                        uint32_t* __restrict__ view_flags, uint8_t* __restrict__ won) {
-    const uint32_t gp = blockIdx.x * kBlock + threadIdx.x;
-    if (gp >= vt.total_pixels) return;
-    bool split;
-    const uint32_t kv = wave_view(vt, gp, split);
-    auto body = [&](uint32_t view) {
-        const ViewDesc& vd = vt.v[view];
-        const uint32_t p = gp - vd.pixel_base;
-        const uint32_t id = decode_id(vd.raster[static_cast<size_t>(p) * 4 + 3]);
-        const float2 lg = reinterpret_cast<const float2*>(vd.grad_raster)[static_cast<size_t>(p) * 2];
-        uint32_t c = 0;
+                                                        uint32_t V, uint32_t* __restrict__ cid,
+                                                        float2* __restrict__ cgrad, uint32_t* __restrict__ view_flags,
+                                                        uint8_t* __restrict__ won) {
+    const uint32_t gk = blockIdx.x * kBlock + threadIdx.x;
+    if (gk >= vt.total_keys) return;  // whole waves: views start at multiples of 64 keys
+    const uint32_t first = __builtin_amdgcn_readfirstlane(gk);
+    uint32_t view = 0;
+    while (view + 1 < vt.n && first >= vt.v[view + 1].key_base) ++view;
+    const ViewDesc& vd = vt.v[view];
+    const uint32_t k = gk - vd.key_base, tile = k >> 4;
+    const uint32_t xu = (tile % vd.key_tx) * kKeyTile + (k & 3u);
+    const uint32_t yu = (tile / vd.key_tx) * kKeyTile + ((k >> 2) & 3u);
+    uint32_t c = 0;
+    if (xu < vd.width && yu < vd.height) {  // keys past the last row or column name no pixel
+        const size_t p = static_cast<size_t>(yu) * vd.width + xu;
+        const uint32_t id = decode_id(vd.raster[p * 4 + 3]);
+        const float2 lg = reinterpret_cast<const float2*>(vd.grad_raster)[p * 2];
         if (id != 0u && id <= T && !(lg.x == 0.0f && lg.y == 0.0f)) {
             c = id;
+            cgrad[gk] = lg;
             float4 p0, p1, p2;
             int32_t bx0, bx1, by0, by1;
-            const int32_t x = static_cast<int32_t>(p % vd.width), y = static_cast<int32_t>(p / vd.width);
+            const int32_t x = static_cast<int32_t>(xu), y = static_cast<int32_t>(yu);
             if (load_corners(vd, idx, id - 1u, V, p0, p1, p2) && backward_box(vd, p0, p1, p2, &bx0, &bx1, &by0, &by1) &&
                 x >= bx0 && x < bx1 && y >= by0 && y < by1) {
                 const float sx = sample_x(vd, x), sy = sample_y(vd, y);
@@ -885,16 +896,13 @@ __global__ __launch_bounds__(kBlock) void k_bwd_prepass(ViewTable vt, const uint
                 if (!inside) atomicOr(&view_flags[view], 1u);
             }
         }
-        cid[gp] = c;
-        // mark the named triangle once per run of equal neighbours in the wave
-        const uint32_t mark = c ? view * T + (c - 1u) : 0xFFFFFFFFu;
-        const uint32_t prev = __shfl_up(mark, 1, 64);
-        if (c && ((threadIdx.x & 63u) == 0 || prev != mark)) won[mark] = 1;
-    };
-    if (!split)
-        body(kv);
-    else
-        body(find_view(vt, gp));
+        cid[gk] = c;
+    }
+    // mark the named triangle once per run of equal neighbours in the wave
+    // (every lane takes part, with no mark where no pixel is named)
+    const uint32_t mark = c ? view * T + (c - 1u) : 0xFFFFFFFFu;
+    const uint32_t prev = __shfl_up(mark, 1, 64);
+    if (c && ((threadIdx.x & 63u) == 0 || prev != mark)) won[mark] = 1;
 }
 
 // Terms of one pixel (rasterize_backward.hlsl:101-151): false when the pixel's
@@ -1019,6 +1027,7 @@ __device__ __forceinline__ void fold_list(const float* __restrict__ lterm, int32
 // order restricted to one triangle is the reference's row-major order).
 __global__ __launch_bounds__(kBlock, 4) void k_corner_small(ViewTable vt, const uint32_t* __restrict__ idx, uint32_t T,
                                                          uint32_t V, const uint32_t* __restrict__ cid,
+                                                         const float2* __restrict__ cgrad,
                                                          const uint32_t* __restrict__ view_flags,
                                                          const uint8_t* __restrict__ won,
                                                          const uint32_t* __restrict__ slot_of,
@@ -1034,8 +1043,8 @@ __global__ __launch_bounds__(kBlock, 4) void k_corner_small(ViewTable vt, const 
     float* lterm = R.term[wave];
     const bool full = view_flags[view] != 0u;
     const float alpha = span_alpha(vd, 0), beta = span_beta(vd, 0);
-    const uint32_t* cv = cid + vd.pixel_base;
-    const float2* gr = reinterpret_cast<const float2*>(vd.grad_raster);
+    const uint32_t* cv = cid + vd.key_base;
+    const float2* gv = cgrad + vd.key_base;
     float4* cview = corner + static_cast<size_t>(view) * T * 3;
     int32_t* seg_first = R.seg_first + w0;
     int32_t* seg_end = R.seg_end + w0;
@@ -1102,9 +1111,10 @@ __global__ __launch_bounds__(kBlock, 4) void k_corner_small(ViewTable vt, const 
                         bool hit = false;
                         if (m < total) {
                             // id and gradient loaded together: no dependent second load
-                            const size_t pix = static_cast<size_t>(y0 + rl) * vd.width + static_cast<size_t>(x);
+                            const uint32_t pix =
+                                key_index(static_cast<uint32_t>(x), static_cast<uint32_t>(y0 + rl), vd.key_tx);
                             const uint32_t c = cv[pix];
-                            const float2 g = gr[pix * 2];
+                            const float2 g = gv[pix];
                             if (c == qt + 1u) hit = pixel_terms(q.p, sample_x(vd, x), syr, g, t9);
                         }
                         unsigned long long mk = __ballot(hit);
@@ -1167,9 +1177,9 @@ __global__ __launch_bounds__(kBlock, 4) void k_corner_small(ViewTable vt, const 
                 bool hit = false;
                 if (m < npix) {
                     // id and gradient loaded together: no dependent second load
-                    const size_t pix = static_cast<size_t>(yy) * vd.width + static_cast<size_t>(x);
+                    const uint32_t pix = key_index(static_cast<uint32_t>(x), static_cast<uint32_t>(yy), vd.key_tx);
                     const uint32_t c = cv[pix];
-                    const float2 g = gr[pix * 2];
+                    const float2 g = gv[pix];
                     if (c == batch + static_cast<uint32_t>(oo) + 1u) {  // oo < bsz: a flat-class lane
                         const uint32_t i = w0 + static_cast<uint32_t>(oo);
                         const float4 pp[3] = {R.p0[i], R.p1[i], R.p2[i]};
@@ -1214,6 +1224,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_corner_small(ViewTable vt, const 
 // 1e-4 relative; k_corner_large_ordered keeps the order).
 __global__ __launch_bounds__(kBlock) void k_corner_large(ViewTable vt, const uint32_t* __restrict__ idx, uint32_t T,
                                                          uint32_t V, const uint32_t* __restrict__ cid,
+                                                         const float2* __restrict__ cgrad,
                                                          const uint32_t* __restrict__ view_flags,
                                                          const uint32_t* __restrict__ slot_of,
                                                          float4* __restrict__ corner, const uint4* __restrict__ queue,
@@ -1233,17 +1244,17 @@ __global__ __launch_bounds__(kBlock) void k_corner_large(ViewTable vt, const uin
         BwdRegion g;
         const bool ok = setup_backward(vd, idx, q.y, V, view_flags[q.x] != 0u, g);
         const float alpha = span_alpha(vd, 0), beta = span_beta(vd, 0);
-        const uint32_t* cv = cid + vd.pixel_base;
-        const float2* gr = reinterpret_cast<const float2*>(vd.grad_raster);
+        const uint32_t* cv = cid + vd.key_base;
+        const float2* gv = cgrad + vd.key_base;
         float s[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
         for (int32_t y = g.ya + static_cast<int32_t>(threadIdx.x >> 6); ok && y <= g.yb; y += kBlock / 64) {
             const float sy = sample_y(vd, y);
             int32_t lo, hi;
             line_span(g.sp, sy, alpha, beta, g.xa, g.xb, &lo, &hi);
             for (int32_t x = lo + static_cast<int32_t>(threadIdx.x & 63u); x <= hi; x += 64) {
-                const size_t pix = static_cast<size_t>(y) * vd.width + static_cast<size_t>(x);
+                const uint32_t pix = key_index(static_cast<uint32_t>(x), static_cast<uint32_t>(y), vd.key_tx);
                 float t9[9];
-                if (cv[pix] == q.y + 1u && pixel_terms(g.p, sample_x(vd, x), sy, gr[pix * 2], t9))
+                if (cv[pix] == q.y + 1u && pixel_terms(g.p, sample_x(vd, x), sy, gv[pix], t9))
 #pragma unroll
                     for (int k = 0; k < 9; ++k) s[k] += t9[k];
             }
@@ -1272,7 +1283,8 @@ __global__ __launch_bounds__(kBlock) void k_corner_large(ViewTable vt, const uin
 // rank), and lanes 0-8 each add one term component over that list in order,
 // so every component is the left fold of rasterize_backward.hlsl:88-151 bit for bit.
 __global__ __launch_bounds__(kBlock) void k_corner_large_ordered(
-    ViewTable vt, const uint32_t* __restrict__ idx, uint32_t T, uint32_t V, const uint32_t* __restrict__ cid, const uint32_t* __restrict__ view_flags,
+    ViewTable vt, const uint32_t* __restrict__ idx, uint32_t T, uint32_t V, const uint32_t* __restrict__ cid,
+    const float2* __restrict__ cgrad, const uint32_t* __restrict__ view_flags,
     const uint32_t* __restrict__ slot_of, float4* __restrict__ corner, const uint4* __restrict__ queue,
     uint32_t* __restrict__ counters) {
     __shared__ float terms[kBlock / 64][64 * 9];
@@ -1289,8 +1301,8 @@ __global__ __launch_bounds__(kBlock) void k_corner_large_ordered(
         BwdRegion g;
         const bool ok = setup_backward(vd, idx, q.y, V, view_flags[q.x] != 0u, g);
         const float alpha = span_alpha(vd, 0), beta = span_beta(vd, 0);
-        const uint32_t* cv = cid + vd.pixel_base;
-        const float2* gr = reinterpret_cast<const float2*>(vd.grad_raster);
+        const uint32_t* cv = cid + vd.key_base;
+        const float2* gv = cgrad + vd.key_base;
         float acc = 0.0f;  // lane k < 9: term component k
         for (int32_t y = g.ya; ok && y <= g.yb; ++y) {
             const float sy = sample_y(vd, y);
@@ -1301,8 +1313,8 @@ __global__ __launch_bounds__(kBlock) void k_corner_large_ordered(
                 float t9[9];
                 bool hit = false;
                 if (x <= hi) {
-                    const size_t pix = static_cast<size_t>(y) * vd.width + static_cast<size_t>(x);
-                    hit = cv[pix] == q.y + 1u && pixel_terms(g.p, sample_x(vd, x), sy, gr[pix * 2], t9);
+                    const uint32_t pix = key_index(static_cast<uint32_t>(x), static_cast<uint32_t>(y), vd.key_tx);
+                    hit = cv[pix] == q.y + 1u && pixel_terms(g.p, sample_x(vd, x), sy, gv[pix], t9);
                 }
                 const unsigned long long bal = __ballot(hit);
                 if (!bal) continue;
@@ -1722,8 +1734,8 @@ hipError_t raster_resolve(const ViewTable& vt, const uint32_t* idx, int want_der
 
 hipError_t backward_prepass(const ViewTable& vt, const uint32_t* idx, uint32_t T, uint32_t V, const RasterScratch& s,
                             hipStream_t st) {
-    k_bwd_prepass<<<(vt.total_pixels + kBlock - 1) / kBlock, kBlock, 0, st>>>(vt, idx, T, V, s.cid,
-                                                                             s.view_flags, s.won);
+    k_bwd_prepass<<<(vt.total_keys + kBlock - 1) / kBlock, kBlock, 0, st>>>(vt, idx, T, V, s.cid, s.cgrad,
+                                                                           s.view_flags, s.won);
     return hipGetLastError();
 }
 
@@ -1767,15 +1779,15 @@ hipError_t corner_gradients(const ViewTable& vt, const uint32_t* idx, uint32_t T
                             float4* corner, const TuneParams& tp, bool ordered, hipStream_t st) {
     hipError_t e;
     const uint32_t bsz = tri_batch(T, vt.n, tp.bwd_waves);
-    k_corner_small<<<dim3(tri_grid(T, bsz), vt.n), kBlock, 0, st>>>(vt, idx, T, V, s.cid, s.view_flags,
+    k_corner_small<<<dim3(tri_grid(T, bsz), vt.n), kBlock, 0, st>>>(vt, idx, T, V, s.cid, s.cgrad, s.view_flags,
                                                                     s.won, s.slot_of, corner, s.queue, s.counters,
                                                                     bsz, tp.flat_work);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (ordered)
-        k_corner_large_ordered<<<kLargeBlocks, kBlock, 0, st>>>(vt, idx, T, V, s.cid, s.view_flags,
+        k_corner_large_ordered<<<kLargeBlocks, kBlock, 0, st>>>(vt, idx, T, V, s.cid, s.cgrad, s.view_flags,
                                                                 s.slot_of, corner, s.queue, s.counters);
     else
-        k_corner_large<<<kLargeBlocks, kBlock, 0, st>>>(vt, idx, T, V, s.cid, s.view_flags, s.slot_of,
+        k_corner_large<<<kLargeBlocks, kBlock, 0, st>>>(vt, idx, T, V, s.cid, s.cgrad, s.view_flags, s.slot_of,
                                                         corner, s.queue, s.counters);
     return hipGetLastError();
 }
