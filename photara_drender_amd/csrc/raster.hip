@@ -87,12 +87,6 @@ uint32_t grid_for(size_t n, uint32_t block = kBlock) {
     return static_cast<uint32_t>(g < 65536 ? (g ? g : 1) : 65536);
 }
 
-__device__ __forceinline__ uint32_t find_view(const ViewTable& vt, uint32_t g) {
-    uint32_t k = 0;
-    while (k + 1 < vt.n && g >= vt.v[k + 1].pixel_base) ++k;
-    return k;
-}
-
 // Pixel-centre NDC of column x / row y (rasterize_forward.hlsl:55-57); every
 // kernel evaluates it inline (a correctly rounded division), so no per-call
 // table launch is needed.
