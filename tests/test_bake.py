@@ -106,12 +106,12 @@ def test_suggest_softmax_scale_closed_form():
     assert oracle.suggest_softmax_scale(pos, cams) == pytest.approx(20 * d2, rel=1e-6)
 
 
-def bake_scene(views=4, seed=5):
+def bake_scene(views=4, seed=5, size=(48, 64)):
     sc = scenes.atlas_scene(nx=40, ny=20, charts_x=4, charts_y=2, atlas_size=64, gutter=2, seed=seed)
-    mats = scenes.atlas_views(sc, views, (48, 64), seed=seed)
+    mats = scenes.atlas_views(sc, views, size, seed=seed)
     rng = np.random.default_rng(seed)
-    images = [rng.uniform(0, 1, (48, 64, 3 + (k % 2))).astype(np.float32) for k in range(views)]
-    masks = [rng.uniform(0, 1, (48, 64)).astype(np.float32) if k % 2 else None for k in range(views)]
+    images = [rng.uniform(0, 1, size + (3 + (k % 2),)).astype(np.float32) for k in range(views)]
+    masks = [rng.uniform(0, 1, size).astype(np.float32) if k % 2 else None for k in range(views)]
     cams = np.stack([scenes.camera_center(m) for m in mats])
     nrm = scenes.vertex_normals(sc.positions, sc.indices)
     return sc, nrm, mats, cams, images, masks
@@ -156,6 +156,25 @@ def test_gpu_bake_pcf_radii_match_oracle(pcf):
                                 visibility_mode="shadow_map", **kw)
     r = oracle.bake(sc.positions, nrm, sc.uv, sc.indices, images, mats, cams, masks, **kw)
     assert g[3].sum() > 100
+    np.testing.assert_array_equal(g[0].view(np.uint32), r[0].view(np.uint32))
+    np.testing.assert_array_equal(g[1].view(np.uint32), r[1].view(np.uint32))
+    np.testing.assert_array_equal(g[2], r[2])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("pcf", [1, 2])
+def test_gpu_bake_at_unaligned_sizes_matches_oracle(pcf):
+    """Views of 45 x 61 and a 61 x 67 atlas: sides that are not multiples of 4, so the
+    raster keys' 4 x 4-pixel tiles (the shadow maps the projection reads) are padded."""
+    import photara_drender_amd as pdr
+
+    sc, nrm, mats, cams, images, masks = bake_scene(size=(45, 61))
+    kw = dict(resolution=(61, 67), blend_mode="weighted_average", mask_floor=0.1, pcf_radius=pcf)
+    g = pdr.TextureBaker().bake(sc.positions, nrm, sc.uv, sc.indices, images, mats, cams, masks, None,
+                                visibility_mode="shadow_map", **kw)
+    r = oracle.bake(sc.positions, nrm, sc.uv, sc.indices, images, mats, cams, masks, **kw)
+    assert g[3].sum() > 100
+    np.testing.assert_array_equal(g[4], r[4])
     np.testing.assert_array_equal(g[0].view(np.uint32), r[0].view(np.uint32))
     np.testing.assert_array_equal(g[1].view(np.uint32), r[1].view(np.uint32))
     np.testing.assert_array_equal(g[2], r[2])

@@ -159,17 +159,19 @@ def test_refine_stop_after_equals_the_full_runs_checkpoint(tv):
             assert_bits(part.color, full.checkpoints[stop // 10 - 1], f"atlas after step {stop}")
 
 
-def test_refine_after_a_bake_resolves_its_shadow_keys():
+@pytest.mark.parametrize("size", [(72, 96), (70, 93)])
+def test_refine_after_a_bake_resolves_its_shadow_keys(size):
     """After a bake on the same context, pdr_refine resolves the bake's shadow-map
     keys of the same views (same mesh, size, viewport and matrix) instead of
     rasterising them again (no candidate pass runs); the refinement equals one on a
-    fresh context bit for bit.  A bake of a moved mesh is not reused."""
+    fresh context bit for bit.  A bake of a moved mesh is not reused.  70 x 93 views
+    have padded key tiles (sides not multiples of 4)."""
     torch = pytest.importorskip("torch")
     from photara_drender_amd import _native
     from photara_drender_amd.bake import bake_atlas
     from photara_drender_amd.refine import refine_atlas
 
-    sc, mats, masks, images, pairs = small_atlas(views=4)
+    sc, mats, masks, images, pairs = small_atlas(views=4, size=size)
     dev = torch.device("cuda:0")
     t = lambda a: torch.tensor(np.ascontiguousarray(a), device=dev)
     nrm = scenes.vertex_normals(sc.positions, sc.indices)
