@@ -76,6 +76,16 @@ __global__ void k_fill_f32(float* p, float v, size_t n) {
         p[i] = v;
 }
 
+// 16-byte stores over a 16-byte aligned p: n / 4 float4, then the n % 4 tail.
+__global__ void k_fill_f32x4(float* p, float v, size_t n) {
+    const size_t n4 = n / 4;
+    float4* q = reinterpret_cast<float4*>(p);
+    for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n4;
+         i += static_cast<size_t>(gridDim.x) * blockDim.x)
+        q[i] = make_float4(v, v, v, v);
+    if (blockIdx.x == 0 && threadIdx.x < n % 4) p[n4 * 4 + threadIdx.x] = v;
+}
+
 __global__ void k_fill_u64(unsigned long long* p, unsigned long long v, size_t n) {
     for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n;
          i += static_cast<size_t>(gridDim.x) * blockDim.x)
@@ -1646,7 +1656,10 @@ hipError_t fill_u32(uint32_t* p, uint32_t value, size_t n, hipStream_t st) {
 
 hipError_t fill_f32(float* p, float value, size_t n, hipStream_t st) {
     if (!n) return hipSuccess;
-    k_fill_f32<<<grid_for(n), kBlock, 0, st>>>(p, value, n);
+    if ((reinterpret_cast<uintptr_t>(p) & 15u) == 0)
+        k_fill_f32x4<<<grid_for(n / 4 ? n / 4 : 1), kBlock, 0, st>>>(p, value, n);
+    else
+        k_fill_f32<<<grid_for(n), kBlock, 0, st>>>(p, value, n);
     return hipGetLastError();
 }
 
